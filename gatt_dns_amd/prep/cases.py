@@ -1,0 +1,204 @@
+"""Problem descriptions for the time-advance path.
+
+A :class:`Case` bundles everything the reference bakes into its generated
+Fortran include files (``parameters.F90``, ``matrices.F90``,
+``boundaryInfo.F90``, ``runDisturbances.F90`` / ``runForcings*.F90``) plus
+the initial state.  Scalars and coefficient arrays are passed through
+:func:`fortran_round` exactly as the reference's ``%.20fd0`` literals are.
+
+Builders:
+  * :func:`lid_driven`      -- ``examples/lidDriven.m`` on a uniform mesh
+  * :func:`turbulence_box`  -- ``examples/turbulenceBox.m`` (periodic box + forcing)
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .boundaries import (BoundaryInfo, calc_sfd_region, disturbance_index, init_boundaries,
+                         lid_driven_flow, periodic_box)
+from .matrices import DirMesh, LineCoeffs, NumMethods, fortran_round, make_matrices, prepare_thomas
+
+FIELDS = ("U", "V", "W", "R", "E")
+
+
+@dataclass
+class Disturbance:
+    """One entry of ``runDisturbances.F90`` (kind 'lidDrivenMovement' or 'holdInlet')."""
+    kind: str
+    box: list
+    X: np.ndarray
+    Y: np.ndarray
+    Z: np.ndarray
+    par: list
+
+
+@dataclass
+class Forcing:
+    """One entry of ``runForcings3D.F90`` (kind 'turbulenceGenerator')."""
+    kind: str
+    box: list
+    par: list
+
+
+@dataclass
+class Case:
+    name: str
+    nx: int
+    ny: int
+    nz: int
+    X: np.ndarray
+    Y: np.ndarray
+    Z: np.ndarray
+    Re: float
+    Ma: float
+    Pr: float
+    gamma: float
+    T0: float
+    deriv: list            # [LineCoeffs x, y, z]  (z is None in 2-D)
+    filt: list
+    types: list            # [typeMapX (ny,nz), typeMapY (nx,nz), typeMapZ (nx,ny)] 1-based
+    filter_dirs: tuple
+    filter_char_time: float  # FilterCharTime as written to parameters.F90 (-1 => full replacement)
+    bi: BoundaryInfo
+    disturb: list = field(default_factory=list)
+    forcing: list = field(default_factory=list)
+    sfd_type: int = 0
+    sfd_delta: float = 0.0
+    sfd_x_val: float = 0.0
+    sfd_x: np.ndarray = None
+    dxmin: tuple = (0.0, 0.0, 0.0)
+    dt: float = 0.0
+    state0: dict = None
+
+    @property
+    def is3d(self):
+        return self.nz > 1
+
+    @property
+    def npoints(self):
+        return self.nx * self.ny * self.nz
+
+    @property
+    def E0(self):
+        return 1.0 / ((self.gamma ** 2 - self.gamma) * self.Ma ** 2)
+
+
+def _coeffs(meshes, flow_region, nm):
+    mats = make_matrices(meshes, flow_region, nm)
+    deriv, filt, types = [], [], []
+    for key, dm in zip("xyz", meshes):
+        types.append(mats[key].types.astype(np.int32))
+        if key == "z" and dm.n == 1:
+            deriv.append(None)
+            filt.append(None)
+            continue
+        d, f = prepare_thomas(mats[key])
+        deriv.append(d)
+        filt.append(f)
+    return deriv, filt, types
+
+
+def _dxmin(X, Y, Z, nz):
+    """``dxmin`` of writeFortranParameters.m:79-89."""
+    dx = [1.0 / np.min(np.diff(X)), 1.0 / np.min(np.diff(Y)), 0.0 if nz == 1 else 1.0 / np.min(np.diff(Z))]
+    return tuple(float(v) for v in fortran_round(np.array(dx)))
+
+
+def cfl_dt(state, Ma, dxmin, nz):
+    """CFL number divided by dt (main.F90:196-214)."""
+    um = np.max(np.abs(state["U"]))
+    vm = np.max(np.abs(state["V"]))
+    c = (1 / Ma + um) * dxmin[0] + (1 / Ma + vm) * dxmin[1]
+    if nz > 1:
+        c += (1 / Ma + np.max(np.abs(state["W"]))) * dxmin[2]
+    return c
+
+
+def lid_driven(n=129, Re=1000.0, Ma=0.5, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0.49,
+               spatial="SL4", cfl=0.5, nx=None, ny=None, lid_u0=1.0):
+    """2-D lid-driven cavity on [0,1]^2 with a uniform mesh (examples/lidDriven.m, uniform variant).
+
+    Filter borders 'reducedOrder' with the x start/end and y end closures switched
+    off (``examples/lidDriven.m:119-122``).  dt = cfl / CFLdt on the initial state
+    after the first boundary application.
+    """
+    nx = nx or n
+    ny = ny or n
+    X = np.linspace(0.0, 1.0, nx)
+    Y = np.linspace(0.0, 1.0, ny)
+    Z = np.array([0.0])
+    meshes = [DirMesh(X), DirMesh(Y), DirMesh(Z, periodic=True, fix_periodic_domain_size=True, xf=1.0)]
+    E0 = 1.0 / ((gamma ** 2 - gamma) * Ma ** 2)
+    P0 = (gamma - 1) * E0
+    fr, walls, corners, bc = lid_driven_flow(nx, ny, 1, E0, P0)
+    nm = NumMethods(spatial_derivs=spatial, spatial_derivs_buffer="EX4", filter_strength=alpha_f,
+                    filter_borders="reducedOrder", filter_borders_start=(False, True, True),
+                    filter_borders_end=(False, False, True))
+    deriv, filt, types = _coeffs(meshes, fr, nm)
+    bi = init_boundaries(bc, corners, nm.neumann_order, nm.neumann2_order)
+    box = disturbance_index(X, Y, Z, (0.0, 1.0), (1.0, 1.0), (-np.inf, np.inf))
+    dist = Disturbance("lidDrivenMovement", box, fortran_round(X[box[0] - 1:box[1]]),
+                       fortran_round(Y[box[2] - 1:box[3]]), fortran_round(Z[box[4] - 1:box[5]]),
+                       [float(fortran_round(lid_u0))])
+    rnd = lambda v: float(fortran_round(v))
+    case = Case("lid_driven_%dx%d" % (nx, ny), nx, ny, 1, X, Y, Z, rnd(Re), rnd(Ma), rnd(Pr), rnd(gamma),
+                rnd(T0), deriv, filt, types, (1, 1, 1), -1.0, bi, [dist], [],
+                dxmin=_dxmin(X, Y, Z, 1))
+    st = {k: np.zeros((nx, ny, 1), order="F") for k in FIELDS}
+    st["R"][:] = 1.0
+    st["E"][:] = E0
+    case.state0 = st
+    # dt from the CFL of the state after the first boundary application (lid at U0)
+    umax = max(abs(lid_u0), 0.0)
+    cfl_dt_val = (1 / case.Ma + umax) * case.dxmin[0] + (1 / case.Ma) * case.dxmin[1]
+    case.dt = rnd(cfl / cfl_dt_val)
+    return case
+
+
+def solenoidal_field(n, seed, k0=4.0, urms=0.5):
+    """Random-phase solenoidal velocity with E(k) ~ k^4 exp(-2 (k/k0)^2) on an n^3 periodic box."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    k1 = np.fft.fftfreq(n, 1.0 / n)
+    kx, ky, kz = np.meshgrid(k1, k1, k1, indexing="ij")
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    kk = np.sqrt(k2)
+    amp = np.zeros_like(kk)
+    nz_ = kk > 0
+    # shell energy E(k) spread over 4 pi k^2 modes -> |u_hat| ~ sqrt(E(k)/k^2)
+    amp[nz_] = np.sqrt(kk[nz_] ** 4 * np.exp(-2 * (kk[nz_] / k0) ** 2) / kk[nz_] ** 2)
+    u = []
+    for _ in range(3):
+        ph = rng.uniform(0, 2 * np.pi, kk.shape)
+        u.append(amp * np.exp(1j * ph))
+    # project out the k-parallel part
+    with np.errstate(invalid="ignore", divide="ignore"):
+        dot = (kx * u[0] + ky * u[1] + kz * u[2]) / np.where(nz_, k2, 1.0)
+    u = [u[0] - kx * dot, u[1] - ky * dot, u[2] - kz * dot]
+    vel = [np.real(np.fft.ifftn(c)) for c in u]
+    rms = np.sqrt(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2) / 3)
+    return [np.asfortranarray(v * (urms / rms)) for v in vel]
+
+
+def turbulence_box(n=256, Re=1000.0, Ma=0.3, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0.49,
+                   spatial="SL4", cfl=0.5, seed=20260, urms=0.5, epsilon=1.0, forcing=True):
+    """3-D periodic turbulence box (examples/turbulenceBox.m with the sizes of the 256^3 config)."""
+    X = np.arange(n) / n
+    meshes = [DirMesh(X.copy(), periodic=True, fix_periodic_domain_size=True, xf=1.0) for _ in range(3)]
+    E0 = 1.0 / ((gamma ** 2 - gamma) * Ma ** 2)
+    fr, walls, corners, bc = periodic_box(n, n, n)
+    nm = NumMethods(spatial_derivs=spatial, spatial_derivs_buffer="EX4", filter_strength=alpha_f,
+                    filter_borders="reducedOrder")
+    deriv, filt, types = _coeffs(meshes, fr, nm)
+    bi = init_boundaries(bc, corners, nm.neumann_order, nm.neumann2_order)
+    rnd = lambda v: float(fortran_round(v))
+    forc = []
+    if forcing:
+        forc.append(Forcing("turbulenceGenerator", [1, n, 1, n, 1, n], [rnd(epsilon), rnd(E0)]))
+    case = Case("turbulence_box_%d" % n, n, n, n, X, X.copy(), X.copy(), rnd(Re), rnd(Ma), rnd(Pr),
+                rnd(gamma), rnd(T0), deriv, filt, types, (1, 1, 1), -1.0, bi, [], forc,
+                dxmin=_dxmin(X, X, X, n))
+    u, v, w = solenoidal_field(n, seed, urms=urms)
+    st = {"U": u, "V": v, "W": w, "R": np.ones((n, n, n), order="F"), "E": np.full((n, n, n), E0, order="F")}
+    case.state0 = st
+    case.dt = rnd(cfl / cfl_dt(st, case.Ma, case.dxmin, n))
+    return case

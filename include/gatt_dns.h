@@ -1,0 +1,209 @@
+/*
+ * gatt_dns.h -- C ABI of the MI355X (gfx950) time-advance library.
+ *
+ * The library replaces the Fortran module procedures of the reference time
+ * loop.  Every entry point below names the reference routine it stands in for
+ * (paths relative to the reference checkout).  All arrays use the Fortran
+ * layout of the reference: column-major, x fastest, 1-based index boxes.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative GATT_E_* code on
+ *     failure; gatt_last_error() gives a message.  Nothing is thrown across
+ *     the ABI and nothing calls exit().
+ *   - A context owns all device memory: the state q = (U,V,W,R,E) of `batch`
+ *     members, stage work arrays, coefficient tables and the boundary program.
+ *     State stays resident on the device between calls; only scalars return.
+ *   - Calls on one context are ordered on the context's HIP stream and are not
+ *     re-entrant.  Host pointers are copied before the call returns.
+ *   - 2-D problems (nz == 1) carry U,V,R,E; W is identically zero and the
+ *     reported W change is 0 (RK4_2D passes W through untouched,
+ *     source/Fortran/timeIntegration.F90:8-156).
+ */
+#ifndef GATT_DNS_H
+#define GATT_DNS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GATT_OK = 0,
+  GATT_E_INVALID = -1,   /* bad argument or descriptor */
+  GATT_E_HIP = -2,       /* HIP runtime error */
+  GATT_E_NOMEM = -3,     /* device allocation failed */
+  GATT_E_UNSUPPORTED = -4
+};
+
+/* Fields of the state vector, in the reference's order U,V,W,R,E. */
+enum { GATT_U = 0, GATT_V = 1, GATT_W = 2, GATT_R = 3, GATT_E = 4 };
+
+/* Operator kinds of a line solve. */
+enum { GATT_DERIV = 0, GATT_FILTER = 1 };
+
+/* Pre-factored compact operator along one direction: the arrays the reference
+ * generates in matrices.F90 (source/writeFortranMatrices.m:70-172) and passes
+ * to solveSystem{X,Y,Z}[per] (source/Fortran/derivs.F90:556-948).
+ *   A : (n-1) x ntypes      sub-diagonal factors (periodic: A(1) = LHS(1,n))
+ *   B : (n-periodic) x nt   inverted pivots
+ *   C : (n-1) x ntypes      scaled super-diagonal (periodic: C(1) = LHS(n,1))
+ *   D : n x ntypes          first row of inv(LHS), periodic only (else NULL)
+ *   R : n x (2nR-1) x nt    RHS band, R(i,nR+l,t) multiplies F(i+l)          */
+typedef struct gatt_lineop {
+  int32_t n;
+  int32_t ntypes;
+  int32_t nR;
+  int32_t periodic;
+  const double* A;
+  const double* B;
+  const double* C;
+  const double* D;
+  const double* R;
+} gatt_lineop;
+
+/* A list of boundary boxes (iUd/vUd, iUn/dUn ... of boundaryInfo.F90).
+ *   box   : count x 6 (column-major), [i1 i2 j1 j2 k1 k2], 1-based inclusive
+ *   value : Dirichlet values (count) or NULL
+ *   dir   : Neumann / 2nd-derivative directions 1..6 = xi,xf,yi,yf,zi,zf or NULL */
+typedef struct gatt_box_list {
+  int32_t count;
+  const int32_t* box;
+  const double* value;
+  const int32_t* dir;
+} gatt_box_list;
+
+/* Boundary-list variable slots: U,V,W,P,E (P acts on R through P = R E). */
+enum { GATT_BC_U = 0, GATT_BC_V = 1, GATT_BC_W = 2, GATT_BC_P = 3, GATT_BC_E = 4 };
+
+/* Disturbance hooks called at the end of applyBoundCond (runDisturbances.F90). */
+enum {
+  GATT_DIST_LID_DRIVEN = 1,  /* source/disturbances/lidDrivenMovement.F90: par[0] = U0 */
+  GATT_DIST_HOLD_INLET = 2   /* source/Fortran/holdInlet.F90: par[0] = holdDensity flag */
+};
+
+typedef struct gatt_disturbance {
+  int32_t kind;
+  int32_t box[6];
+  const double* x;      /* X(box(1):box(2)) as written in runDisturbances.F90 */
+  double par[4];
+} gatt_disturbance;
+
+/* Forcing hooks called at the end of nseq (runForcings{2,3}D.F90). */
+enum {
+  GATT_FORCING_TURBULENCE = 1  /* source/disturbances/turbulenceGenerator.F90: par = {epsilon, e0} */
+};
+
+typedef struct gatt_forcing {
+  int32_t kind;
+  int32_t box[6];
+  double par[4];
+} gatt_forcing;
+
+/* Everything applyBoundCond receives (source/Fortran/nseq.F90:327-350). */
+typedef struct gatt_bc_desc {
+  gatt_box_list dir[5];     /* Dirichlet, slots U,V,W,P,E */
+  gatt_box_list neu[5];     /* Neumann (zero gradient), slots U,V,W,P,E */
+  gatt_box_list sec[5];     /* null second derivative, slots U,V,W,P,E */
+  int32_t nCorners;
+  const int32_t* cL;        /* nCorners x 6 */
+  const int32_t* cD;        /* nCorners x 3, entries -1/0/1 */
+  const int32_t* cAdiabatic;/* nCorners */
+  int32_t neumannLength;
+  int32_t neumann2Length;
+  const double* neumannCoeffs;
+  const double* neumann2Coeffs;
+  int32_t nDisturb;
+  const gatt_disturbance* disturb;
+} gatt_bc_desc;
+
+/* One problem = the content of parameters.F90, matrices.F90, boundaryInfo.F90,
+ * runDisturbances.F90, runForcings*.F90 and SFD.mat of the reference. */
+typedef struct gatt_problem_desc {
+  int32_t nx, ny, nz;
+  double Re, Ma, Pr, gamma, T0;
+  gatt_lineop deriv[3];     /* x, y, z (z ignored when nz == 1) */
+  gatt_lineop filter[3];
+  const int32_t* types[3];  /* typeMapX (ny*nz), typeMapY (nx*nz), typeMapZ (nx*ny), 1-based */
+  int32_t filterDir[3];     /* filterX/Y/Z */
+  double filterCharTime;    /* FilterCharTime; <= 0 means full replacement */
+  gatt_bc_desc bc;
+  int32_t nForcing;
+  const gatt_forcing* forcing;
+  int32_t sfdType;          /* 0 off, 1 scalar chi, 2 chi field */
+  double sfdDelta;          /* SFD_Delta; <= 0 freezes the mean (Delta = inf) */
+  double sfdXval;           /* SFD_X_val (type 1) */
+  const double* sfdX;       /* chi field nx*ny*nz (type 2) or NULL */
+  double dxmin[3];
+} gatt_problem_desc;
+
+typedef struct gatt_ctx gatt_ctx;
+
+/* Last error message of the calling thread ("" if none). */
+const char* gatt_last_error(void);
+
+/* Library and device information: writes a short description into buf. */
+int gatt_device_info(char* buf, int32_t len);
+
+/* Create a context for `batch` independent members of one problem
+ * (replaces main.F90:75-80 set-up: decomp_2d_init + include matrices/boundaryInfo). */
+int gatt_create(const gatt_problem_desc* desc, int32_t batch, gatt_ctx** out);
+int gatt_destroy(gatt_ctx* ctx);
+
+/* Host <-> device state copies of member m (importFlow.F90 / exportFlow.F90
+ * without the MAT file).  q is an array of 5 pointers U,V,W,R,E to nx*ny*nz
+ * doubles each (W may be NULL in 2-D). */
+int gatt_set_state(gatt_ctx* ctx, int32_t member, const double* const* q);
+int gatt_get_state(gatt_ctx* ctx, int32_t member, double* const* q);
+/* SFD mean of member m (main.F90:148-164; defaults to the state at the first step). */
+int gatt_set_mean(gatt_ctx* ctx, int32_t member, const double* const* q);
+int gatt_get_mean(gatt_ctx* ctx, int32_t member, double* const* q);
+
+/* One RK4 step q -> q_new with boundary conditions after every stage
+ * (RK4_3D / RK4_2D, source/Fortran/timeIntegration.F90:338-498, 8-156).
+ * The result replaces the state. */
+int gatt_rk4(gatt_ctx* ctx, double t, double dt);
+
+/* Compact filter in x, y, z with blend factor theta (dt/FilterCharTime;
+ * theta <= 0 replaces the field) -- applyFilters3D / applyFilters2D,
+ * source/Fortran/derivs.F90:399-552, 303-397.  Acts on the state in place. */
+int gatt_apply_filters(gatt_ctx* ctx, double theta);
+
+/* applyBoundCond on the state at time t (source/Fortran/nseq.F90:327-518). */
+int gatt_apply_bound_cond(gatt_ctx* ctx, double t);
+
+/* SFD blend on the state (main.F90:381-402). */
+int gatt_sfd(gatt_ctx* ctx, double dt);
+
+/* Right-hand side of the Navier-Stokes equations, nseq3d / nseq2d
+ * (source/Fortran/nseq.F90:142-323, 10-139), of member 0's state.
+ * dq: 5 host pointers receiving dU,dV,dW,dR,dE. */
+int gatt_rhs(gatt_ctx* ctx, double* const* dq);
+
+/* One full time step of the main loop (main.F90:196-462) for every member:
+ * CFL scalars on the current state, RK4, filter, SFD, boundary conditions,
+ * max changes, then q = q_new.  Outputs (for member 0, may be NULL):
+ *   umax[3]      max |U|,|V|,|W| before the step (main.F90:197-205)
+ *   maxchange[5] max |q_new - q| per field (main.F90:414-418)
+ *   diverged     1 if any change is NaN or all are zero (main.F90:434) */
+int gatt_step(gatt_ctx* ctx, double t, double dt, double* umax, double* maxchange, int32_t* diverged);
+
+/* nsteps fixed-dt steps starting at time t0 (timeControl = 1), replayed from
+ * a captured HIP graph; per-step scalars of the last step are returned. */
+int gatt_advance(gatt_ctx* ctx, int32_t nsteps, double t0, double dt, double* maxchange, int32_t* diverged);
+
+/* Unit entry point: one operator along one direction on nfields device
+ * arrays of nx*ny*nz doubles (calcDerivs{X,Y,Z} / one direction of
+ * applyFilters: solveSystem{X,Y,Z}[per], source/Fortran/derivs.F90:556-948).
+ * in/out are arrays of nfields device pointers; out may not alias in. */
+int gatt_line_solve(gatt_ctx* ctx, int32_t dir, int32_t kind, int32_t nfields,
+                    const double* const* in, double* const* out);
+
+/* Synchronise the context's stream. */
+int gatt_sync(gatt_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GATT_DNS_H */
