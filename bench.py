@@ -1,0 +1,140 @@
+"""Benchmark of the FP64 time step on one MI355X.
+
+Workload: the 3-D periodic turbulence box at 256^3 (Re 1000, Ma 0.3, SL4
+derivatives, 10th-order filter every step, turbulenceGenerator forcing),
+fixed-dt RK4.  A step is one iteration of the reference time loop: RK4 with
+boundary conditions after every stage, the filter in x, y and z, the step
+scalars and the state update.
+
+Prints one JSON line.  ``value`` is Mpoint-steps/s with the state resident in
+HBM; ``roofline`` covers the kernel class that takes the largest share of the
+step, with its algorithmic bytes per launch divided by its mean launch time
+measured with HIP events on the library's stream; ``step_roofline`` applies the
+minimum-pass traffic model of the whole step (8 B x 377 array touches per
+point-step without SFD).  ``cpu_baseline`` times the CPU oracle (a scalar C
+restatement of the reference path) on a bounded sample of the same physics.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 Mpoint-steps/s + %HBM roofline; batched-LST & adjoint cost vs plain"
+HBM_PEAK_GBPS = 8000.0
+MODEL_TOUCHES_3D = 377   # minimum-pass pencil schedule, SFD off
+
+KERNELS = {0: "deriv_x", 1: "deriv_y", 2: "deriv_z", 3: "filter_x", 4: "filter_y", 5: "filter_z",
+           6: "visc", 7: "assemble"}
+# launches of each class in one step: derivatives 2 per stage, filters 1 per step
+LAUNCHES_PER_STEP = {0: 8, 1: 8, 2: 8, 3: 1, 4: 1, 5: 1, 6: 4, 7: 4}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n=64, steps=6):
+    from gatt_dns_amd.prep import cases
+    from oracle.oracle import Oracle
+    case = cases.turbulence_box(n)
+    o = Oracle(case)
+    o.run(case.state0, 0.0, case.dt, 1)
+    t0 = time.perf_counter()
+    o.run(case.state0, 0.0, case.dt, steps)
+    el = time.perf_counter() - t0
+    return {"value": case.npoints * steps / el / 1e6, "unit": "Mpoint-steps/s", "cores": 1, "kind": "port",
+            "sample": "CPU oracle (scalar C restatement, gcc -O2, 1 thread) on the same turbulence box at "
+                      "%d^3, %d steps, %.1f s" % (n, steps, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the time-advance path runs on a single MI355X here; "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from gatt_dns_amd import runtime
+    from gatt_dns_amd.prep import cases
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    log("preparing turbulence box %d^3" % args.n)
+    case = cases.turbulence_box(args.n)
+    s = runtime.Solver(case)
+    s.set_state(case.state0)
+    s.apply_bound_cond(0.0)
+    dt = case.dt
+    log("device: %s; dt = %.6g; warmup %d" % (runtime.device_info(), dt, args.warmup))
+    if args.warmup:
+        s.advance(args.warmup, 0.0, dt)
+    s.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mc, dv = s.advance(args.steps, args.warmup * dt, dt)
+    s.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms_step = el / args.steps * 1e3
+    value = case.npoints * args.steps / el / 1e6
+    log("timed %d steps: %.3f ms/step, %.1f Mpoint-steps/s, max changes %s, diverged %s"
+        % (args.steps, ms_step, value, mc, dv))
+
+    # per-kernel-class timing with HIP events on the library stream
+    per_class = {}
+    for k, name in KERNELS.items():
+        if k in (3, 4, 5):
+            continue
+        ms, by = s.time_kernel(k, 10)
+        per_class[name] = {"ms": ms, "bytes": by, "share_ms_per_step": ms * LAUNCHES_PER_STEP[k]}
+    for k in (3, 4, 5):   # filters overwrite the state: time them last
+        ms, by = s.time_kernel(k, 10)
+        per_class[KERNELS[k]] = {"ms": ms, "bytes": by, "share_ms_per_step": ms * LAUNCHES_PER_STEP[k]}
+    dom = max(per_class, key=lambda n: per_class[n]["share_ms_per_step"])
+    d = per_class[dom]
+    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+    for name, v in per_class.items():
+        log("  %-9s %.4f ms/launch  %.0f GB/s algorithmic  %.3f ms/step"
+            % (name, v["ms"], v["bytes"] / (v["ms"] * 1e-3) / 1e9, v["share_ms_per_step"]))
+    step_bytes = 8.0 * MODEL_TOUCHES_3D * case.npoints
+    step_gbps = step_bytes / (ms_step * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle sample)")
+        cpu = cpu_baseline()
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mpoint-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded random-phase solenoidal velocity, E(k)~k^4 exp(-2(k/4)^2), u_rms 0.5)",
+        "config": {"workload": "3-D periodic turbulence box %d^3, Re 1000, Ma 0.3, SL4 + 10th-order filter "
+                               "(alpha 0.49) every step, turbulenceGenerator forcing, fixed-dt RK4" % args.n,
+                   "grid": [args.n] * 3, "dt": dt, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "bytes_per_launch": d["bytes"], "ms_per_launch": round(d["ms"], 5)},
+        "step_roofline": {"model_bytes_per_point_step": 8 * MODEL_TOUCHES_3D, "achieved": round(step_gbps, 1),
+                          "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(step_gbps / HBM_PEAK_GBPS, 4)},
+        "kernels": {k: {"ms_per_launch": round(v["ms"], 5), "ms_per_step": round(v["share_ms_per_step"], 4)}
+                    for k, v in per_class.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,141 @@
+// Boundary conditions (applyBoundCond, source/Fortran/nseq.F90:327-599) as a
+// compiled program of box operations.
+//
+// The host lowers the reference's ordered lists (Dirichlet U,V,W,E,P; Neumann
+// U,V,W,E,P; corners; null second derivative U,V,W,E,P; disturbances) into
+// BcOps and groups consecutive ops whose read and write sets do not intersect
+// into one "wave".  A wave is one launch: every thread owns one point of one
+// op, so a wave is equivalent to running its ops one after another.
+#include "gatt_internal.h"
+
+namespace gatt {
+
+__device__ __forceinline__ long long pidx(int i, int j, int k, int nx, int ny) {
+  return (long long)i + (long long)nx * ((long long)j + (long long)ny * k);
+}
+
+// One-sided extrapolation of applyNeumann (nseq.F90:561-597) at point (i,j,k)
+// along direction dir (1..6), from a field given by an accessor.
+template <typename F>
+__device__ __forceinline__ double extrap(F&& val, int i, int j, int k, int dir, int L, const double* c) {
+  int di = 0, dj = 0, dk = 0;
+  switch (dir) {
+    case 1: di = 1; break;
+    case 2: di = -1; break;
+    case 3: dj = 1; break;
+    case 4: dj = -1; break;
+    case 5: dk = 1; break;
+    default: dk = -1; break;
+  }
+  double v = c[0] * val(i + di, j + dj, k + dk);
+  for (int q = 2; q <= L; ++q) v = v + c[q - 1] * val(i + q * di, j + q * dj, k + q * dk);
+  return v;
+}
+
+__device__ __forceinline__ double powi18(double x) {
+  const double x2 = x * x;
+  const double x4 = x2 * x2;
+  const double x8 = x4 * x4;
+  const double x16 = x8 * x8;
+  return x16 * x2;
+}
+
+__global__ __launch_bounds__(256) void bc_wave_kernel(BcArgs a) {
+  // find this block's op
+  const long long blk = blockIdx.x;
+  int o = a.op_begin;
+  while (o + 1 < a.op_end && a.ops[o + 1].first_block <= blk) ++o;
+  const BcOp& op = a.ops[o];
+  const long long p = (blk - op.first_block) * blockDim.x + threadIdx.x;
+  if (p >= op.npts) return;
+  const int bi = op.box[1] - op.box[0] + 1;
+  const int bj = op.box[3] - op.box[2] + 1;
+  const int i = op.box[0] + (int)(p % bi);
+  const int j = op.box[2] + (int)((p / bi) % bj);
+  const int k = op.box[4] + (int)(p / ((long long)bi * bj));
+  const int nx = a.nx, ny = a.ny;
+  const long long mo = (long long)blockIdx.y * a.member_stride;
+  double* U = a.q[0] + mo;
+  double* R = a.q[3] + mo;
+  double* E = a.q[4] + mo;
+  const long long n = pidx(i, j, k, nx, ny);
+  switch (op.kind) {
+    case BC_SET: {
+      (a.q[op.field] + mo)[n] = op.value;
+      break;
+    }
+    case BC_SET_R_FROM_P: {
+      R[n] = op.value / (a.gamma - 1.0) / E[n];
+      break;
+    }
+    case BC_NEU: {
+      double* F = a.q[op.field] + mo;
+      auto val = [&](int ii, int jj, int kk) { return F[pidx(ii, jj, kk, nx, ny)]; };
+      F[n] = extrap(val, i, j, k, op.dir, op.L, a.coef[op.coef]);
+      break;
+    }
+    case BC_NEU_P: {
+      auto val = [&](int ii, int jj, int kk) {
+        const long long m = pidx(ii, jj, kk, nx, ny);
+        return R[m] * E[m];
+      };
+      const double Pb = extrap(val, i, j, k, op.dir, op.L, a.coef[op.coef]);
+      R[n] = Pb / E[n];
+      break;
+    }
+    case BC_CORNER: {
+      const double nsum = (double)(abs(op.cd[0]) + abs(op.cd[1]) + abs(op.cd[2]));
+      const double* c = a.coef[0];
+      if (op.adiabatic) {
+        auto valE = [&](int ii, int jj, int kk) { return E[pidx(ii, jj, kk, nx, ny)]; };
+        double acc = 0.0;
+        for (int ax = 0; ax < 3; ++ax) {
+          if (op.cd[ax] == 0) continue;
+          const int d = op.cd[ax] == 1 ? 2 * ax + 1 : 2 * ax + 2;
+          acc = acc + extrap(valE, i, j, k, d, op.L, c);
+        }
+        E[n] = acc / nsum;
+      }
+      auto valP = [&](int ii, int jj, int kk) {
+        const long long m = pidx(ii, jj, kk, nx, ny);
+        return R[m] * E[m];
+      };
+      double acc = 0.0;
+      for (int ax = 0; ax < 3; ++ax) {
+        if (op.cd[ax] == 0) continue;
+        const int d = op.cd[ax] == 1 ? 2 * ax + 1 : 2 * ax + 2;
+        acc = acc + extrap(valP, i, j, k, d, op.L, c);
+      }
+      R[n] = acc / E[n] / nsum;
+      break;
+    }
+    case BC_LID: {
+      // lidDrivenMovement.F90:15-17
+      const double* x = a.aux + op.aux;
+      const int nxb = bi;
+      const double s = 2 * (x[i - op.box[0]] - x[0]) / (x[nxb - 1] - x[0]) - 1;
+      const double v = 1 - powi18(s);
+      U[n] = op.value * (v * v);
+      break;
+    }
+    case BC_HOLD: {
+      // holdInlet.F90:18-48: capture on the first call, restore afterwards
+      double* save = a.hold + (long long)blockIdx.y * a.hold_member_stride + op.aux;
+      const int vars[5] = {0, 1, 2, 4, 3};
+      const int nv = op.hold_r ? 5 : 4;
+      for (int v = 0; v < nv; ++v) {
+        double* F = a.q[vars[v]] + mo;
+        if (a.capture) save[(long long)v * op.npts + p] = F[n];
+        else F[n] = save[(long long)v * op.npts + p];
+      }
+      break;
+    }
+  }
+}
+
+void launch_bc_wave(const BcArgs& a, long long nblocks, int nmembers, hipStream_t st) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(bc_wave_kernel, dim3((unsigned)nblocks, nmembers), dim3(256), 0, st, a);
+}
+
+}  // namespace gatt

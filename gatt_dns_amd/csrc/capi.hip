@@ -1,0 +1,998 @@
+// C ABI of libgatt_dns (include/gatt_dns.h): context set-up, the boundary
+// program compiler, and the orchestration of the time step on one HIP stream.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "gatt_internal.h"
+
+using namespace gatt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(GATT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Slab layout of one member: arrays of np doubles.
+enum : int {
+  SL_QA = 0,    // state ping
+  SL_QB = 5,    // state pong
+  SL_S = 10,    // RK stage state
+  SL_DL = 15,   // RK accumulator
+  SL_D = 20,    // 15 first derivatives, (3f + d)
+  SL_T = 35,    // 9 stresses / heat fluxes
+  SL_G = 44,    // 12 second derivatives
+  SL_MEAN = 56, // SFD mean
+  SL_N = 61
+};
+
+struct DirOps {
+  LineOpDev op[2];           // deriv, filter
+  LineShape shape[2];
+  int* types = nullptr;      // per line, 0-based
+  int n1 = 0, nlines = 0;
+  long long stride = 0;
+};
+
+struct Box { int f; int b[6]; };
+
+bool overlap(const Box& a, const Box& b) {
+  if (a.f != b.f) return false;
+  for (int d = 0; d < 3; ++d)
+    if (a.b[2 * d + 1] < b.b[2 * d] || b.b[2 * d + 1] < a.b[2 * d]) return false;
+  return true;
+}
+
+}  // namespace
+
+struct gatt_ctx {
+  int nx = 0, ny = 0, nz = 0, is3d = 0, batch = 1;
+  long long np = 0;
+  long long slab = 0;          // doubles per member
+  double* mem = nullptr;       // batch * slab
+  FlowConst fc{};
+  double gamma = 1.4;
+  DirOps dir[3];
+  std::vector<void*> dev_allocs;
+  int filterDir[3] = {1, 1, 1};
+  double filterCharTime = -1;
+  // boundary program
+  BcOp* d_ops = nullptr;
+  std::vector<BcOp> ops;
+  std::vector<BcWave> waves;
+  double* d_coef[2] = {nullptr, nullptr};
+  double* d_aux = nullptr;
+  double* d_hold = nullptr;
+  long long hold_stride = 0;
+  bool hold_captured = false;
+  // forcing
+  int forcing = 0;
+  double f_eps = 0, f_e0 = 0;
+  double* d_partial = nullptr;
+  double* d_sums = nullptr;
+  int sum_blocks = 0;
+  // SFD
+  int sfdType = 0;
+  double sfdDelta = 0, sfdXval = 0;
+  double* d_chi = nullptr;
+  bool mean_ready = false;
+  // reductions
+  unsigned long long* d_red = nullptr;  // batch * 8
+  double* d_rec = nullptr;              // 5
+  int* d_flag = nullptr;
+  int cur = 0;                          // 0: state in QA, 1: in QB
+  hipStream_t stream = nullptr;
+  // captured double step
+  hipGraphExec_t graph = nullptr;
+  double graph_dt = 0;
+  int graph_cur = -1;
+
+  double* slab_ptr(int idx) { return mem + (long long)idx * np; }
+  double* state(int which) { return slab_ptr(which == 0 ? SL_QA : SL_QB); }
+};
+
+namespace {
+
+template <typename T>
+int dev_upload(gatt_ctx* c, const T* host, size_t count, T** out) {
+  *out = nullptr;
+  if (count == 0) return GATT_OK;
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, count * sizeof(T)));
+  c->dev_allocs.push_back(p);
+  HIPCHK(hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
+  *out = static_cast<T*>(p);
+  return GATT_OK;
+}
+
+LineShape choose_shape(int dir, int N, int nR) {
+  LineShape s;
+  const int H = nR - 1;
+  (void)H;
+  int G;
+  if (dir == 0) {
+    G = 64;
+    while (G > 2 && (N + G - 1) / G < 4) G >>= 1;
+  } else {
+    G = 8;
+    while ((N + G - 1) / G > 32 && G < 64) G <<= 1;
+    while (G > 2 && (N + G - 1) / G < 2) G >>= 1;
+  }
+  s.G = G;
+  s.m = (N + G - 1) / G;
+  s.M = s.m <= 4 ? 4 : s.m <= 8 ? 8 : s.m <= 16 ? 16 : 32;
+  s.waves_per_block = 4;
+  return s;
+}
+
+int upload_lineop(gatt_ctx* c, const gatt_lineop& h, int expectN, LineOpDev* out, const char* what) {
+  if (h.n != expectN) return fail(GATT_E_INVALID, std::string(what) + ": line length does not match the grid");
+  if (h.nR < 1 || h.nR > 11) return fail(GATT_E_UNSUPPORTED, std::string(what) + ": nR outside 1..11");
+  if (h.n < 2 * h.nR - 1) return fail(GATT_E_INVALID, std::string(what) + ": line shorter than the stencil");
+  if (!h.A || !h.B || !h.C || !h.R || (h.periodic && !h.D))
+    return fail(GATT_E_INVALID, std::string(what) + ": missing coefficient array");
+  out->N = h.n;
+  out->ntypes = h.ntypes;
+  out->nR = h.nR;
+  out->periodic = h.periodic ? 1 : 0;
+  const size_t nt = (size_t)h.ntypes, N = (size_t)h.n;
+  double* p;
+  int rc;
+  if ((rc = dev_upload(c, h.A, (N - 1) * nt, &p))) return rc;
+  out->A = p;
+  if ((rc = dev_upload(c, h.B, (N - out->periodic) * nt, &p))) return rc;
+  out->B = p;
+  if ((rc = dev_upload(c, h.C, (N - 1) * nt, &p))) return rc;
+  out->C = p;
+  if (out->periodic) {
+    if ((rc = dev_upload(c, h.D, N * nt, &p))) return rc;
+    out->D = p;
+  }
+  if ((rc = dev_upload(c, h.R, N * (size_t)(2 * h.nR - 1) * nt, &p))) return rc;
+  out->R = p;
+  return GATT_OK;
+}
+
+// ------------------------------------------------------------ BC compiler
+struct OpSets {
+  std::vector<Box> rd, wr;
+};
+
+int compile_bc(gatt_ctx* c, const gatt_bc_desc& bc) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  const int dims[3] = {nx, ny, nz};
+  std::vector<BcOp> ops;
+  std::vector<OpSets> sets;
+  std::vector<double> aux;
+  long long hold_total = 0;
+  auto in_range = [&](const int b[6]) {
+    for (int d = 0; d < 3; ++d)
+      if (b[2 * d] < 0 || b[2 * d + 1] >= dims[d] || b[2 * d] > b[2 * d + 1]) return false;
+    return true;
+  };
+  auto add = [&](BcOp op, OpSets s) -> int {
+    if (!in_range(op.box)) return fail(GATT_E_INVALID, "boundary box outside the grid");
+    for (auto& b : s.rd)
+      if (!in_range(b.b)) return fail(GATT_E_INVALID, "boundary stencil outside the grid");
+    op.npts = (long long)(op.box[1] - op.box[0] + 1) * (op.box[3] - op.box[2] + 1) * (op.box[5] - op.box[4] + 1);
+    ops.push_back(op);
+    sets.push_back(s);
+    return GATT_OK;
+  };
+  auto box_of = [](const gatt_box_list& bl, int i, int b[6]) {
+    for (int q = 0; q < 6; ++q) b[q] = bl.box[i + bl.count * q] - 1;
+  };
+  // field index of the BC slots U,V,W,P,E (P acts on R)
+  const int slot_field[5] = {GATT_U, GATT_V, GATT_W, GATT_R, GATT_E};
+  auto skip_field = [&](int f) { return !c->is3d && f == GATT_W; };
+  // stencil box along Neumann direction d from a plane box
+  auto stencil = [](const int b[6], int d, int L, int out[6]) {
+    for (int q = 0; q < 6; ++q) out[q] = b[q];
+    switch (d) {
+      case 1: out[1] = b[0] + L; break;
+      case 2: out[0] = b[0] - L; out[1] = b[0]; break;
+      case 3: out[3] = b[2] + L; break;
+      case 4: out[2] = b[2] - L; out[3] = b[2]; break;
+      case 5: out[5] = b[4] + L; break;
+      case 6: out[4] = b[4] - L; out[5] = b[4]; break;
+    }
+  };
+  auto plane = [](const int b[6], int d, int out[6]) {
+    for (int q = 0; q < 6; ++q) out[q] = b[q];
+    const int ax = (d - 1) / 2;
+    out[2 * ax + 1] = out[2 * ax];
+  };
+  auto thin = [](const int b[6], int d) {
+    const int ax = (d - 1) / 2;
+    return b[2 * ax] == b[2 * ax + 1];
+  };
+  int rc;
+  // Dirichlet: U, V, W, E (nseq.F90:359-370)
+  const int dord[4] = {GATT_BC_U, GATT_BC_V, GATT_BC_W, GATT_BC_E};
+  for (int s : dord) {
+    const gatt_box_list& bl = bc.dir[s];
+    const int f = slot_field[s];
+    if (skip_field(f)) continue;
+    for (int i = 0; i < bl.count; ++i) {
+      BcOp op{};
+      op.kind = BC_SET;
+      op.field = f;
+      box_of(bl, i, op.box);
+      op.value = bl.value[i];
+      OpSets st;
+      Box w{f, {}};
+      std::memcpy(w.b, op.box, sizeof(w.b));
+      st.wr.push_back(w);
+      if ((rc = add(op, st))) return rc;
+    }
+  }
+  // Dirichlet pressure (nseq.F90:371-373)
+  {
+    const gatt_box_list& bl = bc.dir[GATT_BC_P];
+    for (int i = 0; i < bl.count; ++i) {
+      BcOp op{};
+      op.kind = BC_SET_R_FROM_P;
+      op.field = GATT_R;
+      box_of(bl, i, op.box);
+      op.value = bl.value[i];
+      OpSets st;
+      Box r{GATT_E, {}}, w{GATT_R, {}};
+      std::memcpy(r.b, op.box, sizeof(r.b));
+      std::memcpy(w.b, op.box, sizeof(w.b));
+      st.rd.push_back(r);
+      st.wr.push_back(w);
+      if ((rc = add(op, st))) return rc;
+    }
+  }
+  auto neumann_lists = [&](const gatt_box_list* lists, int L, int coef) -> int {
+    for (int s : dord) {
+      const gatt_box_list& bl = lists[s];
+      const int f = slot_field[s];
+      if (skip_field(f)) continue;
+      for (int i = 0; i < bl.count; ++i) {
+        int b[6];
+        box_of(bl, i, b);
+        BcOp op{};
+        op.kind = BC_NEU;
+        op.field = f;
+        op.dir = bl.dir[i];
+        if (op.dir < 1 || op.dir > 6) return fail(GATT_E_INVALID, "Neumann direction outside 1..6");
+        op.L = L;
+        op.coef = coef;
+        plane(b, op.dir, op.box);
+        OpSets st;
+        Box r{f, {}}, w{f, {}};
+        stencil(op.box, op.dir, L, r.b);
+        std::memcpy(w.b, op.box, sizeof(w.b));
+        st.rd.push_back(r);
+        st.wr.push_back(w);
+        if ((rc = add(op, st))) return rc;
+      }
+    }
+    const gatt_box_list& bl = lists[GATT_BC_P];
+    for (int i = 0; i < bl.count; ++i) {
+      int b[6];
+      box_of(bl, i, b);
+      BcOp op{};
+      op.kind = BC_NEU_P;
+      op.field = GATT_R;
+      op.dir = bl.dir[i];
+      if (op.dir < 1 || op.dir > 6) return fail(GATT_E_INVALID, "Neumann direction outside 1..6");
+      if (!thin(b, op.dir)) return fail(GATT_E_UNSUPPORTED, "pressure Neumann box thicker than one plane");
+      op.L = L;
+      op.coef = coef;
+      plane(b, op.dir, op.box);
+      OpSets st;
+      Box r1{GATT_R, {}}, r2{GATT_E, {}}, w{GATT_R, {}};
+      stencil(op.box, op.dir, L, r1.b);
+      std::memcpy(r2.b, r1.b, sizeof(r2.b));
+      std::memcpy(w.b, op.box, sizeof(w.b));
+      st.rd.push_back(r1);
+      st.rd.push_back(r2);
+      st.wr.push_back(w);
+      if ((rc = add(op, st))) return rc;
+    }
+    return GATT_OK;
+  };
+  // Neumann (nseq.F90:377-399)
+  if ((rc = neumann_lists(bc.neu, bc.neumannLength, 0))) return rc;
+  // corners (nseq.F90:402-488)
+  for (int i = 0; i < bc.nCorners; ++i) {
+    BcOp op{};
+    op.kind = BC_CORNER;
+    op.field = GATT_R;
+    for (int q = 0; q < 6; ++q) op.box[q] = bc.cL[i + bc.nCorners * q] - 1;
+    for (int q = 0; q < 3; ++q) op.cd[q] = bc.cD[i + bc.nCorners * q];
+    op.adiabatic = bc.cAdiabatic[i] == 1;
+    op.L = bc.neumannLength;
+    if (op.cd[0] == 0 && op.cd[1] == 0 && op.cd[2] == 0) return fail(GATT_E_INVALID, "corner without direction");
+    OpSets st;
+    int bb[6];
+    std::memcpy(bb, op.box, sizeof(bb));
+    for (int ax = 0; ax < 3; ++ax) {
+      if (op.cd[ax] == 0) continue;
+      const int d = op.cd[ax] == 1 ? 2 * ax + 1 : 2 * ax + 2;
+      if (!thin(op.box, d)) return fail(GATT_E_UNSUPPORTED, "corner box thicker than one point across a wall");
+      int sb[6];
+      stencil(op.box, d, op.L, sb);
+      for (int q = 0; q < 3; ++q) {
+        bb[2 * q] = std::min(bb[2 * q], sb[2 * q]);
+        bb[2 * q + 1] = std::max(bb[2 * q + 1], sb[2 * q + 1]);
+      }
+    }
+    Box r1{GATT_R, {}}, r2{GATT_E, {}}, w1{GATT_R, {}}, w2{GATT_E, {}};
+    std::memcpy(r1.b, bb, sizeof(bb));
+    std::memcpy(r2.b, bb, sizeof(bb));
+    std::memcpy(w1.b, op.box, sizeof(bb));
+    std::memcpy(w2.b, op.box, sizeof(bb));
+    st.rd.push_back(r1);
+    st.rd.push_back(r2);
+    st.wr.push_back(w1);
+    if (op.adiabatic) st.wr.push_back(w2);
+    if ((rc = add(op, st))) return rc;
+  }
+  // null second derivative (nseq.F90:491-513)
+  if ((rc = neumann_lists(bc.sec, bc.neumann2Length, 1))) return rc;
+  // disturbances (runDisturbances.F90)
+  for (int i = 0; i < bc.nDisturb; ++i) {
+    const gatt_disturbance& ds = bc.disturb[i];
+    BcOp op{};
+    for (int q = 0; q < 6; ++q) op.box[q] = ds.box[q] - 1;
+    OpSets st;
+    if (ds.kind == GATT_DIST_LID_DRIVEN) {
+      op.kind = BC_LID;
+      op.field = GATT_U;
+      op.value = ds.par[0];
+      op.aux = (int)aux.size();
+      const int nb = op.box[1] - op.box[0] + 1;
+      if (!ds.x) return fail(GATT_E_INVALID, "lid disturbance without x coordinates");
+      for (int q = 0; q < nb; ++q) aux.push_back(ds.x[q]);
+      Box w{GATT_U, {}};
+      std::memcpy(w.b, op.box, sizeof(w.b));
+      st.wr.push_back(w);
+    } else if (ds.kind == GATT_DIST_HOLD_INLET) {
+      op.kind = BC_HOLD;
+      op.hold_r = ds.par[0] > 0.5;
+      op.aux = (int)hold_total;
+      const long long npts =
+          (long long)(op.box[1] - op.box[0] + 1) * (op.box[3] - op.box[2] + 1) * (op.box[5] - op.box[4] + 1);
+      hold_total += 5 * npts;
+      for (int f = 0; f < 5; ++f) {
+        Box b{f, {}};
+        std::memcpy(b.b, op.box, sizeof(b.b));
+        st.rd.push_back(b);
+        st.wr.push_back(b);
+      }
+    } else {
+      return fail(GATT_E_UNSUPPORTED, "unknown disturbance kind");
+    }
+    if ((rc = add(op, st))) return rc;
+  }
+  // group into waves of mutually independent consecutive ops
+  std::vector<BcWave> waves;
+  std::vector<Box> wr_acc, rd_acc;
+  int begin = 0;
+  long long blocks = 0;
+  for (int i = 0; i < (int)ops.size(); ++i) {
+    bool conflict = false;
+    for (auto& w : sets[i].wr) {
+      for (auto& x : wr_acc) conflict |= overlap(w, x);
+      for (auto& x : rd_acc) conflict |= overlap(w, x);
+    }
+    for (auto& r : sets[i].rd)
+      for (auto& x : wr_acc) conflict |= overlap(r, x);
+    if (conflict && i > begin) {
+      waves.push_back({begin, i, blocks});
+      begin = i;
+      blocks = 0;
+      wr_acc.clear();
+      rd_acc.clear();
+    }
+    ops[i].first_block = blocks;
+    blocks += (ops[i].npts + 255) / 256;
+    wr_acc.insert(wr_acc.end(), sets[i].wr.begin(), sets[i].wr.end());
+    rd_acc.insert(rd_acc.end(), sets[i].rd.begin(), sets[i].rd.end());
+  }
+  if ((int)ops.size() > begin) waves.push_back({begin, (int)ops.size(), blocks});
+  c->ops = ops;
+  c->waves = waves;
+  if (!ops.empty() && (rc = dev_upload(c, ops.data(), ops.size(), &c->d_ops))) return rc;
+  if (!aux.empty() && (rc = dev_upload(c, aux.data(), aux.size(), &c->d_aux))) return rc;
+  if (bc.neumannLength > 0 && (rc = dev_upload(c, bc.neumannCoeffs, (size_t)bc.neumannLength, &c->d_coef[0])))
+    return rc;
+  if (bc.neumann2Length > 0 && (rc = dev_upload(c, bc.neumann2Coeffs, (size_t)bc.neumann2Length, &c->d_coef[1])))
+    return rc;
+  if (hold_total > 0) {
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, (size_t)hold_total * c->batch * sizeof(double)));
+    c->dev_allocs.push_back(p);
+    c->d_hold = static_cast<double*>(p);
+    c->hold_stride = hold_total;
+  }
+  return GATT_OK;
+}
+
+// ------------------------------------------------------------- execution
+void run_bc(gatt_ctx* c, double* q0, double t) {
+  (void)t;
+  for (const BcWave& w : c->waves) {
+    BcArgs a{};
+    a.ops = c->d_ops;
+    a.op_begin = w.op_begin;
+    a.op_end = w.op_end;
+    for (int f = 0; f < 5; ++f) a.q[f] = q0 + (long long)f * c->np;
+    a.member_stride = c->slab;
+    a.coef[0] = c->d_coef[0];
+    a.coef[1] = c->d_coef[1];
+    a.aux = c->d_aux;
+    a.hold = c->d_hold;
+    a.hold_member_stride = c->hold_stride;
+    a.nx = c->nx;
+    a.ny = c->ny;
+    a.nz = c->nz;
+    a.gamma = c->gamma;
+    a.capture = c->hold_captured ? 0 : 1;
+    launch_bc_wave(a, w.nblocks, c->batch, c->stream);
+  }
+  c->hold_captured = true;
+}
+
+// One line solve along `dir` for nf fields (member 0 pointers, member stride = slab).
+void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double* const* out, double theta,
+               int mode, int nmembers, long long member_stride) {
+  DirOps& d = c->dir[dir];
+  SolveArgs a{};
+  a.op = d.op[kind];
+  a.types = d.types;
+  a.dir = dir;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  a.n1 = d.n1;
+  a.nlines = d.nlines;
+  a.stride = d.stride;
+  a.G = d.shape[kind].G;
+  a.m = d.shape[kind].m;
+  a.nf = nf;
+  a.member_stride = member_stride;
+  for (int f = 0; f < nf; ++f) {
+    a.in[f] = in[f];
+    a.out[f] = out[f];
+  }
+  a.theta = theta;
+  a.mode = mode;
+  launch_line_solve(a, d.shape[kind], nmembers, c->stream);
+}
+
+// Navier-Stokes RHS of the stage state S and one RK stage (nseq3d/nseq2d + RK4 update).
+void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, double dt) {
+  const long long np = c->np;
+  double* D = c->slab_ptr(SL_D);
+  double* T = c->slab_ptr(SL_T);
+  double* G = c->slab_ptr(SL_G);
+  std::vector<int> flds = c->is3d ? std::vector<int>{0, 1, 2, 3, 4} : std::vector<int>{0, 1, 3, 4};
+  const int ndir = c->is3d ? 3 : 2;
+  if (c->forcing && c->is3d) {
+    SumArgs sa{};
+    sa.np = np;
+    sa.member_stride = c->slab;
+    sa.S = S;
+    sa.partial = c->d_partial;
+    sa.sums = c->d_sums;
+    sa.nblocks = c->sum_blocks;
+    launch_forcing_sums(sa, c->batch, c->stream);
+  }
+  for (int d = 0; d < ndir; ++d) {
+    double* in[kMaxPtr];
+    double* out[kMaxPtr];
+    int nf = 0;
+    for (int f : flds) {
+      in[nf] = S + (long long)f * np;
+      out[nf] = D + (long long)(3 * f + d) * np;
+      ++nf;
+    }
+    run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, c->batch, c->slab);
+  }
+  ViscArgs va{};
+  va.c = c->fc;
+  va.np = np;
+  va.member_stride = c->slab;
+  va.E = S + 4 * np;
+  va.D = D;
+  va.T = T;
+  va.is3d = c->is3d;
+  launch_visc(va, c->batch, c->stream);
+  // second derivatives: x: txx,txy,txz,qx  y: txy,tyy,tyz,qy  z: txz,tyz,tzz,qz
+  const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
+  for (int d = 0; d < ndir; ++d) {
+    double* in[kMaxPtr];
+    double* out[kMaxPtr];
+    int nf = 0;
+    for (int q = 0; q < 4; ++q) {
+      if (!c->is3d && q == 2) continue;
+      in[nf] = T + (long long)tin3[d][q] * np;
+      out[nf] = G + (long long)(4 * d + q) * np;
+      ++nf;
+    }
+    run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, c->batch, c->slab);
+  }
+  AssembleArgs aa{};
+  aa.c = c->fc;
+  aa.np = np;
+  aa.member_stride = c->slab;
+  aa.S = S;
+  aa.Q = Q;
+  aa.Out = Out;
+  aa.Delta = c->slab_ptr(SL_DL);
+  aa.D = D;
+  aa.T = T;
+  aa.G = G;
+  aa.sums = (c->forcing && c->is3d) ? c->d_sums : nullptr;
+  aa.forcing_eps = c->f_eps;
+  aa.forcing_e0 = c->f_e0;
+  aa.stage = stage;
+  aa.dt = dt;
+  aa.is3d = c->is3d;
+  launch_assemble(aa, c->batch, c->stream);
+}
+
+// RK4_3D / RK4_2D: state `from` -> `to` (timeIntegration.F90:382-496)
+void run_rk4(gatt_ctx* c, double* from, double* to, double t, double dt) {
+  double* S = c->slab_ptr(SL_S);
+  run_rhs_stage(c, from, from, S, 1, dt);
+  run_bc(c, S, t + dt / 2.0);
+  run_rhs_stage(c, S, from, S, 2, dt);
+  run_bc(c, S, t + dt / 2.0);
+  run_rhs_stage(c, S, from, S, 3, dt);
+  run_bc(c, S, t + dt);
+  run_rhs_stage(c, S, from, to, 4, dt);
+  run_bc(c, to, t + dt);
+}
+
+void run_filters(gatt_ctx* c, double* q, double theta) {
+  std::vector<int> flds = c->is3d ? std::vector<int>{0, 1, 2, 3, 4} : std::vector<int>{0, 1, 3, 4};
+  const int ndir = c->is3d ? 3 : 2;
+  for (int d = 0; d < ndir; ++d) {
+    if (!c->filterDir[d]) continue;
+    double* ptr[kMaxPtr];
+    int nf = 0;
+    for (int f : flds) ptr[nf++] = q + (long long)f * c->np;
+    run_solve(c, d, GATT_FILTER, nf, ptr, ptr, theta, 1, c->batch, c->slab);
+  }
+}
+
+void run_sfd(gatt_ctx* c, double* q, double dt) {
+  if (c->sfdType <= 0) return;
+  SfdArgs s{};
+  s.np = c->np;
+  s.member_stride = c->slab;
+  s.Q = q;
+  s.Mean = c->slab_ptr(SL_MEAN);
+  s.chi = c->sfdType == 2 ? c->d_chi : nullptr;
+  s.chi_val = c->sfdXval;
+  s.dt = dt;
+  s.delta = c->sfdDelta;
+  s.nf_mask = c->is3d ? 0x1f : 0x1b;
+  launch_sfd(s, c->batch, c->stream);
+}
+
+void ensure_mean(gatt_ctx* c) {
+  if (c->sfdType > 0 && !c->mean_ready) {
+    for (int b = 0; b < c->batch; ++b)
+      launch_copy(c->slab_ptr(SL_MEAN) + b * c->slab, c->state(c->cur) + b * c->slab, 5 * c->np, c->stream);
+    c->mean_ready = true;
+  }
+}
+
+// main.F90:255-420 for all members: from state `cur` into the other buffer.
+void enqueue_step(gatt_ctx* c, int cur, double t, double dt) {
+  double* from = c->state(cur);
+  double* to = c->state(1 - cur);
+  launch_clear_u64(c->d_red, 8 * c->batch, c->stream);
+  run_rk4(c, from, to, t, dt);
+  run_filters(c, to, dt / c->filterCharTime);
+  run_sfd(c, to, dt);
+  run_bc(c, to, t + dt);
+  ReduceArgs r{};
+  r.np = c->np;
+  r.member_stride = c->slab;
+  r.Qn = to;
+  r.Q = from;
+  r.out = c->d_red;
+  r.is3d = c->is3d;
+  launch_step_reduce(r, c->batch, c->stream);
+  launch_record(c->d_red, c->d_rec, c->d_flag, c->batch, c->stream);
+}
+
+int check_ctx(gatt_ctx* c) {
+  if (!c) return fail(GATT_E_INVALID, "null context");
+  return GATT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gatt_last_error(void) { return g_err.c_str(); }
+
+int gatt_device_info(char* buf, int32_t len) {
+  hipDeviceProp_t prop;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  if (buf && len > 0) snprintf(buf, (size_t)len, "%s (%s), %d CUs", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  return GATT_OK;
+}
+
+int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
+  if (!d || !out) return fail(GATT_E_INVALID, "null argument");
+  *out = nullptr;
+  if (batch < 1) return fail(GATT_E_INVALID, "batch must be >= 1");
+  if (d->nx < 2 || d->ny < 2 || d->nz < 1) return fail(GATT_E_INVALID, "grid must be at least 2x2x1");
+  std::unique_ptr<gatt_ctx> c(new gatt_ctx);
+  c->nx = d->nx;
+  c->ny = d->ny;
+  c->nz = d->nz;
+  c->is3d = d->nz > 1;
+  c->batch = batch;
+  c->np = (long long)d->nx * d->ny * d->nz;
+  c->slab = (long long)SL_N * c->np;
+  c->gamma = d->gamma;
+  c->fc.gm1 = d->gamma - 1.0;
+  c->fc.invRe = 1.0 / d->Re;
+  c->fc.muA = 1.0 + 110.0 / d->T0;
+  c->fc.muB = 110.0 / d->T0;
+  c->fc.muC = d->gamma * (d->gamma - 1.0) * d->Ma * d->Ma;
+  c->fc.qk = d->gamma / d->Re / d->Pr;
+  for (int k = 0; k < 3; ++k) c->filterDir[k] = d->filterDir[k];
+  c->filterCharTime = d->filterCharTime;
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  const int dims[3] = {d->nx, d->ny, d->nz};
+  const int ndir = c->is3d ? 3 : 2;
+  int rc;
+  for (int k = 0; k < ndir; ++k) {
+    DirOps& o = c->dir[k];
+    if ((rc = upload_lineop(c.get(), d->deriv[k], dims[k], &o.op[0], "deriv"))) return rc;
+    if ((rc = upload_lineop(c.get(), d->filter[k], dims[k], &o.op[1], "filter"))) return rc;
+    if (k == 0) { o.n1 = d->ny; o.nlines = d->ny * d->nz; o.stride = 1; }
+    if (k == 1) { o.n1 = d->nx; o.nlines = d->nx * d->nz; o.stride = d->nx; }
+    if (k == 2) { o.n1 = d->nx; o.nlines = d->nx * d->ny; o.stride = (long long)d->nx * d->ny; }
+    if (!d->types[k]) return fail(GATT_E_INVALID, "missing type map");
+    std::vector<int> tm(d->types[k], d->types[k] + o.nlines);
+    for (auto& v : tm) {
+      v -= 1;
+      if (v < 0 || v >= o.op[0].ntypes || v >= o.op[1].ntypes) return fail(GATT_E_INVALID, "type id outside 1..ntypes");
+    }
+    if ((rc = dev_upload(c.get(), tm.data(), tm.size(), &o.types))) return rc;
+    for (int kind = 0; kind < 2; ++kind) {
+      o.shape[kind] = choose_shape(k, dims[k], o.op[kind].nR);
+      if (!line_shape_supported(o.shape[kind].M, o.op[kind].nR))
+        return fail(GATT_E_UNSUPPORTED, "line length too long for the compiled register tiles");
+    }
+  }
+  if ((rc = compile_bc(c.get(), d->bc))) return rc;
+  for (int i = 0; i < d->nForcing; ++i) {
+    if (d->forcing[i].kind != GATT_FORCING_TURBULENCE) return fail(GATT_E_UNSUPPORTED, "unknown forcing kind");
+    const int* b = d->forcing[i].box;
+    if (b[0] != 1 || b[1] != d->nx || b[2] != 1 || b[3] != d->ny || b[4] != 1 || b[5] != d->nz)
+      return fail(GATT_E_UNSUPPORTED, "turbulence forcing must cover the whole grid");
+    c->forcing = 1;
+    c->f_eps = d->forcing[i].par[0];
+    c->f_e0 = d->forcing[i].par[1];
+  }
+  if (c->forcing) {
+    c->sum_blocks = 1024;
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, sizeof(double) * 5 * c->sum_blocks * batch));
+    c->dev_allocs.push_back(p);
+    c->d_partial = static_cast<double*>(p);
+    HIPCHK(hipMalloc(&p, sizeof(double) * 8 * batch));
+    c->dev_allocs.push_back(p);
+    c->d_sums = static_cast<double*>(p);
+  }
+  c->sfdType = d->sfdType;
+  c->sfdDelta = d->sfdDelta;
+  c->sfdXval = d->sfdXval;
+  if (c->sfdType == 2) {
+    if (!d->sfdX) return fail(GATT_E_INVALID, "SFD type 2 needs the chi field");
+    if ((rc = dev_upload(c.get(), d->sfdX, (size_t)c->np, &c->d_chi))) return rc;
+  }
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, sizeof(unsigned long long) * 8 * batch));
+  c->dev_allocs.push_back(p);
+  c->d_red = static_cast<unsigned long long*>(p);
+  HIPCHK(hipMalloc(&p, sizeof(double) * 8));
+  c->dev_allocs.push_back(p);
+  c->d_rec = static_cast<double*>(p);
+  HIPCHK(hipMalloc(&p, sizeof(int)));
+  c->dev_allocs.push_back(p);
+  c->d_flag = static_cast<int*>(p);
+  HIPCHK(hipMemset(c->d_flag, 0, sizeof(int)));
+  hipError_t e = hipMalloc(&p, sizeof(double) * (size_t)c->slab * batch);
+  if (e != hipSuccess) return fail(GATT_E_NOMEM, "state allocation failed");
+  c->mem = static_cast<double*>(p);
+  HIPCHK(hipMemset(c->mem, 0, sizeof(double) * (size_t)c->slab * batch));
+  *out = c.release();
+  return GATT_OK;
+}
+
+int gatt_destroy(gatt_ctx* c) {
+  if (!c) return GATT_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->graph) hipGraphExecDestroy(c->graph);
+  for (void* p : c->dev_allocs) hipFree(p);
+  if (c->mem) hipFree(c->mem);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return GATT_OK;
+}
+
+static int copy_fields(gatt_ctx* c, int32_t member, double* dev, const double* const* hq, bool to_dev) {
+  if (member < 0 || member >= c->batch) return fail(GATT_E_INVALID, "member out of range");
+  double* base = dev + (long long)member * c->slab;
+  for (int f = 0; f < 5; ++f) {
+    if (!c->is3d && f == GATT_W) continue;
+    if (!hq[f]) return fail(GATT_E_INVALID, "null field pointer");
+    double* dptr = base + (long long)f * c->np;
+    if (to_dev) HIPCHK(hipMemcpyAsync(dptr, hq[f], sizeof(double) * c->np, hipMemcpyHostToDevice, c->stream));
+    else HIPCHK(hipMemcpyAsync((void*)hq[f], dptr, sizeof(double) * c->np, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_set_state(gatt_ctx* c, int32_t member, const double* const* q) {
+  if (check_ctx(c) || !q) return fail(GATT_E_INVALID, "null argument");
+  return copy_fields(c, member, c->state(c->cur), q, true);
+}
+
+int gatt_get_state(gatt_ctx* c, int32_t member, double* const* q) {
+  if (check_ctx(c) || !q) return fail(GATT_E_INVALID, "null argument");
+  int rc = copy_fields(c, member, c->state(c->cur), (const double* const*)q, false);
+  if (rc) return rc;
+  if (!c->is3d && q[GATT_W]) std::memset(q[GATT_W], 0, sizeof(double) * c->np);
+  return GATT_OK;
+}
+
+int gatt_set_mean(gatt_ctx* c, int32_t member, const double* const* q) {
+  if (check_ctx(c) || !q) return fail(GATT_E_INVALID, "null argument");
+  int rc = copy_fields(c, member, c->slab_ptr(SL_MEAN), q, true);
+  if (!rc) c->mean_ready = true;
+  return rc;
+}
+
+int gatt_get_mean(gatt_ctx* c, int32_t member, double* const* q) {
+  if (check_ctx(c) || !q) return fail(GATT_E_INVALID, "null argument");
+  return copy_fields(c, member, c->slab_ptr(SL_MEAN), (const double* const*)q, false);
+}
+
+int gatt_rk4(gatt_ctx* c, double t, double dt) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  run_rk4(c, c->state(c->cur), c->state(1 - c->cur), t, dt);
+  c->cur = 1 - c->cur;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_apply_filters(gatt_ctx* c, double theta) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  run_filters(c, c->state(c->cur), theta);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_apply_bound_cond(gatt_ctx* c, double t) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  run_bc(c, c->state(c->cur), t);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_sfd(gatt_ctx* c, double dt) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  ensure_mean(c);
+  run_sfd(c, c->state(c->cur), dt);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_rhs(gatt_ctx* c, double* const* dq) {
+  if (check_ctx(c) || !dq) return fail(GATT_E_INVALID, "null argument");
+  double* S = c->state(c->cur);
+  double* O = c->slab_ptr(SL_S);
+  run_rhs_stage(c, S, S, O, 0, 0.0);
+  HIPCHK(hipGetLastError());
+  for (int f = 0; f < 5; ++f) {
+    if (!dq[f]) continue;
+    if (!c->is3d && f == GATT_W) {
+      std::memset(dq[f], 0, sizeof(double) * c->np);
+      continue;
+    }
+    HIPCHK(hipMemcpyAsync(dq[f], O + (long long)f * c->np, sizeof(double) * c->np, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+static int read_scalars(gatt_ctx* c, double* umax, double* maxchange, int32_t* diverged) {
+  unsigned long long red[8];
+  int flag = 0;
+  HIPCHK(hipMemcpyAsync(red, c->d_red, sizeof(red), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double v[8];
+  for (int i = 0; i < 8; ++i) std::memcpy(&v[i], &red[i], sizeof(double));
+  if (maxchange)
+    for (int i = 0; i < 5; ++i) maxchange[i] = v[i];
+  if (umax)
+    for (int i = 0; i < 3; ++i) umax[i] = v[5 + i];
+  if (diverged) *diverged = flag;
+  return GATT_OK;
+}
+
+int gatt_step(gatt_ctx* c, double t, double dt, double* umax, double* maxchange, int32_t* diverged) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  ensure_mean(c);
+  enqueue_step(c, c->cur, t, dt);
+  c->cur = 1 - c->cur;
+  HIPCHK(hipGetLastError());
+  return read_scalars(c, umax, maxchange, diverged);
+}
+
+int gatt_advance(gatt_ctx* c, int32_t nsteps, double t0, double dt, double* maxchange, int32_t* diverged) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  if (nsteps < 0) return fail(GATT_E_INVALID, "nsteps < 0");
+  ensure_mean(c);
+  if (!c->hold_captured) run_bc(c, c->state(c->cur), t0);  // first call captures holdInlet state
+  int s = 0;
+  if (c->graph && (c->graph_dt != dt || c->graph_cur != c->cur)) {
+    hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  if (nsteps >= 2 && !c->graph) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(c, c->cur, t0, dt);
+    enqueue_step(c, 1 - c->cur, t0 + dt, dt);
+    HIPCHK(hipStreamEndCapture(c->stream, &g));
+    HIPCHK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    c->graph_dt = dt;
+    c->graph_cur = c->cur;
+  }
+  for (; s + 2 <= nsteps; s += 2) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+  if (s < nsteps) {
+    enqueue_step(c, c->cur, t0 + s * dt, dt);
+    c->cur = 1 - c->cur;
+    if (c->graph) {
+      hipGraphExecDestroy(c->graph);
+      c->graph = nullptr;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return read_scalars(c, nullptr, maxchange, diverged);
+}
+
+int gatt_line_solve(gatt_ctx* c, int32_t dir, int32_t kind, int32_t nfields, const double* const* in,
+                    double* const* out) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  if (dir < 0 || dir > 2 || (!c->is3d && dir == 2)) return fail(GATT_E_INVALID, "direction out of range");
+  if (kind != GATT_DERIV && kind != GATT_FILTER) return fail(GATT_E_INVALID, "unknown operator kind");
+  if (nfields < 1 || nfields > kMaxPtr || !in || !out) return fail(GATT_E_INVALID, "bad field list");
+  double* ip[kMaxPtr];
+  double* op[kMaxPtr];
+  for (int f = 0; f < nfields; ++f) {
+    if (!in[f] || !out[f] || in[f] == out[f]) return fail(GATT_E_INVALID, "null or aliasing field pointer");
+    ip[f] = const_cast<double*>(in[f]);
+    op[f] = out[f];
+  }
+  run_solve(c, dir, kind, nfields, ip, op, 0.0, 0, 1, 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_sync(gatt_ctx* c) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_time_kernel(gatt_ctx* c, int32_t kernel, int32_t reps, double* avg_ms, double* bytes) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  if (reps < 1) return fail(GATT_E_INVALID, "reps < 1");
+  const long long np = c->np;
+  const int ndir = c->is3d ? 3 : 2;
+  std::vector<int> flds = c->is3d ? std::vector<int>{0, 1, 2, 3, 4} : std::vector<int>{0, 1, 3, 4};
+  double* S = c->state(c->cur);
+  double* D = c->slab_ptr(SL_D);
+  double* T = c->slab_ptr(SL_T);
+  double* G = c->slab_ptr(SL_G);
+  int launches = 0;
+  double arrays = 0;  // arrays touched per rep
+  auto body = [&]() {
+    if (kernel >= GATT_K_DERIV_X && kernel <= GATT_K_DERIV_Z) {
+      const int d = kernel - GATT_K_DERIV_X;
+      double* in[kMaxPtr];
+      double* out[kMaxPtr];
+      int nf = 0;
+      for (int f : flds) { in[nf] = S + (long long)f * np; out[nf] = D + (long long)(3 * f + d) * np; ++nf; }
+      run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, c->batch, c->slab);
+      const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
+      int nf2 = 0;
+      for (int q = 0; q < 4; ++q) {
+        if (!c->is3d && q == 2) continue;
+        in[nf2] = T + (long long)tin3[d][q] * np;
+        out[nf2] = G + (long long)(4 * d + q) * np;
+        ++nf2;
+      }
+      run_solve(c, d, GATT_DERIV, nf2, in, out, 0.0, 0, c->batch, c->slab);
+      launches = 2;
+      arrays = 2.0 * (nf + nf2);
+    } else if (kernel >= GATT_K_FILTER_X && kernel <= GATT_K_FILTER_Z) {
+      const int d = kernel - GATT_K_FILTER_X;
+      double* ptr[kMaxPtr];
+      int nf = 0;
+      for (int f : flds) ptr[nf++] = S + (long long)f * np;
+      run_solve(c, d, GATT_FILTER, nf, ptr, ptr, -1.0, 1, c->batch, c->slab);
+      launches = 1;
+      arrays = 2.0 * nf;
+    } else if (kernel == GATT_K_VISC) {
+      ViscArgs va{};
+      va.c = c->fc; va.np = np; va.member_stride = c->slab; va.E = S + 4 * np; va.D = D; va.T = T;
+      va.is3d = c->is3d;
+      launch_visc(va, c->batch, c->stream);
+      launches = 1;
+      arrays = c->is3d ? 13 + 9 : 7 + 5;
+    } else {
+      AssembleArgs aa{};
+      aa.c = c->fc; aa.np = np; aa.member_stride = c->slab; aa.S = c->slab_ptr(SL_S); aa.Q = S;
+      aa.Out = c->slab_ptr(SL_S); aa.Delta = c->slab_ptr(SL_DL); aa.D = D; aa.T = T; aa.G = G;
+      aa.sums = nullptr; aa.stage = 2; aa.dt = 0.0; aa.is3d = c->is3d;
+      launch_assemble(aa, c->batch, c->stream);
+      launches = 1;
+      // reads S(5) D(15) T(6) G(12) Q(5) Delta(5), writes Delta(5) Out(5); 2-D: 4, 8, 3, 6, 4, 4, 4, 4
+      arrays = c->is3d ? 58 : 37;
+    }
+  };
+  if (kernel < 0 || kernel > GATT_K_ASSEMBLE || (!c->is3d && (kernel == GATT_K_DERIV_Z || kernel == GATT_K_FILTER_Z)))
+    return fail(GATT_E_INVALID, "unknown kernel class");
+  (void)ndir;
+  body();  // warm-up
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; ++r) body();
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  HIPCHK(hipGetLastError());
+  if (avg_ms) *avg_ms = (double)ms / (reps * launches);
+  if (bytes) *bytes = arrays / launches * 8.0 * (double)np * c->batch;
+  return GATT_OK;
+}
+
+}  // extern "C"

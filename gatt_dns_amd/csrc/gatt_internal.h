@@ -1,0 +1,180 @@
+// Internal declarations shared by the HIP translation units of libgatt_dns.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gatt_dns.h"
+
+namespace gatt {
+
+constexpr int kMaxPtr = 8;   // fields per line-solve launch
+
+// Device copy of one pre-factored operator (see gatt_lineop).
+struct LineOpDev {
+  int N = 0, ntypes = 0, nR = 0, periodic = 0;
+  const double* A = nullptr;  // (N-1) x nt
+  const double* B = nullptr;  // (N-periodic) x nt
+  const double* C = nullptr;  // (N-1) x nt
+  const double* D = nullptr;  // N x nt (periodic)
+  const double* R = nullptr;  // N x (2nR-1) x nt
+};
+
+// Launch shape of the line solves along one direction: G lanes cooperate on a
+// line, each owning m consecutive points (m <= M, the compiled register tile).
+struct LineShape {
+  int G = 64;
+  int m = 4;
+  int M = 4;
+  int waves_per_block = 4;
+};
+
+struct SolveArgs {
+  LineOpDev op;
+  const int* types;           // 0-based type per line (a + n1*b)
+  int dir;                    // 0 x, 1 y, 2 z
+  int nx, ny, nz;
+  int n1;                     // extent of the first line index
+  int nlines;
+  long long stride;           // element stride along the line
+  int G, m;
+  int nf;                     // fields per member
+  long long member_stride;    // doubles between members
+  const double* in[kMaxPtr];
+  double* out[kMaxPtr];
+  double theta;               // filter blend (mode 1)
+  int mode;                   // 0 write result; 1 in-place filter with FilterCharTime blend
+};
+
+template <int DIR>
+void launch_line_solve_dir(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st);
+inline void launch_line_solve(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st) {
+  if (a.dir == 0) launch_line_solve_dir<0>(a, s, nmembers, st);
+  else if (a.dir == 1) launch_line_solve_dir<1>(a, s, nmembers, st);
+  else launch_line_solve_dir<2>(a, s, nmembers, st);
+}
+inline bool line_shape_supported(int M, int nR) {
+  return (M == 4 || M == 8 || M == 16 || M == 32) && nR >= 2 && nR <= 11;
+}
+
+// ---------------------------------------------------------------- pointwise
+struct FlowConst {
+  double gm1;        // gamma - 1
+  double invRe;      // 1/Re
+  double muA;        // 1 + 110/T0
+  double muB;        // 110/T0
+  double muC;        // gamma*(gamma-1)*Ma*Ma
+  double qk;         // gamma/Re/Pr
+};
+
+struct ViscArgs {
+  FlowConst c;
+  long long np, member_stride;
+  const double* E;
+  const double* D;   // 15 derivative arrays, D + (3f+d)*np
+  double* T;         // 9 arrays: txx tyy tzz txy txz tyz qx qy qz
+  int is3d;
+};
+void launch_visc(const ViscArgs& a, int nmembers, hipStream_t st);
+
+struct AssembleArgs {
+  FlowConst c;
+  long long np, member_stride;
+  const double* S;      // stage state (RHS evaluated here), 5 arrays
+  const double* Q;      // base state q^n, 5 arrays
+  double* Out;          // stage output (may alias S), 5 arrays
+  double* Delta;        // RK accumulator, 5 arrays
+  const double* D;      // 15 first derivatives
+  const double* T;      // 9 stresses / heat fluxes
+  const double* G;      // 12 second derivatives (x: txx,txy,txz,qx | y: txy,tyy,tyz,qy | z: txz,tyz,tzz,qz)
+  const double* sums;   // forcing sums per member (6 doubles) or nullptr
+  double forcing_eps, forcing_e0;
+  int stage;            // 1..4 ; 0 = write the raw RHS into Out
+  double dt;
+  int is3d;
+};
+void launch_assemble(const AssembleArgs& a, int nmembers, hipStream_t st);
+
+struct SumArgs {
+  long long np, member_stride;
+  const double* S;
+  double* partial;      // nblocks x 6 per member
+  double* sums;         // 6 per member: ET, U, V, W, E, N
+  int nblocks;
+};
+void launch_forcing_sums(const SumArgs& a, int nmembers, hipStream_t st);
+
+struct SfdArgs {
+  long long np, member_stride;
+  double* Q;            // state, 5 arrays
+  double* Mean;         // 5 arrays
+  const double* chi;    // field (type 2) or nullptr
+  double chi_val;
+  double dt, delta;
+  int nf_mask;          // bit f set => field f is damped
+};
+void launch_sfd(const SfdArgs& a, int nmembers, hipStream_t st);
+
+struct ReduceArgs {
+  long long np, member_stride;
+  const double* Qn;     // new state
+  const double* Q;      // old state
+  unsigned long long* out;  // per member 8 slots (bit patterns): mc[5], umax[3]
+  int is3d;
+};
+void launch_clear_u64(unsigned long long* p, int n, hipStream_t st);
+void launch_step_reduce(const ReduceArgs& a, int nmembers, hipStream_t st);
+void launch_copy(double* dst, const double* src, long long n, hipStream_t st);
+void launch_record(const unsigned long long* red, double* rec, int* flag, int nmembers, hipStream_t st);
+
+// --------------------------------------------------------------- boundaries
+enum BcKind : int {
+  BC_SET = 0,        // F = value
+  BC_SET_R_FROM_P,   // R = value/(gamma-1)/E
+  BC_NEU,            // F(plane) = sum c_k F(plane + s k)
+  BC_NEU_P,          // R(plane) = (sum c_k (R E)(plane + s k)) / E(plane)
+  BC_CORNER,         // corner averaging of extrapolations (E if adiabatic, then P)
+  BC_LID,            // lidDrivenMovement on U
+  BC_HOLD            // holdInlet capture / restore
+};
+
+struct BcOp {
+  int kind;
+  int field;         // 0..4 (U V W R E)
+  int box[6];        // 0-based inclusive
+  int dir;           // 1..6 (Neumann) ; corners use cd
+  int L;
+  int coef;          // 0 neumannCoeffs, 1 neumann2Coeffs
+  int cd[3];
+  int adiabatic;
+  int aux;           // offset into the aux table (lid x) / hold slot
+  int hold_r;
+  double value;
+  long long npts;
+  long long first_block;
+};
+
+struct BcWave {
+  int op_begin, op_end;
+  long long nblocks;
+};
+
+struct BcArgs {
+  const BcOp* ops;
+  int op_begin, op_end;
+  double* q[5];
+  long long member_stride;
+  const double* coef[2];
+  const double* aux;
+  double* hold;          // hold-inlet storage
+  long long hold_member_stride;
+  int nx, ny, nz;
+  double gamma;
+  int capture;           // holdInlet: 1 on the first call
+};
+void launch_bc_wave(const BcArgs& a, long long nblocks, int nmembers, hipStream_t st);
+
+}  // namespace gatt

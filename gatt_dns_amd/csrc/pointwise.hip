@@ -1,0 +1,296 @@
+// Point-wise parts of the time step: viscous stresses and heat fluxes, the
+// RHS assembly fused with the RK4 stage update, the forcing sums, the SFD
+// blend and the per-step reductions.  One thread per grid point; every field
+// is an FP64 column-major array (x fastest), so all accesses are unit-stride.
+#include "gatt_internal.h"
+
+namespace gatt {
+
+static inline unsigned grid_for(long long n, int threads) {
+  long long b = (n + threads - 1) / threads;
+  return (unsigned)b;
+}
+
+// nseq.F90:218-239 (3-D) and 72-85 (2-D): Sutherland viscosity, stresses, heat flux.
+__global__ __launch_bounds__(256) void visc_kernel(ViscArgs a) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= a.np) return;
+  const long long mo = (long long)blockIdx.y * a.member_stride;
+  const long long np = a.np;
+  const double* D = a.D + mo;
+  double* T = a.T + mo;
+  const double E = a.E[mo + n];
+  const FlowConst c = a.c;
+  const double mu1 = c.muA / (c.muC * E + c.muB);
+  const double mu2 = c.muC * E;
+  const double mu = mu1 * mu2 * sqrt(mu2);
+  const double rmu = c.invRe * mu;
+  const double DUx = D[0 * np + n], DUy = D[1 * np + n];
+  const double DVx = D[3 * np + n], DVy = D[4 * np + n];
+  const double DEx = D[12 * np + n], DEy = D[13 * np + n];
+  if (a.is3d) {
+    const double DUz = D[2 * np + n], DVz = D[5 * np + n];
+    const double DWx = D[6 * np + n], DWy = D[7 * np + n], DWz = D[8 * np + n];
+    const double DEz = D[14 * np + n];
+    const double tz = -(2.0 / 3.0 * (DUx + DVy + DWz));
+    T[0 * np + n] = rmu * (2.0 * DUx + tz);
+    T[1 * np + n] = rmu * (2.0 * DVy + tz);
+    T[2 * np + n] = rmu * (2.0 * DWz + tz);
+    T[3 * np + n] = rmu * (DUy + DVx);
+    T[4 * np + n] = rmu * (DUz + DWx);
+    T[5 * np + n] = rmu * (DVz + DWy);
+    T[6 * np + n] = -(c.qk * mu * DEx);
+    T[7 * np + n] = -(c.qk * mu * DEy);
+    T[8 * np + n] = -(c.qk * mu * DEz);
+  } else {
+    const double dv = 2.0 / 3.0 * (DUx + DVy);
+    T[0 * np + n] = rmu * (2.0 * DUx - dv);
+    T[1 * np + n] = rmu * (2.0 * DVy - dv);
+    T[3 * np + n] = rmu * (DUy + DVx);
+    T[6 * np + n] = -(c.qk * mu * DEx);
+    T[7 * np + n] = -(c.qk * mu * DEy);
+  }
+}
+
+void launch_visc(const ViscArgs& a, int nmembers, hipStream_t st) {
+  hipLaunchKernelGGL(visc_kernel, dim3(grid_for(a.np, 256), nmembers), dim3(256), 0, st, a);
+}
+
+// nseq.F90:280-318 (3-D), 106-134 (2-D), the turbulenceGenerator forcing
+// (turbulenceGenerator.F90:57-61) and one RK4 stage (timeIntegration.F90:393-487).
+__global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs a) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= a.np) return;
+  const long long mo = (long long)blockIdx.y * a.member_stride;
+  const long long np = a.np;
+  const double* S = a.S + mo;
+  const double* D = a.D + mo;
+  const double* T = a.T + mo;
+  const double* G = a.G + mo;
+  const double U = S[0 * np + n], V = S[1 * np + n], R = S[3 * np + n], E = S[4 * np + n];
+  const FlowConst c = a.c;
+  const double DUx = D[0 * np + n], DUy = D[1 * np + n];
+  const double DVx = D[3 * np + n], DVy = D[4 * np + n];
+  const double DRx = D[9 * np + n], DRy = D[10 * np + n];
+  const double DEx = D[12 * np + n], DEy = D[13 * np + n];
+  const double P = c.gm1 * R * E;
+  const double DPx = c.gm1 * (R * DEx + DRx * E);
+  const double DPy = c.gm1 * (R * DEy + DRy * E);
+  const double txx = T[0 * np + n], tyy = T[1 * np + n], txy = T[3 * np + n];
+  double d[5];
+  if (a.is3d) {
+    const double W = S[2 * np + n];
+    const double DUz = D[2 * np + n], DVz = D[5 * np + n];
+    const double DWx = D[6 * np + n], DWy = D[7 * np + n], DWz = D[8 * np + n];
+    const double DRz = D[11 * np + n], DEz = D[14 * np + n];
+    const double DPz = c.gm1 * (R * DEz + DRz * E);
+    const double tzz = T[2 * np + n], txz = T[4 * np + n], tyz = T[5 * np + n];
+    // second derivatives: x: txx,txy,txz,qx | y: txy,tyy,tyz,qy | z: txz,tyz,tzz,qz
+    const double dtxxdx = G[0 * np + n], dtxydx = G[1 * np + n], dtxzdx = G[2 * np + n], htx = G[3 * np + n];
+    const double dtxydy = G[4 * np + n], dtyydy = G[5 * np + n], dtyzdy = G[6 * np + n], hty = G[7 * np + n];
+    const double dtxzdz = G[8 * np + n], dtyzdz = G[9 * np + n], dtzzdz = G[10 * np + n], htz = G[11 * np + n];
+    const double vtx = dtxxdx + dtxydy + dtxzdz;
+    const double vty = dtyydy + dtxydx + dtyzdz;
+    const double vtz = dtzzdz + dtxzdx + dtyzdy;
+    const double div = DUx + DVy + DWz;
+    d[3] = -(R * div) - DRx * U - DRy * V - DRz * W;
+    d[0] = -(DUx * U + DUy * V + DUz * W) + (vtx - DPx) / R;
+    d[1] = -(DVx * U + DVy * V + DVz * W) + (vty - DPy) / R;
+    d[2] = -(DWx * U + DWy * V + DWz * W) + (vtz - DPz) / R;
+    d[4] = -(D[12 * np + n] * U + DEy * V + DEz * W) +
+           (-(P * div) + txx * DUx + tyy * DVy + tzz * DWz + txy * (DVx + DUy) + txz * (DWx + DUz) +
+            tyz * (DWy + DVz) - htx - hty - htz) / R;
+    if (a.sums) {
+      // turbulenceGenerator.F90:57-61 with the global sums of this member
+      const double* sm = a.sums + 8 * blockIdx.y;
+      const double ET = sm[0], Us = sm[1], Vs = sm[2], Ws = sm[3], Es = sm[4], Nd = sm[5];
+      const double gam = 10 * a.forcing_eps;
+      const double fk = 0.25 * a.forcing_eps * Nd / ET;
+      d[0] = d[0] + fk * U - gam * Us / Nd;
+      d[1] = d[1] + fk * V - gam * Vs / Nd;
+      d[2] = d[2] + fk * W - gam * Ws / Nd;
+      d[4] = d[4] - gam * (Es / Nd - a.forcing_e0);
+    }
+  } else {
+    const double dtxxdx = G[0 * np + n], dtxydx = G[1 * np + n], htx = G[3 * np + n];
+    const double dtxydy = G[4 * np + n], dtyydy = G[5 * np + n], hty = G[7 * np + n];
+    const double vtx = dtxxdx + dtxydy;
+    const double vty = dtyydy + dtxydx;
+    const double div = DUx + DVy;
+    d[3] = -(R * div) - DRx * U - DRy * V;
+    d[0] = -(DUx * U + DUy * V) + (vtx - DPx) / R;
+    d[1] = -(DVx * U + DVy * V) + (vty - DPy) / R;
+    d[2] = 0.0;
+    d[4] = -(DEx * U + DEy * V) + (-(P * div) + txx * DUx + tyy * DVy + txy * (DVx + DUy) - htx - hty) / R;
+  }
+  const double* Q = a.Q + mo;
+  double* O = a.Out + mo;
+  double* Dl = a.Delta + mo;
+  const double dt = a.dt;
+  const int nf = 5;
+#pragma unroll
+  for (int f = 0; f < nf; ++f) {
+    if (!a.is3d && f == 2) continue;
+    const long long o = f * np + n;
+    switch (a.stage) {
+      case 0: O[o] = d[f]; break;
+      case 1: Dl[o] = dt / 6.0 * d[f]; O[o] = Q[o] + dt / 2.0 * d[f]; break;
+      case 2: Dl[o] = Dl[o] + dt / 3.0 * d[f]; O[o] = Q[o] + dt / 2.0 * d[f]; break;
+      case 3: Dl[o] = Dl[o] + dt / 3.0 * d[f]; O[o] = Q[o] + dt * d[f]; break;
+      default: O[o] = Q[o] + Dl[o] + dt / 6.0 * d[f]; break;
+    }
+  }
+}
+
+void launch_assemble(const AssembleArgs& a, int nmembers, hipStream_t st) {
+  hipLaunchKernelGGL(assemble_kernel, dim3(grid_for(a.np, 256), nmembers), dim3(256), 0, st, a);
+}
+
+// Global sums of turbulenceGenerator.F90:39-44, deterministic two-stage reduction.
+__global__ __launch_bounds__(256) void forcing_partial_kernel(SumArgs a) {
+  __shared__ double sh[6][256];
+  const long long mo = (long long)blockIdx.y * a.member_stride;
+  const long long np = a.np;
+  const double* S = a.S + mo;
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < np; n += (long long)gridDim.x * blockDim.x) {
+    const double U = S[n], V = S[np + n], W = S[2 * np + n], E = S[4 * np + n];
+    acc[0] += U * U + V * V + W * W;
+    acc[1] += U;
+    acc[2] += V;
+    acc[3] += W;
+    acc[4] += E;
+  }
+  for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) a.partial[((long long)blockIdx.y * a.nblocks + blockIdx.x) * 5 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void forcing_final_kernel(SumArgs a) {
+  const int mem = blockIdx.x;
+  if (threadIdx.x < 5) {
+    double s = 0;
+    for (int b = 0; b < a.nblocks; ++b) s += a.partial[((long long)mem * a.nblocks + b) * 5 + threadIdx.x];
+    a.sums[8 * mem + threadIdx.x] = s;
+  }
+  if (threadIdx.x == 5) a.sums[8 * mem + 5] = (double)a.np;
+}
+
+void launch_forcing_sums(const SumArgs& a, int nmembers, hipStream_t st) {
+  hipLaunchKernelGGL(forcing_partial_kernel, dim3(a.nblocks, nmembers), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(forcing_final_kernel, dim3(nmembers), dim3(64), 0, st, a);
+}
+
+// main.F90:382-401
+__global__ __launch_bounds__(256) void sfd_kernel(SfdArgs a) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= a.np) return;
+  const long long mo = (long long)blockIdx.y * a.member_stride;
+  const double X = a.chi ? a.chi[n] : a.chi_val;
+  const double dtX = a.dt * X;
+  const double dtD = a.delta > 0 ? a.dt / a.delta : 0.0;
+  for (int f = 0; f < 5; ++f) {
+    if (!(a.nf_mask & (1 << f))) continue;
+    const long long o = mo + f * a.np + n;
+    double M = a.Mean[o];
+    const double q = a.Q[o];
+    if (a.delta > 0) { M = M + dtD * (q - M); a.Mean[o] = M; }
+    a.Q[o] = q - dtX * (q - M);
+  }
+}
+
+void launch_sfd(const SfdArgs& a, int nmembers, hipStream_t st) {
+  hipLaunchKernelGGL(sfd_kernel, dim3(grid_for(a.np, 256), nmembers), dim3(256), 0, st, a);
+}
+
+__global__ void clear_u64_kernel(unsigned long long* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+void launch_clear_u64(unsigned long long* p, int n, hipStream_t st) {
+  hipLaunchKernelGGL(clear_u64_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+}
+
+// max |q_new - q| per field (main.F90:414-418) and max |U|,|V|,|W| of the new
+// state (main.F90:197-203 of the next step).  Non-negative doubles order like
+// their bit patterns, and a NaN pattern orders above +inf, so NaN propagates.
+__global__ __launch_bounds__(256) void step_reduce_kernel(ReduceArgs a) {
+  __shared__ unsigned long long sh[8][256];
+  const long long mo = (long long)blockIdx.y * a.member_stride;
+  const long long np = a.np;
+  unsigned long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < np; n += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      if (!a.is3d && f == 2) continue;
+      const double d = fabs(a.Qn[mo + f * np + n] - a.Q[mo + f * np + n]);
+      const unsigned long long b = (unsigned long long)__double_as_longlong(d);
+      acc[f] = b > acc[f] ? b : acc[f];
+    }
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      if (!a.is3d && f == 2) continue;
+      const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(a.Qn[mo + f * np + n]));
+      acc[5 + f] = b > acc[5 + f] ? b : acc[5 + f];
+    }
+  }
+  for (int k = 0; k < 8; ++k) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 8; ++k) {
+        const unsigned long long o = sh[k][threadIdx.x + w];
+        if (o > sh[k][threadIdx.x]) sh[k][threadIdx.x] = o;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 8) atomicMax(a.out + 8 * blockIdx.y + threadIdx.x, sh[threadIdx.x][0]);
+}
+
+void launch_step_reduce(const ReduceArgs& a, int nmembers, hipStream_t st) {
+  long long blocks = (a.np + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(step_reduce_kernel, dim3((unsigned)blocks, nmembers), dim3(256), 0, st, a);
+}
+
+__global__ __launch_bounds__(256) void copy_kernel(double* dst, const double* src, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+void launch_copy(double* dst, const double* src, long long n, hipStream_t st) {
+  long long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, st, dst, src, n);
+}
+
+// Append member 0's max changes to the step record and raise the sticky
+// divergence flag when a change is NaN or all changes are zero (main.F90:434).
+__global__ void record_kernel(const unsigned long long* red, double* rec, int* flag, int nmembers) {
+  if (threadIdx.x != 0) return;
+  int div = 0;
+  for (int mbr = 0; mbr < nmembers; ++mbr) {
+    double s = 0;
+    int nan = 0;
+    for (int f = 0; f < 5; ++f) {
+      const double v = __longlong_as_double((long long)red[8 * mbr + f]);
+      if (v != v) nan = 1;
+      s += v;
+      if (mbr == 0) rec[f] = v;
+    }
+    if (nan || s == 0.0) div = 1;
+  }
+  if (div) *flag = 1;
+}
+
+void launch_record(const unsigned long long* red, double* rec, int* flag, int nmembers, hipStream_t st) {
+  hipLaunchKernelGGL(record_kernel, dim3(1), dim3(64), 0, st, red, rec, flag, nmembers);
+}
+
+}  // namespace gatt

@@ -1,0 +1,169 @@
+"""Python binding of libgatt_dns (ctypes over the C ABI in include/gatt_dns.h).
+
+:class:`Solver` mirrors the reference's subroutine boundary: ``rk4`` is
+``RK4_3D``/``RK4_2D``, ``apply_filters`` is ``applyFilters3D``/``2D``,
+``apply_bound_cond`` is ``applyBoundCond``, ``rhs`` is ``nseq3d``/``nseq2d``
+and ``step`` is one iteration of the time loop of ``main.F90``.  The state
+lives on the GPU; arrays cross the boundary only in ``set_state``/``get_state``.
+
+There is no CPU fallback: constructing a Solver without the HIP library or a
+GPU raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import DP, build_desc, field_ptrs
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libgatt_dns.so")
+
+EXPORTS = ["gatt_last_error", "gatt_device_info", "gatt_create", "gatt_destroy", "gatt_set_state",
+           "gatt_get_state", "gatt_set_mean", "gatt_get_mean", "gatt_rk4", "gatt_apply_filters",
+           "gatt_apply_bound_cond", "gatt_sfd", "gatt_rhs", "gatt_step", "gatt_advance",
+           "gatt_line_solve", "gatt_sync", "gatt_time_kernel"]
+
+
+class GattError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libgatt_dns.so and declare its signatures; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise GattError("libgatt_dns.so is not built (%s); run __graft_entry__.build()" % path)
+    L = C.CDLL(path)
+    V = C.c_void_p
+    I = C.c_int32
+    L.gatt_last_error.restype = C.c_char_p
+    L.gatt_device_info.argtypes = [C.c_char_p, I]
+    L.gatt_create.argtypes = [V, I, C.POINTER(V)]
+    L.gatt_destroy.argtypes = [V]
+    for name in ("gatt_set_state", "gatt_get_state", "gatt_set_mean", "gatt_get_mean"):
+        getattr(L, name).argtypes = [V, I, C.POINTER(DP)]
+    L.gatt_rk4.argtypes = [V, C.c_double, C.c_double]
+    L.gatt_apply_filters.argtypes = [V, C.c_double]
+    L.gatt_apply_bound_cond.argtypes = [V, C.c_double]
+    L.gatt_sfd.argtypes = [V, C.c_double]
+    L.gatt_rhs.argtypes = [V, C.POINTER(DP)]
+    L.gatt_step.argtypes = [V, C.c_double, C.c_double, DP, DP, C.POINTER(I)]
+    L.gatt_advance.argtypes = [V, I, C.c_double, C.c_double, DP, C.POINTER(I)]
+    L.gatt_line_solve.argtypes = [V, I, I, I, C.POINTER(DP), C.POINTER(DP)]
+    L.gatt_sync.argtypes = [V]
+    L.gatt_time_kernel.argtypes = [V, I, I, DP, DP]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise GattError("gatt error %d: %s" % (rc, _lib.gatt_last_error().decode()))
+
+
+def device_info():
+    L = load_library()
+    buf = C.create_string_buffer(256)
+    _check(L.gatt_device_info(buf, 256))
+    return buf.value.decode()
+
+
+class Solver:
+    """Device-resident time advance of ``batch`` members of one case."""
+
+    def __init__(self, case, batch=1):
+        self.lib = load_library()
+        self.case = case
+        self.batch = batch
+        self.shape = (case.nx, case.ny, case.nz)
+        self.desc = build_desc(case)
+        h = C.c_void_p()
+        _check(self.lib.gatt_create(C.cast(self.desc.ptr, C.c_void_p), batch, C.byref(h)))
+        self.ctx = h
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.gatt_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ---------------------------------------------------------------
+    def _arrays(self, q):
+        return [np.asfortranarray(np.asarray(q[k], dtype=np.float64).reshape(self.shape, order="F"))
+                for k in "UVWRE"]
+
+    def set_state(self, q, member=0):
+        arrs = self._arrays(q)
+        _check(self.lib.gatt_set_state(self.ctx, member, field_ptrs(arrs)))
+
+    def get_state(self, member=0):
+        out = [np.zeros(self.shape, order="F") for _ in range(5)]
+        _check(self.lib.gatt_get_state(self.ctx, member, field_ptrs(out)))
+        return dict(zip("UVWRE", out))
+
+    def set_mean(self, q, member=0):
+        _check(self.lib.gatt_set_mean(self.ctx, member, field_ptrs(self._arrays(q))))
+
+    def get_mean(self, member=0):
+        out = [np.zeros(self.shape, order="F") for _ in range(5)]
+        _check(self.lib.gatt_get_mean(self.ctx, member, field_ptrs(out)))
+        return dict(zip("UVWRE", out))
+
+    # -- reference subroutines -------------------------------------------------
+    def rk4(self, t, dt):
+        _check(self.lib.gatt_rk4(self.ctx, t, dt))
+
+    def apply_filters(self, theta):
+        _check(self.lib.gatt_apply_filters(self.ctx, theta))
+
+    def apply_bound_cond(self, t):
+        _check(self.lib.gatt_apply_bound_cond(self.ctx, t))
+
+    def sfd(self, dt):
+        _check(self.lib.gatt_sfd(self.ctx, dt))
+
+    def rhs(self):
+        out = [np.zeros(self.shape, order="F") for _ in range(5)]
+        _check(self.lib.gatt_rhs(self.ctx, field_ptrs(out)))
+        return dict(zip("UVWRE", out))
+
+    def step(self, t, dt):
+        um = np.zeros(3)
+        mc = np.zeros(5)
+        dv = C.c_int32(0)
+        _check(self.lib.gatt_step(self.ctx, t, dt, um.ctypes.data_as(DP), mc.ctypes.data_as(DP), C.byref(dv)))
+        return um, mc, bool(dv.value)
+
+    def advance(self, nsteps, t0, dt):
+        mc = np.zeros(5)
+        dv = C.c_int32(0)
+        _check(self.lib.gatt_advance(self.ctx, nsteps, t0, dt, mc.ctypes.data_as(DP), C.byref(dv)))
+        return mc, bool(dv.value)
+
+    def sync(self):
+        _check(self.lib.gatt_sync(self.ctx))
+
+    def time_kernel(self, kernel, reps):
+        """(mean ms per launch, algorithmic bytes per launch) of one kernel class."""
+        ms = C.c_double(0)
+        by = C.c_double(0)
+        _check(self.lib.gatt_time_kernel(self.ctx, kernel, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def line_solve(self, direction, kind, fields_in, fields_out):
+        """Device-pointer unit entry point: lists of torch CUDA tensors (float64, Fortran-ordered data)."""
+        n = len(fields_in)
+        pin = (DP * n)(*[C.cast(C.c_void_p(t.data_ptr()), DP) for t in fields_in])
+        pout = (DP * n)(*[C.cast(C.c_void_p(t.data_ptr()), DP) for t in fields_out])
+        _check(self.lib.gatt_line_solve(self.ctx, direction, kind, n, pin, pout))

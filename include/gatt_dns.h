@@ -202,6 +202,22 @@ int gatt_line_solve(gatt_ctx* ctx, int32_t dir, int32_t kind, int32_t nfields,
 /* Synchronise the context's stream. */
 int gatt_sync(gatt_ctx* ctx);
 
+/* Kernel classes of one time step, for timing. */
+enum {
+  GATT_K_DERIV_X = 0, GATT_K_DERIV_Y = 1, GATT_K_DERIV_Z = 2,
+  GATT_K_FILTER_X = 3, GATT_K_FILTER_Y = 4, GATT_K_FILTER_Z = 5,
+  GATT_K_VISC = 6, GATT_K_ASSEMBLE = 7
+};
+
+/* Time one kernel class on the context's stream with HIP events: the launches
+ * that class makes in one RK stage (derivatives: the state fields, then the
+ * stress/heat-flux fields; filters: the state fields in place) are replayed
+ * `reps` times.  Returns the mean duration of one launch in ms and the
+ * algorithmic bytes of that launch (8 B per point for every array read or
+ * written once).  Filter timing overwrites the state.  No reference counterpart
+ * (measurement only). */
+int gatt_time_kernel(gatt_ctx* ctx, int32_t kernel, int32_t reps, double* avg_ms, double* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,242 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle.
+
+Error measure per field: max |gpu - oracle| / max |oracle| (the field's scale),
+so fields that are zero almost everywhere are not divided by zero.  Tolerances:
+line solves and boundary conditions 1e-13, the RHS and one full step 1e-12,
+1000 steps of the 129^2 cavity 1e-9.
+"""
+import numpy as np
+import pytest
+
+from gatt_dns_amd.prep import cases
+
+pytestmark = pytest.mark.gpu
+
+FIELDS2 = "UVRE"
+FIELDS3 = "UVWRE"
+
+
+def rel_err(a, b):
+    scale = np.max(np.abs(b))
+    if scale == 0:
+        return float(np.max(np.abs(a)))
+    return float(np.max(np.abs(a - b)) / scale)
+
+
+def fields_of(case):
+    return FIELDS3 if case.is3d else FIELDS2
+
+
+def perturbed_state(case, seed, amp=0.05):
+    """Initial state plus a smooth seeded perturbation, so that every RHS term is active."""
+    rng = np.random.default_rng(seed)
+    st = {k: np.array(v, order="F", copy=True) for k, v in case.state0.items()}
+    x = np.linspace(0, 1, case.nx)[:, None, None]
+    y = np.linspace(0, 1, case.ny)[None, :, None]
+    z = np.linspace(0, 1, case.nz)[None, None, :]
+    for k in fields_of(case):
+        a, b, c = rng.uniform(1, 3, 3)
+        ph = rng.uniform(0, 2 * np.pi, 3)
+        pert = np.sin(2 * np.pi * a * x + ph[0]) * np.cos(2 * np.pi * b * y + ph[1]) * np.cos(2 * np.pi * c * z + ph[2])
+        scale = amp * (1.0 if k in "UVW" else (0.02 if k == "R" else 0.02 * case.E0))
+        st[k] = np.asfortranarray(st[k] + scale * pert)
+    return st
+
+
+@pytest.fixture(scope="module")
+def rt():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from gatt_dns_amd import runtime
+    runtime.load_library()
+    return runtime
+
+
+CASES = {
+    "cavity33": lambda: cases.lid_driven(33),
+    "cavity129": lambda: cases.lid_driven(129),
+    "box32": lambda: cases.turbulence_box(32),
+    "box48_noforce": lambda: cases.turbulence_box(48, forcing=False),
+}
+
+
+@pytest.mark.parametrize("name", ["cavity33", "cavity129", "box32"])
+@pytest.mark.parametrize("kind", [0, 1])
+def test_line_solve(rt, oracle_mod, name, kind):
+    import torch
+    case = CASES[name]()
+    print("line_solve", name, kind, flush=True)
+    o = oracle_mod.Oracle(case)
+    s = rt.Solver(case)
+    rng = np.random.default_rng(7 + kind)
+    shape = (case.nx, case.ny, case.nz)
+    for d in range(3 if case.is3d else 2):
+        F = np.asfortranarray(rng.standard_normal(shape))
+        want = o.solve(d, kind, F)
+        tin = torch.from_numpy(F.ravel(order="F").copy()).cuda()
+        tout = torch.zeros_like(tin)
+        s.line_solve(d, kind, [tin], [tout])
+        got = tout.cpu().numpy().reshape(shape, order="F")
+        assert rel_err(got, want) < 1e-13, (name, kind, d, rel_err(got, want))
+
+
+@pytest.mark.parametrize("name", ["cavity33", "box32"])
+def test_rhs(rt, oracle_mod, name):
+    case = CASES[name]()
+    print("rhs", name, flush=True)
+    q = perturbed_state(case, 3)
+    o = oracle_mod.Oracle(case)
+    want = o.rhs(q)
+    s = rt.Solver(case)
+    s.set_state(q)
+    got = s.rhs()
+    for k in fields_of(case):
+        assert rel_err(got[k], want[k]) < 1e-12, (name, k, rel_err(got[k], want[k]))
+
+
+def test_boundary_conditions(rt, oracle_mod):
+    case = CASES["cavity33"]()
+    q = perturbed_state(case, 5)
+    o = oracle_mod.Oracle(case)
+    want = o.apply_bound_cond(q, 0.0)
+    s = rt.Solver(case)
+    s.set_state(q)
+    s.apply_bound_cond(0.0)
+    got = s.get_state()
+    for k in FIELDS2:
+        assert rel_err(got[k], want[k]) < 1e-14, (k, rel_err(got[k], want[k]))
+
+
+@pytest.mark.parametrize("name", ["cavity33", "box32"])
+@pytest.mark.parametrize("theta", [-1.0, 0.3])
+def test_filters(rt, oracle_mod, name, theta):
+    case = CASES[name]()
+    q = perturbed_state(case, 11)
+    o = oracle_mod.Oracle(case)
+    want = o.apply_filters(q, theta)
+    s = rt.Solver(case)
+    s.set_state(q)
+    s.apply_filters(theta)
+    got = s.get_state()
+    for k in fields_of(case):
+        assert rel_err(got[k], want[k]) < 1e-13, (name, theta, k)
+
+
+@pytest.mark.parametrize("name", ["cavity129", "box32"])
+def test_one_step(rt, oracle_mod, name):
+    case = CASES[name]()
+    print("one_step", name, flush=True)
+    o = oracle_mod.Oracle(case)
+    q0 = o.apply_bound_cond(case.state0, 0.0)
+    want, um_w, mc_w, dv_w = o.step(q0, 0.0, case.dt)
+    s = rt.Solver(case)
+    s.set_state(case.state0)
+    s.apply_bound_cond(0.0)
+    um, mc, dv = s.step(0.0, case.dt)
+    got = s.get_state()
+    for k in fields_of(case):
+        assert rel_err(got[k], want[k]) <= 1e-12, (name, k, rel_err(got[k], want[k]))
+    assert np.allclose(mc, mc_w, rtol=1e-9, atol=0)
+    assert dv == dv_w
+
+
+def test_rk4_stage_boundaries(rt, oracle_mod):
+    case = CASES["cavity33"]()
+    q = perturbed_state(case, 2)
+    o = oracle_mod.Oracle(case)
+    q = o.apply_bound_cond(q, 0.0)
+    want = o.rk4(q, 0.0, case.dt)
+    s = rt.Solver(case)
+    s.set_state(q)
+    s.rk4(0.0, case.dt)
+    got = s.get_state()
+    for k in FIELDS2:
+        assert rel_err(got[k], want[k]) <= 1e-12, (k, rel_err(got[k], want[k]))
+
+
+def test_cavity_1000_steps(rt, oracle_mod):
+    """1000 fixed-dt steps of the 129^2 lid-driven cavity, Re 1000, Ma 0.5."""
+    case = CASES["cavity129"]()
+    print("cavity129 x1000", flush=True)
+    o = oracle_mod.Oracle(case)
+    q0 = o.apply_bound_cond(case.state0, 0.0)
+    want, mc_w, dv_w = o.run(q0, 0.0, case.dt, 1000)
+    s = rt.Solver(case)
+    s.set_state(case.state0)
+    s.apply_bound_cond(0.0)
+    mc, dv = s.advance(1000, 0.0, case.dt)
+    got = s.get_state()
+    assert not dv and not dv_w
+    for k in FIELDS2:
+        assert rel_err(got[k], want[k]) <= 1e-9, (k, rel_err(got[k], want[k]))
+
+
+def test_advance_equals_steps(rt):
+    """The graph-replayed advance gives bit-identical states to step-by-step calls."""
+    case = CASES["box32"]()
+    a = rt.Solver(case)
+    b = rt.Solver(case)
+    a.set_state(case.state0)
+    b.set_state(case.state0)
+    for n in range(5):
+        a.step(n * case.dt, case.dt)
+    b.advance(5, 0.0, case.dt)
+    ga, gb = a.get_state(), b.get_state()
+    for k in FIELDS3:
+        assert np.array_equal(ga[k], gb[k]), k
+
+
+def test_box_steps(rt, oracle_mod):
+    case = CASES["box32"]()
+    o = oracle_mod.Oracle(case)
+    want, mc_w, _ = o.run(case.state0, 0.0, case.dt, 10)
+    s = rt.Solver(case)
+    s.set_state(case.state0)
+    s.advance(10, 0.0, case.dt)
+    got = s.get_state()
+    for k in FIELDS3:
+        assert rel_err(got[k], want[k]) <= 1e-11, (k, rel_err(got[k], want[k]))
+
+
+def test_sfd_step(rt, oracle_mod):
+    case = CASES["cavity33"]()
+    case.sfd_type, case.sfd_delta, case.sfd_x_val = 1, 0.5, 0.05
+    o = oracle_mod.Oracle(case)
+    q0 = o.apply_bound_cond(case.state0, 0.0)
+    o.init_mean(q0)
+    want, mc_w, _ = o.run(q0, 0.0, case.dt, 3)
+    s = rt.Solver(case)
+    s.set_state(case.state0)
+    s.apply_bound_cond(0.0)
+    s.set_mean(q0)
+    s.advance(3, 0.0, case.dt)
+    got = s.get_state()
+    for k in FIELDS2:
+        assert rel_err(got[k], want[k]) <= 1e-12, k
+    gm, wm = s.get_mean(), o.mean()
+    for k in FIELDS2:
+        assert rel_err(gm[k], wm[k]) <= 1e-12, k
+
+
+def test_divergence_flag_zero_change(rt):
+    # a uniform flow at rest on a periodic box has a zero RHS: the step changes
+    # nothing and the time loop's divergence test (sum of changes == 0) fires
+    case = cases.turbulence_box(16, forcing=False)
+    q = {k: np.zeros((16, 16, 16), order="F") for k in "UVW"}
+    q["R"] = np.ones((16, 16, 16), order="F")
+    q["E"] = np.full((16, 16, 16), case.E0, order="F")
+    s = rt.Solver(case)
+    s.set_state(q)
+    um, mc, dv = s.step(0.0, case.dt)
+    assert dv and np.all(mc == 0)
+
+
+def test_divergence_flag_nan(rt):
+    case = cases.turbulence_box(16, forcing=False)
+    q = {k: np.array(v, order="F", copy=True) for k, v in case.state0.items()}
+    q["R"][3, 4, 5] = 0.0   # division by zero density
+    s = rt.Solver(case)
+    s.set_state(q)
+    um, mc, dv = s.step(0.0, case.dt)
+    assert dv
