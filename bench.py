@@ -58,6 +58,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 256 MiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the time-advance path runs on a single MI355X here; "
@@ -74,6 +77,13 @@ def main():
     log("preparing turbulence box %d^3" % args.n)
     case = cases.turbulence_box(args.n)
     s = runtime.Solver(case)
+    tuned = None
+    if not args.no_tune:
+        from gatt_dns_amd import autotune
+        log("autotuning line-solve shapes")
+        tuned = autotune.tune(s, reps=3, log=log)
+        for (d, k), (v, p, w, ms) in sorted(tuned.items()):
+            log("  dir %d kind %d -> variant %d param %d wpb %d, %.4f ms" % (d, k, v, p, w, ms))
     s.set_state(case.state0)
     s.apply_bound_cond(0.0)
     dt = case.dt
@@ -111,6 +121,14 @@ def main():
     step_bytes = 8.0 * MODEL_TOUCHES_3D * case.npoints
     step_gbps = step_bytes / (ms_step * 1e-3) / 1e9
 
+    if args.calibrate_copy:
+        src = torch.rand(1 << 25, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 3 x %d bytes read + written" % (src.numel() * 8))
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle sample)")
@@ -131,6 +149,8 @@ def main():
                           "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(step_gbps / HBM_PEAK_GBPS, 4)},
         "kernels": {k: {"ms_per_launch": round(v["ms"], 5), "ms_per_step": round(v["share_ms_per_step"], 4)}
                     for k, v in per_class.items()},
+        "line_shapes": {"%s_%s" % ("xyz"[d], "deriv" if k == 0 else "filter"): list(s.get_line_shape(d, k))
+                        for d in range(3) for k in (0, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -9,14 +9,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libgatt_dns.so")
-SOURCES = ["line_solve_x.hip", "line_solve_y.hip", "line_solve_z.hip", "pointwise.hip", "boundary.hip",
-           "capi.hip"]
+SOURCES = ["line_solve_x_ws.hip", "line_solve_y_ws.hip", "line_solve_z_ws.hip", "line_solve_x_lg.hip",
+           "line_solve_y_lg.hip", "line_solve_z_lg.hip", "pointwise.hip", "boundary.hip", "capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GATT_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
          # packed FP32 instructions are disabled for every HIP object
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # every expression is rounded as written (no FMA contraction), like the
+         # reference's statement-by-statement evaluation
+         "-ffp-contract=off",
          "-Wno-unused-result", "-Wno-unused-command-line-argument"]
 
 
@@ -32,7 +35,7 @@ def build(verbose=True):
     """Compile every HIP source and link the shared library. Always recompiles."""
     os.makedirs(OUT_DIR, exist_ok=True)
     objs = [os.path.join(OUT_DIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
-    with ThreadPoolExecutor(max_workers=min(6, len(SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(int(os.environ.get("MAX_JOBS", "8")), len(SOURCES))) as ex:
         list(ex.map(lambda p: _compile(*p), zip(SOURCES, objs)))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs

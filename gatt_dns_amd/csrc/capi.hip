@@ -43,6 +43,8 @@ enum : int {
 
 struct DirOps {
   LineOpDev op[2];           // deriv, filter
+  double* coef[2] = {nullptr, nullptr};       // packed per-point coefficients
+  double* coef_ends[2] = {nullptr, nullptr};  // per type A(1), C(1)
   LineShape shape[2];
   int* types = nullptr;      // per line, 0-based
   int n1 = 0, nlines = 0;
@@ -69,6 +71,7 @@ struct gatt_ctx {
   double gamma = 1.4;
   DirOps dir[3];
   std::vector<void*> dev_allocs;
+  long long max_lds = 65536;
   int filterDir[3] = {1, 1, 1};
   double filterCharTime = -1;
   // boundary program
@@ -120,25 +123,40 @@ int dev_upload(gatt_ctx* c, const T* host, size_t count, T** out) {
   return GATT_OK;
 }
 
-LineShape choose_shape(int dir, int N, int nR) {
-  LineShape s;
-  const int H = nR - 1;
-  (void)H;
-  int G;
-  if (dir == 0) {
-    G = 64;
-    while (G > 2 && (N + G - 1) / G < 4) G >>= 1;
-  } else {
-    G = 8;
-    while ((N + G - 1) / G > 32 && G < 64) G <<= 1;
-    while (G > 2 && (N + G - 1) / G < 2) G >>= 1;
+// Default shape before tuning.  Many lines of at most 256 points: the
+// wave-segment kernel with 16 (or 32) points per lane; otherwise the lane-group
+// kernel with 16 points per lane where the line allows it.
+LineShape choose_shape(int dir, int N, int nR, long long nlines, long long max_lds) {
+  if (nlines >= 1024) {
+    for (int M : {16, 32}) {
+      LineShape s;
+      s.variant = 1;
+      s.M = M;
+      s.G = (N + M - 1) / M;
+      if (line_shape_ok(s, N, nR, dir, max_lds)) return s;
+    }
   }
-  s.G = G;
-  s.m = (N + G - 1) / G;
-  s.M = s.m <= 4 ? 4 : s.m <= 8 ? 8 : s.m <= 16 ? 16 : 32;
-  s.waves_per_block = 4;
-  return s;
+  for (int M : {16, 8, 4, 32}) {
+    for (int wpb : {4, 2, 1}) {
+      LineShape s;
+      s.M = M;
+      s.G = pow2_at_least((N + M - 1) / M);
+      if (s.G < 2) s.G = 2;
+      s.waves_per_block = wpb;
+      if (line_shape_ok(s, N, nR)) return s;
+    }
+  }
+  LineShape bad;
+  bad.M = 0;
+  return bad;
 }
+
+// Per-point coefficients of the wave-segment kernel, [type][point][CS] with
+// CS = 2nR-1+4 rounded up to even:
+// the RHS band, then fa, fb of the forward sweep d = (r - fa d') fb, the
+// backward factor ce of x = d - ce x', and D of the periodic first row.  The
+// periodic index shift of B (derivs.F90:796-798) is resolved here.
+int pack_coeffs(gatt_ctx* c, const gatt_lineop& h, double** coef, double** ends);
 
 int upload_lineop(gatt_ctx* c, const gatt_lineop& h, int expectN, LineOpDev* out, const char* what) {
   if (h.n != expectN) return fail(GATT_E_INVALID, std::string(what) + ": line length does not match the grid");
@@ -166,6 +184,28 @@ int upload_lineop(gatt_ctx* c, const gatt_lineop& h, int expectN, LineOpDev* out
   if ((rc = dev_upload(c, h.R, N * (size_t)(2 * h.nR - 1) * nt, &p))) return rc;
   out->R = p;
   return GATT_OK;
+}
+
+int pack_coeffs(gatt_ctx* c, const gatt_lineop& h, double** coef, double** ends) {
+  const int N = h.n, nt = h.ntypes, NW = 2 * h.nR - 1, CS = (NW + 4 + 1) & ~1, per = h.periodic ? 1 : 0;
+  const int NB = N - per;
+  std::vector<double> pk((size_t)nt * N * CS, 0.0), en((size_t)nt * 2, 0.0);
+  for (int t = 0; t < nt; ++t) {
+    for (int e = 0; e < N; ++e) {
+      double* p = &pk[((size_t)t * N + e) * CS];
+      for (int q = 0; q < NW; ++q) p[q] = h.R[(size_t)e + (size_t)N * (q + (size_t)NW * t)];
+      p[NW] = e > per ? h.A[(size_t)(e - 1) + (size_t)(N - 1) * t] : 0.0;
+      if (per) p[NW + 1] = e >= 1 ? h.B[(size_t)(e - 1) + (size_t)NB * t] : 0.0;
+      else p[NW + 1] = h.B[(size_t)e + (size_t)NB * t];
+      p[NW + 2] = e <= N - 2 ? h.C[(size_t)e + (size_t)(N - 1) * t] : 0.0;
+      p[NW + 3] = per ? h.D[(size_t)e + (size_t)N * t] : 0.0;
+    }
+    en[2 * t] = h.A[(size_t)(N - 1) * t];
+    en[2 * t + 1] = h.C[(size_t)(N - 1) * t];
+  }
+  int rc;
+  if ((rc = dev_upload(c, pk.data(), pk.size(), coef))) return rc;
+  return dev_upload(c, en.data(), en.size(), ends);
 }
 
 // ------------------------------------------------------------ BC compiler
@@ -467,13 +507,14 @@ void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double
   a.nlines = d.nlines;
   a.stride = d.stride;
   a.G = d.shape[kind].G;
-  a.m = d.shape[kind].m;
   a.nf = nf;
   a.member_stride = member_stride;
   for (int f = 0; f < nf; ++f) {
     a.in[f] = in[f];
     a.out[f] = out[f];
   }
+  a.coef = d.coef[kind];
+  a.coef_ends = d.coef_ends[kind];
   a.theta = theta;
   a.mode = mode;
   launch_line_solve(a, d.shape[kind], nmembers, c->stream);
@@ -662,6 +703,13 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
   for (int k = 0; k < 3; ++k) c->filterDir[k] = d->filterDir[k];
   c->filterCharTime = d->filterCharTime;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  {
+    int dev = 0, v = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || v <= 0)
+      HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    c->max_lds = v > 0 ? v : 65536;
+  }
   const int dims[3] = {d->nx, d->ny, d->nz};
   const int ndir = c->is3d ? 3 : 2;
   int rc;
@@ -669,6 +717,8 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
     DirOps& o = c->dir[k];
     if ((rc = upload_lineop(c.get(), d->deriv[k], dims[k], &o.op[0], "deriv"))) return rc;
     if ((rc = upload_lineop(c.get(), d->filter[k], dims[k], &o.op[1], "filter"))) return rc;
+    if ((rc = pack_coeffs(c.get(), d->deriv[k], &o.coef[0], &o.coef_ends[0]))) return rc;
+    if ((rc = pack_coeffs(c.get(), d->filter[k], &o.coef[1], &o.coef_ends[1]))) return rc;
     if (k == 0) { o.n1 = d->ny; o.nlines = d->ny * d->nz; o.stride = 1; }
     if (k == 1) { o.n1 = d->nx; o.nlines = d->nx * d->nz; o.stride = d->nx; }
     if (k == 2) { o.n1 = d->nx; o.nlines = d->nx * d->ny; o.stride = (long long)d->nx * d->ny; }
@@ -680,9 +730,9 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
     }
     if ((rc = dev_upload(c.get(), tm.data(), tm.size(), &o.types))) return rc;
     for (int kind = 0; kind < 2; ++kind) {
-      o.shape[kind] = choose_shape(k, dims[k], o.op[kind].nR);
-      if (!line_shape_supported(o.shape[kind].M, o.op[kind].nR))
-        return fail(GATT_E_UNSUPPORTED, "line length too long for the compiled register tiles");
+      o.shape[kind] = choose_shape(k, dims[k], o.op[kind].nR, o.nlines, c->max_lds);
+      if (!line_shape_ok(o.shape[kind], dims[k], o.op[kind].nR, k, c->max_lds))
+        return fail(GATT_E_UNSUPPORTED, "no line-solve shape fits this line length");
     }
   }
   if ((rc = compile_bc(c.get(), d->bc))) return rc;
@@ -914,6 +964,50 @@ int gatt_line_solve(gatt_ctx* c, int32_t dir, int32_t kind, int32_t nfields, con
 int gatt_sync(gatt_ctx* c) {
   if (check_ctx(c)) return GATT_E_INVALID;
   HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_set_line_shape(gatt_ctx* c, int32_t dir, int32_t kind, int32_t variant, int32_t param, int32_t wpb) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  if (dir < 0 || dir > 2 || (!c->is3d && dir == 2) || (kind != GATT_DERIV && kind != GATT_FILTER))
+    return fail(GATT_E_INVALID, "direction or kind out of range");
+  DirOps& d = c->dir[dir];
+  const int N = d.op[kind].N;
+  LineShape s;
+  s.variant = variant;
+  if (variant == 0) {
+    if (param < 2 || param > 64 || (param & (param - 1)) || wpb < 1 || wpb > 4)
+      return fail(GATT_E_INVALID, "bad lane-group shape");
+    s.G = param;
+    s.M = std::max(4, pow2_at_least((N + param - 1) / param));
+    s.waves_per_block = wpb;
+  } else if (variant == 1) {
+    s.M = param;
+    s.G = param > 0 ? (N + param - 1) / param : 0;
+    s.waves_per_block = s.G;
+  } else {
+    return fail(GATT_E_INVALID, "unknown line-solve variant");
+  }
+  if (!line_shape_ok(s, N, d.op[kind].nR, dir, c->max_lds))
+    return fail(GATT_E_UNSUPPORTED, "line shape does not fit this line");
+  d.shape[kind] = s;
+  if (c->graph) {
+    hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  return GATT_OK;
+}
+
+int gatt_get_line_shape(gatt_ctx* c, int32_t dir, int32_t kind, int32_t* variant, int32_t* G, int32_t* M,
+                        int32_t* wpb) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  if (dir < 0 || dir > 2 || (!c->is3d && dir == 2) || (kind != GATT_DERIV && kind != GATT_FILTER))
+    return fail(GATT_E_INVALID, "direction or kind out of range");
+  const LineShape& s = c->dir[dir].shape[kind];
+  if (variant) *variant = s.variant;
+  if (G) *G = s.G;
+  if (M) *M = s.M;
+  if (wpb) *wpb = s.waves_per_block;
   return GATT_OK;
 }
 

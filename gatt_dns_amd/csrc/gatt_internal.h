@@ -23,11 +23,14 @@ struct LineOpDev {
   const double* R = nullptr;  // N x (2nR-1) x nt
 };
 
-// Launch shape of the line solves along one direction: G lanes cooperate on a
-// line, each owning m consecutive points (m <= M, the compiled register tile).
+// Launch shape of the line solves along one direction: G lanes of a wavefront
+// share a line, each owning M consecutive points (M a power of two, G*M >= N);
+// a workgroup of waves_per_block wavefronts holds (64/G)*waves_per_block lines.
+// variant 1 ("wave-segment", line_solve_ws.h): every lane owns a line, wavefront
+// w owns points [w*M, w*M+M) of 64 lines, G = ceil(N/M) wavefronts per workgroup.
 struct LineShape {
+  int variant = 0;
   int G = 64;
-  int m = 4;
   int M = 4;
   int waves_per_block = 4;
 };
@@ -40,24 +43,55 @@ struct SolveArgs {
   int n1;                     // extent of the first line index
   int nlines;
   long long stride;           // element stride along the line
-  int G, m;
+  int G;
+  int TL;                     // lines per workgroup tile
+  int lds_row;                // LDS words per point row of the tile
+  float inv_n;                // 1/N
   int nf;                     // fields per member
   long long member_stride;    // doubles between members
   const double* in[kMaxPtr];
   double* out[kMaxPtr];
+  const double* coef;         // packed per-point coefficients [type][point][2nR-1+4] (wave-segment)
+  const double* coef_ends;    // per type: A(1), C(1) of the periodic corrections
   double theta;               // filter blend (mode 1)
   int mode;                   // 0 write result; 1 in-place filter with FilterCharTime blend
 };
 
 template <int DIR>
 void launch_line_solve_dir(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st);
+template <int DIR>
+void launch_line_solve_ws_dir(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st);
 inline void launch_line_solve(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st) {
+  if (s.variant == 1) {
+    if (a.dir == 0) launch_line_solve_ws_dir<0>(a, s, nmembers, st);
+    else if (a.dir == 1) launch_line_solve_ws_dir<1>(a, s, nmembers, st);
+    else launch_line_solve_ws_dir<2>(a, s, nmembers, st);
+    return;
+  }
   if (a.dir == 0) launch_line_solve_dir<0>(a, s, nmembers, st);
   else if (a.dir == 1) launch_line_solve_dir<1>(a, s, nmembers, st);
   else launch_line_solve_dir<2>(a, s, nmembers, st);
 }
-inline bool line_shape_supported(int M, int nR) {
-  return (M == 4 || M == 8 || M == 16 || M == 32) && nR >= 2 && nR <= 11;
+// A shape is usable when its register tile is compiled (M in 4..32), the
+// segments cover the line, the wrapped halo stays within one period
+// (M + nR - 1 <= N + 1) and the LDS tile fits in 64 KiB.
+inline bool line_shape_ok(const LineShape& s, int N, int nR, int dir = 1, long long max_lds = 65536) {
+  if (!(s.M == 4 || s.M == 8 || s.M == 16 || s.M == 32) || nR < 2 || nR > 11) return false;
+  if (s.variant == 1) {
+    const int G = (N + s.M - 1) / s.M;
+    if (G < 1 || G > (s.M >= 32 ? 8 : 16) || s.M + nR - 1 > N + 1 || nR - 1 > N) return false;
+    const long long maps = 5LL * G * 64, tile = dir == 0 ? 65LL * N : 0;
+    return 8 * (maps > tile ? maps : tile) <= max_lds;
+  }
+  if (s.G < 2 || s.G > 64 || (s.G & (s.G - 1)) || s.waves_per_block < 1 || s.waves_per_block > 4) return false;
+  if (s.G * s.M < N || s.M + nR - 1 > N + 1 || nR - 1 > N) return false;
+  const long long rows = (long long)(64 / s.G) * s.waves_per_block * s.G + 1;
+  return (long long)s.M * rows * 8 <= 65536;
+}
+inline int pow2_at_least(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
 }
 
 // ---------------------------------------------------------------- pointwise

@@ -4,22 +4,25 @@
 // Restates solveSystem{X,Y,Z} and solveSystem{X,Y,Z}per
 // (source/Fortran/derivs.F90:556-948) for gfx950:
 //
-//   * A group of G lanes of one wavefront owns one line; lane g holds the m
-//     consecutive points [g*m, g*m+m) and their (2nR-1)-point halo in VGPRs.
-//     The band product is computed from registers.
-//   * The Thomas recurrences are first-order affine recurrences.  Each lane
-//     composes the affine map of its segment (the composite's offset follows the
-//     reference recurrence exactly), the group combines the maps with a
-//     log2(G)-step wavefront scan, and each lane then re-runs the reference
-//     recurrence from its exact start value.  Forward: d_i = (r_i - A d_{i-1}) B;
-//     backward: x_i = d_i - C x_{i+1}.
-//   * Periodic lines first reduce x_1 = D . r over the group, then apply the
-//     reference's row-2 / row-N corrections and solve rows 2..N
+//   * A workgroup owns a tile of TL whole lines of one field.  The tile is
+//     moved between HBM and LDS with unit-stride accesses: along x a tile is one
+//     contiguous stretch of TL*N doubles; along y and z every line point of the
+//     tile is a row of TL consecutive doubles.  Halo points of the band product
+//     are read from LDS, so every global byte is loaded once and stored once.
+//   * Inside the tile G lanes of one wavefront share a line; lane g owns the M
+//     points [g*M, g*M+M) (M a power of two).  LDS holds point e of tile line l
+//     at (e % M)*ROW + l*G + e/M, so the lanes of a wavefront read consecutive
+//     LDS words at every step.
+//   * The band product and both Thomas sweeps run in VGPRs.  The Thomas
+//     recurrences are first-order affine recurrences: each lane composes the
+//     affine map of its segment (the offset of the composite follows the
+//     reference recurrence exactly), the G lanes combine their maps with a
+//     log2(G)-step wavefront scan, and each lane re-runs the reference recurrence
+//     from its exact start value.  Forward: d_i = (r_i - A d_{i-1}) B; backward:
+//     x_i = d_i - C x_{i+1}.
+//   * Periodic lines first reduce x_1 = D . r over the lanes of the line, then
+//     apply the reference's row-2 / row-N corrections and solve rows 2..N
 //     (derivs.F90:785-802, including its index shift of B).
-//   * x lines are contiguous: the G lanes of a line are adjacent lanes, so a
-//     wavefront reads one 64*m-point stretch.  y and z lines are strided: lanes
-//     with consecutive line index i are adjacent, so each load instruction reads
-//     64/G consecutive doubles of G rows.
 //   * Filter mode writes in place with the FilterCharTime blend fused
 //     (derivs.F90:344-349).
 #pragma once
@@ -30,55 +33,77 @@ namespace gatt {
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
 __device__ __forceinline__ double shfl_xor_d(double v, int mask) { return __shfl_xor(v, mask, 64); }
 
+// Global offset of the first point of line `line` along direction DIR.
+template <int DIR>
+__device__ __forceinline__ long long line_base(const SolveArgs& a, int line) {
+  if (DIR == 0) return (long long)a.nx * line;                       // line = j + ny k
+  if (DIR == 2) return line;                                         // line = i + nx j
+  const int k = line / a.nx;                                         // line = i + nx k
+  return (long long)(line - k * a.nx) + (long long)a.nx * a.ny * k;
+}
+
 template <int DIR, int NR, int M>
 __global__ __launch_bounds__(256) void line_solve_kernel(SolveArgs a) {
+  extern __shared__ double lds[];
   constexpr int H = NR - 1;
   constexpr int W = M + 2 * H;
-  const int lane = threadIdx.x & 63;
-  const int wave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
   const int G = a.G;
-  const int LPW = 64 / G;
-  constexpr bool xdir = DIR == 0;
-  // lane layout: x -> lanes of a line adjacent; y/z -> lines adjacent
-  const int g = xdir ? (lane & (G - 1)) : (lane / LPW);
-  const int l = xdir ? (lane / G) : (lane & (LPW - 1));
-  const int step = xdir ? 1 : LPW;   // lane distance between neighbouring segments
-  const int line = wave * LPW + l;
-  const bool active = line < a.nlines;
+  const int TL = a.TL;               // lines per tile = (64/G) * waves per block
+  const int ROW = a.lds_row;         // TL*G + 1
+  const int N = a.op.N;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int g = lane & (G - 1);
+  const int l = (tid >> 6) * (64 / G) + lane / G;
+  const int line0 = blockIdx.x * TL;
+  const int ntile = min(TL, a.nlines - line0);
   const int f = (int)blockIdx.y % a.nf;
   const int mem = (int)blockIdx.y / a.nf;
   const double* in = a.in[f] + (long long)mem * a.member_stride;
   double* out = a.out[f] + (long long)mem * a.member_stride;
-  const int N = a.op.N;
-  const int m = a.m;
+  const long long stride = DIR == 0 ? 1 : a.stride;
+
+  // ---- HBM -> LDS, unit stride
+  const int total = ntile * N;
+  if (DIR == 0) {
+    const double* src = in + line_base<0>(a, line0);   // the tile is contiguous
+    for (int idx = tid; idx < total; idx += blockDim.x) {
+      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+      const int ep = idx - lp * N;
+      lds[(ep & (M - 1)) * ROW + lp * G + ((unsigned)ep / M)] = src[idx];
+    }
+  } else {
+    for (int idx = tid; idx < TL * N; idx += blockDim.x) {
+      const int ep = (unsigned)idx / (unsigned)TL;   // TL is a power of two
+      const int lp = idx - ep * TL;
+      if (lp < ntile) lds[(ep & (M - 1)) * ROW + lp * G + ((unsigned)ep / M)] = in[line_base<DIR>(a, line0 + lp) + ep * stride];
+    }
+  }
+  __syncthreads();
+
+  const bool active = l < ntile;
   const int per = a.op.periodic;
   const int e0 = per;
-  const int s = g * m;
+  const int s = g * M;
   int cnt = N - s;
-  cnt = cnt < 0 ? 0 : (cnt > m ? m : cnt);
+  cnt = cnt < 0 ? 0 : (cnt > M ? M : cnt);
   if (!active) cnt = 0;
-  const int la = active ? line : 0;
-  const int ia = la % a.n1, ib = la / a.n1;
-  long long base;
-  if (DIR == 0) base = (long long)a.nx * (ia + (long long)a.ny * ib);
-  else if (DIR == 1) base = ia + (long long)a.nx * a.ny * ib;
-  else base = ia + (long long)a.nx * ib;
-  const long long stride = DIR == 0 ? 1 : a.stride;
-  const int t = a.types[la];
+  const int t = a.types[active ? line0 + l : 0];
   const double* Rt = a.op.R + (size_t)t * N * (2 * NR - 1);
   const double* At = a.op.A + (size_t)t * (N - 1);
   const double* Bt = a.op.B + (size_t)t * (N - per);
   const double* Ct = a.op.C + (size_t)t * (N - 1);
+  const int lbase = l * G;
 
-  // ---- load the segment and its halo (wrapped like the reference's modulo)
+  // ---- segment and halo from LDS (wrapped like the reference's modulo)
   double fw[W];
 #pragma unroll
   for (int u = 0; u < W; ++u) {
     double v = 0.0;
-    if (u < cnt + 2 * H) {
-      int idx = s - H + u;
-      idx = idx < 0 ? idx + N : (idx >= N ? idx - N : idx);
-      v = in[base + (long long)idx * stride];
+    if (cnt > 0 && u < cnt + 2 * H) {
+      int e = s - H + u;
+      e = e < 0 ? e + N : (e >= N ? e - N : e);
+      v = lds[(e & (M - 1)) * ROW + lbase + (unsigned)e / M];
     }
     fw[u] = v;
   }
@@ -106,7 +131,7 @@ __global__ __launch_bounds__(256) void line_solve_kernel(SolveArgs a) {
 #pragma unroll
     for (int tt = 0; tt < M; ++tt)
       if (tt < cnt) part = part + r[tt] * Dt[s + tt];
-    for (int k = 1; k < G; k <<= 1) part += shfl_xor_d(part, k * step);
+    for (int k = 1; k < G; k <<= 1) part += shfl_xor_d(part, k);
     x1 = part;
 #pragma unroll
     for (int tt = 0; tt < M; ++tt) {
@@ -118,7 +143,7 @@ __global__ __launch_bounds__(256) void line_solve_kernel(SolveArgs a) {
     }
   }
 
-  // ---- forward sweep: composite map of the segment
+  // ---- forward sweep: composite map of the segment, scan, re-run
   double P = 0.0, Q = 1.0;
 #pragma unroll
   for (int tt = 0; tt < M; ++tt) {
@@ -130,13 +155,12 @@ __global__ __launch_bounds__(256) void line_solve_kernel(SolveArgs a) {
       Q = -(fa * fb) * Q;
     }
   }
-  // inclusive scan over the segments of the line
   for (int k = 1; k < G; k <<= 1) {
-    const double Pp = shfl_d(P, lane - k * step);
-    const double Qp = shfl_d(Q, lane - k * step);
+    const double Pp = shfl_d(P, lane - k);
+    const double Qp = shfl_d(Q, lane - k);
     if (g >= k) { P = P + Q * Pp; Q = Q * Qp; }
   }
-  double dprev = shfl_d(P, lane - step);
+  double dprev = shfl_d(P, lane - 1);
   if (g == 0) dprev = 0.0;
 #pragma unroll
   for (int tt = 0; tt < M; ++tt) {
@@ -162,11 +186,11 @@ __global__ __launch_bounds__(256) void line_solve_kernel(SolveArgs a) {
     }
   }
   for (int k = 1; k < G; k <<= 1) {
-    const double Pn = shfl_d(P, lane + k * step);
-    const double Qn = shfl_d(Q, lane + k * step);
+    const double Pn = shfl_d(P, lane + k);
+    const double Qn = shfl_d(Q, lane + k);
     if (g + k < G) { P = P + Q * Pn; Q = Q * Qn; }
   }
-  double xnext = shfl_d(P, lane + step);
+  double xnext = shfl_d(P, lane + 1);
   if (g == G - 1) xnext = 0.0;
 #pragma unroll
   for (int tt = M - 1; tt >= 0; --tt) {
@@ -178,7 +202,8 @@ __global__ __launch_bounds__(256) void line_solve_kernel(SolveArgs a) {
     }
   }
 
-  // ---- store (filter mode: in place with the FilterCharTime blend)
+  // ---- results -> LDS (filter mode: FilterCharTime blend), then LDS -> HBM
+  __syncthreads();
 #pragma unroll
   for (int tt = 0; tt < M; ++tt) {
     if (tt < cnt) {
@@ -188,19 +213,36 @@ __global__ __launch_bounds__(256) void line_solve_kernel(SolveArgs a) {
         const double F0 = fw[tt + H];
         v = F0 + a.theta * (v - F0);
       }
-      out[base + (long long)e * stride] = v;
+      lds[tt * ROW + lbase + g] = v;
+    }
+  }
+  __syncthreads();
+  if (DIR == 0) {
+    double* dst = out + line_base<0>(a, line0);
+    for (int idx = tid; idx < total; idx += blockDim.x) {
+      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+      const int ep = idx - lp * N;
+      dst[idx] = lds[(ep & (M - 1)) * ROW + lp * G + ((unsigned)ep / M)];
+    }
+  } else {
+    for (int idx = tid; idx < TL * N; idx += blockDim.x) {
+      const int ep = (unsigned)idx / (unsigned)TL;
+      const int lp = idx - ep * TL;
+      if (lp < ntile) out[line_base<DIR>(a, line0 + lp) + ep * stride] = lds[(ep & (M - 1)) * ROW + lp * G + ((unsigned)ep / M)];
     }
   }
 }
 
 template <int DIR, int NR, int M>
-static void launch_nr_m(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st) {
-  const int lpw = 64 / s.G;
-  const long long waves = (a.nlines + lpw - 1) / lpw;
-  const int threads = 64 * s.waves_per_block;
-  const long long blocks = (waves + s.waves_per_block - 1) / s.waves_per_block;
-  dim3 grid((unsigned)blocks, (unsigned)(a.nf * nmembers));
-  hipLaunchKernelGGL((line_solve_kernel<DIR, NR, M>), grid, dim3(threads), 0, st, a);
+static void launch_nr_m(const SolveArgs& a0, const LineShape& s, int nmembers, hipStream_t st) {
+  SolveArgs a = a0;
+  a.TL = (64 / s.G) * s.waves_per_block;
+  a.lds_row = a.TL * s.G + 1;
+  a.inv_n = 1.0f / (float)a.op.N;
+  const long long tiles = (a.nlines + a.TL - 1) / a.TL;
+  const size_t lds_bytes = (size_t)M * a.lds_row * sizeof(double);
+  dim3 grid((unsigned)tiles, (unsigned)(a.nf * nmembers));
+  hipLaunchKernelGGL((line_solve_kernel<DIR, NR, M>), grid, dim3(64 * s.waves_per_block), lds_bytes, st, a);
 }
 
 template <int DIR, int NR>

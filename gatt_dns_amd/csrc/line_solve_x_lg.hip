@@ -1,4 +1,4 @@
-// Instantiation of the compact line solves along x (see line_solve_impl.h).
+// Lane-group line solves along x (line_solve_impl.h).
 #include "line_solve_impl.h"
 
 namespace gatt {

@@ -1,4 +1,4 @@
-// Instantiation of the compact line solves along z (see line_solve_impl.h).
+// Lane-group line solves along z (line_solve_impl.h).
 #include "line_solve_impl.h"
 
 namespace gatt {

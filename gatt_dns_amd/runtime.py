@@ -22,7 +22,8 @@ LIB_PATH = os.path.join(HERE, "_lib", "libgatt_dns.so")
 EXPORTS = ["gatt_last_error", "gatt_device_info", "gatt_create", "gatt_destroy", "gatt_set_state",
            "gatt_get_state", "gatt_set_mean", "gatt_get_mean", "gatt_rk4", "gatt_apply_filters",
            "gatt_apply_bound_cond", "gatt_sfd", "gatt_rhs", "gatt_step", "gatt_advance",
-           "gatt_line_solve", "gatt_sync", "gatt_time_kernel"]
+           "gatt_line_solve", "gatt_sync", "gatt_time_kernel", "gatt_set_line_shape",
+           "gatt_get_line_shape"]
 
 
 class GattError(RuntimeError):
@@ -58,6 +59,9 @@ def load_library(path=LIB_PATH):
     L.gatt_line_solve.argtypes = [V, I, I, I, C.POINTER(DP), C.POINTER(DP)]
     L.gatt_sync.argtypes = [V]
     L.gatt_time_kernel.argtypes = [V, I, I, DP, DP]
+    L.gatt_set_line_shape.argtypes = [V, I, I, I, I, I]
+    IPt = C.POINTER(I)
+    L.gatt_get_line_shape.argtypes = [V, I, I, IPt, IPt, IPt, IPt]
     _lib = L
     return L
 
@@ -160,6 +164,17 @@ class Solver:
         by = C.c_double(0)
         _check(self.lib.gatt_time_kernel(self.ctx, kernel, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    def set_line_shape(self, direction, kind, variant, param, waves_per_block=4):
+        """variant 0: param = lanes per line; variant 1: param = points per lane."""
+        _check(self.lib.gatt_set_line_shape(self.ctx, direction, kind, variant, param, waves_per_block))
+
+    def get_line_shape(self, direction, kind):
+        """(variant, G, M, waves per block) of the current shape."""
+        v, g, m, w = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        _check(self.lib.gatt_get_line_shape(self.ctx, direction, kind, C.byref(v), C.byref(g), C.byref(m),
+                                            C.byref(w)))
+        return v.value, g.value, m.value, w.value
 
     def line_solve(self, direction, kind, fields_in, fields_out):
         """Device-pointer unit entry point: lists of torch CUDA tensors (float64, Fortran-ordered data)."""

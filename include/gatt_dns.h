@@ -202,6 +202,20 @@ int gatt_line_solve(gatt_ctx* ctx, int32_t dir, int32_t kind, int32_t nfields,
 /* Synchronise the context's stream. */
 int gatt_sync(gatt_ctx* ctx);
 
+/* Launch shape of the line solves along `dir` for operator `kind`.  No
+ * reference counterpart (launch tuning); chosen by a built-in heuristic at
+ * create time and overridden by the autotuner (gatt_dns_amd/autotune.py).
+ *   variant 0, lane groups:   param = G lanes of a wavefront per line (power of
+ *                             two, 2..64), waves_per_block in 1..4;
+ *   variant 1, wave segments: param = M points per lane (4, 8, 16, 32); each
+ *                             lane owns a line, ceil(n/M) wavefronts per
+ *                             workgroup (waves_per_block ignored).
+ * Shapes that do not fit the line or the device's LDS are rejected. */
+int gatt_set_line_shape(gatt_ctx* ctx, int32_t dir, int32_t kind, int32_t variant, int32_t param,
+                        int32_t waves_per_block);
+int gatt_get_line_shape(gatt_ctx* ctx, int32_t dir, int32_t kind, int32_t* variant, int32_t* G, int32_t* M,
+                        int32_t* waves_per_block);
+
 /* Kernel classes of one time step, for timing. */
 enum {
   GATT_K_DERIV_X = 0, GATT_K_DERIV_Y = 1, GATT_K_DERIV_Z = 2,

@@ -81,6 +81,37 @@ def test_line_solve(rt, oracle_mod, name, kind):
         assert rel_err(got, want) < 1e-13, (name, kind, d, rel_err(got, want))
 
 
+@pytest.mark.parametrize("name", ["cavity33", "cavity129", "box32", "box48_noforce"])
+def test_line_solve_all_shapes(rt, oracle_mod, name):
+    """Every accepted launch shape of both kernel variants gives the oracle's result."""
+    import torch
+    case = CASES[name]()
+    o = oracle_mod.Oracle(case)
+    s = rt.Solver(case)
+    rng = np.random.default_rng(21)
+    shape = (case.nx, case.ny, case.nz)
+    dims = shape
+    tried = 0
+    for d in range(3 if case.is3d else 2):
+        F = np.asfortranarray(rng.standard_normal(shape))
+        tin = torch.from_numpy(F.ravel(order="F").copy()).cuda()
+        for kind in (0, 1):
+            want = o.solve(d, kind, F)
+            shapes = [(1, M, 1) for M in (4, 8, 16, 32)]
+            shapes += [(0, G, w) for G in (2, 4, 8, 16, 32, 64) for w in (1, 2, 4)]
+            for v, p, w in shapes:
+                try:
+                    s.set_line_shape(d, kind, v, p, w)
+                except rt.GattError:
+                    continue
+                tout = torch.zeros_like(tin)
+                s.line_solve(d, kind, [tin], [tout])
+                got = tout.cpu().numpy().reshape(shape, order="F")
+                assert rel_err(got, want) < 1e-13, (name, d, kind, v, p, w, rel_err(got, want))
+                tried += 1
+    assert tried > 10
+
+
 @pytest.mark.parametrize("name", ["cavity33", "box32"])
 def test_rhs(rt, oracle_mod, name):
     case = CASES[name]()
@@ -199,30 +230,33 @@ def test_box_steps(rt, oracle_mod):
         assert rel_err(got[k], want[k]) <= 1e-11, (k, rel_err(got[k], want[k]))
 
 
-def test_sfd_step(rt, oracle_mod):
+@pytest.mark.parametrize("nsteps,tol", [(1, 1e-12), (3, 1e-11)])
+def test_sfd_step(rt, oracle_mod, nsteps, tol):
     case = CASES["cavity33"]()
     case.sfd_type, case.sfd_delta, case.sfd_x_val = 1, 0.5, 0.05
     o = oracle_mod.Oracle(case)
     q0 = o.apply_bound_cond(case.state0, 0.0)
     o.init_mean(q0)
-    want, mc_w, _ = o.run(q0, 0.0, case.dt, 3)
+    want, mc_w, _ = o.run(q0, 0.0, case.dt, nsteps)
     s = rt.Solver(case)
     s.set_state(case.state0)
     s.apply_bound_cond(0.0)
     s.set_mean(q0)
-    s.advance(3, 0.0, case.dt)
+    s.advance(nsteps, 0.0, case.dt)
     got = s.get_state()
     for k in FIELDS2:
-        assert rel_err(got[k], want[k]) <= 1e-12, k
+        assert rel_err(got[k], want[k]) <= tol, (k, rel_err(got[k], want[k]))
     gm, wm = s.get_mean(), o.mean()
     for k in FIELDS2:
-        assert rel_err(gm[k], wm[k]) <= 1e-12, k
+        assert rel_err(gm[k], wm[k]) <= tol, k
 
 
 def test_divergence_flag_zero_change(rt):
-    # a uniform flow at rest on a periodic box has a zero RHS: the step changes
-    # nothing and the time loop's divergence test (sum of changes == 0) fires
+    # a uniform flow at rest on a periodic box has a zero RHS; with the filter off
+    # (filtering a constant changes it by rounding) the step changes nothing and
+    # the time loop's divergence test (sum of changes == 0) fires
     case = cases.turbulence_box(16, forcing=False)
+    case.filter_dirs = (0, 0, 0)
     q = {k: np.zeros((16, 16, 16), order="F") for k in "UVW"}
     q["R"] = np.ones((16, 16, 16), order="F")
     q["E"] = np.full((16, 16, 16), case.E0, order="F")
