@@ -47,6 +47,7 @@ struct DirOps {
   double* coef_ends[2] = {nullptr, nullptr};  // per type A(1), C(1)
   LineShape shape[2];
   int* types = nullptr;      // per line, 0-based
+  bool uniform_tiles = false;  // every tile of 64 consecutive lines has one type
   int n1 = 0, nlines = 0;
   long long stride = 0;
 };
@@ -126,14 +127,14 @@ int dev_upload(gatt_ctx* c, const T* host, size_t count, T** out) {
 // Default shape before tuning.  Many lines of at most 256 points: the
 // wave-segment kernel with 16 (or 32) points per lane; otherwise the lane-group
 // kernel with 16 points per lane where the line allows it.
-LineShape choose_shape(int dir, int N, int nR, long long nlines, long long max_lds) {
-  if (nlines >= 1024) {
+LineShape choose_shape(int dir, int N, int nR, long long nlines, long long max_lds, bool uniform_tiles) {
+  if (nlines >= 1024 && uniform_tiles) {
     for (int M : {16, 32}) {
       LineShape s;
       s.variant = 1;
       s.M = M;
       s.G = (N + M - 1) / M;
-      if (line_shape_ok(s, N, nR, dir, max_lds)) return s;
+      if (line_shape_ok(s, N, nR, dir, max_lds, uniform_tiles)) return s;
     }
   }
   for (int M : {16, 8, 4, 32}) {
@@ -729,9 +730,12 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
       if (v < 0 || v >= o.op[0].ntypes || v >= o.op[1].ntypes) return fail(GATT_E_INVALID, "type id outside 1..ntypes");
     }
     if ((rc = dev_upload(c.get(), tm.data(), tm.size(), &o.types))) return rc;
+    o.uniform_tiles = true;
+    for (size_t l = 0; l < tm.size(); ++l)
+      if (tm[l] != tm[l - l % 64]) o.uniform_tiles = false;
     for (int kind = 0; kind < 2; ++kind) {
-      o.shape[kind] = choose_shape(k, dims[k], o.op[kind].nR, o.nlines, c->max_lds);
-      if (!line_shape_ok(o.shape[kind], dims[k], o.op[kind].nR, k, c->max_lds))
+      o.shape[kind] = choose_shape(k, dims[k], o.op[kind].nR, o.nlines, c->max_lds, o.uniform_tiles);
+      if (!line_shape_ok(o.shape[kind], dims[k], o.op[kind].nR, k, c->max_lds, o.uniform_tiles))
         return fail(GATT_E_UNSUPPORTED, "no line-solve shape fits this line length");
     }
   }
@@ -988,7 +992,7 @@ int gatt_set_line_shape(gatt_ctx* c, int32_t dir, int32_t kind, int32_t variant,
   } else {
     return fail(GATT_E_INVALID, "unknown line-solve variant");
   }
-  if (!line_shape_ok(s, N, d.op[kind].nR, dir, c->max_lds))
+  if (!line_shape_ok(s, N, d.op[kind].nR, dir, c->max_lds, d.uniform_tiles))
     return fail(GATT_E_UNSUPPORTED, "line shape does not fit this line");
   d.shape[kind] = s;
   if (c->graph) {

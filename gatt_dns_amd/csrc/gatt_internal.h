@@ -47,6 +47,7 @@ struct SolveArgs {
   int TL;                     // lines per workgroup tile
   int lds_row;                // LDS words per point row of the tile
   float inv_n;                // 1/N
+  int lds_coef_offset;        // wave-segment: LDS word offset of the per-wavefront coefficient slots
   int nf;                     // fields per member
   long long member_stride;    // doubles between members
   const double* in[kMaxPtr];
@@ -75,13 +76,18 @@ inline void launch_line_solve(const SolveArgs& a, const LineShape& s, int nmembe
 // A shape is usable when its register tile is compiled (M in 4..32), the
 // segments cover the line, the wrapped halo stays within one period
 // (M + nR - 1 <= N + 1) and the LDS tile fits in 64 KiB.
-inline bool line_shape_ok(const LineShape& s, int N, int nR, int dir = 1, long long max_lds = 65536) {
+inline bool line_shape_ok(const LineShape& s, int N, int nR, int dir = 1, long long max_lds = 65536,
+                          bool uniform_tiles = false) {
   if (!(s.M == 4 || s.M == 8 || s.M == 16 || s.M == 32) || nR < 2 || nR > 11) return false;
   if (s.variant == 1) {
+    // wave-segment: compiled for 16 and 32 points per lane, and only for
+    // directions whose 64-line tiles each carry one operator type
+    if (!(s.M == 16 || s.M == 32) || !uniform_tiles) return false;
     const int G = (N + s.M - 1) / s.M;
     if (G < 1 || G > (s.M >= 32 ? 8 : 16) || s.M + nR - 1 > N + 1 || nR - 1 > N) return false;
-    const long long maps = 5LL * G * 64, tile = dir == 0 ? 65LL * N : 0;
-    return 8 * (maps > tile ? maps : tile) <= max_lds;
+    const long long cs = (2LL * nR - 1 + 4 + 1) & ~1LL;
+    const long long work = 5LL * G * 64 + G * (s.M * cs + 2), tile = dir == 0 ? 65LL * N : 0;
+    return 8 * (work > tile ? work : tile) <= max_lds;
   }
   if (s.G < 2 || s.G > 64 || (s.G & (s.G - 1)) || s.waves_per_block < 1 || s.waves_per_block > 4) return false;
   if (s.G * s.M < N || s.M + nR - 1 > N + 1 || nR - 1 > N) return false;

@@ -4,7 +4,8 @@
 // Layout: a workgroup owns 64 lines; lane l of every wavefront works on line
 // line0 + l, and wavefront w owns the M points [w*M, w*M+M) of all 64 lines.
 // Every lane of a wavefront is at the same point at the same time, so the
-// coefficients of that point are one wavefront-uniform load shared by 64 lines.
+// coefficients of that point are the same for all 64 lines; each wavefront
+// copies its segment's coefficients to LDS once and the sweeps read broadcasts.
 //   * y and z lines: lanes are consecutive i, so each point of the segment is
 //     one 512-byte row load and store.
 //   * x lines: the 64 lines are one contiguous stretch of 64*N doubles, moved
@@ -22,81 +23,26 @@ namespace gatt {
 constexpr int ws_coef_stride(int nR) { return (2 * nR - 1 + 4 + 1) & ~1; }
 constexpr int kMaxWaveSegments = 16;
 
-template <int DIR, int NR, int M, bool XLDS, bool BLEND>
-__global__ __launch_bounds__(M >= 32 ? 512 : 1024) void line_solve_ws_kernel(SolveArgs a) {
-  extern __shared__ double sh[];
+// Band product, periodic correction, both sweeps and the store of one segment.
+template <int DIR, int NR, int M, bool XLDS, bool BLEND, typename CP>
+__device__ __forceinline__ void ws_solve(const SolveArgs& a, double* sh, double* tile, const double (&fw)[M + 2 * (NR - 1)],
+                                         CP cf, CP cends, int N, int s, int cnt, int w, int G, int lane,
+                                         bool active, double* out, long long base, long long stride) {
   constexpr int H = NR - 1;
   constexpr int NW = 2 * H + 1;
   constexpr int CS = ws_coef_stride(NR);
-  constexpr int W = M + 2 * H;
-  const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int G = blockDim.x >> 6;
-  const int N = a.op.N;
-  const int line0 = blockIdx.x * 64;
-  const int line = line0 + lane;
-  const int nact = min(64, a.nlines - line0);
-  const bool active = lane < nact;
-  const int f = (int)blockIdx.y % a.nf;
-  const int mem = (int)blockIdx.y / a.nf;
-  const double* in = a.in[f] + (long long)mem * a.member_stride;
-  double* out = a.out[f] + (long long)mem * a.member_stride;
-  const long long stride = a.stride;
   const int per = a.op.periodic;
   const int e0 = per;
-  const int s = w * M;
-  int cnt = N - s;
-  cnt = cnt < 0 ? 0 : (cnt > M ? M : cnt);
-  // LDS: scan maps and dot partials; the x tile (N rows of 65 words) shares the
-  // same words, separated from them by barriers
   double* shP = sh;                       // forward maps   [G][64][2]
   double* shB = sh + 2 * G * 64;          // backward maps  [G][64][2]
   double* shD = sh + 4 * G * 64;          // periodic dot partials [G][64]
-  double* tile = sh;
-
-  long long base = 0;
-  if (DIR == 1) {
-    const int k = (active ? line : line0) / a.nx;
-    base = (long long)((active ? line : line0) - k * a.nx) + (long long)a.nx * a.ny * k;
-  } else if (DIR == 2) {
-    base = active ? line : line0;
-  }
-
-  if (XLDS) {
-    const double* src = in + (long long)a.nx * line0;
-    const int total = nact * N;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
-      const int ep = idx - lp * N;
-      tile[ep * 65 + lp] = src[idx];
-    }
-    __syncthreads();
-  }
-
-  // type of the line; wavefront-uniform in the usual case
-  const int t = a.types[active ? line : line0];
-  const int tu = __builtin_amdgcn_readfirstlane(t);
-  const double* __restrict__ cf = a.coef + ((size_t)(active ? t : tu) * N + s) * CS;
-  const double* __restrict__ cends = a.coef_ends + (size_t)(active ? t : tu) * 2;
-
-  // every load is in bounds (wrapped, then clamped for empty segments), so the
-  // window is fetched without branches
-  double fw[W];
-#pragma unroll
-  for (int u = 0; u < W; ++u) {
-    int e = s - H + u;
-    e = e < 0 ? e + N : (e >= N ? e - N : e);
-    e = e >= N ? N - 1 : e;
-    fw[u] = XLDS ? tile[e * 65 + lane] : in[base + (long long)e * stride];
-  }
-  if (XLDS) __syncthreads();   // the tile words are reused by the scan maps
 
   double r[M];
 #pragma unroll
   for (int tt = 0; tt < M; ++tt) {
     double acc = 0.0;
     if (tt < cnt) {
-      const double* c = cf + tt * CS;
+      CP c = cf + tt * CS;
       acc = c[H] * fw[tt + H];
 #pragma unroll
       for (int q = 1; q <= H; ++q) acc = acc + c[H + q] * fw[tt + H + q] + c[H - q] * fw[tt + H - q];
@@ -195,6 +141,78 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024) void line_solve_ws_kernel(Sol
       else if (active) out[base + (long long)e * stride] = v;
     }
   }
+}
+
+template <int DIR, int NR, int M, bool XLDS, bool BLEND>
+__global__ __launch_bounds__(M >= 32 ? 512 : 1024) void line_solve_ws_kernel(SolveArgs a) {
+  extern __shared__ double sh[];
+  constexpr int H = NR - 1;
+  constexpr int CS = ws_coef_stride(NR);
+  constexpr int W = M + 2 * H;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int G = blockDim.x >> 6;
+  const int N = a.op.N;
+  const int line0 = blockIdx.x * 64;
+  const int line = line0 + lane;
+  const int nact = min(64, a.nlines - line0);
+  const bool active = lane < nact;
+  const int f = (int)blockIdx.y % a.nf;
+  const int mem = (int)blockIdx.y / a.nf;
+  const double* in = a.in[f] + (long long)mem * a.member_stride;
+  double* out = a.out[f] + (long long)mem * a.member_stride;
+  const long long stride = a.stride;
+  const int s = w * M;
+  int cnt = N - s;
+  cnt = cnt < 0 ? 0 : (cnt > M ? M : cnt);
+  // LDS: scan maps and dot partials; the x tile (N rows of 65 words) shares the
+  // same words, separated from them by barriers
+  double* tile = sh;
+
+  long long base = 0;
+  if (DIR == 1) {
+    const int k = (active ? line : line0) / a.nx;
+    base = (long long)((active ? line : line0) - k * a.nx) + (long long)a.nx * a.ny * k;
+  } else if (DIR == 2) {
+    base = active ? line : line0;
+  }
+
+  if (XLDS) {
+    const double* src = in + (long long)a.nx * line0;
+    const int total = nact * N;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+      const int ep = idx - lp * N;
+      tile[ep * 65 + lp] = src[idx];
+    }
+    __syncthreads();
+  }
+
+  // every load is in bounds (wrapped, then clamped for empty segments), so the
+  // window is fetched without branches
+  double fw[W];
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    int e = s - H + u;
+    e = e < 0 ? e + N : (e >= N ? e - N : e);
+    e = e >= N ? N - 1 : e;
+    fw[u] = XLDS ? tile[e * 65 + lane] : in[base + (long long)e * stride];
+  }
+  if (XLDS) __syncthreads();   // the tile words are reused by the scan maps
+
+  // operator type of the 64 lines: the host only selects this kernel for
+  // directions whose 64-line tiles each have a single type
+  const int tu = __builtin_amdgcn_readfirstlane(a.types[line0]);
+  // the segment's coefficients (M points) and the two periodic factors go to
+  // this wavefront's LDS slot in one batch; the sweeps then read broadcasts
+  double* cl = sh + a.lds_coef_offset + (size_t)w * (M * CS + 2);
+  const double* src = a.coef + ((size_t)tu * N + s) * CS;
+  for (int q = lane; q < cnt * CS; q += 64) cl[q] = src[q];
+  if (lane < 2) cl[M * CS + lane] = a.coef_ends[(size_t)tu * 2 + lane];
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  ws_solve<DIR, NR, M, XLDS, BLEND>(a, sh, tile, fw, (const double*)cl, (const double*)(cl + M * CS), N, s, cnt, w,
+                                    G, lane, active, out, base, stride);
   if (XLDS) {
     __syncthreads();
     double* dst = out + (long long)a.nx * line0;
@@ -207,10 +225,18 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024) void line_solve_ws_kernel(Sol
   }
 }
 
-inline size_t ws_lds_bytes(int dir, int G, int N) {
-  const size_t maps = (size_t)5 * G * 64;
+// LDS words: scan maps (5*G*64), then per wavefront M*CS + 2 coefficient words.
+// Along x the tile (65*N words) is dead once the windows are in registers, so
+// maps and coefficients reuse its words.
+inline size_t ws_lds_words_front(int dir, int G, int N) {
+  (void)dir;
+  (void)N;
+  return (size_t)5 * G * 64;
+}
+inline size_t ws_lds_bytes(int dir, int G, int N, int M, int nR) {
+  const size_t work = ws_lds_words_front(dir, G, N) + (size_t)G * (M * ws_coef_stride(nR) + 2);
   const size_t tile = dir == 0 ? (size_t)N * 65 : 0;
-  return sizeof(double) * (maps > tile ? maps : tile);
+  return sizeof(double) * (work > tile ? work : tile);
 }
 
 template <int DIR, int NR, int M>
@@ -219,20 +245,19 @@ static void launch_ws_m(const SolveArgs& a0, const LineShape& s, int nmembers, h
   a.inv_n = 1.0f / (float)a.op.N;
   const int G = (a.op.N + M - 1) / M;
   const long long tiles = (a.nlines + 63) / 64;
-  const size_t lds = ws_lds_bytes(DIR, G, a.op.N);
+  a.lds_coef_offset = (int)ws_lds_words_front(DIR, G, a.op.N);
+  const size_t lds = ws_lds_bytes(DIR, G, a.op.N, M, NR);
   dim3 grid((unsigned)tiles, (unsigned)(a.nf * nmembers));
   const bool blend = a.mode == 1 && a.theta > 0.0;
   constexpr bool X = DIR == 0;
   auto k = blend ? line_solve_ws_kernel<DIR, NR, M, X, true> : line_solve_ws_kernel<DIR, NR, M, X, false>;
-  if (lds > 65536) hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(k, grid, dim3(64 * G), lds, st, a);
 }
 
 template <int DIR, int NR>
 static void launch_ws_nr(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st) {
   switch (s.M) {
-    case 4: launch_ws_m<DIR, NR, 4>(a, s, nmembers, st); break;
-    case 8: launch_ws_m<DIR, NR, 8>(a, s, nmembers, st); break;
     case 16: launch_ws_m<DIR, NR, 16>(a, s, nmembers, st); break;
     case 32: launch_ws_m<DIR, NR, 32>(a, s, nmembers, st); break;
   }

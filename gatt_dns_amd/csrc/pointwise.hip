@@ -171,19 +171,27 @@ __global__ __launch_bounds__(256) void forcing_partial_kernel(SumArgs a) {
   if (threadIdx.x < 5) a.partial[((long long)blockIdx.y * a.nblocks + blockIdx.x) * 5 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ void forcing_final_kernel(SumArgs a) {
+// Fixed-order tree over the block partials: thread k sums partials k, k+256, ...
+__global__ __launch_bounds__(256) void forcing_final_kernel(SumArgs a) {
+  __shared__ double sh[5][256];
   const int mem = blockIdx.x;
-  if (threadIdx.x < 5) {
-    double s = 0;
-    for (int b = 0; b < a.nblocks; ++b) s += a.partial[((long long)mem * a.nblocks + b) * 5 + threadIdx.x];
-    a.sums[8 * mem + threadIdx.x] = s;
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x)
+    for (int k = 0; k < 5; ++k) acc[k] += a.partial[((long long)mem * a.nblocks + b) * 5 + k];
+  for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
   }
+  if (threadIdx.x < 5) a.sums[8 * mem + threadIdx.x] = sh[threadIdx.x][0];
   if (threadIdx.x == 5) a.sums[8 * mem + 5] = (double)a.np;
 }
 
 void launch_forcing_sums(const SumArgs& a, int nmembers, hipStream_t st) {
   hipLaunchKernelGGL(forcing_partial_kernel, dim3(a.nblocks, nmembers), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(forcing_final_kernel, dim3(nmembers), dim3(64), 0, st, a);
+  hipLaunchKernelGGL(forcing_final_kernel, dim3(nmembers), dim3(256), 0, st, a);
 }
 
 // main.F90:382-401

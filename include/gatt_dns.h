@@ -207,9 +207,11 @@ int gatt_sync(gatt_ctx* ctx);
  * create time and overridden by the autotuner (gatt_dns_amd/autotune.py).
  *   variant 0, lane groups:   param = G lanes of a wavefront per line (power of
  *                             two, 2..64), waves_per_block in 1..4;
- *   variant 1, wave segments: param = M points per lane (4, 8, 16, 32); each
+ *   variant 1, wave segments: param = M points per lane (16 or 32); each
  *                             lane owns a line, ceil(n/M) wavefronts per
- *                             workgroup (waves_per_block ignored).
+ *                             workgroup (waves_per_block ignored); only for
+ *                             directions whose tiles of 64 consecutive lines
+ *                             each have one operator type.
  * Shapes that do not fit the line or the device's LDS are rejected. */
 int gatt_set_line_shape(gatt_ctx* ctx, int32_t dir, int32_t kind, int32_t variant, int32_t param,
                         int32_t waves_per_block);

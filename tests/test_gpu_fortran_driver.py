@@ -1,0 +1,39 @@
+"""The Fortran driver (gatt_dns_amd/fortran/gatt_driver.F90, amdflang) calls the
+library through its ISO_C_BINDING interface with a case read from include files
+in the reference's generated format; its final state must match the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from gatt_dns_amd.fortran.build_driver import CASES, LIB_DIR
+from gatt_dns_amd.prep import cases
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name,nsteps,tol", [("cavity33", 20, 1e-11), ("box24", 4, 1e-11)])
+def test_fortran_driver(oracle_mod, tmp_path, name, nsteps, tol):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    exe = os.path.join(LIB_DIR, "gatt_driver_%s" % name)
+    assert os.path.exists(exe), "Fortran driver not built: run __graft_entry__.build()"
+    case = CASES[name](cases)
+    shape = (case.nx, case.ny, case.nz)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    np.concatenate([np.asarray(case.state0[k]).ravel(order="F") for k in "UVWRE"]).astype("<f8").tofile(fin)
+    print("fortran driver", name, nsteps, flush=True)
+    r = subprocess.run([exe, str(nsteps), str(fin), str(fout)], capture_output=True, text=True, timeout=150)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(fout, dtype="<f8").reshape((5,) + shape[::-1]).transpose(0, 3, 2, 1)
+    o = oracle_mod.Oracle(case)
+    q0 = o.apply_bound_cond(case.state0, 0.0)
+    want, _, _ = o.run(q0, 0.0, case.dt, nsteps)
+    for i, k in enumerate("UVWRE"):
+        if not case.is3d and k == "W":
+            continue
+        scale = np.max(np.abs(want[k]))
+        err = np.max(np.abs(got[i] - want[k])) / scale
+        assert err <= tol, (name, k, err)
