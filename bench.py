@@ -51,6 +51,35 @@ def cpu_baseline(n=64, steps=6):
                       "%d^3, %d steps, %.1f s" % (n, steps, el)}
 
 
+def batched_throughput(runtime, case, members, steps, shapes=None):
+    """Mpoint-steps/s of `members` members advanced together vs one member advanced alone.
+
+    The members start from the case state (gatt_perturb with no vectors), as
+    the Krylov members of the stability analysis do; a step of the batch is
+    one step of every member.
+    """
+    res = {}
+    for label, nb in (("batched", members), ("single", 1)):
+        s = runtime.Solver(case, batch=nb)
+        if shapes:
+            for (d, k), (v, p, w, _) in shapes.items():
+                s.set_line_shape(d, k, v, p, w)
+        s.set_base(case.state0)
+        s.perturb(0, nb, None, 0.0)
+        s.apply_bound_cond(0.0)
+        s.advance(2, 0.0, case.dt)
+        s.sync()
+        t0 = time.perf_counter()
+        s.advance(steps, 2 * case.dt, case.dt)
+        s.sync()
+        el = time.perf_counter() - t0
+        res[label] = case.npoints * nb * steps / el / 1e6
+        s.close()
+        del s
+    return {"members": members, "steps": steps, "batched_mpts": round(res["batched"], 2),
+            "single_mpts": round(res["single"], 2), "speedup_vs_sequential": round(res["batched"] / res["single"], 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +88,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-lst", action="store_true", help="skip the batched-member (stability analysis) lines")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -118,6 +148,8 @@ def main():
     for name, v in per_class.items():
         log("  %-9s %.4f ms/launch  %.0f GB/s algorithmic  %.3f ms/step"
             % (name, v["ms"], v["bytes"] / (v["ms"] * 1e-3) / 1e9, v["share_ms_per_step"]))
+    shapes_out = {"%s_%s" % ("xyz"[d], "deriv" if k == 0 else "filter"): list(s.get_line_shape(d, k))
+                  for d in range(3) for k in (0, 1)}
     step_bytes = 8.0 * MODEL_TOUCHES_3D * case.npoints
     step_gbps = step_bytes / (ms_step * 1e-3) / 1e9
 
@@ -128,6 +160,23 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("calibration copy: 3 x %d bytes read + written" % (src.numel() * 8))
+
+    lst = None
+    if not args.no_lst:
+        # stability analysis: a block of 32 Krylov members advanced together versus
+        # 32 sequential plain runs (calcInstability.m launches one DNS per vector)
+        lst = {}
+        s.close()
+        del s
+        torch.cuda.empty_cache()
+        for label, mk, nsteps, shapes in (
+                ("turbulence_box_%d" % args.n, lambda: case, 2, tuned),
+                ("turbulence_box_64", lambda: cases.turbulence_box(64), 10, None),
+                ("lid_driven_129", lambda: cases.lid_driven(129), 50, None)):
+            c2 = mk()
+            log("batched members: %s" % label)
+            lst[label] = batched_throughput(runtime, c2, 32, nsteps, shapes)
+            log("  %s" % lst[label])
 
     cpu = None
     if not args.no_cpu:
@@ -149,8 +198,8 @@ def main():
                           "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(step_gbps / HBM_PEAK_GBPS, 4)},
         "kernels": {k: {"ms_per_launch": round(v["ms"], 5), "ms_per_step": round(v["share_ms_per_step"], 4)}
                     for k, v in per_class.items()},
-        "line_shapes": {"%s_%s" % ("xyz"[d], "deriv" if k == 0 else "filter"): list(s.get_line_shape(d, k))
-                        for d in range(3) for k in (0, 1)},
+        "line_shapes": shapes_out,
+        "batched_lst": lst,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

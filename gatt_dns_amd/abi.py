@@ -149,8 +149,14 @@ def build_desc(case, sfd_override=None):
 
 
 def field_ptrs(arrs, ctype=DP):
-    """Array of 5 pointers (U,V,W,R,E) to Fortran-ordered float64 arrays (None -> NULL)."""
+    """Array of 5 pointers (U,V,W,R,E) to Fortran-ordered float64 arrays (None -> NULL).
+
+    The returned object holds references to the arrays, so temporaries stay
+    alive for as long as the pointer array is in use.
+    """
+    arrs = list(arrs)
     out = (ctype * 5)()
+    out._keep = arrs
     for i, a in enumerate(arrs):
         if a is None:
             out[i] = ctype()

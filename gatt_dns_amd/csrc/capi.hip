@@ -28,18 +28,28 @@ int fail(int code, const std::string& msg) {
       return fail(GATT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// Slab layout of one member: arrays of np doubles.
+// Per-member slab: arrays of np doubles.
 enum : int {
   SL_QA = 0,    // state ping
   SL_QB = 5,    // state pong
   SL_S = 10,    // RK stage state
   SL_DL = 15,   // RK accumulator
-  SL_D = 20,    // 15 first derivatives, (3f + d)
-  SL_T = 35,    // 9 stresses / heat fluxes
-  SL_G = 44,    // 12 second derivatives
-  SL_MEAN = 56, // SFD mean
-  SL_N = 61
+  SL_MEAN = 20, // SFD mean (allocated only when SFD is on)
+  SL_N_NOSFD = 20,
+  SL_N_SFD = 25
 };
+
+// Derivative workspace of one member of a work group: arrays of np doubles.
+enum : int {
+  WK_D = 0,     // 15 first derivatives, (3f + d)
+  WK_T = 15,    // 9 stresses / heat fluxes
+  WK_G = 24,    // 12 second derivatives
+  WK_N = 36
+};
+
+// Members whose RHS is evaluated in the same launches share the workspace;
+// groups are sized so that the workspace covers about this many points.
+constexpr long long kWorkPoints = 1LL << 23;
 
 struct DirOps {
   LineOpDev op[2];           // deriv, filter
@@ -68,6 +78,9 @@ struct gatt_ctx {
   long long np = 0;
   long long slab = 0;          // doubles per member
   double* mem = nullptr;       // batch * slab
+  int wgroup = 1;              // members per work group
+  long long work_stride = 0;   // doubles per work-group member
+  double* work = nullptr;      // wgroup * work_stride
   FlowConst fc{};
   double gamma = 1.4;
   DirOps dir[3];
@@ -101,12 +114,20 @@ struct gatt_ctx {
   int* d_flag = nullptr;
   int cur = 0;                          // 0: state in QA, 1: in QB
   hipStream_t stream = nullptr;
+  // stability analysis
+  long long nflow = 0;                  // flow-region points (0: not set, every point)
+  int* d_idx = nullptr;                 // region index -> point
+  int* d_rmap = nullptr;                // point -> region index or -1
+  double* d_base = nullptr;             // base flow, 5 arrays
+  double* d_scratch = nullptr;          // block products: partial sums, H, M
+  size_t scratch_doubles = 0;
   // captured double step
   hipGraphExec_t graph = nullptr;
   double graph_dt = 0;
   int graph_cur = -1;
 
   double* slab_ptr(int idx) { return mem + (long long)idx * np; }
+  double* work_ptr(int idx) { return work + (long long)idx * np; }
   double* state(int which) { return slab_ptr(which == 0 ? SL_QA : SL_QB); }
 };
 
@@ -495,7 +516,7 @@ void run_bc(gatt_ctx* c, double* q0, double t) {
 
 // One line solve along `dir` for nf fields (member 0 pointers, member stride = slab).
 void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double* const* out, double theta,
-               int mode, int nmembers, long long member_stride) {
+               int mode, int nmembers, long long in_stride, long long out_stride) {
   DirOps& d = c->dir[dir];
   SolveArgs a{};
   a.op = d.op[kind];
@@ -509,7 +530,8 @@ void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double
   a.stride = d.stride;
   a.G = d.shape[kind].G;
   a.nf = nf;
-  a.member_stride = member_stride;
+  a.in_stride = in_stride;
+  a.out_stride = out_stride;
   for (int f = 0; f < nf; ++f) {
     a.in[f] = in[f];
     a.out[f] = out[f];
@@ -521,12 +543,14 @@ void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double
   launch_line_solve(a, d.shape[kind], nmembers, c->stream);
 }
 
-// Navier-Stokes RHS of the stage state S and one RK stage (nseq3d/nseq2d + RK4 update).
+// Navier-Stokes RHS of the stage state S and one RK stage (nseq3d/nseq2d + RK4 update),
+// for all members, one work group after the other.
 void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, double dt) {
   const long long np = c->np;
-  double* D = c->slab_ptr(SL_D);
-  double* T = c->slab_ptr(SL_T);
-  double* G = c->slab_ptr(SL_G);
+  double* D = c->work_ptr(WK_D);
+  double* T = c->work_ptr(WK_T);
+  double* G = c->work_ptr(WK_G);
+  const long long ws = c->work_stride;
   std::vector<int> flds = c->is3d ? std::vector<int>{0, 1, 2, 3, 4} : std::vector<int>{0, 1, 3, 4};
   const int ndir = c->is3d ? 3 : 2;
   if (c->forcing && c->is3d) {
@@ -539,58 +563,64 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
     sa.nblocks = c->sum_blocks;
     launch_forcing_sums(sa, c->batch, c->stream);
   }
-  for (int d = 0; d < ndir; ++d) {
-    double* in[kMaxPtr];
-    double* out[kMaxPtr];
-    int nf = 0;
-    for (int f : flds) {
-      in[nf] = S + (long long)f * np;
-      out[nf] = D + (long long)(3 * f + d) * np;
-      ++nf;
+  for (int m0 = 0; m0 < c->batch; m0 += c->wgroup) {
+    const int nm = std::min(c->wgroup, c->batch - m0);
+    const long long mo = (long long)m0 * c->slab;
+    for (int d = 0; d < ndir; ++d) {
+      double* in[kMaxPtr];
+      double* out[kMaxPtr];
+      int nf = 0;
+      for (int f : flds) {
+        in[nf] = S + mo + (long long)f * np;
+        out[nf] = D + (long long)(3 * f + d) * np;
+        ++nf;
+      }
+      run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, c->slab, ws);
     }
-    run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, c->batch, c->slab);
-  }
-  ViscArgs va{};
-  va.c = c->fc;
-  va.np = np;
-  va.member_stride = c->slab;
-  va.E = S + 4 * np;
-  va.D = D;
-  va.T = T;
-  va.is3d = c->is3d;
-  launch_visc(va, c->batch, c->stream);
-  // second derivatives: x: txx,txy,txz,qx  y: txy,tyy,tyz,qy  z: txz,tyz,tzz,qz
-  const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
-  for (int d = 0; d < ndir; ++d) {
-    double* in[kMaxPtr];
-    double* out[kMaxPtr];
-    int nf = 0;
-    for (int q = 0; q < 4; ++q) {
-      if (!c->is3d && q == 2) continue;
-      in[nf] = T + (long long)tin3[d][q] * np;
-      out[nf] = G + (long long)(4 * d + q) * np;
-      ++nf;
+    ViscArgs va{};
+    va.c = c->fc;
+    va.np = np;
+    va.member_stride = c->slab;
+    va.work_stride = ws;
+    va.E = S + mo + 4 * np;
+    va.D = D;
+    va.T = T;
+    va.is3d = c->is3d;
+    launch_visc(va, nm, c->stream);
+    // second derivatives: x: txx,txy,txz,qx  y: txy,tyy,tyz,qy  z: txz,tyz,tzz,qz
+    const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
+    for (int d = 0; d < ndir; ++d) {
+      double* in[kMaxPtr];
+      double* out[kMaxPtr];
+      int nf = 0;
+      for (int q = 0; q < 4; ++q) {
+        if (!c->is3d && q == 2) continue;
+        in[nf] = T + (long long)tin3[d][q] * np;
+        out[nf] = G + (long long)(4 * d + q) * np;
+        ++nf;
+      }
+      run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, ws, ws);
     }
-    run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, c->batch, c->slab);
+    AssembleArgs aa{};
+    aa.c = c->fc;
+    aa.np = np;
+    aa.member_stride = c->slab;
+    aa.work_stride = ws;
+    aa.S = S + mo;
+    aa.Q = Q + mo;
+    aa.Out = Out + mo;
+    aa.Delta = c->slab_ptr(SL_DL) + mo;
+    aa.D = D;
+    aa.T = T;
+    aa.G = G;
+    aa.sums = (c->forcing && c->is3d) ? c->d_sums + 8LL * m0 : nullptr;
+    aa.forcing_eps = c->f_eps;
+    aa.forcing_e0 = c->f_e0;
+    aa.stage = stage;
+    aa.dt = dt;
+    aa.is3d = c->is3d;
+    launch_assemble(aa, nm, c->stream);
   }
-  AssembleArgs aa{};
-  aa.c = c->fc;
-  aa.np = np;
-  aa.member_stride = c->slab;
-  aa.S = S;
-  aa.Q = Q;
-  aa.Out = Out;
-  aa.Delta = c->slab_ptr(SL_DL);
-  aa.D = D;
-  aa.T = T;
-  aa.G = G;
-  aa.sums = (c->forcing && c->is3d) ? c->d_sums : nullptr;
-  aa.forcing_eps = c->f_eps;
-  aa.forcing_e0 = c->f_e0;
-  aa.stage = stage;
-  aa.dt = dt;
-  aa.is3d = c->is3d;
-  launch_assemble(aa, c->batch, c->stream);
 }
 
 // RK4_3D / RK4_2D: state `from` -> `to` (timeIntegration.F90:382-496)
@@ -614,7 +644,7 @@ void run_filters(gatt_ctx* c, double* q, double theta) {
     double* ptr[kMaxPtr];
     int nf = 0;
     for (int f : flds) ptr[nf++] = q + (long long)f * c->np;
-    run_solve(c, d, GATT_FILTER, nf, ptr, ptr, theta, 1, c->batch, c->slab);
+    run_solve(c, d, GATT_FILTER, nf, ptr, ptr, theta, 1, c->batch, c->slab, c->slab);
   }
 }
 
@@ -693,7 +723,9 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
   c->is3d = d->nz > 1;
   c->batch = batch;
   c->np = (long long)d->nx * d->ny * d->nz;
-  c->slab = (long long)SL_N * c->np;
+  c->slab = (long long)(d->sfdType > 0 ? SL_N_SFD : SL_N_NOSFD) * c->np;
+  c->wgroup = (int)std::max(1LL, std::min<long long>(batch, kWorkPoints / c->np));
+  c->work_stride = (long long)WK_N * c->np;
   c->gamma = d->gamma;
   c->fc.gm1 = d->gamma - 1.0;
   c->fc.invRe = 1.0 / d->Re;
@@ -781,6 +813,11 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
   if (e != hipSuccess) return fail(GATT_E_NOMEM, "state allocation failed");
   c->mem = static_cast<double*>(p);
   HIPCHK(hipMemset(c->mem, 0, sizeof(double) * (size_t)c->slab * batch));
+  e = hipMalloc(&p, sizeof(double) * (size_t)c->work_stride * c->wgroup);
+  if (e != hipSuccess) return fail(GATT_E_NOMEM, "workspace allocation failed");
+  c->dev_allocs.push_back(p);
+  c->work = static_cast<double*>(p);
+  HIPCHK(hipMemset(c->work, 0, sizeof(double) * (size_t)c->work_stride * c->wgroup));
   *out = c.release();
   return GATT_OK;
 }
@@ -790,6 +827,10 @@ int gatt_destroy(gatt_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->graph) hipGraphExecDestroy(c->graph);
   for (void* p : c->dev_allocs) hipFree(p);
+  if (c->d_idx) hipFree(c->d_idx);
+  if (c->d_rmap) hipFree(c->d_rmap);
+  if (c->d_base) hipFree(c->d_base);
+  if (c->d_scratch) hipFree(c->d_scratch);
   if (c->mem) hipFree(c->mem);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -825,6 +866,7 @@ int gatt_get_state(gatt_ctx* c, int32_t member, double* const* q) {
 
 int gatt_set_mean(gatt_ctx* c, int32_t member, const double* const* q) {
   if (check_ctx(c) || !q) return fail(GATT_E_INVALID, "null argument");
+  if (c->sfdType <= 0) return fail(GATT_E_INVALID, "SFD is off: no mean to set");
   int rc = copy_fields(c, member, c->slab_ptr(SL_MEAN), q, true);
   if (!rc) c->mean_ready = true;
   return rc;
@@ -832,6 +874,7 @@ int gatt_set_mean(gatt_ctx* c, int32_t member, const double* const* q) {
 
 int gatt_get_mean(gatt_ctx* c, int32_t member, double* const* q) {
   if (check_ctx(c) || !q) return fail(GATT_E_INVALID, "null argument");
+  if (c->sfdType <= 0) return fail(GATT_E_INVALID, "SFD is off: no mean to read");
   return copy_fields(c, member, c->slab_ptr(SL_MEAN), (const double* const*)q, false);
 }
 
@@ -959,7 +1002,7 @@ int gatt_line_solve(gatt_ctx* c, int32_t dir, int32_t kind, int32_t nfields, con
     ip[f] = const_cast<double*>(in[f]);
     op[f] = out[f];
   }
-  run_solve(c, dir, kind, nfields, ip, op, 0.0, 0, 1, 0);
+  run_solve(c, dir, kind, nfields, ip, op, 0.0, 0, 1, 0, 0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return GATT_OK;
@@ -1022,9 +1065,12 @@ int gatt_time_kernel(gatt_ctx* c, int32_t kernel, int32_t reps, double* avg_ms, 
   const int ndir = c->is3d ? 3 : 2;
   std::vector<int> flds = c->is3d ? std::vector<int>{0, 1, 2, 3, 4} : std::vector<int>{0, 1, 3, 4};
   double* S = c->state(c->cur);
-  double* D = c->slab_ptr(SL_D);
-  double* T = c->slab_ptr(SL_T);
-  double* G = c->slab_ptr(SL_G);
+  double* D = c->work_ptr(WK_D);
+  double* T = c->work_ptr(WK_T);
+  double* G = c->work_ptr(WK_G);
+  const long long ws = c->work_stride;
+  const int nm = c->wgroup;  // the first work group
+  int members = nm;
   int launches = 0;
   double arrays = 0;  // arrays touched per rep
   auto body = [&]() {
@@ -1034,7 +1080,7 @@ int gatt_time_kernel(gatt_ctx* c, int32_t kernel, int32_t reps, double* avg_ms, 
       double* out[kMaxPtr];
       int nf = 0;
       for (int f : flds) { in[nf] = S + (long long)f * np; out[nf] = D + (long long)(3 * f + d) * np; ++nf; }
-      run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, c->batch, c->slab);
+      run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, c->slab, ws);
       const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
       int nf2 = 0;
       for (int q = 0; q < 4; ++q) {
@@ -1043,7 +1089,7 @@ int gatt_time_kernel(gatt_ctx* c, int32_t kernel, int32_t reps, double* avg_ms, 
         out[nf2] = G + (long long)(4 * d + q) * np;
         ++nf2;
       }
-      run_solve(c, d, GATT_DERIV, nf2, in, out, 0.0, 0, c->batch, c->slab);
+      run_solve(c, d, GATT_DERIV, nf2, in, out, 0.0, 0, nm, ws, ws);
       launches = 2;
       arrays = 2.0 * (nf + nf2);
     } else if (kernel >= GATT_K_FILTER_X && kernel <= GATT_K_FILTER_Z) {
@@ -1051,22 +1097,23 @@ int gatt_time_kernel(gatt_ctx* c, int32_t kernel, int32_t reps, double* avg_ms, 
       double* ptr[kMaxPtr];
       int nf = 0;
       for (int f : flds) ptr[nf++] = S + (long long)f * np;
-      run_solve(c, d, GATT_FILTER, nf, ptr, ptr, -1.0, 1, c->batch, c->slab);
+      run_solve(c, d, GATT_FILTER, nf, ptr, ptr, -1.0, 1, c->batch, c->slab, c->slab);
+      members = c->batch;
       launches = 1;
       arrays = 2.0 * nf;
     } else if (kernel == GATT_K_VISC) {
       ViscArgs va{};
-      va.c = c->fc; va.np = np; va.member_stride = c->slab; va.E = S + 4 * np; va.D = D; va.T = T;
-      va.is3d = c->is3d;
-      launch_visc(va, c->batch, c->stream);
+      va.c = c->fc; va.np = np; va.member_stride = c->slab; va.work_stride = ws; va.E = S + 4 * np;
+      va.D = D; va.T = T; va.is3d = c->is3d;
+      launch_visc(va, nm, c->stream);
       launches = 1;
       arrays = c->is3d ? 13 + 9 : 7 + 5;
     } else {
       AssembleArgs aa{};
-      aa.c = c->fc; aa.np = np; aa.member_stride = c->slab; aa.S = c->slab_ptr(SL_S); aa.Q = S;
+      aa.c = c->fc; aa.np = np; aa.member_stride = c->slab; aa.work_stride = ws; aa.S = c->slab_ptr(SL_S); aa.Q = S;
       aa.Out = c->slab_ptr(SL_S); aa.Delta = c->slab_ptr(SL_DL); aa.D = D; aa.T = T; aa.G = G;
       aa.sums = nullptr; aa.stage = 2; aa.dt = 0.0; aa.is3d = c->is3d;
-      launch_assemble(aa, c->batch, c->stream);
+      launch_assemble(aa, nm, c->stream);
       launches = 1;
       // reads S(5) D(15) T(6) G(12) Q(5) Delta(5), writes Delta(5) Out(5); 2-D: 4, 8, 3, 6, 4, 4, 4, 4
       arrays = c->is3d ? 58 : 37;
@@ -1089,7 +1136,211 @@ int gatt_time_kernel(gatt_ctx* c, int32_t kernel, int32_t reps, double* avg_ms, 
   hipEventDestroy(e1);
   HIPCHK(hipGetLastError());
   if (avg_ms) *avg_ms = (double)ms / (reps * launches);
-  if (bytes) *bytes = arrays / launches * 8.0 * (double)np * c->batch;
+  if (bytes) *bytes = arrays / launches * 8.0 * (double)np * members;
+  return GATT_OK;
+}
+
+
+// ------------------------------------------------------------ stability analysis
+
+static long long flow_points(const gatt_ctx* c) { return c->nflow > 0 ? c->nflow : c->np; }
+static int slots(const gatt_ctx* c) { return c->is3d ? 5 : 4; }
+
+static int ensure_scratch(gatt_ctx* c, size_t doubles) {
+  if (doubles <= c->scratch_doubles) return GATT_OK;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->d_scratch) HIPCHK(hipFree(c->d_scratch));
+  c->d_scratch = nullptr;
+  c->scratch_doubles = 0;
+  void* p = nullptr;
+  if (hipMalloc(&p, sizeof(double) * doubles) != hipSuccess) return fail(GATT_E_NOMEM, "scratch allocation failed");
+  c->d_scratch = static_cast<double*>(p);
+  c->scratch_doubles = doubles;
+  return GATT_OK;
+}
+
+static bool ranges_overlap(const double* a, long long na, const double* b, long long nb) {
+  return a < b + nb && b < a + na;
+}
+
+int gatt_set_flow_region(gatt_ctx* c, int64_t count, const int64_t* idx) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->d_idx) HIPCHK(hipFree(c->d_idx));
+  if (c->d_rmap) HIPCHK(hipFree(c->d_rmap));
+  c->d_idx = nullptr;
+  c->d_rmap = nullptr;
+  c->nflow = 0;
+  if (!idx || count == c->np) {
+    if (idx)
+      for (int64_t r = 0; r < count; ++r)
+        if (idx[r] != r) goto general;
+    return GATT_OK;
+  }
+general:
+  if (count < 1 || count > c->np) return fail(GATT_E_INVALID, "flow region size out of range");
+  if (c->np > 0x7fffffffLL) return fail(GATT_E_UNSUPPORTED, "flow regions need fewer than 2^31 points");
+  {
+    std::vector<int> fwd((size_t)count), inv((size_t)c->np, -1);
+    for (int64_t r = 0; r < count; ++r) {
+      if (idx[r] < 0 || idx[r] >= c->np) return fail(GATT_E_INVALID, "flow region index outside the grid");
+      if (r > 0 && idx[r] <= idx[r - 1]) return fail(GATT_E_INVALID, "flow region indices must increase");
+      fwd[(size_t)r] = (int)idx[r];
+      inv[(size_t)idx[r]] = (int)r;
+    }
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, sizeof(int) * fwd.size()));
+    c->d_idx = static_cast<int*>(p);
+    HIPCHK(hipMalloc(&p, sizeof(int) * inv.size()));
+    c->d_rmap = static_cast<int*>(p);
+    HIPCHK(hipMemcpy(c->d_idx, fwd.data(), sizeof(int) * fwd.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_rmap, inv.data(), sizeof(int) * inv.size(), hipMemcpyHostToDevice));
+  }
+  c->nflow = count;
+  return GATT_OK;
+}
+
+int gatt_vector_length(gatt_ctx* c, int64_t* n) {
+  if (check_ctx(c) || !n) return fail(GATT_E_INVALID, "null argument");
+  *n = (int64_t)slots(c) * flow_points(c);
+  return GATT_OK;
+}
+
+int gatt_set_base(gatt_ctx* c, const double* const* q) {
+  if (check_ctx(c) || !q) return fail(GATT_E_INVALID, "null argument");
+  if (!c->d_base) {
+    void* p = nullptr;
+    if (hipMalloc(&p, sizeof(double) * 5 * (size_t)c->np) != hipSuccess)
+      return fail(GATT_E_NOMEM, "base flow allocation failed");
+    c->d_base = static_cast<double*>(p);
+    HIPCHK(hipMemsetAsync(c->d_base, 0, sizeof(double) * 5 * (size_t)c->np, c->stream));
+  }
+  for (int f = 0; f < 5; ++f) {
+    if (!c->is3d && f == GATT_W) continue;
+    if (!q[f]) return fail(GATT_E_INVALID, "null field pointer");
+    HIPCHK(hipMemcpyAsync(c->d_base + (long long)f * c->np, q[f], sizeof(double) * c->np, hipMemcpyHostToDevice,
+                          c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_perturb(gatt_ctx* c, int32_t first, int32_t count, const double* V, int64_t ldv, double scale) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  if (!c->d_base) return fail(GATT_E_INVALID, "no base flow set");
+  if (first < 0 || count < 1 || first + count > c->batch) return fail(GATT_E_INVALID, "member range out of bounds");
+  const long long nvec = (long long)slots(c) * flow_points(c);
+  if (V && ldv < nvec) return fail(GATT_E_INVALID, "ldv smaller than the vector length");
+  PerturbArgs a{};
+  a.np = c->np;
+  a.member_stride = c->slab;
+  a.nflow = flow_points(c);
+  a.ldv = ldv;
+  a.state = c->state(c->cur) + (long long)first * c->slab;
+  a.mean = c->sfdType > 0 ? c->slab_ptr(SL_MEAN) + (long long)first * c->slab : nullptr;
+  a.base = c->d_base;
+  a.rmap = c->d_rmap;
+  a.V = V;
+  a.scale = scale;
+  a.is3d = c->is3d;
+  launch_perturb(a, count, c->stream);
+  if (c->sfdType > 0) c->mean_ready = true;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_collect(gatt_ctx* c, int32_t first, int32_t count, int32_t minus_first, const double* ref, double scale,
+                 double* W, int64_t ldw) {
+  if (check_ctx(c) || !W) return fail(GATT_E_INVALID, "null argument");
+  if (first < 0 || count < 1 || first + count > c->batch) return fail(GATT_E_INVALID, "member range out of bounds");
+  if (minus_first >= 0 && minus_first + count > c->batch) return fail(GATT_E_INVALID, "subtracted members out of bounds");
+  const long long nvec = (long long)slots(c) * flow_points(c);
+  if (ldw < nvec) return fail(GATT_E_INVALID, "ldw smaller than the vector length");
+  CollectArgs a{};
+  a.np = c->np;
+  a.member_stride = c->slab;
+  a.nflow = flow_points(c);
+  a.ldw = ldw;
+  a.plus = c->state(c->cur) + (long long)first * c->slab;
+  a.minus = minus_first >= 0 ? c->state(c->cur) + (long long)minus_first * c->slab : nullptr;
+  a.ref = minus_first >= 0 ? nullptr : ref;
+  a.idx = c->d_idx;
+  a.W = W;
+  a.scale = scale;
+  a.is3d = c->is3d;
+  launch_collect(a, count, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_block_dot(gatt_ctx* c, int64_t n, int32_t k, int32_t p, const double* X, int64_t ldx, const double* Y,
+                   int64_t ldy, double* H) {
+  if (check_ctx(c) || !X || !Y || !H) return fail(GATT_E_INVALID, "null argument");
+  if (n < 1 || k < 1 || p < 1 || ldx < n || ldy < n) return fail(GATT_E_INVALID, "bad block dimensions");
+  const int nch = block_dot_chunks(n);
+  const size_t kp = (size_t)k * p;
+  int rc = ensure_scratch(c, kp * (nch + 1));
+  if (rc) return rc;
+  BlockDotArgs a{};
+  a.n = n;
+  a.ldx = ldx;
+  a.ldy = ldy;
+  a.k = k;
+  a.p = p;
+  a.X = X;
+  a.Y = Y;
+  a.partial = c->d_scratch + kp;
+  launch_block_dot(a, c->d_scratch, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(H, c->d_scratch, sizeof(double) * kp, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_block_update(gatt_ctx* c, int64_t n, int32_t k, int32_t p, const double* X, int64_t ldx, const double* M,
+                      double alpha, double beta, double* Y, int64_t ldy) {
+  if (check_ctx(c) || !X || !M || !Y) return fail(GATT_E_INVALID, "null argument");
+  if (n < 1 || k < 1 || p < 1 || ldx < n || ldy < n) return fail(GATT_E_INVALID, "bad block dimensions");
+  if (ranges_overlap(X, (long long)(k - 1) * ldx + n, Y, (long long)(p - 1) * ldy + n))
+    return fail(GATT_E_INVALID, "X and Y overlap");
+  // rows of M in chunks of 256 so that a chunk of 32 columns fits 64 KiB of LDS;
+  // chunk 0 applies alpha, later chunks accumulate
+  constexpr int kChunk = 256;
+  const size_t kp = (size_t)k * p;
+  const int rc = ensure_scratch(c, k > kChunk ? (size_t)kChunk * p : kp);
+  if (rc) return rc;
+  for (int i0 = 0; i0 < k; i0 += kChunk) {
+    const int kc = std::min(kChunk, k - i0);
+    if (kc == k) {
+      HIPCHK(hipMemcpyAsync(c->d_scratch, M, sizeof(double) * kp, hipMemcpyHostToDevice, c->stream));
+    } else {
+      // the chunk's rows of M as a kc x p column-major block
+      std::vector<double> sub((size_t)kc * p);
+      for (int j = 0; j < p; ++j)
+        for (int i = 0; i < kc; ++i) sub[(size_t)j * kc + i] = M[(size_t)j * k + i0 + i];
+      HIPCHK(hipMemcpyAsync(c->d_scratch, sub.data(), sizeof(double) * sub.size(), hipMemcpyHostToDevice,
+                            c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    BlockCombineArgs a{};
+    a.n = n;
+    a.ldx = ldx;
+    a.ldy = ldy;
+    a.k = kc;
+    a.p = p;
+    a.X = X + (long long)i0 * ldx;
+    a.M = c->d_scratch;
+    a.alpha = i0 == 0 ? alpha : 1.0;
+    a.beta = beta;
+    a.Y = Y;
+    launch_block_combine(a, c->stream);
+    // the next chunk overwrites the staged M
+    if (i0 + kChunk < k) HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
   return GATT_OK;
 }
 

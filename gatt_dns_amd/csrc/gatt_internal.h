@@ -49,7 +49,8 @@ struct SolveArgs {
   float inv_n;                // 1/N
   int lds_coef_offset;        // wave-segment: LDS word offset of the per-wavefront coefficient slots
   int nf;                     // fields per member
-  long long member_stride;    // doubles between members
+  long long in_stride;        // doubles between members in the input arrays
+  long long out_stride;       // doubles between members in the output arrays
   const double* in[kMaxPtr];
   double* out[kMaxPtr];
   const double* coef;         // packed per-point coefficients [type][point][2nR-1+4] (wave-segment)
@@ -112,7 +113,8 @@ struct FlowConst {
 
 struct ViscArgs {
   FlowConst c;
-  long long np, member_stride;
+  long long np, member_stride;  // member stride of E
+  long long work_stride;        // member stride of D and T
   const double* E;
   const double* D;   // 15 derivative arrays, D + (3f+d)*np
   double* T;         // 9 arrays: txx tyy tzz txy txz tyz qx qy qz
@@ -122,7 +124,8 @@ void launch_visc(const ViscArgs& a, int nmembers, hipStream_t st);
 
 struct AssembleArgs {
   FlowConst c;
-  long long np, member_stride;
+  long long np, member_stride;  // member stride of S, Q, Out, Delta
+  long long work_stride;        // member stride of D, T, G
   const double* S;      // stage state (RHS evaluated here), 5 arrays
   const double* Q;      // base state q^n, 5 arrays
   double* Out;          // stage output (may alias S), 5 arrays
@@ -169,6 +172,51 @@ void launch_clear_u64(unsigned long long* p, int n, hipStream_t st);
 void launch_step_reduce(const ReduceArgs& a, int nmembers, hipStream_t st);
 void launch_copy(double* dst, const double* src, long long n, hipStream_t st);
 void launch_record(const unsigned long long* red, double* rec, int* flag, int nmembers, hipStream_t st);
+
+// ------------------------------------------------------- stability analysis
+struct PerturbArgs {
+  long long np, member_stride, nflow, ldv;
+  double* state;        // state of the first member, 5 arrays
+  double* mean;         // SFD mean of the first member or nullptr
+  const double* base;   // base flow, 5 arrays
+  const int* rmap;      // point -> flow-region index or -1 (nullptr: every point, identity)
+  const double* V;      // Krylov vectors (column b for member b) or nullptr (state = base)
+  double scale;
+  int is3d;
+};
+void launch_perturb(const PerturbArgs& a, int count, hipStream_t st);
+
+struct CollectArgs {
+  long long np, member_stride, nflow, ldw;
+  const double* plus;   // state of the first member
+  const double* minus;  // state of the first subtracted member or nullptr
+  const double* ref;    // subtracted vector (when minus is nullptr) or nullptr
+  const int* idx;       // flow-region index -> point (nullptr: identity)
+  double* W;
+  double scale;
+  int is3d;
+};
+void launch_collect(const CollectArgs& a, int count, hipStream_t st);
+
+struct BlockDotArgs {
+  long long n, ldx, ldy, rows_per;
+  int k, p;
+  const double* X;
+  const double* Y;
+  double* partial;      // block_dot_chunks(n) x k x p
+};
+int block_dot_chunks(long long n);
+void launch_block_dot(BlockDotArgs a, double* H, hipStream_t st);
+
+struct BlockCombineArgs {
+  long long n, ldx, ldy;
+  int k, p;
+  const double* X;
+  const double* M;      // k x p, column-major, device
+  double alpha, beta;
+  double* Y;
+};
+void launch_block_combine(const BlockCombineArgs& a, hipStream_t st);
 
 // --------------------------------------------------------------- boundaries
 enum BcKind : int {

@@ -59,8 +59,8 @@ __global__ __launch_bounds__(256) void line_solve_kernel(SolveArgs a) {
   const int ntile = min(TL, a.nlines - line0);
   const int f = (int)blockIdx.y % a.nf;
   const int mem = (int)blockIdx.y / a.nf;
-  const double* in = a.in[f] + (long long)mem * a.member_stride;
-  double* out = a.out[f] + (long long)mem * a.member_stride;
+  const double* in = a.in[f] + (long long)mem * a.in_stride;
+  double* out = a.out[f] + (long long)mem * a.out_stride;
   const long long stride = DIR == 0 ? 1 : a.stride;
 
   // ---- HBM -> LDS, unit stride

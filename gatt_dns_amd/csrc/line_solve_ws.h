@@ -159,8 +159,8 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024) void line_solve_ws_kernel(Sol
   const bool active = lane < nact;
   const int f = (int)blockIdx.y % a.nf;
   const int mem = (int)blockIdx.y / a.nf;
-  const double* in = a.in[f] + (long long)mem * a.member_stride;
-  double* out = a.out[f] + (long long)mem * a.member_stride;
+  const double* in = a.in[f] + (long long)mem * a.in_stride;
+  double* out = a.out[f] + (long long)mem * a.out_stride;
   const long long stride = a.stride;
   const int s = w * M;
   int cnt = N - s;

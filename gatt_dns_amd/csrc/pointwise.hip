@@ -16,9 +16,10 @@ __global__ __launch_bounds__(256) void visc_kernel(ViscArgs a) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= a.np) return;
   const long long mo = (long long)blockIdx.y * a.member_stride;
+  const long long wo = (long long)blockIdx.y * a.work_stride;
   const long long np = a.np;
-  const double* D = a.D + mo;
-  double* T = a.T + mo;
+  const double* D = a.D + wo;
+  double* T = a.T + wo;
   const double E = a.E[mo + n];
   const FlowConst c = a.c;
   const double mu1 = c.muA / (c.muC * E + c.muB);
@@ -63,10 +64,11 @@ __global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs a) {
   if (n >= a.np) return;
   const long long mo = (long long)blockIdx.y * a.member_stride;
   const long long np = a.np;
+  const long long wo = (long long)blockIdx.y * a.work_stride;
   const double* S = a.S + mo;
-  const double* D = a.D + mo;
-  const double* T = a.T + mo;
-  const double* G = a.G + mo;
+  const double* D = a.D + wo;
+  const double* T = a.T + wo;
+  const double* G = a.G + wo;
   const double U = S[0 * np + n], V = S[1 * np + n], R = S[3 * np + n], E = S[4 * np + n];
   const FlowConst c = a.c;
   const double DUx = D[0 * np + n], DUy = D[1 * np + n];

@@ -68,6 +68,7 @@ class Case:
     sfd_x: np.ndarray = None
     dxmin: tuple = (0.0, 0.0, 0.0)
     dt: float = 0.0
+    flow_region: np.ndarray = None   # True at flow points (False inside walls)
     state0: dict = None
 
     @property
@@ -144,6 +145,7 @@ def lid_driven(n=129, Re=1000.0, Ma=0.5, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0
     case = Case("lid_driven_%dx%d" % (nx, ny), nx, ny, 1, X, Y, Z, rnd(Re), rnd(Ma), rnd(Pr), rnd(gamma),
                 rnd(T0), deriv, filt, types, (1, 1, 1), -1.0, bi, [dist], [],
                 dxmin=_dxmin(X, Y, Z, 1))
+    case.flow_region = fr
     st = {k: np.zeros((nx, ny, 1), order="F") for k in FIELDS}
     st["R"][:] = 1.0
     st["E"][:] = E0
@@ -197,8 +199,42 @@ def turbulence_box(n=256, Re=1000.0, Ma=0.3, Pr=0.71, gamma=1.4, T0=300.0, alpha
     case = Case("turbulence_box_%d" % n, n, n, n, X, X.copy(), X.copy(), rnd(Re), rnd(Ma), rnd(Pr),
                 rnd(gamma), rnd(T0), deriv, filt, types, (1, 1, 1), -1.0, bi, [], forc,
                 dxmin=_dxmin(X, X, X, n))
+    case.flow_region = fr
     u, v, w = solenoidal_field(n, seed, urms=urms)
     st = {"U": u, "V": v, "W": w, "R": np.ones((n, n, n), order="F"), "E": np.full((n, n, n), E0, order="F")}
     case.state0 = st
     case.dt = rnd(cfl / cfl_dt(st, case.Ma, case.dxmin, n))
+    return case
+
+
+def periodic_rest(nx, ny, nz=1, Re=1000.0, Ma=0.3, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0.49,
+                  spatial="SL4", cfl=0.5):
+    """Fluid at rest in a periodic box (periodicBox.m) on uniform meshes, no forcing.
+
+    The linearised step about this state is diagonal in Fourier modes, so the
+    spectrum of the time-T propagator is known in closed form; used to check
+    the stability analysis.
+    """
+    meshes = []
+    for n in (nx, ny, nz):
+        X = np.arange(n) / n if n > 1 else np.array([0.0])
+        meshes.append(DirMesh(X, periodic=True, fix_periodic_domain_size=True, xf=1.0))
+    X, Y, Z = (m.X for m in meshes)
+    E0 = 1.0 / ((gamma ** 2 - gamma) * Ma ** 2)
+    fr, walls, corners, bc = periodic_box(nx, ny, nz)
+    nm = NumMethods(spatial_derivs=spatial, spatial_derivs_buffer="EX4", filter_strength=alpha_f,
+                    filter_borders="reducedOrder")
+    deriv, filt, types = _coeffs(meshes, fr, nm)
+    bi = init_boundaries(bc, corners, nm.neumann_order, nm.neumann2_order)
+    rnd = lambda v: float(fortran_round(v))
+    case = Case("periodic_rest_%dx%dx%d" % (nx, ny, nz), nx, ny, nz, X, Y, Z, rnd(Re), rnd(Ma), rnd(Pr),
+                rnd(gamma), rnd(T0), deriv, filt, types, (1, 1, 1), -1.0, bi, [], [],
+                dxmin=_dxmin(X, Y, Z, nz))
+    case.flow_region = fr
+    shape = (nx, ny, nz)
+    st = {k: np.zeros(shape, order="F") for k in FIELDS}
+    st["R"][:] = 1.0
+    st["E"][:] = E0
+    case.state0 = st
+    case.dt = rnd(cfl / cfl_dt(st, case.Ma, case.dxmin, nz))
     return case

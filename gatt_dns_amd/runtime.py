@@ -23,7 +23,8 @@ EXPORTS = ["gatt_last_error", "gatt_device_info", "gatt_create", "gatt_destroy",
            "gatt_get_state", "gatt_set_mean", "gatt_get_mean", "gatt_rk4", "gatt_apply_filters",
            "gatt_apply_bound_cond", "gatt_sfd", "gatt_rhs", "gatt_step", "gatt_advance",
            "gatt_line_solve", "gatt_sync", "gatt_time_kernel", "gatt_set_line_shape",
-           "gatt_get_line_shape"]
+           "gatt_get_line_shape", "gatt_set_flow_region", "gatt_vector_length", "gatt_set_base",
+           "gatt_perturb", "gatt_collect", "gatt_block_dot", "gatt_block_update"]
 
 
 class GattError(RuntimeError):
@@ -62,6 +63,14 @@ def load_library(path=LIB_PATH):
     L.gatt_set_line_shape.argtypes = [V, I, I, I, I, I]
     IPt = C.POINTER(I)
     L.gatt_get_line_shape.argtypes = [V, I, I, IPt, IPt, IPt, IPt]
+    I64 = C.c_int64
+    L.gatt_set_flow_region.argtypes = [V, I64, C.POINTER(I64)]
+    L.gatt_vector_length.argtypes = [V, C.POINTER(I64)]
+    L.gatt_set_base.argtypes = [V, C.POINTER(DP)]
+    L.gatt_perturb.argtypes = [V, I, I, V, I64, C.c_double]
+    L.gatt_collect.argtypes = [V, I, I, I, V, C.c_double, V, I64]
+    L.gatt_block_dot.argtypes = [V, I64, I, I, V, I64, V, I64, DP]
+    L.gatt_block_update.argtypes = [V, I64, I, I, V, I64, DP, C.c_double, C.c_double, V, I64]
     _lib = L
     return L
 
@@ -175,6 +184,73 @@ class Solver:
         _check(self.lib.gatt_get_line_shape(self.ctx, direction, kind, C.byref(v), C.byref(g), C.byref(m),
                                             C.byref(w)))
         return v.value, g.value, m.value, w.value
+
+    # -- stability analysis (device blocks are torch float64 tensors of shape
+    #    (columns, n), one Krylov vector per row) -------------------------------
+    def set_flow_region(self, idx=None):
+        """0-based column-major indices of the flow-region points (None: every point)."""
+        if idx is None:
+            _check(self.lib.gatt_set_flow_region(self.ctx, 0, None))
+            return
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        _check(self.lib.gatt_set_flow_region(self.ctx, len(idx), idx.ctypes.data_as(C.POINTER(C.c_int64))))
+
+    def vector_length(self):
+        n = C.c_int64(0)
+        _check(self.lib.gatt_vector_length(self.ctx, C.byref(n)))
+        return n.value
+
+    def set_base(self, q):
+        _check(self.lib.gatt_set_base(self.ctx, field_ptrs(self._arrays(q))))
+
+    @staticmethod
+    def _block(t):
+        if t is None:
+            return None, 0
+        if str(t.dtype) != "torch.float64" or not t.is_cuda or t.stride(-1) != 1:
+            raise GattError("device blocks must be float64 CUDA tensors with unit stride along a vector")
+        return C.c_void_p(t.data_ptr()), (t.stride(0) if t.dim() == 2 else t.shape[-1])
+
+    def _need(self, t, cols, what):
+        if t is not None and (t.dim() != 2 or t.shape[0] < cols or t.shape[1] < self.vector_length()):
+            raise GattError("%s must hold %d vectors of length %d" % (what, cols, self.vector_length()))
+
+    def perturb(self, first, count, V, scale):
+        """Members first..first+count-1 = base + scale * vec2flow(V[b]) (V None: base)."""
+        self._need(V, count, "V")
+        ptr, ld = self._block(V)
+        _check(self.lib.gatt_perturb(self.ctx, first, count, ptr, ld, scale))
+
+    def collect(self, first, count, W, scale, minus_first=-1, ref=None):
+        """W[b] = scale * (flow2vec(member first+b) - flow2vec(member minus_first+b) or ref)."""
+        self._need(W, count, "W")
+        self._need(ref, 1, "ref")
+        wp, ld = self._block(W)
+        rp, _ = self._block(ref)
+        _check(self.lib.gatt_collect(self.ctx, first, count, minus_first, rp, scale, wp, ld))
+
+    def block_dot(self, X, Y):
+        """H = X^T Y (k x p numpy array) for blocks X (k, n) and Y (p, n)."""
+        xp, ldx = self._block(X)
+        yp, ldy = self._block(Y)
+        k, p, n = X.shape[0], Y.shape[0], X.shape[1]
+        if X.dim() != 2 or Y.dim() != 2 or Y.shape[1] != n:
+            raise GattError("X and Y must be (columns, n) blocks of the same n")
+        H = np.zeros((k, p), order="F")
+        _check(self.lib.gatt_block_dot(self.ctx, n, k, p, xp, ldx, yp, ldy, H.ctypes.data_as(DP)))
+        return H
+
+    def block_update(self, X, M, alpha, beta, Y):
+        """Y = alpha Y + beta X M for blocks X (k, n), Y (p, n) and M (k x p)."""
+        xp, ldx = self._block(X)
+        yp, ldy = self._block(Y)
+        M = np.asfortranarray(M, dtype=np.float64)
+        k, p, n = X.shape[0], Y.shape[0], X.shape[1]
+        if X.dim() != 2 or Y.dim() != 2 or Y.shape[1] != n:
+            raise GattError("X and Y must be (columns, n) blocks of the same n")
+        if M.shape != (k, p):
+            raise GattError("M must be %d x %d" % (k, p))
+        _check(self.lib.gatt_block_update(self.ctx, n, k, p, xp, ldx, M.ctypes.data_as(DP), alpha, beta, yp, ldy))
 
     def line_solve(self, direction, kind, fields_in, fields_out):
         """Device-pointer unit entry point: lists of torch CUDA tensors (float64, Fortran-ordered data)."""

@@ -234,6 +234,60 @@ enum {
  * (measurement only). */
 int gatt_time_kernel(gatt_ctx* ctx, int32_t kernel, int32_t reps, double* avg_ms, double* bytes);
 
+/* ------------------------------------------------------------------------
+ * Jacobian-free stability analysis on the members of a context
+ * (calcInstability.m).  The reference runs one DNS process per Krylov vector;
+ * here a block of vectors becomes a block of members, advanced together by
+ * gatt_advance.  Krylov vectors are device arrays in the flow2vec layout
+ * (calcInstability.m:480-515): the flow-region points of U,V,R,E (2-D) or
+ * U,V,W,R,E (3-D), one field after the other, region points in increasing
+ * column-major order.  V, W, X, Y below are device pointers to column-major
+ * blocks of such vectors; H and M are host arrays.
+ * ------------------------------------------------------------------------ */
+
+/* Flow region (calcInstability.m:276-281, flowRegion = ~isnan(baseflow.U)
+ * minus optional buffer zones): `count` strictly increasing 0-based
+ * column-major point indices.  idx == NULL selects every point (the default). */
+int gatt_set_flow_region(gatt_ctx* ctx, int64_t count, const int64_t* idx);
+
+/* Krylov vector length N (calcInstability.m:284-290): 4 or 5 times the
+ * number of flow-region points. */
+int gatt_vector_length(gatt_ctx* ctx, int64_t* n);
+
+/* Base flow q0 (calcInstability.m:313-330, baseflow.U ... baseflow.E), host
+ * arrays as gatt_set_state.  It is also the frozen SFD mean of perturbed
+ * members (buffer-zone damping with Delta = inf). */
+int gatt_set_base(gatt_ctx* ctx, const double* const* q);
+
+/* Members first..first+count-1 become base + scale * vec2flow(V[:, b])
+ * (calcInstability.m:345-350 with scale = epsilon*sqrt(N); vec2flow
+ * 517-582).  V == NULL sets the members to the base flow.  Points outside the
+ * flow region take the base value.  With SFD on, each member's mean is reset
+ * to the base flow. */
+int gatt_perturb(gatt_ctx* ctx, int32_t first, int32_t count, const double* V, int64_t ldv, double scale);
+
+/* W[:, b] = scale * (flow2vec(member first+b) - S_b) for b < count, with
+ * S_b = flow2vec(member minus_first+b) if minus_first >= 0 (second-order
+ * differences, calcInstability.m:370-385), else the vector `ref` (device,
+ * first order: UM = flow2vec(Phi(q0)), 385-388), else 0. */
+int gatt_collect(gatt_ctx* ctx, int32_t first, int32_t count, int32_t minus_first, const double* ref, double scale,
+                 double* W, int64_t ldw);
+
+/* H (k x p, column-major, host) = X^T Y for X: n x k and Y: n x p device
+ * blocks -- the projections H(j,k) = zeta(:,j)'*B of the Gram-Schmidt loop
+ * (calcInstability.m:389-393), a whole block at once.  Sums are formed in a
+ * fixed order, so results are reproducible. */
+int gatt_block_dot(gatt_ctx* ctx, int64_t n, int32_t k, int32_t p, const double* X, int64_t ldx, const double* Y,
+                   int64_t ldy, double* H);
+
+/* Y = alpha*Y + beta*X*M for X: n x k, M: k x p (host), Y: n x p -- the
+ * update uPrime = uPrime - H(j,k)*zeta(:,j) (calcInstability.m:391-392), the
+ * normalisation zeta(:,k+1) = uPrime/H(k+1,k) (395-397) and the modes
+ * psiA = zeta*psiH (437-440), a whole block at once.  X and Y must not
+ * overlap; alpha == 0 does not read Y. */
+int gatt_block_update(gatt_ctx* ctx, int64_t n, int32_t k, int32_t p, const double* X, int64_t ldx, const double* M,
+                      double alpha, double beta, double* Y, int64_t ldy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,169 @@
+"""GPU checks of the batched stability analysis (calcInstability.m on the C ABI).
+
+* block products (gatt_block_dot / gatt_block_update) against numpy;
+* perturb / collect against the same arithmetic in numpy, bit for bit;
+* a batch of members advanced together equals the members advanced alone;
+* the batched finite-difference propagator equals the oracle's on the same
+  vectors (2-D and 3-D periodic boxes, second-order differences);
+* a full-space block Arnoldi on the 12 x 12 periodic box at rest reproduces
+  the closed-form Fourier spectrum of the time-T propagator within 1e-6;
+* the turbulence-box setup (forcing on, first-order differences as the
+  reference's default) runs through the driver with finite Ritz values.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import fourier_rest as FR
+from gatt_dns_amd.prep import cases
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("n,k,p", [(1000, 3, 5), (70001, 40, 32), (4099, 300, 33)])
+def test_block_products(torch_cuda, n, k, p):
+    from gatt_dns_amd.runtime import Solver
+    s = Solver(cases.periodic_rest(12, 12))
+    rng = np.random.Generator(np.random.PCG64(n + k + p))
+    X = rng.standard_normal((k, n))
+    Y = rng.standard_normal((p, n))
+    M = rng.standard_normal((k, p))
+    Xd, Yd = _dev(torch_cuda, X), _dev(torch_cuda, Y)
+    H = s.block_dot(Xd, Yd)
+    np.testing.assert_allclose(H, X @ Y.T, rtol=1e-12, atol=1e-10)
+    H2 = s.block_dot(Xd, Yd)
+    np.testing.assert_array_equal(H, H2)                  # fixed summation order
+    s.block_update(Xd, M, 0.5, -2.0, Yd)
+    s.sync()
+    np.testing.assert_allclose(Yd.cpu().numpy(), 0.5 * Y - 2.0 * (M.T @ X), rtol=1e-12, atol=1e-10)
+    Yd.fill_(float("nan"))
+    torch_cuda.cuda.synchronize()
+    s.block_update(Xd, M, 0.0, 1.0, Yd)                  # alpha = 0 ignores Y
+    s.sync()
+    np.testing.assert_allclose(Yd.cpu().numpy(), M.T @ X, rtol=1e-12, atol=1e-10)
+
+
+def test_perturb_collect_bitwise(torch_cuda):
+    from gatt_dns_amd import lst
+    from gatt_dns_amd.runtime import Solver
+    case = cases.lid_driven(17)
+    region = lst.flow_region_indices(case)
+    s = Solver(case, batch=3)
+    s.set_flow_region(region)
+    n = s.vector_length()
+    assert n == 4 * len(region)
+    rng = np.random.Generator(np.random.PCG64(11))
+    base = {k: rng.standard_normal((17, 17, 1)) for k in "UVWRE"}
+    base["W"][:] = 0
+    s.set_base(base)
+    V = rng.standard_normal((3, n))
+    scale = 1e-3 * math.sqrt(n)
+    s.perturb(0, 3, _dev(torch_cuda, V), scale)
+    for b in range(3):
+        q = s.get_state(b)
+        d = lst.vec2flow(case, region, V[b] * scale)
+        for k in "UVRE":
+            np.testing.assert_array_equal(q[k], base[k] + d[k])
+    W = torch_cuda.zeros((3, n), dtype=torch_cuda.float64, device="cuda")
+    ref = _dev(torch_cuda, np.concatenate([base[k].ravel(order="F")[region] for k in "UVRE"]))
+    s.collect(0, 3, W, 1.0 / scale, ref=ref[None, :])
+    s.sync()
+    pert = np.stack([np.concatenate([(base[k] + lst.vec2flow(case, region, V[b] * scale)[k]).ravel(order="F")[region]
+                                     for k in "UVRE"]) for b in range(3)])
+    expect = (1.0 / scale) * (pert - ref.cpu().numpy()[None, :])
+    np.testing.assert_array_equal(W.cpu().numpy(), expect)
+    # second-order form: member 0 minus member 1
+    s.collect(0, 1, W[0:1], 0.5, minus_first=1)
+    s.sync()
+    np.testing.assert_array_equal(W.cpu().numpy()[0], 0.5 * (pert[0] - pert[1]))
+
+
+def test_batched_members_equal_single(torch_cuda):
+    from gatt_dns_amd.runtime import Solver
+    case = cases.turbulence_box(16, seed=5)
+    nb = 3
+    sb = Solver(case, batch=nb)
+    states = []
+    for b in range(nb):
+        q = {k: np.array(v, copy=True) for k, v in case.state0.items()}
+        q["U"] = q["U"] * (1 + 0.1 * b)
+        q["E"] = q["E"] + 0.01 * b
+        states.append(q)
+        sb.set_state(q, b)
+    sb.advance(6, 0.0, case.dt)
+    for b in range(nb):
+        s1 = Solver(case)
+        s1.set_state(states[b])
+        s1.advance(6, 0.0, case.dt)
+        q1, qb = s1.get_state(), sb.get_state(b)
+        for k in "UVWRE":
+            np.testing.assert_array_equal(qb[k], q1[k])
+        s1.close()
+
+
+@pytest.mark.parametrize("dims", [(12, 12, 1), (12, 12, 12)])
+def test_batched_propagator_matches_oracle(torch_cuda, dims):
+    from gatt_dns_amd import lst
+    from oracle import lst as OL
+    case = cases.periodic_rest(*dims)
+    rng = np.random.Generator(np.random.PCG64(21))
+    base = {k: np.array(v, copy=True) for k, v in case.state0.items()}
+    # a non-trivial base: small smooth velocity and density variations
+    x = np.arange(dims[0]) / dims[0]
+    base["U"] = base["U"] + 0.05 * np.sin(2 * np.pi * x)[:, None, None]
+    base["R"] = base["R"] + 0.02 * np.cos(2 * np.pi * x)[:, None, None]
+    T = 0.05
+    st = lst.Stability(case, base, T, block=4, epsilon=1e-5, first_order=False)
+    P = OL.OraclePropagator(case, base, T, epsilon=1e-5, first_order=False)
+    assert st.n == P.N and st.nsteps == P.nsteps
+    V = rng.standard_normal((4, st.n))
+    V /= np.linalg.norm(V, axis=1)[:, None]
+    W = st.be.empty(4, st.n)
+    st.apply(_dev(torch_cuda, V), W)
+    Wg = st.be.download(W)
+    for b in range(4):
+        Wo = P.apply(V[b])
+        rel = np.linalg.norm(Wg[b] - Wo) / np.linalg.norm(Wo)
+        assert rel < 1e-7, (b, rel)
+    st.close()
+
+
+def test_full_space_block_arnoldi_closed_form(torch_cuda):
+    from gatt_dns_amd import lst
+    case = cases.periodic_rest(12, 12)
+    T = 0.25
+    N = 4 * 144
+    out = lst.calc_instability(case, case.state0, T, krylov_dim=N, block=32, epsilon=1e-5, first_order=False,
+                               seed=3)
+    mu_ref = FR.propagator_eigenvalues(case, out["nsteps"], out["dt"])
+    d, r, c = FR.match(out["mu"], mu_ref)
+    assert d < 1e-6, d
+    lead = np.abs(mu_ref[c]) > 0.5
+    lam_ref = np.log(mu_ref[c][lead].astype(complex)) / T
+    lam = np.log(out["mu"][r][lead].astype(complex)) / T
+    assert np.abs(lam - lam_ref).max() < 1e-6
+    # leading eigenvalue: the four uniform shifts (lambda = 0)
+    assert np.abs(out["lambda"][:4]).max() < 1e-6
+
+
+def test_turbulence_box_driver_runs(torch_cuda):
+    from gatt_dns_amd import lst
+    case = cases.turbulence_box(16, seed=7)
+    out = lst.calc_instability(case, case.state0, 4 * case.dt, krylov_dim=16, block=8, epsilon=1e-7,
+                               first_order=True, n_modes=2)
+    assert np.all(np.isfinite(out["lambda"]))
+    assert out["members_advanced"] == 8 * 3       # base run + two blocks
+    assert len(out["modes"]) == 2 and np.all(np.isfinite(out["modes"][0][0]["U"]))
