@@ -80,6 +80,104 @@ def batched_throughput(runtime, case, members, steps, shapes=None):
             "single_mpts": round(res["single"], 2), "speedup_vs_sequential": round(res["batched"] / res["single"], 3)}
 
 
+def ragged_sweep(runtime, steps=20):
+    """16 lid-driven cavities 256x128 .. 768x256, Re 500..2000: grouped vs one after the other."""
+    from gatt_dns_amd.prep import cases
+    sweep = cases.cavity_sweep()
+    pts = sum(c.npoints for c in sweep)
+
+    def fresh():
+        out = []
+        for c in sweep:
+            s = runtime.Solver(c)
+            s.set_state(c.state0)
+            s.apply_bound_cond(0.0)
+            out.append(s)
+        return out
+
+    res = {}
+    solvers = fresh()
+    for s, c in zip(solvers, sweep):
+        s.advance(2, 0.0, c.dt)
+    for s in solvers:
+        s.sync()
+    t0 = time.perf_counter()
+    for s, c in zip(solvers, sweep):
+        s.advance(steps, 2 * c.dt, c.dt)
+        s.sync()
+    res["sequential"] = pts * steps / (time.perf_counter() - t0) / 1e6
+    for mode in (0, 1):
+        g = runtime.Group(solvers, mode=mode)
+        g.advance(2, 0.0, [c.dt for c in sweep])
+        t0 = time.perf_counter()
+        g.advance(steps, 0.0, [c.dt for c in sweep])
+        for s in solvers:
+            s.sync()
+        res["group_mode%d" % mode] = pts * steps / (time.perf_counter() - t0) / 1e6
+        g.close()
+    for s in solvers:
+        s.close()
+    return {"members": len(sweep), "points": pts, "steps": steps,
+            "sequential_mpts": round(res["sequential"], 2),
+            "group_graph_mpts": round(res["group_mode0"], 2),
+            "group_streams_mpts": round(res["group_mode1"], 2),
+            "speedup_vs_sequential": round(max(res["group_mode0"], res["group_mode1"]) / res["sequential"], 3)}
+
+
+def adjoint_cost(runtime, nx=512, ny=192, nz=64, steps=6):
+    """Tangent and adjoint step cost against the plain step on a periodic box of the adjoint
+    configuration's grid (512x192x64), smooth non-uniform base state, no forcing."""
+    import numpy as np
+    import torch
+    from gatt_dns_amd.adjoint import Propagator, state_tensor
+    from gatt_dns_amd.prep import cases
+    case = cases.periodic_rest(nx, ny, nz, Re=1000.0, Ma=0.5)
+    x = np.arange(nx) / nx
+    y = np.arange(ny) / ny
+    z = np.arange(nz) / nz
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    q = {k: np.array(v, copy=True) for k, v in case.state0.items()}
+    q["U"] = np.asfortranarray(0.3 * np.sin(2 * np.pi * Y) * np.cos(2 * np.pi * Z))
+    q["V"] = np.asfortranarray(0.2 * np.sin(2 * np.pi * X))
+    q["W"] = np.asfortranarray(0.1 * np.cos(2 * np.pi * (X + Y)))
+    q["R"] = np.asfortranarray(1.0 + 0.02 * np.sin(2 * np.pi * Z))
+    prop = Propagator(case)
+    q0 = state_tensor(case, q)
+    v = torch.randn(prop.shape, dtype=torch.float64, device="cuda")
+    res = {}
+    prop.load(q0)
+    prop.steps(2)
+    prop.solver.sync()
+    t0 = time.perf_counter()
+    prop.steps(steps)
+    prop.solver.sync()
+    res["forward"] = (time.perf_counter() - t0) / steps
+    prop.load(q0)
+    prop.load_perturbation(v)
+    prop.solver.tlm_step(0.0, prop.dt)
+    prop.solver.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        prop.solver.tlm_step(0.0, prop.dt)
+    prop.solver.sync()
+    res["tangent"] = (time.perf_counter() - t0) / steps
+    prop.load(q0)
+    prop.load_perturbation(v)
+    prop.solver.adj_step(0.0, prop.dt)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        prop.solver.adj_step(0.0, prop.dt)
+    prop.solver.sync()
+    res["adjoint"] = (time.perf_counter() - t0) / steps
+    prop.close()
+    return {"workload": "periodic box %dx%dx%d, Re 1000, Ma 0.5, smooth base flow, no forcing" % (nx, ny, nz),
+            "forward_ms_per_step": round(res["forward"] * 1e3, 3),
+            "tangent_ms_per_step": round(res["tangent"] * 1e3, 3),
+            "adjoint_ms_per_step": round(res["adjoint"] * 1e3, 3),
+            "adjoint_over_forward": round(res["adjoint"] / res["forward"], 3),
+            "note": "adjoint step includes recomputing the three RK stage states of its base step"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +187,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-lst", action="store_true", help="skip the batched-member (stability analysis) lines")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the tangent/adjoint cost line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -177,6 +276,15 @@ def main():
             log("batched members: %s" % label)
             lst[label] = batched_throughput(runtime, c2, 32, nsteps, shapes)
             log("  %s" % lst[label])
+        log("ragged sweep: 16 lid-driven cavities")
+        lst["ragged_cavity_sweep"] = ragged_sweep(runtime)
+        log("  %s" % lst["ragged_cavity_sweep"])
+
+    adj = None
+    if not args.no_adjoint:
+        log("tangent / adjoint step cost")
+        adj = adjoint_cost(runtime)
+        log("  %s" % adj)
 
     cpu = None
     if not args.no_cpu:
@@ -200,6 +308,7 @@ def main():
                     for k, v in per_class.items()},
         "line_shapes": shapes_out,
         "batched_lst": lst,
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -1,0 +1,418 @@
+// Tangent-linear and adjoint kernels of the time step (no reference
+// counterpart: differentiation of nseq.F90, derivs.F90 and the RK4/filter/SFD
+// sequence as implemented in pointwise.hip and the line solves).
+//
+// Point-wise parts: the tangent of the viscous stresses / heat fluxes and of
+// the RHS assembly, and their transposes, derived from the forward formulas of
+// visc_kernel and assemble_kernel (nseq.F90:218-318, 2-D 72-134).  The linear
+// parts (line solves, RK combinations, filter, SFD) reuse the forward kernels
+// for the tangent; the adjoint uses the transposed solves (line_solve_t.hip)
+// and the transposed RHS band below.
+#include "gatt_internal.h"
+
+namespace gatt {
+
+static inline unsigned grid_for(long long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// -------------------------------------------------------------- band^T
+// out = beta*src + (accumulate ? out : 0) + alpha * sum_l R((e-l) mod N, l) in(e-l)
+template <int DIR>
+__global__ __launch_bounds__(256) void band_t_kernel(BandTArgs a) {
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= a.np) return;
+  const int f = (int)blockIdx.y % a.nf;
+  const int mem = (int)blockIdx.y / a.nf;
+  const long long nxy = (long long)a.nx * a.ny;
+  const int i = (int)(p % a.nx);
+  const int j = (int)((p / a.nx) % a.ny);
+  const int k = (int)(p / nxy);
+  const int e = DIR == 0 ? i : (DIR == 1 ? j : k);
+  const int line = DIR == 0 ? j + a.ny * k : (DIR == 1 ? i + a.nx * k : i + a.nx * j);
+  const long long stride = DIR == 0 ? 1 : (DIR == 1 ? (long long)a.nx : nxy);
+  const int N = a.N, H = a.nR - 1, CS = a.cs;
+  const double* cf = a.coef + (size_t)a.types[line] * N * CS;
+  const double* in = a.in[f] + (long long)mem * a.in_stride;
+  double acc = 0.0;
+  for (int l = -H; l <= H; ++l) {
+    int se = e - l;
+    se = se < 0 ? se + N : (se >= N ? se - N : se);
+    acc = acc + cf[(size_t)se * CS + H + l] * in[p + (long long)(se - e) * stride];
+  }
+  double* out = a.out[f] + (long long)mem * a.out_stride;
+  double v = a.alpha * acc;
+  if (a.src[f]) v = a.beta * a.src[f][(long long)mem * a.out_stride + p] + v;
+  if (a.accumulate) v = out[p] + v;
+  out[p] = v;
+}
+
+void launch_band_t(const BandTArgs& a, int nmembers, hipStream_t st) {
+  dim3 grid(grid_for(a.np, 256), (unsigned)(a.nf * nmembers));
+  if (a.dir == 0) hipLaunchKernelGGL(band_t_kernel<0>, grid, dim3(256), 0, st, a);
+  else if (a.dir == 1) hipLaunchKernelGGL(band_t_kernel<1>, grid, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(band_t_kernel<2>, grid, dim3(256), 0, st, a);
+}
+
+// -------------------------------------------------------------- axpby
+__global__ __launch_bounds__(256) void axpby_kernel(AxpbyArgs a) {
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= a.np) return;
+  const int f = (int)blockIdx.y % a.nf;
+  const int mem = (int)blockIdx.y / a.nf;
+  const double x = a.x[f][(long long)mem * a.x_stride + p];
+  double* y = a.y[f] + (long long)mem * a.y_stride + p;
+  // b == 0 overwrites y (and a == b == 0 clears it, whatever y held)
+  *y = a.b == 0.0 ? (a.a == 0.0 ? 0.0 : a.a * x) : a.a * x + a.b * *y;
+}
+
+void launch_axpby(const AxpbyArgs& a, int nmembers, hipStream_t st) {
+  hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(a.np, 256), (unsigned)(a.nf * nmembers)), dim3(256), 0, st, a);
+}
+
+// -------------------------------------------------------------- viscous terms
+struct Mu {
+  double mu, dmu;   // mu(E) and d mu / d E
+};
+
+__device__ __forceinline__ Mu sutherland(const FlowConst& c, double E) {
+  const double den = c.muC * E + c.muB;
+  const double mu1 = c.muA / den;
+  const double mu2 = c.muC * E;
+  const double mu = mu1 * mu2 * sqrt(mu2);
+  // d ln(mu)/dE = -muC/(muC E + muB) + 3/(2E)
+  return {mu, mu * (1.5 / E - c.muC / den)};
+}
+
+__global__ __launch_bounds__(256) void visc_tl_kernel(ViscLinArgs a) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= a.np) return;
+  const long long np = a.np;
+  const long long mo = (long long)blockIdx.y * a.member_stride;
+  const long long wo = (long long)blockIdx.y * a.work_stride;
+  const long long lo = (long long)blockIdx.y * a.lin_stride;
+  const double* D = a.D + wo;
+  const double* dD = a.dD + lo;
+  double* dT = a.dT + lo;
+  const FlowConst c = a.c;
+  const Mu m = sutherland(c, a.E[mo + n]);
+  const double dE = a.dE[lo + n];
+  const double rmu = c.invRe * m.mu;
+  const double drmu = c.invRe * (m.dmu * dE);
+  const double dmu = m.dmu * dE;
+  const double DUx = D[0 * np + n], DUy = D[1 * np + n], DVx = D[3 * np + n], DVy = D[4 * np + n];
+  const double DEx = D[12 * np + n], DEy = D[13 * np + n];
+  const double tDUx = dD[0 * np + n], tDUy = dD[1 * np + n], tDVx = dD[3 * np + n], tDVy = dD[4 * np + n];
+  const double tDEx = dD[12 * np + n], tDEy = dD[13 * np + n];
+  if (a.is3d) {
+    const double DUz = D[2 * np + n], DVz = D[5 * np + n], DWx = D[6 * np + n], DWy = D[7 * np + n];
+    const double DWz = D[8 * np + n], DEz = D[14 * np + n];
+    const double tDUz = dD[2 * np + n], tDVz = dD[5 * np + n], tDWx = dD[6 * np + n], tDWy = dD[7 * np + n];
+    const double tDWz = dD[8 * np + n], tDEz = dD[14 * np + n];
+    const double tz = -(2.0 / 3.0 * (DUx + DVy + DWz));
+    const double dtz = -(2.0 / 3.0 * (tDUx + tDVy + tDWz));
+    dT[0 * np + n] = drmu * (2.0 * DUx + tz) + rmu * (2.0 * tDUx + dtz);
+    dT[1 * np + n] = drmu * (2.0 * DVy + tz) + rmu * (2.0 * tDVy + dtz);
+    dT[2 * np + n] = drmu * (2.0 * DWz + tz) + rmu * (2.0 * tDWz + dtz);
+    dT[3 * np + n] = drmu * (DUy + DVx) + rmu * (tDUy + tDVx);
+    dT[4 * np + n] = drmu * (DUz + DWx) + rmu * (tDUz + tDWx);
+    dT[5 * np + n] = drmu * (DVz + DWy) + rmu * (tDVz + tDWy);
+    dT[6 * np + n] = -(c.qk * (dmu * DEx + m.mu * tDEx));
+    dT[7 * np + n] = -(c.qk * (dmu * DEy + m.mu * tDEy));
+    dT[8 * np + n] = -(c.qk * (dmu * DEz + m.mu * tDEz));
+  } else {
+    const double dv = 2.0 / 3.0 * (DUx + DVy);
+    const double ddv = 2.0 / 3.0 * (tDUx + tDVy);
+    dT[0 * np + n] = drmu * (2.0 * DUx - dv) + rmu * (2.0 * tDUx - ddv);
+    dT[1 * np + n] = drmu * (2.0 * DVy - dv) + rmu * (2.0 * tDVy - ddv);
+    dT[3 * np + n] = drmu * (DUy + DVx) + rmu * (tDUy + tDVx);
+    dT[6 * np + n] = -(c.qk * (dmu * DEx + m.mu * tDEx));
+    dT[7 * np + n] = -(c.qk * (dmu * DEy + m.mu * tDEy));
+  }
+}
+
+__global__ __launch_bounds__(256) void visc_adj_kernel(ViscLinArgs a) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= a.np) return;
+  const long long np = a.np;
+  const long long mo = (long long)blockIdx.y * a.member_stride;
+  const long long wo = (long long)blockIdx.y * a.work_stride;
+  const long long lo = (long long)blockIdx.y * a.lin_stride;
+  const double* D = a.D + wo;
+  const double* Tb = a.dT + lo;
+  double* Db = a.Dbar + lo;
+  const FlowConst c = a.c;
+  const Mu m = sutherland(c, a.E[mo + n]);
+  const double rmu = c.invRe * m.mu;
+  const double DUx = D[0 * np + n], DUy = D[1 * np + n], DVx = D[3 * np + n], DVy = D[4 * np + n];
+  const double DEx = D[12 * np + n], DEy = D[13 * np + n];
+  const double b0 = Tb[0 * np + n], b1 = Tb[1 * np + n], b3 = Tb[3 * np + n];
+  const double b6 = Tb[6 * np + n], b7 = Tb[7 * np + n];
+  double rmub, mub;
+  if (a.is3d) {
+    const double DUz = D[2 * np + n], DVz = D[5 * np + n], DWx = D[6 * np + n], DWy = D[7 * np + n];
+    const double DWz = D[8 * np + n], DEz = D[14 * np + n];
+    const double b2 = Tb[2 * np + n], b4 = Tb[4 * np + n], b5 = Tb[5 * np + n], b8 = Tb[8 * np + n];
+    const double tz = -(2.0 / 3.0 * (DUx + DVy + DWz));
+    rmub = b0 * (2.0 * DUx + tz) + b1 * (2.0 * DVy + tz) + b2 * (2.0 * DWz + tz) + b3 * (DUy + DVx) +
+           b4 * (DUz + DWx) + b5 * (DVz + DWy);
+    const double tzb = rmu * (b0 + b1 + b2);
+    const double divb = -(2.0 / 3.0) * tzb;
+    Db[0 * np + n] += 2.0 * rmu * b0 + divb;
+    Db[4 * np + n] += 2.0 * rmu * b1 + divb;
+    Db[8 * np + n] += 2.0 * rmu * b2 + divb;
+    Db[1 * np + n] += rmu * b3;
+    Db[3 * np + n] += rmu * b3;
+    Db[2 * np + n] += rmu * b4;
+    Db[6 * np + n] += rmu * b4;
+    Db[5 * np + n] += rmu * b5;
+    Db[7 * np + n] += rmu * b5;
+    Db[12 * np + n] += -(c.qk * m.mu * b6);
+    Db[13 * np + n] += -(c.qk * m.mu * b7);
+    Db[14 * np + n] += -(c.qk * m.mu * b8);
+    mub = c.invRe * rmub - c.qk * (DEx * b6 + DEy * b7 + DEz * b8);
+  } else {
+    const double dv = 2.0 / 3.0 * (DUx + DVy);
+    rmub = b0 * (2.0 * DUx - dv) + b1 * (2.0 * DVy - dv) + b3 * (DUy + DVx);
+    const double divb = -(2.0 / 3.0) * (rmu * (b0 + b1));
+    Db[0 * np + n] += 2.0 * rmu * b0 + divb;
+    Db[4 * np + n] += 2.0 * rmu * b1 + divb;
+    Db[1 * np + n] += rmu * b3;
+    Db[3 * np + n] += rmu * b3;
+    Db[12 * np + n] += -(c.qk * m.mu * b6);
+    Db[13 * np + n] += -(c.qk * m.mu * b7);
+    mub = c.invRe * rmub - c.qk * (DEx * b6 + DEy * b7);
+  }
+  a.dEbar[lo + n] += m.dmu * mub;
+}
+
+void launch_visc_tl(const ViscLinArgs& a, int nmembers, hipStream_t st) {
+  hipLaunchKernelGGL(visc_tl_kernel, dim3(grid_for(a.np, 256), nmembers), dim3(256), 0, st, a);
+}
+void launch_visc_adj(const ViscLinArgs& a, int nmembers, hipStream_t st) {
+  hipLaunchKernelGGL(visc_adj_kernel, dim3(grid_for(a.np, 256), nmembers), dim3(256), 0, st, a);
+}
+
+// -------------------------------------------------------------- RHS assembly
+// Base fields of one point, with the combinations the RHS uses.
+struct Base {
+  double U, V, W, R, E;
+  double DUx, DUy, DUz, DVx, DVy, DVz, DWx, DWy, DWz, DRx, DRy, DRz, DEx, DEy, DEz;
+  double txx, tyy, tzz, txy, txz, tyz;
+  double vtx, vty, vtz, P, DPx, DPy, DPz, div, Phi;
+};
+
+__device__ __forceinline__ Base load_base(const AssembleLinArgs& a, long long n) {
+  const long long np = a.np;
+  const long long mo = (long long)blockIdx.y * a.member_stride;
+  const long long wo = (long long)blockIdx.y * a.work_stride;
+  const double* S = a.S + mo;
+  const double* D = a.D + wo;
+  const double* T = a.T + wo;
+  const double* G = a.G + wo;
+  const double g1 = a.c.gm1;
+  Base b;
+  b.U = S[0 * np + n]; b.V = S[1 * np + n]; b.R = S[3 * np + n]; b.E = S[4 * np + n];
+  b.DUx = D[0 * np + n]; b.DUy = D[1 * np + n]; b.DVx = D[3 * np + n]; b.DVy = D[4 * np + n];
+  b.DRx = D[9 * np + n]; b.DRy = D[10 * np + n]; b.DEx = D[12 * np + n]; b.DEy = D[13 * np + n];
+  b.txx = T[0 * np + n]; b.tyy = T[1 * np + n]; b.txy = T[3 * np + n];
+  b.P = g1 * b.R * b.E;
+  b.DPx = g1 * (b.R * b.DEx + b.DRx * b.E);
+  b.DPy = g1 * (b.R * b.DEy + b.DRy * b.E);
+  if (a.is3d) {
+    b.W = S[2 * np + n];
+    b.DUz = D[2 * np + n]; b.DVz = D[5 * np + n]; b.DWx = D[6 * np + n]; b.DWy = D[7 * np + n];
+    b.DWz = D[8 * np + n]; b.DRz = D[11 * np + n]; b.DEz = D[14 * np + n];
+    b.tzz = T[2 * np + n]; b.txz = T[4 * np + n]; b.tyz = T[5 * np + n];
+    b.DPz = g1 * (b.R * b.DEz + b.DRz * b.E);
+    b.vtx = G[0 * np + n] + G[4 * np + n] + G[8 * np + n];
+    b.vty = G[5 * np + n] + G[1 * np + n] + G[9 * np + n];
+    b.vtz = G[10 * np + n] + G[2 * np + n] + G[6 * np + n];
+    b.div = b.DUx + b.DVy + b.DWz;
+    b.Phi = -(b.P * b.div) + b.txx * b.DUx + b.tyy * b.DVy + b.tzz * b.DWz + b.txy * (b.DVx + b.DUy) +
+            b.txz * (b.DWx + b.DUz) + b.tyz * (b.DWy + b.DVz) - G[3 * np + n] - G[7 * np + n] - G[11 * np + n];
+  } else {
+    b.W = 0; b.DUz = b.DVz = b.DWx = b.DWy = b.DWz = b.DRz = b.DEz = 0;
+    b.tzz = b.txz = b.tyz = 0; b.DPz = 0; b.vtz = 0;
+    b.vtx = G[0 * np + n] + G[4 * np + n];
+    b.vty = G[5 * np + n] + G[1 * np + n];
+    b.div = b.DUx + b.DVy;
+    b.Phi = -(b.P * b.div) + b.txx * b.DUx + b.tyy * b.DVy + b.txy * (b.DVx + b.DUy) - G[3 * np + n] -
+            G[7 * np + n];
+  }
+  return b;
+}
+
+__global__ __launch_bounds__(256) void assemble_tl_kernel(AssembleLinArgs a) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= a.np) return;
+  const long long np = a.np;
+  const long long lo = (long long)blockIdx.y * a.lin_stride;
+  const Base b = load_base(a, n);
+  const double g1 = a.c.gm1;
+  const double* dS = a.dS + lo;
+  const double* dD = a.dD + lo;
+  const double* dT = a.dT + lo;
+  const double* dG = a.dG + lo;
+  const double tU = dS[0 * np + n], tV = dS[1 * np + n], tR = dS[3 * np + n], tE = dS[4 * np + n];
+  const double tDUx = dD[0 * np + n], tDUy = dD[1 * np + n], tDVx = dD[3 * np + n], tDVy = dD[4 * np + n];
+  const double tDRx = dD[9 * np + n], tDRy = dD[10 * np + n], tDEx = dD[12 * np + n], tDEy = dD[13 * np + n];
+  const double ttxx = dT[0 * np + n], ttyy = dT[1 * np + n], ttxy = dT[3 * np + n];
+  const double tP = g1 * (tR * b.E + b.R * tE);
+  const double tDPx = g1 * (tR * b.DEx + b.R * tDEx + tDRx * b.E + b.DRx * tE);
+  const double tDPy = g1 * (tR * b.DEy + b.R * tDEy + tDRy * b.E + b.DRy * tE);
+  const double invR = 1.0 / b.R;
+  const double invR2 = invR * invR;
+  double d[5];
+  if (a.is3d) {
+    const double tW = dS[2 * np + n];
+    const double tDUz = dD[2 * np + n], tDVz = dD[5 * np + n], tDWx = dD[6 * np + n], tDWy = dD[7 * np + n];
+    const double tDWz = dD[8 * np + n], tDRz = dD[11 * np + n], tDEz = dD[14 * np + n];
+    const double ttzz = dT[2 * np + n], ttxz = dT[4 * np + n], ttyz = dT[5 * np + n];
+    const double tDPz = g1 * (tR * b.DEz + b.R * tDEz + tDRz * b.E + b.DRz * tE);
+    const double tvtx = dG[0 * np + n] + dG[4 * np + n] + dG[8 * np + n];
+    const double tvty = dG[5 * np + n] + dG[1 * np + n] + dG[9 * np + n];
+    const double tvtz = dG[10 * np + n] + dG[2 * np + n] + dG[6 * np + n];
+    const double tdiv = tDUx + tDVy + tDWz;
+    d[3] = -(tR * b.div + b.R * tdiv) - (tDRx * b.U + b.DRx * tU) - (tDRy * b.V + b.DRy * tV) -
+           (tDRz * b.W + b.DRz * tW);
+    d[0] = -(tDUx * b.U + b.DUx * tU + tDUy * b.V + b.DUy * tV + tDUz * b.W + b.DUz * tW) +
+           (tvtx - tDPx) * invR - (b.vtx - b.DPx) * tR * invR2;
+    d[1] = -(tDVx * b.U + b.DVx * tU + tDVy * b.V + b.DVy * tV + tDVz * b.W + b.DVz * tW) +
+           (tvty - tDPy) * invR - (b.vty - b.DPy) * tR * invR2;
+    d[2] = -(tDWx * b.U + b.DWx * tU + tDWy * b.V + b.DWy * tV + tDWz * b.W + b.DWz * tW) +
+           (tvtz - tDPz) * invR - (b.vtz - b.DPz) * tR * invR2;
+    const double tPhi = -(tP * b.div + b.P * tdiv) + ttxx * b.DUx + b.txx * tDUx + ttyy * b.DVy + b.tyy * tDVy +
+                        ttzz * b.DWz + b.tzz * tDWz + ttxy * (b.DVx + b.DUy) + b.txy * (tDVx + tDUy) +
+                        ttxz * (b.DWx + b.DUz) + b.txz * (tDWx + tDUz) + ttyz * (b.DWy + b.DVz) +
+                        b.tyz * (tDWy + tDVz) - dG[3 * np + n] - dG[7 * np + n] - dG[11 * np + n];
+    d[4] = -(tDEx * b.U + b.DEx * tU + tDEy * b.V + b.DEy * tV + tDEz * b.W + b.DEz * tW) + tPhi * invR -
+           b.Phi * tR * invR2;
+  } else {
+    const double tvtx = dG[0 * np + n] + dG[4 * np + n];
+    const double tvty = dG[5 * np + n] + dG[1 * np + n];
+    const double tdiv = tDUx + tDVy;
+    d[3] = -(tR * b.div + b.R * tdiv) - (tDRx * b.U + b.DRx * tU) - (tDRy * b.V + b.DRy * tV);
+    d[0] = -(tDUx * b.U + b.DUx * tU + tDUy * b.V + b.DUy * tV) + (tvtx - tDPx) * invR -
+           (b.vtx - b.DPx) * tR * invR2;
+    d[1] = -(tDVx * b.U + b.DVx * tU + tDVy * b.V + b.DVy * tV) + (tvty - tDPy) * invR -
+           (b.vty - b.DPy) * tR * invR2;
+    d[2] = 0.0;
+    const double tPhi = -(tP * b.div + b.P * tdiv) + ttxx * b.DUx + b.txx * tDUx + ttyy * b.DVy + b.tyy * tDVy +
+                        ttxy * (b.DVx + b.DUy) + b.txy * (tDVx + tDUy) - dG[3 * np + n] - dG[7 * np + n];
+    d[4] = -(tDEx * b.U + b.DEx * tU + tDEy * b.V + b.DEy * tV) + tPhi * invR - b.Phi * tR * invR2;
+  }
+  const double* Q = a.dQ + lo;
+  double* O = a.dOut + lo;
+  double* Dl = a.dDelta + lo;
+  const double dt = a.dt;
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    if (!a.is3d && f == 2) continue;
+    const long long o = f * np + n;
+    switch (a.stage) {
+      case 0: O[o] = d[f]; break;
+      case 1: Dl[o] = dt / 6.0 * d[f]; O[o] = Q[o] + dt / 2.0 * d[f]; break;
+      case 2: Dl[o] = Dl[o] + dt / 3.0 * d[f]; O[o] = Q[o] + dt / 2.0 * d[f]; break;
+      case 3: Dl[o] = Dl[o] + dt / 3.0 * d[f]; O[o] = Q[o] + dt * d[f]; break;
+      default: O[o] = Q[o] + Dl[o] + dt / 6.0 * d[f]; break;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void assemble_adj_kernel(AssembleLinArgs a) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= a.np) return;
+  const long long np = a.np;
+  const long long lo = (long long)blockIdx.y * a.lin_stride;
+  const Base b = load_base(a, n);
+  const double g1 = a.c.gm1;
+  const double* db = a.dbar + lo;
+  const double aU = db[0 * np + n], aV = db[1 * np + n], aR = db[3 * np + n], aE = db[4 * np + n];
+  const double aW = a.is3d ? db[2 * np + n] : 0.0;
+  const double invR = 1.0 / b.R;
+  const double invR2 = invR * invR;
+  // momentum
+  double Ub = -(b.DUx * aU) - b.DVx * aV - b.DWx * aW;
+  double Vb = -(b.DUy * aU) - b.DVy * aV - b.DWy * aW;
+  double Wb = -(b.DUz * aU) - b.DVz * aV - b.DWz * aW;
+  double DUxb = -(b.U * aU), DUyb = -(b.V * aU), DUzb = -(b.W * aU);
+  double DVxb = -(b.U * aV), DVyb = -(b.V * aV), DVzb = -(b.W * aV);
+  double DWxb = -(b.U * aW), DWyb = -(b.V * aW), DWzb = -(b.W * aW);
+  const double vtxb = aU * invR, vtyb = aV * invR, vtzb = aW * invR;
+  const double DPxb = -(aU * invR), DPyb = -(aV * invR), DPzb = -(aW * invR);
+  double Rb = -((b.vtx - b.DPx) * invR2 * aU) - (b.vty - b.DPy) * invR2 * aV - (b.vtz - b.DPz) * invR2 * aW;
+  // continuity
+  Rb += -(b.div * aR);
+  double divb = -(b.R * aR);
+  double DRxb = -(b.U * aR), DRyb = -(b.V * aR), DRzb = -(b.W * aR);
+  Ub += -(b.DRx * aR);
+  Vb += -(b.DRy * aR);
+  Wb += -(b.DRz * aR);
+  // energy
+  Ub += -(b.DEx * aE);
+  Vb += -(b.DEy * aE);
+  Wb += -(b.DEz * aE);
+  double DExb = -(b.U * aE), DEyb = -(b.V * aE), DEzb = -(b.W * aE);
+  const double Phib = aE * invR;
+  Rb += -(b.Phi * invR2 * aE);
+  const double Pb = -(b.div * Phib);
+  divb += -(b.P * Phib);
+  const double txxb = b.DUx * Phib, tyyb = b.DVy * Phib, tzzb = b.DWz * Phib;
+  const double txyb = (b.DVx + b.DUy) * Phib, txzb = (b.DWx + b.DUz) * Phib, tyzb = (b.DWy + b.DVz) * Phib;
+  DUxb += b.txx * Phib;
+  DVyb += b.tyy * Phib;
+  DWzb += b.tzz * Phib;
+  DVxb += b.txy * Phib;
+  DUyb += b.txy * Phib;
+  DWxb += b.txz * Phib;
+  DUzb += b.txz * Phib;
+  DWyb += b.tyz * Phib;
+  DVzb += b.tyz * Phib;
+  // pressure gradient DP = g1 (R DE + DR E) and P = g1 R E
+  Rb += g1 * (b.DEx * DPxb + b.DEy * DPyb + b.DEz * DPzb) + g1 * b.E * Pb;
+  DExb += g1 * b.R * DPxb;
+  DEyb += g1 * b.R * DPyb;
+  DEzb += g1 * b.R * DPzb;
+  DRxb += g1 * b.E * DPxb;
+  DRyb += g1 * b.E * DPyb;
+  DRzb += g1 * b.E * DPzb;
+  double Eb = g1 * (b.DRx * DPxb + b.DRy * DPyb + b.DRz * DPzb) + g1 * b.R * Pb;
+  // divergence
+  DUxb += divb;
+  DVyb += divb;
+  DWzb += divb;
+
+  double* Sb = a.Sbar + lo;
+  Sb[0 * np + n] += Ub;
+  Sb[1 * np + n] += Vb;
+  if (a.is3d) Sb[2 * np + n] += Wb;
+  Sb[3 * np + n] += Rb;
+  Sb[4 * np + n] += Eb;
+  double* Db = a.Dbar + lo;
+  Db[0 * np + n] = DUxb; Db[1 * np + n] = DUyb; Db[2 * np + n] = DUzb;
+  Db[3 * np + n] = DVxb; Db[4 * np + n] = DVyb; Db[5 * np + n] = DVzb;
+  Db[6 * np + n] = DWxb; Db[7 * np + n] = DWyb; Db[8 * np + n] = DWzb;
+  Db[9 * np + n] = DRxb; Db[10 * np + n] = DRyb; Db[11 * np + n] = DRzb;
+  Db[12 * np + n] = DExb; Db[13 * np + n] = DEyb; Db[14 * np + n] = DEzb;
+  double* Tb = a.Tbar + lo;
+  Tb[0 * np + n] = txxb; Tb[1 * np + n] = tyyb; Tb[2 * np + n] = tzzb;
+  Tb[3 * np + n] = txyb; Tb[4 * np + n] = txzb; Tb[5 * np + n] = tyzb;
+  Tb[6 * np + n] = 0.0; Tb[7 * np + n] = 0.0; Tb[8 * np + n] = 0.0;
+  double* Gb = a.Gbar + lo;
+  // x: txx,txy,txz,qx | y: txy,tyy,tyz,qy | z: txz,tyz,tzz,qz
+  Gb[0 * np + n] = vtxb; Gb[1 * np + n] = vtyb; Gb[2 * np + n] = vtzb; Gb[3 * np + n] = -Phib;
+  Gb[4 * np + n] = vtxb; Gb[5 * np + n] = vtyb; Gb[6 * np + n] = vtzb; Gb[7 * np + n] = -Phib;
+  Gb[8 * np + n] = vtxb; Gb[9 * np + n] = vtyb; Gb[10 * np + n] = vtzb; Gb[11 * np + n] = -Phib;
+  if (!a.is3d) {
+    Gb[2 * np + n] = 0.0; Gb[6 * np + n] = 0.0;
+    Gb[8 * np + n] = 0.0; Gb[9 * np + n] = 0.0; Gb[10 * np + n] = 0.0; Gb[11 * np + n] = 0.0;
+  }
+}
+
+void launch_assemble_tl(const AssembleLinArgs& a, int nmembers, hipStream_t st) {
+  hipLaunchKernelGGL(assemble_tl_kernel, dim3(grid_for(a.np, 256), nmembers), dim3(256), 0, st, a);
+}
+void launch_assemble_adj(const AssembleLinArgs& a, int nmembers, hipStream_t st) {
+  hipLaunchKernelGGL(assemble_adj_kernel, dim3(grid_for(a.np, 256), nmembers), dim3(256), 0, st, a);
+}
+
+}  // namespace gatt

@@ -114,6 +114,9 @@ struct gatt_ctx {
   int* d_flag = nullptr;
   int cur = 0;                          // 0: state in QA, 1: in QB
   hipStream_t stream = nullptr;
+  // tangent / adjoint arrays, per member (allocated on first use)
+  double* lin = nullptr;
+  long long lin_stride = 0;
   // stability analysis
   long long nflow = 0;                  // flow-region points (0: not set, every point)
   int* d_idx = nullptr;                 // region index -> point
@@ -808,16 +811,18 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
   HIPCHK(hipMalloc(&p, sizeof(int)));
   c->dev_allocs.push_back(p);
   c->d_flag = static_cast<int*>(p);
-  HIPCHK(hipMemset(c->d_flag, 0, sizeof(int)));
+  HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
   hipError_t e = hipMalloc(&p, sizeof(double) * (size_t)c->slab * batch);
   if (e != hipSuccess) return fail(GATT_E_NOMEM, "state allocation failed");
   c->mem = static_cast<double*>(p);
-  HIPCHK(hipMemset(c->mem, 0, sizeof(double) * (size_t)c->slab * batch));
+  HIPCHK(hipMemsetAsync(c->mem, 0, sizeof(double) * (size_t)c->slab * batch, c->stream));
   e = hipMalloc(&p, sizeof(double) * (size_t)c->work_stride * c->wgroup);
   if (e != hipSuccess) return fail(GATT_E_NOMEM, "workspace allocation failed");
   c->dev_allocs.push_back(p);
   c->work = static_cast<double*>(p);
-  HIPCHK(hipMemset(c->work, 0, sizeof(double) * (size_t)c->work_stride * c->wgroup));
+  HIPCHK(hipMemsetAsync(c->work, 0, sizeof(double) * (size_t)c->work_stride * c->wgroup, c->stream));
+  // the clears run on the context's (non-blocking) stream: finish them before any other stream's work
+  HIPCHK(hipStreamSynchronize(c->stream));
   *out = c.release();
   return GATT_OK;
 }
@@ -831,6 +836,7 @@ int gatt_destroy(gatt_ctx* c) {
   if (c->d_rmap) hipFree(c->d_rmap);
   if (c->d_base) hipFree(c->d_base);
   if (c->d_scratch) hipFree(c->d_scratch);
+  if (c->lin) hipFree(c->lin);
   if (c->mem) hipFree(c->mem);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -1339,6 +1345,723 @@ int gatt_block_update(gatt_ctx* c, int64_t n, int32_t k, int32_t p, const double
     // the next chunk overwrites the staged M
     if (i0 + kChunk < k) HIPCHK(hipStreamSynchronize(c->stream));
   }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+
+// ------------------------------------------------------------ ragged groups
+
+}  // extern "C"
+
+struct gatt_group {
+  std::vector<gatt_ctx*> ctx;
+  int mode = 0;                         // 0: one graph with a branch per member; 1: per-member graphs
+  hipGraphExec_t graph = nullptr;
+  std::vector<double> graph_dt;
+  std::vector<int> graph_cur;
+  std::vector<hipEvent_t> ev;           // fork/join events (members + 1)
+};
+
+namespace {
+
+void group_drop_graph(gatt_group* g) {
+  if (g->graph) {
+    (void)hipGraphExecDestroy(g->graph);
+    g->graph = nullptr;
+  }
+}
+
+// One double step of every member, as parallel branches forked from the first
+// member's stream and joined back into it.
+int group_capture(gatt_group* g, const double* t0, const double* dt) {
+  gatt_ctx* c0 = g->ctx[0];
+  hipGraph_t graph;
+  HIPCHK(hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
+  HIPCHK(hipEventRecord(g->ev[0], c0->stream));
+  for (size_t m = 1; m < g->ctx.size(); ++m) HIPCHK(hipStreamWaitEvent(g->ctx[m]->stream, g->ev[0], 0));
+  for (size_t m = 0; m < g->ctx.size(); ++m) {
+    gatt_ctx* c = g->ctx[m];
+    enqueue_step(c, c->cur, t0[m], dt[m]);
+    enqueue_step(c, 1 - c->cur, t0[m] + dt[m], dt[m]);
+  }
+  for (size_t m = 1; m < g->ctx.size(); ++m) {
+    HIPCHK(hipEventRecord(g->ev[m], g->ctx[m]->stream));
+    HIPCHK(hipStreamWaitEvent(c0->stream, g->ev[m], 0));
+  }
+  HIPCHK(hipStreamEndCapture(c0->stream, &graph));
+  hipError_t e = hipGraphInstantiate(&g->graph, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(GATT_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  g->graph_dt.assign(dt, dt + g->ctx.size());
+  g->graph_cur.clear();
+  for (gatt_ctx* c : g->ctx) g->graph_cur.push_back(c->cur);
+  return GATT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gatt_group_create(gatt_ctx* const* ctxs, int32_t n, int32_t mode, gatt_group** out) {
+  if (!ctxs || !out || n < 1) return fail(GATT_E_INVALID, "bad group arguments");
+  if (mode != 0 && mode != 1) return fail(GATT_E_INVALID, "group mode must be 0 or 1");
+  *out = nullptr;
+  std::unique_ptr<gatt_group> g(new gatt_group);
+  for (int32_t m = 0; m < n; ++m) {
+    if (!ctxs[m]) return fail(GATT_E_INVALID, "null context in group");
+    for (int32_t k = 0; k < m; ++k)
+      if (ctxs[k] == ctxs[m]) return fail(GATT_E_INVALID, "a context appears twice in the group");
+    g->ctx.push_back(ctxs[m]);
+  }
+  g->mode = mode;
+  g->ev.resize(n);
+  for (auto& e : g->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  *out = g.release();
+  return GATT_OK;
+}
+
+int gatt_group_destroy(gatt_group* g) {
+  if (!g) return GATT_OK;
+  for (gatt_ctx* c : g->ctx) (void)hipStreamSynchronize(c->stream);
+  group_drop_graph(g);
+  for (auto& e : g->ev) (void)hipEventDestroy(e);
+  delete g;
+  return GATT_OK;
+}
+
+int gatt_group_advance(gatt_group* g, int32_t nsteps, const double* t0, const double* dt, double* maxchange,
+                       int32_t* diverged) {
+  if (!g || !t0 || !dt) return fail(GATT_E_INVALID, "null argument");
+  if (nsteps < 0) return fail(GATT_E_INVALID, "nsteps < 0");
+  const size_t n = g->ctx.size();
+  for (gatt_ctx* c : g->ctx) {
+    ensure_mean(c);
+    if (!c->hold_captured) run_bc(c, c->state(c->cur), 0.0);
+  }
+  if (g->mode == 1) {
+    // per-member double-step graphs, launched round-robin on the members' streams
+    for (size_t m = 0; m < n; ++m) {
+      gatt_ctx* c = g->ctx[m];
+      if (c->graph && (c->graph_dt != dt[m] || c->graph_cur != c->cur)) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+      }
+      if (nsteps >= 2 && !c->graph) {
+        hipGraph_t gr;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_step(c, c->cur, t0[m], dt[m]);
+        enqueue_step(c, 1 - c->cur, t0[m] + dt[m], dt[m]);
+        HIPCHK(hipStreamEndCapture(c->stream, &gr));
+        HIPCHK(hipGraphInstantiate(&c->graph, gr, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(gr));
+        c->graph_dt = dt[m];
+        c->graph_cur = c->cur;
+      }
+    }
+    for (int s = 0; s + 2 <= nsteps; s += 2)
+      for (gatt_ctx* c : g->ctx) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+  } else {
+    bool stale = !g->graph;
+    for (size_t m = 0; m < n && !stale; ++m)
+      stale = g->graph_dt[m] != dt[m] || g->graph_cur[m] != g->ctx[m]->cur;
+    if (stale) group_drop_graph(g);
+    if (nsteps >= 2 && !g->graph) {
+      int rc = group_capture(g, t0, dt);
+      if (rc) return rc;
+    }
+    // the graph runs on the first member's stream: it follows work pending on every member's stream
+    if (nsteps >= 2)
+      for (size_t m = 1; m < n; ++m) {
+        HIPCHK(hipEventRecord(g->ev[m], g->ctx[m]->stream));
+        HIPCHK(hipStreamWaitEvent(g->ctx[0]->stream, g->ev[m], 0));
+      }
+    for (int s = 0; s + 2 <= nsteps; s += 2) HIPCHK(hipGraphLaunch(g->graph, g->ctx[0]->stream));
+    // later work on a member's own stream must follow the group's launches
+    if (nsteps >= 2) {
+      HIPCHK(hipEventRecord(g->ev[0], g->ctx[0]->stream));
+      for (size_t m = 1; m < n; ++m) HIPCHK(hipStreamWaitEvent(g->ctx[m]->stream, g->ev[0], 0));
+    }
+  }
+  if (nsteps % 2) {
+    for (size_t m = 0; m < n; ++m) {
+      gatt_ctx* c = g->ctx[m];
+      enqueue_step(c, c->cur, t0[m] + (nsteps - 1) * dt[m], dt[m]);
+      c->cur = 1 - c->cur;
+      if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+      }
+    }
+    group_drop_graph(g);
+  }
+  HIPCHK(hipGetLastError());
+  for (size_t m = 0; m < n; ++m) {
+    int rc = read_scalars(g->ctx[m], nullptr, maxchange ? maxchange + 5 * m : nullptr, diverged ? diverged + m : nullptr);
+    if (rc) return rc;
+  }
+  return GATT_OK;
+}
+
+
+// ------------------------------------------------------ tangent and adjoint
+
+}  // extern "C"
+
+namespace {
+
+// Per-member linear slab: arrays of np doubles.
+enum : int {
+  LN_Q = 0,     // tangent / adjoint state (the vector the caller sets and reads)
+  LN_S = 5,     // tangent stage state / adjoint of a stage state
+  LN_DL = 10,   // tangent RK accumulator / adjoint of a stage slope
+  LN_B1 = 15,   // adjoint: stage states S1, S2, S3 of the base step
+  LN_B2 = 20,
+  LN_B3 = 25,
+  LN_D = 30,    // 15 first-derivative tangents / adjoints
+  LN_T = 45,    // 9 stress and heat-flux tangents / adjoints
+  LN_G = 54,    // 12 second-derivative tangents / adjoints
+  LN_N = 66
+};
+
+int ensure_lin(gatt_ctx* c) {
+  if (c->lin) return GATT_OK;
+  c->lin_stride = (long long)LN_N * c->np;
+  void* p = nullptr;
+  if (hipMalloc(&p, sizeof(double) * (size_t)c->lin_stride * c->batch) != hipSuccess)
+    return fail(GATT_E_NOMEM, "tangent/adjoint allocation failed");
+  c->lin = static_cast<double*>(p);
+  // on the context's stream, so that later copies into the slab are ordered after the clear
+  HIPCHK(hipMemsetAsync(c->lin, 0, sizeof(double) * (size_t)c->lin_stride * c->batch, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+double* lin_ptr(gatt_ctx* c, int idx) { return c->lin + (long long)idx * c->np; }
+
+int lin_supported(gatt_ctx* c) {
+  if (!c->ops.empty())
+    return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps are implemented for problems without boundary "
+                                    "conditions (periodic directions)");
+  if (c->forcing) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps do not cover the turbulence forcing");
+  if (c->sfdType > 0 && c->sfdDelta > 0)
+    return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps need a frozen SFD mean (Delta = inf)");
+  const int ndir = c->is3d ? 3 : 2;
+  for (int d = 0; d < ndir; ++d)
+    if (line_solve_t_points_per_lane(c->dir[d].op[0].N) == 0)
+      return fail(GATT_E_UNSUPPORTED, "transposed line solves are compiled for lines of at most 512 points");
+  return GATT_OK;
+}
+
+std::vector<int> state_fields(const gatt_ctx* c) {
+  return c->is3d ? std::vector<int>{0, 1, 2, 3, 4} : std::vector<int>{0, 1, 3, 4};
+}
+
+// y = a x + b y over the state fields of every member
+void fields_axpby(gatt_ctx* c, double* y, long long ys, const double* x, long long xs, double a, double b) {
+  AxpbyArgs ar{};
+  ar.np = c->np;
+  ar.x_stride = xs;
+  ar.y_stride = ys;
+  ar.nf = 0;
+  for (int f : state_fields(c)) {
+    ar.x[ar.nf] = x + (long long)f * c->np;
+    ar.y[ar.nf] = y + (long long)f * c->np;
+    ++ar.nf;
+  }
+  ar.a = a;
+  ar.b = b;
+  launch_axpby(ar, c->batch, c->stream);
+}
+
+// Base derivatives, stresses and their derivatives at S (member m0 pointer,
+// member stride ss) into the work arrays, for nm members.
+void base_dtg(gatt_ctx* c, const double* S, long long ss, int nm) {
+  const long long np = c->np, ws = c->work_stride;
+  double* D = c->work_ptr(WK_D);
+  double* T = c->work_ptr(WK_T);
+  double* G = c->work_ptr(WK_G);
+  const int ndir = c->is3d ? 3 : 2;
+  for (int d = 0; d < ndir; ++d) {
+    double* in[kMaxPtr];
+    double* out[kMaxPtr];
+    int nf = 0;
+    for (int f : state_fields(c)) {
+      in[nf] = const_cast<double*>(S) + (long long)f * np;
+      out[nf] = D + (long long)(3 * f + d) * np;
+      ++nf;
+    }
+    run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, ss, ws);
+  }
+  ViscArgs va{};
+  va.c = c->fc;
+  va.np = np;
+  va.member_stride = ss;
+  va.work_stride = ws;
+  va.E = S + 4 * np;
+  va.D = D;
+  va.T = T;
+  va.is3d = c->is3d;
+  launch_visc(va, nm, c->stream);
+  const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
+  for (int d = 0; d < ndir; ++d) {
+    double* in[kMaxPtr];
+    double* out[kMaxPtr];
+    int nf = 0;
+    for (int q = 0; q < 4; ++q) {
+      if (!c->is3d && q == 2) continue;
+      in[nf] = T + (long long)tin3[d][q] * np;
+      out[nf] = G + (long long)(4 * d + q) * np;
+      ++nf;
+    }
+    run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, ws, ws);
+  }
+}
+
+// forward assembly of one stage for nm members from member m0 (the tail of run_rhs_stage)
+void forward_assemble(gatt_ctx* c, double* S, double* Q, double* Out, int stage, double dt, int m0, int nm) {
+  AssembleArgs aa{};
+  const long long mo = (long long)m0 * c->slab;
+  aa.c = c->fc;
+  aa.np = c->np;
+  aa.member_stride = c->slab;
+  aa.work_stride = c->work_stride;
+  aa.S = S + mo;
+  aa.Q = Q + mo;
+  aa.Out = Out + mo;
+  aa.Delta = c->slab_ptr(SL_DL) + mo;
+  aa.D = c->work_ptr(WK_D);
+  aa.T = c->work_ptr(WK_T);
+  aa.G = c->work_ptr(WK_G);
+  aa.sums = nullptr;
+  aa.stage = stage;
+  aa.dt = dt;
+  aa.is3d = c->is3d;
+  launch_assemble(aa, nm, c->stream);
+}
+
+// Tangent of one RK stage for members m0..m0+nm-1 at the base stage state
+// S (slab stride); base D, T, G must be in the work arrays.
+void tl_stage(gatt_ctx* c, const double* S, double* dS, double* dQ, double* dOut, int stage, double dt, int m0,
+              int nm) {
+  const long long np = c->np, ls = c->lin_stride, lo = (long long)m0 * ls;
+  double* dD = lin_ptr(c, LN_D) + lo;
+  double* dT = lin_ptr(c, LN_T) + lo;
+  double* dG = lin_ptr(c, LN_G) + lo;
+  const int ndir = c->is3d ? 3 : 2;
+  for (int d = 0; d < ndir; ++d) {
+    double* in[kMaxPtr];
+    double* out[kMaxPtr];
+    int nf = 0;
+    for (int f : state_fields(c)) {
+      in[nf] = dS + lo + (long long)f * np;
+      out[nf] = dD + (long long)(3 * f + d) * np;
+      ++nf;
+    }
+    run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, ls, ls);
+  }
+  ViscLinArgs v{};
+  v.c = c->fc;
+  v.np = np;
+  v.member_stride = c->slab;
+  v.work_stride = c->work_stride;
+  v.lin_stride = ls;
+  v.E = S + (long long)m0 * c->slab + 4 * np;
+  v.D = c->work_ptr(WK_D);
+  v.dE = dS + lo + 4 * np;
+  v.dD = dD;
+  v.dT = dT;
+  v.is3d = c->is3d;
+  launch_visc_tl(v, nm, c->stream);
+  const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
+  for (int d = 0; d < ndir; ++d) {
+    double* in[kMaxPtr];
+    double* out[kMaxPtr];
+    int nf = 0;
+    for (int q = 0; q < 4; ++q) {
+      if (!c->is3d && q == 2) continue;
+      in[nf] = dT + (long long)tin3[d][q] * np;
+      out[nf] = dG + (long long)(4 * d + q) * np;
+      ++nf;
+    }
+    run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, ls, ls);
+  }
+  AssembleLinArgs a{};
+  a.c = c->fc;
+  a.np = np;
+  a.member_stride = c->slab;
+  a.work_stride = c->work_stride;
+  a.lin_stride = ls;
+  a.S = S + (long long)m0 * c->slab;
+  a.D = c->work_ptr(WK_D);
+  a.T = c->work_ptr(WK_T);
+  a.G = c->work_ptr(WK_G);
+  a.dS = dS + lo;
+  a.dD = dD;
+  a.dT = dT;
+  a.dG = dG;
+  a.dQ = dQ + lo;
+  a.dOut = dOut + lo;
+  a.dDelta = lin_ptr(c, LN_DL) + lo;
+  a.stage = stage;
+  a.dt = dt;
+  a.is3d = c->is3d;
+  launch_assemble_tl(a, nm, c->stream);
+}
+
+// Transposed operator of `kind` along `dir` applied to nf fields: out = beta*src
+// + (accumulate ? out : 0) + alpha * R^T (Thomas^T in); the Thomas part runs
+// in place on `in`, which is overwritten.
+void run_solve_t(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double* const* out,
+                 double* const* src, double alpha, double beta, int accumulate, int nm, long long in_stride,
+                 long long out_stride) {
+  DirOps& d = c->dir[dir];
+  SolveArgs a{};
+  a.op = d.op[kind];
+  a.types = d.types;
+  a.dir = dir;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  a.n1 = d.n1;
+  a.nlines = d.nlines;
+  a.stride = d.stride;
+  a.cs = (2 * d.op[kind].nR - 1 + 4 + 1) & ~1;
+  a.nf = nf;
+  a.in_stride = in_stride;
+  a.out_stride = in_stride;
+  for (int f = 0; f < nf; ++f) {
+    a.in[f] = in[f];
+    a.out[f] = in[f];
+  }
+  a.coef = d.coef[kind];
+  a.coef_ends = d.coef_ends[kind];
+  launch_line_solve_t(a, nm, c->stream);
+  BandTArgs b{};
+  b.dir = dir;
+  b.nx = c->nx;
+  b.ny = c->ny;
+  b.nz = c->nz;
+  b.N = d.op[kind].N;
+  b.nR = d.op[kind].nR;
+  b.cs = a.cs;
+  b.np = c->np;
+  b.coef = d.coef[kind];
+  b.types = d.types;
+  b.nf = nf;
+  b.in_stride = in_stride;
+  b.out_stride = out_stride;
+  for (int f = 0; f < nf; ++f) {
+    b.in[f] = in[f];
+    b.out[f] = out[f];
+    b.src[f] = src ? src[f] : nullptr;
+  }
+  b.alpha = alpha;
+  b.beta = beta;
+  b.accumulate = accumulate;
+  launch_band_t(b, nm, c->stream);
+}
+
+// Sbar (lin slab, all members) = J(S)^T kbar for the base stage state S
+// (member 0 pointer, member stride ss); one work group after the other.
+void adj_rhs(gatt_ctx* c, const double* S, long long ss, double* kbar, double* sbar) {
+  const long long np = c->np, ls = c->lin_stride;
+  const int ndir = c->is3d ? 3 : 2;
+  const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
+  for (int m0 = 0; m0 < c->batch; m0 += c->wgroup) {
+    const int nm = std::min(c->wgroup, c->batch - m0);
+    const long long lo = (long long)m0 * ls;
+    base_dtg(c, S + (long long)m0 * ss, ss, nm);
+    double* Db = lin_ptr(c, LN_D) + lo;
+    double* Tb = lin_ptr(c, LN_T) + lo;
+    double* Gb = lin_ptr(c, LN_G) + lo;
+    double* sb = sbar + lo;
+    {
+      AxpbyArgs z{};
+      z.np = np;
+      z.x_stride = ls;
+      z.y_stride = ls;
+      for (int f : state_fields(c)) {
+        z.x[z.nf] = sb + (long long)f * np;
+        z.y[z.nf] = sb + (long long)f * np;
+        ++z.nf;
+      }
+      z.a = 0.0;
+      z.b = 0.0;
+      launch_axpby(z, nm, c->stream);
+    }
+    AssembleLinArgs a{};
+    a.c = c->fc;
+    a.np = np;
+    a.member_stride = ss;
+    a.work_stride = c->work_stride;
+    a.lin_stride = ls;
+    a.S = S + (long long)m0 * ss;
+    a.D = c->work_ptr(WK_D);
+    a.T = c->work_ptr(WK_T);
+    a.G = c->work_ptr(WK_G);
+    a.dbar = kbar + lo;
+    a.Sbar = sb;
+    a.Dbar = Db;
+    a.Tbar = Tb;
+    a.Gbar = Gb;
+    a.is3d = c->is3d;
+    launch_assemble_adj(a, nm, c->stream);
+    // Tbar += d^T Gbar, direction by direction
+    for (int d = 0; d < ndir; ++d) {
+      double* in[kMaxPtr];
+      double* out[kMaxPtr];
+      int nf = 0;
+      for (int q = 0; q < 4; ++q) {
+        if (!c->is3d && q == 2) continue;
+        in[nf] = Gb + (long long)(4 * d + q) * np;
+        out[nf] = Tb + (long long)tin3[d][q] * np;
+        ++nf;
+      }
+      run_solve_t(c, d, GATT_DERIV, nf, in, out, nullptr, 1.0, 0.0, 1, nm, ls, ls);
+    }
+    ViscLinArgs v{};
+    v.c = c->fc;
+    v.np = np;
+    v.member_stride = ss;
+    v.work_stride = c->work_stride;
+    v.lin_stride = ls;
+    v.E = S + (long long)m0 * ss + 4 * np;
+    v.D = c->work_ptr(WK_D);
+    v.dT = Tb;
+    v.Dbar = Db;
+    v.dEbar = sb + 4 * np;
+    v.is3d = c->is3d;
+    launch_visc_adj(v, nm, c->stream);
+    // Sbar += d^T Dbar
+    for (int d = 0; d < ndir; ++d) {
+      double* in[kMaxPtr];
+      double* out[kMaxPtr];
+      int nf = 0;
+      for (int f : state_fields(c)) {
+        in[nf] = Db + (long long)(3 * f + d) * np;
+        out[nf] = sb + (long long)f * np;
+        ++nf;
+      }
+      run_solve_t(c, d, GATT_DERIV, nf, in, out, nullptr, 1.0, 0.0, 1, nm, ls, ls);
+    }
+  }
+}
+
+// One step of the base state with its tangent (main.F90:255-420 and its derivative).
+void enqueue_tlm_step(gatt_ctx* c, double t, double dt) {
+  (void)t;
+  double* q = c->state(c->cur);
+  double* qn = c->state(1 - c->cur);
+  double* S = c->slab_ptr(SL_S);
+  double* dq = lin_ptr(c, LN_Q);
+  double* dS = lin_ptr(c, LN_S);
+  launch_clear_u64(c->d_red, 8 * c->batch, c->stream);
+  for (int stage = 1; stage <= 4; ++stage) {
+    double* Sin = stage == 1 ? q : S;
+    double* Out = stage == 4 ? qn : S;
+    double* dSin = stage == 1 ? dq : dS;
+    double* dOut = stage == 4 ? dq : dS;
+    for (int m0 = 0; m0 < c->batch; m0 += c->wgroup) {
+      const int nm = std::min(c->wgroup, c->batch - m0);
+      base_dtg(c, Sin + (long long)m0 * c->slab, c->slab, nm);
+      tl_stage(c, Sin, dSin, dq, dOut, stage, dt, m0, nm);   // before the base stage state is overwritten
+      forward_assemble(c, Sin, q, Out, stage, dt, m0, nm);
+    }
+  }
+  const double theta = dt / c->filterCharTime;
+  run_filters(c, qn, theta);
+  const int ndir = c->is3d ? 3 : 2;
+  for (int d = 0; d < ndir; ++d) {
+    if (!c->filterDir[d]) continue;
+    double* ptr[kMaxPtr];
+    int nf = 0;
+    for (int f : state_fields(c)) ptr[nf++] = dq + (long long)f * c->np;
+    run_solve(c, d, GATT_FILTER, nf, ptr, ptr, theta, 1, c->batch, c->lin_stride, c->lin_stride);
+  }
+  if (c->sfdType > 0) {
+    run_sfd(c, qn, dt);
+    SfdArgs sa{};
+    sa.np = c->np;
+    sa.member_stride = c->lin_stride;
+    sa.Q = dq;
+    sa.Mean = nullptr;
+    sa.chi = c->sfdType == 2 ? c->d_chi : nullptr;
+    sa.chi_val = c->sfdXval;
+    sa.dt = dt;
+    sa.delta = 0.0;
+    sa.nf_mask = c->is3d ? 0x1f : 0x1b;
+    launch_sfd(sa, c->batch, c->stream);
+  }
+  ReduceArgs r{};
+  r.np = c->np;
+  r.member_stride = c->slab;
+  r.Qn = qn;
+  r.Q = q;
+  r.out = c->d_red;
+  r.is3d = c->is3d;
+  launch_step_reduce(r, c->batch, c->stream);
+  launch_record(c->d_red, c->d_rec, c->d_flag, c->batch, c->stream);
+  c->cur = 1 - c->cur;
+}
+
+// Adjoint of one step about the base state q (unchanged): qbar <- J(q)^T qbar.
+void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
+  (void)t;
+  const long long ls = c->lin_stride, sl = c->slab;
+  double* q = c->state(c->cur);
+  double* S = c->slab_ptr(SL_S);
+  double* QB = lin_ptr(c, LN_Q);
+  double* SB = lin_ptr(c, LN_S);
+  double* KB = lin_ptr(c, LN_DL);
+  double* B1 = lin_ptr(c, LN_B1);
+  double* B2 = lin_ptr(c, LN_B2);
+  double* B3 = lin_ptr(c, LN_B3);
+  // stage states of the base step (timeIntegration.F90:382-487)
+  run_rhs_stage(c, q, q, S, 1, dt);
+  fields_axpby(c, B1, ls, S, sl, 1.0, 0.0);
+  run_rhs_stage(c, S, q, S, 2, dt);
+  fields_axpby(c, B2, ls, S, sl, 1.0, 0.0);
+  run_rhs_stage(c, S, q, S, 3, dt);
+  fields_axpby(c, B3, ls, S, sl, 1.0, 0.0);
+  // SFD with a frozen mean is diagonal
+  if (c->sfdType > 0) {
+    SfdArgs sa{};
+    sa.np = c->np;
+    sa.member_stride = ls;
+    sa.Q = QB;
+    sa.Mean = nullptr;
+    sa.chi = c->sfdType == 2 ? c->d_chi : nullptr;
+    sa.chi_val = c->sfdXval;
+    sa.dt = dt;
+    sa.delta = 0.0;
+    sa.nf_mask = c->is3d ? 0x1f : 0x1b;
+    launch_sfd(sa, c->batch, c->stream);
+  }
+  // filters in reverse order: qbar <- (1-theta) qbar + theta F^T qbar (theta <= 0: F^T qbar)
+  const double theta = dt / c->filterCharTime;
+  const int ndir = c->is3d ? 3 : 2;
+  for (int d = ndir - 1; d >= 0; --d) {
+    if (!c->filterDir[d]) continue;
+    fields_axpby(c, KB, ls, QB, ls, 1.0, 0.0);
+    double* in[kMaxPtr];
+    double* out[kMaxPtr];
+    int nf = 0;
+    for (int f : state_fields(c)) {
+      in[nf] = KB + (long long)f * c->np;
+      out[nf] = QB + (long long)f * c->np;
+      ++nf;
+    }
+    const bool blend = theta > 0.0;
+    run_solve_t(c, d, GATT_FILTER, nf, in, out, blend ? out : nullptr, blend ? theta : 1.0, blend ? 1.0 - theta : 0.0,
+                0, c->batch, ls, ls);
+  }
+  // RK4 (the accumulator form of timeIntegration.F90:382-496), backwards
+  fields_axpby(c, KB, ls, QB, ls, dt / 6.0, 0.0);
+  adj_rhs(c, B3, ls, KB, SB);
+  fields_axpby(c, B3, ls, SB, ls, 1.0, 0.0);          // B3 now accumulates the stage adjoints
+  fields_axpby(c, KB, ls, QB, ls, dt / 3.0, 0.0);
+  fields_axpby(c, KB, ls, SB, ls, dt, 1.0);
+  adj_rhs(c, B2, ls, KB, SB);
+  fields_axpby(c, B3, ls, SB, ls, 1.0, 1.0);
+  fields_axpby(c, KB, ls, QB, ls, dt / 3.0, 0.0);
+  fields_axpby(c, KB, ls, SB, ls, dt / 2.0, 1.0);
+  adj_rhs(c, B1, ls, KB, SB);
+  fields_axpby(c, B3, ls, SB, ls, 1.0, 1.0);
+  fields_axpby(c, KB, ls, QB, ls, dt / 6.0, 0.0);
+  fields_axpby(c, KB, ls, SB, ls, dt / 2.0, 1.0);
+  adj_rhs(c, q, sl, KB, SB);
+  fields_axpby(c, QB, ls, B3, ls, 1.0, 1.0);
+  fields_axpby(c, QB, ls, SB, ls, 1.0, 1.0);
+}
+
+int copy_lin_fields(gatt_ctx* c, int32_t member, const double* const* hq, bool to_dev) {
+  if (member < 0 || member >= c->batch) return fail(GATT_E_INVALID, "member out of range");
+  int rc = ensure_lin(c);
+  if (rc) return rc;
+  double* base = lin_ptr(c, LN_Q) + (long long)member * c->lin_stride;
+  for (int f = 0; f < 5; ++f) {
+    if (!c->is3d && f == GATT_W) continue;
+    if (!hq[f]) return fail(GATT_E_INVALID, "null field pointer");
+    double* dptr = base + (long long)f * c->np;
+    if (to_dev) HIPCHK(hipMemcpyAsync(dptr, hq[f], sizeof(double) * c->np, hipMemcpyHostToDevice, c->stream));
+    else HIPCHK(hipMemcpyAsync((void*)hq[f], dptr, sizeof(double) * c->np, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gatt_set_perturbation(gatt_ctx* c, int32_t member, const double* const* q) {
+  if (check_ctx(c) || !q) return fail(GATT_E_INVALID, "null argument");
+  return copy_lin_fields(c, member, q, true);
+}
+
+int gatt_get_perturbation(gatt_ctx* c, int32_t member, double* const* q) {
+  if (check_ctx(c) || !q) return fail(GATT_E_INVALID, "null argument");
+  int rc = copy_lin_fields(c, member, (const double* const*)q, false);
+  if (rc) return rc;
+  if (!c->is3d && q[GATT_W]) std::memset(q[GATT_W], 0, sizeof(double) * c->np);
+  return GATT_OK;
+}
+
+int gatt_copy_fields(gatt_ctx* c, int32_t member, int32_t slot, int32_t to_ctx, double* dev) {
+  if (check_ctx(c) || !dev) return fail(GATT_E_INVALID, "null argument");
+  if (member < 0 || member >= c->batch) return fail(GATT_E_INVALID, "member out of range");
+  double* base;
+  if (slot == GATT_SLOT_STATE) {
+    base = c->state(c->cur) + (long long)member * c->slab;
+  } else if (slot == GATT_SLOT_PERTURBATION) {
+    int rc = ensure_lin(c);
+    if (rc) return rc;
+    base = lin_ptr(c, LN_Q) + (long long)member * c->lin_stride;
+  } else {
+    return fail(GATT_E_INVALID, "unknown field slot");
+  }
+  const size_t bytes = sizeof(double) * 5 * (size_t)c->np;
+  if (to_ctx) HIPCHK(hipMemcpyAsync(base, dev, bytes, hipMemcpyDeviceToDevice, c->stream));
+  else HIPCHK(hipMemcpyAsync(dev, base, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_line_solve_t(gatt_ctx* c, int32_t dir, int32_t kind, int32_t nfields, double* const* in,
+                      double* const* out) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  if (dir < 0 || dir > 2 || (!c->is3d && dir == 2)) return fail(GATT_E_INVALID, "direction out of range");
+  if (kind != GATT_DERIV && kind != GATT_FILTER) return fail(GATT_E_INVALID, "unknown operator kind");
+  if (nfields < 1 || nfields > kMaxPtr || !in || !out) return fail(GATT_E_INVALID, "bad field list");
+  if (line_solve_t_points_per_lane(c->dir[dir].op[kind].N) == 0)
+    return fail(GATT_E_UNSUPPORTED, "transposed line solves are compiled for lines of at most 512 points");
+  for (int f = 0; f < nfields; ++f)
+    if (!in[f] || !out[f] || in[f] == out[f]) return fail(GATT_E_INVALID, "null or aliasing field pointer");
+  run_solve_t(c, dir, kind, nfields, in, out, nullptr, 1.0, 0.0, 0, 1, 0, 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_tlm_step(gatt_ctx* c, double t, double dt, double* maxchange, int32_t* diverged) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  int rc = lin_supported(c);
+  if (rc) return rc;
+  if ((rc = ensure_lin(c))) return rc;
+  ensure_mean(c);
+  enqueue_tlm_step(c, t, dt);
+  HIPCHK(hipGetLastError());
+  return read_scalars(c, nullptr, maxchange, diverged);
+}
+
+int gatt_adj_step(gatt_ctx* c, double t, double dt) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  int rc = lin_supported(c);
+  if (rc) return rc;
+  if ((rc = ensure_lin(c))) return rc;
+  enqueue_adj_step(c, t, dt);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return GATT_OK;

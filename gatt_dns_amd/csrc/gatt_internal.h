@@ -48,6 +48,7 @@ struct SolveArgs {
   int lds_row;                // LDS words per point row of the tile
   float inv_n;                // 1/N
   int lds_coef_offset;        // wave-segment: LDS word offset of the per-wavefront coefficient slots
+  int cs;                     // doubles per packed coefficient record (transposed solve)
   int nf;                     // fields per member
   long long in_stride;        // doubles between members in the input arrays
   long long out_stride;       // doubles between members in the output arrays
@@ -153,7 +154,7 @@ void launch_forcing_sums(const SumArgs& a, int nmembers, hipStream_t st);
 struct SfdArgs {
   long long np, member_stride;
   double* Q;            // state, 5 arrays
-  double* Mean;         // 5 arrays
+  double* Mean;         // 5 arrays, or nullptr: Q is a perturbation about a frozen mean
   const double* chi;    // field (type 2) or nullptr
   double chi_val;
   double dt, delta;
@@ -172,6 +173,75 @@ void launch_clear_u64(unsigned long long* p, int n, hipStream_t st);
 void launch_step_reduce(const ReduceArgs& a, int nmembers, hipStream_t st);
 void launch_copy(double* dst, const double* src, long long n, hipStream_t st);
 void launch_record(const unsigned long long* red, double* rec, int* flag, int nmembers, hipStream_t st);
+
+// ------------------------------------------------ tangent and adjoint step
+// Transposed Thomas sweeps of one direction (line_solve_t.hip); in may equal out.
+void launch_line_solve_t(const SolveArgs& a, int nmembers, hipStream_t st);
+int line_solve_t_points_per_lane(int N);
+
+// Transposed RHS band along one direction:
+//   out = beta*src + (accumulate ? out : 0) + alpha * R^T in
+struct BandTArgs {
+  int dir, nx, ny, nz, N, nR, cs;
+  long long np;
+  const double* coef;     // packed records [type][point][cs], band first
+  const int* types;       // per line
+  int nf;
+  long long in_stride, out_stride;
+  const double* in[kMaxPtr];
+  double* out[kMaxPtr];
+  const double* src[kMaxPtr];   // or nullptr
+  double alpha, beta;
+  int accumulate;
+};
+void launch_band_t(const BandTArgs& a, int nmembers, hipStream_t st);
+
+// Tangent of visc: dT from base E, D and tangents dE, dD.  Adjoint: accumulates
+// Ebar and Dbar from Tbar (Dbar, Ebar accumulate; Tbar is read).
+struct ViscLinArgs {
+  FlowConst c;
+  long long np, member_stride, work_stride, lin_stride;
+  const double* E;      // base E (state stride)
+  const double* D;      // base derivatives (work stride)
+  const double* dE;     // tangent / adjoint E (lin stride)
+  double* dEbar;        // adjoint: Ebar (lin stride)
+  const double* dD;     // tangent dD (lin stride)
+  double* dT;           // tangent output / adjoint input Tbar (lin stride)
+  double* Dbar;         // adjoint output (lin stride)
+  int is3d;
+};
+void launch_visc_tl(const ViscLinArgs& a, int nmembers, hipStream_t st);
+void launch_visc_adj(const ViscLinArgs& a, int nmembers, hipStream_t st);
+
+// Tangent of the RHS assembly (dd from dS, dD, dT, dG at base S, D, T, G) fused
+// with the linear RK stage update of the tangent state; adjoint: from dbar,
+// accumulates Sbar and writes Dbar, Tbar (stresses; heat fluxes zero) and Gbar.
+struct AssembleLinArgs {
+  FlowConst c;
+  long long np, member_stride, work_stride, lin_stride;
+  const double* S; const double* D; const double* T; const double* G;   // base
+  // tangent
+  const double* dS; const double* dD; const double* dT; const double* dG;
+  const double* dQ; double* dOut; double* dDelta;
+  int stage; double dt;
+  // adjoint
+  const double* dbar;   // 5 arrays (lin stride)
+  double* Sbar;         // 5 arrays, accumulated (lin stride)
+  double* Dbar; double* Tbar; double* Gbar;   // written (lin stride)
+  int is3d;
+};
+void launch_assemble_tl(const AssembleLinArgs& a, int nmembers, hipStream_t st);
+void launch_assemble_adj(const AssembleLinArgs& a, int nmembers, hipStream_t st);
+
+// y[f] = a*x[f] + b*y[f] over nf arrays of np points per member (b == 0 writes a*x)
+struct AxpbyArgs {
+  long long np, x_stride, y_stride;
+  int nf;
+  const double* x[kMaxPtr];
+  double* y[kMaxPtr];
+  double a, b;
+};
+void launch_axpby(const AxpbyArgs& a, int nmembers, hipStream_t st);
 
 // ------------------------------------------------------- stability analysis
 struct PerturbArgs {

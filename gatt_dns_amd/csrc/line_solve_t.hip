@@ -1,0 +1,237 @@
+// Transposed compact line solves: the adjoint of solveSystem{X,Y,Z}[per]
+// (source/Fortran/derivs.F90:556-948) without its RHS band, which is applied
+// afterwards by band_t_kernel (adjoint.hip).
+//
+// The forward solve on rows e0..N-1 (e0 = 1 for periodic lines, else 0) is
+//   d_e = (r_e - fa_e d_{e-1}) fb_e,    x_e = d_e - ce_e x_{e+1}      (L d = r, U x = d)
+// with the packed per-point coefficients fa, fb, ce (line_solve_ws.h).  Its
+// transpose applied to xbar is
+//   z_e   = xbar_e - ce_{e-1} z_{e-1}                                  (U^T z = xbar)
+//   rho_e = (z_e - fa_{e+1} rho_{e+1}) fb_e                            (L^T rho = z)
+// For periodic lines the forward step also formed x_0 = sum_e D_e r_e and
+// corrected r_1 -= A(1) x_0, r_{N-1} -= C(1) x_0; transposed, with
+// t = xbar_0 - A(1) rho_1 - C(1) rho_{N-1}:  rbar_0 = D_0 t and
+// rbar_e = rho_e + D_e t.
+//
+// Layout as the forward wave-segment kernel: lane = line, wavefront w owns
+// points [w*M, w*M+M) of 64 lines, segments are chained by composing affine
+// maps through LDS; x lines go through an LDS tile.  Each lane reads its own
+// line's coefficients, so tiles may mix line types.  The result may overwrite
+// the input (every lane reads its points before any store).
+#include "gatt_internal.h"
+#include "line_solve_ws.h"
+
+namespace gatt {
+
+template <int DIR, int M, bool XLDS>
+__global__ __launch_bounds__(1024) void line_solve_t_kernel(SolveArgs a) {
+  extern __shared__ double sh[];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int G = blockDim.x >> 6;
+  const int N = a.op.N;
+  const int CS = a.cs;
+  const int NW = 2 * a.op.nR - 1;
+  const int line0 = blockIdx.x * 64;
+  const int line = line0 + lane;
+  const int nact = min(64, a.nlines - line0);
+  const bool active = lane < nact;
+  const int f = (int)blockIdx.y % a.nf;
+  const int mem = (int)blockIdx.y / a.nf;
+  const double* in = a.in[f] + (long long)mem * a.in_stride;
+  double* out = a.out[f] + (long long)mem * a.out_stride;
+  const long long stride = a.stride;
+  const int s = w * M;
+  int cnt = N - s;
+  cnt = cnt < 0 ? 0 : (cnt > M ? M : cnt);
+  double* shP = sh;                 // forward maps   [G][64][2]
+  double* shB = sh + 2 * G * 64;    // backward maps  [G][64][2]
+  double* shD = sh + 4 * G * 64;    // periodic partials [G][64]
+  double* tile = sh;
+
+  const int lsafe = active ? line : line0;
+  long long base = 0;
+  if (DIR == 1) {
+    const int k = lsafe / a.nx;
+    base = (long long)(lsafe - k * a.nx) + (long long)a.nx * a.ny * k;
+  } else if (DIR == 2) {
+    base = lsafe;
+  } else if (!XLDS) {
+    base = (long long)a.nx * lsafe;
+  }
+  if (XLDS) {
+    const double* src = in + (long long)a.nx * line0;
+    const int total = nact * N;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+      const int ep = idx - lp * N;
+      tile[ep * 65 + lp] = src[idx];
+    }
+    __syncthreads();
+  }
+  double r[M];
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    const int e = min(s + tt, N - 1);
+    r[tt] = XLDS ? tile[e * 65 + lane] : in[base + (long long)e * stride];
+  }
+  if (XLDS) __syncthreads();   // the tile words are reused by the maps
+
+  const int type = a.types[lsafe];
+  const double* cf = a.coef + (size_t)type * N * CS;   // record of point e at cf + e*CS
+  const int per = a.op.periodic;
+  const int e0 = per;
+
+  // U^T z = xbar
+  double P = 0.0, Q = 1.0;
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    const int e = s + tt;
+    if (tt < cnt && e >= e0) {
+      const double c = e > e0 ? cf[(e - 1) * CS + NW + 2] : 0.0;
+      P = r[tt] - c * P;
+      Q = -c * Q;
+    }
+  }
+  shP[(w * 64 + lane) * 2] = P;
+  shP[(w * 64 + lane) * 2 + 1] = Q;
+  __syncthreads();
+  double zprev = 0.0;
+#pragma unroll
+  for (int k = 0; k < kMaxWaveSegments; ++k)
+    if (k < w) zprev = shP[(k * 64 + lane) * 2] + shP[(k * 64 + lane) * 2 + 1] * zprev;
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    const int e = s + tt;
+    if (tt < cnt && e >= e0) {
+      const double c = e > e0 ? cf[(e - 1) * CS + NW + 2] : 0.0;
+      const double z = r[tt] - c * zprev;
+      r[tt] = z;
+      zprev = z;
+    }
+  }
+
+  // L^T rho = z
+  P = 0.0;
+  Q = 1.0;
+#pragma unroll
+  for (int tt = M - 1; tt >= 0; --tt) {
+    const int e = s + tt;
+    if (tt < cnt && e >= e0) {
+      const double fa1 = e < N - 1 ? cf[(e + 1) * CS + NW] : 0.0;
+      const double fb = cf[e * CS + NW + 1];
+      P = (r[tt] - fa1 * P) * fb;
+      Q = -(fa1 * fb) * Q;
+    }
+  }
+  shB[(w * 64 + lane) * 2] = P;
+  shB[(w * 64 + lane) * 2 + 1] = Q;
+  __syncthreads();
+  double rnext = 0.0;
+#pragma unroll
+  for (int k = kMaxWaveSegments - 1; k >= 0; --k)
+    if (k > w && k < G) rnext = shB[(k * 64 + lane) * 2] + shB[(k * 64 + lane) * 2 + 1] * rnext;
+#pragma unroll
+  for (int tt = M - 1; tt >= 0; --tt) {
+    const int e = s + tt;
+    if (tt < cnt && e >= e0) {
+      const double fa1 = e < N - 1 ? cf[(e + 1) * CS + NW] : 0.0;
+      const double rho = (r[tt] - fa1 * rnext) * cf[e * CS + NW + 1];
+      r[tt] = rho;
+      rnext = rho;
+    }
+  }
+
+  // transposed periodic correction
+  if (per) {
+    const double A1 = a.coef_ends[(size_t)type * 2], C1 = a.coef_ends[(size_t)type * 2 + 1];
+    double part = 0.0;
+#pragma unroll
+    for (int tt = 0; tt < M; ++tt) {
+      const int e = s + tt;
+      if (tt < cnt) {
+        if (e == 0) part = part + r[tt];
+        if (e == 1) part = part - A1 * r[tt];
+        if (e == N - 1) part = part - C1 * r[tt];
+      }
+    }
+    shD[w * 64 + lane] = part;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < kMaxWaveSegments; ++k)
+      if (k < G) t += shD[k * 64 + lane];
+#pragma unroll
+    for (int tt = 0; tt < M; ++tt) {
+      const int e = s + tt;
+      if (tt < cnt) {
+        const double De = cf[e * CS + NW + 3];
+        r[tt] = e == 0 ? De * t : r[tt] + De * t;
+      }
+    }
+  }
+
+  if (XLDS) __syncthreads();   // all maps read before the tile is rewritten
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    if (tt < cnt) {
+      const int e = s + tt;
+      if (XLDS) tile[e * 65 + lane] = r[tt];
+      else if (active) out[base + (long long)e * stride] = r[tt];
+    }
+  }
+  if (XLDS) {
+    __syncthreads();
+    double* dst = out + (long long)a.nx * line0;
+    const int total = nact * N;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+      const int ep = idx - lp * N;
+      dst[idx] = tile[ep * 65 + lp];
+    }
+  }
+}
+
+// x lines are staged through an LDS tile of 65*N words when it fits in 160 KiB
+// of LDS; longer x lines are read in place (strided across lanes).
+constexpr int kMaxTileN = 300;
+
+size_t line_solve_t_lds_bytes(int dir, int N, int M) {
+  const int G = (N + M - 1) / M;
+  const size_t maps = (size_t)5 * G * 64;
+  const size_t t = (dir == 0 && N <= kMaxTileN) ? (size_t)65 * N : 0;
+  return sizeof(double) * (maps > t ? maps : t);
+}
+
+template <int DIR, int M>
+static void launch_t_m(const SolveArgs& a0, int nmembers, hipStream_t st) {
+  SolveArgs a = a0;
+  a.inv_n = 1.0f / (float)a.op.N;
+  const int G = (a.op.N + M - 1) / M;
+  const long long tiles = (a.nlines + 63) / 64;
+  const size_t lds = line_solve_t_lds_bytes(DIR, a.op.N, M);
+  dim3 grid((unsigned)tiles, (unsigned)(a.nf * nmembers));
+  auto k = (DIR == 0 && a.op.N <= kMaxTileN) ? line_solve_t_kernel<DIR, M, DIR == 0>
+                                             : line_solve_t_kernel<DIR, M, false>;
+  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k, grid, dim3(64 * G), lds, st, a);
+}
+
+// M = 16 for lines up to 256 points, 32 up to 512 (at most 16 wavefronts).
+int line_solve_t_points_per_lane(int N) { return N <= 256 ? 16 : (N <= 512 ? 32 : 0); }
+
+void launch_line_solve_t(const SolveArgs& a, int nmembers, hipStream_t st) {
+  const int M = line_solve_t_points_per_lane(a.op.N);
+  if (a.dir == 0) {
+    if (M == 16) launch_t_m<0, 16>(a, nmembers, st);
+    else launch_t_m<0, 32>(a, nmembers, st);
+  } else if (a.dir == 1) {
+    if (M == 16) launch_t_m<1, 16>(a, nmembers, st);
+    else launch_t_m<1, 32>(a, nmembers, st);
+  } else {
+    if (M == 16) launch_t_m<2, 16>(a, nmembers, st);
+    else launch_t_m<2, 32>(a, nmembers, st);
+  }
+}
+
+}  // namespace gatt

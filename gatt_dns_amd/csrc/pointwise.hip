@@ -207,9 +207,10 @@ __global__ __launch_bounds__(256) void sfd_kernel(SfdArgs a) {
   for (int f = 0; f < 5; ++f) {
     if (!(a.nf_mask & (1 << f))) continue;
     const long long o = mo + f * a.np + n;
-    double M = a.Mean[o];
+    // without a mean array the blend acts on a perturbation of a frozen mean
+    double M = a.Mean ? a.Mean[o] : 0.0;
     const double q = a.Q[o];
-    if (a.delta > 0) { M = M + dtD * (q - M); a.Mean[o] = M; }
+    if (a.delta > 0 && a.Mean) { M = M + dtD * (q - M); a.Mean[o] = M; }
     a.Q[o] = q - dtX * (q - M);
   }
 }

@@ -238,3 +238,11 @@ def periodic_rest(nx, ny, nz=1, Re=1000.0, Ma=0.3, Pr=0.71, gamma=1.4, T0=300.0,
     case.state0 = st
     case.dt = rnd(cfl / cfl_dt(st, case.Ma, case.dxmin, nz))
     return case
+
+
+def cavity_sweep(count=16, nx_range=(256, 768), ny_range=(128, 256), re_range=(500.0, 2000.0), Ma=0.5):
+    """Ragged parameter sweep of lid-driven cavities: grid sizes and Reynolds numbers spread linearly."""
+    nxs = np.round(np.linspace(nx_range[0], nx_range[1], count)).astype(int)
+    nys = np.round(np.linspace(ny_range[0], ny_range[1], count)).astype(int)
+    res = np.linspace(re_range[0], re_range[1], count)
+    return [lid_driven(nx=int(a), ny=int(b), Re=float(r), Ma=Ma) for a, b, r in zip(nxs, nys, res)]

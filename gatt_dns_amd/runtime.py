@@ -24,7 +24,12 @@ EXPORTS = ["gatt_last_error", "gatt_device_info", "gatt_create", "gatt_destroy",
            "gatt_apply_bound_cond", "gatt_sfd", "gatt_rhs", "gatt_step", "gatt_advance",
            "gatt_line_solve", "gatt_sync", "gatt_time_kernel", "gatt_set_line_shape",
            "gatt_get_line_shape", "gatt_set_flow_region", "gatt_vector_length", "gatt_set_base",
-           "gatt_perturb", "gatt_collect", "gatt_block_dot", "gatt_block_update"]
+           "gatt_perturb", "gatt_collect", "gatt_block_dot", "gatt_block_update", "gatt_group_create",
+           "gatt_group_destroy", "gatt_group_advance", "gatt_set_perturbation", "gatt_get_perturbation",
+           "gatt_copy_fields", "gatt_tlm_step", "gatt_adj_step", "gatt_line_solve_t"]
+
+SLOT_STATE = 0
+SLOT_PERTURBATION = 1
 
 
 class GattError(RuntimeError):
@@ -71,6 +76,15 @@ def load_library(path=LIB_PATH):
     L.gatt_collect.argtypes = [V, I, I, I, V, C.c_double, V, I64]
     L.gatt_block_dot.argtypes = [V, I64, I, I, V, I64, V, I64, DP]
     L.gatt_block_update.argtypes = [V, I64, I, I, V, I64, DP, C.c_double, C.c_double, V, I64]
+    L.gatt_group_create.argtypes = [C.POINTER(V), I, I, C.POINTER(V)]
+    L.gatt_group_destroy.argtypes = [V]
+    L.gatt_group_advance.argtypes = [V, I, DP, DP, DP, C.POINTER(I)]
+    L.gatt_set_perturbation.argtypes = [V, I, C.POINTER(DP)]
+    L.gatt_get_perturbation.argtypes = [V, I, C.POINTER(DP)]
+    L.gatt_copy_fields.argtypes = [V, I, I, I, V]
+    L.gatt_tlm_step.argtypes = [V, C.c_double, C.c_double, DP, C.POINTER(I)]
+    L.gatt_adj_step.argtypes = [V, C.c_double, C.c_double]
+    L.gatt_line_solve_t.argtypes = [V, I, I, I, C.POINTER(DP), C.POINTER(DP)]
     _lib = L
     return L
 
@@ -252,9 +266,83 @@ class Solver:
             raise GattError("M must be %d x %d" % (k, p))
         _check(self.lib.gatt_block_update(self.ctx, n, k, p, xp, ldx, M.ctypes.data_as(DP), alpha, beta, yp, ldy))
 
+    # -- tangent and adjoint ---------------------------------------------------
+    def set_perturbation(self, q, member=0):
+        arrs = self._arrays(q)
+        _check(self.lib.gatt_set_perturbation(self.ctx, member, field_ptrs(arrs)))
+
+    def get_perturbation(self, member=0):
+        out = [np.zeros(self.shape, order="F") for _ in range(5)]
+        _check(self.lib.gatt_get_perturbation(self.ctx, member, field_ptrs(out)))
+        return dict(zip("UVWRE", out))
+
+    def copy_fields(self, slot, to_ctx, tensor, member=0):
+        """Device copy of the state (slot 0) or perturbation (slot 1) to/from a float64
+        CUDA tensor of 5*nx*ny*nz contiguous values (U,V,W,R,E, each x fastest)."""
+        npts = self.shape[0] * self.shape[1] * self.shape[2]
+        if (str(tensor.dtype) != "torch.float64" or not tensor.is_cuda or not tensor.is_contiguous()
+                or tensor.numel() != 5 * npts):
+            raise GattError("expected a contiguous float64 CUDA tensor of %d values" % (5 * npts))
+        _check(self.lib.gatt_copy_fields(self.ctx, member, slot, 1 if to_ctx else 0, C.c_void_p(tensor.data_ptr())))
+
+    def tlm_step(self, t, dt):
+        """State and tangent advance one step; returns (maxchange, diverged) of the state."""
+        mc = np.zeros(5)
+        dv = C.c_int32(0)
+        _check(self.lib.gatt_tlm_step(self.ctx, t, dt, mc.ctypes.data_as(DP), C.byref(dv)))
+        return mc, bool(dv.value)
+
+    def adj_step(self, t, dt):
+        """Perturbation <- J^T perturbation about the current state (the state is not advanced)."""
+        _check(self.lib.gatt_adj_step(self.ctx, t, dt))
+
+    def line_solve_t(self, direction, kind, fields_in, fields_out):
+        """Transposed line solve (device tensors); fields_in are overwritten (workspace)."""
+        n = len(fields_in)
+        pin = (DP * n)(*[C.cast(C.c_void_p(t.data_ptr()), DP) for t in fields_in])
+        pout = (DP * n)(*[C.cast(C.c_void_p(t.data_ptr()), DP) for t in fields_out])
+        _check(self.lib.gatt_line_solve_t(self.ctx, direction, kind, n, pin, pout))
+
     def line_solve(self, direction, kind, fields_in, fields_out):
         """Device-pointer unit entry point: lists of torch CUDA tensors (float64, Fortran-ordered data)."""
         n = len(fields_in)
         pin = (DP * n)(*[C.cast(C.c_void_p(t.data_ptr()), DP) for t in fields_in])
         pout = (DP * n)(*[C.cast(C.c_void_p(t.data_ptr()), DP) for t in fields_out])
         _check(self.lib.gatt_line_solve(self.ctx, direction, kind, n, pin, pout))
+
+
+class Group:
+    """Ragged group of solvers (different cases) advanced together (gatt_group_*).
+
+    mode 0 replays one HIP graph per double step with a branch per member;
+    mode 1 replays per-member graphs round-robin on the members' streams.
+    """
+
+    def __init__(self, solvers, mode=0):
+        self.lib = load_library()
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s.ctx for s in self.solvers])
+        h = C.c_void_p()
+        _check(self.lib.gatt_group_create(arr, len(self.solvers), mode, C.byref(h)))
+        self.handle = h
+
+    def advance(self, nsteps, t0, dt):
+        n = len(self.solvers)
+        t0 = np.ascontiguousarray(np.broadcast_to(np.asarray(t0, float), (n,)))
+        dt = np.ascontiguousarray(np.asarray(dt, float))
+        mc = np.zeros((n, 5))
+        dv = (C.c_int32 * n)()
+        _check(self.lib.gatt_group_advance(self.handle, nsteps, t0.ctypes.data_as(DP), dt.ctypes.data_as(DP),
+                                           mc.ctypes.data_as(DP), dv))
+        return mc, [bool(v) for v in dv]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.gatt_group_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

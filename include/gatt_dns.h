@@ -288,6 +288,67 @@ int gatt_block_dot(gatt_ctx* ctx, int64_t n, int32_t k, int32_t p, const double*
 int gatt_block_update(gatt_ctx* ctx, int64_t n, int32_t k, int32_t p, const double* X, int64_t ldx, const double* M,
                       double alpha, double beta, double* Y, int64_t ldy);
 
+/* ------------------------------------------------------------------------
+ * Tangent-linear and adjoint steps (no reference counterpart: the
+ * differentiation of the step of main.F90:255-420 as implemented here).
+ * Each member carries a perturbation q' (5 fields, same layout as the state).
+ *   gatt_tlm_step : q <- Phi(q) and q' <- J(q) q'   (J = dPhi/dq at the old q)
+ *   gatt_adj_step : q' <- J(q)^T q'                 (q is not advanced)
+ * The adjoint is the exact transpose of the tangent: RK4 stages, the
+ * Navier-Stokes RHS (nseq.F90), the compact solves as transposed Thomas
+ * sweeps with the transposed periodic correction and the transposed RHS band,
+ * the filter blend and the SFD damping with a frozen mean.  Implemented for
+ * problems without boundary conditions (periodic directions), without the
+ * turbulence forcing and with SFD off or frozen (Delta = inf); other problems
+ * return GATT_E_UNSUPPORTED.
+ * ------------------------------------------------------------------------ */
+enum { GATT_SLOT_STATE = 0, GATT_SLOT_PERTURBATION = 1 };
+
+/* Host copies of member m's perturbation, as gatt_set_state / gatt_get_state. */
+int gatt_set_perturbation(gatt_ctx* ctx, int32_t member, const double* const* q);
+int gatt_get_perturbation(gatt_ctx* ctx, int32_t member, double* const* q);
+
+/* Device-to-device copy of member m's state or perturbation (slot) to or from
+ * a caller buffer of 5*nx*ny*nz doubles (U,V,W,R,E one after the other):
+ * to_ctx != 0 copies dev into the context.  For checkpoints kept on the GPU. */
+int gatt_copy_fields(gatt_ctx* ctx, int32_t member, int32_t slot, int32_t to_ctx, double* dev);
+
+/* Unit entry point: the transpose of gatt_line_solve along `dir` (the
+ * adjoint of solveSystem{X,Y,Z}[per], source/Fortran/derivs.F90:556-948) on
+ * nfields device arrays.  The in arrays are used as workspace and are
+ * overwritten; out may not alias in. */
+int gatt_line_solve_t(gatt_ctx* ctx, int32_t dir, int32_t kind, int32_t nfields, double* const* in,
+                      double* const* out);
+
+/* One step of the state with its tangent (all members); scalars as gatt_step. */
+int gatt_tlm_step(gatt_ctx* ctx, double t, double dt, double* maxchange, int32_t* diverged);
+
+/* Adjoint of one step about the current state (all members). */
+int gatt_adj_step(gatt_ctx* ctx, double t, double dt);
+
+/* ------------------------------------------------------------------------
+ * Ragged groups: independent problems of different grid sizes and
+ * parameters (a parameter sweep, runBatch.m / one DNS process per case in the
+ * reference) advanced together.  Each member is its own context; a group
+ * replays one captured HIP graph per double step in which every member's step
+ * is a parallel branch, so the small members' launches overlap on the GPU.
+ * ------------------------------------------------------------------------ */
+typedef struct gatt_group gatt_group;
+
+/* Group n distinct contexts of one device.  mode 0: one graph with a branch
+ * per member; mode 1: per-member graphs launched round-robin on the members'
+ * own streams.  The contexts stay owned by the caller and must outlive the
+ * group. */
+int gatt_group_create(gatt_ctx* const* ctxs, int32_t n, int32_t mode, gatt_group** out);
+int gatt_group_destroy(gatt_group* group);
+
+/* nsteps fixed-dt steps of every member (member m from time t0[m] with step
+ * dt[m]), each exactly as gatt_advance on that context alone.  maxchange
+ * (n x 5) and diverged (n) receive each member's last-step scalars (may be
+ * NULL). */
+int gatt_group_advance(gatt_group* group, int32_t nsteps, const double* t0, const double* dt, double* maxchange,
+                       int32_t* diverged);
+
 #ifdef __cplusplus
 }
 #endif

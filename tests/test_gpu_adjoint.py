@@ -1,0 +1,159 @@
+"""GPU checks of the tangent-linear and adjoint steps.
+
+* dot-product test <J v, w> = <v, J^T w> within 1e-11 (one step and a
+  checkpointed multi-step propagator), 2-D and 3-D periodic boxes;
+* the tangent against central differences of the oracle's step (the
+  reference's non-linear step restated on the CPU);
+* the transposed line solves against the transposes of the forward solves
+  (dense operators assembled column by column on the GPU);
+* the gradient of E(T) = 1/2 ||q(T)||^2 from torch.autograd against a
+  finite-difference directional derivative.
+"""
+import numpy as np
+import pytest
+
+from gatt_dns_amd.prep import cases
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _case(kind):
+    if kind == "box2d":
+        c = cases.periodic_rest(32, 24)
+        rng = np.random.Generator(np.random.PCG64(1))
+        x = np.arange(32) / 32
+        y = np.arange(24) / 24
+        X, Y = np.meshgrid(x, y, indexing="ij")
+        q = {k: np.array(v, copy=True) for k, v in c.state0.items()}
+        q["U"] = q["U"] + (0.3 * np.sin(2 * np.pi * Y) + 0.05 * rng.standard_normal((32, 24)))[:, :, None]
+        q["V"] = q["V"] + (0.2 * np.cos(2 * np.pi * X))[:, :, None]
+        q["R"] = q["R"] + 0.05 * np.cos(2 * np.pi * (X + Y))[:, :, None]
+        q["E"] = q["E"] * (1 + 0.02 * np.sin(4 * np.pi * X))[:, :, None]
+        return c, q
+    c = cases.turbulence_box(16, seed=3, forcing=False)
+    q = {k: np.array(v, copy=True) for k, v in c.state0.items()}
+    q["R"] = q["R"] + 0.03 * np.sin(2 * np.pi * np.arange(16) / 16)[None, :, None]
+    return c, q
+
+
+def _rand(prop, seed):
+    import torch
+    rng = np.random.Generator(np.random.PCG64(seed))
+    v = rng.standard_normal(prop.shape)
+    if prop.case.nz == 1:
+        v[2] = 0.0
+    return torch.from_numpy(v).cuda()
+
+
+@pytest.mark.parametrize("kind", ["box2d", "box3d"])
+def test_dot_product_one_step(torch_cuda, kind):
+    from gatt_dns_amd.adjoint import Propagator, state_tensor
+    case, q = _case(kind)
+    prop = Propagator(case)
+    q0 = state_tensor(case, q)
+    for seed in (1, 2):
+        v, w = _rand(prop, seed), _rand(prop, seed + 10)
+        _, Jv = prop.tangent(q0, v)
+        JTw = prop.adjoint(q0, w)
+        a = float((Jv * w).sum())
+        b = float((v * JTw).sum())
+        assert abs(a - b) <= 1e-11 * max(abs(a), abs(b)), (a, b, abs(a - b) / abs(a))
+    prop.close()
+
+
+@pytest.mark.parametrize("kind", ["box2d", "box3d"])
+def test_tangent_matches_oracle_differences(torch_cuda, kind, oracle_mod):
+    from gatt_dns_amd.adjoint import Propagator, state_tensor
+    case, q = _case(kind)
+    prop = Propagator(case)
+    q0 = state_tensor(case, q)
+    v = _rand(prop, 5)
+    _, Jv = prop.tangent(q0, v)
+    Jv = Jv.cpu().numpy()
+    o = oracle_mod.Oracle(case)
+    vh = v.cpu().numpy()
+    eps = 1e-5
+    outs = []
+    for sgn in (1.0, -1.0):
+        qq = {k: np.asfortranarray(q[k] + sgn * eps * np.transpose(vh[i], (2, 1, 0)))
+              for i, k in enumerate("UVWRE")}
+        st, _, _, _ = o.step(qq, 0.0, case.dt)
+        outs.append(np.stack([np.transpose(st[k], (2, 1, 0)) for k in "UVWRE"]))
+    fd = (outs[0] - outs[1]) / (2 * eps)
+    if case.nz == 1:
+        fd[2] = 0.0
+    rel = np.linalg.norm(Jv - fd) / np.linalg.norm(fd)
+    assert rel < 1e-7, rel
+    prop.close()
+
+
+@pytest.mark.parametrize("name", ["periodic", "cavity"])
+def test_transposed_solves_are_transposes(torch_cuda, name):
+    """Dense operators: gatt_line_solve_t equals the transpose of gatt_line_solve (deriv, filter; x, y)."""
+    from gatt_dns_amd.runtime import Solver
+    torch = torch_cuda
+    case = cases.periodic_rest(12, 14) if name == "periodic" else cases.lid_driven(nx=15, ny=13)
+    s = Solver(case)
+    n = case.nx * case.ny
+    for d in (0, 1):
+        for kind in (0, 1):
+            A = np.zeros((n, n))
+            AT = np.zeros((n, n))
+            for j in range(n):
+                e = torch.zeros(n, dtype=torch.float64, device="cuda")
+                e[j] = 1.0
+                out = torch.zeros_like(e)
+                s.line_solve(d, kind, [e.clone()], [out])
+                A[:, j] = out.cpu().numpy()
+                out_t = torch.zeros_like(e)
+                s.line_solve_t(d, kind, [e.clone()], [out_t])
+                AT[:, j] = out_t.cpu().numpy()
+            assert np.abs(A).max() > 0.1
+            np.testing.assert_allclose(AT, A.T, rtol=0, atol=1e-12 * np.abs(A).max())
+    s.close()
+
+
+@pytest.mark.parametrize("kind", ["box2d", "box3d"])
+def test_checkpointed_propagator_dot_product_and_gradient(torch_cuda, kind):
+    import torch
+    from gatt_dns_amd.adjoint import Propagator, advance, optimal_energy_gradient, state_tensor
+    case, q = _case(kind)
+    prop = Propagator(case)
+    q0 = state_tensor(case, q)
+    nsteps, every = 7, 3
+    # J = 1/2 ||q(T) - q_ref||^2 through torch.autograd
+    qref = q0.clone()
+    x = q0.clone().requires_grad_(True)
+    qT = advance(prop, x, nsteps, checkpoint_every=every)
+    J = 0.5 * ((qT - qref) ** 2).sum()
+    J.backward()
+    g = x.grad.detach()
+    v = _rand(prop, 9) * 1e-3
+    eps = 1e-4
+
+    def Jof(z):
+        with torch.no_grad():
+            out = advance(prop, z, nsteps, checkpoint_every=every)
+            return float(0.5 * ((out - qref) ** 2).sum())
+
+    fd = (Jof(q0 + eps * v) - Jof(q0 - eps * v)) / (2 * eps)
+    an = float((g * v).sum())
+    assert abs(fd - an) <= 1e-7 * abs(an), (fd, an)
+    # the linearised propagator: <J^n v, w> = <v, (J^n)^T w> through the tangent and the adjoint
+    w = _rand(prop, 11)
+    _, Jv = prop.tangent(q0, v, n=nsteps)
+    E, lam = optimal_energy_gradient(prop, q0, v, nsteps, checkpoint_every=every)
+    assert abs(E - 0.5 * float((Jv * Jv).sum())) <= 1e-12 * E
+    # d E / d v = (J^n)^T J^n v, so <lam, v> = ||J^n v||^2
+    a = float((lam * v).sum())
+    b = float((Jv * Jv).sum())
+    assert abs(a - b) <= 1e-11 * abs(b), (a, b)
+    prop.close()
