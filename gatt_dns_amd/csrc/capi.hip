@@ -117,6 +117,10 @@ struct gatt_ctx {
   // tangent / adjoint arrays, per member (allocated on first use)
   double* lin = nullptr;
   long long lin_stride = 0;
+  // adjoint: derivatives, stresses and their derivatives of the four RK stages of
+  // the base step, per member (allocated on first use)
+  double* tape = nullptr;
+  long long tape_stride = 0;
   // stability analysis
   long long nflow = 0;                  // flow-region points (0: not set, every point)
   int* d_idx = nullptr;                 // region index -> point
@@ -837,6 +841,7 @@ int gatt_destroy(gatt_ctx* c) {
   if (c->d_base) hipFree(c->d_base);
   if (c->d_scratch) hipFree(c->d_scratch);
   if (c->lin) hipFree(c->lin);
+  if (c->tape) hipFree(c->tape);
   if (c->mem) hipFree(c->mem);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -1540,6 +1545,19 @@ int ensure_lin(gatt_ctx* c) {
 
 double* lin_ptr(gatt_ctx* c, int idx) { return c->lin + (long long)idx * c->np; }
 
+int ensure_tape(gatt_ctx* c) {
+  if (c->tape) return GATT_OK;
+  c->tape_stride = 4LL * WK_N * c->np;
+  void* p = nullptr;
+  if (hipMalloc(&p, sizeof(double) * (size_t)c->tape_stride * c->batch) != hipSuccess)
+    return fail(GATT_E_NOMEM, "adjoint stage tape allocation failed");
+  c->tape = static_cast<double*>(p);
+  return GATT_OK;
+}
+
+// derivative array `wk` (WK_D, WK_T, WK_G) of RK stage s (0..3) in the tape, member 0
+double* tape_ptr(gatt_ctx* c, int s, int wk) { return c->tape + ((long long)s * WK_N + wk) * c->np; }
+
 int lin_supported(gatt_ctx* c) {
   if (!c->ops.empty())
     return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps are implemented for problems without boundary "
@@ -1576,12 +1594,9 @@ void fields_axpby(gatt_ctx* c, double* y, long long ys, const double* x, long lo
 }
 
 // Base derivatives, stresses and their derivatives at S (member m0 pointer,
-// member stride ss) into the work arrays, for nm members.
-void base_dtg(gatt_ctx* c, const double* S, long long ss, int nm) {
-  const long long np = c->np, ws = c->work_stride;
-  double* D = c->work_ptr(WK_D);
-  double* T = c->work_ptr(WK_T);
-  double* G = c->work_ptr(WK_G);
+// member stride ss) into D, T, G (member stride ws), for nm members.
+void base_dtg(gatt_ctx* c, const double* S, long long ss, int nm, double* D, double* T, double* G, long long ws) {
+  const long long np = c->np;
   const int ndir = c->is3d ? 3 : 2;
   for (int d = 0; d < ndir; ++d) {
     double* in[kMaxPtr];
@@ -1619,21 +1634,23 @@ void base_dtg(gatt_ctx* c, const double* S, long long ss, int nm) {
   }
 }
 
-// forward assembly of one stage for nm members from member m0 (the tail of run_rhs_stage)
-void forward_assemble(gatt_ctx* c, double* S, double* Q, double* Out, int stage, double dt, int m0, int nm) {
+// forward assembly of one stage for nm members from member m0 (the tail of
+// run_rhs_stage) from the derivative arrays D, T, G (member stride ws)
+void forward_assemble(gatt_ctx* c, double* S, double* Q, double* Out, int stage, double dt, int m0, int nm,
+                      const double* D, const double* T, const double* G, long long ws) {
   AssembleArgs aa{};
   const long long mo = (long long)m0 * c->slab;
   aa.c = c->fc;
   aa.np = c->np;
   aa.member_stride = c->slab;
-  aa.work_stride = c->work_stride;
+  aa.work_stride = ws;
   aa.S = S + mo;
   aa.Q = Q + mo;
   aa.Out = Out + mo;
   aa.Delta = c->slab_ptr(SL_DL) + mo;
-  aa.D = c->work_ptr(WK_D);
-  aa.T = c->work_ptr(WK_T);
-  aa.G = c->work_ptr(WK_G);
+  aa.D = D;
+  aa.T = T;
+  aa.G = G;
   aa.sums = nullptr;
   aa.stage = stage;
   aa.dt = dt;
@@ -1764,15 +1781,16 @@ void run_solve_t(gatt_ctx* c, int dir, int kind, int nf, double* const* in, doub
 }
 
 // Sbar (lin slab, all members) = J(S)^T kbar for the base stage state S
-// (member 0 pointer, member stride ss); one work group after the other.
-void adj_rhs(gatt_ctx* c, const double* S, long long ss, double* kbar, double* sbar) {
+// (member 0 pointer, member stride ss) whose derivatives, stresses and their
+// derivatives are in D, T, G (member stride ws).
+void adj_rhs(gatt_ctx* c, const double* S, long long ss, double* kbar, double* sbar, const double* Dw,
+             const double* Tw, const double* Gw, long long ws) {
   const long long np = c->np, ls = c->lin_stride;
   const int ndir = c->is3d ? 3 : 2;
   const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
-  for (int m0 = 0; m0 < c->batch; m0 += c->wgroup) {
-    const int nm = std::min(c->wgroup, c->batch - m0);
-    const long long lo = (long long)m0 * ls;
-    base_dtg(c, S + (long long)m0 * ss, ss, nm);
+  {
+    const int m0 = 0, nm = c->batch;
+    const long long lo = 0;
     double* Db = lin_ptr(c, LN_D) + lo;
     double* Tb = lin_ptr(c, LN_T) + lo;
     double* Gb = lin_ptr(c, LN_G) + lo;
@@ -1795,12 +1813,12 @@ void adj_rhs(gatt_ctx* c, const double* S, long long ss, double* kbar, double* s
     a.c = c->fc;
     a.np = np;
     a.member_stride = ss;
-    a.work_stride = c->work_stride;
+    a.work_stride = ws;
     a.lin_stride = ls;
     a.S = S + (long long)m0 * ss;
-    a.D = c->work_ptr(WK_D);
-    a.T = c->work_ptr(WK_T);
-    a.G = c->work_ptr(WK_G);
+    a.D = Dw;
+    a.T = Tw;
+    a.G = Gw;
     a.dbar = kbar + lo;
     a.Sbar = sb;
     a.Dbar = Db;
@@ -1825,10 +1843,10 @@ void adj_rhs(gatt_ctx* c, const double* S, long long ss, double* kbar, double* s
     v.c = c->fc;
     v.np = np;
     v.member_stride = ss;
-    v.work_stride = c->work_stride;
+    v.work_stride = ws;
     v.lin_stride = ls;
     v.E = S + (long long)m0 * ss + 4 * np;
-    v.D = c->work_ptr(WK_D);
+    v.D = Dw;
     v.dT = Tb;
     v.Dbar = Db;
     v.dEbar = sb + 4 * np;
@@ -1865,9 +1883,11 @@ void enqueue_tlm_step(gatt_ctx* c, double t, double dt) {
     double* dOut = stage == 4 ? dq : dS;
     for (int m0 = 0; m0 < c->batch; m0 += c->wgroup) {
       const int nm = std::min(c->wgroup, c->batch - m0);
-      base_dtg(c, Sin + (long long)m0 * c->slab, c->slab, nm);
+      base_dtg(c, Sin + (long long)m0 * c->slab, c->slab, nm, c->work_ptr(WK_D), c->work_ptr(WK_T),
+               c->work_ptr(WK_G), c->work_stride);
       tl_stage(c, Sin, dSin, dq, dOut, stage, dt, m0, nm);   // before the base stage state is overwritten
-      forward_assemble(c, Sin, q, Out, stage, dt, m0, nm);
+      forward_assemble(c, Sin, q, Out, stage, dt, m0, nm, c->work_ptr(WK_D), c->work_ptr(WK_T), c->work_ptr(WK_G),
+                       c->work_stride);
     }
   }
   const double theta = dt / c->filterCharTime;
@@ -1918,13 +1938,20 @@ void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
   double* B1 = lin_ptr(c, LN_B1);
   double* B2 = lin_ptr(c, LN_B2);
   double* B3 = lin_ptr(c, LN_B3);
-  // stage states of the base step (timeIntegration.F90:382-487)
-  run_rhs_stage(c, q, q, S, 1, dt);
+  // stage states of the base step (timeIntegration.F90:382-487); the derivatives,
+  // stresses and their derivatives of every stage go to the tape for the adjoint
+  const long long ts = c->tape_stride;
+  const int nb = c->batch;
+  base_dtg(c, q, sl, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
+  forward_assemble(c, q, q, S, 1, dt, 0, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
   fields_axpby(c, B1, ls, S, sl, 1.0, 0.0);
-  run_rhs_stage(c, S, q, S, 2, dt);
+  base_dtg(c, S, sl, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
+  forward_assemble(c, S, q, S, 2, dt, 0, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
   fields_axpby(c, B2, ls, S, sl, 1.0, 0.0);
-  run_rhs_stage(c, S, q, S, 3, dt);
+  base_dtg(c, S, sl, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
+  forward_assemble(c, S, q, S, 3, dt, 0, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
   fields_axpby(c, B3, ls, S, sl, 1.0, 0.0);
+  base_dtg(c, S, sl, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
   // SFD with a frozen mean is diagonal
   if (c->sfdType > 0) {
     SfdArgs sa{};
@@ -1959,19 +1986,19 @@ void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
   }
   // RK4 (the accumulator form of timeIntegration.F90:382-496), backwards
   fields_axpby(c, KB, ls, QB, ls, dt / 6.0, 0.0);
-  adj_rhs(c, B3, ls, KB, SB);
+  adj_rhs(c, B3, ls, KB, SB, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
   fields_axpby(c, B3, ls, SB, ls, 1.0, 0.0);          // B3 now accumulates the stage adjoints
   fields_axpby(c, KB, ls, QB, ls, dt / 3.0, 0.0);
   fields_axpby(c, KB, ls, SB, ls, dt, 1.0);
-  adj_rhs(c, B2, ls, KB, SB);
+  adj_rhs(c, B2, ls, KB, SB, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
   fields_axpby(c, B3, ls, SB, ls, 1.0, 1.0);
   fields_axpby(c, KB, ls, QB, ls, dt / 3.0, 0.0);
   fields_axpby(c, KB, ls, SB, ls, dt / 2.0, 1.0);
-  adj_rhs(c, B1, ls, KB, SB);
+  adj_rhs(c, B1, ls, KB, SB, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
   fields_axpby(c, B3, ls, SB, ls, 1.0, 1.0);
   fields_axpby(c, KB, ls, QB, ls, dt / 6.0, 0.0);
   fields_axpby(c, KB, ls, SB, ls, dt / 2.0, 1.0);
-  adj_rhs(c, q, sl, KB, SB);
+  adj_rhs(c, q, sl, KB, SB, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
   fields_axpby(c, QB, ls, B3, ls, 1.0, 1.0);
   fields_axpby(c, QB, ls, SB, ls, 1.0, 1.0);
 }
@@ -2061,6 +2088,7 @@ int gatt_adj_step(gatt_ctx* c, double t, double dt) {
   int rc = lin_supported(c);
   if (rc) return rc;
   if ((rc = ensure_lin(c))) return rc;
+  if ((rc = ensure_tape(c))) return rc;
   enqueue_adj_step(c, t, dt);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
