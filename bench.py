@@ -27,6 +27,10 @@ METRIC = "FP64 Mpoint-steps/s + %HBM roofline; batched-LST & adjoint cost vs pla
 HBM_PEAK_GBPS = 8000.0
 MODEL_TOUCHES_3D = 377   # minimum-pass pencil schedule, SFD off
 
+# kernel symbols of the classes as rocprofv3 names them (profiles/*_hbm_bytes.json)
+CLASS_SYMBOL = {"assemble": "gatt::assemble_kernel(gatt::AssembleArgs)", "visc": "gatt::visc_kernel(gatt::ViscArgs)"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "turbulence_box_256_hbm_bytes.json")
+
 KERNELS = {0: "deriv_x", 1: "deriv_y", 2: "deriv_z", 3: "filter_x", 4: "filter_y", 5: "filter_z",
            6: "visc", 7: "assemble"}
 # launches of each class in one step: derivatives 2 per stage, filters 1 per step
@@ -247,6 +251,13 @@ def main():
     for name, v in per_class.items():
         log("  %-9s %.4f ms/launch  %.0f GB/s algorithmic  %.3f ms/step"
             % (name, v["ms"], v["bytes"] / (v["ms"] * 1e-3) / 1e9, v["share_ms_per_step"]))
+    # measured HBM bytes per launch of the dominant kernel (rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+    # the same workload, calibrated against a device copy; committed under profiles/)
+    traffic = None
+    if args.n == 256 and dom in CLASS_SYMBOL and os.path.exists(TRAFFIC_FILE):
+        rec = json.load(open(TRAFFIC_FILE))["kernels"].get(CLASS_SYMBOL[dom])
+        if rec:
+            traffic = rec["read_bytes_per_dispatch"] + rec["write_bytes_per_dispatch"]
     shapes_out = {"%s_%s" % ("xyz"[d], "deriv" if k == 0 else "filter"): list(s.get_line_shape(d, k))
                   for d in range(3) for k in (0, 1)}
     step_bytes = 8.0 * MODEL_TOUCHES_3D * case.npoints
@@ -300,7 +311,8 @@ def main():
                                "(alpha 0.49) every step, turbulenceGenerator forcing, fixed-dt RK4" % args.n,
                    "grid": [args.n] * 3, "dt": dt, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/turbulence_box_256_hbm_bytes.json" if traffic else None,
                      "bytes_per_launch": d["bytes"], "ms_per_launch": round(d["ms"], 5)},
         "step_roofline": {"model_bytes_per_point_step": 8 * MODEL_TOUCHES_3D, "achieved": round(step_gbps, 1),
                           "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(step_gbps / HBM_PEAK_GBPS, 4)},
