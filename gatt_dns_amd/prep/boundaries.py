@@ -230,6 +230,62 @@ def lid_driven_flow(nx, ny, nz, E0, P0, t_wall_relative=None):
     return fr, walls, corners, bc
 
 
+def boundary_layer_isothermal(X, Y, Z, E0, P0, z_periodic=True, t_wall_relative=None):
+    """BC list of ``source/boundaries/boundaryLayerIsothermal.m`` (no cavities or roughness).
+
+    Returns ``(flow_region, walls, corners, bc, x_break_point, hold_inlet)``:
+    ``x_break_point`` is the row ``[position, a1, a2, b1, b2]`` of ``mesh.x.breakPoint``
+    (``:363-369``) or None, ``hold_inlet`` is True when the domain starts
+    downstream of the leading edge and the inflow is held (``:53-67``).
+    """
+    nx, ny, nz = len(X), len(Y), len(Z)
+    bc = BCList()
+    full = lambda i1, i2, j1, j2, k1, k2: [i1, i2, j1, j2, k1, k2]
+    hold = False
+    if X[0] <= 0:                                         # inflow, :4-52
+        for var, val in (("u", 1.0), ("v", 0.0), ("w", 0.0), ("e", E0)):
+            bc.add(var, "dir", "xi", val, full(1, 1, 1, ny, 1, nz))
+    else:
+        hold = True
+    bc.add("p", "neu", "xi", 0.0, full(1, 1, 1, ny, 1, nz))
+    for var in "uvwe":                                    # outflow, :82-141
+        bc.add(var, "sec", "xf", 0.0, full(nx, nx, 1, ny, 1, nz))
+    bc.add("p", "dir", "xf", P0, full(nx, nx, 1, ny, 1, nz))
+    for var in "uvwep":                                   # top, :143-202
+        bc.add(var, "sec", "yf", 0.0, full(1, nx, ny, ny, 1, nz))
+    if nz > 1 and not z_periodic:                         # spanwise walls, :204-326
+        for d, k in (("zi", 1), ("zf", nz)):
+            for var in "uvwep":
+                bc.add(var, "neu", d, 0.0, full(1, nx, 1, ny, k, k))
+    fr = np.ones((nx, ny, nz), dtype=bool)               # :328-332
+    wall_j = int(np.argmin(np.abs(np.asarray(Y)))) + 1
+    fr[:, :wall_j, :] = False
+    fr, walls, corners, inside = find_walls_for_boundaries(fr)
+    bp = None
+    if X[0] < 0:                                          # :363-369
+        i0 = int(np.flatnonzero(np.asarray(X) >= 0)[0]) + 1
+        i_last = int(np.flatnonzero(np.asarray(X) < 0)[-1]) + 1
+        j0 = int(np.flatnonzero(np.asarray(Y) >= 0)[0]) + 1
+        for w in walls["up"]:
+            w[0] = max(w[0], i0)
+        walls["up"].append([1, i_last, j0, j0, 1, nz])
+        bp = [i_last, j0, j0, 1, nz]
+    for key, d in (("front", "xi"), ("back", "xf"), ("up", "yi"), ("down", "yf"), ("right", "zi"), ("left", "zf")):
+        for box in walls[key]:                            # :371-402
+            bc.add("p", "neu", d, 0.0, box)
+    if X[0] < 0:                                          # free slip upstream of the plate, :404-432
+        bc.add("u", "neu", "yi", 0.0, walls["up"][-1])
+        bc.add("w", "neu", "yi", 0.0, walls["up"][-1])
+    e_wall = E0 * t_wall_relative if t_wall_relative is not None else E0
+    for box in inside:                                    # :434-492
+        bc.add("p", "dir", "yi", P0, box)
+        bc.add("u", "dir", "yi", 0.0, box)
+        bc.add("v", "dir", "yi", 0.0, box)
+        bc.add("w", "dir", "yi", 0.0, box)
+        bc.add("e", "dir", "yi", e_wall, box)
+    return fr, walls, corners, bc, bp, hold
+
+
 def periodic_box(nx, ny, nz):
     """``source/boundaries/periodicBox.m``: all flow, no walls."""
     fr, walls, corners, inside = find_walls_for_boundaries(np.ones((nx, ny, nz), dtype=bool))

@@ -14,9 +14,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .boundaries import (BoundaryInfo, calc_sfd_region, disturbance_index, init_boundaries,
+from .boundaries import (BoundaryInfo, boundary_layer_isothermal, calc_sfd_region, disturbance_index, init_boundaries,
                          lid_driven_flow, periodic_box)
-from .matrices import DirMesh, LineCoeffs, NumMethods, fortran_round, make_matrices, prepare_thomas
+from .matrices import BufferSide, DirMesh, LineCoeffs, NumMethods, fortran_round, make_matrices, prepare_thomas
 
 FIELDS = ("U", "V", "W", "R", "E")
 
@@ -246,3 +246,108 @@ def cavity_sweep(count=16, nx_range=(256, 768), ny_range=(128, 256), re_range=(5
     nys = np.round(np.linspace(ny_range[0], ny_range[1], count)).astype(int)
     res = np.linspace(re_range[0], re_range[1], count)
     return [lid_driven(nx=int(a), ny=int(b), Re=float(r), Ma=Ma) for a, b, r in zip(nxs, nys, res)]
+
+
+def _buffer_points(x_edge, h0, n, growth, sign):
+    """n buffer points beyond x_edge whose spacing rises from h0 to growth*h0 along a sigmoid."""
+    k = np.arange(1, n + 1)
+    ramp = 1.0 / (1.0 + np.exp(-(k - 0.5 * n) / (0.12 * n)))
+    ramp = (ramp - ramp[0]) / (ramp[-1] - ramp[0])
+    h = h0 * (1.0 + (growth - 1.0) * ramp)
+    return x_edge + sign * np.cumsum(h)
+
+
+def blasius(eta_max=20.0, n=4001):
+    """Blasius solution of f''' + f f''/2 = 0, f(0) = f'(0) = 0, f'(inf) = 1 (RK4 shooting on f''(0))."""
+    h = eta_max / (n - 1)
+    rhs = lambda y: np.array([y[1], y[2], -0.5 * y[0] * y[2]])
+
+    def integrate(s):
+        y = np.array([0.0, 0.0, s])
+        out = np.zeros((n, 3))
+        out[0] = y
+        for i in range(1, n):
+            k1 = rhs(y)
+            k2 = rhs(y + h / 2 * k1)
+            k3 = rhs(y + h / 2 * k2)
+            k4 = rhs(y + h * k3)
+            y = y + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+            out[i] = y
+        return out
+
+    lo, hi = 0.2, 0.5
+    for _ in range(50):
+        mid = 0.5 * (lo + hi)
+        if integrate(mid)[-1, 1] > 1.0:
+            hi = mid
+        else:
+            lo = mid
+    return np.linspace(0.0, eta_max, n), integrate(0.5 * (lo + hi))
+
+
+def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wall_ratio=1.02, Re=1000.0,
+                   Ma=0.5, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0.49, spatial="SL4", cfl=0.5, sfd_chi=0.02,
+                   sfd_delta=10.0, noise=1e-5, seed=20261, buffer_growth=8.0):
+    """2-D isothermal-wall boundary layer (boundaryLayerIsothermal.m) on a stretched mesh with buffers.
+
+    The mesh is built here (a 'file' mesh for the reference): x uniform on x_range with nbuf-point
+    buffers at both ends, y geometric from the wall with ratio ``wall_ratio`` up to ``y_top`` and an
+    nbuf-point buffer above; buffer spacings grow along a sigmoid to ``buffer_growth`` times the edge
+    spacing.  Numerics as parameters.m: SL4 with EX4 in the buffers, 'reducedOrder' filter borders with
+    the x-start, x-end and y-end closures off, SFD type 2 (chi in the x buffers only, applyY = false).
+    Initial flow: Blasius velocity scaled so that the displacement thickness is 1 at x = 0 (uniform flow
+    upstream of the plate), uniform R and E, plus seeded N(0, noise^2) on u and v in the flow region.
+    """
+    npx, npy = nx - 2 * nbuf, ny - nbuf
+    Xp = np.linspace(x_range[0], x_range[1], npx)
+    dx = Xp[1] - Xp[0]
+    X = np.concatenate([_buffer_points(Xp[0], dx, nbuf, buffer_growth, -1.0)[::-1], Xp,
+                        _buffer_points(Xp[-1], dx, nbuf, buffer_growth, 1.0)])
+    dy0 = y_top * (wall_ratio - 1.0) / (wall_ratio ** (npy - 1) - 1.0)
+    Yp = np.concatenate([[0.0], np.cumsum(dy0 * wall_ratio ** np.arange(npy - 1))])
+    Yp[-1] = y_top
+    Y = np.concatenate([Yp, _buffer_points(Yp[-1], Yp[-1] - Yp[-2], nbuf, buffer_growth, 1.0)])
+    Z = np.array([0.0])
+    E0 = 1.0 / ((gamma ** 2 - gamma) * Ma ** 2)
+    P0 = (gamma - 1) * E0
+    fr, walls, corners, bc, bp, hold = boundary_layer_isothermal(X, Y, Z, E0, P0)
+    buf = BufferSide(n=nbuf, upwind=False, transition=0.2)
+    meshes = [DirMesh(X, buffer_i=buf, buffer_f=buf, break_points=[bp] if bp else []),
+              DirMesh(Y, buffer_f=buf),
+              DirMesh(Z, periodic=True, fix_periodic_domain_size=True, xf=1.0)]
+    nm = NumMethods(spatial_derivs=spatial, spatial_derivs_buffer="EX4", filter_strength=alpha_f,
+                    filter_borders="reducedOrder", filter_borders_start=(False, True, True),
+                    filter_borders_end=(False, False, True))
+    deriv, filt, types = _coeffs(meshes, fr, nm)
+    bi = init_boundaries(bc, corners, nm.neumann_order, nm.neumann2_order)
+    rnd = lambda v: float(fortran_round(v))
+    chi = calc_sfd_region(nx, ny, 1, ((nbuf, nbuf), (0, nbuf), (0, 0)), sfd_chi, apply=(2, 0, 2))
+    case = Case("boundary_layer_%dx%d" % (nx, ny), nx, ny, 1, X, Y, Z, rnd(Re), rnd(Ma), rnd(Pr), rnd(gamma),
+                rnd(T0), deriv, filt, types, (1, 1, 1), -1.0, bi, [], [],
+                sfd_type=2, sfd_delta=rnd(sfd_delta), sfd_x_val=rnd(sfd_chi),
+                sfd_x=np.asfortranarray(fortran_round(chi)), dxmin=_dxmin(X, Y, Z, 1))
+    case.flow_region = fr
+    case.buffers = ((nbuf, nbuf), (0, nbuf), (0, 0))
+    eta, sol = blasius()
+    x0 = Re / 1.7207876573 ** 2                 # displacement thickness 1 at x = 0
+    xe = np.maximum(X, 0.0)[:, None] + x0
+    et = Y[None, :] * np.sqrt(Re / xe)
+    fp = np.interp(et, eta, sol[:, 1], right=1.0)
+    f = np.where(et <= eta[-1], np.interp(et, eta, sol[:, 0]), sol[-1, 0] + (et - eta[-1]))
+    U = fp.copy()
+    V = 0.5 * np.sqrt(1.0 / (Re * xe)) * (et * fp - f)
+    upstream = X < 0
+    U[upstream, :] = 1.0
+    V[upstream, :] = 0.0
+    rng = np.random.Generator(np.random.PCG64(seed))
+    U = U + noise * rng.standard_normal(U.shape)
+    V = V + noise * rng.standard_normal(V.shape)
+    st = {"U": np.asfortranarray(U[:, :, None]), "V": np.asfortranarray(V[:, :, None]),
+          "W": np.zeros((nx, ny, 1), order="F"), "R": np.ones((nx, ny, 1), order="F"),
+          "E": np.full((nx, ny, 1), E0, order="F")}
+    wall = ~fr
+    for k in ("U", "V"):
+        st[k][wall] = 0.0
+    case.state0 = st
+    case.dt = rnd(cfl / cfl_dt(st, case.Ma, case.dxmin, 1))
+    return case
