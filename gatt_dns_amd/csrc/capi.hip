@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -114,6 +115,12 @@ struct gatt_ctx {
   int* d_flag = nullptr;
   int cur = 0;                          // 0: state in QA, 1: in QB
   hipStream_t stream = nullptr;
+  // the three direction solves of an RHS pass run as parallel branches on these
+  // streams, forked from and joined back into `stream`
+  hipStream_t aux[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
+  bool fork_dirs = true;
   // tangent / adjoint arrays, per member (allocated on first use)
   double* lin = nullptr;
   long long lin_stride = 0;
@@ -523,7 +530,7 @@ void run_bc(gatt_ctx* c, double* q0, double t) {
 
 // One line solve along `dir` for nf fields (member 0 pointers, member stride = slab).
 void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double* const* out, double theta,
-               int mode, int nmembers, long long in_stride, long long out_stride) {
+               int mode, int nmembers, long long in_stride, long long out_stride, hipStream_t st = nullptr) {
   DirOps& d = c->dir[dir];
   SolveArgs a{};
   a.op = d.op[kind];
@@ -547,7 +554,7 @@ void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double
   a.coef_ends = d.coef_ends[kind];
   a.theta = theta;
   a.mode = mode;
-  launch_line_solve(a, d.shape[kind], nmembers, c->stream);
+  launch_line_solve(a, d.shape[kind], nmembers, st ? st : c->stream);
 }
 
 // Navier-Stokes RHS of the stage state S and one RK stage (nseq3d/nseq2d + RK4 update),
@@ -560,6 +567,21 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
   const long long ws = c->work_stride;
   std::vector<int> flds = c->is3d ? std::vector<int>{0, 1, 2, 3, 4} : std::vector<int>{0, 1, 3, 4};
   const int ndir = c->is3d ? 3 : 2;
+  // the direction solves are independent: fork them onto the auxiliary streams
+  auto fork = [&]() {
+    if (!c->fork_dirs) return;
+    (void)hipEventRecord(c->ev_fork, c->stream);
+    for (int d = 0; d < ndir; ++d) (void)hipStreamWaitEvent(c->aux[d], c->ev_fork, 0);
+  };
+  auto join = [&]() {
+    if (!c->fork_dirs) return;
+    for (int d = 0; d < ndir; ++d) {
+      (void)hipEventRecord(c->ev_join[d], c->aux[d]);
+      (void)hipStreamWaitEvent(c->stream, c->ev_join[d], 0);
+    }
+  };
+  auto dir_stream = [&](int d) { return c->fork_dirs ? c->aux[d] : c->stream; };
+  fork();   // the forcing sums below run beside the first-derivative solves
   if (c->forcing && c->is3d) {
     SumArgs sa{};
     sa.np = np;
@@ -573,6 +595,7 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
   for (int m0 = 0; m0 < c->batch; m0 += c->wgroup) {
     const int nm = std::min(c->wgroup, c->batch - m0);
     const long long mo = (long long)m0 * c->slab;
+    if (m0 > 0) fork();
     for (int d = 0; d < ndir; ++d) {
       double* in[kMaxPtr];
       double* out[kMaxPtr];
@@ -582,8 +605,9 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
         out[nf] = D + (long long)(3 * f + d) * np;
         ++nf;
       }
-      run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, c->slab, ws);
+      run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, c->slab, ws, dir_stream(d));
     }
+    join();
     ViscArgs va{};
     va.c = c->fc;
     va.np = np;
@@ -594,6 +618,7 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
     va.T = T;
     va.is3d = c->is3d;
     launch_visc(va, nm, c->stream);
+    fork();
     // second derivatives: x: txx,txy,txz,qx  y: txy,tyy,tyz,qy  z: txz,tyz,tzz,qz
     const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
     for (int d = 0; d < ndir; ++d) {
@@ -606,8 +631,9 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
         out[nf] = G + (long long)(4 * d + q) * np;
         ++nf;
       }
-      run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, ws, ws);
+      run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, ws, ws, dir_stream(d));
     }
+    join();
     AssembleArgs aa{};
     aa.c = c->fc;
     aa.np = np;
@@ -743,6 +769,13 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
   for (int k = 0; k < 3; ++k) c->filterDir[k] = d->filterDir[k];
   c->filterCharTime = d->filterCharTime;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (int k = 0; k < 3; ++k) {
+    HIPCHK(hipStreamCreateWithFlags(&c->aux[k], hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming));
+  }
+  HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  // GATT_SERIAL_DIRECTIONS=1 keeps every launch on one stream (for comparisons)
+  if (const char* e = std::getenv("GATT_SERIAL_DIRECTIONS")) c->fork_dirs = e[0] != '1';
   {
     int dev = 0, v = 0;
     HIPCHK(hipGetDevice(&dev));
@@ -843,6 +876,12 @@ int gatt_destroy(gatt_ctx* c) {
   if (c->lin) hipFree(c->lin);
   if (c->tape) hipFree(c->tape);
   if (c->mem) hipFree(c->mem);
+  for (int k = 0; k < 3; ++k) {
+    if (c->aux[k]) (void)hipStreamSynchronize(c->aux[k]);
+    if (c->aux[k]) (void)hipStreamDestroy(c->aux[k]);
+    if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return GATT_OK;
@@ -1126,8 +1165,8 @@ int gatt_time_kernel(gatt_ctx* c, int32_t kernel, int32_t reps, double* avg_ms, 
       aa.sums = nullptr; aa.stage = 2; aa.dt = 0.0; aa.is3d = c->is3d;
       launch_assemble(aa, nm, c->stream);
       launches = 1;
-      // reads S(5) D(15) T(6) G(12) Q(5) Delta(5), writes Delta(5) Out(5); 2-D: 4, 8, 3, 6, 4, 4, 4, 4
-      arrays = c->is3d ? 58 : 37;
+      // reads S(5) D(15) G(12) Q(5) Delta(5), writes Delta(5) Out(5); 2-D: 4, 8, 6, 4, 4, 4, 4
+      arrays = c->is3d ? 52 : 34;
     }
   };
   if (kernel < 0 || kernel > GATT_K_ASSEMBLE || (!c->is3d && (kernel == GATT_K_DERIV_Z || kernel == GATT_K_FILTER_Z)))
@@ -1441,6 +1480,17 @@ int gatt_group_advance(gatt_group* g, int32_t nsteps, const double* t0, const do
   if (!g || !t0 || !dt) return fail(GATT_E_INVALID, "null argument");
   if (nsteps < 0) return fail(GATT_E_INVALID, "nsteps < 0");
   const size_t n = g->ctx.size();
+  // members of a group launch their direction solves on their own stream: the group
+  // already overlaps whole members as parallel branches
+  for (gatt_ctx* c : g->ctx) {
+    if (c->fork_dirs) {
+      c->fork_dirs = false;
+      if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+      }
+    }
+  }
   for (gatt_ctx* c : g->ctx) {
     ensure_mean(c);
     if (!c->hold_captured) run_bc(c, c->state(c->cur), 0.0);

@@ -67,7 +67,6 @@ __global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs a) {
   const long long wo = (long long)blockIdx.y * a.work_stride;
   const double* S = a.S + mo;
   const double* D = a.D + wo;
-  const double* T = a.T + wo;
   const double* G = a.G + wo;
   const double U = S[0 * np + n], V = S[1 * np + n], R = S[3 * np + n], E = S[4 * np + n];
   const FlowConst c = a.c;
@@ -78,7 +77,12 @@ __global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs a) {
   const double P = c.gm1 * R * E;
   const double DPx = c.gm1 * (R * DEx + DRx * E);
   const double DPy = c.gm1 * (R * DEy + DRy * E);
-  const double txx = T[0 * np + n], tyy = T[1 * np + n], txy = T[3 * np + n];
+  // the stresses are recomputed from the derivatives and E already in registers, with the
+  // expressions of visc_kernel (so they equal the values it stored) instead of re-read
+  const double mu1 = c.muA / (c.muC * E + c.muB);
+  const double mu2 = c.muC * E;
+  const double mu = mu1 * mu2 * sqrt(mu2);
+  const double rmu = c.invRe * mu;
   double d[5];
   if (a.is3d) {
     const double W = S[2 * np + n];
@@ -86,7 +90,9 @@ __global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs a) {
     const double DWx = D[6 * np + n], DWy = D[7 * np + n], DWz = D[8 * np + n];
     const double DRz = D[11 * np + n], DEz = D[14 * np + n];
     const double DPz = c.gm1 * (R * DEz + DRz * E);
-    const double tzz = T[2 * np + n], txz = T[4 * np + n], tyz = T[5 * np + n];
+    const double tz = -(2.0 / 3.0 * (DUx + DVy + DWz));
+    const double txx = rmu * (2.0 * DUx + tz), tyy = rmu * (2.0 * DVy + tz), tzz = rmu * (2.0 * DWz + tz);
+    const double txy = rmu * (DUy + DVx), txz = rmu * (DUz + DWx), tyz = rmu * (DVz + DWy);
     // second derivatives: x: txx,txy,txz,qx | y: txy,tyy,tyz,qy | z: txz,tyz,tzz,qz
     const double dtxxdx = G[0 * np + n], dtxydx = G[1 * np + n], dtxzdx = G[2 * np + n], htx = G[3 * np + n];
     const double dtxydy = G[4 * np + n], dtyydy = G[5 * np + n], dtyzdy = G[6 * np + n], hty = G[7 * np + n];
@@ -114,6 +120,8 @@ __global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs a) {
       d[4] = d[4] - gam * (Es / Nd - a.forcing_e0);
     }
   } else {
+    const double dv = 2.0 / 3.0 * (DUx + DVy);
+    const double txx = rmu * (2.0 * DUx - dv), tyy = rmu * (2.0 * DVy - dv), txy = rmu * (DUy + DVx);
     const double dtxxdx = G[0 * np + n], dtxydx = G[1 * np + n], htx = G[3 * np + n];
     const double dtxydy = G[4 * np + n], dtyydy = G[5 * np + n], hty = G[7 * np + n];
     const double vtx = dtxxdx + dtxydy;
