@@ -139,3 +139,292 @@ void launch_bc_wave(const BcArgs& a, long long nblocks, int nmembers, hipStream_
 }
 
 }  // namespace gatt
+
+namespace gatt {
+
+// ------------------------------------------------------------------------
+// Tangent and adjoint of the boundary program.  For one wave, bc_wave_tl_kernel
+// applies the tangent of every op to dq (using the base state before the op) and
+// then the op itself to the base; bc_wave_adj_kernel applies the transpose of the
+// wave's tangent to an adjoint lam, the base being the state before the wave.
+// Ops of one wave neither write what another reads nor write the same point, so
+// the transposed stencils may only meet on read points: they accumulate with
+// atomics.
+// ------------------------------------------------------------------------
+
+struct OpPoint {
+  int i, j, k;
+  long long n, p;
+};
+
+__device__ __forceinline__ bool op_point(const BcLinArgs& a, const BcOp*& op_out, OpPoint& pt) {
+  const long long blk = blockIdx.x;
+  int o = a.op_begin;
+  while (o + 1 < a.op_end && a.ops[o + 1].first_block <= blk) ++o;
+  const BcOp& op = a.ops[o];
+  op_out = &op;
+  pt.p = (blk - op.first_block) * blockDim.x + threadIdx.x;
+  if (pt.p >= op.npts) return false;
+  const int bi = op.box[1] - op.box[0] + 1;
+  const int bj = op.box[3] - op.box[2] + 1;
+  pt.i = op.box[0] + (int)(pt.p % bi);
+  pt.j = op.box[2] + (int)((pt.p / bi) % bj);
+  pt.k = op.box[4] + (int)(pt.p / ((long long)bi * bj));
+  pt.n = pidx(pt.i, pt.j, pt.k, a.nx, a.ny);
+  return true;
+}
+
+__device__ __forceinline__ void dir_step(int dir, int& di, int& dj, int& dk) {
+  di = dj = dk = 0;
+  switch (dir) {
+    case 1: di = 1; break;
+    case 2: di = -1; break;
+    case 3: dj = 1; break;
+    case 4: dj = -1; break;
+    case 5: dk = 1; break;
+    default: dk = -1; break;
+  }
+}
+
+__global__ __launch_bounds__(256) void bc_wave_tl_kernel(BcLinArgs a) {
+  const BcOp* opp;
+  OpPoint pt;
+  if (!op_point(a, opp, pt)) return;
+  const BcOp& op = *opp;
+  const int nx = a.nx, ny = a.ny;
+  const long long mo = (long long)blockIdx.y * a.member_stride, lo = (long long)blockIdx.y * a.lin_stride;
+  double* R = a.q[3] + mo;
+  double* E = a.q[4] + mo;
+  double* tR = a.dq[3] + lo;
+  double* tE = a.dq[4] + lo;
+  const long long n = pt.n;
+  switch (op.kind) {
+    case BC_SET:
+      (a.dq[op.field] + lo)[n] = 0.0;
+      (a.q[op.field] + mo)[n] = op.value;
+      break;
+    case BC_SET_R_FROM_P: {
+      const double Rn = op.value / (a.gamma - 1.0) / E[n];
+      tR[n] = -(Rn / E[n] * tE[n]);
+      R[n] = Rn;
+      break;
+    }
+    case BC_NEU: {
+      double* F = a.q[op.field] + mo;
+      double* tF = a.dq[op.field] + lo;
+      auto val = [&](int ii, int jj, int kk) { return F[pidx(ii, jj, kk, nx, ny)]; };
+      auto tval = [&](int ii, int jj, int kk) { return tF[pidx(ii, jj, kk, nx, ny)]; };
+      tF[n] = extrap(tval, pt.i, pt.j, pt.k, op.dir, op.L, a.coef[op.coef]);
+      F[n] = extrap(val, pt.i, pt.j, pt.k, op.dir, op.L, a.coef[op.coef]);
+      break;
+    }
+    case BC_NEU_P: {
+      auto valP = [&](int ii, int jj, int kk) {
+        const long long m = pidx(ii, jj, kk, nx, ny);
+        return R[m] * E[m];
+      };
+      auto tvalP = [&](int ii, int jj, int kk) {
+        const long long m = pidx(ii, jj, kk, nx, ny);
+        return tR[m] * E[m] + R[m] * tE[m];
+      };
+      const double Pb = extrap(valP, pt.i, pt.j, pt.k, op.dir, op.L, a.coef[op.coef]);
+      const double tPb = extrap(tvalP, pt.i, pt.j, pt.k, op.dir, op.L, a.coef[op.coef]);
+      const double Rn = Pb / E[n];
+      tR[n] = (tPb - Rn * tE[n]) / E[n];
+      R[n] = Rn;
+      break;
+    }
+    case BC_CORNER: {
+      const double nsum = (double)(abs(op.cd[0]) + abs(op.cd[1]) + abs(op.cd[2]));
+      const double* c = a.coef[0];
+      if (op.adiabatic) {
+        auto valE = [&](int ii, int jj, int kk) { return E[pidx(ii, jj, kk, nx, ny)]; };
+        auto tvalE = [&](int ii, int jj, int kk) { return tE[pidx(ii, jj, kk, nx, ny)]; };
+        double acc = 0.0, tacc = 0.0;
+        for (int ax = 0; ax < 3; ++ax) {
+          if (op.cd[ax] == 0) continue;
+          const int d = op.cd[ax] == 1 ? 2 * ax + 1 : 2 * ax + 2;
+          acc = acc + extrap(valE, pt.i, pt.j, pt.k, d, op.L, c);
+          tacc = tacc + extrap(tvalE, pt.i, pt.j, pt.k, d, op.L, c);
+        }
+        tE[n] = tacc / nsum;
+        E[n] = acc / nsum;
+      }
+      auto valP = [&](int ii, int jj, int kk) {
+        const long long m = pidx(ii, jj, kk, nx, ny);
+        return R[m] * E[m];
+      };
+      auto tvalP = [&](int ii, int jj, int kk) {
+        const long long m = pidx(ii, jj, kk, nx, ny);
+        return tR[m] * E[m] + R[m] * tE[m];
+      };
+      double acc = 0.0, tacc = 0.0;
+      for (int ax = 0; ax < 3; ++ax) {
+        if (op.cd[ax] == 0) continue;
+        const int d = op.cd[ax] == 1 ? 2 * ax + 1 : 2 * ax + 2;
+        acc = acc + extrap(valP, pt.i, pt.j, pt.k, d, op.L, c);
+        tacc = tacc + extrap(tvalP, pt.i, pt.j, pt.k, d, op.L, c);
+      }
+      const double Rn = acc / E[n] / nsum;
+      tR[n] = (tacc / nsum - Rn * tE[n]) / E[n];
+      R[n] = Rn;
+      break;
+    }
+    case BC_LID: {
+      const double* x = a.aux + op.aux;
+      const int nxb = op.box[1] - op.box[0] + 1;
+      const double s = 2 * (x[pt.i - op.box[0]] - x[0]) / (x[nxb - 1] - x[0]) - 1;
+      const double v = 1 - powi18(s);
+      (a.dq[0] + lo)[n] = 0.0;
+      (a.q[0] + mo)[n] = op.value * (v * v);
+      break;
+    }
+    case BC_HOLD: {
+      const double* save = a.hold + (long long)blockIdx.y * a.hold_member_stride + op.aux;
+      const int vars[5] = {0, 1, 2, 4, 3};
+      const int nv = op.hold_r ? 5 : 4;
+      for (int v = 0; v < nv; ++v) {
+        (a.dq[vars[v]] + lo)[n] = 0.0;
+        (a.q[vars[v]] + mo)[n] = save[(long long)v * op.npts + pt.p];
+      }
+      break;
+    }
+  }
+}
+
+// lam[m] += c_k * w at the stencil points of an extrapolation from (i,j,k) along dir
+template <typename F>
+__device__ __forceinline__ void extrap_t(F&& add, int i, int j, int k, int dir, int L, const double* c) {
+  int di, dj, dk;
+  dir_step(dir, di, dj, dk);
+  for (int q = 1; q <= L; ++q) add(i + q * di, j + q * dj, k + q * dk, c[q - 1]);
+}
+
+__global__ __launch_bounds__(256) void bc_wave_adj_kernel(BcLinArgs a) {
+  const BcOp* opp;
+  OpPoint pt;
+  if (!op_point(a, opp, pt)) return;
+  const BcOp& op = *opp;
+  const int nx = a.nx, ny = a.ny;
+  const long long mo = (long long)blockIdx.y * a.member_stride, lo = (long long)blockIdx.y * a.lin_stride;
+  const double* R = a.q[3] + mo;
+  const double* E = a.q[4] + mo;
+  double* bR = a.dq[3] + lo;
+  double* bE = a.dq[4] + lo;
+  const long long n = pt.n;
+  switch (op.kind) {
+    case BC_SET:
+      (a.dq[op.field] + lo)[n] = 0.0;
+      break;
+    case BC_SET_R_FROM_P: {
+      const double Rn = op.value / (a.gamma - 1.0) / E[n];
+      const double t = bR[n];
+      bR[n] = 0.0;
+      bE[n] += -(Rn / E[n] * t);
+      break;
+    }
+    case BC_NEU: {
+      double* bF = a.dq[op.field] + lo;
+      const double t = bF[n];
+      bF[n] = 0.0;
+      extrap_t([&](int ii, int jj, int kk, double ck) { atomicAdd(&bF[pidx(ii, jj, kk, nx, ny)], ck * t); },
+               pt.i, pt.j, pt.k, op.dir, op.L, a.coef[op.coef]);
+      break;
+    }
+    case BC_NEU_P: {
+      auto valP = [&](int ii, int jj, int kk) {
+        const long long m = pidx(ii, jj, kk, nx, ny);
+        return R[m] * E[m];
+      };
+      const double Pb = extrap(valP, pt.i, pt.j, pt.k, op.dir, op.L, a.coef[op.coef]);
+      const double Rn = Pb / E[n];
+      const double t = bR[n];
+      bR[n] = 0.0;
+      bE[n] += -(Rn / E[n] * t);
+      const double Pbar = t / E[n];
+      extrap_t([&](int ii, int jj, int kk, double ck) {
+                 const long long m = pidx(ii, jj, kk, nx, ny);
+                 atomicAdd(&bR[m], ck * E[m] * Pbar);
+                 atomicAdd(&bE[m], ck * R[m] * Pbar);
+               },
+               pt.i, pt.j, pt.k, op.dir, op.L, a.coef[op.coef]);
+      break;
+    }
+    case BC_CORNER: {
+      const double nsum = (double)(abs(op.cd[0]) + abs(op.cd[1]) + abs(op.cd[2]));
+      const double* c = a.coef[0];
+      // base values after the E part of the op
+      double En = E[n];
+      if (op.adiabatic) {
+        auto valE = [&](int ii, int jj, int kk) { return E[pidx(ii, jj, kk, nx, ny)]; };
+        double acc = 0.0;
+        for (int ax = 0; ax < 3; ++ax) {
+          if (op.cd[ax] == 0) continue;
+          const int d = op.cd[ax] == 1 ? 2 * ax + 1 : 2 * ax + 2;
+          acc = acc + extrap(valE, pt.i, pt.j, pt.k, d, op.L, c);
+        }
+        En = acc / nsum;
+      }
+      auto valP = [&](int ii, int jj, int kk) {
+        const long long m = pidx(ii, jj, kk, nx, ny);
+        return R[m] * E[m];
+      };
+      double acc = 0.0;
+      for (int ax = 0; ax < 3; ++ax) {
+        if (op.cd[ax] == 0) continue;
+        const int d = op.cd[ax] == 1 ? 2 * ax + 1 : 2 * ax + 2;
+        acc = acc + extrap(valP, pt.i, pt.j, pt.k, d, op.L, c);
+      }
+      const double Rn = acc / En / nsum;
+      // transpose of R(b) = acc / E(b) / nsum
+      const double t = bR[n];
+      bR[n] = 0.0;
+      double bEn = bE[n] - Rn / En * t;
+      const double bacc = t / (En * nsum);
+      for (int ax = 0; ax < 3; ++ax) {
+        if (op.cd[ax] == 0) continue;
+        const int d = op.cd[ax] == 1 ? 2 * ax + 1 : 2 * ax + 2;
+        extrap_t([&](int ii, int jj, int kk, double ck) {
+                   const long long m = pidx(ii, jj, kk, nx, ny);
+                   atomicAdd(&bR[m], ck * E[m] * bacc);
+                   atomicAdd(&bE[m], ck * R[m] * bacc);
+                 },
+                 pt.i, pt.j, pt.k, d, op.L, c);
+      }
+      if (op.adiabatic) {
+        // transpose of E(b) = (sum of extrapolations of E) / nsum: E(b) itself is overwritten
+        const double w = bEn / nsum;
+        bEn = 0.0;
+        for (int ax = 0; ax < 3; ++ax) {
+          if (op.cd[ax] == 0) continue;
+          const int d = op.cd[ax] == 1 ? 2 * ax + 1 : 2 * ax + 2;
+          extrap_t([&](int ii, int jj, int kk, double ck) { atomicAdd(&bE[pidx(ii, jj, kk, nx, ny)], ck * w); },
+                   pt.i, pt.j, pt.k, d, op.L, c);
+        }
+      }
+      bE[n] = bEn;
+      break;
+    }
+    case BC_LID:
+      (a.dq[0] + lo)[n] = 0.0;
+      break;
+    case BC_HOLD: {
+      const int vars[5] = {0, 1, 2, 4, 3};
+      const int nv = op.hold_r ? 5 : 4;
+      for (int v = 0; v < nv; ++v) (a.dq[vars[v]] + lo)[n] = 0.0;
+      break;
+    }
+  }
+}
+
+void launch_bc_wave_tl(const BcLinArgs& a, long long nblocks, int nmembers, hipStream_t st) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(bc_wave_tl_kernel, dim3((unsigned)nblocks, nmembers), dim3(256), 0, st, a);
+}
+
+void launch_bc_wave_adj(const BcLinArgs& a, long long nblocks, int nmembers, hipStream_t st) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(bc_wave_adj_kernel, dim3((unsigned)nblocks, nmembers), dim3(256), 0, st, a);
+}
+
+}  // namespace gatt

@@ -1009,7 +1009,9 @@ int gatt_advance(gatt_ctx* c, int32_t nsteps, double t0, double dt, double* maxc
   if (check_ctx(c)) return GATT_E_INVALID;
   if (nsteps < 0) return fail(GATT_E_INVALID, "nsteps < 0");
   ensure_mean(c);
-  if (!c->hold_captured) run_bc(c, c->state(c->cur), t0);  // first call captures holdInlet state
+  // holdInlet captures the inflow at the first boundary application; a problem that holds its
+  // inflow and has not had one gets it here (main.F90:112); otherwise the state is left as given
+  if (c->d_hold && !c->hold_captured) run_bc(c, c->state(c->cur), t0);
   int s = 0;
   if (c->graph && (c->graph_dt != dt || c->graph_cur != c->cur)) {
     hipGraphExecDestroy(c->graph);
@@ -1493,7 +1495,7 @@ int gatt_group_advance(gatt_group* g, int32_t nsteps, const double* t0, const do
   }
   for (gatt_ctx* c : g->ctx) {
     ensure_mean(c);
-    if (!c->hold_captured) run_bc(c, c->state(c->cur), 0.0);
+    if (c->d_hold && !c->hold_captured) run_bc(c, c->state(c->cur), 0.0);
   }
   if (g->mode == 1) {
     // per-member double-step graphs, launched round-robin on the members' streams
@@ -1577,7 +1579,13 @@ enum : int {
   LN_D = 30,    // 15 first-derivative tangents / adjoints
   LN_T = 45,    // 9 stress and heat-flux tangents / adjoints
   LN_G = 54,    // 12 second-derivative tangents / adjoints
-  LN_N = 66
+  LN_P1 = 66,   // adjoint: the states before each boundary-condition application of the base
+  LN_P2 = 71,   //   step (after stages 1..4, and after the SFD)
+  LN_P3 = 76,
+  LN_P4 = 81,
+  LN_P5 = 86,
+  LN_TMP = 91,  // base state replayed through the boundary program
+  LN_N = 96
 };
 
 int ensure_lin(gatt_ctx* c) {
@@ -1609,9 +1617,6 @@ int ensure_tape(gatt_ctx* c) {
 double* tape_ptr(gatt_ctx* c, int s, int wk) { return c->tape + ((long long)s * WK_N + wk) * c->np; }
 
 int lin_supported(gatt_ctx* c) {
-  if (!c->ops.empty())
-    return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps are implemented for problems without boundary "
-                                    "conditions (periodic directions)");
   if (c->forcing) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps do not cover the turbulence forcing");
   if (c->sfdType > 0 && c->sfdDelta > 0)
     return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps need a frozen SFD mean (Delta = inf)");
@@ -1917,6 +1922,92 @@ void adj_rhs(gatt_ctx* c, const double* S, long long ss, double* kbar, double* s
   }
 }
 
+BcLinArgs bc_lin_args(gatt_ctx* c, const BcWave& w, double* q, long long qs, double* d, long long ds) {
+  BcLinArgs a{};
+  a.ops = c->d_ops;
+  a.op_begin = w.op_begin;
+  a.op_end = w.op_end;
+  for (int f = 0; f < 5; ++f) {
+    a.q[f] = q + (long long)f * c->np;
+    a.dq[f] = d + (long long)f * c->np;
+  }
+  a.member_stride = qs;
+  a.lin_stride = ds;
+  a.coef[0] = c->d_coef[0];
+  a.coef[1] = c->d_coef[1];
+  a.aux = c->d_aux;
+  a.hold = c->d_hold;
+  a.hold_member_stride = c->hold_stride;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  a.gamma = c->gamma;
+  return a;
+}
+
+// Boundary program on the base q (member stride qs) with its tangent d (stride ds).
+void run_bc_tl(gatt_ctx* c, double* q, long long qs, double* d, long long ds) {
+  for (const BcWave& w : c->waves)
+    launch_bc_wave_tl(bc_lin_args(c, w, q, qs, d, ds), w.nblocks, c->batch, c->stream);
+}
+
+bool wave_needs_base(const gatt_ctx* c, const BcWave& w) {
+  for (int o = w.op_begin; o < w.op_end; ++o) {
+    const int k = c->ops[(size_t)o].kind;
+    if (k == BC_SET_R_FROM_P || k == BC_NEU_P || k == BC_CORNER) return true;
+  }
+  return false;
+}
+
+// lam <- (d BC / d q at qpre)^T lam, waves in reverse order; the base state before
+// a wave that depends on it is rebuilt by replaying the earlier waves on a copy of qpre.
+void run_bc_adj(gatt_ctx* c, double* qpre, long long qs, double* lam) {
+  const long long ls = c->lin_stride;
+  double* tmp = lin_ptr(c, LN_TMP);
+  for (int wi = (int)c->waves.size() - 1; wi >= 0; --wi) {
+    const BcWave& w = c->waves[(size_t)wi];
+    double* base = qpre;
+    long long bs = qs;
+    if (wave_needs_base(c, w)) {
+      AxpbyArgs cp{};
+      cp.np = c->np;
+      cp.x_stride = qs;
+      cp.y_stride = ls;
+      cp.nf = 5;
+      for (int f = 0; f < 5; ++f) {
+        cp.x[f] = qpre + (long long)f * c->np;
+        cp.y[f] = tmp + (long long)f * c->np;
+      }
+      cp.a = 1.0;
+      cp.b = 0.0;
+      launch_axpby(cp, c->batch, c->stream);
+      for (int v = 0; v < wi; ++v) {
+        const BcWave& pw = c->waves[(size_t)v];
+        BcArgs a{};
+        a.ops = c->d_ops;
+        a.op_begin = pw.op_begin;
+        a.op_end = pw.op_end;
+        for (int f = 0; f < 5; ++f) a.q[f] = tmp + (long long)f * c->np;
+        a.member_stride = ls;
+        a.coef[0] = c->d_coef[0];
+        a.coef[1] = c->d_coef[1];
+        a.aux = c->d_aux;
+        a.hold = c->d_hold;
+        a.hold_member_stride = c->hold_stride;
+        a.nx = c->nx;
+        a.ny = c->ny;
+        a.nz = c->nz;
+        a.gamma = c->gamma;
+        a.capture = 0;
+        launch_bc_wave(a, pw.nblocks, c->batch, c->stream);
+      }
+      base = tmp;
+      bs = ls;
+    }
+    launch_bc_wave_adj(bc_lin_args(c, w, base, bs, lam, ls), w.nblocks, c->batch, c->stream);
+  }
+}
+
 // One step of the base state with its tangent (main.F90:255-420 and its derivative).
 void enqueue_tlm_step(gatt_ctx* c, double t, double dt) {
   (void)t;
@@ -1939,6 +2030,7 @@ void enqueue_tlm_step(gatt_ctx* c, double t, double dt) {
       forward_assemble(c, Sin, q, Out, stage, dt, m0, nm, c->work_ptr(WK_D), c->work_ptr(WK_T), c->work_ptr(WK_G),
                        c->work_stride);
     }
+    run_bc_tl(c, Out, c->slab, dOut, c->lin_stride);    // timeIntegration.F90: BCs after every stage
   }
   const double theta = dt / c->filterCharTime;
   run_filters(c, qn, theta);
@@ -1964,6 +2056,7 @@ void enqueue_tlm_step(gatt_ctx* c, double t, double dt) {
     sa.nf_mask = c->is3d ? 0x1f : 0x1b;
     launch_sfd(sa, c->batch, c->stream);
   }
+  run_bc_tl(c, qn, c->slab, dq, c->lin_stride);
   ReduceArgs r{};
   r.np = c->np;
   r.member_stride = c->slab;
@@ -1992,16 +2085,36 @@ void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
   // stresses and their derivatives of every stage go to the tape for the adjoint
   const long long ts = c->tape_stride;
   const int nb = c->batch;
+  double* qn = c->state(1 - c->cur);     // scratch for the last stage; the base q is not modified
+  double* P1 = lin_ptr(c, LN_P1);
+  double* P2 = lin_ptr(c, LN_P2);
+  double* P3 = lin_ptr(c, LN_P3);
+  double* P4 = lin_ptr(c, LN_P4);
+  double* P5 = lin_ptr(c, LN_P5);
   base_dtg(c, q, sl, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
   forward_assemble(c, q, q, S, 1, dt, 0, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
+  fields_axpby(c, P1, ls, S, sl, 1.0, 0.0);
+  run_bc(c, S, t + dt / 2.0);
   fields_axpby(c, B1, ls, S, sl, 1.0, 0.0);
   base_dtg(c, S, sl, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
   forward_assemble(c, S, q, S, 2, dt, 0, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
+  fields_axpby(c, P2, ls, S, sl, 1.0, 0.0);
+  run_bc(c, S, t + dt / 2.0);
   fields_axpby(c, B2, ls, S, sl, 1.0, 0.0);
   base_dtg(c, S, sl, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
   forward_assemble(c, S, q, S, 3, dt, 0, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
+  fields_axpby(c, P3, ls, S, sl, 1.0, 0.0);
+  run_bc(c, S, t + dt);
   fields_axpby(c, B3, ls, S, sl, 1.0, 0.0);
   base_dtg(c, S, sl, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
+  forward_assemble(c, S, q, qn, 4, dt, 0, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
+  fields_axpby(c, P4, ls, qn, sl, 1.0, 0.0);
+  run_bc(c, qn, t + dt);
+  run_filters(c, qn, dt / c->filterCharTime);
+  run_sfd(c, qn, dt);
+  fields_axpby(c, P5, ls, qn, sl, 1.0, 0.0);
+  // the final boundary conditions (main.F90:405)
+  run_bc_adj(c, P5, ls, QB);
   // SFD with a frozen mean is diagonal
   if (c->sfdType > 0) {
     SfdArgs sa{};
@@ -2034,17 +2147,23 @@ void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
     run_solve_t(c, d, GATT_FILTER, nf, in, out, blend ? out : nullptr, blend ? theta : 1.0, blend ? 1.0 - theta : 0.0,
                 0, c->batch, ls, ls);
   }
-  // RK4 (the accumulator form of timeIntegration.F90:382-496), backwards
+  // boundary conditions after the last stage
+  run_bc_adj(c, P4, ls, QB);
+  // RK4 (the accumulator form of timeIntegration.F90:382-496), backwards, with the
+  // transposed boundary conditions after every stage
   fields_axpby(c, KB, ls, QB, ls, dt / 6.0, 0.0);
   adj_rhs(c, B3, ls, KB, SB, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
+  run_bc_adj(c, P3, ls, SB);
   fields_axpby(c, B3, ls, SB, ls, 1.0, 0.0);          // B3 now accumulates the stage adjoints
   fields_axpby(c, KB, ls, QB, ls, dt / 3.0, 0.0);
   fields_axpby(c, KB, ls, SB, ls, dt, 1.0);
   adj_rhs(c, B2, ls, KB, SB, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
+  run_bc_adj(c, P2, ls, SB);
   fields_axpby(c, B3, ls, SB, ls, 1.0, 1.0);
   fields_axpby(c, KB, ls, QB, ls, dt / 3.0, 0.0);
   fields_axpby(c, KB, ls, SB, ls, dt / 2.0, 1.0);
   adj_rhs(c, B1, ls, KB, SB, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
+  run_bc_adj(c, P1, ls, SB);
   fields_axpby(c, B3, ls, SB, ls, 1.0, 1.0);
   fields_axpby(c, KB, ls, QB, ls, dt / 6.0, 0.0);
   fields_axpby(c, KB, ls, SB, ls, dt / 2.0, 1.0);
@@ -2126,6 +2245,8 @@ int gatt_tlm_step(gatt_ctx* c, double t, double dt, double* maxchange, int32_t* 
   if (check_ctx(c)) return GATT_E_INVALID;
   int rc = lin_supported(c);
   if (rc) return rc;
+  if (c->d_hold && !c->hold_captured)
+    return fail(GATT_E_INVALID, "apply the boundary conditions once first (holdInlet captures the inflow)");
   if ((rc = ensure_lin(c))) return rc;
   ensure_mean(c);
   enqueue_tlm_step(c, t, dt);
@@ -2137,6 +2258,9 @@ int gatt_adj_step(gatt_ctx* c, double t, double dt) {
   if (check_ctx(c)) return GATT_E_INVALID;
   int rc = lin_supported(c);
   if (rc) return rc;
+  if (c->d_hold && !c->hold_captured)
+    return fail(GATT_E_INVALID, "apply the boundary conditions once first (holdInlet captures the inflow)");
+  ensure_mean(c);
   if ((rc = ensure_lin(c))) return rc;
   if ((rc = ensure_tape(c))) return rc;
   enqueue_adj_step(c, t, dt);

@@ -335,4 +335,22 @@ struct BcArgs {
 };
 void launch_bc_wave(const BcArgs& a, long long nblocks, int nmembers, hipStream_t st);
 
+// Tangent / adjoint of one wave of the boundary program (boundary.hip).
+struct BcLinArgs {
+  const BcOp* ops;
+  int op_begin, op_end;
+  double* q[5];          // base state (updated by the tangent kernel, read by the adjoint)
+  long long member_stride;
+  double* dq[5];         // tangent or adjoint fields
+  long long lin_stride;
+  const double* coef[2];
+  const double* aux;
+  const double* hold;
+  long long hold_member_stride;
+  int nx, ny, nz;
+  double gamma;
+};
+void launch_bc_wave_tl(const BcLinArgs& a, long long nblocks, int nmembers, hipStream_t st);
+void launch_bc_wave_adj(const BcLinArgs& a, long long nblocks, int nmembers, hipStream_t st);
+
 }  // namespace gatt

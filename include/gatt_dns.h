@@ -189,7 +189,10 @@ int gatt_rhs(gatt_ctx* ctx, double* const* dq);
 int gatt_step(gatt_ctx* ctx, double t, double dt, double* umax, double* maxchange, int32_t* diverged);
 
 /* nsteps fixed-dt steps starting at time t0 (timeControl = 1), replayed from
- * a captured HIP graph; per-step scalars of the last step are returned. */
+ * a captured HIP graph; per-step scalars of the last step are returned.  The
+ * state is used as given, as after main.F90:112; only a problem that holds its
+ * inflow (holdInlet) and has had no boundary application yet gets one first,
+ * which captures the inflow. */
 int gatt_advance(gatt_ctx* ctx, int32_t nsteps, double t0, double dt, double* maxchange, int32_t* diverged);
 
 /* Unit entry point: one operator along one direction on nfields device
@@ -297,10 +300,10 @@ int gatt_block_update(gatt_ctx* ctx, int64_t n, int32_t k, int32_t p, const doub
  * The adjoint is the exact transpose of the tangent: RK4 stages, the
  * Navier-Stokes RHS (nseq.F90), the compact solves as transposed Thomas
  * sweeps with the transposed periodic correction and the transposed RHS band,
- * the filter blend and the SFD damping with a frozen mean.  Implemented for
- * problems without boundary conditions (periodic directions), without the
- * turbulence forcing and with SFD off or frozen (Delta = inf); other problems
- * return GATT_E_UNSUPPORTED.
+ * the filter blend, the SFD damping with a frozen mean and the boundary
+ * program (Dirichlet, Neumann and null-second-derivative boxes, pressure
+ * conditions through R = P/E, corners, lid and held inflow).  The turbulence
+ * forcing and an evolving SFD mean (Delta > 0) return GATT_E_UNSUPPORTED.
  * ------------------------------------------------------------------------ */
 enum { GATT_SLOT_STATE = 0, GATT_SLOT_PERTURBATION = 1 };
 

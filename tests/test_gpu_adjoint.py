@@ -1,7 +1,9 @@
 """GPU checks of the tangent-linear and adjoint steps.
 
 * dot-product test <J v, w> = <v, J^T w> within 1e-11 (one step and a
-  checkpointed multi-step propagator), 2-D and 3-D periodic boxes;
+  checkpointed multi-step propagator): 2-D and 3-D periodic boxes, the lid-driven
+  cavity (walls, corners, lid) and the boundary layer (inflow/outflow/top boxes,
+  free-slip segment, buffers, SFD with a frozen mean);
 * the tangent against central differences of the oracle's step (the
   reference's non-linear step restated on the CPU);
 * the transposed line solves against the transposes of the forward solves
@@ -38,6 +40,23 @@ def _case(kind):
         q["R"] = q["R"] + 0.05 * np.cos(2 * np.pi * (X + Y))[:, :, None]
         q["E"] = q["E"] * (1 + 0.02 * np.sin(4 * np.pi * X))[:, :, None]
         return c, q
+    if kind == "cavity":
+        # walls: Dirichlet and pressure Neumann boxes, corners, the lid profile
+        c = cases.lid_driven(nx=29, ny=23)
+        rng = np.random.Generator(np.random.PCG64(2))
+        q = {k: np.array(v, copy=True) for k, v in c.state0.items()}
+        x = np.linspace(0, 1, 29)[:, None, None]
+        y = np.linspace(0, 1, 23)[None, :, None]
+        q["U"] = q["U"] + 0.2 * np.sin(np.pi * x) * np.sin(np.pi * y) + 0.01 * rng.standard_normal(q["U"].shape)
+        q["V"] = q["V"] + 0.1 * np.sin(2 * np.pi * x) * np.sin(np.pi * y)
+        q["R"] = q["R"] + 0.02 * np.cos(np.pi * x)
+        q["E"] = q["E"] * (1 + 0.01 * np.cos(np.pi * y))
+        return c, {k: np.asfortranarray(v) for k, v in q.items()}
+    if kind == "bl":
+        # inflow, outflow and top boxes (null second derivatives), the wall with its free-slip
+        # segment, EX4 buffers, SFD with a frozen mean in the buffers
+        c = cases.boundary_layer(nx=96, ny=48, nbuf=10, sfd_delta=-1.0)
+        return c, {k: np.array(v, copy=True, order="F") for k, v in c.state0.items()}
     c = cases.turbulence_box(16, seed=3, forcing=False)
     q = {k: np.array(v, copy=True) for k, v in c.state0.items()}
     q["R"] = q["R"] + 0.03 * np.sin(2 * np.pi * np.arange(16) / 16)[None, :, None]
@@ -53,7 +72,7 @@ def _rand(prop, seed):
     return torch.from_numpy(v).cuda()
 
 
-@pytest.mark.parametrize("kind", ["box2d", "box3d"])
+@pytest.mark.parametrize("kind", ["box2d", "box3d", "cavity", "bl"])
 def test_dot_product_one_step(torch_cuda, kind):
     from gatt_dns_amd.adjoint import Propagator, state_tensor
     case, q = _case(kind)
@@ -69,7 +88,7 @@ def test_dot_product_one_step(torch_cuda, kind):
     prop.close()
 
 
-@pytest.mark.parametrize("kind", ["box2d", "box3d"])
+@pytest.mark.parametrize("kind", ["box2d", "box3d", "cavity", "bl"])
 def test_tangent_matches_oracle_differences(torch_cuda, kind, oracle_mod):
     from gatt_dns_amd.adjoint import Propagator, state_tensor
     case, q = _case(kind)
@@ -121,7 +140,7 @@ def test_transposed_solves_are_transposes(torch_cuda, name):
     s.close()
 
 
-@pytest.mark.parametrize("kind", ["box2d", "box3d"])
+@pytest.mark.parametrize("kind", ["box2d", "box3d", "cavity", "bl"])
 def test_checkpointed_propagator_dot_product_and_gradient(torch_cuda, kind):
     import torch
     from gatt_dns_amd.adjoint import Propagator, advance, optimal_energy_gradient, state_tensor
@@ -137,7 +156,7 @@ def test_checkpointed_propagator_dot_product_and_gradient(torch_cuda, kind):
     J.backward()
     g = x.grad.detach()
     v = _rand(prop, 9) * 1e-3
-    eps = 1e-4
+    eps = 1e-3        # perturbations of 1e-6: truncation ~1e-12 relative, rounding noise ~1e-9
 
     def Jof(z):
         with torch.no_grad():
