@@ -287,8 +287,11 @@ def blasius(eta_max=20.0, n=4001):
 
 def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wall_ratio=1.02, Re=1000.0,
                    Ma=0.5, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0.49, spatial="SL4", cfl=0.5, sfd_chi=0.02,
-                   sfd_delta=10.0, noise=1e-5, seed=20261, buffer_growth=8.0):
-    """2-D isothermal-wall boundary layer (boundaryLayerIsothermal.m) on a stretched mesh with buffers.
+                   sfd_delta=10.0, noise=1e-5, seed=20261, buffer_growth=8.0, nz=1, z_length=4.0):
+    """Isothermal-wall boundary layer (boundaryLayerIsothermal.m) on a stretched mesh with buffers.
+
+    With nz > 1 the 2-D configuration is extruded over a spanwise-periodic uniform z of length
+    z_length (the flow region, boundary boxes and initial profiles are the same in every plane).
 
     The mesh is built here (a 'file' mesh for the reference): x uniform on x_range with nbuf-point
     buffers at both ends, y geometric from the wall with ratio ``wall_ratio`` up to ``y_top`` and an
@@ -307,25 +310,26 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
     Yp = np.concatenate([[0.0], np.cumsum(dy0 * wall_ratio ** np.arange(npy - 1))])
     Yp[-1] = y_top
     Y = np.concatenate([Yp, _buffer_points(Yp[-1], Yp[-1] - Yp[-2], nbuf, buffer_growth, 1.0)])
-    Z = np.array([0.0])
+    Z = np.arange(nz) * (z_length / nz) if nz > 1 else np.array([0.0])
     E0 = 1.0 / ((gamma ** 2 - gamma) * Ma ** 2)
     P0 = (gamma - 1) * E0
     fr, walls, corners, bc, bp, hold = boundary_layer_isothermal(X, Y, Z, E0, P0)
     buf = BufferSide(n=nbuf, upwind=False, transition=0.2)
     meshes = [DirMesh(X, buffer_i=buf, buffer_f=buf, break_points=[bp] if bp else []),
               DirMesh(Y, buffer_f=buf),
-              DirMesh(Z, periodic=True, fix_periodic_domain_size=True, xf=1.0)]
+              DirMesh(Z, periodic=True, fix_periodic_domain_size=True, xf=z_length if nz > 1 else 1.0)]
     nm = NumMethods(spatial_derivs=spatial, spatial_derivs_buffer="EX4", filter_strength=alpha_f,
                     filter_borders="reducedOrder", filter_borders_start=(False, True, True),
                     filter_borders_end=(False, False, True))
     deriv, filt, types = _coeffs(meshes, fr, nm)
     bi = init_boundaries(bc, corners, nm.neumann_order, nm.neumann2_order)
     rnd = lambda v: float(fortran_round(v))
-    chi = calc_sfd_region(nx, ny, 1, ((nbuf, nbuf), (0, nbuf), (0, 0)), sfd_chi, apply=(2, 0, 2))
-    case = Case("boundary_layer_%dx%d" % (nx, ny), nx, ny, 1, X, Y, Z, rnd(Re), rnd(Ma), rnd(Pr), rnd(gamma),
+    chi = calc_sfd_region(nx, ny, nz, ((nbuf, nbuf), (0, nbuf), (0, 0)), sfd_chi, apply=(2, 0, 2))
+    name = "boundary_layer_%dx%d" % (nx, ny) + ("x%d" % nz if nz > 1 else "")
+    case = Case(name, nx, ny, nz, X, Y, Z, rnd(Re), rnd(Ma), rnd(Pr), rnd(gamma),
                 rnd(T0), deriv, filt, types, (1, 1, 1), -1.0, bi, [], [],
                 sfd_type=2, sfd_delta=rnd(sfd_delta), sfd_x_val=rnd(sfd_chi),
-                sfd_x=np.asfortranarray(fortran_round(chi)), dxmin=_dxmin(X, Y, Z, 1))
+                sfd_x=np.asfortranarray(fortran_round(chi)), dxmin=_dxmin(X, Y, Z, nz))
     case.flow_region = fr
     case.buffers = ((nbuf, nbuf), (0, nbuf), (0, 0))
     eta, sol = blasius()
@@ -340,14 +344,15 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
     U[upstream, :] = 1.0
     V[upstream, :] = 0.0
     rng = np.random.Generator(np.random.PCG64(seed))
-    U = U + noise * rng.standard_normal(U.shape)
-    V = V + noise * rng.standard_normal(V.shape)
-    st = {"U": np.asfortranarray(U[:, :, None]), "V": np.asfortranarray(V[:, :, None]),
-          "W": np.zeros((nx, ny, 1), order="F"), "R": np.ones((nx, ny, 1), order="F"),
-          "E": np.full((nx, ny, 1), E0, order="F")}
+    shape = (nx, ny, nz)
+    U = U[:, :, None] + noise * rng.standard_normal(shape)
+    V = V[:, :, None] + noise * rng.standard_normal(shape)
+    st = {"U": np.asfortranarray(U), "V": np.asfortranarray(V),
+          "W": np.zeros(shape, order="F"), "R": np.ones(shape, order="F"),
+          "E": np.full(shape, E0, order="F")}
     wall = ~fr
     for k in ("U", "V"):
         st[k][wall] = 0.0
     case.state0 = st
-    case.dt = rnd(cfl / cfl_dt(st, case.Ma, case.dxmin, 1))
+    case.dt = rnd(cfl / cfl_dt(st, case.Ma, case.dxmin, nz))
     return case

@@ -52,6 +52,13 @@ def _case(kind):
         q["R"] = q["R"] + 0.02 * np.cos(np.pi * x)
         q["E"] = q["E"] * (1 + 0.01 * np.cos(np.pi * y))
         return c, {k: np.asfortranarray(v) for k, v in q.items()}
+    if kind == "bl3d":
+        # the adjoint configuration in small: spanwise-periodic boundary layer with walls
+        c = cases.boundary_layer(nx=64, ny=40, nbuf=8, nz=12, sfd_delta=-1.0)
+        q = {k: np.array(v, copy=True, order="F") for k, v in c.state0.items()}
+        z = np.arange(12) / 12
+        q["W"] = np.asfortranarray(q["W"] + 1e-3 * np.sin(2 * np.pi * z)[None, None, :] * c.flow_region)
+        return c, q
     if kind == "bl":
         # inflow, outflow and top boxes (null second derivatives), the wall with its free-slip
         # segment, EX4 buffers, SFD with a frozen mean in the buffers
@@ -72,7 +79,7 @@ def _rand(prop, seed):
     return torch.from_numpy(v).cuda()
 
 
-@pytest.mark.parametrize("kind", ["box2d", "box3d", "cavity", "bl"])
+@pytest.mark.parametrize("kind", ["box2d", "box3d", "cavity", "bl", "bl3d"])
 def test_dot_product_one_step(torch_cuda, kind):
     from gatt_dns_amd.adjoint import Propagator, state_tensor
     case, q = _case(kind)
@@ -88,7 +95,7 @@ def test_dot_product_one_step(torch_cuda, kind):
     prop.close()
 
 
-@pytest.mark.parametrize("kind", ["box2d", "box3d", "cavity", "bl"])
+@pytest.mark.parametrize("kind", ["box2d", "box3d", "cavity", "bl", "bl3d"])
 def test_tangent_matches_oracle_differences(torch_cuda, kind, oracle_mod):
     from gatt_dns_amd.adjoint import Propagator, state_tensor
     case, q = _case(kind)
