@@ -36,6 +36,18 @@ class GattError(RuntimeError):
     pass
 
 
+def _after_torch(*tensors):
+    """Wait for torch's pending work on the tensors' device before the library reads or writes them.
+
+    The library runs on its own non-blocking stream, which is not ordered after torch's current
+    stream (a fill or copy torch has queued on a tensor may still be running)."""
+    for t in tensors:
+        if t is not None:
+            import torch
+            torch.cuda.current_stream(t.device).synchronize()
+            return
+
+
 _lib = None
 
 
@@ -223,6 +235,7 @@ class Solver:
             return None, 0
         if str(t.dtype) != "torch.float64" or not t.is_cuda or t.stride(-1) != 1:
             raise GattError("device blocks must be float64 CUDA tensors with unit stride along a vector")
+        _after_torch(t)
         return C.c_void_p(t.data_ptr()), (t.stride(0) if t.dim() == 2 else t.shape[-1])
 
     def _need(self, t, cols, what):
@@ -283,6 +296,7 @@ class Solver:
         if (str(tensor.dtype) != "torch.float64" or not tensor.is_cuda or not tensor.is_contiguous()
                 or tensor.numel() != 5 * npts):
             raise GattError("expected a contiguous float64 CUDA tensor of %d values" % (5 * npts))
+        _after_torch(tensor)
         _check(self.lib.gatt_copy_fields(self.ctx, member, slot, 1 if to_ctx else 0, C.c_void_p(tensor.data_ptr())))
 
     def tlm_step(self, t, dt):
@@ -299,6 +313,7 @@ class Solver:
     def line_solve_t(self, direction, kind, fields_in, fields_out):
         """Transposed line solve (device tensors); fields_in are overwritten (workspace)."""
         n = len(fields_in)
+        _after_torch(*fields_in, *fields_out)
         pin = (DP * n)(*[C.cast(C.c_void_p(t.data_ptr()), DP) for t in fields_in])
         pout = (DP * n)(*[C.cast(C.c_void_p(t.data_ptr()), DP) for t in fields_out])
         _check(self.lib.gatt_line_solve_t(self.ctx, direction, kind, n, pin, pout))
@@ -306,6 +321,7 @@ class Solver:
     def line_solve(self, direction, kind, fields_in, fields_out):
         """Device-pointer unit entry point: lists of torch CUDA tensors (float64, Fortran-ordered data)."""
         n = len(fields_in)
+        _after_torch(*fields_in, *fields_out)
         pin = (DP * n)(*[C.cast(C.c_void_p(t.data_ptr()), DP) for t in fields_in])
         pout = (DP * n)(*[C.cast(C.c_void_p(t.data_ptr()), DP) for t in fields_out])
         _check(self.lib.gatt_line_solve(self.ctx, direction, kind, n, pin, pout))
