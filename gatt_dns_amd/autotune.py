@@ -1,8 +1,8 @@
 """Launch-shape autotuner of the compact line solves.
 
 For every direction and operator (derivative, filter) of a :class:`Solver`,
-times the line-solve kernel class over the candidate shapes -- the wave-segment
-kernel with 4..32 points per lane, and the lane-group kernel with G lanes per
+times the line-solve kernel class over the candidate shapes -- the fused and the
+plain wave-segment kernels with 16 or 32 points per lane, and the lane-group kernel with G lanes per
 line (powers of two, at most 32 points per lane) and 1, 2 or 4 wavefronts per
 workgroup -- with HIP events on the library stream, and keeps the fastest.
 Filter timing overwrites the state, so call :func:`tune` before setting it (or
@@ -17,7 +17,7 @@ KERNEL_CLASS = {(0, GATT_DERIV): 0, (1, GATT_DERIV): 1, (2, GATT_DERIV): 2,
 
 def candidates(n):
     """(variant, param, waves per block) triples; the library rejects shapes that do not fit."""
-    out = [(1, M, 0) for M in (16, 32)]
+    out = [(v, M, 0) for v in (2, 1) for M in (16, 32)]
     out += [(0, G, w) for G in (2, 4, 8, 16, 32, 64) if (n + G - 1) // G <= 32 for w in (1, 2, 4)]
     return out
 

@@ -56,6 +56,8 @@ struct DirOps {
   LineOpDev op[2];           // deriv, filter
   double* coef[2] = {nullptr, nullptr};       // packed per-point coefficients
   double* coef_ends[2] = {nullptr, nullptr};  // per type A(1), C(1)
+  double* fcoef[2] = {nullptr, nullptr};      // fused per-point coefficients (line_solve_wsp.h)
+  double* fcoef_ends[2] = {nullptr, nullptr}; // per type A(1) B(2nd row), C(1) B(last row)
   LineShape shape[2];
   int* types = nullptr;      // per line, 0-based
   bool uniform_tiles = false;  // every tile of 64 consecutive lines has one type
@@ -159,17 +161,19 @@ int dev_upload(gatt_ctx* c, const T* host, size_t count, T** out) {
   return GATT_OK;
 }
 
-// Default shape before tuning.  Many lines of at most 256 points: the
+// Default shape before tuning.  Many lines of at most 256 points: the fused
 // wave-segment kernel with 16 (or 32) points per lane; otherwise the lane-group
 // kernel with 16 points per lane where the line allows it.
 LineShape choose_shape(int dir, int N, int nR, long long nlines, long long max_lds, bool uniform_tiles) {
   if (nlines >= 1024 && uniform_tiles) {
-    for (int M : {16, 32}) {
-      LineShape s;
-      s.variant = 1;
-      s.M = M;
-      s.G = (N + M - 1) / M;
-      if (line_shape_ok(s, N, nR, dir, max_lds, uniform_tiles)) return s;
+    for (int v : {2, 1}) {
+      for (int M : {16, 32}) {
+        LineShape s;
+        s.variant = v;
+        s.M = M;
+        s.G = (N + M - 1) / M;
+        if (line_shape_ok(s, N, nR, dir, max_lds, uniform_tiles)) return s;
+      }
     }
   }
   for (int M : {16, 8, 4, 32}) {
@@ -238,6 +242,40 @@ int pack_coeffs(gatt_ctx* c, const gatt_lineop& h, double** coef, double** ends)
     }
     en[2 * t] = h.A[(size_t)(N - 1) * t];
     en[2 * t + 1] = h.C[(size_t)(N - 1) * t];
+  }
+  int rc;
+  if ((rc = dev_upload(c, pk.data(), pk.size(), coef))) return rc;
+  return dev_upload(c, en.data(), en.size(), ends);
+}
+
+// Fused records of the branch-free wave-segment kernel, [type][point][FS] with
+// FS = 2nR-1+3 rounded up to even: the band row times the forward factor B of
+// the point, fa' = A B, ce = C, D' = D / B.  The periodic row 0 is not part of
+// the Thomas system: its band row stays unscaled (only its D . r term is used)
+// and its forward factor is zero, as is that of the first solved row; the last
+// row's backward factor is zero.  The end factors are A(1) and C(1) times the B
+// of the rows they correct (rows 2 and N).
+int pack_coeffs_fused(gatt_ctx* c, const gatt_lineop& h, double** coef, double** ends) {
+  const int N = h.n, nt = h.ntypes, NW = 2 * h.nR - 1, FS = (NW + 3 + 1) & ~1, per = h.periodic ? 1 : 0;
+  const int NB = N - per;
+  std::vector<double> pk((size_t)nt * N * FS, 0.0), en((size_t)nt * 2, 0.0);
+  for (int t = 0; t < nt; ++t) {
+    auto Bof = [&](int e) {   // forward factor of row e (periodic: B is shifted by one row)
+      if (per) return e >= 1 ? h.B[(size_t)(e - 1) + (size_t)NB * t] : 1.0;
+      return h.B[(size_t)e + (size_t)NB * t];
+    };
+    for (int e = 0; e < N; ++e) {
+      double* p = &pk[((size_t)t * N + e) * FS];
+      const double b = Bof(e);
+      for (int q = 0; q < NW; ++q) p[q] = h.R[(size_t)e + (size_t)N * (q + (size_t)NW * t)] * b;
+      p[NW] = e > per ? h.A[(size_t)(e - 1) + (size_t)(N - 1) * t] * b : 0.0;
+      p[NW + 1] = e <= N - 2 ? h.C[(size_t)e + (size_t)(N - 1) * t] : 0.0;
+      p[NW + 2] = per ? h.D[(size_t)e + (size_t)N * t] / b : 0.0;
+    }
+    if (per) {
+      en[2 * t] = h.A[(size_t)(N - 1) * t] * Bof(1);
+      en[2 * t + 1] = h.C[(size_t)(N - 1) * t] * Bof(N - 1);
+    }
   }
   int rc;
   if ((rc = dev_upload(c, pk.data(), pk.size(), coef))) return rc;
@@ -552,6 +590,8 @@ void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double
   }
   a.coef = d.coef[kind];
   a.coef_ends = d.coef_ends[kind];
+  a.fcoef = d.fcoef[kind];
+  a.fcoef_ends = d.fcoef_ends[kind];
   a.theta = theta;
   a.mode = mode;
   launch_line_solve(a, d.shape[kind], nmembers, st ? st : c->stream);
@@ -792,6 +832,8 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
     if ((rc = upload_lineop(c.get(), d->filter[k], dims[k], &o.op[1], "filter"))) return rc;
     if ((rc = pack_coeffs(c.get(), d->deriv[k], &o.coef[0], &o.coef_ends[0]))) return rc;
     if ((rc = pack_coeffs(c.get(), d->filter[k], &o.coef[1], &o.coef_ends[1]))) return rc;
+    if ((rc = pack_coeffs_fused(c.get(), d->deriv[k], &o.fcoef[0], &o.fcoef_ends[0]))) return rc;
+    if ((rc = pack_coeffs_fused(c.get(), d->filter[k], &o.fcoef[1], &o.fcoef_ends[1]))) return rc;
     if (k == 0) { o.n1 = d->ny; o.nlines = d->ny * d->nz; o.stride = 1; }
     if (k == 1) { o.n1 = d->nx; o.nlines = d->nx * d->nz; o.stride = d->nx; }
     if (k == 2) { o.n1 = d->nx; o.nlines = d->nx * d->ny; o.stride = (long long)d->nx * d->ny; }
@@ -1080,7 +1122,7 @@ int gatt_set_line_shape(gatt_ctx* c, int32_t dir, int32_t kind, int32_t variant,
     s.G = param;
     s.M = std::max(4, pow2_at_least((N + param - 1) / param));
     s.waves_per_block = wpb;
-  } else if (variant == 1) {
+  } else if (variant == 1 || variant == 2) {
     s.M = param;
     s.G = param > 0 ? (N + param - 1) / param : 0;
     s.waves_per_block = s.G;

@@ -28,6 +28,8 @@ struct LineOpDev {
 // a workgroup of waves_per_block wavefronts holds (64/G)*waves_per_block lines.
 // variant 1 ("wave-segment", line_solve_ws.h): every lane owns a line, wavefront
 // w owns points [w*M, w*M+M) of 64 lines, G = ceil(N/M) wavefronts per workgroup.
+// variant 2 ("fused wave-segment", line_solve_wsp.h): the same layout with fused
+// coefficient records and branch-free sweeps.
 struct LineShape {
   int variant = 0;
   int G = 64;
@@ -50,12 +52,16 @@ struct SolveArgs {
   int lds_coef_offset;        // wave-segment: LDS word offset of the per-wavefront coefficient slots
   int cs;                     // doubles per packed coefficient record (transposed solve)
   int nf;                     // fields per member
+  int nmem;                   // members (persistent wave-segment kernel)
+  int xvec;                   // x lines 16-byte aligned (persistent wave-segment kernel)
   long long in_stride;        // doubles between members in the input arrays
   long long out_stride;       // doubles between members in the output arrays
   const double* in[kMaxPtr];
   double* out[kMaxPtr];
   const double* coef;         // packed per-point coefficients [type][point][2nR-1+4] (wave-segment)
   const double* coef_ends;    // per type: A(1), C(1) of the periodic corrections
+  const double* fcoef;        // fused per-point coefficients [type][point][2nR-1+3] (persistent wave-segment)
+  const double* fcoef_ends;   // per type: A(1) B and C(1) B of the periodic corrections
   double theta;               // filter blend (mode 1)
   int mode;                   // 0 write result; 1 in-place filter with FilterCharTime blend
 };
@@ -64,7 +70,26 @@ template <int DIR>
 void launch_line_solve_dir(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st);
 template <int DIR>
 void launch_line_solve_ws_dir(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st);
+template <int DIR>
+void launch_line_solve_wsp_dir(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st);
+// each direction of each variant is instantiated in its own translation unit
+// (line_solve_{x,y,z}_{lg,ws,wsp}.hip)
+extern template void launch_line_solve_dir<0>(const SolveArgs&, const LineShape&, int, hipStream_t);
+extern template void launch_line_solve_dir<1>(const SolveArgs&, const LineShape&, int, hipStream_t);
+extern template void launch_line_solve_dir<2>(const SolveArgs&, const LineShape&, int, hipStream_t);
+extern template void launch_line_solve_ws_dir<0>(const SolveArgs&, const LineShape&, int, hipStream_t);
+extern template void launch_line_solve_ws_dir<1>(const SolveArgs&, const LineShape&, int, hipStream_t);
+extern template void launch_line_solve_ws_dir<2>(const SolveArgs&, const LineShape&, int, hipStream_t);
+extern template void launch_line_solve_wsp_dir<0>(const SolveArgs&, const LineShape&, int, hipStream_t);
+extern template void launch_line_solve_wsp_dir<1>(const SolveArgs&, const LineShape&, int, hipStream_t);
+extern template void launch_line_solve_wsp_dir<2>(const SolveArgs&, const LineShape&, int, hipStream_t);
 inline void launch_line_solve(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st) {
+  if (s.variant == 2) {
+    if (a.dir == 0) launch_line_solve_wsp_dir<0>(a, s, nmembers, st);
+    else if (a.dir == 1) launch_line_solve_wsp_dir<1>(a, s, nmembers, st);
+    else launch_line_solve_wsp_dir<2>(a, s, nmembers, st);
+    return;
+  }
   if (s.variant == 1) {
     if (a.dir == 0) launch_line_solve_ws_dir<0>(a, s, nmembers, st);
     else if (a.dir == 1) launch_line_solve_ws_dir<1>(a, s, nmembers, st);
@@ -81,6 +106,15 @@ inline void launch_line_solve(const SolveArgs& a, const LineShape& s, int nmembe
 inline bool line_shape_ok(const LineShape& s, int N, int nR, int dir = 1, long long max_lds = 65536,
                           bool uniform_tiles = false) {
   if (!(s.M == 4 || s.M == 8 || s.M == 16 || s.M == 32) || nR < 2 || nR > 11) return false;
+  if (s.variant == 2) {
+    // fused wave-segment: as variant 1, every segment read straight from HBM
+    // (no LDS tile along x); LDS holds the scan maps and coefficient slots
+    if (!(s.M == 16 || s.M == 32) || !uniform_tiles) return false;
+    const int G = (N + s.M - 1) / s.M;
+    if (G < 1 || G > (s.M >= 32 ? 8 : 16) || s.M + nR - 1 > N + 1 || nR - 1 > N) return false;
+    const long long cs = (2LL * nR - 1 + 3 + 1) & ~1LL;
+    return 8 * (5LL * G * 64 + G * (s.M * cs + 2)) <= max_lds;
+  }
   if (s.variant == 1) {
     // wave-segment: compiled for 16 and 32 points per lane, and only for
     // directions whose 64-line tiles each carry one operator type

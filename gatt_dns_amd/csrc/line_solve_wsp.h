@@ -1,0 +1,331 @@
+// Persistent wave-segment line solves with the next item's loads in flight.
+//
+// Same algorithm and per-point arithmetic as line_solve_ws.h (band product,
+// periodic first-row dot product and corrections, segmented Thomas sweeps
+// chained by an LDS scan of affine maps), restating solveSystem{X,Y,Z}[per]
+// (source/Fortran/derivs.F90:556-948).  What changes is the schedule:
+//
+//   * The grid holds as many workgroups as fit on the GPU at once; each walks
+//     the items (64-line tile, field, member) with a stride of the grid size.
+//   * While a workgroup runs the sweeps of one item, the loads of its next item
+//     are already in flight: the next window goes into the window registers as
+//     soon as the band product has consumed them, and the next segment's
+//     coefficients into a few registers per lane, copied to the wavefront's
+//     LDS slot when the item starts.  Barriers only order LDS, so the loads stay
+//     in flight across them.
+//   * Lane l always works on line line0 + l.  Along y and z every point of a
+//     segment is one 512-byte row; along x each lane reads and writes its own
+//     contiguous stretch of the line (pairs of doubles where aligned), the
+//     lines of a tile being 8*nx bytes apart, so no LDS tile is needed.
+#pragma once
+#include "gatt_internal.h"
+
+namespace gatt {
+
+// Fused per-point coefficient record of this kernel (built by the host from the
+// reference's A, B, C, D, R):  the band row scaled by the forward factor,
+// c'_q = R_q B, then fa' = A B, ce = C, D' = D / B (D at the periodic row 0,
+// whose band row is kept unscaled), padded to an even count.  With it the
+// forward sweep d_e = (r_e - A d_{e-1}) B becomes d_e = r'_e - fa' d_{e-1},
+// and every update is one fused multiply-add.  The two periodic end factors
+// are A(1) B and C(1) B of their rows.
+constexpr int wsp_coef_stride(int nR) { return (2 * nR - 1 + 3 + 1) & ~1; }
+constexpr int kMaxSegments = 16;   // wavefronts (segments) per line
+
+// Compiler-only fence between the phases of a sweep: the coefficients are read
+// from LDS again in each phase instead of being held in registers across it.
+__device__ __forceinline__ void wsp_phase() { asm volatile("" ::: "memory"); }
+
+// Sweeps of one item: r holds the scaled band product on entry and the
+// solution on exit; returns the periodic first-row value.  Every point of the
+// segment runs through the same arithmetic: the packed forward factor is zero
+// at the first solved row (and at the periodic row 0 before it) and the
+// backward factor is zero at the last row, so no point needs a branch.
+template <int NR, int M>
+__device__ __forceinline__ double wsp_sweeps(double (&r)[M], const double* cf, const double* cends, double* shD,
+                                             double* shP, double* shB, int cnt, int w, int G, int lane, int per) {
+  double* shQ = shP + G * 64;
+  double* shC = shB + G * 64;
+  constexpr int H = NR - 1;
+  constexpr int NW = 2 * H + 1;
+  constexpr int CS = wsp_coef_stride(NR);
+  double x1 = 0.0;
+  if (per) {
+    double part = 0.0;
+#pragma unroll
+    for (int tt = 0; tt < M; ++tt) part = fma(r[tt], cf[tt * CS + NW + 2], part);
+    shD[w * 64 + lane] = part;
+    __syncthreads();
+    wsp_phase();
+#pragma unroll
+    for (int k = 0; k < kMaxSegments; ++k)
+      if (k < G) x1 += shD[k * 64 + lane];
+    // row 2 and row N corrections (derivs.F90:792-793)
+    if (w == 0) r[1] = fma(-cends[0], x1, r[1]);
+    if (w == G - 1) {
+#pragma unroll
+      for (int tt = 0; tt < M; ++tt)
+        if (tt == cnt - 1) r[tt] = fma(-cends[1], x1, r[tt]);
+    }
+  }
+
+  // forward sweep: composite affine map of the segment, scan, re-run
+  double P = 0.0, Q = 1.0;
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    const double fa = cf[tt * CS + NW];
+    P = fma(-fa, P, r[tt]);
+    Q = -fa * Q;
+  }
+  shP[w * 64 + lane] = P;
+  shQ[w * 64 + lane] = Q;
+  __syncthreads();
+  wsp_phase();
+  double dprev = 0.0;
+#pragma unroll
+  for (int k = 0; k < kMaxSegments; ++k)
+    if (k < w) dprev = fma(shQ[k * 64 + lane], dprev, shP[k * 64 + lane]);
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    const double d = fma(-cf[tt * CS + NW], dprev, r[tt]);
+    r[tt] = d;
+    dprev = d;
+  }
+
+  // backward sweep
+  wsp_phase();
+  P = 0.0;
+  Q = 1.0;
+#pragma unroll
+  for (int tt = M - 1; tt >= 0; --tt) {
+    const double ce = cf[tt * CS + NW + 1];
+    P = fma(-ce, P, r[tt]);
+    Q = -ce * Q;
+  }
+  shB[w * 64 + lane] = P;
+  shC[w * 64 + lane] = Q;
+  __syncthreads();
+  wsp_phase();
+  double xnext = 0.0;
+#pragma unroll
+  for (int k = kMaxSegments - 1; k >= 0; --k)
+    if (k > w && k < G) xnext = fma(shC[k * 64 + lane], xnext, shB[k * 64 + lane]);
+#pragma unroll
+  for (int tt = M - 1; tt >= 0; --tt) {
+    const double x = fma(-cf[tt * CS + NW + 1], xnext, r[tt]);
+    r[tt] = x;
+    xnext = x;
+  }
+  return x1;
+}
+
+template <int NR, int M, bool BLEND>
+__device__ __forceinline__ void wsp_band(double (&r)[M], double (&f0)[BLEND ? M : 1],
+                                         const double (&fw)[M + 2 * (NR - 1)], const double* cf) {
+  constexpr int H = NR - 1;
+  constexpr int CS = wsp_coef_stride(NR);
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    const double* c = cf + tt * CS;
+    double acc = c[H] * fw[tt + H];
+#pragma unroll
+    for (int q = 1; q <= H; ++q) acc = fma(c[H - q], fw[tt + H - q], fma(c[H + q], fw[tt + H + q], acc));
+    r[tt] = acc;
+  
+    if (BLEND) f0[tt] = fw[tt + H];
+  }
+}
+
+template <int DIR>
+__device__ __forceinline__ long long wsp_base(const SolveArgs& a, int line) {
+  if (DIR == 1) {
+    const int k = line / a.nx;
+    return (long long)(line - k * a.nx) + (long long)a.nx * a.ny * k;
+  }
+  return line;   // DIR 2 (x uses the contiguous stretch)
+}
+
+// x: lane l holds the window of line line0 + l, points st .. st+W-1 (wrapped),
+// read straight from its contiguous line.  Where the window does not wrap and
+// the line is 16-byte aligned, it is read as pairs of doubles.
+template <int W>
+__device__ __forceinline__ void wsp_load_x(const double* src, int N, int st, bool vec, double (&fw)[W]) {
+  constexpr int NP = (W + 2) / 2;
+  const int off = st & 1;
+  const int a0 = st - off;
+  if (vec && a0 >= 0 && a0 + 2 * NP <= N) {
+    double t[2 * NP];
+    const double2* p = reinterpret_cast<const double2*>(src + a0);
+#pragma unroll
+    for (int u = 0; u < NP; ++u) {
+      const double2 v = p[u];
+      t[2 * u] = v.x;
+      t[2 * u + 1] = v.y;
+    }
+#pragma unroll
+    for (int u = 0; u < W; ++u) fw[u] = off ? t[u + 1] : t[u];
+  } else {
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      int e = st + u;
+      e = e < 0 ? e + N : (e >= N ? e - N : e);
+      e = e >= N ? N - 1 : e;
+      fw[u] = src[e];
+    }
+  }
+}
+
+// Two workgroups per compute unit where the registers allow it (64 lines x 256
+// points: 16 wavefronts of 16 points, or 8 of 32).
+template <int NR, int M, bool BLEND>
+constexpr int wsp_min_waves() { return M >= 32 ? 4 : ((NR <= 4 && !BLEND) ? 8 : 1); }
+
+template <int DIR, int NR, int M, bool BLEND>
+__global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>())) void line_solve_wsp_kernel(SolveArgs a) {
+  extern __shared__ double sh[];
+  constexpr int H = NR - 1;
+  constexpr int CS = wsp_coef_stride(NR);
+  constexpr int W = M + 2 * H;
+  constexpr int SLOT = M * CS + 2;             // coefficient words of one segment + the two end factors
+  constexpr bool XL = DIR == 0;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int G = blockDim.x >> 6;
+  const int N = a.op.N;
+  const int per = a.op.periodic;
+  const int s = w * M;
+  int cnt = N - s;
+  cnt = cnt < 0 ? 0 : (cnt > M ? M : cnt);
+  double* shD = sh;                              // periodic dot partials [G][64]
+  double* shP = sh + G * 64;                     // forward maps  [2][G][64]
+  double* shB = sh + 3 * G * 64;                 // backward maps [2][G][64]
+  double* cl = sh + 5 * G * 64 + w * SLOT;       // this wavefront's coefficients
+  const long long stride = a.stride;
+  const int line0 = blockIdx.x * 64;
+  const int nact = min(64, a.nlines - line0);
+  const int f = (int)blockIdx.y % a.nf;
+  const int mem = (int)blockIdx.y / a.nf;
+  const double* in = a.in[f] + (long long)mem * a.in_stride;
+  double* out = a.out[f] + (long long)mem * a.out_stride;
+  const int line = lane < nact ? line0 + lane : line0;
+
+  double fw[W];
+  if (XL) {
+    wsp_load_x<W>(in + (long long)a.nx * line, N, s - H, a.xvec != 0, fw);
+  } else {
+    const long long base = wsp_base<DIR>(a, line);
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      int e = s - H + u;
+      e = e < 0 ? e + N : (e >= N ? e - N : e);
+      e = e >= N ? N - 1 : e;
+      fw[u] = in[base + (long long)e * stride];
+    }
+  }
+  // the segment's coefficients; points past the line end get zeros, which make
+  // their band rows, forward factors and backward factors vanish.  Only this
+  // wavefront reads its slot, and its LDS accesses run in order.
+  {
+    const int tu = __builtin_amdgcn_readfirstlane(a.types[line0]);
+    const double* src = a.fcoef + ((size_t)tu * N + s) * CS;
+    const double* ends = a.fcoef_ends + (size_t)tu * 2;
+    for (int jj = lane; jj < SLOT; jj += 64) {
+      double v = 0.0;
+      if (jj < cnt * CS) v = src[jj];
+      else if (jj >= M * CS) v = ends[jj - M * CS];
+      cl[jj] = v;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  const double* cf = cl;
+  const double* cends = cl + M * CS;
+  double r[M];
+  double f0[BLEND ? M : 1];
+  wsp_band<NR, M, BLEND>(r, f0, fw, cf);
+  const double x1 = wsp_sweeps<NR, M>(r, cf, cends, shD, shP, shB, cnt, w, G, lane, per);
+  if (per && w == 0) r[0] = x1;
+  if (BLEND) {
+#pragma unroll
+    for (int tt = 0; tt < M; ++tt) r[tt] = fma(a.theta, r[tt] - f0[tt], f0[tt]);
+  }
+  const bool full = cnt == M && nact == 64;
+  if (XL) {
+    double* dst = out + (long long)a.nx * line + s;
+    if (full && a.xvec) {
+      double2* p = reinterpret_cast<double2*>(dst);
+#pragma unroll
+      for (int u = 0; u < M / 2; ++u) p[u] = make_double2(r[2 * u], r[2 * u + 1]);
+    } else if (lane < nact) {
+#pragma unroll
+      for (int tt = 0; tt < M; ++tt)
+        if (tt < cnt) dst[tt] = r[tt];
+    }
+  } else {
+    const long long base = wsp_base<DIR>(a, line) + (long long)s * stride;
+    if (full) {
+#pragma unroll
+      for (int tt = 0; tt < M; ++tt) out[base + (long long)tt * stride] = r[tt];
+    } else if (lane < nact) {
+#pragma unroll
+      for (int tt = 0; tt < M; ++tt)
+        if (tt < cnt) out[base + (long long)tt * stride] = r[tt];
+    }
+  }
+}
+
+// LDS words: scan maps (5*G*64), then one coefficient slot per wavefront.
+inline size_t wsp_lds_bytes(int G, int M, int nR) {
+  return sizeof(double) * ((size_t)5 * G * 64 + (size_t)G * (M * wsp_coef_stride(nR) + 2));
+}
+
+template <int DIR, int NR, int M, bool BLEND>
+static void launch_wsp_k(const SolveArgs& a, int G, hipStream_t st) {
+  static_assert(M % 2 == 0, "pairs of points per lane");
+  auto k = line_solve_wsp_kernel<DIR, NR, M, BLEND>;
+  const size_t lds = wsp_lds_bytes(G, M, NR);
+  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  dim3 grid((unsigned)((a.nlines + 63) / 64), (unsigned)(a.nf * a.nmem));
+  hipLaunchKernelGGL(k, grid, dim3(64 * G), lds, st, a);
+}
+
+template <int DIR, int NR, int M>
+static void launch_wsp_m(const SolveArgs& a0, int nmembers, hipStream_t st) {
+  SolveArgs a = a0;
+  a.inv_n = 1.0f / (float)a.op.N;
+  a.nmem = nmembers;
+  // x lines are read and written as pairs of doubles when every line starts 16-byte aligned
+  bool vec = DIR == 0 && a.nx % 2 == 0 && a.in_stride % 2 == 0 && a.out_stride % 2 == 0;
+  for (int f = 0; f < a.nf && vec; ++f)
+    vec = ((uintptr_t)a.in[f] % 16 == 0) && ((uintptr_t)a.out[f] % 16 == 0);
+  a.xvec = vec ? 1 : 0;
+  const int G = (a.op.N + M - 1) / M;
+  if (a.mode == 1 && a.theta > 0.0) launch_wsp_k<DIR, NR, M, true>(a, G, st);
+  else launch_wsp_k<DIR, NR, M, false>(a, G, st);
+}
+
+template <int DIR, int NR>
+static void launch_wsp_nr(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st) {
+  switch (s.M) {
+    case 16: launch_wsp_m<DIR, NR, 16>(a, nmembers, st); break;
+    case 32: launch_wsp_m<DIR, NR, 32>(a, nmembers, st); break;
+  }
+}
+
+template <int DIR>
+void launch_line_solve_wsp_dir(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st) {
+  switch (a.op.nR) {
+    case 2: launch_wsp_nr<DIR, 2>(a, s, nmembers, st); break;
+    case 3: launch_wsp_nr<DIR, 3>(a, s, nmembers, st); break;
+    case 4: launch_wsp_nr<DIR, 4>(a, s, nmembers, st); break;
+    case 5: launch_wsp_nr<DIR, 5>(a, s, nmembers, st); break;
+    case 6: launch_wsp_nr<DIR, 6>(a, s, nmembers, st); break;
+    case 7: launch_wsp_nr<DIR, 7>(a, s, nmembers, st); break;
+    case 8: launch_wsp_nr<DIR, 8>(a, s, nmembers, st); break;
+    case 9: launch_wsp_nr<DIR, 9>(a, s, nmembers, st); break;
+    case 10: launch_wsp_nr<DIR, 10>(a, s, nmembers, st); break;
+    case 11: launch_wsp_nr<DIR, 11>(a, s, nmembers, st); break;
+    default: break;
+  }
+}
+
+}  // namespace gatt

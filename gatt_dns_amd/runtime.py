@@ -201,7 +201,7 @@ class Solver:
         return ms.value, by.value
 
     def set_line_shape(self, direction, kind, variant, param, waves_per_block=4):
-        """variant 0: param = lanes per line; variant 1: param = points per lane."""
+        """variant 0: param = lanes per line; variants 1 and 2: param = points per lane."""
         _check(self.lib.gatt_set_line_shape(self.ctx, direction, kind, variant, param, waves_per_block))
 
     def get_line_shape(self, direction, kind):

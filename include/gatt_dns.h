@@ -214,7 +214,11 @@ int gatt_sync(gatt_ctx* ctx);
  *                             lane owns a line, ceil(n/M) wavefronts per
  *                             workgroup (waves_per_block ignored); only for
  *                             directions whose tiles of 64 consecutive lines
- *                             each have one operator type.
+ *                             each have one operator type;
+ *   variant 2, fused wave segments: as variant 1, with per-point coefficient
+ *                             records folded into single fused multiply-adds
+ *                             and no branches in the sweeps (results agree
+ *                             with variants 0 and 1 to rounding).
  * Shapes that do not fit the line or the device's LDS are rejected. */
 int gatt_set_line_shape(gatt_ctx* ctx, int32_t dir, int32_t kind, int32_t variant, int32_t param,
                         int32_t waves_per_block);
