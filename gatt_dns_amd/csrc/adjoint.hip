@@ -8,6 +8,8 @@
 // parts (line solves, RK combinations, filter, SFD) reuse the forward kernels
 // for the tangent; the adjoint uses the transposed solves (line_solve_t.hip)
 // and the transposed RHS band below.
+#include <algorithm>
+
 #include "gatt_internal.h"
 
 namespace gatt {
@@ -16,16 +18,19 @@ static inline unsigned grid_for(long long n, int threads) { return (unsigned)((n
 
 // -------------------------------------------------------------- band^T
 // out = beta*src + (accumulate ? out : 0) + alpha * sum_l R((e-l) mod N, l) in(e-l)
+// Grid: x over i in blocks of 256 threads, y over j, z over (k, field, member),
+// so every index comes from the grid without integer division per point.
 template <int DIR>
-__global__ __launch_bounds__(256) void band_t_kernel(BandTArgs a) {
-  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (p >= a.np) return;
-  const int f = (int)blockIdx.y % a.nf;
-  const int mem = (int)blockIdx.y / a.nf;
+__global__ __launch_bounds__(256) void band_t_kernel(BandTArgs a, int m0) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.nx) return;
+  const int j = blockIdx.y;
+  const int k = (int)blockIdx.z % a.nz;
+  const int fm = (int)blockIdx.z / a.nz;
+  const int f = fm % a.nf;
+  const int mem = m0 + fm / a.nf;
   const long long nxy = (long long)a.nx * a.ny;
-  const int i = (int)(p % a.nx);
-  const int j = (int)((p / a.nx) % a.ny);
-  const int k = (int)(p / nxy);
+  const long long p = (long long)i + (long long)a.nx * j + nxy * k;
   const int e = DIR == 0 ? i : (DIR == 1 ? j : k);
   const int line = DIR == 0 ? j + a.ny * k : (DIR == 1 ? i + a.nx * k : i + a.nx * j);
   const long long stride = DIR == 0 ? 1 : (DIR == 1 ? (long long)a.nx : nxy);
@@ -46,10 +51,15 @@ __global__ __launch_bounds__(256) void band_t_kernel(BandTArgs a) {
 }
 
 void launch_band_t(const BandTArgs& a, int nmembers, hipStream_t st) {
-  dim3 grid(grid_for(a.np, 256), (unsigned)(a.nf * nmembers));
-  if (a.dir == 0) hipLaunchKernelGGL(band_t_kernel<0>, grid, dim3(256), 0, st, a);
-  else if (a.dir == 1) hipLaunchKernelGGL(band_t_kernel<1>, grid, dim3(256), 0, st, a);
-  else hipLaunchKernelGGL(band_t_kernel<2>, grid, dim3(256), 0, st, a);
+  // members in chunks that keep the z extent of the grid within its limit
+  const int per_chunk = std::max(1, 65535 / (a.nz * a.nf));
+  for (int m0 = 0; m0 < nmembers; m0 += per_chunk) {
+    const int nm = std::min(per_chunk, nmembers - m0);
+    dim3 grid((unsigned)((a.nx + 255) / 256), (unsigned)a.ny, (unsigned)(a.nz * a.nf * nm));
+    if (a.dir == 0) hipLaunchKernelGGL(band_t_kernel<0>, grid, dim3(256), 0, st, a, m0);
+    else if (a.dir == 1) hipLaunchKernelGGL(band_t_kernel<1>, grid, dim3(256), 0, st, a, m0);
+    else hipLaunchKernelGGL(band_t_kernel<2>, grid, dim3(256), 0, st, a, m0);
+  }
 }
 
 // -------------------------------------------------------------- axpby

@@ -1122,7 +1122,7 @@ int gatt_set_line_shape(gatt_ctx* c, int32_t dir, int32_t kind, int32_t variant,
     s.G = param;
     s.M = std::max(4, pow2_at_least((N + param - 1) / param));
     s.waves_per_block = wpb;
-  } else if (variant == 1 || variant == 2) {
+  } else if (variant >= 1 && variant <= 3) {
     s.M = param;
     s.G = param > 0 ? (N + param - 1) / param : 0;
     s.waves_per_block = s.G;

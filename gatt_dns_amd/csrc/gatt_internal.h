@@ -29,7 +29,8 @@ struct LineOpDev {
 // variant 1 ("wave-segment", line_solve_ws.h): every lane owns a line, wavefront
 // w owns points [w*M, w*M+M) of 64 lines, G = ceil(N/M) wavefronts per workgroup.
 // variant 2 ("fused wave-segment", line_solve_wsp.h): the same layout with fused
-// coefficient records and branch-free sweeps.
+// coefficient records and branch-free sweeps; variant 3: variant 2 with the x
+// lines staged through an LDS tile.
 struct LineShape {
   int variant = 0;
   int G = 64;
@@ -52,8 +53,9 @@ struct SolveArgs {
   int lds_coef_offset;        // wave-segment: LDS word offset of the per-wavefront coefficient slots
   int cs;                     // doubles per packed coefficient record (transposed solve)
   int nf;                     // fields per member
-  int nmem;                   // members (persistent wave-segment kernel)
-  int xvec;                   // x lines 16-byte aligned (persistent wave-segment kernel)
+  int nmem;                   // members (fused wave-segment kernel)
+  int xvec;                   // x lines 16-byte aligned (fused wave-segment kernel)
+  int xtile;                  // x lines staged through an LDS tile (fused wave-segment kernel)
   long long in_stride;        // doubles between members in the input arrays
   long long out_stride;       // doubles between members in the output arrays
   const double* in[kMaxPtr];
@@ -84,10 +86,12 @@ extern template void launch_line_solve_wsp_dir<0>(const SolveArgs&, const LineSh
 extern template void launch_line_solve_wsp_dir<1>(const SolveArgs&, const LineShape&, int, hipStream_t);
 extern template void launch_line_solve_wsp_dir<2>(const SolveArgs&, const LineShape&, int, hipStream_t);
 inline void launch_line_solve(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st) {
-  if (s.variant == 2) {
-    if (a.dir == 0) launch_line_solve_wsp_dir<0>(a, s, nmembers, st);
-    else if (a.dir == 1) launch_line_solve_wsp_dir<1>(a, s, nmembers, st);
-    else launch_line_solve_wsp_dir<2>(a, s, nmembers, st);
+  if (s.variant == 2 || s.variant == 3) {
+    SolveArgs b = a;
+    b.xtile = s.variant == 3;
+    if (a.dir == 0) launch_line_solve_wsp_dir<0>(b, s, nmembers, st);
+    else if (a.dir == 1) launch_line_solve_wsp_dir<1>(b, s, nmembers, st);
+    else launch_line_solve_wsp_dir<2>(b, s, nmembers, st);
     return;
   }
   if (s.variant == 1) {
@@ -106,14 +110,16 @@ inline void launch_line_solve(const SolveArgs& a, const LineShape& s, int nmembe
 inline bool line_shape_ok(const LineShape& s, int N, int nR, int dir = 1, long long max_lds = 65536,
                           bool uniform_tiles = false) {
   if (!(s.M == 4 || s.M == 8 || s.M == 16 || s.M == 32) || nR < 2 || nR > 11) return false;
-  if (s.variant == 2) {
+  if (s.variant == 2 || s.variant == 3) {
     // fused wave-segment: as variant 1, every segment read straight from HBM
-    // (no LDS tile along x); LDS holds the scan maps and coefficient slots
-    if (!(s.M == 16 || s.M == 32) || !uniform_tiles) return false;
+    // (variant 2) or, along x only, through an LDS tile (variant 3); LDS holds
+    // the scan maps and coefficient slots (in the tile's words for variant 3)
+    if (!(s.M == 16 || s.M == 32) || !uniform_tiles || (s.variant == 3 && dir != 0)) return false;
     const int G = (N + s.M - 1) / s.M;
     if (G < 1 || G > (s.M >= 32 ? 8 : 16) || s.M + nR - 1 > N + 1 || nR - 1 > N) return false;
     const long long cs = (2LL * nR - 1 + 3 + 1) & ~1LL;
-    return 8 * (5LL * G * 64 + G * (s.M * cs + 2)) <= max_lds;
+    const long long work = 5LL * G * 64 + G * (s.M * cs + 2), tile = s.variant == 3 ? 65LL * N : 0;
+    return 8 * (work > tile ? work : tile) <= max_lds;
   }
   if (s.variant == 1) {
     // wave-segment: compiled for 16 and 32 points per lane, and only for

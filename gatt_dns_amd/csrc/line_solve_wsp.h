@@ -1,22 +1,23 @@
-// Persistent wave-segment line solves with the next item's loads in flight.
+// Fused wave-segment line solves.
 //
-// Same algorithm and per-point arithmetic as line_solve_ws.h (band product,
-// periodic first-row dot product and corrections, segmented Thomas sweeps
-// chained by an LDS scan of affine maps), restating solveSystem{X,Y,Z}[per]
-// (source/Fortran/derivs.F90:556-948).  What changes is the schedule:
+// Same algorithm as line_solve_ws.h (band product, periodic first-row dot
+// product and corrections, segmented Thomas sweeps chained by an LDS scan of
+// affine maps), restating solveSystem{X,Y,Z}[per]
+// (source/Fortran/derivs.F90:556-948), with a leaner instruction stream:
 //
-//   * The grid holds as many workgroups as fit on the GPU at once; each walks
-//     the items (64-line tile, field, member) with a stride of the grid size.
-//   * While a workgroup runs the sweeps of one item, the loads of its next item
-//     are already in flight: the next window goes into the window registers as
-//     soon as the band product has consumed them, and the next segment's
-//     coefficients into a few registers per lane, copied to the wavefront's
-//     LDS slot when the item starts.  Barriers only order LDS, so the loads stay
-//     in flight across them.
-//   * Lane l always works on line line0 + l.  Along y and z every point of a
-//     segment is one 512-byte row; along x each lane reads and writes its own
-//     contiguous stretch of the line (pairs of doubles where aligned), the
-//     lines of a tile being 8*nx bytes apart, so no LDS tile is needed.
+//   * A workgroup owns 64 lines; lane l of every wavefront works on line
+//     line0 + l and wavefront w on the points [w*M, w*M+M) of all of them.
+//   * Per-point coefficient records are pre-multiplied on the host so that the
+//     band product and every sweep update are single fused multiply-adds.
+//   * No point carries a branch: coefficients past the line end are zero, the
+//     forward factor of the first solved row and the backward factor of the
+//     last row are zero, so every point runs the same arithmetic.
+//   * Scan maps keep P and Q in separate LDS rows of 64 lanes (conflict-free).
+//   * Along y and z every point of a segment is one 512-byte row; along x each
+//     lane reads and writes its own contiguous stretch of its line (pairs of
+//     doubles where aligned), so no LDS tile is needed.
+//   * Results differ from the reference's sequential recurrence by rounding
+//     only (the fused operations and the scan's composite maps).
 #pragma once
 #include "gatt_internal.h"
 
@@ -180,13 +181,19 @@ __device__ __forceinline__ void wsp_load_x(const double* src, int N, int st, boo
 template <int NR, int M, bool BLEND>
 constexpr int wsp_min_waves() { return M >= 32 ? 4 : ((NR <= 4 && !BLEND) ? 8 : 1); }
 
-template <int DIR, int NR, int M, bool BLEND>
+// XT (x lines only): the 64 lines of the workgroup are one contiguous stretch,
+// moved between HBM and an LDS tile (N rows of 65 words) with unit-stride
+// accesses; the scan maps and coefficient slots reuse the tile's words between
+// the window reads and the result writes.  Without XT each lane reads and
+// writes its own line directly.
+template <int DIR, int NR, int M, bool BLEND, bool XT>
 __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>())) void line_solve_wsp_kernel(SolveArgs a) {
   extern __shared__ double sh[];
   constexpr int H = NR - 1;
   constexpr int CS = wsp_coef_stride(NR);
   constexpr int W = M + 2 * H;
   constexpr int SLOT = M * CS + 2;             // coefficient words of one segment + the two end factors
+  constexpr int CP = (SLOT + 63) / 64;          // coefficient words per lane
   constexpr bool XL = DIR == 0;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -200,6 +207,7 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>(
   double* shP = sh + G * 64;                     // forward maps  [2][G][64]
   double* shB = sh + 3 * G * 64;                 // backward maps [2][G][64]
   double* cl = sh + 5 * G * 64 + w * SLOT;       // this wavefront's coefficients
+  double* tile = sh;                             // XT: N rows of 65 words
   const long long stride = a.stride;
   const int line0 = blockIdx.x * 64;
   const int nact = min(64, a.nlines - line0);
@@ -209,8 +217,43 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>(
   double* out = a.out[f] + (long long)mem * a.out_stride;
   const int line = lane < nact ? line0 + lane : line0;
 
+  // the segment's coefficients, in flight with the field loads; points past the
+  // line end get zeros, which make their band rows, forward factors and
+  // backward factors vanish
+  double cpre[CP];
+  {
+    const int tu = __builtin_amdgcn_readfirstlane(a.types[line0]);
+    const double* src = a.fcoef + ((size_t)tu * N + s) * CS;
+    const double* ends = a.fcoef_ends + (size_t)tu * 2;
+#pragma unroll
+    for (int k = 0; k < CP; ++k) {
+      const int jj = lane + 64 * k;
+      double v = 0.0;
+      if (jj < cnt * CS) v = src[jj];
+      else if (jj >= M * CS && jj < SLOT) v = ends[jj - M * CS];
+      cpre[k] = v;
+    }
+  }
+
   double fw[W];
-  if (XL) {
+  if (XL && XT) {
+    const double* src = in + (long long)a.nx * line0;
+    const int total = nact * N;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+      const int ep = idx - lp * N;
+      tile[ep * 65 + lp] = src[idx];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      int e = s - H + u;
+      e = e < 0 ? e + N : (e >= N ? e - N : e);
+      e = e >= N ? N - 1 : e;
+      fw[u] = tile[e * 65 + lane];
+    }
+    __syncthreads();   // the tile's words now hold the maps and coefficient slots
+  } else if (XL) {
     wsp_load_x<W>(in + (long long)a.nx * line, N, s - H, a.xvec != 0, fw);
   } else {
     const long long base = wsp_base<DIR>(a, line);
@@ -222,21 +265,13 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>(
       fw[u] = in[base + (long long)e * stride];
     }
   }
-  // the segment's coefficients; points past the line end get zeros, which make
-  // their band rows, forward factors and backward factors vanish.  Only this
-  // wavefront reads its slot, and its LDS accesses run in order.
-  {
-    const int tu = __builtin_amdgcn_readfirstlane(a.types[line0]);
-    const double* src = a.fcoef + ((size_t)tu * N + s) * CS;
-    const double* ends = a.fcoef_ends + (size_t)tu * 2;
-    for (int jj = lane; jj < SLOT; jj += 64) {
-      double v = 0.0;
-      if (jj < cnt * CS) v = src[jj];
-      else if (jj >= M * CS) v = ends[jj - M * CS];
-      cl[jj] = v;
-    }
-    __builtin_amdgcn_wave_barrier();
+  // only this wavefront reads its slot, and its LDS accesses run in order
+#pragma unroll
+  for (int k = 0; k < CP; ++k) {
+    const int jj = lane + 64 * k;
+    if (jj < SLOT) cl[jj] = cpre[k];
   }
+  __builtin_amdgcn_wave_barrier();
   const double* cf = cl;
   const double* cends = cl + M * CS;
   double r[M];
@@ -249,7 +284,20 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>(
     for (int tt = 0; tt < M; ++tt) r[tt] = fma(a.theta, r[tt] - f0[tt], f0[tt]);
   }
   const bool full = cnt == M && nact == 64;
-  if (XL) {
+  if (XL && XT) {
+    __syncthreads();   // every map and slot read before the tile is rewritten
+#pragma unroll
+    for (int tt = 0; tt < M; ++tt)
+      if (tt < cnt) tile[(s + tt) * 65 + lane] = r[tt];
+    __syncthreads();
+    double* dst = out + (long long)a.nx * line0;
+    const int total = nact * N;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+      const int ep = idx - lp * N;
+      dst[idx] = tile[ep * 65 + lp];
+    }
+  } else if (XL) {
     double* dst = out + (long long)a.nx * line + s;
     if (full && a.xvec) {
       double2* p = reinterpret_cast<double2*>(dst);
@@ -274,15 +322,18 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>(
 }
 
 // LDS words: scan maps (5*G*64), then one coefficient slot per wavefront.
-inline size_t wsp_lds_bytes(int G, int M, int nR) {
-  return sizeof(double) * ((size_t)5 * G * 64 + (size_t)G * (M * wsp_coef_stride(nR) + 2));
+// XT: the x tile (65*N words) holds them in turn.
+inline size_t wsp_lds_bytes(int G, int M, int nR, bool xt = false, int N = 0) {
+  const size_t work = (size_t)5 * G * 64 + (size_t)G * (M * wsp_coef_stride(nR) + 2);
+  const size_t tile = xt ? (size_t)65 * N : 0;
+  return sizeof(double) * (work > tile ? work : tile);
 }
 
-template <int DIR, int NR, int M, bool BLEND>
+template <int DIR, int NR, int M, bool BLEND, bool XT>
 static void launch_wsp_k(const SolveArgs& a, int G, hipStream_t st) {
   static_assert(M % 2 == 0, "pairs of points per lane");
-  auto k = line_solve_wsp_kernel<DIR, NR, M, BLEND>;
-  const size_t lds = wsp_lds_bytes(G, M, NR);
+  auto k = line_solve_wsp_kernel<DIR, NR, M, BLEND, XT>;
+  const size_t lds = wsp_lds_bytes(G, M, NR, XT, a.op.N);
   if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   dim3 grid((unsigned)((a.nlines + 63) / 64), (unsigned)(a.nf * a.nmem));
   hipLaunchKernelGGL(k, grid, dim3(64 * G), lds, st, a);
@@ -299,8 +350,16 @@ static void launch_wsp_m(const SolveArgs& a0, int nmembers, hipStream_t st) {
     vec = ((uintptr_t)a.in[f] % 16 == 0) && ((uintptr_t)a.out[f] % 16 == 0);
   a.xvec = vec ? 1 : 0;
   const int G = (a.op.N + M - 1) / M;
-  if (a.mode == 1 && a.theta > 0.0) launch_wsp_k<DIR, NR, M, true>(a, G, st);
-  else launch_wsp_k<DIR, NR, M, false>(a, G, st);
+  const bool blend = a.mode == 1 && a.theta > 0.0;
+  if constexpr (DIR == 0) {
+    if (a.xtile) {
+      if (blend) launch_wsp_k<DIR, NR, M, true, true>(a, G, st);
+      else launch_wsp_k<DIR, NR, M, false, true>(a, G, st);
+      return;
+    }
+  }
+  if (blend) launch_wsp_k<DIR, NR, M, true, false>(a, G, st);
+  else launch_wsp_k<DIR, NR, M, false, false>(a, G, st);
 }
 
 template <int DIR, int NR>

@@ -218,7 +218,9 @@ int gatt_sync(gatt_ctx* ctx);
  *   variant 2, fused wave segments: as variant 1, with per-point coefficient
  *                             records folded into single fused multiply-adds
  *                             and no branches in the sweeps (results agree
- *                             with variants 0 and 1 to rounding).
+ *                             with variants 0 and 1 to rounding);
+ *   variant 3 (x only): variant 2 with the 64 lines of a workgroup staged
+ *                             through an LDS tile.
  * Shapes that do not fit the line or the device's LDS are rejected. */
 int gatt_set_line_shape(gatt_ctx* ctx, int32_t dir, int32_t kind, int32_t variant, int32_t param,
                         int32_t waves_per_block);
