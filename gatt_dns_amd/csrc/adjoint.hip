@@ -34,14 +34,16 @@ __global__ __launch_bounds__(256) void band_t_kernel(BandTArgs a, int m0) {
   const int e = DIR == 0 ? i : (DIR == 1 ? j : k);
   const int line = DIR == 0 ? j + a.ny * k : (DIR == 1 ? i + a.nx * k : i + a.nx * j);
   const long long stride = DIR == 0 ? 1 : (DIR == 1 ? (long long)a.nx : nxy);
-  const int N = a.N, H = a.nR - 1, CS = a.cs;
-  const double* cf = a.coef + (size_t)a.types[line] * N * CS;
+  const int N = a.N, H = a.nR - 1, NW = 2 * a.nR - 1;
+  // the reference's band layout R(point, diagonal, type): along x consecutive
+  // threads read consecutive words of one diagonal
+  const double* Rt = a.R + (size_t)a.types[line] * N * NW;
   const double* in = a.in[f] + (long long)mem * a.in_stride;
   double acc = 0.0;
   for (int l = -H; l <= H; ++l) {
     int se = e - l;
     se = se < 0 ? se + N : (se >= N ? se - N : se);
-    acc = acc + cf[(size_t)se * CS + H + l] * in[p + (long long)(se - e) * stride];
+    acc = acc + Rt[(size_t)(H + l) * N + se] * in[p + (long long)(se - e) * stride];
   }
   double* out = a.out[f] + (long long)mem * a.out_stride;
   double v = a.alpha * acc;

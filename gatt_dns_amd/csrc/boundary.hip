@@ -138,6 +138,31 @@ void launch_bc_wave(const BcArgs& a, long long nblocks, int nmembers, hipStream_
   hipLaunchKernelGGL(bc_wave_kernel, dim3((unsigned)nblocks, nmembers), dim3(256), 0, st, a);
 }
 
+// dst <- src for all five fields on the points of the boxes of one wave's ops
+// (the points a replay of that wave may have written).
+__global__ __launch_bounds__(256) void bc_restore_kernel(BcCopyArgs a) {
+  const long long blk = blockIdx.x;
+  int o = a.op_begin;
+  while (o + 1 < a.op_end && a.ops[o + 1].first_block <= blk) ++o;
+  const BcOp& op = a.ops[o];
+  const long long p = (blk - op.first_block) * blockDim.x + threadIdx.x;
+  if (p >= op.npts) return;
+  const int bi = op.box[1] - op.box[0] + 1;
+  const int bj = op.box[3] - op.box[2] + 1;
+  const int i = op.box[0] + (int)(p % bi);
+  const int j = op.box[2] + (int)((p / bi) % bj);
+  const int k = op.box[4] + (int)(p / ((long long)bi * bj));
+  const long long n = pidx(i, j, k, a.nx, a.ny);
+  const long long ms = (long long)blockIdx.y * a.src_stride, md = (long long)blockIdx.y * a.dst_stride;
+#pragma unroll
+  for (int f = 0; f < 5; ++f) a.dst[f][md + n] = a.src[f][ms + n];
+}
+
+void launch_bc_restore(const BcCopyArgs& a, long long nblocks, int nmembers, hipStream_t st) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(bc_restore_kernel, dim3((unsigned)nblocks, nmembers), dim3(256), 0, st, a);
+}
+
 }  // namespace gatt
 
 namespace gatt {

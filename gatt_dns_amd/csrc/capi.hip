@@ -1862,6 +1862,7 @@ void run_solve_t(gatt_ctx* c, int dir, int kind, int nf, double* const* in, doub
   b.cs = a.cs;
   b.np = c->np;
   b.coef = d.coef[kind];
+  b.R = d.op[kind].R;
   b.types = d.types;
   b.nf = nf;
   b.in_stride = in_stride;
@@ -2006,23 +2007,45 @@ bool wave_needs_base(const gatt_ctx* c, const BcWave& w) {
 void run_bc_adj(gatt_ctx* c, double* qpre, long long qs, double* lam) {
   const long long ls = c->lin_stride;
   double* tmp = lin_ptr(c, LN_TMP);
+  // tmp holds qpre with the first `applied` waves replayed on it (-1: not a copy of qpre yet)
+  int applied = -1;
   for (int wi = (int)c->waves.size() - 1; wi >= 0; --wi) {
     const BcWave& w = c->waves[(size_t)wi];
     double* base = qpre;
     long long bs = qs;
     if (wave_needs_base(c, w)) {
-      AxpbyArgs cp{};
-      cp.np = c->np;
-      cp.x_stride = qs;
-      cp.y_stride = ls;
-      cp.nf = 5;
-      for (int f = 0; f < 5; ++f) {
-        cp.x[f] = qpre + (long long)f * c->np;
-        cp.y[f] = tmp + (long long)f * c->np;
+      if (applied < 0) {
+        AxpbyArgs cp{};
+        cp.np = c->np;
+        cp.x_stride = qs;
+        cp.y_stride = ls;
+        cp.nf = 5;
+        for (int f = 0; f < 5; ++f) {
+          cp.x[f] = qpre + (long long)f * c->np;
+          cp.y[f] = tmp + (long long)f * c->np;
+        }
+        cp.a = 1.0;
+        cp.b = 0.0;
+        launch_axpby(cp, c->batch, c->stream);
+      } else {
+        // the replayed waves wrote only points of their boxes: put qpre back there
+        for (int v = 0; v < applied; ++v) {
+          const BcWave& pw = c->waves[(size_t)v];
+          BcCopyArgs r{};
+          r.ops = c->d_ops;
+          r.op_begin = pw.op_begin;
+          r.op_end = pw.op_end;
+          for (int f = 0; f < 5; ++f) {
+            r.src[f] = qpre + (long long)f * c->np;
+            r.dst[f] = tmp + (long long)f * c->np;
+          }
+          r.src_stride = qs;
+          r.dst_stride = ls;
+          r.nx = c->nx;
+          r.ny = c->ny;
+          launch_bc_restore(r, pw.nblocks, c->batch, c->stream);
+        }
       }
-      cp.a = 1.0;
-      cp.b = 0.0;
-      launch_axpby(cp, c->batch, c->stream);
       for (int v = 0; v < wi; ++v) {
         const BcWave& pw = c->waves[(size_t)v];
         BcArgs a{};
@@ -2043,6 +2066,7 @@ void run_bc_adj(gatt_ctx* c, double* qpre, long long qs, double* lam) {
         a.capture = 0;
         launch_bc_wave(a, pw.nblocks, c->batch, c->stream);
       }
+      applied = wi;
       base = tmp;
       bs = ls;
     }

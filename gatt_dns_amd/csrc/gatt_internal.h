@@ -375,6 +375,16 @@ struct BcArgs {
 };
 void launch_bc_wave(const BcArgs& a, long long nblocks, int nmembers, hipStream_t st);
 
+struct BcCopyArgs {
+  const BcOp* ops;
+  int op_begin, op_end;
+  const double* src[5];
+  double* dst[5];
+  long long src_stride, dst_stride;   // member strides
+  int nx, ny;
+};
+void launch_bc_restore(const BcCopyArgs& a, long long nblocks, int nmembers, hipStream_t st);
+
 // Tangent / adjoint of one wave of the boundary program (boundary.hip).
 struct BcLinArgs {
   const BcOp* ops;
