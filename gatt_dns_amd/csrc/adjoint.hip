@@ -52,7 +52,97 @@ __global__ __launch_bounds__(256) void band_t_kernel(BandTArgs a, int m0) {
   out[p] = v;
 }
 
+// y and z: lane = line (64 consecutive lines, every point a 512-byte row) and
+// wavefront w = the M points [w*M, w*M+M) of the lines; each lane holds the
+// window of its segment plus H points either side in registers, so every input
+// row is read once per wavefront.  Same sums, in the same order, as
+// band_t_kernel.
+template <int DIR, int NR, int M>
+__global__ __launch_bounds__(1024) void band_t_seg_kernel(BandTArgs a) {
+  constexpr int H = NR - 1;
+  constexpr int W = M + 2 * H;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int N = a.N;
+  const int nlines = DIR == 1 ? a.nx * a.nz : a.nx * a.ny;
+  const int line0 = blockIdx.x * 64;
+  const int nact = min(64, nlines - line0);
+  const int line = lane < nact ? line0 + lane : line0;
+  const int f = (int)blockIdx.y % a.nf;
+  const int mem = (int)blockIdx.y / a.nf;
+  const long long nxy = (long long)a.nx * a.ny;
+  long long base;
+  if (DIR == 1) {
+    const int k = line / a.nx;
+    base = (long long)(line - k * a.nx) + nxy * k;
+  } else {
+    base = line;
+  }
+  const long long stride = DIR == 1 ? (long long)a.nx : nxy;
+  const int s = w * M;
+  int cnt = N - s;
+  cnt = cnt < 0 ? 0 : (cnt > M ? M : cnt);
+  const double* in = a.in[f] + (long long)mem * a.in_stride;
+  double fw[W];
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    int e = s - H + u;
+    e = e < 0 ? e + N : (e >= N ? e - N : e);
+    e = e >= N ? N - 1 : e;
+    fw[u] = in[base + (long long)e * stride];
+  }
+  const double* Rt = a.R + (size_t)a.types[line] * N * (2 * NR - 1);
+  double* out = a.out[f] + (long long)mem * a.out_stride;
+  const double* src = a.src[f] ? a.src[f] + (long long)mem * a.out_stride : nullptr;
+  if (lane >= nact) return;
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    if (tt < cnt) {
+      const int e = s + tt;
+      double acc = 0.0;
+#pragma unroll
+      for (int l = -H; l <= H; ++l) {
+        int se = e - l;
+        se = se < 0 ? se + N : (se >= N ? se - N : se);
+        acc = acc + Rt[(size_t)(H + l) * N + se] * fw[tt + H - l];
+      }
+      const long long p = base + (long long)e * stride;
+      double v = a.alpha * acc;
+      if (src) v = a.beta * src[p] + v;
+      if (a.accumulate) v = out[p] + v;
+      out[p] = v;
+    }
+  }
+}
+
+template <int DIR, int M>
+static bool launch_band_t_seg_m(const BandTArgs& a, int nmembers, hipStream_t st) {
+  const int nlines = DIR == 1 ? a.nx * a.nz : a.nx * a.ny;
+  const int G = (a.N + M - 1) / M;
+  dim3 grid((unsigned)((nlines + 63) / 64), (unsigned)(a.nf * nmembers));
+  switch (a.nR) {
+#define GATT_BAND_T_NR(v) \
+  case v: hipLaunchKernelGGL((band_t_seg_kernel<DIR, v, M>), grid, dim3(64 * G), 0, st, a); return true;
+    GATT_BAND_T_NR(2) GATT_BAND_T_NR(3) GATT_BAND_T_NR(4) GATT_BAND_T_NR(5)
+#undef GATT_BAND_T_NR
+  }
+  return false;
+}
+
+// Segment kernel along y and z for lines of up to 256 points (16 wavefronts of
+// 16 points) and bands of up to 9 diagonals, where the windows stay in
+// registers; otherwise the point kernel.
+template <int DIR>
+static bool launch_band_t_seg(const BandTArgs& a, int nmembers, hipStream_t st) {
+  if ((long long)a.nf * nmembers > 65535) return false;
+  const int H = a.nR - 1;
+  if (a.N <= 256 && 16 + H <= a.N) return launch_band_t_seg_m<DIR, 16>(a, nmembers, st);
+  return false;
+}
+
 void launch_band_t(const BandTArgs& a, int nmembers, hipStream_t st) {
+  if (a.dir == 1 && launch_band_t_seg<1>(a, nmembers, st)) return;
+  if (a.dir == 2 && launch_band_t_seg<2>(a, nmembers, st)) return;
   // members in chunks that keep the z extent of the grid within its limit
   const int per_chunk = std::max(1, 65535 / (a.nz * a.nf));
   for (int m0 = 0; m0 < nmembers; m0 += per_chunk) {

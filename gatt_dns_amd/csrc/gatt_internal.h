@@ -225,6 +225,7 @@ struct BandTArgs {
   int dir, nx, ny, nz, N, nR, cs;
   long long np;
   const double* coef;     // packed records [type][point][cs], band first
+  const double* R;        // band rows as the reference stores them: N x (2nR-1) x ntypes
   const int* types;       // per line
   int nf;
   long long in_stride, out_stride;
