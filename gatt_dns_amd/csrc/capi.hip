@@ -141,6 +141,16 @@ struct gatt_ctx {
   hipGraphExec_t graph = nullptr;
   double graph_dt = 0;
   int graph_cur = -1;
+  // captured adjoint step (its base state buffer, time and step size)
+  hipGraphExec_t adj_graph = nullptr;
+  double adj_graph_t = 0, adj_graph_dt = 0;
+  int adj_graph_cur = -1;
+  void drop_graphs() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (adj_graph) (void)hipGraphExecDestroy(adj_graph);
+    graph = nullptr;
+    adj_graph = nullptr;
+  }
 
   double* slab_ptr(int idx) { return mem + (long long)idx * np; }
   double* work_ptr(int idx) { return work + (long long)idx * np; }
@@ -909,7 +919,7 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
 int gatt_destroy(gatt_ctx* c) {
   if (!c) return GATT_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
-  if (c->graph) hipGraphExecDestroy(c->graph);
+  c->drop_graphs();
   for (void* p : c->dev_allocs) hipFree(p);
   if (c->d_idx) hipFree(c->d_idx);
   if (c->d_rmap) hipFree(c->d_rmap);
@@ -1132,10 +1142,7 @@ int gatt_set_line_shape(gatt_ctx* c, int32_t dir, int32_t kind, int32_t variant,
   if (!line_shape_ok(s, N, d.op[kind].nR, dir, c->max_lds, d.uniform_tiles))
     return fail(GATT_E_UNSUPPORTED, "line shape does not fit this line");
   d.shape[kind] = s;
-  if (c->graph) {
-    hipGraphExecDestroy(c->graph);
-    c->graph = nullptr;
-  }
+  c->drop_graphs();
   return GATT_OK;
 }
 
@@ -2329,7 +2336,28 @@ int gatt_adj_step(gatt_ctx* c, double t, double dt) {
   ensure_mean(c);
   if ((rc = ensure_lin(c))) return rc;
   if ((rc = ensure_tape(c))) return rc;
-  enqueue_adj_step(c, t, dt);
+  // the step's few hundred launches (most of them boundary waves) are captured once
+  // per base buffer, time and step size and replayed
+  if (c->adj_graph && (c->adj_graph_t != t || c->adj_graph_dt != dt || c->adj_graph_cur != c->cur)) {
+    (void)hipGraphExecDestroy(c->adj_graph);
+    c->adj_graph = nullptr;
+  }
+  if (!c->adj_graph) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_adj_step(c, t, dt);
+    HIPCHK(hipStreamEndCapture(c->stream, &g));
+    const hipError_t e = hipGraphInstantiate(&c->adj_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      c->adj_graph = nullptr;
+      return fail(GATT_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    c->adj_graph_t = t;
+    c->adj_graph_dt = dt;
+    c->adj_graph_cur = c->cur;
+  }
+  HIPCHK(hipGraphLaunch(c->adj_graph, c->stream));
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return GATT_OK;
