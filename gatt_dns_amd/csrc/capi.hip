@@ -58,6 +58,7 @@ struct DirOps {
   double* coef_ends[2] = {nullptr, nullptr};  // per type A(1), C(1)
   double* fcoef[2] = {nullptr, nullptr};      // fused per-point coefficients (line_solve_wsp.h)
   double* fcoef_ends[2] = {nullptr, nullptr}; // per type A(1) B(2nd row), C(1) B(last row)
+  double* fcoef_t[2] = {nullptr, nullptr};    // fused records coefficient-major (x lane-group solves)
   LineShape shape[2];
   int* types = nullptr;      // per line, 0-based
   bool uniform_tiles = false;  // every tile of 64 consecutive lines has one type
@@ -265,7 +266,7 @@ int pack_coeffs(gatt_ctx* c, const gatt_lineop& h, double** coef, double** ends)
 // and its forward factor is zero, as is that of the first solved row; the last
 // row's backward factor is zero.  The end factors are A(1) and C(1) times the B
 // of the rows they correct (rows 2 and N).
-int pack_coeffs_fused(gatt_ctx* c, const gatt_lineop& h, double** coef, double** ends) {
+int pack_coeffs_fused(gatt_ctx* c, const gatt_lineop& h, double** coef, double** ends, double** coef_t) {
   const int N = h.n, nt = h.ntypes, NW = 2 * h.nR - 1, FS = (NW + 3 + 1) & ~1, per = h.periodic ? 1 : 0;
   const int NB = N - per;
   std::vector<double> pk((size_t)nt * N * FS, 0.0), en((size_t)nt * 2, 0.0);
@@ -287,8 +288,15 @@ int pack_coeffs_fused(gatt_ctx* c, const gatt_lineop& h, double** coef, double**
       en[2 * t + 1] = h.C[(size_t)(N - 1) * t] * Bof(N - 1);
     }
   }
+  // the same records coefficient-major, [type][k][point] for k < 2nR-1+3
+  const int NK = NW + 3;
+  std::vector<double> tr((size_t)nt * NK * N, 0.0);
+  for (int t = 0; t < nt; ++t)
+    for (int k = 0; k < NK; ++k)
+      for (int e = 0; e < N; ++e) tr[((size_t)t * NK + k) * N + e] = pk[((size_t)t * N + e) * FS + k];
   int rc;
   if ((rc = dev_upload(c, pk.data(), pk.size(), coef))) return rc;
+  if ((rc = dev_upload(c, tr.data(), tr.size(), coef_t))) return rc;
   return dev_upload(c, en.data(), en.size(), ends);
 }
 
@@ -602,6 +610,7 @@ void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double
   a.coef_ends = d.coef_ends[kind];
   a.fcoef = d.fcoef[kind];
   a.fcoef_ends = d.fcoef_ends[kind];
+  a.fcoef_t = d.fcoef_t[kind];
   a.theta = theta;
   a.mode = mode;
   launch_line_solve(a, d.shape[kind], nmembers, st ? st : c->stream);
@@ -842,8 +851,8 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
     if ((rc = upload_lineop(c.get(), d->filter[k], dims[k], &o.op[1], "filter"))) return rc;
     if ((rc = pack_coeffs(c.get(), d->deriv[k], &o.coef[0], &o.coef_ends[0]))) return rc;
     if ((rc = pack_coeffs(c.get(), d->filter[k], &o.coef[1], &o.coef_ends[1]))) return rc;
-    if ((rc = pack_coeffs_fused(c.get(), d->deriv[k], &o.fcoef[0], &o.fcoef_ends[0]))) return rc;
-    if ((rc = pack_coeffs_fused(c.get(), d->filter[k], &o.fcoef[1], &o.fcoef_ends[1]))) return rc;
+    if ((rc = pack_coeffs_fused(c.get(), d->deriv[k], &o.fcoef[0], &o.fcoef_ends[0], &o.fcoef_t[0]))) return rc;
+    if ((rc = pack_coeffs_fused(c.get(), d->filter[k], &o.fcoef[1], &o.fcoef_ends[1], &o.fcoef_t[1]))) return rc;
     if (k == 0) { o.n1 = d->ny; o.nlines = d->ny * d->nz; o.stride = 1; }
     if (k == 1) { o.n1 = d->nx; o.nlines = d->nx * d->nz; o.stride = d->nx; }
     if (k == 2) { o.n1 = d->nx; o.nlines = d->nx * d->ny; o.stride = (long long)d->nx * d->ny; }
@@ -1132,6 +1141,10 @@ int gatt_set_line_shape(gatt_ctx* c, int32_t dir, int32_t kind, int32_t variant,
     s.G = param;
     s.M = std::max(4, pow2_at_least((N + param - 1) / param));
     s.waves_per_block = wpb;
+  } else if (variant == 4) {
+    s.M = param;
+    s.G = 64;
+    s.waves_per_block = 4;
   } else if (variant >= 1 && variant <= 3) {
     s.M = param;
     s.G = param > 0 ? (N + param - 1) / param : 0;

@@ -30,7 +30,8 @@ struct LineOpDev {
 // w owns points [w*M, w*M+M) of 64 lines, G = ceil(N/M) wavefronts per workgroup.
 // variant 2 ("fused wave-segment", line_solve_wsp.h): the same layout with fused
 // coefficient records and branch-free sweeps; variant 3: variant 2 with the x
-// lines staged through an LDS tile.
+// lines staged through an LDS tile; variant 4 ("x lane group", line_solve_xg.h):
+// one wavefront per x line, M consecutive points per lane (G = 64).
 struct LineShape {
   int variant = 0;
   int G = 64;
@@ -64,6 +65,7 @@ struct SolveArgs {
   const double* coef_ends;    // per type: A(1), C(1) of the periodic corrections
   const double* fcoef;        // fused per-point coefficients [type][point][2nR-1+3] (persistent wave-segment)
   const double* fcoef_ends;   // per type: A(1) B and C(1) B of the periodic corrections
+  const double* fcoef_t;      // the fused records coefficient-major: [type][coefficient][point] (x, variant 4)
   double theta;               // filter blend (mode 1)
   int mode;                   // 0 write result; 1 in-place filter with FilterCharTime blend
 };
@@ -85,7 +87,12 @@ extern template void launch_line_solve_ws_dir<2>(const SolveArgs&, const LineSha
 extern template void launch_line_solve_wsp_dir<0>(const SolveArgs&, const LineShape&, int, hipStream_t);
 extern template void launch_line_solve_wsp_dir<1>(const SolveArgs&, const LineShape&, int, hipStream_t);
 extern template void launch_line_solve_wsp_dir<2>(const SolveArgs&, const LineShape&, int, hipStream_t);
+void launch_line_solve_xg(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st);
 inline void launch_line_solve(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st) {
+  if (s.variant == 4) {
+    launch_line_solve_xg(a, s, nmembers, st);
+    return;
+  }
   if (s.variant == 2 || s.variant == 3) {
     SolveArgs b = a;
     b.xtile = s.variant == 3;
@@ -110,6 +117,12 @@ inline void launch_line_solve(const SolveArgs& a, const LineShape& s, int nmembe
 inline bool line_shape_ok(const LineShape& s, int N, int nR, int dir = 1, long long max_lds = 65536,
                           bool uniform_tiles = false) {
   if (!(s.M == 4 || s.M == 8 || s.M == 16 || s.M == 32) || nR < 2 || nR > 11) return false;
+  if (s.variant == 4) {
+    // one wavefront per x line, M points per lane
+    // (16 points per lane only for bands up to 13 diagonals, which keep the window in registers)
+    return dir == 0 && (s.M == 4 || s.M == 8 || (s.M == 16 && nR <= 7)) && 64LL * s.M >= N && nR - 1 <= N &&
+           nR <= 11;
+  }
   if (s.variant == 2 || s.variant == 3) {
     // fused wave-segment: as variant 1, every segment read straight from HBM
     // (variant 2) or, along x only, through an LDS tile (variant 3); LDS holds

@@ -1,0 +1,180 @@
+// Lane-group line solves along x: one wavefront per line.
+//
+// Restates solveSystemX and solveSystemXper (source/Fortran/derivs.F90:556-615,
+// 737-806) with the fused coefficient records of line_solve_wsp.h.  An x line is
+// contiguous, so lane l of the wavefront owns the M consecutive points
+// [l*M, l*M+M) and every load and store of the wavefront covers one contiguous
+// stretch of the line:
+//
+//   * window: the lane's points and H halo points either side, read straight
+//     from HBM (the halo comes from the neighbouring lanes' cache lines);
+//   * coefficients: a per-type table stored coefficient-major (table k holds
+//     coefficient k of every point), so the lanes read consecutive words;
+//   * Thomas sweeps: each lane composes the affine map of its points, the 64
+//     maps are combined by a log2(64)-step wavefront scan (shuffles), and each
+//     lane re-runs its points from the exact start value;
+//   * periodic lines: x1 = D . r is a wavefront reduction.
+//
+// Any operator type per line (one line per wavefront), any line length up to
+// 64*M points.  Results differ from the reference's sequential recurrence by
+// rounding only.
+#pragma once
+#include "line_solve_wsp.h"
+
+namespace gatt {
+
+__device__ __forceinline__ double xg_shfl_up(double v, int d) { return __shfl_up(v, d, 64); }
+__device__ __forceinline__ double xg_shfl_down(double v, int d) { return __shfl_down(v, d, 64); }
+__device__ __forceinline__ double xg_shfl_xor(double v, int m) { return __shfl_xor(v, m, 64); }
+
+template <int NR, int M, bool BLEND>
+__global__ __launch_bounds__(256) void line_solve_xg_kernel(SolveArgs a) {
+  constexpr int H = NR - 1;
+  constexpr int NW = 2 * H + 1;
+  constexpr int W = M + 2 * H;
+  const int lane = threadIdx.x & 63;
+  const int line = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (line >= a.nlines) return;   // whole wavefronts
+  const int f = (int)blockIdx.y % a.nf;
+  const int mem = (int)blockIdx.y / a.nf;
+  const int N = a.op.N;
+  const int per = a.op.periodic;
+  const double* src = a.in[f] + (long long)mem * a.in_stride + (long long)a.nx * line;
+  double* dst = a.out[f] + (long long)mem * a.out_stride + (long long)a.nx * line;
+  const int s = lane * M;
+  const int tu = __builtin_amdgcn_readfirstlane(a.types[line]);
+  // coefficient k of point e: ct[k * N + e]; points past the line end read as zero
+  const double* ct = a.fcoef_t + (size_t)tu * (NW + 3) * N;
+  auto coef = [&](int k, int e) { return e < N ? ct[(size_t)k * N + e] : 0.0; };
+
+  double fw[W];
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    int e = s - H + u;
+    e = e < 0 ? e + N : (e >= N ? e - N : e);
+    e = e >= N ? e - N : e;
+    e = e < 0 ? 0 : (e >= N ? N - 1 : e);
+    fw[u] = src[e];
+  }
+  // band product, scaled by the forward factor
+  double r[M];
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    const int e = s + tt;
+    double acc = coef(H, e) * fw[tt + H];
+#pragma unroll
+    for (int q = 1; q <= H; ++q) acc = fma(coef(H - q, e), fw[tt + H - q], fma(coef(H + q, e), fw[tt + H + q], acc));
+    r[tt] = acc;
+  }
+  double x1 = 0.0;
+  if (per) {
+    double part = 0.0;
+#pragma unroll
+    for (int tt = 0; tt < M; ++tt) part = fma(r[tt], coef(NW + 2, s + tt), part);
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) part += xg_shfl_xor(part, m);
+    x1 = part;
+    // row 2 and row N corrections (derivs.F90:792-793)
+    const double A1 = a.fcoef_ends[(size_t)tu * 2], C1 = a.fcoef_ends[(size_t)tu * 2 + 1];
+#pragma unroll
+    for (int tt = 0; tt < M; ++tt) {
+      if (s + tt == 1) r[tt] = fma(-A1, x1, r[tt]);
+      if (s + tt == N - 1) r[tt] = fma(-C1, x1, r[tt]);
+    }
+  }
+  // forward sweep: d_e = r_e - fa'_e d_{e-1}
+  double P = 0.0, Q = 1.0;
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    const double fa = coef(NW, s + tt);
+    P = fma(-fa, P, r[tt]);
+    Q = -fa * Q;
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double Pp = xg_shfl_up(P, d), Qp = xg_shfl_up(Q, d);
+    if (lane >= d) {
+      P = fma(Q, Pp, P);
+      Q = Q * Qp;
+    }
+  }
+  double dprev = xg_shfl_up(P, 1);
+  if (lane == 0) dprev = 0.0;
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    const double d = fma(-coef(NW, s + tt), dprev, r[tt]);
+    r[tt] = d;
+    dprev = d;
+  }
+  // backward sweep: x_e = d_e - ce_e x_{e+1}
+  P = 0.0;
+  Q = 1.0;
+#pragma unroll
+  for (int tt = M - 1; tt >= 0; --tt) {
+    const double ce = coef(NW + 1, s + tt);
+    P = fma(-ce, P, r[tt]);
+    Q = -ce * Q;
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double Pn = xg_shfl_down(P, d), Qn = xg_shfl_down(Q, d);
+    if (lane + d < 64) {
+      P = fma(Q, Pn, P);
+      Q = Q * Qn;
+    }
+  }
+  double xnext = xg_shfl_down(P, 1);
+  if (lane == 63) xnext = 0.0;
+#pragma unroll
+  for (int tt = M - 1; tt >= 0; --tt) {
+    const double x = fma(-coef(NW + 1, s + tt), xnext, r[tt]);
+    r[tt] = x;
+    xnext = x;
+  }
+  if (per && lane == 0) r[0] = x1;
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    const int e = s + tt;
+    if (e < N) {
+      double v = r[tt];
+      if (BLEND) v = fma(a.theta, v - fw[tt + H], fw[tt + H]);
+      dst[e] = v;
+    }
+  }
+}
+
+template <int NR, int M>
+static void launch_xg_m(const SolveArgs& a, int nmembers, hipStream_t st) {
+  dim3 grid((unsigned)((a.nlines + 3) / 4), (unsigned)(a.nf * nmembers));
+  if (a.mode == 1 && a.theta > 0.0)
+    hipLaunchKernelGGL((line_solve_xg_kernel<NR, M, true>), grid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((line_solve_xg_kernel<NR, M, false>), grid, dim3(256), 0, st, a);
+}
+
+template <int NR>
+static void launch_xg_nr(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st) {
+  switch (s.M) {
+    case 4: launch_xg_m<NR, 4>(a, nmembers, st); break;
+    case 8: launch_xg_m<NR, 8>(a, nmembers, st); break;
+    case 16: launch_xg_m<NR, 16>(a, nmembers, st); break;
+  }
+}
+
+inline void launch_line_solve_xg_impl(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st) {
+  switch (a.op.nR) {
+    case 2: launch_xg_nr<2>(a, s, nmembers, st); break;
+    case 3: launch_xg_nr<3>(a, s, nmembers, st); break;
+    case 4: launch_xg_nr<4>(a, s, nmembers, st); break;
+    case 5: launch_xg_nr<5>(a, s, nmembers, st); break;
+    case 6: launch_xg_nr<6>(a, s, nmembers, st); break;
+    case 7: launch_xg_nr<7>(a, s, nmembers, st); break;
+    case 8: launch_xg_nr<8>(a, s, nmembers, st); break;
+    case 9: launch_xg_nr<9>(a, s, nmembers, st); break;
+    case 10: launch_xg_nr<10>(a, s, nmembers, st); break;
+    case 11: launch_xg_nr<11>(a, s, nmembers, st); break;
+    default: break;
+  }
+}
+
+}  // namespace gatt

@@ -220,7 +220,10 @@ int gatt_sync(gatt_ctx* ctx);
  *                             and no branches in the sweeps (results agree
  *                             with variants 0 and 1 to rounding);
  *   variant 3 (x only): variant 2 with the 64 lines of a workgroup staged
- *                             through an LDS tile.
+ *                             through an LDS tile;
+ *   variant 4 (x only), one wavefront per line: param = M points per lane
+ *                             (4, 8 or 16; 64*M >= n); any operator type
+ *                             per line.
  * Shapes that do not fit the line or the device's LDS are rejected. */
 int gatt_set_line_shape(gatt_ctx* ctx, int32_t dir, int32_t kind, int32_t variant, int32_t param,
                         int32_t waves_per_block);

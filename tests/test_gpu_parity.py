@@ -83,7 +83,7 @@ def test_line_solve(rt, oracle_mod, name, kind):
 
 @pytest.mark.parametrize("name", ["cavity33", "cavity129", "box32", "box48_noforce"])
 def test_line_solve_all_shapes(rt, oracle_mod, name):
-    """Every accepted launch shape of both kernel variants gives the oracle's result."""
+    """Every accepted launch shape of every kernel variant gives the oracle's result."""
     import torch
     case = CASES[name]()
     o = oracle_mod.Oracle(case)
@@ -97,7 +97,8 @@ def test_line_solve_all_shapes(rt, oracle_mod, name):
         tin = torch.from_numpy(F.ravel(order="F").copy()).cuda()
         for kind in (0, 1):
             want = o.solve(d, kind, F)
-            shapes = [(1, M, 1) for M in (16, 32)]
+            shapes = [(v, M, 1) for v in (1, 2, 3) for M in (16, 32)]
+            shapes += [(4, M, 1) for M in (4, 8, 16)]
             shapes += [(0, G, w) for G in (2, 4, 8, 16, 32, 64) for w in (1, 2, 4)]
             for v, p, w in shapes:
                 try:
