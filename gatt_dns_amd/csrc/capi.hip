@@ -177,8 +177,10 @@ int dev_upload(gatt_ctx* c, const T* host, size_t count, T** out) {
 // kernel with 16 points per lane where the line allows it.
 LineShape choose_shape(int dir, int N, int nR, long long nlines, long long max_lds, bool uniform_tiles) {
   if (nlines >= 1024 && uniform_tiles) {
+    // 32 points per lane along x and for wide bands (fewer, fuller wavefronts), else 16
+    const bool wide = dir == 0 || nR >= 5;
     for (int v : {2, 1}) {
-      for (int M : {16, 32}) {
+      for (int M : {wide ? 32 : 16, wide ? 16 : 32}) {
         LineShape s;
         s.variant = v;
         s.M = M;

@@ -29,8 +29,8 @@ struct LineOpDev {
 // variant 1 ("wave-segment", line_solve_ws.h): every lane owns a line, wavefront
 // w owns points [w*M, w*M+M) of 64 lines, G = ceil(N/M) wavefronts per workgroup.
 // variant 2 ("fused wave-segment", line_solve_wsp.h): the same layout with fused
-// coefficient records and branch-free sweeps; variant 3: variant 2 with the x
-// lines staged through an LDS tile; variant 4 ("x lane group", line_solve_xg.h):
+// coefficient records and branch-free sweeps; variant 3: the same sweeps with
+// 32 x lines per workgroup staged through an LDS tile; variant 4 ("x lane group", line_solve_xg.h):
 // one wavefront per x line, M consecutive points per lane (G = 64).
 struct LineShape {
   int variant = 0;
@@ -129,10 +129,16 @@ inline bool line_shape_ok(const LineShape& s, int N, int nR, int dir = 1, long l
     // the scan maps and coefficient slots (in the tile's words for variant 3)
     if (!(s.M == 16 || s.M == 32) || !uniform_tiles || (s.variant == 3 && dir != 0)) return false;
     const int G = (N + s.M - 1) / s.M;
-    if (G < 1 || G > (s.M >= 32 ? 8 : 16) || s.M + nR - 1 > N + 1 || nR - 1 > N) return false;
+    const int gmax = s.variant == 3 ? 16 : (s.M >= 32 ? 8 : 16);   // variant 3: two segments per wavefront
+    if (G < 1 || G > gmax || s.M + nR - 1 > N + 1 || nR - 1 > N) return false;
     const long long cs = (2LL * nR - 1 + 3 + 1) & ~1LL;
-    const long long work = 5LL * G * 64 + G * (s.M * cs + 2), tile = s.variant == 3 ? 65LL * N : 0;
-    return 8 * (work > tile ? work : tile) <= max_lds;
+    if (s.variant == 3) {   // 32 lines per workgroup through a tile of 33*N words
+      const long long GA = (G + 1) & ~1LL;
+      const long long work3 = 5LL * GA * 32 + GA * (s.M * cs + 2), tile3 = 33LL * N;
+      return 8 * (work3 > tile3 ? work3 : tile3) <= max_lds;
+    }
+    const long long work = 5LL * G * 64 + G * (s.M * cs + 2);
+    return 8 * work <= max_lds;
   }
   if (s.variant == 1) {
     // wave-segment: compiled for 16 and 32 points per lane, and only for

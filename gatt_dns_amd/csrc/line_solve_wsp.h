@@ -37,33 +37,39 @@ constexpr int kMaxSegments = 16;   // wavefronts (segments) per line
 // from LDS again in each phase instead of being held in registers across it.
 __device__ __forceinline__ void wsp_phase() { asm volatile("" ::: "memory"); }
 
-// Sweeps of one item: r holds the scaled band product on entry and the
+// Sweeps of one segment: r holds the scaled band product on entry and the
 // solution on exit; returns the periodic first-row value.  Every point of the
 // segment runs through the same arithmetic: the packed forward factor is zero
 // at the first solved row (and at the periodic row 0 before it) and the
 // backward factor is zero at the last row, so no point needs a branch.
-template <int NR, int M>
+// The caller is segment sg (of G) of line l (of the L lines of the workgroup);
+// the maps of segment k of line l sit at [k*L + l].  KMAX bounds G.
+template <int NR, int M, int KMAX = kMaxSegments>
 __device__ __forceinline__ double wsp_sweeps(double (&r)[M], const double* cf, const double* cends, double* shD,
-                                             double* shP, double* shB, int cnt, int w, int G, int lane, int per) {
-  double* shQ = shP + G * 64;
-  double* shC = shB + G * 64;
+                                             double* shP, double* shB, int cnt, int sg, int G, int l, int per,
+                                             int L = 64, int GA = 0) {
+  // GA: segments the map arrays hold (at least G)
+  if (GA < G) GA = G;
+  double* shQ = shP + GA * L;
+  double* shC = shB + GA * L;
   constexpr int H = NR - 1;
   constexpr int NW = 2 * H + 1;
   constexpr int CS = wsp_coef_stride(NR);
+  const int me = sg * L + l;
   double x1 = 0.0;
   if (per) {
     double part = 0.0;
 #pragma unroll
     for (int tt = 0; tt < M; ++tt) part = fma(r[tt], cf[tt * CS + NW + 2], part);
-    shD[w * 64 + lane] = part;
+    shD[me] = part;
     __syncthreads();
     wsp_phase();
 #pragma unroll
-    for (int k = 0; k < kMaxSegments; ++k)
-      if (k < G) x1 += shD[k * 64 + lane];
+    for (int k = 0; k < KMAX; ++k)
+      if (k < G) x1 += shD[k * L + l];
     // row 2 and row N corrections (derivs.F90:792-793)
-    if (w == 0) r[1] = fma(-cends[0], x1, r[1]);
-    if (w == G - 1) {
+    if (sg == 0) r[1] = fma(-cends[0], x1, r[1]);
+    if (sg == G - 1) {
 #pragma unroll
       for (int tt = 0; tt < M; ++tt)
         if (tt == cnt - 1) r[tt] = fma(-cends[1], x1, r[tt]);
@@ -78,14 +84,14 @@ __device__ __forceinline__ double wsp_sweeps(double (&r)[M], const double* cf, c
     P = fma(-fa, P, r[tt]);
     Q = -fa * Q;
   }
-  shP[w * 64 + lane] = P;
-  shQ[w * 64 + lane] = Q;
+  shP[me] = P;
+  shQ[me] = Q;
   __syncthreads();
   wsp_phase();
   double dprev = 0.0;
 #pragma unroll
-  for (int k = 0; k < kMaxSegments; ++k)
-    if (k < w) dprev = fma(shQ[k * 64 + lane], dprev, shP[k * 64 + lane]);
+  for (int k = 0; k < KMAX; ++k)
+    if (k < sg) dprev = fma(shQ[k * L + l], dprev, shP[k * L + l]);
 #pragma unroll
   for (int tt = 0; tt < M; ++tt) {
     const double d = fma(-cf[tt * CS + NW], dprev, r[tt]);
@@ -103,14 +109,14 @@ __device__ __forceinline__ double wsp_sweeps(double (&r)[M], const double* cf, c
     P = fma(-ce, P, r[tt]);
     Q = -ce * Q;
   }
-  shB[w * 64 + lane] = P;
-  shC[w * 64 + lane] = Q;
+  shB[me] = P;
+  shC[me] = Q;
   __syncthreads();
   wsp_phase();
   double xnext = 0.0;
 #pragma unroll
-  for (int k = kMaxSegments - 1; k >= 0; --k)
-    if (k > w && k < G) xnext = fma(shC[k * 64 + lane], xnext, shB[k * 64 + lane]);
+  for (int k = KMAX - 1; k >= 0; --k)
+    if (k > sg && k < G) xnext = fma(shC[k * L + l], xnext, shB[k * L + l]);
 #pragma unroll
   for (int tt = M - 1; tt >= 0; --tt) {
     const double x = fma(-cf[tt * CS + NW + 1], xnext, r[tt]);
@@ -181,12 +187,7 @@ __device__ __forceinline__ void wsp_load_x(const double* src, int N, int st, boo
 template <int NR, int M, bool BLEND>
 constexpr int wsp_min_waves() { return M >= 32 ? 4 : ((NR <= 4 && !BLEND) ? 8 : 1); }
 
-// XT (x lines only): the 64 lines of the workgroup are one contiguous stretch,
-// moved between HBM and an LDS tile (N rows of 65 words) with unit-stride
-// accesses; the scan maps and coefficient slots reuse the tile's words between
-// the window reads and the result writes.  Without XT each lane reads and
-// writes its own line directly.
-template <int DIR, int NR, int M, bool BLEND, bool XT>
+template <int DIR, int NR, int M, bool BLEND>
 __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>())) void line_solve_wsp_kernel(SolveArgs a) {
   extern __shared__ double sh[];
   constexpr int H = NR - 1;
@@ -207,7 +208,6 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>(
   double* shP = sh + G * 64;                     // forward maps  [2][G][64]
   double* shB = sh + 3 * G * 64;                 // backward maps [2][G][64]
   double* cl = sh + 5 * G * 64 + w * SLOT;       // this wavefront's coefficients
-  double* tile = sh;                             // XT: N rows of 65 words
   const long long stride = a.stride;
   const int line0 = blockIdx.x * 64;
   const int nact = min(64, a.nlines - line0);
@@ -236,24 +236,7 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>(
   }
 
   double fw[W];
-  if (XL && XT) {
-    const double* src = in + (long long)a.nx * line0;
-    const int total = nact * N;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
-      const int ep = idx - lp * N;
-      tile[ep * 65 + lp] = src[idx];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < W; ++u) {
-      int e = s - H + u;
-      e = e < 0 ? e + N : (e >= N ? e - N : e);
-      e = e >= N ? N - 1 : e;
-      fw[u] = tile[e * 65 + lane];
-    }
-    __syncthreads();   // the tile's words now hold the maps and coefficient slots
-  } else if (XL) {
+  if (XL) {
     wsp_load_x<W>(in + (long long)a.nx * line, N, s - H, a.xvec != 0, fw);
   } else {
     const long long base = wsp_base<DIR>(a, line);
@@ -284,20 +267,7 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>(
     for (int tt = 0; tt < M; ++tt) r[tt] = fma(a.theta, r[tt] - f0[tt], f0[tt]);
   }
   const bool full = cnt == M && nact == 64;
-  if (XL && XT) {
-    __syncthreads();   // every map and slot read before the tile is rewritten
-#pragma unroll
-    for (int tt = 0; tt < M; ++tt)
-      if (tt < cnt) tile[(s + tt) * 65 + lane] = r[tt];
-    __syncthreads();
-    double* dst = out + (long long)a.nx * line0;
-    const int total = nact * N;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
-      const int ep = idx - lp * N;
-      dst[idx] = tile[ep * 65 + lp];
-    }
-  } else if (XL) {
+  if (XL) {
     double* dst = out + (long long)a.nx * line + s;
     if (full && a.xvec) {
       double2* p = reinterpret_cast<double2*>(dst);
@@ -321,22 +291,135 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>(
   }
 }
 
+// Variant 3, x lines through an LDS tile: a workgroup owns 32 lines, one
+// contiguous stretch of 32*N doubles moved between HBM and the tile (N rows of
+// 33 words) with unit-stride accesses.  Lanes 0-31 of wavefront w work on
+// segment 2w of the 32 lines, lanes 32-63 on segment 2w+1; the scan maps and
+// the coefficient slots (one per segment) reuse the tile's words between the
+// window reads and the result writes.  At N = 256 the tile takes 66 KiB, so
+// two workgroups share a compute unit.
+template <int NR, int M, bool BLEND>
+__global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
+  extern __shared__ double sh[];
+  constexpr int H = NR - 1;
+  constexpr int CS = wsp_coef_stride(NR);
+  constexpr int W = M + 2 * H;
+  constexpr int SLOT = M * CS + 2;
+  constexpr int CP = (SLOT + 31) / 32;         // coefficient words per lane (32 lanes per segment)
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int l = lane & 31;
+  const int sg = 2 * w + (lane >> 5);
+  const int N = a.op.N;
+  const int G = (N + M - 1) / M;               // segments per line
+  const int GA = (G + 1) & ~1;                 // segments of the wavefronts (the last may lie past the line)
+  const int per = a.op.periodic;
+  const int s = sg * M;
+  int cnt = N - s;
+  cnt = cnt < 0 ? 0 : (cnt > M ? M : cnt);
+  double* tile = sh;
+  double* shD = sh;                            // periodic dot partials [GA][32]
+  double* shP = sh + GA * 32;                  // forward maps  [2][GA][32]
+  double* shB = sh + 3 * GA * 32;              // backward maps [2][GA][32]
+  double* cl = sh + 5 * GA * 32 + sg * SLOT;   // this segment's coefficients
+  const int line0 = blockIdx.x * 32;
+  const int nact = min(32, a.nlines - line0);
+  const int f = (int)blockIdx.y % a.nf;
+  const int mem = (int)blockIdx.y / a.nf;
+  const double* in = a.in[f] + (long long)mem * a.in_stride + (long long)a.nx * line0;
+  double* out = a.out[f] + (long long)mem * a.out_stride + (long long)a.nx * line0;
+
+  double cpre[CP];
+  {
+    const int tu = __builtin_amdgcn_readfirstlane(a.types[line0]);
+    const double* src = a.fcoef + ((size_t)tu * N + (sg < G ? s : 0)) * CS;
+    const double* ends = a.fcoef_ends + (size_t)tu * 2;
+#pragma unroll
+    for (int k = 0; k < CP; ++k) {
+      const int jj = l + 32 * k;
+      double v = 0.0;
+      if (jj < cnt * CS) v = src[jj];
+      else if (jj >= M * CS && jj < SLOT) v = ends[jj - M * CS];
+      cpre[k] = v;
+    }
+  }
+  const int total = nact * N;
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+    const int ep = idx - lp * N;
+    tile[ep * 33 + lp] = in[idx];
+  }
+  __syncthreads();
+  double fw[W];
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    int e = s - H + u;
+    e = e < 0 ? e + N : (e >= N ? e - N : e);
+    e = e >= N ? e - N : e;
+    e = e < 0 ? 0 : (e >= N ? N - 1 : e);
+    fw[u] = tile[e * 33 + l];
+  }
+  __syncthreads();   // the tile's words now hold the maps and the coefficient slots
+  // a segment's slot is written and read by the same half wavefront, whose LDS accesses run in order
+#pragma unroll
+  for (int k = 0; k < CP; ++k) {
+    const int jj = l + 32 * k;
+    if (jj < SLOT) cl[jj] = cpre[k];
+  }
+  __builtin_amdgcn_wave_barrier();
+  const double* cf = cl;
+  const double* cends = cl + M * CS;
+  double r[M];
+  double f0[BLEND ? M : 1];
+  wsp_band<NR, M, BLEND>(r, f0, fw, cf);
+  const double x1 = wsp_sweeps<NR, M, 2 * kMaxSegments>(r, cf, cends, shD, shP, shB, cnt, sg, G, l, per, 32, GA);
+  if (per && sg == 0) r[0] = x1;
+  if (BLEND) {
+#pragma unroll
+    for (int tt = 0; tt < M; ++tt) r[tt] = fma(a.theta, r[tt] - f0[tt], f0[tt]);
+  }
+  __syncthreads();   // every map and slot read before the tile is rewritten
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt)
+    if (tt < cnt) tile[(s + tt) * 33 + l] = r[tt];
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+    const int ep = idx - lp * N;
+    out[idx] = tile[ep * 33 + lp];
+  }
+}
+
 // LDS words: scan maps (5*G*64), then one coefficient slot per wavefront.
-// XT: the x tile (65*N words) holds them in turn.
-inline size_t wsp_lds_bytes(int G, int M, int nR, bool xt = false, int N = 0) {
-  const size_t work = (size_t)5 * G * 64 + (size_t)G * (M * wsp_coef_stride(nR) + 2);
-  const size_t tile = xt ? (size_t)65 * N : 0;
+inline size_t wsp_lds_bytes(int G, int M, int nR) {
+  return sizeof(double) * ((size_t)5 * G * 64 + (size_t)G * (M * wsp_coef_stride(nR) + 2));
+}
+// Variant 3: the tile (33*N words), which holds the maps and slots in turn.
+inline size_t wspx_lds_bytes(int N, int M, int nR) {
+  const int G = ((N + M - 1) / M + 1) & ~1;
+  const size_t work = (size_t)5 * G * 32 + (size_t)G * (M * wsp_coef_stride(nR) + 2);
+  const size_t tile = (size_t)33 * N;
   return sizeof(double) * (work > tile ? work : tile);
 }
 
-template <int DIR, int NR, int M, bool BLEND, bool XT>
+template <int DIR, int NR, int M, bool BLEND>
 static void launch_wsp_k(const SolveArgs& a, int G, hipStream_t st) {
   static_assert(M % 2 == 0, "pairs of points per lane");
-  auto k = line_solve_wsp_kernel<DIR, NR, M, BLEND, XT>;
-  const size_t lds = wsp_lds_bytes(G, M, NR, XT, a.op.N);
+  auto k = line_solve_wsp_kernel<DIR, NR, M, BLEND>;
+  const size_t lds = wsp_lds_bytes(G, M, NR);
   if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   dim3 grid((unsigned)((a.nlines + 63) / 64), (unsigned)(a.nf * a.nmem));
   hipLaunchKernelGGL(k, grid, dim3(64 * G), lds, st, a);
+}
+
+template <int NR, int M, bool BLEND>
+static void launch_wspx_k(const SolveArgs& a, hipStream_t st) {
+  auto k = line_solve_wspx_kernel<NR, M, BLEND>;
+  const size_t lds = wspx_lds_bytes(a.op.N, M, NR);
+  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const int G = (a.op.N + M - 1) / M;
+  dim3 grid((unsigned)((a.nlines + 31) / 32), (unsigned)(a.nf * a.nmem));
+  hipLaunchKernelGGL(k, grid, dim3(64 * ((G + 1) / 2)), lds, st, a);
 }
 
 template <int DIR, int NR, int M>
@@ -353,13 +436,13 @@ static void launch_wsp_m(const SolveArgs& a0, int nmembers, hipStream_t st) {
   const bool blend = a.mode == 1 && a.theta > 0.0;
   if constexpr (DIR == 0) {
     if (a.xtile) {
-      if (blend) launch_wsp_k<DIR, NR, M, true, true>(a, G, st);
-      else launch_wsp_k<DIR, NR, M, false, true>(a, G, st);
+      if (blend) launch_wspx_k<NR, M, true>(a, st);
+      else launch_wspx_k<NR, M, false>(a, st);
       return;
     }
   }
-  if (blend) launch_wsp_k<DIR, NR, M, true, false>(a, G, st);
-  else launch_wsp_k<DIR, NR, M, false, false>(a, G, st);
+  if (blend) launch_wsp_k<DIR, NR, M, true>(a, G, st);
+  else launch_wsp_k<DIR, NR, M, false>(a, G, st);
 }
 
 template <int DIR, int NR>
