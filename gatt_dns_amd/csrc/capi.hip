@@ -1847,33 +1847,12 @@ void tl_stage(gatt_ctx* c, const double* S, double* dS, double* dQ, double* dOut
 }
 
 // Transposed operator of `kind` along `dir` applied to nf fields: out = beta*src
-// + (accumulate ? out : 0) + alpha * R^T (Thomas^T in); the Thomas part runs
-// in place on `in`, which is overwritten.
+// + (accumulate ? out : 0) + alpha * R^T (Thomas^T in); `in` is workspace (the
+// two-launch path runs the Thomas part in place on it).
 void run_solve_t(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double* const* out,
                  double* const* src, double alpha, double beta, int accumulate, int nm, long long in_stride,
                  long long out_stride) {
   DirOps& d = c->dir[dir];
-  SolveArgs a{};
-  a.op = d.op[kind];
-  a.types = d.types;
-  a.dir = dir;
-  a.nx = c->nx;
-  a.ny = c->ny;
-  a.nz = c->nz;
-  a.n1 = d.n1;
-  a.nlines = d.nlines;
-  a.stride = d.stride;
-  a.cs = (2 * d.op[kind].nR - 1 + 4 + 1) & ~1;
-  a.nf = nf;
-  a.in_stride = in_stride;
-  a.out_stride = in_stride;
-  for (int f = 0; f < nf; ++f) {
-    a.in[f] = in[f];
-    a.out[f] = in[f];
-  }
-  a.coef = d.coef[kind];
-  a.coef_ends = d.coef_ends[kind];
-  launch_line_solve_t(a, nm, c->stream);
   BandTArgs b{};
   b.dir = dir;
   b.nx = c->nx;
@@ -1881,10 +1860,14 @@ void run_solve_t(gatt_ctx* c, int dir, int kind, int nf, double* const* in, doub
   b.nz = c->nz;
   b.N = d.op[kind].N;
   b.nR = d.op[kind].nR;
-  b.cs = a.cs;
+  b.cs = (2 * d.op[kind].nR - 1 + 4 + 1) & ~1;
   b.np = c->np;
   b.coef = d.coef[kind];
   b.R = d.op[kind].R;
+  b.fcoef_t = d.fcoef_t[kind];
+  b.fcoef_ends = d.fcoef_ends[kind];
+  b.periodic = d.op[kind].periodic;
+  b.nlines = d.nlines;
   b.types = d.types;
   b.nf = nf;
   b.in_stride = in_stride;
@@ -1897,6 +1880,29 @@ void run_solve_t(gatt_ctx* c, int dir, int kind, int nf, double* const* in, doub
   b.alpha = alpha;
   b.beta = beta;
   b.accumulate = accumulate;
+  // along x: one launch, one wavefront per line
+  if (dir == 0 && launch_line_solve_xgt(b, nm, c->stream)) return;
+  SolveArgs a{};
+  a.op = d.op[kind];
+  a.types = d.types;
+  a.dir = dir;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  a.n1 = d.n1;
+  a.nlines = d.nlines;
+  a.stride = d.stride;
+  a.cs = b.cs;
+  a.nf = nf;
+  a.in_stride = in_stride;
+  a.out_stride = in_stride;
+  for (int f = 0; f < nf; ++f) {
+    a.in[f] = in[f];
+    a.out[f] = in[f];
+  }
+  a.coef = d.coef[kind];
+  a.coef_ends = d.coef_ends[kind];
+  launch_line_solve_t(a, nm, c->stream);
   launch_band_t(b, nm, c->stream);
 }
 

@@ -245,6 +245,9 @@ struct BandTArgs {
   long long np;
   const double* coef;     // packed records [type][point][cs], band first
   const double* R;        // band rows as the reference stores them: N x (2nR-1) x ntypes
+  const double* fcoef_t;  // fused records coefficient-major (x transposed solve)
+  const double* fcoef_ends;
+  int periodic, nlines;
   const int* types;       // per line
   int nf;
   long long in_stride, out_stride;
@@ -255,6 +258,9 @@ struct BandTArgs {
   int accumulate;
 };
 void launch_band_t(const BandTArgs& a, int nmembers, hipStream_t st);
+// Transposed solve and band along x in one launch (line_solve_xg.h); false if the
+// line does not fit (the caller then runs launch_line_solve_t and launch_band_t).
+bool launch_line_solve_xgt(const BandTArgs& a, int nmembers, hipStream_t st);
 
 // Tangent of visc: dT from base E, D and tangents dE, dD.  Adjoint: accumulates
 // Ebar and Dbar from Tbar (Dbar, Ebar accumulate; Tbar is read).

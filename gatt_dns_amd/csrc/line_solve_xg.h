@@ -178,3 +178,175 @@ inline void launch_line_solve_xg_impl(const SolveArgs& a, const LineShape& s, in
 }
 
 }  // namespace gatt
+
+namespace gatt {
+
+// Transposed x solve with its transposed band, one wavefront per line: the
+// adjoint of the forward x solve above (and of solveSystemX[per],
+// source/Fortran/derivs.F90:556-615, 737-806).  With the fused records the
+// forward operator is r' = B'F (rows scaled by the forward factor),
+// x1 = D'.r', r'_1 -= A1' x1, r'_{N-1} -= C1' x1, L'd = r', Ux = d, out_0 = x1.
+// Its transpose applied to xbar:
+//   U^T z = xbar:   z_e   = xbar_e - ce_{e-1} z_{e-1}      (first solved row: no predecessor)
+//   L'^T rho = z:   rho_e = z_e - fa'_{e+1} rho_{e+1}
+//   periodic:       t = xbar_0 - A1' rho_1 - C1' rho_{N-1};  r'bar_0 = D'_0 t, r'bar_e = rho_e + D'_e t
+//   B'^T:           Fbar_j = sum_l B'((j-l) mod N, l) r'bar_{j-l}
+// and out = beta*src + (accumulate ? out : 0) + alpha*Fbar.  The line's r'bar
+// passes through an LDS buffer of N words per wavefront for the band.
+template <int NR, int M>
+__global__ __launch_bounds__(256) void line_solve_xgt_kernel(BandTArgs a) {
+  extern __shared__ double sh[];
+  constexpr int H = NR - 1;
+  constexpr int NW = 2 * H + 1;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int line = (int)blockIdx.x * 4 + wv;
+  if (line >= a.nlines) return;   // whole wavefronts; only wavefront-level synchronisation below
+  const int f = (int)blockIdx.y % a.nf;
+  const int mem = (int)blockIdx.y / a.nf;
+  const int N = a.N;
+  const int per = a.periodic;
+  const int e0 = per;
+  const double* in = a.in[f] + (long long)mem * a.in_stride + (long long)a.nx * line;
+  double* out = a.out[f] + (long long)mem * a.out_stride + (long long)a.nx * line;
+  const double* src = a.src[f] ? a.src[f] + (long long)mem * a.out_stride + (long long)a.nx * line : nullptr;
+  const int tu = __builtin_amdgcn_readfirstlane(a.types[line]);
+  const double* ct = a.fcoef_t + (size_t)tu * (NW + 3) * N;
+  auto coef = [&](int k, int e) { return (e >= 0 && e < N) ? ct[(size_t)k * N + e] : 0.0; };
+  const int s = lane * M;
+  double r[M];
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) r[tt] = s + tt < N ? in[s + tt] : 0.0;
+
+  // U^T z = xbar
+  double P = 0.0, Q = 1.0;
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    const int e = s + tt;
+    const double cm = e > e0 ? coef(NW + 1, e - 1) : 0.0;
+    P = fma(-cm, P, r[tt]);
+    Q = -cm * Q;
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double Pp = xg_shfl_up(P, d), Qp = xg_shfl_up(Q, d);
+    if (lane >= d) {
+      P = fma(Q, Pp, P);
+      Q = Q * Qp;
+    }
+  }
+  double zprev = xg_shfl_up(P, 1);
+  if (lane == 0) zprev = 0.0;
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    const int e = s + tt;
+    const double cm = e > e0 ? coef(NW + 1, e - 1) : 0.0;
+    const double z = fma(-cm, zprev, r[tt]);
+    r[tt] = z;
+    zprev = z;
+  }
+  // L'^T rho = z
+  P = 0.0;
+  Q = 1.0;
+#pragma unroll
+  for (int tt = M - 1; tt >= 0; --tt) {
+    const double fp = coef(NW, s + tt + 1);
+    P = fma(-fp, P, r[tt]);
+    Q = -fp * Q;
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double Pn = xg_shfl_down(P, d), Qn = xg_shfl_down(Q, d);
+    if (lane + d < 64) {
+      P = fma(Q, Pn, P);
+      Q = Q * Qn;
+    }
+  }
+  double rnext = xg_shfl_down(P, 1);
+  if (lane == 63) rnext = 0.0;
+#pragma unroll
+  for (int tt = M - 1; tt >= 0; --tt) {
+    const double rho = fma(-coef(NW, s + tt + 1), rnext, r[tt]);
+    r[tt] = rho;
+    rnext = rho;
+  }
+  // transposed periodic correction (row 0 carried xbar_0 through both sweeps unchanged)
+  if (per) {
+    const double A1 = a.fcoef_ends[(size_t)tu * 2], C1 = a.fcoef_ends[(size_t)tu * 2 + 1];
+    double part = 0.0;
+#pragma unroll
+    for (int tt = 0; tt < M; ++tt) {
+      const int e = s + tt;
+      if (e == 0) part = part + r[tt];
+      if (e == 1) part = part - A1 * r[tt];
+      if (e == N - 1) part = part - C1 * r[tt];
+    }
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) part += xg_shfl_xor(part, m);
+#pragma unroll
+    for (int tt = 0; tt < M; ++tt) {
+      const int e = s + tt;
+      const double De = coef(NW + 2, e);
+      r[tt] = e == 0 ? De * part : fma(De, part, r[tt]);
+    }
+  }
+  // transposed band through the wavefront's line buffer
+  double* buf = sh + (size_t)wv * N;
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt)
+    if (s + tt < N) buf[s + tt] = r[tt];
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int tt = 0; tt < M; ++tt) {
+    const int j = s + tt;
+    if (j < N) {
+      double acc = 0.0;
+#pragma unroll
+      for (int l = -H; l <= H; ++l) {
+        int se = j - l;
+        se = se < 0 ? se + N : (se >= N ? se - N : se);
+        acc = fma(ct[(size_t)(H + l) * N + se], buf[se], acc);
+      }
+      double v = a.alpha * acc;
+      if (src) v = a.beta * src[j] + v;
+      if (a.accumulate) v = out[j] + v;
+      out[j] = v;
+    }
+  }
+}
+
+template <int NR, int M>
+static void launch_xgt_m(const BandTArgs& a, int nmembers, hipStream_t st) {
+  dim3 grid((unsigned)((a.nlines + 3) / 4), (unsigned)(a.nf * nmembers));
+  hipLaunchKernelGGL((line_solve_xgt_kernel<NR, M>), grid, dim3(256), sizeof(double) * 4 * (size_t)a.N, st, a);
+}
+
+template <int NR>
+static bool launch_xgt_nr(const BandTArgs& a, int nmembers, hipStream_t st) {
+  const int M = a.N <= 256 ? 4 : (a.N <= 512 ? 8 : 0);
+  if (M == 4) launch_xgt_m<NR, 4>(a, nmembers, st);
+  else if (M == 8) launch_xgt_m<NR, 8>(a, nmembers, st);
+  else return false;
+  return true;
+}
+
+inline bool launch_line_solve_xgt_impl(const BandTArgs& a, int nmembers, hipStream_t st) {
+  if (a.nR - 1 > a.N) return false;
+  switch (a.nR) {
+    case 2: return launch_xgt_nr<2>(a, nmembers, st);
+    case 3: return launch_xgt_nr<3>(a, nmembers, st);
+    case 4: return launch_xgt_nr<4>(a, nmembers, st);
+    case 5: return launch_xgt_nr<5>(a, nmembers, st);
+    case 6: return launch_xgt_nr<6>(a, nmembers, st);
+    case 7: return launch_xgt_nr<7>(a, nmembers, st);
+    case 8: return launch_xgt_nr<8>(a, nmembers, st);
+    case 9: return launch_xgt_nr<9>(a, nmembers, st);
+    case 10: return launch_xgt_nr<10>(a, nmembers, st);
+    case 11: return launch_xgt_nr<11>(a, nmembers, st);
+    default: return false;
+  }
+}
+
+}  // namespace gatt

@@ -14,7 +14,11 @@
  *     members, stage work arrays, coefficient tables and the boundary program.
  *     State stays resident on the device between calls; only scalars return.
  *   - Calls on one context are ordered on the context's HIP stream and are not
- *     re-entrant.  Host pointers are copied before the call returns.
+ *     re-entrant.  Host pointers are copied before the call returns.  That
+ *     stream is non-blocking: device pointers passed in must hold their data
+ *     when the call is made (synchronise the stream that produced them first);
+ *     the entry points that take device pointers return after their own work
+ *     on them has completed.
  *   - 2-D problems (nz == 1) carry U,V,R,E; W is identically zero and the
  *     reported W change is 0 (RK4_2D passes W through untouched,
  *     source/Fortran/timeIntegration.F90:8-156).
