@@ -143,29 +143,38 @@ def adjoint_cost(runtime, nx=512, ny=192, nz=64, steps=6):
     prop.load(q0)
     prop.solver.apply_bound_cond(0.0)
     q0 = prop.save()
-    prop.steps(2)
-    prop.solver.sync()
-    t0 = time.perf_counter()
-    prop.steps(steps)
-    prop.solver.sync()
-    res["forward"] = (time.perf_counter() - t0) / steps
-    prop.load(q0)
-    prop.load_perturbation(v)
-    prop.solver.tlm_step(0.0, prop.dt)
-    prop.solver.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
+
+    def measure(tag):
+        prop.load(q0)
+        prop.steps(2)
+        prop.solver.sync()
+        t0 = time.perf_counter()
+        prop.steps(steps)
+        prop.solver.sync()
+        res["forward" + tag] = (time.perf_counter() - t0) / steps
+        prop.load(q0)
+        prop.load_perturbation(v)
         prop.solver.tlm_step(0.0, prop.dt)
-    prop.solver.sync()
-    res["tangent"] = (time.perf_counter() - t0) / steps
-    prop.load(q0)
-    prop.load_perturbation(v)
-    prop.solver.adj_step(0.0, prop.dt)
-    t0 = time.perf_counter()
-    for _ in range(steps):
+        prop.solver.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            prop.solver.tlm_step(0.0, prop.dt)
+        prop.solver.sync()
+        res["tangent" + tag] = (time.perf_counter() - t0) / steps
+        prop.load(q0)
+        prop.load_perturbation(v)
         prop.solver.adj_step(0.0, prop.dt)
-    prop.solver.sync()
-    res["adjoint"] = (time.perf_counter() - t0) / steps
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            prop.solver.adj_step(0.0, prop.dt)
+        prop.solver.sync()
+        res["adjoint" + tag] = (time.perf_counter() - t0) / steps
+
+    measure("")
+    # the same with the line-solve shapes picked by the autotuner for this case
+    from gatt_dns_amd import autotune
+    autotune.tune(prop.solver, reps=3)
+    measure("_tuned")
     prop.close()
     return {"workload": "spanwise-periodic boundary layer %dx%dx%d, Re 1000, Ma 0.5, walls, buffers, "
                         "SFD (frozen mean) in the buffers" % (nx, ny, nz),
@@ -173,6 +182,10 @@ def adjoint_cost(runtime, nx=512, ny=192, nz=64, steps=6):
             "tangent_ms_per_step": round(res["tangent"] * 1e3, 3),
             "adjoint_ms_per_step": round(res["adjoint"] * 1e3, 3),
             "adjoint_over_forward": round(res["adjoint"] / res["forward"], 3),
+            "tuned_shapes": {"forward_ms_per_step": round(res["forward_tuned"] * 1e3, 3),
+                             "tangent_ms_per_step": round(res["tangent_tuned"] * 1e3, 3),
+                             "adjoint_ms_per_step": round(res["adjoint_tuned"] * 1e3, 3),
+                             "adjoint_over_forward": round(res["adjoint_tuned"] / res["forward_tuned"], 3)},
             "note": "adjoint step includes recomputing its base step's stage states and boundary applications"}
 
 
