@@ -155,6 +155,35 @@ def test_filters(rt, oracle_mod, name, theta):
         assert rel_err(got[k], want[k]) < 1e-13, (name, theta, k)
 
 
+@pytest.mark.parametrize("name", ["cavity33", "box32"])
+def test_filters_every_shape(rt, oracle_mod, name):
+    """The blended in-place filter (FilterCharTime > 0) through every accepted shape of every variant,
+    one direction at a time, against the oracle's applyFilters."""
+    case = CASES[name]()
+    o = oracle_mod.Oracle(case)
+    q = perturbed_state(case, 17)
+    want = o.apply_filters(q, 0.3)
+    shapes = [(v, M, 1) for v in (1, 2, 3) for M in (16, 32)] + [(4, M, 1) for M in (4, 8, 16)]
+    shapes += [(0, G, w) for G in (4, 16, 64) for w in (1, 4)]
+    tried = 0
+    for d in range(3 if case.is3d else 2):
+        for v, p, w in shapes:
+            s = rt.Solver(case)
+            try:
+                s.set_line_shape(d, 1, v, p, w)
+            except rt.GattError:
+                s.close()
+                continue
+            s.set_state(q)
+            s.apply_filters(0.3)
+            got = s.get_state()
+            s.close()
+            for k in fields_of(case):
+                assert rel_err(got[k], want[k]) < 1e-13, (name, d, v, p, w, k, rel_err(got[k], want[k]))
+            tried += 1
+    assert tried >= 8
+
+
 @pytest.mark.parametrize("name", ["cavity129", "box32"])
 def test_one_step(rt, oracle_mod, name):
     case = CASES[name]()
