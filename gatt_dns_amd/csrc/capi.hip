@@ -173,14 +173,16 @@ int dev_upload(gatt_ctx* c, const T* host, size_t count, T** out) {
 }
 
 // Default shape before tuning.  Many lines of at most 256 points: the fused
-// wave-segment kernel with 16 (or 32) points per lane; otherwise the lane-group
-// kernel with 16 points per lane where the line allows it.
+// wave-segment kernel (along x through the 32-line tile); otherwise the
+// lane-group kernel with 16 points per lane where the line allows it.
 LineShape choose_shape(int dir, int N, int nR, long long nlines, long long max_lds, bool uniform_tiles) {
   if (nlines >= 1024 && uniform_tiles) {
-    // 32 points per lane along x and for wide bands (fewer, fuller wavefronts), else 16
+    // x: the 32-line LDS tile (unit-stride loads and stores) with 16 points per lane;
+    // y and z: 32 points per lane for wide bands (fewer, fuller wavefronts), else 16
     const bool wide = dir == 0 || nR >= 5;
-    for (int v : {2, 1}) {
-      for (int M : {wide ? 32 : 16, wide ? 16 : 32}) {
+    for (int v : {3, 2, 1}) {
+      if (v == 3 && dir != 0) continue;
+      for (int M : {(wide && v != 3) ? 32 : 16, (wide && v != 3) ? 16 : 32}) {
         LineShape s;
         s.variant = v;
         s.M = M;

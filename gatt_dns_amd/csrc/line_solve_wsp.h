@@ -344,10 +344,24 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
     }
   }
   const int total = nact * N;
-  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
-    const int ep = idx - lp * N;
-    tile[ep * 33 + lp] = in[idx];
+  // pairs of doubles where the stretch is 16-byte aligned and a pair never straddles two lines
+  const bool pairs = a.xvec && (N & 1) == 0;
+  if (pairs) {
+    const double2* in2 = reinterpret_cast<const double2*>(in);
+    for (int i2 = threadIdx.x; i2 < total / 2; i2 += blockDim.x) {
+      const double2 v = in2[i2];
+      const int idx = 2 * i2;
+      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+      const int ep = idx - lp * N;
+      tile[ep * 33 + lp] = v.x;
+      tile[(ep + 1) * 33 + lp] = v.y;
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+      const int ep = idx - lp * N;
+      tile[ep * 33 + lp] = in[idx];
+    }
   }
   __syncthreads();
   double fw[W];
@@ -383,10 +397,20 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
   for (int tt = 0; tt < M; ++tt)
     if (tt < cnt) tile[(s + tt) * 33 + l] = r[tt];
   __syncthreads();
-  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
-    const int ep = idx - lp * N;
-    out[idx] = tile[ep * 33 + lp];
+  if (pairs) {
+    double2* out2 = reinterpret_cast<double2*>(out);
+    for (int i2 = threadIdx.x; i2 < total / 2; i2 += blockDim.x) {
+      const int idx = 2 * i2;
+      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+      const int ep = idx - lp * N;
+      out2[i2] = make_double2(tile[ep * 33 + lp], tile[(ep + 1) * 33 + lp]);
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+      const int ep = idx - lp * N;
+      out[idx] = tile[ep * 33 + lp];
+    }
   }
 }
 
