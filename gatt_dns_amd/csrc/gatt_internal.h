@@ -132,9 +132,9 @@ inline bool line_shape_ok(const LineShape& s, int N, int nR, int dir = 1, long l
     const int gmax = s.variant == 3 ? 16 : (s.M >= 32 ? 8 : 16);   // variant 3: two segments per wavefront
     if (G < 1 || G > gmax || s.M + nR - 1 > N + 1 || nR - 1 > N) return false;
     const long long cs = (2LL * nR - 1 + 3 + 1) & ~1LL;
-    if (s.variant == 3) {   // 32 lines per workgroup through a tile of 33*N words
+    if (s.variant == 3) {   // 32 lines per workgroup through a tile of 32*N words
       const long long GA = (G + 1) & ~1LL;
-      const long long work3 = 5LL * GA * 32 + GA * (s.M * cs + 2), tile3 = 33LL * N;
+      const long long work3 = 5LL * GA * 32 + GA * (s.M * cs + 2), tile3 = 32LL * N;
       return 8 * (work3 > tile3 ? work3 : tile3) <= max_lds;
     }
     const long long work = 5LL * G * 64 + G * (s.M * cs + 2);

@@ -293,10 +293,10 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>(
 
 // Variant 3, x lines through an LDS tile: a workgroup owns 32 lines, one
 // contiguous stretch of 32*N doubles moved between HBM and the tile (N rows of
-// 33 words) with unit-stride accesses.  Lanes 0-31 of wavefront w work on
+// 32 words) with unit-stride accesses.  Lanes 0-31 of wavefront w work on
 // segment 2w of the 32 lines, lanes 32-63 on segment 2w+1; the scan maps and
 // the coefficient slots (one per segment) reuse the tile's words between the
-// window reads and the result writes.  At N = 256 the tile takes 66 KiB, so
+// window reads and the result writes.  At N = 256 the tile takes 64 KiB, so
 // two workgroups share a compute unit.
 template <int NR, int M, bool BLEND>
 __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
@@ -318,6 +318,9 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
   int cnt = N - s;
   cnt = cnt < 0 ? 0 : (cnt > M ? M : cnt);
   double* tile = sh;
+  // tile word of point e of line l: rows of 32 words, the line index XOR-swizzled by the row
+  // so that the pair writes and reads (one line, consecutive points) spread over the banks
+  auto tix = [](int e, int ll) { return e * 32 + (ll ^ ((e >> 1) & 31)); };
   double* shD = sh;                            // periodic dot partials [GA][32]
   double* shP = sh + GA * 32;                  // forward maps  [2][GA][32]
   double* shB = sh + 3 * GA * 32;              // backward maps [2][GA][32]
@@ -353,14 +356,14 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
       const int idx = 2 * i2;
       const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
       const int ep = idx - lp * N;
-      tile[ep * 33 + lp] = v.x;
-      tile[(ep + 1) * 33 + lp] = v.y;
+      tile[tix(ep, lp)] = v.x;
+      tile[tix(ep + 1, lp)] = v.y;
     }
   } else {
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
       const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
       const int ep = idx - lp * N;
-      tile[ep * 33 + lp] = in[idx];
+      tile[tix(ep, lp)] = in[idx];
     }
   }
   __syncthreads();
@@ -371,7 +374,7 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
     e = e < 0 ? e + N : (e >= N ? e - N : e);
     e = e >= N ? e - N : e;
     e = e < 0 ? 0 : (e >= N ? N - 1 : e);
-    fw[u] = tile[e * 33 + l];
+    fw[u] = tile[tix(e, l)];
   }
   __syncthreads();   // the tile's words now hold the maps and the coefficient slots
   // a segment's slot is written and read by the same half wavefront, whose LDS accesses run in order
@@ -395,7 +398,7 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
   __syncthreads();   // every map and slot read before the tile is rewritten
 #pragma unroll
   for (int tt = 0; tt < M; ++tt)
-    if (tt < cnt) tile[(s + tt) * 33 + l] = r[tt];
+    if (tt < cnt) tile[tix(s + tt, l)] = r[tt];
   __syncthreads();
   if (pairs) {
     double2* out2 = reinterpret_cast<double2*>(out);
@@ -403,13 +406,13 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
       const int idx = 2 * i2;
       const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
       const int ep = idx - lp * N;
-      out2[i2] = make_double2(tile[ep * 33 + lp], tile[(ep + 1) * 33 + lp]);
+      out2[i2] = make_double2(tile[tix(ep, lp)], tile[tix(ep + 1, lp)]);
     }
   } else {
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
       const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
       const int ep = idx - lp * N;
-      out[idx] = tile[ep * 33 + lp];
+      out[idx] = tile[tix(ep, lp)];
     }
   }
 }
@@ -418,11 +421,11 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
 inline size_t wsp_lds_bytes(int G, int M, int nR) {
   return sizeof(double) * ((size_t)5 * G * 64 + (size_t)G * (M * wsp_coef_stride(nR) + 2));
 }
-// Variant 3: the tile (33*N words), which holds the maps and slots in turn.
+// Variant 3: the tile (32*N words), which holds the maps and slots in turn.
 inline size_t wspx_lds_bytes(int N, int M, int nR) {
   const int G = ((N + M - 1) / M + 1) & ~1;
   const size_t work = (size_t)5 * G * 32 + (size_t)G * (M * wsp_coef_stride(nR) + 2);
-  const size_t tile = (size_t)33 * N;
+  const size_t tile = (size_t)32 * N;
   return sizeof(double) * (work > tile ? work : tile);
 }
 
