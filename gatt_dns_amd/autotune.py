@@ -19,6 +19,7 @@ def candidates(n):
     """(variant, param, waves per block) triples; the library rejects shapes that do not fit."""
     out = [(4, M, 0) for M in (4, 8, 16)]
     out += [(v, M, 0) for v in (2, 3, 1) for M in (16, 32)]
+    out += [(3, M, 2) for M in (16, 32)]   # x tile of 16 lines
     out += [(0, G, w) for G in (2, 4, 8, 16, 32, 64) if (n + G - 1) // G <= 32 for w in (1, 2, 4)]
     return out
 

@@ -1152,7 +1152,8 @@ int gatt_set_line_shape(gatt_ctx* c, int32_t dir, int32_t kind, int32_t variant,
   } else if (variant >= 1 && variant <= 3) {
     s.M = param;
     s.G = param > 0 ? (N + param - 1) / param : 0;
-    s.waves_per_block = s.G;
+    // variant 3: waves_per_block selects the tile (1: 32 lines, 2: 16 lines)
+    s.waves_per_block = variant == 3 ? (wpb == 2 ? 2 : 1) : s.G;
   } else {
     return fail(GATT_E_INVALID, "unknown line-solve variant");
   }

@@ -56,7 +56,7 @@ struct SolveArgs {
   int nf;                     // fields per member
   int nmem;                   // members (fused wave-segment kernel)
   int xvec;                   // x lines 16-byte aligned (fused wave-segment kernel)
-  int xtile;                  // x lines staged through an LDS tile (fused wave-segment kernel)
+  int xtile;                  // x lines staged through an LDS tile: lines per workgroup (16 or 32), 0 = no tile
   long long in_stride;        // doubles between members in the input arrays
   long long out_stride;       // doubles between members in the output arrays
   const double* in[kMaxPtr];
@@ -95,7 +95,7 @@ inline void launch_line_solve(const SolveArgs& a, const LineShape& s, int nmembe
   }
   if (s.variant == 2 || s.variant == 3) {
     SolveArgs b = a;
-    b.xtile = s.variant == 3;
+    b.xtile = s.variant == 3 ? 32 / (s.waves_per_block == 2 ? 2 : 1) : 0;   // lines per workgroup
     if (a.dir == 0) launch_line_solve_wsp_dir<0>(b, s, nmembers, st);
     else if (a.dir == 1) launch_line_solve_wsp_dir<1>(b, s, nmembers, st);
     else launch_line_solve_wsp_dir<2>(b, s, nmembers, st);
@@ -132,9 +132,11 @@ inline bool line_shape_ok(const LineShape& s, int N, int nR, int dir = 1, long l
     const int gmax = s.variant == 3 ? 16 : (s.M >= 32 ? 8 : 16);   // variant 3: two segments per wavefront
     if (G < 1 || G > gmax || s.M + nR - 1 > N + 1 || nR - 1 > N) return false;
     const long long cs = (2LL * nR - 1 + 3 + 1) & ~1LL;
-    if (s.variant == 3) {   // 32 lines per workgroup through a tile of 32*N words
-      const long long GA = (G + 1) & ~1LL;
-      const long long work3 = 5LL * GA * 32 + GA * (s.M * cs + 2), tile3 = 32LL * N;
+    if (s.variant == 3) {   // 32 (waves_per_block 1) or 16 (2) lines per workgroup through a tile of lw*N words
+      const long long lw = s.waves_per_block == 2 ? 16 : 32, spw = 64 / lw;
+      const long long GA = (G + spw - 1) / spw * spw;
+      if (GA / spw > 8 || GA > 32) return false;   // at most 512 threads
+      const long long work3 = 5LL * GA * lw + GA * (s.M * cs + 2), tile3 = lw * N;
       return 8 * (work3 > tile3 ? work3 : tile3) <= max_lds;
     }
     const long long work = 5LL * G * 64 + G * (s.M * cs + 2);

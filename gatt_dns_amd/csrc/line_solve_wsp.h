@@ -291,42 +291,43 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>(
   }
 }
 
-// Variant 3, x lines through an LDS tile: a workgroup owns 32 lines, one
-// contiguous stretch of 32*N doubles moved between HBM and the tile (N rows of
-// 32 words) with unit-stride accesses.  Lanes 0-31 of wavefront w work on
-// segment 2w of the 32 lines, lanes 32-63 on segment 2w+1; the scan maps and
-// the coefficient slots (one per segment) reuse the tile's words between the
-// window reads and the result writes.  At N = 256 the tile takes 64 KiB, so
-// two workgroups share a compute unit.
-template <int NR, int M, bool BLEND>
+// Variant 3, x lines through an LDS tile: a workgroup owns LW (16 or 32) lines,
+// one contiguous stretch of LW*N doubles moved between HBM and the tile (N rows
+// of LW words) with unit-stride accesses.  Lane l of wavefront w works on line
+// l % LW and segment (64/LW)*w + l/LW; the scan maps and the coefficient slots
+// (one per segment) reuse the tile's words between the window reads and the
+// result writes.  At N = 256 a 32-line tile takes 64 KiB (two workgroups per
+// compute unit), a 16-line tile 32 KiB (four).
+template <int NR, int M, bool BLEND, int LW>
 __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
   extern __shared__ double sh[];
   constexpr int H = NR - 1;
   constexpr int CS = wsp_coef_stride(NR);
   constexpr int W = M + 2 * H;
   constexpr int SLOT = M * CS + 2;
-  constexpr int CP = (SLOT + 31) / 32;         // coefficient words per lane (32 lanes per segment)
+  constexpr int SPW = 64 / LW;                 // segments per wavefront
+  constexpr int CP = (SLOT + LW - 1) / LW;      // coefficient words per lane (LW lanes per segment)
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int l = lane & 31;
-  const int sg = 2 * w + (lane >> 5);
+  const int l = lane & (LW - 1);
+  const int sg = SPW * w + lane / LW;
   const int N = a.op.N;
   const int G = (N + M - 1) / M;               // segments per line
-  const int GA = (G + 1) & ~1;                 // segments of the wavefronts (the last may lie past the line)
+  const int GA = (G + SPW - 1) / SPW * SPW;     // segments of the wavefronts (the last may lie past the line)
   const int per = a.op.periodic;
   const int s = sg * M;
   int cnt = N - s;
   cnt = cnt < 0 ? 0 : (cnt > M ? M : cnt);
   double* tile = sh;
-  // tile word of point e of line l: rows of 32 words, the line index XOR-swizzled by the row
+  // tile word of point e of line l: rows of LW words, the line index XOR-swizzled by the row
   // so that the pair writes and reads (one line, consecutive points) spread over the banks
-  auto tix = [](int e, int ll) { return e * 32 + (ll ^ ((e >> 1) & 31)); };
-  double* shD = sh;                            // periodic dot partials [GA][32]
-  double* shP = sh + GA * 32;                  // forward maps  [2][GA][32]
-  double* shB = sh + 3 * GA * 32;              // backward maps [2][GA][32]
-  double* cl = sh + 5 * GA * 32 + sg * SLOT;   // this segment's coefficients
-  const int line0 = blockIdx.x * 32;
-  const int nact = min(32, a.nlines - line0);
+  auto tix = [](int e, int ll) { return e * LW + (ll ^ ((e >> 1) & (LW - 1))); };
+  double* shD = sh;                            // periodic dot partials [GA][LW]
+  double* shP = sh + GA * LW;                  // forward maps  [2][GA][LW]
+  double* shB = sh + 3 * GA * LW;              // backward maps [2][GA][LW]
+  double* cl = sh + 5 * GA * LW + sg * SLOT;   // this segment's coefficients
+  const int line0 = blockIdx.x * LW;
+  const int nact = min(LW, a.nlines - line0);
   const int f = (int)blockIdx.y % a.nf;
   const int mem = (int)blockIdx.y / a.nf;
   const double* in = a.in[f] + (long long)mem * a.in_stride + (long long)a.nx * line0;
@@ -339,7 +340,7 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
     const double* ends = a.fcoef_ends + (size_t)tu * 2;
 #pragma unroll
     for (int k = 0; k < CP; ++k) {
-      const int jj = l + 32 * k;
+      const int jj = l + LW * k;
       double v = 0.0;
       if (jj < cnt * CS) v = src[jj];
       else if (jj >= M * CS && jj < SLOT) v = ends[jj - M * CS];
@@ -377,10 +378,10 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
     fw[u] = tile[tix(e, l)];
   }
   __syncthreads();   // the tile's words now hold the maps and the coefficient slots
-  // a segment's slot is written and read by the same half wavefront, whose LDS accesses run in order
+  // a segment's slot is written and read by the same wavefront, whose LDS accesses run in order
 #pragma unroll
   for (int k = 0; k < CP; ++k) {
-    const int jj = l + 32 * k;
+    const int jj = l + LW * k;
     if (jj < SLOT) cl[jj] = cpre[k];
   }
   __builtin_amdgcn_wave_barrier();
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
   double r[M];
   double f0[BLEND ? M : 1];
   wsp_band<NR, M, BLEND>(r, f0, fw, cf);
-  const double x1 = wsp_sweeps<NR, M, 2 * kMaxSegments>(r, cf, cends, shD, shP, shB, cnt, sg, G, l, per, 32, GA);
+  const double x1 = wsp_sweeps<NR, M, 2 * kMaxSegments>(r, cf, cends, shD, shP, shB, cnt, sg, G, l, per, LW, GA);
   if (per && sg == 0) r[0] = x1;
   if (BLEND) {
 #pragma unroll
@@ -421,11 +422,12 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
 inline size_t wsp_lds_bytes(int G, int M, int nR) {
   return sizeof(double) * ((size_t)5 * G * 64 + (size_t)G * (M * wsp_coef_stride(nR) + 2));
 }
-// Variant 3: the tile (32*N words), which holds the maps and slots in turn.
-inline size_t wspx_lds_bytes(int N, int M, int nR) {
-  const int G = ((N + M - 1) / M + 1) & ~1;
-  const size_t work = (size_t)5 * G * 32 + (size_t)G * (M * wsp_coef_stride(nR) + 2);
-  const size_t tile = (size_t)32 * N;
+// Variant 3: the tile (lw*N words), which holds the maps and slots in turn.
+inline size_t wspx_lds_bytes(int N, int M, int nR, int lw) {
+  const int spw = 64 / lw;
+  const int G = ((N + M - 1) / M + spw - 1) / spw * spw;
+  const size_t work = (size_t)5 * G * lw + (size_t)G * (M * wsp_coef_stride(nR) + 2);
+  const size_t tile = (size_t)lw * N;
   return sizeof(double) * (work > tile ? work : tile);
 }
 
@@ -439,14 +441,15 @@ static void launch_wsp_k(const SolveArgs& a, int G, hipStream_t st) {
   hipLaunchKernelGGL(k, grid, dim3(64 * G), lds, st, a);
 }
 
-template <int NR, int M, bool BLEND>
+template <int NR, int M, bool BLEND, int LW>
 static void launch_wspx_k(const SolveArgs& a, hipStream_t st) {
-  auto k = line_solve_wspx_kernel<NR, M, BLEND>;
-  const size_t lds = wspx_lds_bytes(a.op.N, M, NR);
+  auto k = line_solve_wspx_kernel<NR, M, BLEND, LW>;
+  const size_t lds = wspx_lds_bytes(a.op.N, M, NR, LW);
   if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  constexpr int SPW = 64 / LW;
   const int G = (a.op.N + M - 1) / M;
-  dim3 grid((unsigned)((a.nlines + 31) / 32), (unsigned)(a.nf * a.nmem));
-  hipLaunchKernelGGL(k, grid, dim3(64 * ((G + 1) / 2)), lds, st, a);
+  dim3 grid((unsigned)((a.nlines + LW - 1) / LW), (unsigned)(a.nf * a.nmem));
+  hipLaunchKernelGGL(k, grid, dim3(64 * ((G + SPW - 1) / SPW)), lds, st, a);
 }
 
 template <int DIR, int NR, int M>
@@ -462,9 +465,14 @@ static void launch_wsp_m(const SolveArgs& a0, int nmembers, hipStream_t st) {
   const int G = (a.op.N + M - 1) / M;
   const bool blend = a.mode == 1 && a.theta > 0.0;
   if constexpr (DIR == 0) {
+    if (a.xtile == 16) {
+      if (blend) launch_wspx_k<NR, M, true, 16>(a, st);
+      else launch_wspx_k<NR, M, false, 16>(a, st);
+      return;
+    }
     if (a.xtile) {
-      if (blend) launch_wspx_k<NR, M, true>(a, st);
-      else launch_wspx_k<NR, M, false>(a, st);
+      if (blend) launch_wspx_k<NR, M, true, 32>(a, st);
+      else launch_wspx_k<NR, M, false, 32>(a, st);
       return;
     }
   }

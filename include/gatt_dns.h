@@ -223,8 +223,9 @@ int gatt_sync(gatt_ctx* ctx);
  *                             records folded into single fused multiply-adds
  *                             and no branches in the sweeps (results agree
  *                             with variants 0 and 1 to rounding);
- *   variant 3 (x only): variant 2 with the 64 lines of a workgroup staged
- *                             through an LDS tile;
+ *   variant 3 (x only): the same sweeps with the lines of a workgroup
+ *                             staged through an LDS tile: 32 lines
+ *                             (waves_per_block 1) or 16 (waves_per_block 2);
  *   variant 4 (x only), one wavefront per line: param = M points per lane
  *                             (4, 8 or 16; 64*M >= n); any operator type
  *                             per line.

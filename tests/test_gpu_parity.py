@@ -98,7 +98,7 @@ def test_line_solve_all_shapes(rt, oracle_mod, name):
         for kind in (0, 1):
             want = o.solve(d, kind, F)
             shapes = [(v, M, 1) for v in (1, 2, 3) for M in (16, 32)]
-            shapes += [(4, M, 1) for M in (4, 8, 16)]
+            shapes += [(3, M, 2) for M in (16, 32)] + [(4, M, 1) for M in (4, 8, 16)]
             shapes += [(0, G, w) for G in (2, 4, 8, 16, 32, 64) for w in (1, 2, 4)]
             for v, p, w in shapes:
                 try:
@@ -164,6 +164,7 @@ def test_filters_every_shape(rt, oracle_mod, name):
     q = perturbed_state(case, 17)
     want = o.apply_filters(q, 0.3)
     shapes = [(v, M, 1) for v in (1, 2, 3) for M in (16, 32)] + [(4, M, 1) for M in (4, 8, 16)]
+    shapes += [(3, M, 2) for M in (16, 32)]
     shapes += [(0, G, w) for G in (4, 16, 64) for w in (1, 4)]
     tried = 0
     for d in range(3 if case.is3d else 2):
