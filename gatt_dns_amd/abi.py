@@ -47,7 +47,7 @@ class ProblemDesc(C.Structure):
                 ("filterDir", I32 * 3), ("filterCharTime", C.c_double),
                 ("bc", BCDesc), ("nForcing", I32), ("forcing", C.POINTER(Forcing)),
                 ("sfdType", I32), ("sfdDelta", C.c_double), ("sfdXval", C.c_double), ("sfdX", DP),
-                ("dxmin", C.c_double * 3)]
+                ("dxmin", C.c_double * 3), ("timeStepping", I32)]
 
 
 GATT_U, GATT_V, GATT_W, GATT_R, GATT_E = range(5)
@@ -106,6 +106,7 @@ def build_desc(case, sfd_override=None):
         p.types[k] = d.iptr(case.types[k])
         p.filterDir[k] = int(case.filter_dirs[k])
     p.filterCharTime = case.filter_char_time
+    p.timeStepping = int(getattr(case, "time_stepping", 1))
     bi = case.bi
     for s, v in enumerate(BC_SLOTS):
         boxes, vals = bi.d[v]

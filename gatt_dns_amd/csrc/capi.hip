@@ -92,6 +92,7 @@ struct gatt_ctx {
   long long max_lds = 65536;
   int filterDir[3] = {1, 1, 1};
   double filterCharTime = -1;
+  int stepping = 1;     // timeStepping: 1 RK4, 2 Euler, 3 SSPRK3
   // boundary program
   BcOp* d_ops = nullptr;
   std::vector<BcOp> ops;
@@ -732,6 +733,35 @@ void run_rk4(gatt_ctx* c, double* from, double* to, double t, double dt) {
   run_bc(c, to, t + dt);
 }
 
+// EULER_3D / EULER_2D (timeIntegration.F90:500-566, 158-222).  EULER_3D applies the
+// boundary conditions at t and EULER_2D at t + dt; the boundary program does not
+// depend on time, so one call serves both.
+void run_euler(gatt_ctx* c, double* from, double* to, double t, double dt) {
+  run_rhs_stage(c, from, from, to, 5, dt);
+  run_bc(c, to, c->is3d ? t : t + dt);
+}
+
+// SSPRK3_3D / SSPRK3_2D (timeIntegration.F90:569-685, 225-335); the Delta slab holds
+// k1 and then k1 + k2.
+void run_ssprk3(gatt_ctx* c, double* from, double* to, double t, double dt) {
+  double* S = c->slab_ptr(SL_S);
+  run_rhs_stage(c, from, from, S, 6, dt);
+  run_bc(c, S, t + dt);
+  run_rhs_stage(c, S, from, S, 7, dt);
+  run_bc(c, S, t + dt / 2.0);
+  run_rhs_stage(c, S, from, to, 8, dt);
+  run_bc(c, to, t + dt);
+}
+
+// The integrator timeStepping selects (main.F90:255-350).
+void run_time_step(gatt_ctx* c, double* from, double* to, double t, double dt) {
+  switch (c->stepping) {
+    case 2: run_euler(c, from, to, t, dt); break;
+    case 3: run_ssprk3(c, from, to, t, dt); break;
+    default: run_rk4(c, from, to, t, dt); break;
+  }
+}
+
 void run_filters(gatt_ctx* c, double* q, double theta) {
   std::vector<int> flds = c->is3d ? std::vector<int>{0, 1, 2, 3, 4} : std::vector<int>{0, 1, 3, 4};
   const int ndir = c->is3d ? 3 : 2;
@@ -772,7 +802,7 @@ void enqueue_step(gatt_ctx* c, int cur, double t, double dt) {
   double* from = c->state(cur);
   double* to = c->state(1 - cur);
   launch_clear_u64(c->d_red, 8 * c->batch, c->stream);
-  run_rk4(c, from, to, t, dt);
+  run_time_step(c, from, to, t, dt);
   run_filters(c, to, dt / c->filterCharTime);
   run_sfd(c, to, dt);
   run_bc(c, to, t + dt);
@@ -831,6 +861,7 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
   c->fc.qk = d->gamma / d->Re / d->Pr;
   for (int k = 0; k < 3; ++k) c->filterDir[k] = d->filterDir[k];
   c->filterCharTime = d->filterCharTime;
+  c->stepping = d->timeStepping == 2 || d->timeStepping == 3 ? d->timeStepping : 1;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (int k = 0; k < 3; ++k) {
     HIPCHK(hipStreamCreateWithFlags(&c->aux[k], hipStreamNonBlocking));
@@ -996,6 +1027,24 @@ int gatt_get_mean(gatt_ctx* c, int32_t member, double* const* q) {
 int gatt_rk4(gatt_ctx* c, double t, double dt) {
   if (check_ctx(c)) return GATT_E_INVALID;
   run_rk4(c, c->state(c->cur), c->state(1 - c->cur), t, dt);
+  c->cur = 1 - c->cur;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_euler(gatt_ctx* c, double t, double dt) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  run_euler(c, c->state(c->cur), c->state(1 - c->cur), t, dt);
+  c->cur = 1 - c->cur;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_ssprk3(gatt_ctx* c, double t, double dt) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  run_ssprk3(c, c->state(c->cur), c->state(1 - c->cur), t, dt);
   c->cur = 1 - c->cur;
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1685,6 +1734,7 @@ double* tape_ptr(gatt_ctx* c, int s, int wk) { return c->tape + ((long long)s * 
 
 int lin_supported(gatt_ctx* c) {
   if (c->forcing) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps do not cover the turbulence forcing");
+  if (c->stepping != 1) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps follow the RK4 integrator only");
   if (c->sfdType > 0 && c->sfdDelta > 0)
     return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps need a frozen SFD mean (Delta = inf)");
   const int ndir = c->is3d ? 3 : 2;

@@ -197,7 +197,7 @@ struct AssembleArgs {
   const double* G;      // 12 second derivatives (x: txx,txy,txz,qx | y: txy,tyy,tyz,qy | z: txz,tyz,tzz,qz)
   const double* sums;   // forcing sums per member (6 doubles) or nullptr
   double forcing_eps, forcing_e0;
-  int stage;            // 1..4 ; 0 = write the raw RHS into Out
+  int stage;            // 0 raw RHS into Out, 1..4 RK4, 5 Euler, 6..8 SSPRK3
   double dt;
   int is3d;
 };

@@ -58,7 +58,9 @@ void launch_visc(const ViscArgs& a, int nmembers, hipStream_t st) {
 }
 
 // nseq.F90:280-318 (3-D), 106-134 (2-D), the turbulenceGenerator forcing
-// (turbulenceGenerator.F90:57-61) and one RK4 stage (timeIntegration.F90:393-487).
+// (turbulenceGenerator.F90:57-61) and one stage update: stage 0 stores the right-hand
+// side, 1-4 are the RK4 stages (timeIntegration.F90:393-487), 5 the Euler update and
+// 6-8 the SSPRK3 stages.
 __global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs a) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= a.np) return;
@@ -147,7 +149,18 @@ __global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs a) {
       case 1: Dl[o] = dt / 6.0 * d[f]; O[o] = Q[o] + dt / 2.0 * d[f]; break;
       case 2: Dl[o] = Dl[o] + dt / 3.0 * d[f]; O[o] = Q[o] + dt / 2.0 * d[f]; break;
       case 3: Dl[o] = Dl[o] + dt / 3.0 * d[f]; O[o] = Q[o] + dt * d[f]; break;
-      default: O[o] = Q[o] + Dl[o] + dt / 6.0 * d[f]; break;
+      case 4: O[o] = Q[o] + Dl[o] + dt / 6.0 * d[f]; break;
+      // EULER (timeIntegration.F90:553-557)
+      case 5: O[o] = Q[o] + dt * d[f]; break;
+      // SSPRK3 (timeIntegration.F90:623-627, 647-651, 672-676): Delta keeps k1, then k1 + k2
+      case 6: Dl[o] = d[f]; O[o] = Q[o] + dt * d[f]; break;
+      case 7: {
+        const double k = Dl[o] + d[f];
+        Dl[o] = k;
+        O[o] = Q[o] + dt * k / 4.0;
+        break;
+      }
+      default: O[o] = Q[o] + dt * (Dl[o] + 4.0 * d[f]) / 6.0; break;
     }
   }
 }

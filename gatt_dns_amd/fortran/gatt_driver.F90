@@ -106,6 +106,7 @@ program gatt_driver
   desc%forcing = c_loc(forc)
   desc%sfdType = SFD; desc%sfdDelta = SFD_Delta; desc%sfdXval = SFD_X_val
   desc%dxmin = dxmin
+  desc%timeStepping = timeStepping
 
   rc = gatt_create(desc, 1_c_int32_t, ctx)
   call check(rc, 'gatt_create')

@@ -58,6 +58,7 @@ module gatt_hip
     real(c_double) :: sfdDelta = 0, sfdXval = 0
     type(c_ptr) :: sfdX = c_null_ptr
     real(c_double) :: dxmin(3) = 0
+    integer(c_int32_t) :: timeStepping = 1
   end type
 
   interface

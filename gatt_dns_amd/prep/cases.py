@@ -67,6 +67,7 @@ class Case:
     sfd_x_val: float = 0.0
     sfd_x: np.ndarray = None
     dxmin: tuple = (0.0, 0.0, 0.0)
+    time_stepping: int = 1           # timeStepping: 1 RK4, 2 Euler, 3 SSPRK3 (writeFortranParameters.m:50-58)
     dt: float = 0.0
     flow_region: np.ndarray = None   # True at flow points (False inside walls)
     state0: dict = None

@@ -98,6 +98,7 @@ def write_includes(case, outdir, dt=None):
     p.append("    real*8 :: SFD_Delta = %s" % _lit(case.sfd_delta))
     p.append("    real*8 :: SFD_X_val = %s" % _lit(case.sfd_x_val))
     p.append("    real*8,dimension(3) :: dxmin = (/%s,%s,%s/)" % tuple(_lit(v) for v in case.dxmin))
+    p.append("    integer :: timeStepping = %d" % getattr(case, "time_stepping", 1))
     open(os.path.join(outdir, "parameters.F90"), "w").write("\n".join(p) + "\n")
 
     m = []

@@ -26,7 +26,8 @@ EXPORTS = ["gatt_last_error", "gatt_device_info", "gatt_create", "gatt_destroy",
            "gatt_get_line_shape", "gatt_set_flow_region", "gatt_vector_length", "gatt_set_base",
            "gatt_perturb", "gatt_collect", "gatt_block_dot", "gatt_block_update", "gatt_group_create",
            "gatt_group_destroy", "gatt_group_advance", "gatt_set_perturbation", "gatt_get_perturbation",
-           "gatt_copy_fields", "gatt_tlm_step", "gatt_adj_step", "gatt_line_solve_t"]
+           "gatt_copy_fields", "gatt_tlm_step", "gatt_adj_step", "gatt_line_solve_t", "gatt_euler",
+           "gatt_ssprk3"]
 
 SLOT_STATE = 0
 SLOT_PERTURBATION = 1
@@ -68,6 +69,8 @@ def load_library(path=LIB_PATH):
     for name in ("gatt_set_state", "gatt_get_state", "gatt_set_mean", "gatt_get_mean"):
         getattr(L, name).argtypes = [V, I, C.POINTER(DP)]
     L.gatt_rk4.argtypes = [V, C.c_double, C.c_double]
+    L.gatt_euler.argtypes = [V, C.c_double, C.c_double]
+    L.gatt_ssprk3.argtypes = [V, C.c_double, C.c_double]
     L.gatt_apply_filters.argtypes = [V, C.c_double]
     L.gatt_apply_bound_cond.argtypes = [V, C.c_double]
     L.gatt_sfd.argtypes = [V, C.c_double]
@@ -162,6 +165,12 @@ class Solver:
     # -- reference subroutines -------------------------------------------------
     def rk4(self, t, dt):
         _check(self.lib.gatt_rk4(self.ctx, t, dt))
+
+    def euler(self, t, dt):
+        _check(self.lib.gatt_euler(self.ctx, t, dt))
+
+    def ssprk3(self, t, dt):
+        _check(self.lib.gatt_ssprk3(self.ctx, t, dt))
 
     def apply_filters(self, theta):
         _check(self.lib.gatt_apply_filters(self.ctx, theta))

@@ -139,6 +139,8 @@ typedef struct gatt_problem_desc {
   double sfdXval;           /* SFD_X_val (type 1) */
   const double* sfdX;       /* chi field nx*ny*nz (type 2) or NULL */
   double dxmin[3];
+  int32_t timeStepping;     /* timeStepping of parameters.F90: 1 RK4 (also 0), 2 Euler, 3 SSPRK3
+                               (writeFortranParameters.m:50-58, main.F90:255-350) */
 } gatt_problem_desc;
 
 typedef struct gatt_ctx gatt_ctx;
@@ -167,6 +169,14 @@ int gatt_get_mean(gatt_ctx* ctx, int32_t member, double* const* q);
  * (RK4_3D / RK4_2D, source/Fortran/timeIntegration.F90:338-498, 8-156).
  * The result replaces the state. */
 int gatt_rk4(gatt_ctx* ctx, double t, double dt);
+
+/* One explicit Euler step (EULER_3D / EULER_2D, timeIntegration.F90:500-566,
+ * 158-222) and one three-stage SSP Runge-Kutta step (SSPRK3_3D / SSPRK3_2D,
+ * timeIntegration.F90:569-685, 225-335), boundary conditions after every stage.
+ * The result replaces the state.  gatt_step and gatt_advance use the integrator
+ * named by the descriptor's timeStepping. */
+int gatt_euler(gatt_ctx* ctx, double t, double dt);
+int gatt_ssprk3(gatt_ctx* ctx, double t, double dt);
 
 /* Compact filter in x, y, z with blend factor theta (dt/FilterCharTime;
  * theta <= 0 replaces the field) -- applyFilters3D / applyFilters2D,

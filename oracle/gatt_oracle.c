@@ -15,6 +15,9 @@
  *   disturbances (static)   lidDrivenMovement, holdInlet
  *                           source/disturbances/lidDrivenMovement.F90:1-19, source/Fortran/holdInlet.F90:3-48
  *   oracle_rk4              RK4_2D / RK4_3D           source/Fortran/timeIntegration.F90:8-156, 338-498
+ *   oracle_euler            EULER_2D / EULER_3D       source/Fortran/timeIntegration.F90:158-222, 500-566
+ *   oracle_ssprk3           SSPRK3_2D / SSPRK3_3D     source/Fortran/timeIntegration.F90:225-335, 569-685
+ *   oracle_time_step        timeStepping select       source/Fortran/main.F90:255-350
  *   oracle_sfd              SFD block of main         source/Fortran/main.F90:381-402
  *   oracle_step             one main-loop iteration   source/Fortran/main.F90:196-462
  *
@@ -593,6 +596,72 @@ void oracle_rk4(oracle_ctx* c, double t, double dt, const double* const* q, doub
   oracle_apply_bound_cond(c, t + dt, stage);
 }
 
+/* EULER: timeIntegration.F90:158-222 (2-D), 500-566 (3-D).  EULER_3D applies the
+ * boundary conditions at t, EULER_2D at t + dt; the boundary values here do not
+ * depend on time, so both calls give the same state. */
+void oracle_euler(oracle_ctx* c, double t, double dt, const double* const* q, double* const* qn) {
+  const size_t np = c->np;
+  const int is3d = c->p->nz > 1;
+  double* d[5];
+  for (int f = 0; f < 5; ++f) d[f] = c->work[45 + f];
+  double* stage[5];
+  for (int f = 0; f < 5; ++f) stage[f] = qn[f];
+  if (!is3d) stage[GATT_W] = (double*)q[GATT_W];
+  oracle_nseq(c, q, d);
+  for (int f = 0; f < 5; ++f) {
+    if (!is3d && f == GATT_W) continue;
+    for (size_t n = 0; n < np; ++n) stage[f][n] = q[f][n] + dt * d[f][n];
+  }
+  oracle_apply_bound_cond(c, is3d ? t : t + dt, stage);
+}
+
+/* SSPRK3: timeIntegration.F90:225-335 (2-D), 569-685 (3-D).  The running sum k1 + k2
+ * is kept so that the third update evaluates (k1 + k2) + 4 k3 in the reference's
+ * left-to-right order, and every update is (dt * k) / c as written there. */
+void oracle_ssprk3(oracle_ctx* c, double t, double dt, const double* const* q, double* const* qn) {
+  const size_t np = c->np;
+  const int is3d = c->p->nz > 1;
+  double* d[5]; double* ks[5];
+  for (int f = 0; f < 5; ++f) { d[f] = c->work[45 + f]; ks[f] = c->work[50 + f]; }
+  double* stage[5];
+  for (int f = 0; f < 5; ++f) stage[f] = qn[f];
+  if (!is3d) stage[GATT_W] = (double*)q[GATT_W];
+  oracle_nseq(c, q, d);
+  for (int f = 0; f < 5; ++f) {
+    if (!is3d && f == GATT_W) continue;
+    for (size_t n = 0; n < np; ++n) {
+      ks[f][n] = d[f][n];
+      stage[f][n] = q[f][n] + dt * d[f][n];
+    }
+  }
+  oracle_apply_bound_cond(c, t + dt, stage);
+  oracle_nseq(c, (const double* const*)stage, d);
+  for (int f = 0; f < 5; ++f) {
+    if (!is3d && f == GATT_W) continue;
+    for (size_t n = 0; n < np; ++n) {
+      ks[f][n] = ks[f][n] + d[f][n];
+      stage[f][n] = q[f][n] + dt * ks[f][n] / 4.0;
+    }
+  }
+  oracle_apply_bound_cond(c, t + dt / 2.0, stage);
+  oracle_nseq(c, (const double* const*)stage, d);
+  for (int f = 0; f < 5; ++f) {
+    if (!is3d && f == GATT_W) continue;
+    for (size_t n = 0; n < np; ++n) stage[f][n] = q[f][n] + dt * (ks[f][n] + 4.0 * d[f][n]) / 6.0;
+  }
+  oracle_apply_bound_cond(c, t + dt, stage);
+}
+
+/* The integrator that timeStepping selects (main.F90:255-350): 2 Euler, 3 SSPRK3,
+ * anything else RK4. */
+void oracle_time_step(oracle_ctx* c, double t, double dt, const double* const* q, double* const* qn) {
+  switch (c->p->timeStepping) {
+    case 2: oracle_euler(c, t, dt, q, qn); break;
+    case 3: oracle_ssprk3(c, t, dt, q, qn); break;
+    default: oracle_rk4(c, t, dt, q, qn); break;
+  }
+}
+
 /* ------------------------------------------------------------------------- */
 /* SFD and the main-loop step: main.F90:119-165, 196-462                     */
 /* ------------------------------------------------------------------------- */
@@ -640,7 +709,7 @@ int oracle_step(oracle_ctx* c, double t, double dt, double* const* q, double* um
   if (umax) memcpy(umax, um, sizeof(um));
   /* Wnew starts at zero in 2-D and is only touched by boundary conditions (main.F90:184-188) */
   if (p->nz == 1) memset(qn[GATT_W], 0, np * sizeof(double));
-  oracle_rk4(c, t, dt, (const double* const*)q, qn);
+  oracle_time_step(c, t, dt, (const double* const*)q, qn);
   double theta = dt / p->filterCharTime;
   oracle_apply_filters(c, theta, qn);
   oracle_sfd(c, dt, qn);

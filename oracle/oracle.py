@@ -41,6 +41,9 @@ def lib():
         L.oracle_nseq.argtypes = [V, C.POINTER(DP), C.POINTER(DP)]
         L.oracle_apply_bound_cond.argtypes = [V, C.c_double, C.POINTER(DP)]
         L.oracle_rk4.argtypes = [V, C.c_double, C.c_double, C.POINTER(DP), C.POINTER(DP)]
+        L.oracle_euler.argtypes = [V, C.c_double, C.c_double, C.POINTER(DP), C.POINTER(DP)]
+        L.oracle_ssprk3.argtypes = [V, C.c_double, C.c_double, C.POINTER(DP), C.POINTER(DP)]
+        L.oracle_time_step.argtypes = [V, C.c_double, C.c_double, C.POINTER(DP), C.POINTER(DP)]
         L.oracle_init_mean.argtypes = [V, C.POINTER(DP)]
         L.oracle_get_mean.argtypes = [V, C.POINTER(DP)]
         L.oracle_sfd.argtypes = [V, C.c_double, C.POINTER(DP)]
@@ -105,6 +108,26 @@ class Oracle:
         if self.case.nz == 1:
             out["W"] = st["W"]
         return out
+
+    def _integrate(self, fn, q, t, dt):
+        st = self._state(q)
+        out = {k: np.zeros(self.shape, order="F") for k in "UVWRE"}
+        fn(self.ctx, t, dt, self._ptrs(st), self._ptrs(out))
+        if self.case.nz == 1:
+            out["W"] = st["W"]
+        return out
+
+    def euler(self, q, t, dt):
+        """EULER_2D / EULER_3D (timeIntegration.F90:158-222, 500-566)."""
+        return self._integrate(self.lib.oracle_euler, q, t, dt)
+
+    def ssprk3(self, q, t, dt):
+        """SSPRK3_2D / SSPRK3_3D (timeIntegration.F90:225-335, 569-685)."""
+        return self._integrate(self.lib.oracle_ssprk3, q, t, dt)
+
+    def time_step(self, q, t, dt):
+        """The integrator the case's time_stepping selects (main.F90:255-350)."""
+        return self._integrate(self.lib.oracle_time_step, q, t, dt)
 
     def init_mean(self, q):
         st = self._state(q)

@@ -183,3 +183,17 @@ def test_checkpointed_propagator_dot_product_and_gradient(torch_cuda, kind):
     b = float((Jv * Jv).sum())
     assert abs(a - b) <= 1e-11 * abs(b), (a, b)
     prop.close()
+
+
+@pytest.mark.parametrize("stepping", [2, 3])
+def test_linearised_steps_need_rk4(torch_cuda, stepping):
+    """The tangent and adjoint steps are RK4's; other integrators are refused with an error."""
+    from gatt_dns_amd import runtime
+    c, q = _case("box3d")
+    c.time_stepping = stepping
+    s = runtime.Solver(c)
+    s.set_state(q)
+    with pytest.raises(runtime.GattError, match="RK4"):
+        s.tlm_step(0.0, c.dt)
+    with pytest.raises(runtime.GattError, match="RK4"):
+        s.adj_step(0.0, c.dt)

@@ -217,6 +217,45 @@ def test_rk4_stage_boundaries(rt, oracle_mod):
         assert rel_err(got[k], want[k]) <= 1e-12, (k, rel_err(got[k], want[k]))
 
 
+@pytest.mark.parametrize("name", ["cavity33", "box32"])
+@pytest.mark.parametrize("method", ["euler", "ssprk3"])
+def test_other_integrators(rt, oracle_mod, name, method):
+    """EULER_2D/3D and SSPRK3_2D/3D with their stage boundary conditions."""
+    case = CASES[name]()
+    print("integrator", method, name, flush=True)
+    q = perturbed_state(case, 5)
+    o = oracle_mod.Oracle(case)
+    q = o.apply_bound_cond(q, 0.0)
+    want = getattr(o, method)(q, 0.0, case.dt)
+    s = rt.Solver(case)
+    s.set_state(q)
+    getattr(s, method)(0.0, case.dt)
+    got = s.get_state()
+    for k in fields_of(case):
+        assert rel_err(got[k], want[k]) <= 1e-12, (name, method, k, rel_err(got[k], want[k]))
+
+
+@pytest.mark.parametrize("name", ["cavity33", "box32"])
+@pytest.mark.parametrize("stepping", [2, 3])
+def test_time_stepping_advance(rt, oracle_mod, name, stepping):
+    """Main-loop steps (integrator, filter, SFD, BCs) with timeStepping 2 (Euler) and 3 (SSPRK3)."""
+    case = CASES[name]()
+    case.time_stepping = stepping
+    print("advance timeStepping", stepping, name, flush=True)
+    o = oracle_mod.Oracle(case)
+    q0 = o.apply_bound_cond(case.state0, 0.0)
+    want, mc_w, dv_w = o.run(q0, 0.0, case.dt, 5)
+    s = rt.Solver(case)
+    s.set_state(case.state0)
+    s.apply_bound_cond(0.0)
+    mc, dv = s.advance(5, 0.0, case.dt)
+    got = s.get_state()
+    assert dv == dv_w
+    for k in fields_of(case):
+        assert rel_err(got[k], want[k]) <= 1e-11, (name, stepping, k, rel_err(got[k], want[k]))
+    assert np.allclose(mc, mc_w, rtol=1e-8, atol=0)
+
+
 def test_cavity_1000_steps(rt, oracle_mod):
     """1000 fixed-dt steps of the 129^2 lid-driven cavity, Re 1000, Ma 0.5."""
     case = CASES["cavity129"]()

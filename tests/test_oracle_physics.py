@@ -5,7 +5,7 @@
   evaluated symbolically (sympy) for smooth fields; the discrete RHS must
   converge to it at the order of the compact scheme (4 on periodic grids).
 * RK4: the step error of the full RK4 with stage boundary conditions falls by
-  2^4 when dt halves.
+  2^4 when dt halves; EULER by 2^1 and SSPRK3 by 2^3.
 * applyBoundCond: the values it imposes satisfy the conditions they encode
   (a one-sided 6th-order derivative of the pressure vanishes at the walls;
   Dirichlet nodes carry their values; the lid profile).
@@ -125,6 +125,44 @@ def test_rk4_fourth_order(oracle_mod):
     d2 = np.max(np.abs(results[1]["U"] - results[2]["U"]))
     d3 = np.max(np.abs(results[2]["U"] - results[3]["U"]))
     assert 12 < d1 / d2 < 20 and 12 < d2 / d3 < 20, (d1, d2, d3)
+
+
+@pytest.mark.parametrize("method,order", [("euler", 1), ("ssprk3", 3)])
+def test_other_integrators_order(oracle_mod, method, order):
+    # EULER and SSPRK3 over a fixed interval with 1, 2, 4, 8 substeps: successive
+    # differences fall by 2^order (a wrong stage weight or stage time drops the order)
+    case = cases.turbulence_box(16, forcing=False, urms=0.05, Re=100.0)
+    o = oracle_mod.Oracle(case)
+    step = getattr(o, method)
+    T = case.dt / 4
+    results = []
+    for nsub in (1, 2, 4, 8):
+        q = {k: np.array(v, order="F") for k, v in case.state0.items()}
+        h = T / nsub
+        for s in range(nsub):
+            q = step(q, s * h, h)
+        results.append(q)
+    lo, hi = 2 ** order * 0.85, 2 ** order * 1.15
+    for k in "UVWRE":
+        d = [np.max(np.abs(results[i][k] - results[i + 1][k])) for i in range(3)]
+        assert lo < d[0] / d[1] < hi and lo < d[1] / d[2] < hi, (k, d)
+
+
+def test_time_stepping_selects_integrator(oracle_mod):
+    """timeStepping 1/2/3 runs RK4/Euler/SSPRK3 inside the main-loop step (main.F90:255-350)."""
+    case = cases.lid_driven(17)
+    q = case.state0
+    for ts, method in ((1, "rk4"), (2, "euler"), (3, "ssprk3")):
+        case.time_stepping = ts
+        o = oracle_mod.Oracle(case)
+        a = o.time_step(q, 0.0, case.dt)
+        b = getattr(o, method)(q, 0.0, case.dt)
+        for k in "UVRE":
+            assert np.array_equal(a[k], b[k]), (ts, k)
+    # the three integrators differ after one step
+    o = oracle_mod.Oracle(case)
+    r, e, s3 = o.rk4(q, 0.0, case.dt), o.euler(q, 0.0, case.dt), o.ssprk3(q, 0.0, case.dt)
+    assert not np.array_equal(r["U"], e["U"]) and not np.array_equal(r["U"], s3["U"])
 
 
 def test_cavity_boundary_conditions(oracle_mod):
