@@ -24,11 +24,11 @@ class BoxList(C.Structure):
 
 
 class Disturbance(C.Structure):
-    _fields_ = [("kind", I32), ("box", I32 * 6), ("x", DP), ("par", C.c_double * 4)]
+    _fields_ = [("kind", I32), ("box", I32 * 6), ("x", DP), ("par", C.c_double * 4), ("var", I32)]
 
 
 class Forcing(C.Structure):
-    _fields_ = [("kind", I32), ("box", I32 * 6), ("par", C.c_double * 4)]
+    _fields_ = [("kind", I32), ("box", I32 * 6), ("par", C.c_double * 4), ("var", I32 * 2)]
 
 
 class BCDesc(C.Structure):
@@ -53,8 +53,9 @@ class ProblemDesc(C.Structure):
 GATT_U, GATT_V, GATT_W, GATT_R, GATT_E = range(5)
 GATT_DERIV, GATT_FILTER = 0, 1
 BC_SLOTS = "UVWPE"
-DIST_KIND = {"lidDrivenMovement": 1, "holdInlet": 2}
-FORCING_KIND = {"turbulenceGenerator": 1}
+DIST_KIND = {"lidDrivenMovement": 1, "holdInlet": 2, "periodicSine": 3, "packet_2d": 4}
+FORCING_KIND = {"turbulenceGenerator": 1, "pressureGradient": 2}
+FIELD_INDEX = {"U": GATT_U, "V": GATT_V, "W": GATT_W, "R": GATT_R, "E": GATT_E}
 
 
 class Desc:
@@ -128,6 +129,8 @@ def build_desc(case, sfd_override=None):
         darr[i].x = d.dptr(ds.X)
         for c, v in enumerate(ds.par[:4]):
             darr[i].par[c] = v
+        if ds.var:
+            darr[i].var = FIELD_INDEX[ds.var[0].upper()]
     d.keep.append(darr)
     p.bc.nDisturb = nd
     p.bc.disturb = C.cast(darr, C.POINTER(Disturbance))
@@ -139,6 +142,8 @@ def build_desc(case, sfd_override=None):
             farr[i].box[c] = fc.box[c]
         for c, v in enumerate(fc.par[:4]):
             farr[i].par[c] = v
+        for c, v in enumerate(fc.var[:2]):
+            farr[i].var[c] = FIELD_INDEX[v.upper()]
     d.keep.append(farr)
     p.nForcing = nf
     p.forcing = C.cast(farr, C.POINTER(Forcing))

@@ -130,7 +130,52 @@ __global__ __launch_bounds__(256) void bc_wave_kernel(BcArgs a) {
       }
       break;
     }
+    case BC_SINE: {
+      // periodicSine.F90:19-21; aux = x(1:nx), alpha (single-precision value), omega
+      const double t = *a.clock + a.toff;
+      const double* x = a.aux + op.aux;
+      const double alpha = x[bi], omega = x[bi + 1];
+      double* F = a.q[op.field] + mo;
+      F[n] = F[n] + op.value * sin(alpha * (x[i - op.box[0]] - x[0])) * sin(omega * t);
+      break;
+    }
+    case BC_PACKET: {
+      // packet_2d.F90:27-39; aux = x(1:nx), x0, Tend, Tp, sigmaT, sigmaX (single-precision
+      // values), omega, nmodes; op.L = int(nmodes)
+      const double t = *a.clock + a.toff;
+      const double* x = a.aux + op.aux;
+      const double* k = x + bi;
+      const double x0 = k[0], Tend = k[1], Tp = k[2], sT = k[3], sX = k[4], omega = k[5], nmodes = k[6];
+      if (t > Tend) break;
+      double* F = a.q[op.field] + mo;
+      const double dT = t - Tp;
+      const double dX = x[i - op.box[0]] - x0;
+      double acc = F[n];
+      for (int m = 1; m <= op.L; ++m) {
+        const double th = (double)m * omega * dT;
+        const float re = (float)cos(th);   // aux is a single-precision complex
+        acc = acc + exp(-(dT * dT) / (2.0 * (sT * sT))) * exp(-(dX * dX) / (2.0 * (sX * sX))) * (double)re / nmodes;
+      }
+      F[n] = op.value * acc;
+      break;
+    }
   }
+}
+
+__global__ void clock_set_kernel(double* clock, double t) {
+  if (threadIdx.x == 0) clock[0] = t;
+}
+
+__global__ void clock_add_kernel(double* clock, double dt) {
+  if (threadIdx.x == 0) clock[0] = clock[0] + dt;
+}
+
+void launch_clock_set(double* clock, double t, hipStream_t st) {
+  hipLaunchKernelGGL(clock_set_kernel, dim3(1), dim3(64), 0, st, clock, t);
+}
+
+void launch_clock_add(double* clock, double dt, hipStream_t st) {
+  hipLaunchKernelGGL(clock_add_kernel, dim3(1), dim3(64), 0, st, clock, dt);
 }
 
 void launch_bc_wave(const BcArgs& a, long long nblocks, int nmembers, hipStream_t st) {

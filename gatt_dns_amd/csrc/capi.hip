@@ -93,6 +93,12 @@ struct gatt_ctx {
   int filterDir[3] = {1, 1, 1};
   double filterCharTime = -1;
   int stepping = 1;     // timeStepping: 1 RK4, 2 Euler, 3 SSPRK3
+  double* d_clock = nullptr;   // time of the step in progress (main.F90:424 accumulates it)
+  bool timed = false;          // the boundary program has time-dependent disturbances
+  // pressureGradient forcing (runForcings{2,3}D.F90)
+  int pg = 0, pg_r = 0, pg_u = 0;
+  int pg_box[6] = {0, 0, 0, 0, 0, 0};
+  double pg_dpdx = 0;
   // boundary program
   BcOp* d_ops = nullptr;
   std::vector<BcOp> ops;
@@ -515,6 +521,42 @@ int compile_bc(gatt_ctx* c, const gatt_bc_desc& bc) {
         st.rd.push_back(b);
         st.wr.push_back(b);
       }
+    } else if (ds.kind == GATT_DIST_PERIODIC_SINE || ds.kind == GATT_DIST_PACKET_2D) {
+      // periodicSine.F90 / packet_2d.F90 on the field passed as u.  Their pi, alpha, x0,
+      // Tend, Tp, sigmaT and sigmaX are default (single precision) reals: the values
+      // are rounded to float here, as the assignments in those routines round them.
+      if (ds.var < 0 || ds.var > 4) return fail(GATT_E_INVALID, "disturbance variable outside U..E");
+      if (!ds.x) return fail(GATT_E_INVALID, "disturbance without x coordinates");
+      const int nb = op.box[1] - op.box[0] + 1;
+      op.field = ds.var;
+      op.aux = (int)aux.size();
+      for (int q = 0; q < nb; ++q) aux.push_back(ds.x[q]);
+      const float pi = (float)(4.0 * std::atan(1.0));
+      const double xl = ds.x[nb - 1], xr = ds.x[0];
+      if (ds.kind == GATT_DIST_PERIODIC_SINE) {
+        op.kind = BC_SINE;
+        op.value = ds.par[1];
+        aux.push_back((double)(float)(2.0 * (double)pi / (xl - xr)));
+        aux.push_back(ds.par[0]);
+      } else {
+        op.kind = BC_PACKET;
+        const double omega = ds.par[0], nmodes = ds.par[1];
+        op.value = ds.par[2];
+        op.L = (int)nmodes;
+        const float Tend = (float)((2.0 * (double)pi / omega) * 2.5e-1);
+        aux.push_back((double)(float)((xl + xr) / 2));
+        aux.push_back((double)Tend);
+        aux.push_back((double)(float)((double)Tend * 5e-1));
+        aux.push_back((double)(float)((double)Tend * 2.5e-1));
+        aux.push_back((double)(float)((xl - xr) / 8.0));
+        aux.push_back(omega);
+        aux.push_back(nmodes);
+      }
+      Box w{ds.var, {}};
+      std::memcpy(w.b, op.box, sizeof(w.b));
+      st.wr.push_back(w);
+      st.rd.push_back(w);
+      c->timed = true;
     } else {
       return fail(GATT_E_UNSUPPORTED, "unknown disturbance kind");
     }
@@ -565,8 +607,8 @@ int compile_bc(gatt_ctx* c, const gatt_bc_desc& bc) {
 }
 
 // ------------------------------------------------------------- execution
-void run_bc(gatt_ctx* c, double* q0, double t) {
-  (void)t;
+// applyBoundCond at time *clock + toff (the clock holds the t of the step in progress)
+void run_bc(gatt_ctx* c, double* q0, double toff) {
   for (const BcWave& w : c->waves) {
     BcArgs a{};
     a.ops = c->d_ops;
@@ -584,6 +626,8 @@ void run_bc(gatt_ctx* c, double* q0, double t) {
     a.nz = c->nz;
     a.gamma = c->gamma;
     a.capture = c->hold_captured ? 0 : 1;
+    a.clock = c->d_clock;
+    a.toff = toff;
     launch_bc_wave(a, w.nblocks, c->batch, c->stream);
   }
   c->hold_captured = true;
@@ -619,6 +663,17 @@ void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double
   a.theta = theta;
   a.mode = mode;
   launch_line_solve(a, d.shape[kind], nmembers, st ? st : c->stream);
+}
+
+// the pressureGradient forcing of the assemble kernel
+void set_pressure_gradient(const gatt_ctx* c, AssembleArgs& aa) {
+  aa.pg = c->pg;
+  aa.pg_r = c->pg_r;
+  aa.pg_u = c->pg_u;
+  for (int q = 0; q < 6; ++q) aa.pg_box[q] = c->pg_box[q];
+  aa.pg_dpdx = c->pg_dpdx;
+  aa.nx = c->nx;
+  aa.ny = c->ny;
 }
 
 // Navier-Stokes RHS of the stage state S and one RK stage (nseq3d/nseq2d + RK4 update),
@@ -716,49 +771,50 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
     aa.stage = stage;
     aa.dt = dt;
     aa.is3d = c->is3d;
+    set_pressure_gradient(c, aa);
     launch_assemble(aa, nm, c->stream);
   }
 }
 
-// RK4_3D / RK4_2D: state `from` -> `to` (timeIntegration.F90:382-496)
-void run_rk4(gatt_ctx* c, double* from, double* to, double t, double dt) {
+// RK4_3D / RK4_2D: state `from` -> `to` (timeIntegration.F90:382-496); the stage
+// boundary conditions act at t + dt/2, t + dt/2, t + dt, t + dt with t on the clock.
+void run_rk4(gatt_ctx* c, double* from, double* to, double dt) {
   double* S = c->slab_ptr(SL_S);
   run_rhs_stage(c, from, from, S, 1, dt);
-  run_bc(c, S, t + dt / 2.0);
+  run_bc(c, S, dt / 2.0);
   run_rhs_stage(c, S, from, S, 2, dt);
-  run_bc(c, S, t + dt / 2.0);
+  run_bc(c, S, dt / 2.0);
   run_rhs_stage(c, S, from, S, 3, dt);
-  run_bc(c, S, t + dt);
+  run_bc(c, S, dt);
   run_rhs_stage(c, S, from, to, 4, dt);
-  run_bc(c, to, t + dt);
+  run_bc(c, to, dt);
 }
 
-// EULER_3D / EULER_2D (timeIntegration.F90:500-566, 158-222).  EULER_3D applies the
-// boundary conditions at t and EULER_2D at t + dt; the boundary program does not
-// depend on time, so one call serves both.
-void run_euler(gatt_ctx* c, double* from, double* to, double t, double dt) {
+// EULER_3D / EULER_2D (timeIntegration.F90:500-566, 158-222): EULER_3D applies the
+// boundary conditions at t, EULER_2D at t + dt.
+void run_euler(gatt_ctx* c, double* from, double* to, double dt) {
   run_rhs_stage(c, from, from, to, 5, dt);
-  run_bc(c, to, c->is3d ? t : t + dt);
+  run_bc(c, to, c->is3d ? 0.0 : dt);
 }
 
 // SSPRK3_3D / SSPRK3_2D (timeIntegration.F90:569-685, 225-335); the Delta slab holds
 // k1 and then k1 + k2.
-void run_ssprk3(gatt_ctx* c, double* from, double* to, double t, double dt) {
+void run_ssprk3(gatt_ctx* c, double* from, double* to, double dt) {
   double* S = c->slab_ptr(SL_S);
   run_rhs_stage(c, from, from, S, 6, dt);
-  run_bc(c, S, t + dt);
+  run_bc(c, S, dt);
   run_rhs_stage(c, S, from, S, 7, dt);
-  run_bc(c, S, t + dt / 2.0);
+  run_bc(c, S, dt / 2.0);
   run_rhs_stage(c, S, from, to, 8, dt);
-  run_bc(c, to, t + dt);
+  run_bc(c, to, dt);
 }
 
 // The integrator timeStepping selects (main.F90:255-350).
-void run_time_step(gatt_ctx* c, double* from, double* to, double t, double dt) {
+void run_time_step(gatt_ctx* c, double* from, double* to, double dt) {
   switch (c->stepping) {
-    case 2: run_euler(c, from, to, t, dt); break;
-    case 3: run_ssprk3(c, from, to, t, dt); break;
-    default: run_rk4(c, from, to, t, dt); break;
+    case 2: run_euler(c, from, to, dt); break;
+    case 3: run_ssprk3(c, from, to, dt); break;
+    default: run_rk4(c, from, to, dt); break;
   }
 }
 
@@ -797,15 +853,17 @@ void ensure_mean(gatt_ctx* c) {
   }
 }
 
-// main.F90:255-420 for all members: from state `cur` into the other buffer.
-void enqueue_step(gatt_ctx* c, int cur, double t, double dt) {
+// main.F90:255-424 for all members: from state `cur` into the other buffer, at the
+// time on the clock, which then advances by dt.
+void enqueue_step(gatt_ctx* c, int cur, double dt) {
   double* from = c->state(cur);
   double* to = c->state(1 - cur);
   launch_clear_u64(c->d_red, 8 * c->batch, c->stream);
-  run_time_step(c, from, to, t, dt);
+  run_time_step(c, from, to, dt);
   run_filters(c, to, dt / c->filterCharTime);
   run_sfd(c, to, dt);
-  run_bc(c, to, t + dt);
+  run_bc(c, to, dt);
+  launch_clock_add(c->d_clock, dt, c->stream);
   ReduceArgs r{};
   r.np = c->np;
   r.member_stride = c->slab;
@@ -909,7 +967,25 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
   }
   if ((rc = compile_bc(c.get(), d->bc))) return rc;
   for (int i = 0; i < d->nForcing; ++i) {
+    if (d->forcing[i].kind == GATT_FORCING_PRESSURE_GRADIENT) {
+      // pressureGradient.F90 (runForcings{2,3}D.F90)
+      const gatt_forcing& f = d->forcing[i];
+      if (c->pg) return fail(GATT_E_UNSUPPORTED, "at most one pressureGradient forcing");
+      if (f.var[0] < 0 || f.var[0] > 4 || f.var[1] < 0 || f.var[1] > 4)
+        return fail(GATT_E_INVALID, "pressureGradient variables outside U..E");
+      const int lim[3] = {d->nx, d->ny, d->nz};
+      for (int q = 0; q < 3; ++q)
+        if (f.box[2 * q] < 1 || f.box[2 * q + 1] > lim[q] || f.box[2 * q] > f.box[2 * q + 1])
+          return fail(GATT_E_INVALID, "pressureGradient box outside the grid");
+      c->pg = 1;
+      c->pg_r = f.var[0];
+      c->pg_u = f.var[1];
+      for (int q = 0; q < 6; ++q) c->pg_box[q] = f.box[q] - 1;
+      c->pg_dpdx = f.par[0];
+      continue;
+    }
     if (d->forcing[i].kind != GATT_FORCING_TURBULENCE) return fail(GATT_E_UNSUPPORTED, "unknown forcing kind");
+    if (d->nz == 1) return fail(GATT_E_UNSUPPORTED, "the turbulence forcing is three-dimensional");
     const int* b = d->forcing[i].box;
     if (b[0] != 1 || b[1] != d->nx || b[2] != 1 || b[3] != d->ny || b[4] != 1 || b[5] != d->nz)
       return fail(GATT_E_UNSUPPORTED, "turbulence forcing must cover the whole grid");
@@ -944,6 +1020,10 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
   HIPCHK(hipMalloc(&p, sizeof(int)));
   c->dev_allocs.push_back(p);
   c->d_flag = static_cast<int*>(p);
+  HIPCHK(hipMalloc(&p, sizeof(double)));
+  c->dev_allocs.push_back(p);
+  c->d_clock = static_cast<double*>(p);
+  launch_clock_set(c->d_clock, 0.0, c->stream);
   HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
   hipError_t e = hipMalloc(&p, sizeof(double) * (size_t)c->slab * batch);
   if (e != hipSuccess) return fail(GATT_E_NOMEM, "state allocation failed");
@@ -1026,7 +1106,8 @@ int gatt_get_mean(gatt_ctx* c, int32_t member, double* const* q) {
 
 int gatt_rk4(gatt_ctx* c, double t, double dt) {
   if (check_ctx(c)) return GATT_E_INVALID;
-  run_rk4(c, c->state(c->cur), c->state(1 - c->cur), t, dt);
+  launch_clock_set(c->d_clock, t, c->stream);
+  run_rk4(c, c->state(c->cur), c->state(1 - c->cur), dt);
   c->cur = 1 - c->cur;
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1035,7 +1116,8 @@ int gatt_rk4(gatt_ctx* c, double t, double dt) {
 
 int gatt_euler(gatt_ctx* c, double t, double dt) {
   if (check_ctx(c)) return GATT_E_INVALID;
-  run_euler(c, c->state(c->cur), c->state(1 - c->cur), t, dt);
+  launch_clock_set(c->d_clock, t, c->stream);
+  run_euler(c, c->state(c->cur), c->state(1 - c->cur), dt);
   c->cur = 1 - c->cur;
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1044,7 +1126,8 @@ int gatt_euler(gatt_ctx* c, double t, double dt) {
 
 int gatt_ssprk3(gatt_ctx* c, double t, double dt) {
   if (check_ctx(c)) return GATT_E_INVALID;
-  run_ssprk3(c, c->state(c->cur), c->state(1 - c->cur), t, dt);
+  launch_clock_set(c->d_clock, t, c->stream);
+  run_ssprk3(c, c->state(c->cur), c->state(1 - c->cur), dt);
   c->cur = 1 - c->cur;
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1061,7 +1144,8 @@ int gatt_apply_filters(gatt_ctx* c, double theta) {
 
 int gatt_apply_bound_cond(gatt_ctx* c, double t) {
   if (check_ctx(c)) return GATT_E_INVALID;
-  run_bc(c, c->state(c->cur), t);
+  launch_clock_set(c->d_clock, t, c->stream);
+  run_bc(c, c->state(c->cur), 0.0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return GATT_OK;
@@ -1113,7 +1197,8 @@ static int read_scalars(gatt_ctx* c, double* umax, double* maxchange, int32_t* d
 int gatt_step(gatt_ctx* c, double t, double dt, double* umax, double* maxchange, int32_t* diverged) {
   if (check_ctx(c)) return GATT_E_INVALID;
   ensure_mean(c);
-  enqueue_step(c, c->cur, t, dt);
+  launch_clock_set(c->d_clock, t, c->stream);
+  enqueue_step(c, c->cur, dt);
   c->cur = 1 - c->cur;
   HIPCHK(hipGetLastError());
   return read_scalars(c, umax, maxchange, diverged);
@@ -1125,7 +1210,8 @@ int gatt_advance(gatt_ctx* c, int32_t nsteps, double t0, double dt, double* maxc
   ensure_mean(c);
   // holdInlet captures the inflow at the first boundary application; a problem that holds its
   // inflow and has not had one gets it here (main.F90:112); otherwise the state is left as given
-  if (c->d_hold && !c->hold_captured) run_bc(c, c->state(c->cur), t0);
+  launch_clock_set(c->d_clock, t0, c->stream);
+  if (c->d_hold && !c->hold_captured) run_bc(c, c->state(c->cur), 0.0);
   int s = 0;
   if (c->graph && (c->graph_dt != dt || c->graph_cur != c->cur)) {
     hipGraphExecDestroy(c->graph);
@@ -1134,8 +1220,8 @@ int gatt_advance(gatt_ctx* c, int32_t nsteps, double t0, double dt, double* maxc
   if (nsteps >= 2 && !c->graph) {
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(c, c->cur, t0, dt);
-    enqueue_step(c, 1 - c->cur, t0 + dt, dt);
+    enqueue_step(c, c->cur, dt);
+    enqueue_step(c, 1 - c->cur, dt);
     HIPCHK(hipStreamEndCapture(c->stream, &g));
     HIPCHK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(g));
@@ -1144,7 +1230,7 @@ int gatt_advance(gatt_ctx* c, int32_t nsteps, double t0, double dt, double* maxc
   }
   for (; s + 2 <= nsteps; s += 2) HIPCHK(hipGraphLaunch(c->graph, c->stream));
   if (s < nsteps) {
-    enqueue_step(c, c->cur, t0 + s * dt, dt);
+    enqueue_step(c, c->cur, dt);
     c->cur = 1 - c->cur;
     if (c->graph) {
       hipGraphExecDestroy(c->graph);
@@ -1281,6 +1367,7 @@ int gatt_time_kernel(gatt_ctx* c, int32_t kernel, int32_t reps, double* avg_ms, 
       aa.c = c->fc; aa.np = np; aa.member_stride = c->slab; aa.work_stride = ws; aa.S = c->slab_ptr(SL_S); aa.Q = S;
       aa.Out = c->slab_ptr(SL_S); aa.Delta = c->slab_ptr(SL_DL); aa.D = D; aa.T = T; aa.G = G;
       aa.sums = nullptr; aa.stage = 2; aa.dt = 0.0; aa.is3d = c->is3d;
+      set_pressure_gradient(c, aa);
       launch_assemble(aa, nm, c->stream);
       launches = 1;
       // reads S(5) D(15) G(12) Q(5) Delta(5), writes Delta(5) Out(5); 2-D: 4, 8, 6, 4, 4, 4, 4
@@ -1537,7 +1624,7 @@ void group_drop_graph(gatt_group* g) {
 
 // One double step of every member, as parallel branches forked from the first
 // member's stream and joined back into it.
-int group_capture(gatt_group* g, const double* t0, const double* dt) {
+int group_capture(gatt_group* g, const double* dt) {
   gatt_ctx* c0 = g->ctx[0];
   hipGraph_t graph;
   HIPCHK(hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
@@ -1545,8 +1632,8 @@ int group_capture(gatt_group* g, const double* t0, const double* dt) {
   for (size_t m = 1; m < g->ctx.size(); ++m) HIPCHK(hipStreamWaitEvent(g->ctx[m]->stream, g->ev[0], 0));
   for (size_t m = 0; m < g->ctx.size(); ++m) {
     gatt_ctx* c = g->ctx[m];
-    enqueue_step(c, c->cur, t0[m], dt[m]);
-    enqueue_step(c, 1 - c->cur, t0[m] + dt[m], dt[m]);
+    enqueue_step(c, c->cur, dt[m]);
+    enqueue_step(c, 1 - c->cur, dt[m]);
   }
   for (size_t m = 1; m < g->ctx.size(); ++m) {
     HIPCHK(hipEventRecord(g->ev[m], g->ctx[m]->stream));
@@ -1598,6 +1685,7 @@ int gatt_group_advance(gatt_group* g, int32_t nsteps, const double* t0, const do
   if (!g || !t0 || !dt) return fail(GATT_E_INVALID, "null argument");
   if (nsteps < 0) return fail(GATT_E_INVALID, "nsteps < 0");
   const size_t n = g->ctx.size();
+  for (size_t m = 0; m < n; ++m) launch_clock_set(g->ctx[m]->d_clock, t0[m], g->ctx[m]->stream);
   // members of a group launch their direction solves on their own stream: the group
   // already overlaps whole members as parallel branches
   for (gatt_ctx* c : g->ctx) {
@@ -1624,8 +1712,8 @@ int gatt_group_advance(gatt_group* g, int32_t nsteps, const double* t0, const do
       if (nsteps >= 2 && !c->graph) {
         hipGraph_t gr;
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_step(c, c->cur, t0[m], dt[m]);
-        enqueue_step(c, 1 - c->cur, t0[m] + dt[m], dt[m]);
+        enqueue_step(c, c->cur, dt[m]);
+        enqueue_step(c, 1 - c->cur, dt[m]);
         HIPCHK(hipStreamEndCapture(c->stream, &gr));
         HIPCHK(hipGraphInstantiate(&c->graph, gr, nullptr, nullptr, 0));
         HIPCHK(hipGraphDestroy(gr));
@@ -1641,7 +1729,7 @@ int gatt_group_advance(gatt_group* g, int32_t nsteps, const double* t0, const do
       stale = g->graph_dt[m] != dt[m] || g->graph_cur[m] != g->ctx[m]->cur;
     if (stale) group_drop_graph(g);
     if (nsteps >= 2 && !g->graph) {
-      int rc = group_capture(g, t0, dt);
+      int rc = group_capture(g, dt);
       if (rc) return rc;
     }
     // the graph runs on the first member's stream: it follows work pending on every member's stream
@@ -1660,7 +1748,7 @@ int gatt_group_advance(gatt_group* g, int32_t nsteps, const double* t0, const do
   if (nsteps % 2) {
     for (size_t m = 0; m < n; ++m) {
       gatt_ctx* c = g->ctx[m];
-      enqueue_step(c, c->cur, t0[m] + (nsteps - 1) * dt[m], dt[m]);
+      enqueue_step(c, c->cur, dt[m]);
       c->cur = 1 - c->cur;
       if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
@@ -1735,6 +1823,8 @@ double* tape_ptr(gatt_ctx* c, int s, int wk) { return c->tape + ((long long)s * 
 int lin_supported(gatt_ctx* c) {
   if (c->forcing) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps do not cover the turbulence forcing");
   if (c->stepping != 1) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps follow the RK4 integrator only");
+  if (c->timed) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps do not cover time-dependent disturbances");
+  if (c->pg) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps do not cover the pressureGradient forcing");
   if (c->sfdType > 0 && c->sfdDelta > 0)
     return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps need a frozen SFD mean (Delta = inf)");
   const int ndir = c->is3d ? 3 : 2;
@@ -1827,6 +1917,7 @@ void forward_assemble(gatt_ctx* c, double* S, double* Q, double* Out, int stage,
   aa.stage = stage;
   aa.dt = dt;
   aa.is3d = c->is3d;
+  set_pressure_gradient(c, aa);
   launch_assemble(aa, nm, c->stream);
 }
 
@@ -2145,6 +2236,7 @@ void run_bc_adj(gatt_ctx* c, double* qpre, long long qs, double* lam) {
         a.nz = c->nz;
         a.gamma = c->gamma;
         a.capture = 0;
+        a.clock = c->d_clock;
         launch_bc_wave(a, pw.nblocks, c->batch, c->stream);
       }
       applied = wi;
@@ -2241,22 +2333,22 @@ void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
   base_dtg(c, q, sl, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
   forward_assemble(c, q, q, S, 1, dt, 0, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
   fields_axpby(c, P1, ls, S, sl, 1.0, 0.0);
-  run_bc(c, S, t + dt / 2.0);
+  run_bc(c, S, dt / 2.0);
   fields_axpby(c, B1, ls, S, sl, 1.0, 0.0);
   base_dtg(c, S, sl, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
   forward_assemble(c, S, q, S, 2, dt, 0, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
   fields_axpby(c, P2, ls, S, sl, 1.0, 0.0);
-  run_bc(c, S, t + dt / 2.0);
+  run_bc(c, S, dt / 2.0);
   fields_axpby(c, B2, ls, S, sl, 1.0, 0.0);
   base_dtg(c, S, sl, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
   forward_assemble(c, S, q, S, 3, dt, 0, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
   fields_axpby(c, P3, ls, S, sl, 1.0, 0.0);
-  run_bc(c, S, t + dt);
+  run_bc(c, S, dt);
   fields_axpby(c, B3, ls, S, sl, 1.0, 0.0);
   base_dtg(c, S, sl, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
   forward_assemble(c, S, q, qn, 4, dt, 0, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
   fields_axpby(c, P4, ls, qn, sl, 1.0, 0.0);
-  run_bc(c, qn, t + dt);
+  run_bc(c, qn, dt);
   run_filters(c, qn, dt / c->filterCharTime);
   run_sfd(c, qn, dt);
   fields_axpby(c, P5, ls, qn, sl, 1.0, 0.0);

@@ -200,6 +200,12 @@ struct AssembleArgs {
   int stage;            // 0 raw RHS into Out, 1..4 RK4, 5 Euler, 6..8 SSPRK3
   double dt;
   int is3d;
+  // pressureGradient forcing: d(pg_u) += pg_dpdx / pg_r on the 0-based box pg_box
+  int pg;
+  int pg_r, pg_u;
+  int pg_box[6];
+  double pg_dpdx;
+  int nx, ny;
 };
 void launch_assemble(const AssembleArgs& a, int nmembers, hipStream_t st);
 
@@ -364,7 +370,9 @@ enum BcKind : int {
   BC_NEU_P,          // R(plane) = (sum c_k (R E)(plane + s k)) / E(plane)
   BC_CORNER,         // corner averaging of extrapolations (E if adiabatic, then P)
   BC_LID,            // lidDrivenMovement on U
-  BC_HOLD            // holdInlet capture / restore
+  BC_HOLD,           // holdInlet capture / restore
+  BC_SINE,           // periodicSine on `field` (time dependent)
+  BC_PACKET          // packet_2d on `field` (time dependent)
 };
 
 struct BcOp {
@@ -400,8 +408,13 @@ struct BcArgs {
   int nx, ny, nz;
   double gamma;
   int capture;           // holdInlet: 1 on the first call
+  const double* clock;   // the time t of the step in progress (one double on the device)
+  double toff;           // the ops see t = *clock + toff
 };
 void launch_bc_wave(const BcArgs& a, long long nblocks, int nmembers, hipStream_t st);
+// the device clock: set to t, or advanced by dt as main.F90:424 does (t = t + dt)
+void launch_clock_set(double* clock, double t, hipStream_t st);
+void launch_clock_add(double* clock, double dt, hipStream_t st);
 
 struct BcCopyArgs {
   const BcOp* ops;

@@ -135,6 +135,19 @@ __global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs a) {
     d[2] = 0.0;
     d[4] = -(DEx * U + DEy * V) + (-(P * div) + txx * DUx + tyy * DVy + txy * (DVx + DUy) - htx - hty) / R;
   }
+  if (a.pg) {
+    // pressureGradient.F90:13 (du = du + dpdx/r) on its box, after the other forcings
+    const int i = (int)(n % a.nx);
+    const int j = (int)((n / a.nx) % a.ny);
+    const int k = (int)(n / ((long long)a.nx * a.ny));
+    if (i >= a.pg_box[0] && i <= a.pg_box[1] && j >= a.pg_box[2] && j <= a.pg_box[3] && k >= a.pg_box[4] &&
+        k <= a.pg_box[5]) {
+      const double r = S[(long long)a.pg_r * np + n];
+#pragma unroll
+      for (int f = 0; f < 5; ++f)
+        if (f == a.pg_u) d[f] = d[f] + a.pg_dpdx / r;
+    }
+  }
   const double* Q = a.Q + mo;
   double* O = a.Out + mo;
   double* Dl = a.Delta + mo;

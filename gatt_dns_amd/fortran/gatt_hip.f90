@@ -26,12 +26,14 @@ module gatt_hip
     integer(c_int32_t) :: box(6) = 0
     type(c_ptr) :: x = c_null_ptr
     real(c_double) :: par(4) = 0
+    integer(c_int32_t) :: var = 0
   end type
 
   type, bind(C) :: gatt_forcing
     integer(c_int32_t) :: kind = 0
     integer(c_int32_t) :: box(6) = 0
     real(c_double) :: par(4) = 0
+    integer(c_int32_t) :: var(2) = 0
   end type
 
   type, bind(C) :: gatt_bc_desc

@@ -230,6 +230,28 @@ def lid_driven_flow(nx, ny, nz, E0, P0, t_wall_relative=None):
     return fr, walls, corners, bc
 
 
+def poiseuille_flow(nx, ny, nz, E0, P0, lower_wall_velocity=0.0, upper_wall_velocity=0.0, t_wall_relative=None):
+    """BC list of ``source/boundaries/poiseuilleFlow.m`` (no cavities or roughness):
+    the first and last y rows are walls, with moving-wall U values (``:148-168``)."""
+    fr = np.ones((nx, ny, nz), dtype=bool)
+    fr[:, [0, -1], :] = False
+    fr, walls, corners, inside = find_walls_for_boundaries(fr)
+    bc = BCList()
+    for key, d in (("front", "xi"), ("back", "xf"), ("up", "yi"), ("down", "yf"), ("right", "zi"), ("left", "zf")):
+        for box in walls[key]:
+            bc.add("p", "neu", d, 0.0, box)
+    e_wall = E0 * t_wall_relative if t_wall_relative is not None else E0
+    for box in inside:
+        bc.add("p", "dir", "yi", P0, box)
+        bc.add("u", "dir", "yi", 0.0, box)
+        bc.add("v", "dir", "yi", 0.0, box)
+        bc.add("w", "dir", "yi", 0.0, box)
+        bc.add("e", "dir", "yi", e_wall, box)
+    bc.add("u", "dir", "yi", lower_wall_velocity, [1, nx, 1, 1, 1, nz])
+    bc.add("u", "dir", "yf", upper_wall_velocity, [1, nx, ny, ny, 1, nz])
+    return fr, walls, corners, bc
+
+
 def boundary_layer_isothermal(X, Y, Z, E0, P0, z_periodic=True, t_wall_relative=None):
     """BC list of ``source/boundaries/boundaryLayerIsothermal.m`` (no cavities or roughness).
 

@@ -9,13 +9,14 @@ the initial state.  Scalars and coefficient arrays are passed through
 Builders:
   * :func:`lid_driven`      -- ``examples/lidDriven.m`` on a uniform mesh
   * :func:`turbulence_box`  -- ``examples/turbulenceBox.m`` (periodic box + forcing)
+  * :func:`poiseuille`      -- ``examples/poiseuille.m`` (channel + pressureGradient forcing)
 """
 from dataclasses import dataclass, field
 
 import numpy as np
 
 from .boundaries import (BoundaryInfo, boundary_layer_isothermal, calc_sfd_region, disturbance_index, init_boundaries,
-                         lid_driven_flow, periodic_box)
+                         lid_driven_flow, periodic_box, poiseuille_flow)
 from .matrices import BufferSide, DirMesh, LineCoeffs, NumMethods, fortran_round, make_matrices, prepare_thomas
 
 FIELDS = ("U", "V", "W", "R", "E")
@@ -23,21 +24,25 @@ FIELDS = ("U", "V", "W", "R", "E")
 
 @dataclass
 class Disturbance:
-    """One entry of ``runDisturbances.F90`` (kind 'lidDrivenMovement' or 'holdInlet')."""
+    """One entry of ``runDisturbances.F90`` (kind 'lidDrivenMovement', 'holdInlet',
+    'periodicSine' or 'packet_2d'); ``var`` is the flow variable passed as ``u``."""
     kind: str
     box: list
     X: np.ndarray
     Y: np.ndarray
     Z: np.ndarray
     par: list
+    var: str = ""
 
 
 @dataclass
 class Forcing:
-    """One entry of ``runForcings3D.F90`` (kind 'turbulenceGenerator')."""
+    """One entry of ``runForcings{2,3}D.F90`` (kind 'turbulenceGenerator' or
+    'pressureGradient'); ``var`` names the variables passed ('RU' for pressureGradient)."""
     kind: str
     box: list
     par: list
+    var: str = ""
 
 
 @dataclass
@@ -236,6 +241,67 @@ def periodic_rest(nx, ny, nz=1, Re=1000.0, Ma=0.3, Pr=0.71, gamma=1.4, T0=300.0,
     st = {k: np.zeros(shape, order="F") for k in FIELDS}
     st["R"][:] = 1.0
     st["E"][:] = E0
+    case.state0 = st
+    case.dt = rnd(cfl / cfl_dt(st, case.Ma, case.dxmin, nz))
+    return case
+
+
+def tanh_mesh_both(xi, xf, n, par):
+    """``generateMesh.m:103-121``, type 'tanh', local 'b' (refined at both ends)."""
+    eta = np.tanh(np.linspace(-par, par, n))
+    eta = eta / eta[-1]
+    eta = (eta + 1) / 2
+    return (xf - xi) * eta + xi
+
+
+def poiseuille(nx=64, ny=48, nz=1, Re=2000.0, Ma=0.5, Pr=0.71, gamma=1.4, T0=300.0, L=4.0, H=1.0,
+               dpdx=None, alpha_f=0.49, spatial="SL6", cfl=0.5, profile=True, disturb=(), y_mesh="tanh"):
+    """Plane channel of ``examples/poiseuille.m``: x periodic (uniform, length L), y on
+    [-H/2, H/2] with the tanh mesh refined at both walls, z periodic in 3-D, and the
+    pressureGradient forcing ``dU += dpdx/R`` everywhere (``dpdx = 16/Re``).
+
+    The x mesh uses a fixed periodic domain size (the example's ``fixPeriodicDomainSize =
+    false`` mesh needs ``generateMesh``).  The initial state is the laminar parabola
+    ``U = dpdx Re / (2 mu) (H^2/4 - y^2)`` at rest temperature (``profile=True``) or
+    ``U = 1`` with no-slip walls (``profile=False``).  ``disturb`` adds entries of
+    ``runDisturbances.F90``: tuples ``(kind, x range, y range, var, par)``.
+    """
+    dpdx = 16.0 / Re if dpdx is None else dpdx
+    X = np.arange(nx) / nx * L
+    Y = tanh_mesh_both(-H / 2, H / 2, ny, 2.0) if y_mesh == "tanh" else np.linspace(-H / 2, H / 2, ny)
+    Z = np.arange(nz) / nz if nz > 1 else np.array([0.0])
+    meshes = [DirMesh(X, periodic=True, fix_periodic_domain_size=True, xf=L), DirMesh(Y),
+              DirMesh(Z, periodic=True, fix_periodic_domain_size=True, xf=1.0)]
+    E0 = 1.0 / ((gamma ** 2 - gamma) * Ma ** 2)
+    P0 = (gamma - 1) * E0
+    fr, walls, corners, bc = poiseuille_flow(nx, ny, nz, E0, P0)
+    nm = NumMethods(spatial_derivs=spatial, spatial_derivs_buffer="EX4", filter_strength=alpha_f,
+                    filter_borders="reducedOrder", filter_borders_start=(False, True, True),
+                    filter_borders_end=(False, False, True))
+    deriv, filt, types = _coeffs(meshes, fr, nm)
+    bi = init_boundaries(bc, corners, nm.neumann_order, nm.neumann2_order)
+    rnd = lambda v: float(fortran_round(v))
+    whole = disturbance_index(X, Y, Z, (-np.inf, np.inf), (-np.inf, np.inf), (-np.inf, np.inf))
+    forc = [Forcing("pressureGradient", whole, [rnd(dpdx)], var="RU")]
+    dist = []
+    for kind, xr, yr, var, par in disturb:
+        box = disturbance_index(X, Y, Z, xr, yr, (-np.inf, np.inf))
+        dist.append(Disturbance(kind, box, fortran_round(X[box[0] - 1:box[1]]), fortran_round(Y[box[2] - 1:box[3]]),
+                                fortran_round(Z[box[4] - 1:box[5]]), [rnd(v) for v in par], var=var))
+    case = Case("poiseuille_%dx%dx%d" % (nx, ny, nz), nx, ny, nz, X, Y, Z, rnd(Re), rnd(Ma), rnd(Pr), rnd(gamma),
+                rnd(T0), deriv, filt, types, (1, 1, 1), -1.0, bi, dist, forc, dxmin=_dxmin(X, Y, Z, nz))
+    case.flow_region = fr
+    shape = (nx, ny, nz)
+    st = {k: np.zeros(shape, order="F") for k in FIELDS}
+    st["R"][:] = 1.0
+    st["E"][:] = E0
+    if profile:
+        # mu = 1 at the reference temperature (T = 1 when E = E0)
+        u = dpdx * Re / 2.0 * (H * H / 4.0 - Y ** 2)
+        st["U"][:] = u[None, :, None]
+    else:
+        st["U"][:] = 1.0
+        st["U"][:, [0, -1], :] = 0.0
     case.state0 = st
     case.dt = rnd(cfl / cfl_dt(st, case.Ma, case.dxmin, nz))
     return case

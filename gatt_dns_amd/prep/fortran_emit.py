@@ -166,7 +166,8 @@ def write_includes(case, outdir, dt=None):
         _assign_int(b, "cAdiabatic", bi.cAdiabatic)
     open(os.path.join(outdir, "boundaryInfo.F90"), "w").write("\n".join(b) + "\n")
 
-    kinds = {"lidDrivenMovement": 1, "holdInlet": 2}
+    kinds = {"lidDrivenMovement": 1, "holdInlet": 2, "periodicSine": 3, "packet_2d": 4}
+    fields = {"U": 0, "V": 1, "W": 2, "R": 3, "E": 4}
     t = ["    nDist = %d" % len(case.disturb), "    allocate(dist(max(nDist,1)))"]
     nx_tot = sum(len(ds.X) for ds in case.disturb)
     t.append("    allocate(distX(max(%d,1)))" % nx_tot)
@@ -176,13 +177,17 @@ def write_includes(case, outdir, dt=None):
         t.append("    dist(%d)%%box = (/%s/)" % (i + 1, ",".join(str(v) for v in ds.box)))
         for k, v in enumerate(ds.par[:4]):
             t.append("    dist(%d)%%par(%d) = %s" % (i + 1, k + 1, _lit(v)))
+        if ds.var:
+            t.append("    dist(%d)%%var = %d" % (i + 1, fields[ds.var[0].upper()]))
         _assign_real(t, "distX(%d:%d)" % (off + 1, off + len(ds.X)), ds.X)
         t.append("    distOff(%d) = %d" % (i + 1, off + 1))
         off += len(ds.X)
     t.append("    nForc = %d" % len(case.forcing))
     t.append("    allocate(forc(max(nForc,1)))")
     for i, fc in enumerate(case.forcing):
-        t.append("    forc(%d)%%kind = 1" % (i + 1))
+        t.append("    forc(%d)%%kind = %d" % (i + 1, {"turbulenceGenerator": 1, "pressureGradient": 2}[fc.kind]))
+        for k, v in enumerate(fc.var[:2]):
+            t.append("    forc(%d)%%var(%d) = %d" % (i + 1, k + 1, fields[v.upper()]))
         t.append("    forc(%d)%%box = (/%s/)" % (i + 1, ",".join(str(v) for v in fc.box)))
         for k, v in enumerate(fc.par[:4]):
             t.append("    forc(%d)%%par(%d) = %s" % (i + 1, k + 1, _lit(v)))

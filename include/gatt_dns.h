@@ -82,8 +82,10 @@ enum { GATT_BC_U = 0, GATT_BC_V = 1, GATT_BC_W = 2, GATT_BC_P = 3, GATT_BC_E = 4
 
 /* Disturbance hooks called at the end of applyBoundCond (runDisturbances.F90). */
 enum {
-  GATT_DIST_LID_DRIVEN = 1,  /* source/disturbances/lidDrivenMovement.F90: par[0] = U0 */
-  GATT_DIST_HOLD_INLET = 2   /* source/Fortran/holdInlet.F90: par[0] = holdDensity flag */
+  GATT_DIST_LID_DRIVEN = 1,    /* source/disturbances/lidDrivenMovement.F90: par[0] = U0 (on U) */
+  GATT_DIST_HOLD_INLET = 2,    /* source/Fortran/holdInlet.F90: par[0] = holdDensity flag */
+  GATT_DIST_PERIODIC_SINE = 3, /* source/disturbances/periodicSine.F90: par = {omega, amplitude}, on var */
+  GATT_DIST_PACKET_2D = 4      /* source/disturbances/packet_2d.F90: par = {omega, nmodes, amplitude}, on var */
 };
 
 typedef struct gatt_disturbance {
@@ -91,17 +93,21 @@ typedef struct gatt_disturbance {
   int32_t box[6];
   const double* x;      /* X(box(1):box(2)) as written in runDisturbances.F90 */
   double par[4];
+  int32_t var;          /* field the routine receives as u (GATT_U..GATT_E); kinds 3 and 4 */
 } gatt_disturbance;
 
 /* Forcing hooks called at the end of nseq (runForcings{2,3}D.F90). */
 enum {
-  GATT_FORCING_TURBULENCE = 1  /* source/disturbances/turbulenceGenerator.F90: par = {epsilon, e0} */
+  GATT_FORCING_TURBULENCE = 1,        /* source/disturbances/turbulenceGenerator.F90: par = {epsilon, e0} */
+  GATT_FORCING_PRESSURE_GRADIENT = 2  /* source/disturbances/pressureGradient.F90: par[0] = dpdx;
+                                         var = {field passed as r, field passed as u} ('RU') */
 };
 
 typedef struct gatt_forcing {
   int32_t kind;
   int32_t box[6];
   double par[4];
+  int32_t var[2];
 } gatt_forcing;
 
 /* Everything applyBoundCond receives (source/Fortran/nseq.F90:327-350). */
@@ -204,6 +210,8 @@ int gatt_step(gatt_ctx* ctx, double t, double dt, double* umax, double* maxchang
 
 /* nsteps fixed-dt steps starting at time t0 (timeControl = 1), replayed from
  * a captured HIP graph; per-step scalars of the last step are returned.  The
+ * time advances on the device as t = t + dt after each step (main.F90:424), so
+ * time-dependent disturbances see the same times as the reference loop.  The
  * state is used as given, as after main.F90:112; only a problem that holds its
  * inflow (holdInlet) and has had no boundary application yet gets one first,
  * which captures the inflow. */

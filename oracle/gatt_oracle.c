@@ -178,10 +178,24 @@ void oracle_apply_filters(oracle_ctx* c, double theta, double* const* q) {
 /* Forcing: turbulenceGenerator.F90:1-44                                     */
 /* ------------------------------------------------------------------------- */
 
-static void forcing(const gatt_problem_desc* p, const double* const* q, double* const* dq) {
+static void forcing(const gatt_problem_desc* p, const double* const* q, double* const* dq, int is3d) {
   for (int fi = 0; fi < p->nForcing; ++fi) {
     const gatt_forcing* fc = &p->forcing[fi];
-    if (fc->kind != GATT_FORCING_TURBULENCE) continue;
+    if (fc->kind == GATT_FORCING_PRESSURE_GRADIENT) {
+      /* pressureGradient.F90:13: du = du + dpdx/r on the box */
+      const int* bx = fc->box;
+      const double dpdx = fc->par[0];
+      const double* r = q[fc->var[0]];
+      double* du = dq[fc->var[1]];
+      for (int k = bx[4]; k <= bx[5]; ++k)
+        for (int j = bx[2]; j <= bx[3]; ++j)
+          for (int i = bx[0]; i <= bx[1]; ++i) {
+            size_t n = IDX(p, i, j, k);
+            du[n] = du[n] + dpdx / r[n];
+          }
+      continue;
+    }
+    if (fc->kind != GATT_FORCING_TURBULENCE || !is3d) continue;
     const int* bx = fc->box;
     const double eps = fc->par[0], e0 = fc->par[1];
     const double* U = q[GATT_U]; const double* V = q[GATT_V]; const double* W = q[GATT_W];
@@ -283,7 +297,7 @@ void oracle_nseq(oracle_ctx* c, const double* const* q, double* const* dq) {
                tauxy[n] * (DVx[n] + DUy[n]) + tauxz[n] * (DWx[n] + DUz[n]) + tauyz[n] * (DWy[n] + DVz[n]) -
                htx[n] - hty[n] - htz[n]) / R[n];
     }
-    forcing(p, q, dq);
+    forcing(p, q, dq, 1);
   } else {
     double *DUx = w[0], *DUy = w[1], *DVx = w[2], *DVy = w[3], *DRx = w[4], *DRy = w[5];
     double *DEx = w[6], *DEy = w[7], *P = w[8], *DPx = w[9], *DPy = w[10], *mu = w[11];
@@ -328,6 +342,7 @@ void oracle_nseq(oracle_ctx* c, const double* const* q, double* const* dq) {
                hty[n]) / R[n];
     }
     if (dq[GATT_W]) memset(dq[GATT_W], 0, np * sizeof(double));
+    forcing(p, q, dq, 0); /* runForcings2D.F90 (nseq.F90:137) */
   }
 }
 
@@ -415,7 +430,6 @@ static double powi18(double x) { return __builtin_powi(x, 18); }
 
 static void disturbances(oracle_ctx* c, double t, double* const* q) {
   const gatt_problem_desc* p = c->p;
-  (void)t;
   for (int d = 0; d < p->bc.nDisturb; ++d) {
     const gatt_disturbance* ds = &p->bc.disturb[d];
     const int* bx = ds->box;
@@ -445,6 +459,49 @@ static void disturbances(oracle_ctx* c, double t, double* const* q) {
         size_t m = 0;
         BOXLOOP(bx, { for (int v = 0; v < nv; ++v) q[vars[v]][n] = c->hold_save[d][v * nb + m]; ++m; });
       }
+    } else if (ds->kind == GATT_DIST_PERIODIC_SINE) {
+      /* periodicSine.F90:13-22; pi and alpha are default (single precision) reals */
+      const int nxb = bx[1] - bx[0] + 1;
+      const double* x = ds->x;
+      const double omega = ds->par[0], amplitude = ds->par[1];
+      const float pi = (float)(4.0 * atan(1.0));
+      const float alpha = (float)(2.0 * (double)pi / (x[nxb - 1] - x[0]));
+      double* u = q[ds->var];
+      BOXLOOP(bx, {
+        int i = ii - bx[0];
+        u[n] = u[n] + amplitude * sin((double)alpha * (x[i] - x[0])) * sin(omega * t);
+      });
+    } else if (ds->kind == GATT_DIST_PACKET_2D) {
+      /* packet_2d.F90:15-39; pi, x0, Tp, Tend, sigmaT, sigmaX are single precision and
+       * aux is a single-precision complex.  Every point of an x station receives the
+       * same modes: u <- amplitude * (u + sum_m ...), and nothing once t > Tend. */
+      const int nxb = bx[1] - bx[0] + 1;
+      const double* x = ds->x;
+      const double omega = ds->par[0], nmodes = ds->par[1], amplitude = ds->par[2];
+      const int nmode = (int)nmodes;
+      const float pi = (float)(4.0 * atan(1.0));
+      const float x0 = (float)((x[nxb - 1] + x[0]) / 2);
+      const float Tend = (float)((2.0 * (double)pi / omega) * 2.5e-1);
+      const float Tp = (float)((double)Tend * 5e-1);
+      const float sigmaT = (float)((double)Tend * 2.5e-1);
+      const float sigmaX = (float)((x[nxb - 1] - x[0]) / 8.0);
+      if (t > (double)Tend) continue;
+      double* u = q[ds->var];
+      BOXLOOP(bx, {
+        int i = ii - bx[0];
+        double acc = u[n];
+        for (int m = 1; m <= nmode; ++m) {
+          const double th = (double)m * omega * (t - (double)Tp);
+          const float aux_re = (float)cos(th);
+          const double dT = t - (double)Tp;
+          const double dX = x[i] - (double)x0;
+          const double sT = (double)sigmaT;
+          const double sX = (double)sigmaX;
+          acc = acc + exp(-(dT * dT) / (2.0 * (sT * sT))) * exp(-(dX * dX) / (2.0 * (sX * sX))) * (double)aux_re /
+                          nmodes;
+        }
+        u[n] = amplitude * acc;
+      });
     }
   }
 }

@@ -157,8 +157,10 @@ class Oracle:
         mc = np.zeros(5)
         ptrs = self._ptrs(st)
         dv = 0
+        t = t0
         for s in range(nsteps):
-            dv = self.lib.oracle_step(self.ctx, t0 + s * dt, dt, ptrs, um.ctypes.data_as(DP), mc.ctypes.data_as(DP))
+            dv = self.lib.oracle_step(self.ctx, t, dt, ptrs, um.ctypes.data_as(DP), mc.ctypes.data_as(DP))
             if dv:
                 break
+            t = t + dt   # main.F90:424
         return st, mc, bool(dv)

@@ -58,7 +58,17 @@ CASES = {
     "cavity129": lambda: cases.lid_driven(129),
     "box32": lambda: cases.turbulence_box(32),
     "box48_noforce": lambda: cases.turbulence_box(48, forcing=False),
+    "channel2d": lambda: cases.poiseuille(nx=32, ny=24),
+    "channel3d": lambda: cases.poiseuille(nx=32, ny=24, nz=16),
 }
+
+
+def disturbed_channel(nz):
+    """Poiseuille channel with the pressureGradient forcing, periodicSine on the lower wall's V
+    and packet_2d on the upper wall's V, both active over the first steps."""
+    return cases.poiseuille(nx=32, ny=24, nz=nz, disturb=[
+        ("periodicSine", (0.5, 3.0), (-0.5, -0.5), "V", [50.0, 1e-3]),
+        ("packet_2d", (1.0, 3.5), (0.5, 0.5), "V", [50.0, 4.0, 1e-3])])
 
 
 @pytest.mark.parametrize("name", ["cavity33", "cavity129", "box32"])
@@ -113,7 +123,7 @@ def test_line_solve_all_shapes(rt, oracle_mod, name):
     assert tried > 10
 
 
-@pytest.mark.parametrize("name", ["cavity33", "box32"])
+@pytest.mark.parametrize("name", ["cavity33", "box32", "channel2d", "channel3d"])
 def test_rhs(rt, oracle_mod, name):
     case = CASES[name]()
     print("rhs", name, flush=True)
@@ -254,6 +264,51 @@ def test_time_stepping_advance(rt, oracle_mod, name, stepping):
     for k in fields_of(case):
         assert rel_err(got[k], want[k]) <= 1e-11, (name, stepping, k, rel_err(got[k], want[k]))
     assert np.allclose(mc, mc_w, rtol=1e-8, atol=0)
+
+
+@pytest.mark.parametrize("nz", [1, 16])
+@pytest.mark.parametrize("stepping", [1, 2, 3])
+def test_time_dependent_disturbances(rt, oracle_mod, nz, stepping):
+    """periodicSine and packet_2d see the time of every stage (t + dt/2, t + dt, and t for
+    EULER_3D) while the clock advances over graph-replayed steps."""
+    case = disturbed_channel(nz)
+    case.time_stepping = stepping
+    print("disturbed channel nz", nz, "timeStepping", stepping, flush=True)
+    t0 = 3 * case.dt
+    o = oracle_mod.Oracle(case)
+    q0 = o.apply_bound_cond(case.state0, t0)
+    want, mc_w, dv_w = o.run(q0, t0, case.dt, 6)
+    s = rt.Solver(case)
+    s.set_state(case.state0)
+    s.apply_bound_cond(t0)
+    mc, dv = s.advance(6, t0, case.dt)
+    got = s.get_state()
+    assert dv == dv_w
+    # V and W are three orders below U here (only the wall disturbances drive them), so the
+    # velocity components are measured against the velocity scale max |U|
+    vscale = np.max(np.abs(want["U"]))
+    for k in fields_of(case):
+        err = rel_err(got[k], want[k]) if k not in "VW" else np.max(np.abs(got[k] - want[k])) / vscale
+        assert err <= 1e-11, (nz, stepping, k, err)
+    # the disturbances changed the state: V is non-zero on both walls
+    assert np.max(np.abs(got["V"][:, 0, :])) > 1e-6 and np.max(np.abs(got["V"][:, -1, :])) > 1e-6
+
+
+def test_disturbed_advance_equals_steps(rt):
+    """Step-by-step calls with explicit times and the graph with the device clock agree bitwise."""
+    case = disturbed_channel(1)
+    a = rt.Solver(case)
+    b = rt.Solver(case)
+    a.set_state(case.state0)
+    b.set_state(case.state0)
+    t = 0.0
+    for n in range(5):
+        a.step(t, case.dt)
+        t = t + case.dt
+    b.advance(5, 0.0, case.dt)
+    ga, gb = a.get_state(), b.get_state()
+    for k in FIELDS2:
+        assert np.array_equal(ga[k], gb[k]), k
 
 
 def test_cavity_1000_steps(rt, oracle_mod):
