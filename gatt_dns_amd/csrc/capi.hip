@@ -62,6 +62,7 @@ struct DirOps {
   LineShape shape[2];
   int* types = nullptr;      // per line, 0-based
   bool uniform_tiles = false;  // every tile of 64 consecutive lines has one type
+  bool identity[2] = {false, false};  // the operator is the identity (no line solve)
   int n1 = 0, nlines = 0;
   long long stride = 0;
 };
@@ -142,6 +143,8 @@ struct gatt_ctx {
   long long nflow = 0;                  // flow-region points (0: not set, every point)
   int* d_idx = nullptr;                 // region index -> point
   int* d_rmap = nullptr;                // point -> region index or -1
+  bool single_beta = false;             // singleBeta packing of the Krylov vectors
+  double* d_zw = nullptr;               // its z weights: cos, sin, 2/nz cos, 2/nz sin
   double* d_base = nullptr;             // base flow, 5 arrays
   double* d_scratch = nullptr;          // block products: partial sums, H, M
   size_t scratch_doubles = 0;
@@ -637,6 +640,16 @@ void run_bc(gatt_ctx* c, double* q0, double toff) {
 void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double* const* out, double theta,
                int mode, int nmembers, long long in_stride, long long out_stride, hipStream_t st = nullptr) {
   DirOps& d = c->dir[dir];
+  if (d.identity[kind]) {
+    // the identity operator: a derivative or replacing filter copies, a blend
+    // f + theta (f - f) leaves the field as it is
+    if (mode == 1) return;
+    for (int m = 0; m < nmembers; ++m)
+      for (int f = 0; f < nf; ++f)
+        if (in[f] + m * in_stride != out[f] + m * out_stride)
+          launch_copy(out[f] + m * out_stride, in[f] + m * in_stride, c->np, st ? st : c->stream);
+    return;
+  }
   SolveArgs a{};
   a.op = d.op[kind];
   a.types = d.types;
@@ -960,6 +973,23 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
     for (size_t l = 0; l < tm.size(); ++l)
       if (tm[l] != tm[l - l % 64]) o.uniform_tiles = false;
     for (int kind = 0; kind < 2; ++kind) {
+      // a one-point stencil: makeMatrices.m:14-20 replaces the filter of a direction too
+      // short for its stencil by LHS = RHS = 1, the identity, which needs no solve
+      const gatt_lineop& h = kind == 0 ? d->deriv[k] : d->filter[k];
+      if (h.nR == 1) {
+        // R = 1, A = C = 0, B = 1, and for the periodic layout D = e_1 (first row of the
+        // inverse): derivs.F90:556-948 then returns F exactly
+        const long long nb = (long long)(h.n - (h.periodic ? 1 : 0)) * h.ntypes;
+        bool ident = true;
+        for (long long e = 0; ident && e < (long long)h.n * h.ntypes; ++e) ident = h.R[e] == 1.0;
+        for (long long e = 0; ident && e < nb; ++e) ident = h.B[e] == 1.0;
+        for (long long e = 0; ident && e < (long long)(h.n - 1) * h.ntypes; ++e) ident = h.A[e] == 0.0 && h.C[e] == 0.0;
+        for (long long e = 0; ident && h.periodic && e < (long long)h.n * h.ntypes; ++e)
+          ident = h.D[e] == (e % h.n == 0 ? 1.0 : 0.0);
+        if (!ident) return fail(GATT_E_UNSUPPORTED, "one-point operators other than the identity");
+        o.identity[kind] = true;
+        continue;
+      }
       o.shape[kind] = choose_shape(k, dims[k], o.op[kind].nR, o.nlines, c->max_lds, o.uniform_tiles);
       if (!line_shape_ok(o.shape[kind], dims[k], o.op[kind].nR, k, c->max_lds, o.uniform_tiles))
         return fail(GATT_E_UNSUPPORTED, "no line-solve shape fits this line length");
@@ -1271,6 +1301,7 @@ int gatt_set_line_shape(gatt_ctx* c, int32_t dir, int32_t kind, int32_t variant,
   if (dir < 0 || dir > 2 || (!c->is3d && dir == 2) || (kind != GATT_DERIV && kind != GATT_FILTER))
     return fail(GATT_E_INVALID, "direction or kind out of range");
   DirOps& d = c->dir[dir];
+  if (d.identity[kind]) return fail(GATT_E_UNSUPPORTED, "the identity operator has no line solve");
   const int N = d.op[kind].N;
   LineShape s;
   s.variant = variant;
@@ -1398,7 +1429,10 @@ int gatt_time_kernel(gatt_ctx* c, int32_t kernel, int32_t reps, double* avg_ms, 
 
 // ------------------------------------------------------------ stability analysis
 
-static long long flow_points(const gatt_ctx* c) { return c->nflow > 0 ? c->nflow : c->np; }
+static long long packed_points(const gatt_ctx* c) {
+  return c->single_beta ? (long long)c->nx * c->ny : c->np;
+}
+static long long flow_points(const gatt_ctx* c) { return c->nflow > 0 ? c->nflow : packed_points(c); }
 static int slots(const gatt_ctx* c) { return c->is3d ? 5 : 4; }
 
 static int ensure_scratch(gatt_ctx* c, size_t doubles) {
@@ -1426,19 +1460,20 @@ int gatt_set_flow_region(gatt_ctx* c, int64_t count, const int64_t* idx) {
   c->d_idx = nullptr;
   c->d_rmap = nullptr;
   c->nflow = 0;
-  if (!idx || count == c->np) {
+  const long long npk = packed_points(c);
+  if (!idx || count == npk) {
     if (idx)
       for (int64_t r = 0; r < count; ++r)
         if (idx[r] != r) goto general;
     return GATT_OK;
   }
 general:
-  if (count < 1 || count > c->np) return fail(GATT_E_INVALID, "flow region size out of range");
+  if (count < 1 || count > npk) return fail(GATT_E_INVALID, "flow region size out of range");
   if (c->np > 0x7fffffffLL) return fail(GATT_E_UNSUPPORTED, "flow regions need fewer than 2^31 points");
   {
-    std::vector<int> fwd((size_t)count), inv((size_t)c->np, -1);
+    std::vector<int> fwd((size_t)count), inv((size_t)npk, -1);
     for (int64_t r = 0; r < count; ++r) {
-      if (idx[r] < 0 || idx[r] >= c->np) return fail(GATT_E_INVALID, "flow region index outside the grid");
+      if (idx[r] < 0 || idx[r] >= npk) return fail(GATT_E_INVALID, "flow region index outside the grid");
       if (r > 0 && idx[r] <= idx[r - 1]) return fail(GATT_E_INVALID, "flow region indices must increase");
       fwd[(size_t)r] = (int)idx[r];
       inv[(size_t)idx[r]] = (int)r;
@@ -1452,6 +1487,30 @@ general:
     HIPCHK(hipMemcpy(c->d_rmap, inv.data(), sizeof(int) * inv.size(), hipMemcpyHostToDevice));
   }
   c->nflow = count;
+  return GATT_OK;
+}
+
+int gatt_set_single_beta(gatt_ctx* c, int32_t on) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  if (on && !c->is3d) return fail(GATT_E_INVALID, "the singleBeta packing needs a 3-D grid");
+  if ((bool)on == c->single_beta) return GATT_OK;
+  // the region refers to the other layout: back to every point
+  int rc = gatt_set_flow_region(c, 0, nullptr);
+  if (rc) return rc;
+  c->single_beta = on != 0;
+  if (c->single_beta && !c->d_zw) {
+    // Ztemp = linspace(0, 2 pi, nz+1) without its end (calcInstability.m:495-497, 566-568)
+    const int nz = c->nz;
+    std::vector<double> w(4 * (size_t)nz);
+    for (int k = 0; k < nz; ++k) {
+      const double z = k * (2.0 * M_PI) / nz;
+      w[k] = std::cos(z);
+      w[nz + k] = std::sin(z);
+      w[2 * nz + k] = std::cos(z) * 2 / nz;
+      w[3 * nz + k] = std::sin(z) * 2 / nz;
+    }
+    if ((rc = dev_upload(c, w.data(), w.size(), &c->d_zw))) return rc;
+  }
   return GATT_OK;
 }
 
@@ -1498,6 +1557,10 @@ int gatt_perturb(gatt_ctx* c, int32_t first, int32_t count, const double* V, int
   a.V = V;
   a.scale = scale;
   a.is3d = c->is3d;
+  a.single_beta = c->single_beta;
+  a.plane = (long long)c->nx * c->ny;
+  a.nz = c->nz;
+  a.zw = c->d_zw;
   launch_perturb(a, count, c->stream);
   if (c->sfdType > 0) c->mean_ready = true;
   HIPCHK(hipGetLastError());
@@ -1524,6 +1587,10 @@ int gatt_collect(gatt_ctx* c, int32_t first, int32_t count, int32_t minus_first,
   a.W = W;
   a.scale = scale;
   a.is3d = c->is3d;
+  a.single_beta = c->single_beta;
+  a.plane = (long long)c->nx * c->ny;
+  a.nz = c->nz;
+  a.zw = c->d_zw;
   launch_collect(a, count, c->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1825,6 +1892,9 @@ int lin_supported(gatt_ctx* c) {
   if (c->stepping != 1) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps follow the RK4 integrator only");
   if (c->timed) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps do not cover time-dependent disturbances");
   if (c->pg) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps do not cover the pressureGradient forcing");
+  for (int d = 0; d < (c->is3d ? 3 : 2); ++d)
+    if (c->dir[d].identity[0] || c->dir[d].identity[1])
+      return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps need a line operator in every direction");
   if (c->sfdType > 0 && c->sfdDelta > 0)
     return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps need a frozen SFD mean (Delta = inf)");
   const int ndir = c->is3d ? 3 : 2;

@@ -327,6 +327,12 @@ struct PerturbArgs {
   const double* V;      // Krylov vectors (column b for member b) or nullptr (state = base)
   double scale;
   int is3d;
+  // singleBeta packing: rmap covers the z = 0 plane (plane points), and the value of
+  // plane k is weighted by zw[k] = cos(z_k) (zw[nz + k] = sin(z_k) for W)
+  int single_beta;
+  long long plane;
+  int nz;
+  const double* zw;
 };
 void launch_perturb(const PerturbArgs& a, int count, hipStream_t st);
 
@@ -339,6 +345,12 @@ struct CollectArgs {
   double* W;
   double scale;
   int is3d;
+  // singleBeta packing: idx points into the z = 0 plane, and a field is projected as
+  // sum_k zw[2nz + k] q(plane k) (zw[3nz + k] for W), the weights cos(z_k) 2/nz, sin(z_k) 2/nz
+  int single_beta;
+  long long plane;
+  int nz;
+  const double* zw;
 };
 void launch_collect(const CollectArgs& a, int count, hipStream_t st);
 

@@ -12,14 +12,18 @@
 namespace gatt {
 
 // state(member) = base + scale * vec2flow(V[:, member]); the SFD mean of the
-// member, when given, is reset to the base (frozen mean = base flow).
+// member, when given, is reset to the base (frozen mean = base flow).  With the
+// singleBeta packing (calcInstability.m:557-579) the vector holds one z plane and
+// plane k of the disturbance is (scale v) cos(z_k), or (scale v) sin(z_k) for W.
 __global__ __launch_bounds__(256) void perturb_kernel(PerturbArgs a) {
   const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (p >= a.np) return;
   const int b = blockIdx.y;
   double* q = a.state + (long long)b * a.member_stride;
   double* mean = a.mean ? a.mean + (long long)b * a.member_stride : nullptr;
-  const int r = a.rmap ? a.rmap[p] : (int)p;
+  const long long pp = a.single_beta ? p % a.plane : p;
+  const int kz = a.single_beta ? (int)(p / a.plane) : 0;
+  const int r = a.rmap ? a.rmap[pp] : (int)pp;
   const double* v = a.V ? a.V + (long long)b * a.ldv : nullptr;
   int s = 0;
 #pragma unroll
@@ -28,11 +32,23 @@ __global__ __launch_bounds__(256) void perturb_kernel(PerturbArgs a) {
     const double base = a.base[o];
     double val = base;
     const bool active = a.is3d || f != 2;
-    if (active && v && r >= 0) val = base + a.scale * v[(long long)s * a.nflow + r];
+    if (active && v && r >= 0) {
+      const double d = a.scale * v[(long long)s * a.nflow + r];
+      val = a.single_beta ? base + d * a.zw[(f == 2 ? a.nz : 0) + kz] : base + d;
+    }
     if (active) ++s;
     q[o] = val;
     if (mean) mean[o] = base;
   }
+}
+
+// singleBeta projection of one field at plane point pp (flow2vec, calcInstability.m:493-509):
+// sum over the z planes in order of weight * value
+__device__ __forceinline__ double sb_project(const double* q, long long pp, long long plane, int nz,
+                                             const double* w) {
+  double acc = w[0] * q[pp];
+  for (int k = 1; k < nz; ++k) acc = acc + w[k] * q[pp + k * plane];
+  return acc;
 }
 
 // W[:, b] = scale * (flow2vec(member first+b) - flow2vec(member minus+b) or ref or 0)
@@ -50,9 +66,17 @@ __global__ __launch_bounds__(256) void collect_kernel(CollectArgs a) {
     if (!a.is3d && f == 2) continue;
     const long long o = (long long)f * a.np + p;
     const long long vo = (long long)s * a.nflow + r;
-    double d = q[o];
-    if (m) d = d - m[o];
-    else if (a.ref) d = d - a.ref[vo];
+    double d;
+    if (a.single_beta) {
+      const double* zw = a.zw + (f == 2 ? 3 : 2) * a.nz;
+      d = sb_project(q + (long long)f * a.np, p, a.plane, a.nz, zw);
+      if (m) d = d - sb_project(m + (long long)f * a.np, p, a.plane, a.nz, zw);
+      else if (a.ref) d = d - a.ref[vo];
+    } else {
+      d = q[o];
+      if (m) d = d - m[o];
+      else if (a.ref) d = d - a.ref[vo];
+    }
     w[vo] = a.scale * d;
     ++s;
   }

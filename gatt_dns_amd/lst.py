@@ -29,16 +29,19 @@ from .runtime import GattError, Solver
 
 
 # ----------------------------------------------------------------- packing
-def flow_region_indices(case, remove_buffer=False, buffers=None):
+def flow_region_indices(case, remove_buffer=False, buffers=None, single_beta=False):
     """0-based column-major indices of the flow-region points (calcInstability.m:276-281).
 
     ``case.flow_region`` is True at flow points (the reference takes
     ``~isnan(baseflow.U)``, i.e. everything outside walls).  With
     ``remove_buffer`` the ``buffers`` = ((xi, xf), (yi, yf), (zi, zf)) point
-    counts at each end are removed as well.
+    counts at each end are removed as well.  With ``single_beta`` the indices are
+    those of the z = 0 plane (``flowRegion(:,:,1)``, calcInstability.m:505, 555).
     """
     fr = np.ones((case.nx, case.ny, case.nz), dtype=bool) if case.flow_region is None else \
         np.array(case.flow_region, dtype=bool).reshape((case.nx, case.ny, case.nz))
+    if single_beta:
+        fr = fr[:, :, :1]
     if remove_buffer and buffers:
         fr = fr.copy()
         for d, (bi, bf) in enumerate(buffers):
@@ -52,8 +55,9 @@ def flow_region_indices(case, remove_buffer=False, buffers=None):
     return np.flatnonzero(fr.ravel(order="F")).astype(np.int64)
 
 
-def initial_disturbance(case, region):
-    """Gaussian bump at the finest mesh point in every field, normalised (calcInstability.m:691-741)."""
+def initial_disturbance(case, region, single_beta=False):
+    """Gaussian bump at the finest mesh point in every field, normalised (calcInstability.m:691-741);
+    with ``single_beta`` the bump lives in the x-y plane and fills all five slots (:726-737)."""
     X, Y, Z = (np.asarray(case.X, float), np.asarray(case.Y, float), np.asarray(case.Z, float))
 
     def centre(a):
@@ -62,6 +66,10 @@ def initial_disturbance(case, region):
     ax = 10.0 / (X[-1] - X[0]) ** 2
     ay = 10.0 / (Y[-1] - Y[0]) ** 2
     arg = ax * (X[:, None, None] - centre(X)) ** 2 + ay * (Y[None, :, None] - centre(Y)) ** 2
+    if single_beta:
+        eta = np.exp(-arg).ravel(order="F")[region]
+        z = np.concatenate([eta] * 5)
+        return z / np.linalg.norm(z)
     if case.nz > 1:
         az = 10.0 / (Z[-1] - Z[0]) ** 2
         arg = arg + az * (Z[None, None, :] - centre(Z)) ** 2
@@ -71,10 +79,14 @@ def initial_disturbance(case, region):
     return z / np.linalg.norm(z)
 
 
-def vec2flow(case, region, vec):
-    """Host-side vec2flow (calcInstability.m:517-582): fields U..E with zeros outside the region."""
-    shape = (case.nx, case.ny, case.nz)
-    np_ = case.nx * case.ny * case.nz
+def vec2flow(case, region, vec, single_beta=False):
+    """Host-side vec2flow (calcInstability.m:517-582): fields U..E with zeros outside the region.
+
+    With ``single_beta`` the result is the x-y plane amplitude (nx, ny, 1) of the
+    five fields, before the cos/sin z factors (the layout vec2modes writes).
+    """
+    shape = (case.nx, case.ny, 1 if single_beta else case.nz)
+    np_ = shape[0] * shape[1] * shape[2]
     slots = "UVWRE" if case.nz > 1 else "UVRE"
     nf = len(region)
     out = {}
@@ -251,18 +263,23 @@ class Stability:
     zeta (calcInstability.m:345).
     """
 
-    def __init__(self, case, base, T, block=32, epsilon=1e-7, first_order=True, region=None, tune=False):
+    def __init__(self, case, base, T, block=32, epsilon=1e-7, first_order=True, region=None, tune=False,
+                 single_beta=False):
         self.case = case
+        self.single_beta = bool(single_beta)
         self.T = float(T)
         self.nsteps = max(1, int(math.ceil(self.T / case.dt - 1e-9)))
         self.dt = self.T / self.nsteps
         self.block = block
         self.first_order = first_order
-        self.region = flow_region_indices(case) if region is None else np.asarray(region, dtype=np.int64)
+        self.region = flow_region_indices(case, single_beta=self.single_beta) if region is None else \
+            np.asarray(region, dtype=np.int64)
         self.solver = Solver(case, batch=block if first_order else 2 * block)
         if tune:
             from .autotune import tune as _tune
             _tune(self.solver)
+        if self.single_beta:
+            self.solver.set_single_beta(True)
         self.solver.set_flow_region(self.region)
         self.solver.set_base(base)
         self.n = self.solver.vector_length()
@@ -310,7 +327,7 @@ class Stability:
 
 
 def calc_instability(case, base, T, krylov_dim=64, block=32, epsilon=1e-7, first_order=True, seed=20260,
-                     start="random", n_modes=0, region=None, log=None, tune=False):
+                     start="random", n_modes=0, region=None, log=None, tune=False, single_beta=False):
     """Leading eigenvalues (and optionally modes) of the time-T propagator about ``base``.
 
     start = "random": ``block`` seeded N(0,1) vectors (numpy PCG64 ``seed``);
@@ -318,15 +335,17 @@ def calc_instability(case, base, T, krylov_dim=64, block=32, epsilon=1e-7, first
     in the others.  Returns a dict with lambda, mu, H, the Krylov dimension, the
     propagator's step count and dt, and ``modes`` (list of field dicts of the
     real and imaginary parts of the first ``n_modes`` Ritz vectors,
-    vec2modes of calcInstability.m:437-446).
+    vec2modes of calcInstability.m:437-446).  ``single_beta`` selects the
+    singleBeta packing (one spanwise wavenumber, the z mesh one period of it;
+    the reference's default for 3-D bases, calcInstability.m:14).
     """
     st = Stability(case, base, T, block=block, epsilon=epsilon, first_order=first_order, region=region,
-                   tune=tune)
+                   tune=tune, single_beta=single_beta)
     try:
         rng = np.random.Generator(np.random.PCG64(seed))
         X0h = rng.standard_normal((block, st.n))
         if start == "bump":
-            X0h[0] = initial_disturbance(case, st.region)
+            X0h[0] = initial_disturbance(case, st.region, single_beta)
         X0 = st.be.empty(block, st.n)
         st.be.upload(X0h, X0)
         V, H, _ = block_arnoldi(st.apply, X0, krylov_dim, st.be, log=log)
@@ -341,8 +360,8 @@ def calc_instability(case, base, T, krylov_dim=64, block=32, epsilon=1e-7, first
             st.be.update(basis, np.ascontiguousarray(y[:, :k].real), 0.0, 1.0, re)
             st.be.update(basis, np.ascontiguousarray(y[:, :k].imag), 0.0, 1.0, im)
             reh, imh = st.be.download(re), st.be.download(im)
-            out["modes"] = [(vec2flow(case, st.region, reh[i]), vec2flow(case, st.region, imh[i]))
-                            for i in range(k)]
+            out["modes"] = [(vec2flow(case, st.region, reh[i], single_beta),
+                              vec2flow(case, st.region, imh[i], single_beta)) for i in range(k)]
         return out
     finally:
         st.close()

@@ -27,7 +27,7 @@ EXPORTS = ["gatt_last_error", "gatt_device_info", "gatt_create", "gatt_destroy",
            "gatt_perturb", "gatt_collect", "gatt_block_dot", "gatt_block_update", "gatt_group_create",
            "gatt_group_destroy", "gatt_group_advance", "gatt_set_perturbation", "gatt_get_perturbation",
            "gatt_copy_fields", "gatt_tlm_step", "gatt_adj_step", "gatt_line_solve_t", "gatt_euler",
-           "gatt_ssprk3"]
+           "gatt_ssprk3", "gatt_set_single_beta"]
 
 SLOT_STATE = 0
 SLOT_PERTURBATION = 1
@@ -70,6 +70,7 @@ def load_library(path=LIB_PATH):
         getattr(L, name).argtypes = [V, I, C.POINTER(DP)]
     L.gatt_rk4.argtypes = [V, C.c_double, C.c_double]
     L.gatt_euler.argtypes = [V, C.c_double, C.c_double]
+    L.gatt_set_single_beta.argtypes = [V, C.c_int32]
     L.gatt_ssprk3.argtypes = [V, C.c_double, C.c_double]
     L.gatt_apply_filters.argtypes = [V, C.c_double]
     L.gatt_apply_bound_cond.argtypes = [V, C.c_double]
@@ -229,6 +230,10 @@ class Solver:
             return
         idx = np.ascontiguousarray(idx, dtype=np.int64)
         _check(self.lib.gatt_set_flow_region(self.ctx, len(idx), idx.ctypes.data_as(C.POINTER(C.c_int64))))
+
+    def set_single_beta(self, on=True):
+        """singleBeta packing of the Krylov vectors (resets the flow region)."""
+        _check(self.lib.gatt_set_single_beta(self.ctx, 1 if on else 0))
 
     def vector_length(self):
         n = C.c_int64(0)

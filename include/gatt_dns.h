@@ -285,6 +285,15 @@ int gatt_time_kernel(gatt_ctx* ctx, int32_t kernel, int32_t reps, double* avg_ms
  * column-major point indices.  idx == NULL selects every point (the default). */
 int gatt_set_flow_region(gatt_ctx* ctx, int64_t count, const int64_t* idx);
 
+/* singleBeta packing (calcInstability.m:14, 493-509, 557-579) on a 3-D grid
+ * (on = 1) or the full 3-D packing (on = 0, the default).  A singleBeta vector
+ * holds U,V,W,R,E at the flow-region points of the z = 0 plane: perturbations
+ * are v cos(z_k) on plane k (v sin(z_k) for W), z_k = 2 pi k / nz, and
+ * collected states are projected as sum_k (2/nz) cos(z_k) q_k (sin for W).
+ * Switching resets the flow region; gatt_set_flow_region then takes indices
+ * of the z = 0 plane (0 <= idx < nx*ny). */
+int gatt_set_single_beta(gatt_ctx* ctx, int32_t on);
+
 /* Krylov vector length N (calcInstability.m:284-290): 4 or 5 times the
  * number of flow-region points. */
 int gatt_vector_length(gatt_ctx* ctx, int64_t* n);

@@ -4,7 +4,9 @@ TEST INFRASTRUCTURE ONLY: imported by ``tests/`` as the checker of
 ``gatt_dns_amd/lst.py``; the product never imports it.
 
 * :func:`flow2vec` / :func:`vec2flow` follow calcInstability.m:480-515 and
-  517-582 (2-D drops W; no singleBeta packing).
+  517-582 (2-D drops W); with ``single_beta`` the singleBeta branches
+  (:493-509, :557-579): one x-y plane of amplitudes, cos(z) for U,V,R,E and
+  sin(z) for W, projected with the weights (2/nz) cos(z_k), (2/nz) sin(z_k).
 * :class:`OraclePropagator` is the reference's per-vector DNS launch
   (calcInstability.m:323-388): the perturbed flow gets the boundary
   conditions of main.F90:112 and is advanced nsteps fixed-dt steps by the
@@ -26,11 +28,36 @@ def slots(case):
     return "UVWRE" if case.nz > 1 else "UVRE"
 
 
-def flow2vec(case, region, q):
-    return np.concatenate([np.asarray(q[k], float).ravel(order="F")[region] for k in slots(case)])
+def _zplanes(nz):
+    return np.linspace(0.0, 2.0 * np.pi, nz + 1)[:-1]
 
 
-def vec2flow(case, region, vec):
+def flow2vec(case, region, q, single_beta=False):
+    if not single_beta:
+        return np.concatenate([np.asarray(q[k], float).ravel(order="F")[region] for k in slots(case)])
+    z = _zplanes(case.nz)
+    out = []
+    for k in "UVWRE":
+        w = (np.sin(z) if k == "W" else np.cos(z)) * 2 / case.nz
+        a = np.asarray(q[k], float).reshape((case.nx, case.ny, case.nz), order="F")
+        acc = w[0] * a[:, :, 0]
+        for kz in range(1, case.nz):      # sum(..., 3) in plane order
+            acc = acc + w[kz] * a[:, :, kz]
+        out.append(acc.ravel(order="F")[region])
+    return np.concatenate(out)
+
+
+def vec2flow(case, region, vec, single_beta=False):
+    if single_beta:
+        z = _zplanes(case.nz)
+        nf = len(region)
+        out = {}
+        for s_, k in enumerate("UVWRE"):
+            a = np.zeros(case.nx * case.ny)
+            a[region] = vec[s_ * nf:(s_ + 1) * nf]
+            w = np.sin(z) if k == "W" else np.cos(z)
+            out[k] = a.reshape((case.nx, case.ny, 1), order="F") * w[None, None, :]
+        return out
     shape = (case.nx, case.ny, case.nz)
     npts = case.nx * case.ny * case.nz
     nf = len(region)
@@ -45,17 +72,19 @@ def vec2flow(case, region, vec):
 
 
 class OraclePropagator:
-    def __init__(self, case, base, T, epsilon=1e-7, first_order=True, region=None):
+    def __init__(self, case, base, T, epsilon=1e-7, first_order=True, region=None, single_beta=False):
         self.case = case
+        self.sb = single_beta
         self.base = {k: np.asarray(base[k], float) for k in "UVWRE"}
         self.nsteps = max(1, int(math.ceil(T / case.dt - 1e-9)))
         self.dt = T / self.nsteps
-        self.region = np.flatnonzero(np.ones(case.nx * case.ny * case.nz, bool)) if region is None else region
+        npts = case.nx * case.ny * (1 if single_beta else case.nz)
+        self.region = np.flatnonzero(np.ones(npts, bool)) if region is None else region
         self.N = len(slots(case)) * len(self.region)
         self.eps = epsilon
         self.first_order = first_order
         self.oracle = Oracle(case)
-        self.UM = flow2vec(case, self.region, self.phi(self.base)) if first_order else None
+        self.UM = flow2vec(case, self.region, self.phi(self.base), single_beta) if first_order else None
 
     def phi(self, q):
         o = self.oracle
@@ -66,15 +95,15 @@ class OraclePropagator:
         return st
 
     def perturbed(self, z, sign=1.0):
-        d = vec2flow(self.case, self.region, z * (sign * self.eps * math.sqrt(self.N)))
+        d = vec2flow(self.case, self.region, z * (sign * self.eps * math.sqrt(self.N)), self.sb)
         return {k: self.base[k] + d[k] for k in "UVWRE"}
 
     def apply(self, z):
         s = self.eps * math.sqrt(self.N)
-        UP = flow2vec(self.case, self.region, self.phi(self.perturbed(z)))
+        UP = flow2vec(self.case, self.region, self.phi(self.perturbed(z)), self.sb)
         if self.first_order:
             return (UP - self.UM) / s
-        UMm = flow2vec(self.case, self.region, self.phi(self.perturbed(z, -1.0)))
+        UMm = flow2vec(self.case, self.region, self.phi(self.perturbed(z, -1.0)), self.sb)
         return (UP - UMm) / (2 * s)
 
 

@@ -93,14 +93,27 @@ def rk4_amplification(L, dt):
     return G
 
 
-def propagator_eigenvalues(case, nsteps, dt):
-    """All eigenvalues mu of the time-(nsteps*dt) propagator (one per mode and slot)."""
-    dims = [case.nx, case.ny] + ([case.nz] if case.nz > 1 else [])
+def spectral_z_symbol(nz):
+    """Symbol of the 4-point spectral z stencil (LHS 1, RHS [0, pi/4], makeMatrices.m:28-38) at
+    the first z harmonic on a z period of 1: i nz 2 (pi/4) sin(2 pi/nz) = 2 pi i for nz = 4."""
+    return 1j * nz * 2 * (np.pi / 4) * np.sin(2 * np.pi / nz)
+
+
+def propagator_eigenvalues(case, nsteps, dt, single_beta=False):
+    """All eigenvalues mu of the time-(nsteps*dt) propagator (one per mode and slot).
+
+    With ``single_beta`` the modes are the x-y modes times the first z harmonic
+    (U,V,R,E ~ cos, W ~ sin: the subspace the singleBeta packing spans); the z
+    filter of a 4-point z mesh is the identity.
+    """
+    dims = [case.nx, case.ny] + ([case.nz] if case.nz > 1 and not single_beta else [])
     dsym = [derivative_symbol(n) for n in dims]
     fsym = [filter_symbol(n) for n in dims]
     out = []
     for idx in itertools.product(*[range(n) for n in dims]):
         dvec = [dsym[k][i] for k, i in enumerate(idx)]
+        if single_beta:
+            dvec.append(spectral_z_symbol(case.nz))
         F = np.prod([fsym[k][i] for k, i in enumerate(idx)])
         A = F * rk4_amplification(mode_operator(dvec, case), dt)
         out.extend(np.linalg.eigvals(A) ** nsteps)

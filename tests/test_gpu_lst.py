@@ -167,3 +167,44 @@ def test_turbulence_box_driver_runs(torch_cuda):
     assert np.all(np.isfinite(out["lambda"]))
     assert out["members_advanced"] == 8 * 3       # base run + two blocks
     assert len(out["modes"]) == 2 and np.all(np.isfinite(out["modes"][0][0]["U"]))
+
+
+def test_single_beta_propagator_matches_oracle(torch_cuda):
+    """singleBeta packing (calcInstability.m:493-509, 557-579) on the 4-point spectral z mesh:
+    the batched GPU propagator against the oracle's."""
+    from gatt_dns_amd import lst
+    from oracle import lst as OL
+    case = cases.periodic_rest(12, 12, 4)
+    base = {k: np.array(v, copy=True) for k, v in case.state0.items()}
+    x = np.arange(12) / 12
+    base["U"] = base["U"] + 0.05 * np.sin(2 * np.pi * x)[:, None, None]
+    base["R"] = base["R"] + 0.02 * np.cos(2 * np.pi * x)[:, None, None]
+    T = 0.05
+    st = lst.Stability(case, base, T, block=4, epsilon=1e-5, first_order=False, single_beta=True)
+    P = OL.OraclePropagator(case, base, T, epsilon=1e-5, first_order=False, single_beta=True)
+    assert st.n == P.N == 5 * 144 and st.nsteps == P.nsteps
+    rng = np.random.Generator(np.random.PCG64(22))
+    V = rng.standard_normal((4, st.n))
+    V /= np.linalg.norm(V, axis=1)[:, None]
+    W = st.be.empty(4, st.n)
+    st.apply(_dev(torch_cuda, V), W)
+    Wg = st.be.download(W)
+    for b in range(4):
+        Wo = P.apply(V[b])
+        rel = np.linalg.norm(Wg[b] - Wo) / np.linalg.norm(Wo)
+        assert rel < 1e-7, (b, rel)
+    st.close()
+
+
+def test_single_beta_block_arnoldi_closed_form(torch_cuda):
+    """calc_instability with the singleBeta packing, Krylov dimension = N: every Ritz value is
+    an eigenvalue of the closed-form x-y modes times the first z harmonic."""
+    from gatt_dns_amd import lst
+    case = cases.periodic_rest(12, 12, 4)
+    T = 0.25
+    N = 5 * 144
+    out = lst.calc_instability(case, case.state0, T, krylov_dim=N, block=40, epsilon=1e-5, first_order=False,
+                               seed=4, single_beta=True)
+    mu_ref = FR.propagator_eigenvalues(case, out["nsteps"], out["dt"], single_beta=True)
+    d, r, c = FR.match(out["mu"], mu_ref)
+    assert d < 1e-6, d

@@ -311,6 +311,25 @@ def test_disturbed_advance_equals_steps(rt):
         assert np.array_equal(ga[k], gb[k]), k
 
 
+@pytest.mark.parametrize("name", ["rest16x16x4", "channel_nz8"])
+def test_short_z_steps(rt, oracle_mod, name):
+    """z directions too short for the filter stencil (identity filter, makeMatrices.m:14-20),
+    and the 4-point spectral z derivative (makeMatrices.m:28-38)."""
+    case = cases.periodic_rest(16, 16, 4) if name == "rest16x16x4" else cases.poiseuille(nx=32, ny=24, nz=8)
+    print("short z", name, flush=True)
+    q = perturbed_state(case, 7)
+    o = oracle_mod.Oracle(case)
+    q = o.apply_bound_cond(q, 0.0)
+    want, _, dv_w = o.run(q, 0.0, case.dt, 5)
+    s = rt.Solver(case)
+    s.set_state(q)
+    mc, dv = s.advance(5, 0.0, case.dt)
+    got = s.get_state()
+    assert dv == dv_w
+    for k in FIELDS3:
+        assert rel_err(got[k], want[k]) <= 1e-11, (name, k, rel_err(got[k], want[k]))
+
+
 def test_cavity_1000_steps(rt, oracle_mod):
     """1000 fixed-dt steps of the 129^2 lid-driven cavity, Re 1000, Ma 0.5."""
     case = CASES["cavity129"]()

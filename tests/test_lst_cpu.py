@@ -157,3 +157,68 @@ def test_block_arnoldi_breakdown_refill():
         assert np.abs(ritz - e).min() < 1e-10
     # the Arnoldi relation A V_m = V_{m+p} H holds through the refill
     np.testing.assert_allclose(V[:m] @ A.T, H.T @ B, atol=1e-10)
+
+
+def test_single_beta_packing_round_trip(oracle_mod):
+    """flow2vec(vec2flow(v)) = v for the singleBeta packing: (2/nz) sum_k cos^2(z_k) = 1 and
+    the same for sin, so the projection weights invert the plane-k factors."""
+    from oracle import lst as OL
+    case = cases.periodic_rest(12, 14, 4)
+    region = lst.flow_region_indices(case, single_beta=True)
+    assert len(region) == 12 * 14
+    rng = np.random.Generator(np.random.PCG64(13))
+    v = rng.standard_normal(5 * len(region))
+    q = OL.vec2flow(case, region, v, single_beta=True)
+    np.testing.assert_allclose(OL.flow2vec(case, region, q, single_beta=True), v, rtol=0, atol=1e-15)
+    # W carries sin(z): zero on the planes z = 0 and z = pi
+    assert np.max(np.abs(q["W"][:, :, 0])) == 0.0 and np.max(np.abs(q["W"][:, :, 2])) < 1e-15
+    plane = lst.vec2flow(case, region, v, single_beta=True)
+    for k in "UVWRE":
+        np.testing.assert_array_equal(plane[k][:, :, 0], q[k][:, :, 0] if k != "W" else q[k][:, :, 1])
+    z = lst.initial_disturbance(case, region, single_beta=True)
+    assert abs(np.linalg.norm(z) - 1) < 1e-14 and z.shape == v.shape
+
+
+def test_single_beta_subspace_is_invariant(oracle_mod):
+    """With the 4-point spectral z stencil (makeMatrices.m:28-38) the linearised step at rest
+    maps U,V,R,E ~ cos(z), W ~ sin(z) into itself: the propagated disturbance is rebuilt
+    from its singleBeta projection.  Pairing W with cos(z) instead would lose it."""
+    from oracle import lst as OL
+    case = cases.periodic_rest(12, 12, 4)
+    assert case.deriv[2].nR == 2 and case.filt[2].nR == 1     # spectral z derivative, identity filter
+    region = lst.flow_region_indices(case, single_beta=True)
+    P = OL.OraclePropagator(case, case.state0, 4 * case.dt, epsilon=1e-6, first_order=False, single_beta=True)
+    rng = np.random.Generator(np.random.PCG64(5))
+    v = rng.standard_normal(P.N)
+    v /= np.linalg.norm(v)
+    s = P.eps * np.sqrt(P.N)
+    qp, qm = P.phi(P.perturbed(v)), P.phi(P.perturbed(v, -1.0))
+    D = {k: (qp[k] - qm[k]) / (2 * s) for k in "UVWRE"}
+    rebuilt = OL.vec2flow(case, region, OL.flow2vec(case, region, D, single_beta=True), single_beta=True)
+    scale = max(np.max(np.abs(D[k])) for k in "UVWRE")
+    for k in "UVWRE":
+        assert np.max(np.abs(rebuilt[k] - D[k])) <= 1e-6 * scale, k
+    # the propagator did act: the result differs from the initial disturbance
+    assert np.linalg.norm(P.apply(v) - v) > 1e-3
+
+
+def test_oracle_single_beta_spectrum_closed_form(oracle_mod):
+    """The oracle's singleBeta propagator on a 12 x 12 x 4 box at rest (spectral z): every
+    eigenvalue of its finite-difference Jacobian is one of the x-y modes times the first z
+    harmonic in closed form."""
+    from oracle import lst as OL
+    case = cases.periodic_rest(12, 12, 4)
+    T = 0.05
+    P = OL.OraclePropagator(case, case.state0, T, epsilon=1e-5, first_order=False, single_beta=True)
+    N = P.N
+    assert N == 5 * 144
+    J = np.zeros((N, N))
+    e = np.zeros(N)
+    for i in range(N):
+        e[:] = 0
+        e[i] = 1
+        J[:, i] = P.apply(e)
+    mu = np.linalg.eigvals(J)
+    mu_ref = FR.propagator_eigenvalues(case, P.nsteps, P.dt, single_beta=True)
+    d, r, c = FR.match(mu, mu_ref)
+    assert d < 1e-6, d
