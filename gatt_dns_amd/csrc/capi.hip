@@ -59,6 +59,7 @@ struct DirOps {
   double* fcoef[2] = {nullptr, nullptr};      // fused per-point coefficients (line_solve_wsp.h)
   double* fcoef_ends[2] = {nullptr, nullptr}; // per type A(1) B(2nd row), C(1) B(last row)
   double* fcoef_t[2] = {nullptr, nullptr};    // fused records coefficient-major (x lane-group solves)
+  double* fcoef_p[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // lane-interleaved, M = 4, 8, 16
   LineShape shape[2];
   int* types = nullptr;      // per line, 0-based
   bool uniform_tiles = false;  // every tile of 64 consecutive lines has one type
@@ -280,7 +281,8 @@ int pack_coeffs(gatt_ctx* c, const gatt_lineop& h, double** coef, double** ends)
 // and its forward factor is zero, as is that of the first solved row; the last
 // row's backward factor is zero.  The end factors are A(1) and C(1) times the B
 // of the rows they correct (rows 2 and N).
-int pack_coeffs_fused(gatt_ctx* c, const gatt_lineop& h, double** coef, double** ends, double** coef_t) {
+int pack_coeffs_fused(gatt_ctx* c, const gatt_lineop& h, double** coef, double** ends, double** coef_t,
+                      double** coef_p) {
   const int N = h.n, nt = h.ntypes, NW = 2 * h.nR - 1, FS = (NW + 3 + 1) & ~1, per = h.periodic ? 1 : 0;
   const int NB = N - per;
   std::vector<double> pk((size_t)nt * N * FS, 0.0), en((size_t)nt * 2, 0.0);
@@ -311,6 +313,24 @@ int pack_coeffs_fused(gatt_ctx* c, const gatt_lineop& h, double** coef, double**
   int rc;
   if ((rc = dev_upload(c, pk.data(), pk.size(), coef))) return rc;
   if ((rc = dev_upload(c, tr.data(), tr.size(), coef_t))) return rc;
+  // lane-interleaved copies for one wavefront per line with M points per lane
+  // (M = 4, 8, 16): [type][k][tt][lane] holds point lane*M + tt, zero past the line,
+  // so that a wavefront reads each coefficient of its tt-th points as 64 consecutive doubles
+  const int Ms[3] = {4, 8, 16};
+  for (int mi = 0; mi < 3; ++mi) {
+    const int M = Ms[mi];
+    coef_p[mi] = nullptr;
+    if (64 * M < N) continue;
+    std::vector<double> il((size_t)nt * NK * 64 * M, 0.0);
+    for (int t = 0; t < nt; ++t)
+      for (int k = 0; k < NK; ++k)
+        for (int tt = 0; tt < M; ++tt)
+          for (int l = 0; l < 64; ++l) {
+            const int e = l * M + tt;
+            if (e < N) il[(((size_t)t * NK + k) * M + tt) * 64 + l] = tr[((size_t)t * NK + k) * N + e];
+          }
+    if ((rc = dev_upload(c, il.data(), il.size(), &coef_p[mi]))) return rc;
+  }
   return dev_upload(c, en.data(), en.size(), ends);
 }
 
@@ -673,6 +693,8 @@ void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double
   a.fcoef = d.fcoef[kind];
   a.fcoef_ends = d.fcoef_ends[kind];
   a.fcoef_t = d.fcoef_t[kind];
+  a.fcoef_p = d.shape[kind].variant == 4 ? d.fcoef_p[kind][d.shape[kind].M == 4 ? 0 : (d.shape[kind].M == 8 ? 1 : 2)]
+                                          : nullptr;
   a.theta = theta;
   a.mode = mode;
   launch_line_solve(a, d.shape[kind], nmembers, st ? st : c->stream);
@@ -957,8 +979,10 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
     if ((rc = upload_lineop(c.get(), d->filter[k], dims[k], &o.op[1], "filter"))) return rc;
     if ((rc = pack_coeffs(c.get(), d->deriv[k], &o.coef[0], &o.coef_ends[0]))) return rc;
     if ((rc = pack_coeffs(c.get(), d->filter[k], &o.coef[1], &o.coef_ends[1]))) return rc;
-    if ((rc = pack_coeffs_fused(c.get(), d->deriv[k], &o.fcoef[0], &o.fcoef_ends[0], &o.fcoef_t[0]))) return rc;
-    if ((rc = pack_coeffs_fused(c.get(), d->filter[k], &o.fcoef[1], &o.fcoef_ends[1], &o.fcoef_t[1]))) return rc;
+    if ((rc = pack_coeffs_fused(c.get(), d->deriv[k], &o.fcoef[0], &o.fcoef_ends[0], &o.fcoef_t[0], o.fcoef_p[0])))
+      return rc;
+    if ((rc = pack_coeffs_fused(c.get(), d->filter[k], &o.fcoef[1], &o.fcoef_ends[1], &o.fcoef_t[1], o.fcoef_p[1])))
+      return rc;
     if (k == 0) { o.n1 = d->ny; o.nlines = d->ny * d->nz; o.stride = 1; }
     if (k == 1) { o.n1 = d->nx; o.nlines = d->nx * d->nz; o.stride = d->nx; }
     if (k == 2) { o.n1 = d->nx; o.nlines = d->nx * d->ny; o.stride = (long long)d->nx * d->ny; }

@@ -66,6 +66,7 @@ struct SolveArgs {
   const double* fcoef;        // fused per-point coefficients [type][point][2nR-1+3] (persistent wave-segment)
   const double* fcoef_ends;   // per type: A(1) B and C(1) B of the periodic corrections
   const double* fcoef_t;      // the fused records coefficient-major: [type][coefficient][point] (x, variant 4)
+  const double* fcoef_p;      // the same lane-interleaved for M points per lane: [type][k][tt][lane] (variant 4)
   double theta;               // filter blend (mode 1)
   int mode;                   // 0 write result; 1 in-place filter with FilterCharTime blend
 };

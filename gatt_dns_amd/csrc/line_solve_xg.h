@@ -29,6 +29,7 @@ __device__ __forceinline__ double xg_shfl_xor(double v, int m) { return __shfl_x
 
 template <int NR, int M, bool BLEND>
 __global__ __launch_bounds__(256) void line_solve_xg_kernel(SolveArgs a) {
+  extern __shared__ double shx[];
   constexpr int H = NR - 1;
   constexpr int NW = 2 * H + 1;
   constexpr int W = M + 2 * H;
@@ -43,10 +44,18 @@ __global__ __launch_bounds__(256) void line_solve_xg_kernel(SolveArgs a) {
   double* dst = a.out[f] + (long long)mem * a.out_stride + (long long)a.nx * line;
   const int s = lane * M;
   const int tu = __builtin_amdgcn_readfirstlane(a.types[line]);
-  // coefficient k of point e: ct[k * N + e]; points past the line end read as zero
-  const double* ct = a.fcoef_t + (size_t)tu * (NW + 3) * N;
-  auto coef = [&](int k, int e) { return e < N ? ct[(size_t)k * N + e] : 0.0; };
+  // coefficient k of this lane's point s + tt, lane-interleaved (64 consecutive doubles per
+  // (k, tt) across the wavefront); points past the line end hold zeros
+  const double* ct = a.fcoef_p + (size_t)tu * (NW + 3) * 64 * M + lane;
+  auto coef = [&](int k, int tt) { return ct[((size_t)k * M + tt) * 64]; };
 
+  // the line goes through this wavefront's LDS buffer: unit-stride loads and stores
+  // in HBM, each lane's window of M + 2H points read from LDS
+  double* buf = shx + (size_t)(threadIdx.x >> 6) * N;
+  for (int e = lane; e < N; e += 64) buf[e] = src[e];
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   double fw[W];
 #pragma unroll
   for (int u = 0; u < W; ++u) {
@@ -54,23 +63,23 @@ __global__ __launch_bounds__(256) void line_solve_xg_kernel(SolveArgs a) {
     e = e < 0 ? e + N : (e >= N ? e - N : e);
     e = e >= N ? e - N : e;
     e = e < 0 ? 0 : (e >= N ? N - 1 : e);
-    fw[u] = src[e];
+    fw[u] = buf[e];
   }
   // band product, scaled by the forward factor
   double r[M];
 #pragma unroll
   for (int tt = 0; tt < M; ++tt) {
     const int e = s + tt;
-    double acc = coef(H, e) * fw[tt + H];
+    double acc = coef(H, tt) * fw[tt + H];
 #pragma unroll
-    for (int q = 1; q <= H; ++q) acc = fma(coef(H - q, e), fw[tt + H - q], fma(coef(H + q, e), fw[tt + H + q], acc));
+    for (int q = 1; q <= H; ++q) acc = fma(coef(H - q, tt), fw[tt + H - q], fma(coef(H + q, tt), fw[tt + H + q], acc));
     r[tt] = acc;
   }
   double x1 = 0.0;
   if (per) {
     double part = 0.0;
 #pragma unroll
-    for (int tt = 0; tt < M; ++tt) part = fma(r[tt], coef(NW + 2, s + tt), part);
+    for (int tt = 0; tt < M; ++tt) part = fma(r[tt], coef(NW + 2, tt), part);
 #pragma unroll
     for (int m = 1; m < 64; m <<= 1) part += xg_shfl_xor(part, m);
     x1 = part;
@@ -86,7 +95,7 @@ __global__ __launch_bounds__(256) void line_solve_xg_kernel(SolveArgs a) {
   double P = 0.0, Q = 1.0;
 #pragma unroll
   for (int tt = 0; tt < M; ++tt) {
-    const double fa = coef(NW, s + tt);
+    const double fa = coef(NW, tt);
     P = fma(-fa, P, r[tt]);
     Q = -fa * Q;
   }
@@ -102,7 +111,7 @@ __global__ __launch_bounds__(256) void line_solve_xg_kernel(SolveArgs a) {
   if (lane == 0) dprev = 0.0;
 #pragma unroll
   for (int tt = 0; tt < M; ++tt) {
-    const double d = fma(-coef(NW, s + tt), dprev, r[tt]);
+    const double d = fma(-coef(NW, tt), dprev, r[tt]);
     r[tt] = d;
     dprev = d;
   }
@@ -111,7 +120,7 @@ __global__ __launch_bounds__(256) void line_solve_xg_kernel(SolveArgs a) {
   Q = 1.0;
 #pragma unroll
   for (int tt = M - 1; tt >= 0; --tt) {
-    const double ce = coef(NW + 1, s + tt);
+    const double ce = coef(NW + 1, tt);
     P = fma(-ce, P, r[tt]);
     Q = -ce * Q;
   }
@@ -127,29 +136,38 @@ __global__ __launch_bounds__(256) void line_solve_xg_kernel(SolveArgs a) {
   if (lane == 63) xnext = 0.0;
 #pragma unroll
   for (int tt = M - 1; tt >= 0; --tt) {
-    const double x = fma(-coef(NW + 1, s + tt), xnext, r[tt]);
+    const double x = fma(-coef(NW + 1, tt), xnext, r[tt]);
     r[tt] = x;
     xnext = x;
   }
   if (per && lane == 0) r[0] = x1;
+  // every lane has read its window (the shuffles above order the wavefront)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
   for (int tt = 0; tt < M; ++tt) {
     const int e = s + tt;
     if (e < N) {
       double v = r[tt];
       if (BLEND) v = fma(a.theta, v - fw[tt + H], fw[tt + H]);
-      dst[e] = v;
+      buf[e] = v;
     }
   }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int e = lane; e < N; e += 64) dst[e] = buf[e];
 }
 
 template <int NR, int M>
 static void launch_xg_m(const SolveArgs& a, int nmembers, hipStream_t st) {
   dim3 grid((unsigned)((a.nlines + 3) / 4), (unsigned)(a.nf * nmembers));
+  const size_t lds = sizeof(double) * 4 * (size_t)a.op.N;
   if (a.mode == 1 && a.theta > 0.0)
-    hipLaunchKernelGGL((line_solve_xg_kernel<NR, M, true>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((line_solve_xg_kernel<NR, M, true>), grid, dim3(256), lds, st, a);
   else
-    hipLaunchKernelGGL((line_solve_xg_kernel<NR, M, false>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((line_solve_xg_kernel<NR, M, false>), grid, dim3(256), lds, st, a);
 }
 
 template <int NR>
