@@ -60,6 +60,7 @@ struct DirOps {
   double* fcoef_ends[2] = {nullptr, nullptr}; // per type A(1) B(2nd row), C(1) B(last row)
   double* fcoef_t[2] = {nullptr, nullptr};    // fused records coefficient-major (x lane-group solves)
   double* fcoef_p[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // lane-interleaved, M = 4, 8, 16
+  double* fcoef_pt[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // transposed solve, lane-interleaved, M = 4, 8
   LineShape shape[2];
   int* types = nullptr;      // per line, 0-based
   bool uniform_tiles = false;  // every tile of 64 consecutive lines has one type
@@ -202,6 +203,18 @@ LineShape choose_shape(int dir, int N, int nR, long long nlines, long long max_l
       }
     }
   }
+  if (dir == 0 && N >= 256) {
+    // long x lines whose tiles mix line types: one wavefront per line, the fewest points
+    // per lane that cover it
+    for (int M : {4, 8, 16}) {
+      LineShape s;
+      s.variant = 4;
+      s.M = M;
+      s.G = 64;
+      s.waves_per_block = 4;
+      if (line_shape_ok(s, N, nR, dir, max_lds, uniform_tiles)) return s;
+    }
+  }
   for (int M : {16, 8, 4, 32}) {
     for (int wpb : {4, 2, 1}) {
       LineShape s;
@@ -282,7 +295,7 @@ int pack_coeffs(gatt_ctx* c, const gatt_lineop& h, double** coef, double** ends)
 // row's backward factor is zero.  The end factors are A(1) and C(1) times the B
 // of the rows they correct (rows 2 and N).
 int pack_coeffs_fused(gatt_ctx* c, const gatt_lineop& h, double** coef, double** ends, double** coef_t,
-                      double** coef_p) {
+                      double** coef_p, double** coef_pt) {
   const int N = h.n, nt = h.ntypes, NW = 2 * h.nR - 1, FS = (NW + 3 + 1) & ~1, per = h.periodic ? 1 : 0;
   const int NB = N - per;
   std::vector<double> pk((size_t)nt * N * FS, 0.0), en((size_t)nt * 2, 0.0);
@@ -330,6 +343,34 @@ int pack_coeffs_fused(gatt_ctx* c, const gatt_lineop& h, double** coef, double**
             if (e < N) il[(((size_t)t * NK + k) * M + tt) * 64 + l] = tr[((size_t)t * NK + k) * N + e];
           }
     if ((rc = dev_upload(c, il.data(), il.size(), &coef_p[mi]))) return rc;
+  }
+  // the records of the transposed solve, lane-interleaved for M = 4, 8: for output point
+  // j, rows 0..2nR-2 hold the band column j of offset l = row - (nR-1), taken from row
+  // j - l (circularly); then the shifted sweep factors ce(j-1) (zero for j <= periodic),
+  // fa'(j+1), and D(j)
+  const int H = h.nR - 1;
+  for (int mi = 0; mi < 2; ++mi) {
+    const int M = Ms[mi];
+    coef_pt[mi] = nullptr;
+    if (64 * M < N) continue;
+    std::vector<double> il((size_t)nt * NK * 64 * M, 0.0);
+    auto at = [&](int t, int k, int e) { return tr[((size_t)t * NK + k) * N + e]; };
+    for (int t = 0; t < nt; ++t)
+      for (int tt = 0; tt < M; ++tt)
+        for (int l = 0; l < 64; ++l) {
+          const int j = l * M + tt;
+          if (j >= N) continue;
+          auto put = [&](int k, double v) { il[(((size_t)t * NK + k) * M + tt) * 64 + l] = v; };
+          for (int o = -H; o <= H; ++o) {
+            int se = j - o;
+            se = se < 0 ? se + N : (se >= N ? se - N : se);
+            put(H + o, at(t, H + o, se));
+          }
+          put(NW, j > per ? at(t, NW + 1, j - 1) : 0.0);
+          put(NW + 1, j + 1 < N ? at(t, NW, j + 1) : 0.0);
+          put(NW + 2, at(t, NW + 2, j));
+        }
+    if ((rc = dev_upload(c, il.data(), il.size(), &coef_pt[mi]))) return rc;
   }
   return dev_upload(c, en.data(), en.size(), ends);
 }
@@ -979,9 +1020,11 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
     if ((rc = upload_lineop(c.get(), d->filter[k], dims[k], &o.op[1], "filter"))) return rc;
     if ((rc = pack_coeffs(c.get(), d->deriv[k], &o.coef[0], &o.coef_ends[0]))) return rc;
     if ((rc = pack_coeffs(c.get(), d->filter[k], &o.coef[1], &o.coef_ends[1]))) return rc;
-    if ((rc = pack_coeffs_fused(c.get(), d->deriv[k], &o.fcoef[0], &o.fcoef_ends[0], &o.fcoef_t[0], o.fcoef_p[0])))
+    if ((rc = pack_coeffs_fused(c.get(), d->deriv[k], &o.fcoef[0], &o.fcoef_ends[0], &o.fcoef_t[0], o.fcoef_p[0],
+                                o.fcoef_pt[0])))
       return rc;
-    if ((rc = pack_coeffs_fused(c.get(), d->filter[k], &o.fcoef[1], &o.fcoef_ends[1], &o.fcoef_t[1], o.fcoef_p[1])))
+    if ((rc = pack_coeffs_fused(c.get(), d->filter[k], &o.fcoef[1], &o.fcoef_ends[1], &o.fcoef_t[1], o.fcoef_p[1],
+                                o.fcoef_pt[1])))
       return rc;
     if (k == 0) { o.n1 = d->ny; o.nlines = d->ny * d->nz; o.stride = 1; }
     if (k == 1) { o.n1 = d->nx; o.nlines = d->nx * d->nz; o.stride = d->nx; }
@@ -2103,6 +2146,7 @@ void run_solve_t(gatt_ctx* c, int dir, int kind, int nf, double* const* in, doub
   b.coef = d.coef[kind];
   b.R = d.op[kind].R;
   b.fcoef_t = d.fcoef_t[kind];
+  b.fcoef_pt = d.fcoef_pt[kind][d.op[kind].N <= 256 ? 0 : 1];
   b.fcoef_ends = d.fcoef_ends[kind];
   b.periodic = d.op[kind].periodic;
   b.nlines = d.nlines;

@@ -255,6 +255,7 @@ struct BandTArgs {
   const double* coef;     // packed records [type][point][cs], band first
   const double* R;        // band rows as the reference stores them: N x (2nR-1) x ntypes
   const double* fcoef_t;  // fused records coefficient-major (x transposed solve)
+  const double* fcoef_pt; // the transposed solve's records lane-interleaved (M = 4 for N <= 256, else 8)
   const double* fcoef_ends;
   int periodic, nlines;
   const int* types;       // per line

@@ -224,24 +224,31 @@ __global__ __launch_bounds__(256) void line_solve_xgt_kernel(BandTArgs a) {
   const int mem = (int)blockIdx.y / a.nf;
   const int N = a.N;
   const int per = a.periodic;
-  const int e0 = per;
   const double* in = a.in[f] + (long long)mem * a.in_stride + (long long)a.nx * line;
   double* out = a.out[f] + (long long)mem * a.out_stride + (long long)a.nx * line;
   const double* src = a.src[f] ? a.src[f] + (long long)mem * a.out_stride + (long long)a.nx * line : nullptr;
   const int tu = __builtin_amdgcn_readfirstlane(a.types[line]);
-  const double* ct = a.fcoef_t + (size_t)tu * (NW + 3) * N;
-  auto coef = [&](int k, int e) { return (e >= 0 && e < N) ? ct[(size_t)k * N + e] : 0.0; };
+  // lane-interleaved records of this lane's points s + tt: rows 0..NW-1 the transposed
+  // band column, NW the shifted ce(e-1), NW+1 the shifted fa'(e+1), NW+2 D(e)
+  const double* ct = a.fcoef_pt + (size_t)tu * (NW + 3) * 64 * M + lane;
+  auto cp = [&](int k, int tt) { return ct[((size_t)k * M + tt) * 64]; };
   const int s = lane * M;
+  // the line in through this wavefront's LDS buffer (unit-stride loads)
+  double* buf = sh + (size_t)wv * N;
+  double* obuf = sh + (size_t)(4 + wv) * N;
+  for (int e = lane; e < N; e += 64) buf[e] = in[e];
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   double r[M];
 #pragma unroll
-  for (int tt = 0; tt < M; ++tt) r[tt] = s + tt < N ? in[s + tt] : 0.0;
+  for (int tt = 0; tt < M; ++tt) r[tt] = s + tt < N ? buf[s + tt] : 0.0;
 
   // U^T z = xbar
   double P = 0.0, Q = 1.0;
 #pragma unroll
   for (int tt = 0; tt < M; ++tt) {
-    const int e = s + tt;
-    const double cm = e > e0 ? coef(NW + 1, e - 1) : 0.0;
+    const double cm = cp(NW, tt);
     P = fma(-cm, P, r[tt]);
     Q = -cm * Q;
   }
@@ -257,8 +264,7 @@ __global__ __launch_bounds__(256) void line_solve_xgt_kernel(BandTArgs a) {
   if (lane == 0) zprev = 0.0;
 #pragma unroll
   for (int tt = 0; tt < M; ++tt) {
-    const int e = s + tt;
-    const double cm = e > e0 ? coef(NW + 1, e - 1) : 0.0;
+    const double cm = cp(NW, tt);
     const double z = fma(-cm, zprev, r[tt]);
     r[tt] = z;
     zprev = z;
@@ -268,7 +274,7 @@ __global__ __launch_bounds__(256) void line_solve_xgt_kernel(BandTArgs a) {
   Q = 1.0;
 #pragma unroll
   for (int tt = M - 1; tt >= 0; --tt) {
-    const double fp = coef(NW, s + tt + 1);
+    const double fp = cp(NW + 1, tt);
     P = fma(-fp, P, r[tt]);
     Q = -fp * Q;
   }
@@ -284,7 +290,7 @@ __global__ __launch_bounds__(256) void line_solve_xgt_kernel(BandTArgs a) {
   if (lane == 63) rnext = 0.0;
 #pragma unroll
   for (int tt = M - 1; tt >= 0; --tt) {
-    const double rho = fma(-coef(NW, s + tt + 1), rnext, r[tt]);
+    const double rho = fma(-cp(NW + 1, tt), rnext, r[tt]);
     r[tt] = rho;
     rnext = rho;
   }
@@ -304,12 +310,12 @@ __global__ __launch_bounds__(256) void line_solve_xgt_kernel(BandTArgs a) {
 #pragma unroll
     for (int tt = 0; tt < M; ++tt) {
       const int e = s + tt;
-      const double De = coef(NW + 2, e);
+      const double De = cp(NW + 2, tt);
       r[tt] = e == 0 ? De * part : fma(De, part, r[tt]);
     }
   }
-  // transposed band through the wavefront's line buffer
-  double* buf = sh + (size_t)wv * N;
+  // transposed band through the wavefront's line buffer (the input was read into
+  // registers before the sweeps, so the buffer can be overwritten)
 #pragma unroll
   for (int tt = 0; tt < M; ++tt)
     if (s + tt < N) buf[s + tt] = r[tt];
@@ -325,20 +331,27 @@ __global__ __launch_bounds__(256) void line_solve_xgt_kernel(BandTArgs a) {
       for (int l = -H; l <= H; ++l) {
         int se = j - l;
         se = se < 0 ? se + N : (se >= N ? se - N : se);
-        acc = fma(ct[(size_t)(H + l) * N + se], buf[se], acc);
+        acc = fma(cp(H + l, tt), buf[se], acc);
       }
-      double v = a.alpha * acc;
-      if (src) v = a.beta * src[j] + v;
-      if (a.accumulate) v = out[j] + v;
-      out[j] = v;
+      obuf[j] = a.alpha * acc;
     }
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // unit-stride combination with src and out
+  for (int j = lane; j < N; j += 64) {
+    double v = obuf[j];
+    if (src) v = a.beta * src[j] + v;
+    if (a.accumulate) v = out[j] + v;
+    out[j] = v;
   }
 }
 
 template <int NR, int M>
 static void launch_xgt_m(const BandTArgs& a, int nmembers, hipStream_t st) {
   dim3 grid((unsigned)((a.nlines + 3) / 4), (unsigned)(a.nf * nmembers));
-  hipLaunchKernelGGL((line_solve_xgt_kernel<NR, M>), grid, dim3(256), sizeof(double) * 4 * (size_t)a.N, st, a);
+  hipLaunchKernelGGL((line_solve_xgt_kernel<NR, M>), grid, dim3(256), sizeof(double) * 8 * (size_t)a.N, st, a);
 }
 
 template <int NR>
