@@ -32,6 +32,33 @@ def _compile(src, obj):
     return obj
 
 
+FLOWIO_LIB = os.path.join(OUT_DIR, "libgatt_flowio.so")
+HDF5_DIR = os.environ.get("GATT_HDF5_DIR", "/opt/conda")
+
+
+def build_flowio(verbose=True):
+    """Host library of the MAT 7.3 flow files (flow_io.cpp, libhdf5).  Skipped with a
+    message when the HDF5 headers are not found; returns the library path or None."""
+    inc = os.path.join(HDF5_DIR, "include")
+    libdir = os.path.join(HDF5_DIR, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        if verbose:
+            print("flow-file library skipped: no hdf5.h under", inc)
+        return None
+    os.makedirs(OUT_DIR, exist_ok=True)
+    cxx = os.environ.get("CXX", shutil.which("g++") or "g++")
+    tmp = FLOWIO_LIB + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-I", inc, os.path.join(SRC, "flow_io.cpp"),
+           "-L", libdir, "-lhdf5", "-Wl,-rpath," + libdir, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("flow-file library build failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    os.replace(tmp, FLOWIO_LIB)
+    if verbose:
+        print("built", FLOWIO_LIB)
+    return FLOWIO_LIB
+
+
 def build(verbose=True):
     """Compile every HIP source and link the shared library. Always recompiles."""
     os.makedirs(OUT_DIR, exist_ok=True)
@@ -48,6 +75,7 @@ def build(verbose=True):
         os.remove(o)
     if verbose:
         print("built", LIB)
+    build_flowio(verbose)
     return LIB
 
 

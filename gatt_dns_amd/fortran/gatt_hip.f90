@@ -105,6 +105,24 @@ module gatt_hip
       type(c_ptr), value :: ctx
       real(c_double), value :: t
     end function
+    ! flow files flow_%010d.mat (include/gatt_flow_io.h, libgatt_flowio.so): q holds
+    ! c_loc(U), ..., c_loc(E); inside_wall is c_null_ptr or an integer(c_int8_t) mask
+    integer(c_int) function gatt_flow_write(path, nx, ny, nz, t, q, inside_wall) bind(C, name="gatt_flow_write")
+      import :: c_ptr, c_int, c_double, c_int32_t, c_char
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int32_t), value :: nx, ny, nz
+      real(c_double), value :: t
+      type(c_ptr), dimension(5), intent(in) :: q
+      type(c_ptr), value :: inside_wall
+    end function
+    integer(c_int) function gatt_flow_read(path, nx, ny, nz, t, q, inside_wall) bind(C, name="gatt_flow_read")
+      import :: c_ptr, c_int, c_double, c_int32_t, c_char
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int32_t), value :: nx, ny, nz
+      real(c_double), intent(out) :: t
+      type(c_ptr), dimension(5), intent(in) :: q
+      type(c_ptr), value :: inside_wall
+    end function
     integer(c_int) function gatt_sfd(ctx, dt) bind(C, name="gatt_sfd")
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx

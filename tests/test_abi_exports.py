@@ -1,4 +1,5 @@
-"""The C-ABI library loads and exports every function include/gatt_dns.h declares (no GPU needed)."""
+"""The C-ABI libraries load and export every function include/gatt_dns.h and
+include/gatt_flow_io.h declare (no GPU needed)."""
 import ctypes
 import os
 import re
@@ -7,10 +8,11 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "gatt_dns.h")
+FLOW_HEADER = os.path.join(ROOT, "include", "gatt_flow_io.h")
 
 
-def declared():
-    text = open(HEADER).read()
+def declared(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(gatt_\w+)\s*\(", text, flags=re.M)))
 
@@ -26,4 +28,14 @@ def test_library_exports_every_declared_symbol():
         pytest.skip("libgatt_dns.so not built")
     lib = ctypes.CDLL(LIB_PATH)
     missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_flow_library_exports_every_declared_symbol():
+    from gatt_dns_amd import flowio
+    assert sorted(flowio.EXPORTS) == declared(FLOW_HEADER)
+    if not os.path.exists(flowio.LIB_PATH):
+        pytest.skip("libgatt_flowio.so not built")
+    lib = ctypes.CDLL(flowio.LIB_PATH)
+    missing = [n for n in declared(FLOW_HEADER) if not hasattr(lib, n)]
     assert not missing, missing
