@@ -57,7 +57,9 @@ __global__ __launch_bounds__(256) void band_t_kernel(BandTArgs a, int m0) {
 // window of its segment plus H points either side in registers, so every input
 // row is read once per wavefront.  Same sums, in the same order, as
 // band_t_kernel.
-template <int DIR, int NR, int M>
+// UNI: the 64 lines of a block share one type, so the coefficient addresses are
+// wave-uniform and go through the scalar cache.
+template <int DIR, int NR, int M, bool UNI>
 __global__ __launch_bounds__(1024) void band_t_seg_kernel(BandTArgs a) {
   constexpr int H = NR - 1;
   constexpr int W = M + 2 * H;
@@ -91,7 +93,8 @@ __global__ __launch_bounds__(1024) void band_t_seg_kernel(BandTArgs a) {
     e = e >= N ? N - 1 : e;
     fw[u] = in[base + (long long)e * stride];
   }
-  const double* Rt = a.R + (size_t)a.types[line] * N * (2 * NR - 1);
+  const int ty = UNI ? __builtin_amdgcn_readfirstlane(a.types[line0]) : a.types[line];
+  const double* Rt = a.R + (size_t)ty * N * (2 * NR - 1);
   double* out = a.out[f] + (long long)mem * a.out_stride;
   const double* src = a.src[f] ? a.src[f] + (long long)mem * a.out_stride : nullptr;
   if (lane >= nact) return;
@@ -121,8 +124,11 @@ static bool launch_band_t_seg_m(const BandTArgs& a, int nmembers, hipStream_t st
   const int G = (a.N + M - 1) / M;
   dim3 grid((unsigned)((nlines + 63) / 64), (unsigned)(a.nf * nmembers));
   switch (a.nR) {
-#define GATT_BAND_T_NR(v) \
-  case v: hipLaunchKernelGGL((band_t_seg_kernel<DIR, v, M>), grid, dim3(64 * G), 0, st, a); return true;
+#define GATT_BAND_T_NR(v)                                                                   \
+  case v:                                                                                   \
+    if (a.uniform) hipLaunchKernelGGL((band_t_seg_kernel<DIR, v, M, true>), grid, dim3(64 * G), 0, st, a); \
+    else hipLaunchKernelGGL((band_t_seg_kernel<DIR, v, M, false>), grid, dim3(64 * G), 0, st, a);        \
+    return true;
     GATT_BAND_T_NR(2) GATT_BAND_T_NR(3) GATT_BAND_T_NR(4) GATT_BAND_T_NR(5)
 #undef GATT_BAND_T_NR
   }
@@ -168,6 +174,29 @@ __global__ __launch_bounds__(256) void axpby_kernel(AxpbyArgs a) {
 
 void launch_axpby(const AxpbyArgs& a, int nmembers, hipStream_t st) {
   hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(a.np, 256), (unsigned)(a.nf * nmembers)), dim3(256), 0, st, a);
+}
+
+__global__ __launch_bounds__(256) void adj_combine_kernel(AdjCombineArgs a) {
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= a.np) return;
+  const int f = (int)blockIdx.y % a.nf;
+  const long long o = (long long)((int)blockIdx.y / a.nf) * a.stride + p;
+  const double sb = a.sb[f][o];
+  if (a.mode == 0) {
+    double* acc = a.acc[f] + o;
+    *acc = a.first ? 1.0 * sb : 1.0 * sb + 1.0 * *acc;
+    const double t = a.a1 * a.qb[f][o];
+    a.k[f][o] = a.a2 * sb + 1.0 * t;
+  } else {
+    double* qb = a.qb[f] + o;
+    const double t = 1.0 * a.acc[f][o] + 1.0 * *qb;
+    *qb = 1.0 * sb + 1.0 * t;
+  }
+}
+
+void launch_adj_combine(const AdjCombineArgs& a, int nmembers, hipStream_t st) {
+  hipLaunchKernelGGL(adj_combine_kernel, dim3(grid_for(a.np, 256), (unsigned)(a.nf * nmembers)), dim3(256), 0, st,
+                     a);
 }
 
 // -------------------------------------------------------------- viscous terms

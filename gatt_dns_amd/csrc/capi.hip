@@ -1992,6 +1992,27 @@ void fields_axpby(gatt_ctx* c, double* y, long long ys, const double* x, long lo
   launch_axpby(ar, c->batch, c->stream);
 }
 
+// the combinations of the backward RK4 pass in one launch (AdjCombineArgs)
+void adj_combine(gatt_ctx* c, double* QB, const double* SB, double* ACC, double* KB, double a1, double a2, int first,
+                 int mode) {
+  AdjCombineArgs ar{};
+  ar.np = c->np;
+  ar.stride = c->lin_stride;
+  for (int f : state_fields(c)) {
+    const long long o = (long long)f * c->np;
+    ar.sb[ar.nf] = SB + o;
+    ar.qb[ar.nf] = QB + o;
+    ar.acc[ar.nf] = ACC + o;
+    ar.k[ar.nf] = KB ? KB + o : nullptr;
+    ++ar.nf;
+  }
+  ar.a1 = a1;
+  ar.a2 = a2;
+  ar.first = first;
+  ar.mode = mode;
+  launch_adj_combine(ar, c->batch, c->stream);
+}
+
 // Base derivatives, stresses and their derivatives at S (member m0 pointer,
 // member stride ss) into D, T, G (member stride ws), for nm members.
 void base_dtg(gatt_ctx* c, const double* S, long long ss, int nm, double* D, double* T, double* G, long long ws) {
@@ -2147,6 +2168,7 @@ void run_solve_t(gatt_ctx* c, int dir, int kind, int nf, double* const* in, doub
   b.R = d.op[kind].R;
   b.fcoef_t = d.fcoef_t[kind];
   b.fcoef_pt = d.fcoef_pt[kind][d.op[kind].N <= 256 ? 0 : 1];
+  b.uniform = d.uniform_tiles ? 1 : 0;
   b.fcoef_ends = d.fcoef_ends[kind];
   b.periodic = d.op[kind].periodic;
   b.nlines = d.nlines;
@@ -2184,6 +2206,7 @@ void run_solve_t(gatt_ctx* c, int dir, int kind, int nf, double* const* in, doub
   }
   a.coef = d.coef[kind];
   a.coef_ends = d.coef_ends[kind];
+  a.uniform = d.uniform_tiles ? 1 : 0;
   launch_line_solve_t(a, nm, c->stream);
   launch_band_t(b, nm, c->stream);
 }
@@ -2531,22 +2554,17 @@ void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
   fields_axpby(c, KB, ls, QB, ls, dt / 6.0, 0.0);
   adj_rhs(c, B3, ls, KB, SB, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
   run_bc_adj(c, P3, ls, SB);
-  fields_axpby(c, B3, ls, SB, ls, 1.0, 0.0);          // B3 now accumulates the stage adjoints
-  fields_axpby(c, KB, ls, QB, ls, dt / 3.0, 0.0);
-  fields_axpby(c, KB, ls, SB, ls, dt, 1.0);
+  // B3 now accumulates the stage adjoints; KB = dt/3 QB + dt SB
+  adj_combine(c, QB, SB, B3, KB, dt / 3.0, dt, 1, 0);
   adj_rhs(c, B2, ls, KB, SB, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
   run_bc_adj(c, P2, ls, SB);
-  fields_axpby(c, B3, ls, SB, ls, 1.0, 1.0);
-  fields_axpby(c, KB, ls, QB, ls, dt / 3.0, 0.0);
-  fields_axpby(c, KB, ls, SB, ls, dt / 2.0, 1.0);
+  adj_combine(c, QB, SB, B3, KB, dt / 3.0, dt / 2.0, 0, 0);
   adj_rhs(c, B1, ls, KB, SB, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
   run_bc_adj(c, P1, ls, SB);
-  fields_axpby(c, B3, ls, SB, ls, 1.0, 1.0);
-  fields_axpby(c, KB, ls, QB, ls, dt / 6.0, 0.0);
-  fields_axpby(c, KB, ls, SB, ls, dt / 2.0, 1.0);
+  adj_combine(c, QB, SB, B3, KB, dt / 6.0, dt / 2.0, 0, 0);
   adj_rhs(c, q, sl, KB, SB, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
-  fields_axpby(c, QB, ls, B3, ls, 1.0, 1.0);
-  fields_axpby(c, QB, ls, SB, ls, 1.0, 1.0);
+  // QB = SB + (B3 + QB)
+  adj_combine(c, QB, SB, B3, nullptr, 0.0, 0.0, 0, 1);
 }
 
 int copy_lin_fields(gatt_ctx* c, int32_t member, const double* const* hq, bool to_dev) {

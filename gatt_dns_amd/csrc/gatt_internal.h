@@ -51,6 +51,7 @@ struct SolveArgs {
   int TL;                     // lines per workgroup tile
   int lds_row;                // LDS words per point row of the tile
   float inv_n;                // 1/N
+  int uniform;                // every 64-line tile carries one line type (transposed solves)
   int lds_coef_offset;        // wave-segment: LDS word offset of the per-wavefront coefficient slots
   int cs;                     // doubles per packed coefficient record (transposed solve)
   int nf;                     // fields per member
@@ -258,6 +259,7 @@ struct BandTArgs {
   const double* fcoef_pt; // the transposed solve's records lane-interleaved (M = 4 for N <= 256, else 8)
   const double* fcoef_ends;
   int periodic, nlines;
+  int uniform;            // every 64-line tile carries one line type (wave-uniform coefficients)
   const int* types;       // per line
   int nf;
   long long in_stride, out_stride;
@@ -318,6 +320,23 @@ struct AxpbyArgs {
   double a, b;
 };
 void launch_axpby(const AxpbyArgs& a, int nmembers, hipStream_t st);
+
+// The field combinations between two adjoint RHS evaluations of the backward RK4
+// pass, in one launch (the same operations, in the same order, as the separate
+// y = a x + b y updates they replace):
+//   mode 0:  acc = sb (+ acc unless first);  k = a2 sb + 1.0 (a1 qb)
+//   mode 1:  qb = 1.0 sb + 1.0 (1.0 acc + 1.0 qb)
+struct AdjCombineArgs {
+  long long np, stride;   // member stride of all four arrays
+  int nf;
+  const double* sb[kMaxPtr];
+  double* qb[kMaxPtr];
+  double* acc[kMaxPtr];
+  double* k[kMaxPtr];
+  double a1, a2;
+  int first, mode;
+};
+void launch_adj_combine(const AdjCombineArgs& a, int nmembers, hipStream_t st);
 
 // ------------------------------------------------------- stability analysis
 struct PerturbArgs {

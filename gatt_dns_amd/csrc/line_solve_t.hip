@@ -23,7 +23,9 @@
 
 namespace gatt {
 
-template <int DIR, int M, bool XLDS>
+// UNI: the block's 64 lines share one type, so the coefficient addresses are
+// wave-uniform and the records go through the scalar cache.
+template <int DIR, int M, bool XLDS, bool UNI>
 __global__ __launch_bounds__(1024) void line_solve_t_kernel(SolveArgs a) {
   extern __shared__ double sh[];
   const int lane = threadIdx.x & 63;
@@ -77,7 +79,7 @@ __global__ __launch_bounds__(1024) void line_solve_t_kernel(SolveArgs a) {
   }
   if (XLDS) __syncthreads();   // the tile words are reused by the maps
 
-  const int type = a.types[lsafe];
+  const int type = UNI ? __builtin_amdgcn_readfirstlane(a.types[line0]) : a.types[lsafe];
   const double* cf = a.coef + (size_t)type * N * CS;   // record of point e at cf + e*CS
   const int per = a.op.periodic;
   const int e0 = per;
@@ -211,8 +213,9 @@ static void launch_t_m(const SolveArgs& a0, int nmembers, hipStream_t st) {
   const long long tiles = (a.nlines + 63) / 64;
   const size_t lds = line_solve_t_lds_bytes(DIR, a.op.N, M);
   dim3 grid((unsigned)tiles, (unsigned)(a.nf * nmembers));
-  auto k = (DIR == 0 && a.op.N <= kMaxTileN) ? line_solve_t_kernel<DIR, M, DIR == 0>
-                                             : line_solve_t_kernel<DIR, M, false>;
+  const bool tile = DIR == 0 && a.op.N <= kMaxTileN;
+  auto k = a.uniform ? (tile ? line_solve_t_kernel<DIR, M, DIR == 0, true> : line_solve_t_kernel<DIR, M, false, true>)
+                     : (tile ? line_solve_t_kernel<DIR, M, DIR == 0, false> : line_solve_t_kernel<DIR, M, false, false>);
   if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(k, grid, dim3(64 * G), lds, st, a);
 }
