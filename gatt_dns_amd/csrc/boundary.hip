@@ -59,6 +59,12 @@ __global__ __launch_bounds__(256) void bc_wave_kernel(BcArgs a) {
   double* R = a.q[3] + mo;
   double* E = a.q[4] + mo;
   const long long n = pidx(i, j, k, nx, ny);
+  if (a.undo) {
+    // a point is written by one op of a wave only, and read by no other op of it
+    double* u = a.undo + (long long)blockIdx.y * a.undo_member_stride + op.undo;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) u[(long long)f * op.npts + p] = (a.q[f] + mo)[n];
+  }
   switch (op.kind) {
     case BC_SET: {
       (a.q[op.field] + mo)[n] = op.value;
@@ -183,9 +189,9 @@ void launch_bc_wave(const BcArgs& a, long long nblocks, int nmembers, hipStream_
   hipLaunchKernelGGL(bc_wave_kernel, dim3((unsigned)nblocks, nmembers), dim3(256), 0, st, a);
 }
 
-// dst <- src for all five fields on the points of the boxes of one wave's ops
-// (the points a replay of that wave may have written).
-__global__ __launch_bounds__(256) void bc_restore_kernel(BcCopyArgs a) {
+// the five field values one wave's ops saved before writing, put back on their points
+// (the transposed boundary program walks the waves backwards with it)
+__global__ __launch_bounds__(256) void bc_undo_kernel(BcUndoArgs a) {
   const long long blk = blockIdx.x;
   int o = a.op_begin;
   while (o + 1 < a.op_end && a.ops[o + 1].first_block <= blk) ++o;
@@ -198,14 +204,15 @@ __global__ __launch_bounds__(256) void bc_restore_kernel(BcCopyArgs a) {
   const int j = op.box[2] + (int)((p / bi) % bj);
   const int k = op.box[4] + (int)(p / ((long long)bi * bj));
   const long long n = pidx(i, j, k, a.nx, a.ny);
-  const long long ms = (long long)blockIdx.y * a.src_stride, md = (long long)blockIdx.y * a.dst_stride;
+  const long long mo = (long long)blockIdx.y * a.member_stride;
+  const double* u = a.undo + (long long)blockIdx.y * a.undo_member_stride + op.undo;
 #pragma unroll
-  for (int f = 0; f < 5; ++f) a.dst[f][md + n] = a.src[f][ms + n];
+  for (int f = 0; f < 5; ++f) a.q[f][mo + n] = u[(long long)f * op.npts + p];
 }
 
-void launch_bc_restore(const BcCopyArgs& a, long long nblocks, int nmembers, hipStream_t st) {
+void launch_bc_undo(const BcUndoArgs& a, long long nblocks, int nmembers, hipStream_t st) {
   if (nblocks <= 0) return;
-  hipLaunchKernelGGL(bc_restore_kernel, dim3((unsigned)nblocks, nmembers), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(bc_undo_kernel, dim3((unsigned)nblocks, nmembers), dim3(256), 0, st, a);
 }
 
 }  // namespace gatt

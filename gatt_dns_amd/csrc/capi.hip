@@ -98,6 +98,8 @@ struct gatt_ctx {
   int stepping = 1;     // timeStepping: 1 RK4, 2 Euler, 3 SSPRK3
   double* d_clock = nullptr;   // time of the step in progress (main.F90:424 accumulates it)
   bool timed = false;          // the boundary program has time-dependent disturbances
+  long long undo_total = 0;    // doubles per member of the boundary program's undo log
+  double* d_undo = nullptr;    // undo log of the transposed boundary program (allocated on first use)
   // pressureGradient forcing (runForcings{2,3}D.F90)
   int pg = 0, pg_r = 0, pg_u = 0;
   int pg_box[6] = {0, 0, 0, 0, 0, 0};
@@ -397,6 +399,8 @@ int compile_bc(gatt_ctx* c, const gatt_bc_desc& bc) {
     for (auto& b : s.rd)
       if (!in_range(b.b)) return fail(GATT_E_INVALID, "boundary stencil outside the grid");
     op.npts = (long long)(op.box[1] - op.box[0] + 1) * (op.box[3] - op.box[2] + 1) * (op.box[5] - op.box[4] + 1);
+    op.undo = c->undo_total;
+    c->undo_total += 5 * op.npts;
     ops.push_back(op);
     sets.push_back(s);
     return GATT_OK;
@@ -1933,6 +1937,12 @@ int ensure_lin(gatt_ctx* c) {
   if (hipMalloc(&p, sizeof(double) * (size_t)c->lin_stride * c->batch) != hipSuccess)
     return fail(GATT_E_NOMEM, "tangent/adjoint allocation failed");
   c->lin = static_cast<double*>(p);
+  if (c->undo_total > 0) {
+    if (hipMalloc(&p, sizeof(double) * (size_t)c->undo_total * c->batch) != hipSuccess)
+      return fail(GATT_E_NOMEM, "boundary undo log allocation failed");
+    c->d_undo = static_cast<double*>(p);
+    c->dev_allocs.push_back(p);
+  }
   // on the context's stream, so that later copies into the slab are ordered after the clear
   HIPCHK(hipMemsetAsync(c->lin, 0, sizeof(double) * (size_t)c->lin_stride * c->batch, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -2335,76 +2345,72 @@ bool wave_needs_base(const gatt_ctx* c, const BcWave& w) {
   return false;
 }
 
-// lam <- (d BC / d q at qpre)^T lam, waves in reverse order; the base state before
-// a wave that depends on it is rebuilt by replaying the earlier waves on a copy of qpre.
+// lam <- (d BC / d q at qpre)^T lam, waves in reverse order.  The base state before
+// a wave that depends on it comes from one forward replay of the waves on a copy of
+// qpre that logs the values each wave overwrites, undone wave by wave going back.
 void run_bc_adj(gatt_ctx* c, double* qpre, long long qs, double* lam) {
   const long long ls = c->lin_stride;
   double* tmp = lin_ptr(c, LN_TMP);
-  // tmp holds qpre with the first `applied` waves replayed on it (-1: not a copy of qpre yet)
-  int applied = -1;
-  for (int wi = (int)c->waves.size() - 1; wi >= 0; --wi) {
-    const BcWave& w = c->waves[(size_t)wi];
-    double* base = qpre;
-    long long bs = qs;
-    if (wave_needs_base(c, w)) {
-      if (applied < 0) {
-        AxpbyArgs cp{};
-        cp.np = c->np;
-        cp.x_stride = qs;
-        cp.y_stride = ls;
-        cp.nf = 5;
-        for (int f = 0; f < 5; ++f) {
-          cp.x[f] = qpre + (long long)f * c->np;
-          cp.y[f] = tmp + (long long)f * c->np;
-        }
-        cp.a = 1.0;
-        cp.b = 0.0;
-        launch_axpby(cp, c->batch, c->stream);
-      } else {
-        // the replayed waves wrote only points of their boxes: put qpre back there
-        for (int v = 0; v < applied; ++v) {
-          const BcWave& pw = c->waves[(size_t)v];
-          BcCopyArgs r{};
-          r.ops = c->d_ops;
-          r.op_begin = pw.op_begin;
-          r.op_end = pw.op_end;
-          for (int f = 0; f < 5; ++f) {
-            r.src[f] = qpre + (long long)f * c->np;
-            r.dst[f] = tmp + (long long)f * c->np;
-          }
-          r.src_stride = qs;
-          r.dst_stride = ls;
-          r.nx = c->nx;
-          r.ny = c->ny;
-          launch_bc_restore(r, pw.nblocks, c->batch, c->stream);
-        }
-      }
-      for (int v = 0; v < wi; ++v) {
-        const BcWave& pw = c->waves[(size_t)v];
-        BcArgs a{};
-        a.ops = c->d_ops;
-        a.op_begin = pw.op_begin;
-        a.op_end = pw.op_end;
-        for (int f = 0; f < 5; ++f) a.q[f] = tmp + (long long)f * c->np;
-        a.member_stride = ls;
-        a.coef[0] = c->d_coef[0];
-        a.coef[1] = c->d_coef[1];
-        a.aux = c->d_aux;
-        a.hold = c->d_hold;
-        a.hold_member_stride = c->hold_stride;
-        a.nx = c->nx;
-        a.ny = c->ny;
-        a.nz = c->nz;
-        a.gamma = c->gamma;
-        a.capture = 0;
-        a.clock = c->d_clock;
-        launch_bc_wave(a, pw.nblocks, c->batch, c->stream);
-      }
-      applied = wi;
-      base = tmp;
-      bs = ls;
+  const int nw = (int)c->waves.size();
+  int last = -1;   // the last wave whose transpose needs its base state
+  for (int wi = 0; wi < nw; ++wi)
+    if (wave_needs_base(c, c->waves[(size_t)wi])) last = wi;
+  if (last >= 0) {
+    AxpbyArgs cp{};
+    cp.np = c->np;
+    cp.x_stride = qs;
+    cp.y_stride = ls;
+    cp.nf = 5;
+    for (int f = 0; f < 5; ++f) {
+      cp.x[f] = qpre + (long long)f * c->np;
+      cp.y[f] = tmp + (long long)f * c->np;
     }
-    launch_bc_wave_adj(bc_lin_args(c, w, base, bs, lam, ls), w.nblocks, c->batch, c->stream);
+    cp.a = 1.0;
+    cp.b = 0.0;
+    launch_axpby(cp, c->batch, c->stream);
+    for (int v = 0; v < last; ++v) {
+      const BcWave& pw = c->waves[(size_t)v];
+      BcArgs a{};
+      a.ops = c->d_ops;
+      a.op_begin = pw.op_begin;
+      a.op_end = pw.op_end;
+      for (int f = 0; f < 5; ++f) a.q[f] = tmp + (long long)f * c->np;
+      a.member_stride = ls;
+      a.coef[0] = c->d_coef[0];
+      a.coef[1] = c->d_coef[1];
+      a.aux = c->d_aux;
+      a.hold = c->d_hold;
+      a.hold_member_stride = c->hold_stride;
+      a.nx = c->nx;
+      a.ny = c->ny;
+      a.nz = c->nz;
+      a.gamma = c->gamma;
+      a.capture = 0;
+      a.clock = c->d_clock;
+      a.undo = c->d_undo;
+      a.undo_member_stride = c->undo_total;
+      launch_bc_wave(a, pw.nblocks, c->batch, c->stream);
+    }
+  }
+  // tmp now holds the base state before wave `last`
+  for (int wi = nw - 1; wi >= 0; --wi) {
+    const BcWave& w = c->waves[(size_t)wi];
+    if (wi < last) {
+      BcUndoArgs u{};
+      u.ops = c->d_ops;
+      u.op_begin = w.op_begin;
+      u.op_end = w.op_end;
+      for (int f = 0; f < 5; ++f) u.q[f] = tmp + (long long)f * c->np;
+      u.member_stride = ls;
+      u.undo = c->d_undo;
+      u.undo_member_stride = c->undo_total;
+      u.nx = c->nx;
+      u.ny = c->ny;
+      launch_bc_undo(u, w.nblocks, c->batch, c->stream);
+    }
+    const bool based = wi <= last && wave_needs_base(c, w);
+    launch_bc_wave_adj(bc_lin_args(c, w, based ? tmp : qpre, based ? ls : qs, lam, ls), w.nblocks, c->batch,
+                       c->stream);
   }
 }
 

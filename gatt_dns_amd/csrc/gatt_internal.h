@@ -422,6 +422,7 @@ struct BcOp {
   double value;
   long long npts;
   long long first_block;
+  long long undo;    // offset of this op's 5 x npts saved values in the undo log
 };
 
 struct BcWave {
@@ -443,21 +444,25 @@ struct BcArgs {
   int capture;           // holdInlet: 1 on the first call
   const double* clock;   // the time t of the step in progress (one double on the device)
   double toff;           // the ops see t = *clock + toff
+  double* undo;          // when set, every op point saves its 5 field values here before the op
+  long long undo_member_stride;
 };
 void launch_bc_wave(const BcArgs& a, long long nblocks, int nmembers, hipStream_t st);
 // the device clock: set to t, or advanced by dt as main.F90:424 does (t = t + dt)
 void launch_clock_set(double* clock, double t, hipStream_t st);
 void launch_clock_add(double* clock, double dt, hipStream_t st);
 
-struct BcCopyArgs {
+// puts back the values the ops of one wave saved in the undo log (BcArgs::undo)
+struct BcUndoArgs {
   const BcOp* ops;
   int op_begin, op_end;
-  const double* src[5];
-  double* dst[5];
-  long long src_stride, dst_stride;   // member strides
+  double* q[5];
+  long long member_stride;
+  const double* undo;
+  long long undo_member_stride;
   int nx, ny;
 };
-void launch_bc_restore(const BcCopyArgs& a, long long nblocks, int nmembers, hipStream_t st);
+void launch_bc_undo(const BcUndoArgs& a, long long nblocks, int nmembers, hipStream_t st);
 
 // Tangent / adjoint of one wave of the boundary program (boundary.hip).
 struct BcLinArgs {
