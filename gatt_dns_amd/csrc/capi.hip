@@ -675,8 +675,9 @@ int compile_bc(gatt_ctx* c, const gatt_bc_desc& bc) {
 }
 
 // ------------------------------------------------------------- execution
-// applyBoundCond at time *clock + toff (the clock holds the t of the step in progress)
-void run_bc(gatt_ctx* c, double* q0, double toff) {
+// applyBoundCond at time *clock + toff (the clock holds the t of the step in progress);
+// with `undo`, every op point first saves its five values there (member stride undo_total)
+void run_bc(gatt_ctx* c, double* q0, double toff, double* undo = nullptr) {
   for (const BcWave& w : c->waves) {
     BcArgs a{};
     a.ops = c->d_ops;
@@ -696,6 +697,8 @@ void run_bc(gatt_ctx* c, double* q0, double toff) {
     a.capture = c->hold_captured ? 0 : 1;
     a.clock = c->d_clock;
     a.toff = toff;
+    a.undo = undo;
+    a.undo_member_stride = c->undo_total;
     launch_bc_wave(a, w.nblocks, c->batch, c->stream);
   }
   c->hold_captured = true;
@@ -1921,14 +1924,13 @@ enum : int {
   LN_D = 30,    // 15 first-derivative tangents / adjoints
   LN_T = 45,    // 9 stress and heat-flux tangents / adjoints
   LN_G = 54,    // 12 second-derivative tangents / adjoints
-  LN_P1 = 66,   // adjoint: the states before each boundary-condition application of the base
-  LN_P2 = 71,   //   step (after stages 1..4, and after the SFD)
-  LN_P3 = 76,
-  LN_P4 = 81,
-  LN_P5 = 86,
-  LN_TMP = 91,  // base state replayed through the boundary program
-  LN_N = 96
+  LN_B4 = 66,   // adjoint: the base step's state after the stage-4 boundary conditions
+  LN_N = 71
 };
+
+// the boundary programs of one base step write five undo logs (stages 1..4 and the
+// final application), each undo_total doubles per member
+constexpr int kUndoLogs = 5;
 
 int ensure_lin(gatt_ctx* c) {
   if (c->lin) return GATT_OK;
@@ -1938,7 +1940,7 @@ int ensure_lin(gatt_ctx* c) {
     return fail(GATT_E_NOMEM, "tangent/adjoint allocation failed");
   c->lin = static_cast<double*>(p);
   if (c->undo_total > 0) {
-    if (hipMalloc(&p, sizeof(double) * (size_t)c->undo_total * c->batch) != hipSuccess)
+    if (hipMalloc(&p, sizeof(double) * (size_t)kUndoLogs * c->undo_total * c->batch) != hipSuccess)
       return fail(GATT_E_NOMEM, "boundary undo log allocation failed");
     c->d_undo = static_cast<double*>(p);
     c->dev_allocs.push_back(p);
@@ -2337,80 +2339,31 @@ void run_bc_tl(gatt_ctx* c, double* q, long long qs, double* d, long long ds) {
     launch_bc_wave_tl(bc_lin_args(c, w, q, qs, d, ds), w.nblocks, c->batch, c->stream);
 }
 
-bool wave_needs_base(const gatt_ctx* c, const BcWave& w) {
-  for (int o = w.op_begin; o < w.op_end; ++o) {
-    const int k = c->ops[(size_t)o].kind;
-    if (k == BC_SET_R_FROM_P || k == BC_NEU_P || k == BC_CORNER) return true;
-  }
-  return false;
+// undo log k (0..4) of the base step's boundary applications
+double* undo_log(gatt_ctx* c, int k) {
+  return c->d_undo ? c->d_undo + (long long)k * c->undo_total * c->batch : nullptr;
 }
 
-// lam <- (d BC / d q at qpre)^T lam, waves in reverse order.  The base state before
-// a wave that depends on it comes from one forward replay of the waves on a copy of
-// qpre that logs the values each wave overwrites, undone wave by wave going back.
-void run_bc_adj(gatt_ctx* c, double* qpre, long long qs, double* lam) {
+// lam <- (d BC / d q)^T lam, waves in reverse order.  `post` holds the state after the
+// boundary application that wrote undo log `undo`; walking back, each wave's saved
+// values are put back first, so the wave's transpose sees the state before it.
+// `post` ends as the state before the application.
+void run_bc_adj(gatt_ctx* c, double* post, long long ps, double* lam, const double* undo) {
   const long long ls = c->lin_stride;
-  double* tmp = lin_ptr(c, LN_TMP);
-  const int nw = (int)c->waves.size();
-  int last = -1;   // the last wave whose transpose needs its base state
-  for (int wi = 0; wi < nw; ++wi)
-    if (wave_needs_base(c, c->waves[(size_t)wi])) last = wi;
-  if (last >= 0) {
-    AxpbyArgs cp{};
-    cp.np = c->np;
-    cp.x_stride = qs;
-    cp.y_stride = ls;
-    cp.nf = 5;
-    for (int f = 0; f < 5; ++f) {
-      cp.x[f] = qpre + (long long)f * c->np;
-      cp.y[f] = tmp + (long long)f * c->np;
-    }
-    cp.a = 1.0;
-    cp.b = 0.0;
-    launch_axpby(cp, c->batch, c->stream);
-    for (int v = 0; v < last; ++v) {
-      const BcWave& pw = c->waves[(size_t)v];
-      BcArgs a{};
-      a.ops = c->d_ops;
-      a.op_begin = pw.op_begin;
-      a.op_end = pw.op_end;
-      for (int f = 0; f < 5; ++f) a.q[f] = tmp + (long long)f * c->np;
-      a.member_stride = ls;
-      a.coef[0] = c->d_coef[0];
-      a.coef[1] = c->d_coef[1];
-      a.aux = c->d_aux;
-      a.hold = c->d_hold;
-      a.hold_member_stride = c->hold_stride;
-      a.nx = c->nx;
-      a.ny = c->ny;
-      a.nz = c->nz;
-      a.gamma = c->gamma;
-      a.capture = 0;
-      a.clock = c->d_clock;
-      a.undo = c->d_undo;
-      a.undo_member_stride = c->undo_total;
-      launch_bc_wave(a, pw.nblocks, c->batch, c->stream);
-    }
-  }
-  // tmp now holds the base state before wave `last`
-  for (int wi = nw - 1; wi >= 0; --wi) {
+  for (int wi = (int)c->waves.size() - 1; wi >= 0; --wi) {
     const BcWave& w = c->waves[(size_t)wi];
-    if (wi < last) {
-      BcUndoArgs u{};
-      u.ops = c->d_ops;
-      u.op_begin = w.op_begin;
-      u.op_end = w.op_end;
-      for (int f = 0; f < 5; ++f) u.q[f] = tmp + (long long)f * c->np;
-      u.member_stride = ls;
-      u.undo = c->d_undo;
-      u.undo_member_stride = c->undo_total;
-      u.nx = c->nx;
-      u.ny = c->ny;
-      launch_bc_undo(u, w.nblocks, c->batch, c->stream);
-    }
-    const bool based = wi <= last && wave_needs_base(c, w);
-    launch_bc_wave_adj(bc_lin_args(c, w, based ? tmp : qpre, based ? ls : qs, lam, ls), w.nblocks, c->batch,
-                       c->stream);
+    BcUndoArgs u{};
+    u.ops = c->d_ops;
+    u.op_begin = w.op_begin;
+    u.op_end = w.op_end;
+    for (int f = 0; f < 5; ++f) u.q[f] = post + (long long)f * c->np;
+    u.member_stride = ps;
+    u.undo = undo;
+    u.undo_member_stride = c->undo_total;
+    u.nx = c->nx;
+    u.ny = c->ny;
+    launch_bc_undo(u, w.nblocks, c->batch, c->stream);
+    launch_bc_wave_adj(bc_lin_args(c, w, post, ps, lam, ls), w.nblocks, c->batch, c->stream);
   }
 }
 
@@ -2492,35 +2445,29 @@ void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
   const long long ts = c->tape_stride;
   const int nb = c->batch;
   double* qn = c->state(1 - c->cur);     // scratch for the last stage; the base q is not modified
-  double* P1 = lin_ptr(c, LN_P1);
-  double* P2 = lin_ptr(c, LN_P2);
-  double* P3 = lin_ptr(c, LN_P3);
-  double* P4 = lin_ptr(c, LN_P4);
-  double* P5 = lin_ptr(c, LN_P5);
+  double* B4 = lin_ptr(c, LN_B4);
+  // every boundary application logs what it overwrites, for the transposed program
   base_dtg(c, q, sl, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
   forward_assemble(c, q, q, S, 1, dt, 0, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
-  fields_axpby(c, P1, ls, S, sl, 1.0, 0.0);
-  run_bc(c, S, dt / 2.0);
+  run_bc(c, S, dt / 2.0, undo_log(c, 0));
   fields_axpby(c, B1, ls, S, sl, 1.0, 0.0);
   base_dtg(c, S, sl, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
   forward_assemble(c, S, q, S, 2, dt, 0, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
-  fields_axpby(c, P2, ls, S, sl, 1.0, 0.0);
-  run_bc(c, S, dt / 2.0);
+  run_bc(c, S, dt / 2.0, undo_log(c, 1));
   fields_axpby(c, B2, ls, S, sl, 1.0, 0.0);
   base_dtg(c, S, sl, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
   forward_assemble(c, S, q, S, 3, dt, 0, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
-  fields_axpby(c, P3, ls, S, sl, 1.0, 0.0);
-  run_bc(c, S, dt);
+  run_bc(c, S, dt, undo_log(c, 2));
   fields_axpby(c, B3, ls, S, sl, 1.0, 0.0);
   base_dtg(c, S, sl, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
   forward_assemble(c, S, q, qn, 4, dt, 0, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
-  fields_axpby(c, P4, ls, qn, sl, 1.0, 0.0);
-  run_bc(c, qn, dt);
+  run_bc(c, qn, dt, undo_log(c, 3));
+  fields_axpby(c, B4, ls, qn, sl, 1.0, 0.0);
   run_filters(c, qn, dt / c->filterCharTime);
   run_sfd(c, qn, dt);
-  fields_axpby(c, P5, ls, qn, sl, 1.0, 0.0);
-  // the final boundary conditions (main.F90:405)
-  run_bc_adj(c, P5, ls, QB);
+  run_bc(c, qn, dt, undo_log(c, 4));
+  // the final boundary conditions (main.F90:405), walked back on qn
+  run_bc_adj(c, qn, sl, QB, undo_log(c, 4));
   // SFD with a frozen mean is diagonal
   if (c->sfdType > 0) {
     SfdArgs sa{};
@@ -2554,19 +2501,19 @@ void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
                 0, c->batch, ls, ls);
   }
   // boundary conditions after the last stage
-  run_bc_adj(c, P4, ls, QB);
+  run_bc_adj(c, B4, ls, QB, undo_log(c, 3));
   // RK4 (the accumulator form of timeIntegration.F90:382-496), backwards, with the
   // transposed boundary conditions after every stage
   fields_axpby(c, KB, ls, QB, ls, dt / 6.0, 0.0);
   adj_rhs(c, B3, ls, KB, SB, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
-  run_bc_adj(c, P3, ls, SB);
+  run_bc_adj(c, B3, ls, SB, undo_log(c, 2));   // B3 is not read again as a state
   // B3 now accumulates the stage adjoints; KB = dt/3 QB + dt SB
   adj_combine(c, QB, SB, B3, KB, dt / 3.0, dt, 1, 0);
   adj_rhs(c, B2, ls, KB, SB, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
-  run_bc_adj(c, P2, ls, SB);
+  run_bc_adj(c, B2, ls, SB, undo_log(c, 1));   // B2 is not read again as a state
   adj_combine(c, QB, SB, B3, KB, dt / 3.0, dt / 2.0, 0, 0);
   adj_rhs(c, B1, ls, KB, SB, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
-  run_bc_adj(c, P1, ls, SB);
+  run_bc_adj(c, B1, ls, SB, undo_log(c, 0));   // B1 is not read again as a state
   adj_combine(c, QB, SB, B3, KB, dt / 6.0, dt / 2.0, 0, 0);
   adj_rhs(c, q, sl, KB, SB, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
   // QB = SB + (B3 + QB)
