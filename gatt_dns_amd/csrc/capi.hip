@@ -2219,6 +2219,7 @@ void run_solve_t(gatt_ctx* c, int dir, int kind, int nf, double* const* in, doub
   a.coef = d.coef[kind];
   a.coef_ends = d.coef_ends[kind];
   a.uniform = d.uniform_tiles ? 1 : 0;
+  if (launch_line_solve_band_t(a, b, nm, c->max_lds, c->stream)) return;
   launch_line_solve_t(a, nm, c->stream);
   launch_band_t(b, nm, c->stream);
 }

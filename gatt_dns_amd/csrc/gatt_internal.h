@@ -247,6 +247,11 @@ void launch_record(const unsigned long long* red, double* rec, int* flag, int nm
 // Transposed Thomas sweeps of one direction (line_solve_t.hip); in may equal out.
 void launch_line_solve_t(const SolveArgs& a, int nmembers, hipStream_t st);
 int line_solve_t_points_per_lane(int N);
+struct BandTArgs;
+// y and z: the transposed solve and the transposed band in one launch; false when the
+// line does not qualify (the caller then runs launch_line_solve_t and launch_band_t)
+bool launch_line_solve_band_t(const SolveArgs& a, const BandTArgs& b, int nmembers, long long max_lds,
+                              hipStream_t st);
 
 // Transposed RHS band along one direction:
 //   out = beta*src + (accumulate ? out : 0) + alpha * R^T in
