@@ -189,32 +189,6 @@ void launch_bc_wave(const BcArgs& a, long long nblocks, int nmembers, hipStream_
   hipLaunchKernelGGL(bc_wave_kernel, dim3((unsigned)nblocks, nmembers), dim3(256), 0, st, a);
 }
 
-// the five field values one wave's ops saved before writing, put back on their points
-// (the transposed boundary program walks the waves backwards with it)
-__global__ __launch_bounds__(256) void bc_undo_kernel(BcUndoArgs a) {
-  const long long blk = blockIdx.x;
-  int o = a.op_begin;
-  while (o + 1 < a.op_end && a.ops[o + 1].first_block <= blk) ++o;
-  const BcOp& op = a.ops[o];
-  const long long p = (blk - op.first_block) * blockDim.x + threadIdx.x;
-  if (p >= op.npts) return;
-  const int bi = op.box[1] - op.box[0] + 1;
-  const int bj = op.box[3] - op.box[2] + 1;
-  const int i = op.box[0] + (int)(p % bi);
-  const int j = op.box[2] + (int)((p / bi) % bj);
-  const int k = op.box[4] + (int)(p / ((long long)bi * bj));
-  const long long n = pidx(i, j, k, a.nx, a.ny);
-  const long long mo = (long long)blockIdx.y * a.member_stride;
-  const double* u = a.undo + (long long)blockIdx.y * a.undo_member_stride + op.undo;
-#pragma unroll
-  for (int f = 0; f < 5; ++f) a.q[f][mo + n] = u[(long long)f * op.npts + p];
-}
-
-void launch_bc_undo(const BcUndoArgs& a, long long nblocks, int nmembers, hipStream_t st) {
-  if (nblocks <= 0) return;
-  hipLaunchKernelGGL(bc_undo_kernel, dim3((unsigned)nblocks, nmembers), dim3(256), 0, st, a);
-}
-
 }  // namespace gatt
 
 namespace gatt {
@@ -384,6 +358,14 @@ __global__ __launch_bounds__(256) void bc_wave_adj_kernel(BcLinArgs a) {
   const BcOp& op = *opp;
   const int nx = a.nx, ny = a.ny;
   const long long mo = (long long)blockIdx.y * a.member_stride, lo = (long long)blockIdx.y * a.lin_stride;
+  if (a.undo) {
+    // the wave wrote only its own points, each from one op: putting this point back
+    // makes the base the state before the wave for every read of this thread and of
+    // the other ops (which read no point the wave writes)
+    const double* u = a.undo + (long long)blockIdx.y * a.undo_member_stride + op.undo;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) (a.q[f] + mo)[pt.n] = u[(long long)f * op.npts + pt.p];
+  }
   const double* R = a.q[3] + mo;
   const double* E = a.q[4] + mo;
   double* bR = a.dq[3] + lo;

@@ -2353,18 +2353,10 @@ void run_bc_adj(gatt_ctx* c, double* post, long long ps, double* lam, const doub
   const long long ls = c->lin_stride;
   for (int wi = (int)c->waves.size() - 1; wi >= 0; --wi) {
     const BcWave& w = c->waves[(size_t)wi];
-    BcUndoArgs u{};
-    u.ops = c->d_ops;
-    u.op_begin = w.op_begin;
-    u.op_end = w.op_end;
-    for (int f = 0; f < 5; ++f) u.q[f] = post + (long long)f * c->np;
-    u.member_stride = ps;
-    u.undo = undo;
-    u.undo_member_stride = c->undo_total;
-    u.nx = c->nx;
-    u.ny = c->ny;
-    launch_bc_undo(u, w.nblocks, c->batch, c->stream);
-    launch_bc_wave_adj(bc_lin_args(c, w, post, ps, lam, ls), w.nblocks, c->batch, c->stream);
+    BcLinArgs la = bc_lin_args(c, w, post, ps, lam, ls);
+    la.undo = undo;
+    la.undo_member_stride = c->undo_total;
+    launch_bc_wave_adj(la, w.nblocks, c->batch, c->stream);
   }
 }
 

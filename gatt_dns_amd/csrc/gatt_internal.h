@@ -457,17 +457,6 @@ void launch_bc_wave(const BcArgs& a, long long nblocks, int nmembers, hipStream_
 void launch_clock_set(double* clock, double t, hipStream_t st);
 void launch_clock_add(double* clock, double dt, hipStream_t st);
 
-// puts back the values the ops of one wave saved in the undo log (BcArgs::undo)
-struct BcUndoArgs {
-  const BcOp* ops;
-  int op_begin, op_end;
-  double* q[5];
-  long long member_stride;
-  const double* undo;
-  long long undo_member_stride;
-  int nx, ny;
-};
-void launch_bc_undo(const BcUndoArgs& a, long long nblocks, int nmembers, hipStream_t st);
 
 // Tangent / adjoint of one wave of the boundary program (boundary.hip).
 struct BcLinArgs {
@@ -483,6 +472,10 @@ struct BcLinArgs {
   long long hold_member_stride;
   int nx, ny, nz;
   double gamma;
+  // adjoint only: when set, every op point first puts back the five values its wave
+  // overwrote (undo log of BcArgs::undo), so the base is the state before the wave
+  const double* undo;
+  long long undo_member_stride;
 };
 void launch_bc_wave_tl(const BcLinArgs& a, long long nblocks, int nmembers, hipStream_t st);
 void launch_bc_wave_adj(const BcLinArgs& a, long long nblocks, int nmembers, hipStream_t st);
