@@ -512,12 +512,12 @@ __global__ __launch_bounds__(256) void assemble_adj_kernel(AssembleLinArgs a) {
   DVyb += divb;
   DWzb += divb;
 
-  double* Sb = a.Sbar + lo;
-  Sb[0 * np + n] += Ub;
-  Sb[1 * np + n] += Vb;
-  if (a.is3d) Sb[2 * np + n] += Wb;
-  Sb[3 * np + n] += Rb;
-  Sb[4 * np + n] += Eb;
+  double* Sb = a.Sbar + lo;   // the stage-state adjoint starts here (later kernels accumulate)
+  Sb[0 * np + n] = Ub;
+  Sb[1 * np + n] = Vb;
+  if (a.is3d) Sb[2 * np + n] = Wb;
+  Sb[3 * np + n] = Rb;
+  Sb[4 * np + n] = Eb;
   double* Db = a.Dbar + lo;
   Db[0 * np + n] = DUxb; Db[1 * np + n] = DUyb; Db[2 * np + n] = DUzb;
   Db[3 * np + n] = DVxb; Db[4 * np + n] = DVyb; Db[5 * np + n] = DVzb;

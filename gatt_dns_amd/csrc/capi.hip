@@ -2238,21 +2238,7 @@ void adj_rhs(gatt_ctx* c, const double* S, long long ss, double* kbar, double* s
     double* Db = lin_ptr(c, LN_D) + lo;
     double* Tb = lin_ptr(c, LN_T) + lo;
     double* Gb = lin_ptr(c, LN_G) + lo;
-    double* sb = sbar + lo;
-    {
-      AxpbyArgs z{};
-      z.np = np;
-      z.x_stride = ls;
-      z.y_stride = ls;
-      for (int f : state_fields(c)) {
-        z.x[z.nf] = sb + (long long)f * np;
-        z.y[z.nf] = sb + (long long)f * np;
-        ++z.nf;
-      }
-      z.a = 0.0;
-      z.b = 0.0;
-      launch_axpby(z, nm, c->stream);
-    }
+    double* sb = sbar + lo;   // set (not accumulated) by the point-wise adjoint below
     AssembleLinArgs a{};
     a.c = c->fc;
     a.np = np;

@@ -298,7 +298,7 @@ void launch_visc_adj(const ViscLinArgs& a, int nmembers, hipStream_t st);
 
 // Tangent of the RHS assembly (dd from dS, dD, dT, dG at base S, D, T, G) fused
 // with the linear RK stage update of the tangent state; adjoint: from dbar,
-// accumulates Sbar and writes Dbar, Tbar (stresses; heat fluxes zero) and Gbar.
+// sets Sbar and writes Dbar, Tbar (stresses; heat fluxes zero) and Gbar.
 struct AssembleLinArgs {
   FlowConst c;
   long long np, member_stride, work_stride, lin_stride;
@@ -309,7 +309,7 @@ struct AssembleLinArgs {
   int stage; double dt;
   // adjoint
   const double* dbar;   // 5 arrays (lin stride)
-  double* Sbar;         // 5 arrays, accumulated (lin stride)
+  double* Sbar;         // 5 arrays, set (lin stride)
   double* Dbar; double* Tbar; double* Gbar;   // written (lin stride)
   int is3d;
 };
