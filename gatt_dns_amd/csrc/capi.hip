@@ -2466,12 +2466,18 @@ void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
   const int ndir = c->is3d ? 3 : 2;
   for (int d = ndir - 1; d >= 0; --d) {
     if (!c->filterDir[d]) continue;
-    fields_axpby(c, KB, ls, QB, ls, 1.0, 0.0);
+    // the one-launch transposed solves read their whole input before writing, so the
+    // filter can act in place; the two-launch path gets a copy
+    const LineOpDev& fo = c->dir[d].op[GATT_FILTER];
+    const bool in_place = d == 0 ? (fo.N <= 512 && fo.nR >= 2 && fo.nR <= 11 && fo.nR - 1 <= fo.N)
+                                 : line_solve_band_t_fits(d, fo.N, fo.nR, (int)state_fields(c).size(), c->batch,
+                                                          c->max_lds);
+    if (!in_place) fields_axpby(c, KB, ls, QB, ls, 1.0, 0.0);
     double* in[kMaxPtr];
     double* out[kMaxPtr];
     int nf = 0;
     for (int f : state_fields(c)) {
-      in[nf] = KB + (long long)f * c->np;
+      in[nf] = (in_place ? QB : KB) + (long long)f * c->np;
       out[nf] = QB + (long long)f * c->np;
       ++nf;
     }

@@ -252,6 +252,7 @@ struct BandTArgs;
 // line does not qualify (the caller then runs launch_line_solve_t and launch_band_t)
 bool launch_line_solve_band_t(const SolveArgs& a, const BandTArgs& b, int nmembers, long long max_lds,
                               hipStream_t st);
+bool line_solve_band_t_fits(int dir, int N, int nR, int nf, int nmembers, long long max_lds);
 
 // Transposed RHS band along one direction:
 //   out = beta*src + (accumulate ? out : 0) + alpha * R^T in

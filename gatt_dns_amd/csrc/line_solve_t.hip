@@ -408,16 +408,21 @@ static bool launch_band_t_nr(const SolveArgs& a, const BandTArgs& b, int nmember
   return false;
 }
 
-bool launch_line_solve_band_t(const SolveArgs& a, const BandTArgs& b, int nmembers, long long max_lds,
-                              hipStream_t st) {
-  if (a.dir != 1 && a.dir != 2) return false;
-  const int N = a.op.N, H = a.op.nR - 1;
+bool line_solve_band_t_fits(int dir, int N, int nR, int nf, int nmembers, long long max_lds) {
+  if (dir != 1 && dir != 2) return false;
+  const int H = nR - 1;
   const int M = line_solve_t_points_per_lane(N);
   if (M == 0 || H < 1 || H > 5 || H > M) return false;
   const int G = (N + M - 1) / M;
   if (N - (G - 1) * M < H) return false;                  // the last segment must hold a halo
-  if ((long long)line_solve_band_t_lds_bytes(N, M, a.op.nR) > max_lds) return false;
-  if ((long long)a.nf * nmembers > 65535) return false;
+  if ((long long)line_solve_band_t_lds_bytes(N, M, nR) > max_lds) return false;
+  return (long long)nf * nmembers <= 65535;
+}
+
+bool launch_line_solve_band_t(const SolveArgs& a, const BandTArgs& b, int nmembers, long long max_lds,
+                              hipStream_t st) {
+  if (!line_solve_band_t_fits(a.dir, a.op.N, a.op.nR, a.nf, nmembers, max_lds)) return false;
+  const int M = line_solve_t_points_per_lane(a.op.N);
   if (a.dir == 1) return M == 16 ? launch_band_t_nr<1, 16>(a, b, nmembers, st) : launch_band_t_nr<1, 32>(a, b, nmembers, st);
   return M == 16 ? launch_band_t_nr<2, 16>(a, b, nmembers, st) : launch_band_t_nr<2, 32>(a, b, nmembers, st);
 }
