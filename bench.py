@@ -128,7 +128,7 @@ def ragged_sweep(runtime, steps=20):
             "speedup_vs_sequential": round(max(res["group_mode0"], res["group_mode1"]) / res["sequential"], 3)}
 
 
-def adjoint_cost(runtime, nx=512, ny=192, nz=64, steps=6):
+def adjoint_cost(runtime, nx=512, ny=192, nz=64, steps=6, only_adjoint=False):
     """Tangent and adjoint step cost against the plain step on the adjoint configuration: the
     spanwise-periodic boundary layer 512x192x64 (the 2-D boundary layer extruded in z; walls, buffers,
     SFD with a frozen mean in the buffers), Blasius base with seeded noise."""
@@ -170,6 +170,20 @@ def adjoint_cost(runtime, nx=512, ny=192, nz=64, steps=6):
         prop.solver.sync()
         res["adjoint" + tag] = (time.perf_counter() - t0) / steps
 
+    if only_adjoint:
+        # one warm-up adjoint step, then `steps` timed ones (the profiled form)
+        prop.load(q0)
+        prop.load_perturbation(v)
+        prop.solver.adj_step(0.0, prop.dt)
+        prop.solver.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            prop.solver.adj_step(0.0, prop.dt)
+        prop.solver.sync()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        prop.close()
+        return {"workload": "spanwise-periodic boundary layer %dx%dx%d adjoint steps" % (nx, ny, nz),
+                "steps": steps, "adjoint_ms_per_step": round(ms, 3)}
     measure("")
     # the same with the line-solve shapes picked by the autotuner for this case
     from gatt_dns_amd import autotune
@@ -199,6 +213,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-lst", action="store_true", help="skip the batched-member (stability analysis) lines")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the tangent/adjoint cost line")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only time --steps adjoint steps of the boundary layer (prints their JSON; for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -214,6 +230,9 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.adjoint_only:
+        print(json.dumps(adjoint_cost(runtime, steps=args.steps, only_adjoint=True)), flush=True)
+        return
     log("preparing turbulence box %d^3" % args.n)
     case = cases.turbulence_box(args.n)
     s = runtime.Solver(case)
