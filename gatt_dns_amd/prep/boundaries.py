@@ -252,8 +252,11 @@ def poiseuille_flow(nx, ny, nz, E0, P0, lower_wall_velocity=0.0, upper_wall_velo
     return fr, walls, corners, bc
 
 
-def boundary_layer_isothermal(X, Y, Z, E0, P0, z_periodic=True, t_wall_relative=None):
-    """BC list of ``source/boundaries/boundaryLayerIsothermal.m`` (no cavities or roughness).
+def boundary_layer_isothermal(X, Y, Z, E0, P0, z_periodic=True, t_wall_relative=None, adiabatic=False):
+    """BC list of ``source/boundaries/boundaryLayerIsothermal.m`` (no cavities or roughness);
+    ``adiabatic=True`` gives ``boundaryLayerAdiabatic.m``: zero-gradient E on every wall
+    box beside the pressure condition, adiabatic corners (``:363``), E0 inside the walls,
+    and that file's z wall limits (``:387-392``: the left limits carry 'zi', the right 'zf').
 
     Returns ``(flow_region, walls, corners, bc, x_break_point, hold_inlet)``:
     ``x_break_point`` is the row ``[position, a1, a2, b1, b2]`` of ``mesh.x.breakPoint``
@@ -292,13 +295,18 @@ def boundary_layer_isothermal(X, Y, Z, E0, P0, z_periodic=True, t_wall_relative=
             w[0] = max(w[0], i0)
         walls["up"].append([1, i_last, j0, j0, 1, nz])
         bp = [i_last, j0, j0, 1, nz]
-    for key, d in (("front", "xi"), ("back", "xf"), ("up", "yi"), ("down", "yf"), ("right", "zi"), ("left", "zf")):
+    zwalls = (("left", "zi"), ("right", "zf")) if adiabatic else (("right", "zi"), ("left", "zf"))
+    for key, d in (("front", "xi"), ("back", "xf"), ("up", "yi"), ("down", "yf")) + zwalls:
         for box in walls[key]:                            # :371-402
             bc.add("p", "neu", d, 0.0, box)
+            if adiabatic:                                 # boundaryLayerAdiabatic.m:407-418
+                bc.add("e", "neu", d, 0.0, box)
+    if adiabatic:
+        corners["adiabatic"] = [1] * len(corners["limits"])
     if X[0] < 0:                                          # free slip upstream of the plate, :404-432
         bc.add("u", "neu", "yi", 0.0, walls["up"][-1])
         bc.add("w", "neu", "yi", 0.0, walls["up"][-1])
-    e_wall = E0 * t_wall_relative if t_wall_relative is not None else E0
+    e_wall = E0 * t_wall_relative if t_wall_relative is not None and not adiabatic else E0
     for box in inside:                                    # :434-492
         bc.add("p", "dir", "yi", P0, box)
         bc.add("u", "dir", "yi", 0.0, box)

@@ -88,6 +88,24 @@ def test_steps_small(rt, oracle_mod, small, nsteps, tol):
     assert dv == dv_w
 
 
+def test_adiabatic_steps(rt, oracle_mod):
+    """boundaryLayerAdiabatic.m walls (zero-gradient E, adiabatic corners): 10 steps."""
+    case = cases.boundary_layer(nx=160, ny=64, nbuf=12, adiabatic=True)
+    print("adiabatic boundary layer 160x64", flush=True)
+    o = oracle_mod.Oracle(case)
+    q0 = o.apply_bound_cond(case.state0, 0.0)
+    o.init_mean(q0)
+    want, _, dv_w = o.run(q0, 0.0, case.dt, 10)
+    s = rt.Solver(case)
+    s.set_state(case.state0)
+    s.apply_bound_cond(0.0)
+    mc, dv = s.advance(10, 0.0, case.dt)
+    got = s.get_state()
+    assert dv == dv_w
+    for k in FIELDS:
+        assert rel_err(got[k], want[k]) <= 1e-10, (k, rel_err(got[k], want[k]))
+
+
 # V is O(1e-3) here: after 3 steps its largest difference (2e-14 absolute, in the first row above
 # the wall) is 3e-11 of max|V| while U, R and E stay at 1e-13 of theirs
 @pytest.mark.parametrize("nsteps,tol", [(1, 1e-12), (3, 1e-10)])

@@ -70,3 +70,26 @@ def test_steps_stay_bounded(bl, oracle_mod):
     st, mc, dv = o.run(q0, 0.0, bl.dt, 20)
     assert not dv and np.all(np.isfinite(st["U"]))
     assert np.abs(st["U"]).max() < 1.5
+
+
+def test_adiabatic_wall(oracle_mod):
+    """boundaryLayerAdiabatic.m: the wall carries zero-gradient E (one-sided extrapolation of
+    the neumann order) instead of E = E0; no slip and zero wall-normal pressure gradient stay."""
+    bl = cases.boundary_layer(nx=160, ny=64, nbuf=12, adiabatic=True)
+    o = oracle_mod.Oracle(bl)
+    rng = np.random.default_rng(4)
+    q = {k: np.asfortranarray(v + 0.01 * rng.standard_normal(v.shape)) for k, v in bl.state0.items()}
+    q["W"][:] = 0.0
+    st = o.apply_bound_cond(q, 0.0)
+    X = np.asarray(bl.X)
+    c = np.asarray(bl.bi.neumann)
+    L = len(c)
+    plate = X >= 0
+    extE = sum(c[k] * st["E"][:, 1 + k, 0] for k in range(L))
+    np.testing.assert_allclose(st["E"][1:-1, 0, 0], extE[1:-1], rtol=1e-13)
+    assert not np.allclose(st["E"][1:-1, 0, 0], bl.E0)          # not the isothermal value
+    assert np.all(st["U"][plate, 0, 0] == 0.0) and np.all(st["V"][:, 0, 0] == 0.0)
+    P = (bl.gamma - 1) * st["R"] * st["E"]
+    extP = sum(c[k] * P[1:-1, 1 + k, 0] for k in range(L))
+    np.testing.assert_allclose(P[1:-1, 0, 0], extP, rtol=1e-12)
+    assert np.all(bl.bi.cAdiabatic == 1)
