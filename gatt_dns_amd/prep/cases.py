@@ -18,6 +18,7 @@ import numpy as np
 from .boundaries import (BoundaryInfo, boundary_layer_isothermal, calc_sfd_region, disturbance_index, init_boundaries,
                          lid_driven_flow, periodic_box, poiseuille_flow)
 from .matrices import BufferSide, DirMesh, LineCoeffs, NumMethods, fortran_round, make_matrices, prepare_thomas
+from .mesh import MeshSpec, generate_mesh
 
 FIELDS = ("U", "V", "W", "R", "E")
 
@@ -246,14 +247,6 @@ def periodic_rest(nx, ny, nz=1, Re=1000.0, Ma=0.3, Pr=0.71, gamma=1.4, T0=300.0,
     return case
 
 
-def tanh_mesh_both(xi, xf, n, par):
-    """``generateMesh.m:103-121``, type 'tanh', local 'b' (refined at both ends)."""
-    eta = np.tanh(np.linspace(-par, par, n))
-    eta = eta / eta[-1]
-    eta = (eta + 1) / 2
-    return (xf - xi) * eta + xi
-
-
 def poiseuille(nx=64, ny=48, nz=1, Re=2000.0, Ma=0.5, Pr=0.71, gamma=1.4, T0=300.0, L=4.0, H=1.0,
                dpdx=None, alpha_f=0.49, spatial="SL6", cfl=0.5, profile=True, disturb=(), y_mesh="tanh"):
     """Plane channel of ``examples/poiseuille.m``: x periodic (uniform, length L), y on
@@ -268,7 +261,8 @@ def poiseuille(nx=64, ny=48, nz=1, Re=2000.0, Ma=0.5, Pr=0.71, gamma=1.4, T0=300
     """
     dpdx = 16.0 / Re if dpdx is None else dpdx
     X = np.arange(nx) / nx * L
-    Y = tanh_mesh_both(-H / 2, H / 2, ny, 2.0) if y_mesh == "tanh" else np.linspace(-H / 2, H / 2, ny)
+    Y = generate_mesh(-H / 2, H / 2, MeshSpec(n=ny, type="tanh", local="b", par=2.0)) if y_mesh == "tanh" \
+        else np.linspace(-H / 2, H / 2, ny)
     Z = np.arange(nz) / nz if nz > 1 else np.array([0.0])
     meshes = [DirMesh(X, periodic=True, fix_periodic_domain_size=True, xf=L), DirMesh(Y),
               DirMesh(Z, periodic=True, fix_periodic_domain_size=True, xf=1.0)]
