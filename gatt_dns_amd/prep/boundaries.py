@@ -252,11 +252,14 @@ def poiseuille_flow(nx, ny, nz, E0, P0, lower_wall_velocity=0.0, upper_wall_velo
     return fr, walls, corners, bc
 
 
-def boundary_layer_isothermal(X, Y, Z, E0, P0, z_periodic=True, t_wall_relative=None, adiabatic=False):
+def boundary_layer_isothermal(X, Y, Z, E0, P0, z_periodic=True, t_wall_relative=None, adiabatic=False,
+                              z_symmetry=False):
     """BC list of ``source/boundaries/boundaryLayerIsothermal.m`` (no cavities or roughness);
     ``adiabatic=True`` gives ``boundaryLayerAdiabatic.m``: zero-gradient E on every wall
     box beside the pressure condition, adiabatic corners (``:363``), E0 inside the walls,
     and that file's z wall limits (``:387-392``: the left limits carry 'zi', the right 'zf').
+    ``z_symmetry=True`` gives the z ends of ``boundaryLayerIsothermalSymmZ.m:205-326``:
+    zero-gradient u, v, e, p and w = 0 on both z planes.
 
     Returns ``(flow_region, walls, corners, bc, x_break_point, hold_inlet)``:
     ``x_break_point`` is the row ``[position, a1, a2, b1, b2]`` of ``mesh.x.breakPoint``
@@ -278,7 +281,11 @@ def boundary_layer_isothermal(X, Y, Z, E0, P0, z_periodic=True, t_wall_relative=
     bc.add("p", "dir", "xf", P0, full(nx, nx, 1, ny, 1, nz))
     for var in "uvwep":                                   # top, :143-202
         bc.add(var, "sec", "yf", 0.0, full(1, nx, ny, ny, 1, nz))
-    if nz > 1 and not z_periodic:                         # spanwise walls, :204-326
+    if nz > 1 and z_symmetry:                             # symmetry planes, SymmZ.m:205-326
+        for d, k in (("zi", 1), ("zf", nz)):
+            for var in "uvwep":
+                bc.add(var, "dir" if var == "w" else "neu", d, 0.0, full(1, nx, 1, ny, k, k))
+    elif nz > 1 and not z_periodic:                       # spanwise walls, :204-326
         for d, k in (("zi", 1), ("zf", nz)):
             for var in "uvwep":
                 bc.add(var, "neu", d, 0.0, full(1, nx, 1, ny, k, k))

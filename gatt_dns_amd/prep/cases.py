@@ -349,11 +349,12 @@ def blasius(eta_max=20.0, n=4001):
 def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wall_ratio=1.02, Re=1000.0,
                    Ma=0.5, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0.49, spatial="SL4", cfl=0.5, sfd_chi=0.02,
                    sfd_delta=10.0, noise=1e-5, seed=20261, buffer_growth=8.0, nz=1, z_length=4.0,
-                   adiabatic=False):
+                   adiabatic=False, z_symmetry=False):
     """Isothermal-wall boundary layer (boundaryLayerIsothermal.m; ``adiabatic=True``: the walls of
     boundaryLayerAdiabatic.m) on a stretched mesh with buffers.
 
-    With nz > 1 the 2-D configuration is extruded over a spanwise-periodic uniform z of length
+    With nz > 1 the 2-D configuration is extruded (``z_symmetry``: between the symmetry planes of
+    boundaryLayerIsothermalSymmZ.m on [0, z_length]; otherwise) over a spanwise-periodic uniform z of length
     z_length (the flow region, boundary boxes and initial profiles are the same in every plane).
 
     The mesh is built here (a 'file' mesh for the reference): x uniform on x_range with nbuf-point
@@ -373,13 +374,19 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
     Yp = np.concatenate([[0.0], np.cumsum(dy0 * wall_ratio ** np.arange(npy - 1))])
     Yp[-1] = y_top
     Y = np.concatenate([Yp, _buffer_points(Yp[-1], Yp[-1] - Yp[-2], nbuf, buffer_growth, 1.0)])
-    Z = np.arange(nz) * (z_length / nz) if nz > 1 else np.array([0.0])
+    z_sym = z_symmetry and nz > 1
+    if z_sym:      # symmetry planes at both ends of a uniform z on [0, z_length]
+        Z = np.linspace(0.0, z_length, nz)
+    else:
+        Z = np.arange(nz) * (z_length / nz) if nz > 1 else np.array([0.0])
     E0 = 1.0 / ((gamma ** 2 - gamma) * Ma ** 2)
     P0 = (gamma - 1) * E0
-    fr, walls, corners, bc, bp, hold = boundary_layer_isothermal(X, Y, Z, E0, P0, adiabatic=adiabatic)
+    fr, walls, corners, bc, bp, hold = boundary_layer_isothermal(X, Y, Z, E0, P0, adiabatic=adiabatic,
+                                                                 z_periodic=not z_sym, z_symmetry=z_sym)
     buf = BufferSide(n=nbuf, upwind=False, transition=0.2)
     meshes = [DirMesh(X, buffer_i=buf, buffer_f=buf, break_points=[bp] if bp else []),
               DirMesh(Y, buffer_f=buf),
+              DirMesh(Z) if z_sym else
               DirMesh(Z, periodic=True, fix_periodic_domain_size=True, xf=z_length if nz > 1 else 1.0)]
     nm = NumMethods(spatial_derivs=spatial, spatial_derivs_buffer="EX4", filter_strength=alpha_f,
                     filter_borders="reducedOrder", filter_borders_start=(False, True, True),

@@ -106,6 +106,26 @@ def test_adiabatic_steps(rt, oracle_mod):
         assert rel_err(got[k], want[k]) <= 1e-10, (k, rel_err(got[k], want[k]))
 
 
+def test_z_symmetry_steps(rt, oracle_mod):
+    """boundaryLayerIsothermalSymmZ.m: symmetry planes at both z ends (non-periodic z). 4 steps."""
+    case = cases.boundary_layer(nx=64, ny=40, nbuf=8, nz=12, z_symmetry=True)
+    print("symmetric-z boundary layer 64x40x12", flush=True)
+    o = oracle_mod.Oracle(case)
+    q0 = o.apply_bound_cond(case.state0, 0.0)
+    o.init_mean(q0)
+    want, _, dv_w = o.run(q0, 0.0, case.dt, 4)
+    s = rt.Solver(case)
+    s.set_state(case.state0)
+    s.apply_bound_cond(0.0)
+    mc, dv = s.advance(4, 0.0, case.dt)
+    got = s.get_state()
+    assert dv == dv_w
+    errs = {k: rel_err(got[k], want[k]) for k in "UVRE"}
+    errs["W"] = float(np.max(np.abs(got["W"] - want["W"])) / np.max(np.abs(want["U"])))
+    for k, e in errs.items():
+        assert e <= 1e-10, (k, e)
+
+
 # V is O(1e-3) here: after 3 steps its largest difference (2e-14 absolute, in the first row above
 # the wall) is 3e-11 of max|V| while U, R and E stay at 1e-13 of theirs
 @pytest.mark.parametrize("nsteps,tol", [(1, 1e-12), (3, 1e-10)])

@@ -93,3 +93,23 @@ def test_adiabatic_wall(oracle_mod):
     extP = sum(c[k] * P[1:-1, 1 + k, 0] for k in range(L))
     np.testing.assert_allclose(P[1:-1, 0, 0], extP, rtol=1e-12)
     assert np.all(bl.bi.cAdiabatic == 1)
+
+
+def test_z_symmetry_planes(oracle_mod):
+    """boundaryLayerIsothermalSymmZ.m: on both z planes w = 0 and u, v, e, p have zero z gradient
+    (one-sided extrapolation of the neumann order)."""
+    bl = cases.boundary_layer(nx=64, ny=40, nbuf=8, nz=12, z_symmetry=True)
+    assert not bl.deriv[2].periodic
+    o = oracle_mod.Oracle(bl)
+    rng = np.random.default_rng(5)
+    q = {k: np.asfortranarray(v + 0.01 * rng.standard_normal(v.shape)) for k, v in bl.state0.items()}
+    st = o.apply_bound_cond(q, 0.0)
+    c = np.asarray(bl.bi.neumann)
+    L = len(c)
+    assert np.all(st["W"][:, :, 0] == 0.0) and np.all(st["W"][:, :, -1] == 0.0)
+    for k in "UV":
+        lo = sum(c[m] * st[k][:, :, 1 + m] for m in range(L))
+        hi = sum(c[m] * st[k][:, :, -2 - m] for m in range(L))
+        # rows touched later by the wall and inflow conditions are excluded
+        np.testing.assert_allclose(st[k][1:-1, 1:-1, 0], lo[1:-1, 1:-1], rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(st[k][1:-1, 1:-1, -1], hi[1:-1, 1:-1], rtol=1e-12, atol=1e-14)
