@@ -253,13 +253,15 @@ def poiseuille_flow(nx, ny, nz, E0, P0, lower_wall_velocity=0.0, upper_wall_velo
 
 
 def boundary_layer_isothermal(X, Y, Z, E0, P0, z_periodic=True, t_wall_relative=None, adiabatic=False,
-                              z_symmetry=False):
+                              z_symmetry=False, pressure_inlet=False):
     """BC list of ``source/boundaries/boundaryLayerIsothermal.m`` (no cavities or roughness);
     ``adiabatic=True`` gives ``boundaryLayerAdiabatic.m``: zero-gradient E on every wall
     box beside the pressure condition, adiabatic corners (``:363``), E0 inside the walls,
     and that file's z wall limits (``:387-392``: the left limits carry 'zi', the right 'zf').
     ``z_symmetry=True`` gives the z ends of ``boundaryLayerIsothermalSymmZ.m:205-326``:
-    zero-gradient u, v, e, p and w = 0 on both z planes.
+    zero-gradient u, v, e, p and w = 0 on both z planes.  ``pressure_inlet=True`` gives the
+    pressure conditions of ``boundaryLayerIsothermalPressureInlet.m``: P = P0 at the inflow
+    (``:53-64``) and zero-gradient P at the outflow (``:130-140``) in place of the reverse.
 
     Returns ``(flow_region, walls, corners, bc, x_break_point, hold_inlet)``:
     ``x_break_point`` is the row ``[position, a1, a2, b1, b2]`` of ``mesh.x.breakPoint``
@@ -273,12 +275,18 @@ def boundary_layer_isothermal(X, Y, Z, E0, P0, z_periodic=True, t_wall_relative=
     if X[0] <= 0:                                         # inflow, :4-52
         for var, val in (("u", 1.0), ("v", 0.0), ("w", 0.0), ("e", E0)):
             bc.add(var, "dir", "xi", val, full(1, 1, 1, ny, 1, nz))
+        if pressure_inlet:
+            bc.add("p", "dir", "xi", P0, full(1, 1, 1, ny, 1, nz))
     else:
         hold = True
-    bc.add("p", "neu", "xi", 0.0, full(1, 1, 1, ny, 1, nz))
+    if not pressure_inlet:
+        bc.add("p", "neu", "xi", 0.0, full(1, 1, 1, ny, 1, nz))
     for var in "uvwe":                                    # outflow, :82-141
         bc.add(var, "sec", "xf", 0.0, full(nx, nx, 1, ny, 1, nz))
-    bc.add("p", "dir", "xf", P0, full(nx, nx, 1, ny, 1, nz))
+    if pressure_inlet:
+        bc.add("p", "neu", "xf", 0.0, full(nx, nx, 1, ny, 1, nz))
+    else:
+        bc.add("p", "dir", "xf", P0, full(nx, nx, 1, ny, 1, nz))
     for var in "uvwep":                                   # top, :143-202
         bc.add(var, "sec", "yf", 0.0, full(1, nx, ny, ny, 1, nz))
     if nz > 1 and z_symmetry:                             # symmetry planes, SymmZ.m:205-326

@@ -349,7 +349,7 @@ def blasius(eta_max=20.0, n=4001):
 def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wall_ratio=1.02, Re=1000.0,
                    Ma=0.5, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0.49, spatial="SL4", cfl=0.5, sfd_chi=0.02,
                    sfd_delta=10.0, noise=1e-5, seed=20261, buffer_growth=8.0, nz=1, z_length=4.0,
-                   adiabatic=False, z_symmetry=False):
+                   adiabatic=False, z_symmetry=False, pressure_inlet=False):
     """Isothermal-wall boundary layer (boundaryLayerIsothermal.m; ``adiabatic=True``: the walls of
     boundaryLayerAdiabatic.m) on a stretched mesh with buffers.
 
@@ -382,7 +382,8 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
     E0 = 1.0 / ((gamma ** 2 - gamma) * Ma ** 2)
     P0 = (gamma - 1) * E0
     fr, walls, corners, bc, bp, hold = boundary_layer_isothermal(X, Y, Z, E0, P0, adiabatic=adiabatic,
-                                                                 z_periodic=not z_sym, z_symmetry=z_sym)
+                                                                 z_periodic=not z_sym, z_symmetry=z_sym,
+                                                                 pressure_inlet=pressure_inlet)
     buf = BufferSide(n=nbuf, upwind=False, transition=0.2)
     meshes = [DirMesh(X, buffer_i=buf, buffer_f=buf, break_points=[bp] if bp else []),
               DirMesh(Y, buffer_f=buf),

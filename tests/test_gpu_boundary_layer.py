@@ -88,10 +88,12 @@ def test_steps_small(rt, oracle_mod, small, nsteps, tol):
     assert dv == dv_w
 
 
-def test_adiabatic_steps(rt, oracle_mod):
-    """boundaryLayerAdiabatic.m walls (zero-gradient E, adiabatic corners): 10 steps."""
-    case = cases.boundary_layer(nx=160, ny=64, nbuf=12, adiabatic=True)
-    print("adiabatic boundary layer 160x64", flush=True)
+@pytest.mark.parametrize("variant", ["adiabatic", "pressure_inlet"])
+def test_wall_variant_steps(rt, oracle_mod, variant):
+    """boundaryLayerAdiabatic.m walls (zero-gradient E, adiabatic corners) and the pressure
+    conditions of boundaryLayerIsothermalPressureInlet.m: 10 steps."""
+    case = cases.boundary_layer(nx=160, ny=64, nbuf=12, **{variant: True})
+    print("boundary layer 160x64", variant, flush=True)
     o = oracle_mod.Oracle(case)
     q0 = o.apply_bound_cond(case.state0, 0.0)
     o.init_mean(q0)

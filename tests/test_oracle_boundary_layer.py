@@ -113,3 +113,23 @@ def test_z_symmetry_planes(oracle_mod):
         # rows touched later by the wall and inflow conditions are excluded
         np.testing.assert_allclose(st[k][1:-1, 1:-1, 0], lo[1:-1, 1:-1], rtol=1e-12, atol=1e-14)
         np.testing.assert_allclose(st[k][1:-1, 1:-1, -1], hi[1:-1, 1:-1], rtol=1e-12, atol=1e-14)
+
+
+def test_pressure_inlet(oracle_mod):
+    """boundaryLayerIsothermalPressureInlet.m: P = P0 on the inflow plane (R = P0/(gamma-1)/E) and a
+    zero x gradient of P at the outflow (one-sided extrapolation), the reverse of the default case."""
+    bl = cases.boundary_layer(nx=160, ny=64, nbuf=12, pressure_inlet=True)
+    o = oracle_mod.Oracle(bl)
+    rng = np.random.default_rng(6)
+    q = {k: np.asfortranarray(v + 0.01 * rng.standard_normal(v.shape)) for k, v in bl.state0.items()}
+    q["W"][:] = 0.0
+    st = o.apply_bound_cond(q, 0.0)
+    gm1 = bl.gamma - 1
+    P0 = gm1 * bl.E0
+    P = gm1 * st["R"] * st["E"]
+    np.testing.assert_allclose(P[0, 1:-1, 0], P0, rtol=1e-14)
+    # the pressure extrapolation sets R = P_ext / E with the E of that moment; the null second
+    # derivative of E applied afterwards (applyBoundCond's order) then moves E on the plane
+    c = np.asarray(bl.bi.neumann)
+    ext = sum(c[k] * P[-2 - k, 1:-1, 0] for k in range(len(c)))
+    np.testing.assert_allclose(gm1 * st["R"][-1, 1:-1, 0] * q["E"][-1, 1:-1, 0], ext, rtol=1e-12)
