@@ -22,6 +22,7 @@ Block Arnoldi with block size 1 and a single Gram-Schmidt pass is the
 reference's scalar algorithm.
 """
 import math
+import os
 
 import numpy as np
 
@@ -203,7 +204,7 @@ def orthonormalize(be, W, T, rng=None):
     return R2 @ R1, p
 
 
-def block_arnoldi(apply, X0, m, be, passes=2, log=None):
+def block_arnoldi(apply, X0, m, be, passes=2, log=None, checkpoint=None, save_every=1):
     """Block Arnoldi of the operator ``apply(V, W)`` (W = A V for blocks of p columns).
 
     ``X0`` is the (p, n) starting block (on the backend), ``m`` the Krylov
@@ -211,6 +212,12 @@ def block_arnoldi(apply, X0, m, be, passes=2, log=None):
     n), the (m + p) x m block Hessenberg matrix and the R of the starting
     block.  With p = 1 and passes = 1 this is the loop of calcInstability.m:
     384-398 (classical Gram-Schmidt).
+
+    ``checkpoint``: a .npz file holding the basis, H and the number of finished
+    blocks, written every ``save_every`` blocks and at the end; when it exists
+    the iteration resumes from it (calcInstability.m:297-310, 406-423: the
+    results file with zeta, H and M), with X0 ignored.  A larger m continues the
+    saved run.
     """
     p, n = X0.shape
     if m % p:
@@ -218,10 +225,29 @@ def block_arnoldi(apply, X0, m, be, passes=2, log=None):
     nb = m // p
     V = be.empty(m + p, n)
     T = be.empty(p, n)
-    be.update(X0, np.eye(p), 0.0, 1.0, V[0:p])
-    R0, _ = orthonormalize(be, V[0:p], T)
     H = np.zeros((m + p, m))
-    for j in range(nb):
+    j0 = 0
+    if checkpoint is not None and os.path.exists(checkpoint):
+        with np.load(checkpoint, allow_pickle=False) as d:
+            if int(d["p"]) != p or d["V"].shape[1] != n:
+                raise ValueError("checkpoint %s holds blocks of another shape" % checkpoint)
+            j0 = min(int(d["blocks"]), nb)
+            be.upload(np.ascontiguousarray(d["V"][:(j0 + 1) * p]), V[0:(j0 + 1) * p])
+            H[:(j0 + 1) * p, :j0 * p] = d["H"][:(j0 + 1) * p, :j0 * p]
+            R0 = np.array(d["R0"])
+        if log:
+            log("resuming from %s after %d blocks" % (checkpoint, j0))
+    else:
+        be.update(X0, np.eye(p), 0.0, 1.0, V[0:p])
+        R0, _ = orthonormalize(be, V[0:p], T)
+
+    def save(blocks):
+        tmp = checkpoint + ".tmp.npz"
+        np.savez(tmp, V=be.download(V[0:(blocks + 1) * p]), H=H[:(blocks + 1) * p, :blocks * p],
+                 blocks=blocks, p=p, R0=R0)
+        os.replace(tmp, checkpoint)
+
+    for j in range(j0, nb):
         cur = V[j * p:(j + 1) * p]
         nxt = V[(j + 1) * p:(j + 2) * p]
         apply(cur, nxt)
@@ -242,6 +268,8 @@ def block_arnoldi(apply, X0, m, be, passes=2, log=None):
         if log:
             log("block %d/%d: subdiagonal norm %.3e" % (j + 1, nb, np.linalg.norm(
                 H[(j + 1) * p:(j + 2) * p, j * p:(j + 1) * p])))
+        if checkpoint is not None and ((j + 1) % save_every == 0 or j + 1 == nb):
+            save(j + 1)
     return V, H, R0
 
 
@@ -327,7 +355,8 @@ class Stability:
 
 
 def calc_instability(case, base, T, krylov_dim=64, block=32, epsilon=1e-7, first_order=True, seed=20260,
-                     start="random", n_modes=0, region=None, log=None, tune=False, single_beta=False):
+                     start="random", n_modes=0, region=None, log=None, tune=False, single_beta=False,
+                     checkpoint=None, save_every=1):
     """Leading eigenvalues (and optionally modes) of the time-T propagator about ``base``.
 
     start = "random": ``block`` seeded N(0,1) vectors (numpy PCG64 ``seed``);
@@ -337,7 +366,8 @@ def calc_instability(case, base, T, krylov_dim=64, block=32, epsilon=1e-7, first
     real and imaginary parts of the first ``n_modes`` Ritz vectors,
     vec2modes of calcInstability.m:437-446).  ``single_beta`` selects the
     singleBeta packing (one spanwise wavenumber, the z mesh one period of it;
-    the reference's default for 3-D bases, calcInstability.m:14).
+    the reference's default for 3-D bases, calcInstability.m:14).  ``checkpoint`` /
+    ``save_every``: the restart file of :func:`block_arnoldi`.
     """
     st = Stability(case, base, T, block=block, epsilon=epsilon, first_order=first_order, region=region,
                    tune=tune, single_beta=single_beta)
@@ -348,7 +378,8 @@ def calc_instability(case, base, T, krylov_dim=64, block=32, epsilon=1e-7, first
             X0h[0] = initial_disturbance(case, st.region, single_beta)
         X0 = st.be.empty(block, st.n)
         st.be.upload(X0h, X0)
-        V, H, _ = block_arnoldi(st.apply, X0, krylov_dim, st.be, log=log)
+        V, H, _ = block_arnoldi(st.apply, X0, krylov_dim, st.be, log=log, checkpoint=checkpoint,
+                                save_every=save_every)
         lam, mu, y = ritz(H, krylov_dim, st.T)
         out = {"lambda": lam, "mu": mu, "H": H, "m": krylov_dim, "nsteps": st.nsteps, "dt": st.dt,
                "members_advanced": st.advanced_members}

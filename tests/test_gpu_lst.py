@@ -208,3 +208,16 @@ def test_single_beta_block_arnoldi_closed_form(torch_cuda):
     mu_ref = FR.propagator_eigenvalues(case, out["nsteps"], out["dt"], single_beta=True)
     d, r, c = FR.match(out["mu"], mu_ref)
     assert d < 1e-6, d
+
+
+def test_driver_restart(torch_cuda, tmp_path):
+    """calc_instability resumed from its checkpoint reproduces the uninterrupted run's Ritz values."""
+    from gatt_dns_amd import lst
+    case = cases.periodic_rest(12, 12)
+    kw = dict(block=8, epsilon=1e-5, first_order=False, seed=9)
+    full = lst.calc_instability(case, case.state0, 0.1, krylov_dim=32, **kw)
+    ck = str(tmp_path / "results.npz")
+    lst.calc_instability(case, case.state0, 0.1, krylov_dim=16, checkpoint=ck, **kw)
+    resumed = lst.calc_instability(case, case.state0, 0.1, krylov_dim=32, checkpoint=ck, **kw)
+    np.testing.assert_allclose(resumed["H"], full["H"], rtol=0, atol=1e-12)
+    assert resumed["members_advanced"] < full["members_advanced"]

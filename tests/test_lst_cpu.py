@@ -222,3 +222,31 @@ def test_oracle_single_beta_spectrum_closed_form(oracle_mod):
     mu_ref = FR.propagator_eigenvalues(case, P.nsteps, P.dt, single_beta=True)
     d, r, c = FR.match(mu, mu_ref)
     assert d < 1e-6, d
+
+
+def test_block_arnoldi_restart(tmp_path):
+    """The checkpoint file (calcInstability.m's results file): a run stopped after 3 blocks and
+    resumed to 6 gives the basis and H of an uninterrupted 6-block run, bit for bit."""
+    A, _ = _known_matrix(40, 7)
+    be = lst.NumpyBlocks()
+    X0 = np.random.Generator(np.random.PCG64(8)).standard_normal((4, 40))
+
+    def apply(V, W):
+        W[...] = V @ A.T
+
+    Vf, Hf, Rf = lst.block_arnoldi(apply, X0, 24, be)
+    ck = str(tmp_path / "results.npz")
+    lst.block_arnoldi(apply, X0, 12, be, checkpoint=ck, save_every=2)
+    calls = []
+
+    def counted(V, W):
+        calls.append(1)
+        apply(V, W)
+
+    Vr, Hr, Rr = lst.block_arnoldi(counted, np.zeros((4, 40)), 24, be, checkpoint=ck)
+    assert len(calls) == 3                     # only the blocks after the saved ones were applied
+    np.testing.assert_array_equal(Hr, Hf)
+    np.testing.assert_array_equal(Vr, Vf)
+    np.testing.assert_array_equal(Rr, Rf)
+    with pytest.raises(ValueError):
+        lst.block_arnoldi(apply, np.zeros((2, 40)), 24, be, checkpoint=ck)
