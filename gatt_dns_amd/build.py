@@ -48,8 +48,10 @@ def build_flowio(verbose=True):
     os.makedirs(OUT_DIR, exist_ok=True)
     cxx = os.environ.get("CXX", shutil.which("g++") or "g++")
     tmp = FLOWIO_LIB + ".tmp"
-    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-I", inc, os.path.join(SRC, "flow_io.cpp"),
-           "-L", libdir, "-lhdf5", "-Wl,-rpath," + libdir, "-o", tmp]
+    # the C++ runtime is linked in statically: the HDF5 directory on the run path must not
+    # supply the process's libstdc++ (an older one there would break the HIP runtime)
+    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-static-libstdc++", "-static-libgcc", "-I", inc,
+           os.path.join(SRC, "flow_io.cpp"), "-L", libdir, "-lhdf5", "-Wl,-rpath," + libdir, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("flow-file library build failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))

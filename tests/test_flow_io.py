@@ -129,3 +129,52 @@ def test_mat73_layout(lib, tmp_path, shape, dims):
         for x, fn in ((ty, "H5Tclose"), (a, "H5Aclose"), (sp, "H5Sclose"), (d, "H5Dclose")):
             getattr(h, fn)(x)
     h.H5Fclose(f)
+
+
+FORTRAN_PROGRAM = """
+program t
+  use iso_c_binding
+  use gatt_hip
+  implicit none
+  integer, parameter :: nx = 5, ny = 4, nz = 3
+  real(c_double), target :: U(nx,ny,nz), V(nx,ny,nz), W(nx,ny,nz), R(nx,ny,nz), E(nx,ny,nz)
+  real(c_double), target :: U2(nx,ny,nz), V2(nx,ny,nz), W2(nx,ny,nz), R2(nx,ny,nz), E2(nx,ny,nz)
+  integer(c_int8_t), target :: wall(nx,ny,nz), wall2(nx,ny,nz)
+  type(c_ptr) :: q(5), q2(5)
+  real(c_double) :: t2
+  integer(c_int) :: rc
+  call random_number(U); call random_number(V); call random_number(W)
+  call random_number(R); call random_number(E)
+  wall = 0_c_int8_t
+  wall(1, :, :) = 1_c_int8_t
+  q = (/ c_loc(U), c_loc(V), c_loc(W), c_loc(R), c_loc(E) /)
+  q2 = (/ c_loc(U2), c_loc(V2), c_loc(W2), c_loc(R2), c_loc(E2) /)
+  rc = gatt_flow_write('flow_0000000003.mat'//c_null_char, nx, ny, nz, 1.5d0, q, c_loc(wall))
+  if (rc /= 0) stop 1
+  rc = gatt_flow_read('flow_0000000003.mat'//c_null_char, nx, ny, nz, t2, q2, c_loc(wall2))
+  if (rc /= 0) stop 2
+  if (t2 /= 1.5d0) stop 3
+  if (any(wall2 /= wall)) stop 4
+  if (any(U2(2:,:,:) /= U(2:,:,:)) .or. any(E2(2:,:,:) /= E(2:,:,:))) stop 5
+  if (any(U2(1,:,:) /= 0d0) .or. any(R2(1,:,:) /= 1d0)) stop 6
+  print *, 'fortran flow io ok'
+end program
+"""
+
+
+def test_fortran_interfaces(lib, tmp_path):
+    """gatt_hip.f90's interfaces of gatt_flow_write / gatt_flow_read from a Fortran program (no GPU
+    call: the program links the GPU library but only touches the flow files)."""
+    import subprocess
+    from gatt_dns_amd.fortran.build_driver import FC, LIB_DIR
+    mod = os.path.join(LIB_DIR, "fortran_build", "gatt_hip.o")
+    if not (os.path.exists(FC) and os.path.exists(mod) and os.path.exists(os.path.join(LIB_DIR, "libgatt_dns.so"))):
+        pytest.skip("amdflang or the Fortran module / GPU library not built")
+    src = tmp_path / "t.f90"
+    src.write_text(FORTRAN_PROGRAM)
+    exe = tmp_path / "t"
+    r = subprocess.run([FC, "-O2", "-I", os.path.dirname(mod), str(src), mod, "-L", LIB_DIR, "-lgatt_flowio",
+                        "-lgatt_dns", "-Wl,-rpath," + LIB_DIR, "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    r = subprocess.run([str(exe)], capture_output=True, text=True, cwd=tmp_path, timeout=60)
+    assert r.returncode == 0 and "fortran flow io ok" in r.stdout, r.stdout + r.stderr
