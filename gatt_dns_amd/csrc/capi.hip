@@ -117,6 +117,9 @@ struct gatt_ctx {
   int forcing = 0;
   double f_eps = 0, f_e0 = 0;
   double* d_partial = nullptr;
+  double* d_apartial = nullptr;    // forcing-sum partials written by the assemble kernel (np % 256 == 0)
+  int ablocks = 0;                 // assemble blocks per member
+  bool sums_from_assemble = false; // the last assemble left the next stage's partials there
   double* d_sums = nullptr;
   int sum_blocks = 0;
   // SFD
@@ -789,11 +792,23 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
     sa.np = np;
     sa.member_stride = c->slab;
     sa.S = S;
-    sa.partial = c->d_partial;
     sa.sums = c->d_sums;
-    sa.nblocks = c->sum_blocks;
-    launch_forcing_sums(sa, c->batch, c->stream);
+    if (c->sums_from_assemble && S == c->slab_ptr(SL_S)) {
+      // the previous stage's assemble kernel summed the values it wrote here
+      sa.partial = c->d_apartial;
+      sa.nblocks = c->ablocks;
+      launch_forcing_final(sa, c->batch, c->stream);
+    } else {
+      sa.partial = c->d_partial;
+      sa.nblocks = c->sum_blocks;
+      launch_forcing_sums(sa, c->batch, c->stream);
+    }
   }
+  c->sums_from_assemble = false;
+  // the stage states of RK4 (stages 1-3) and SSPRK3 (6-7) feed the next stage's sums when no
+  // boundary op touches them in between
+  const bool fuse_sums = c->forcing && c->is3d && c->d_apartial && c->waves.empty() && Out == c->slab_ptr(SL_S) &&
+                         (stage == 1 || stage == 2 || stage == 3 || stage == 6 || stage == 7);
   for (int m0 = 0; m0 < c->batch; m0 += c->wgroup) {
     const int nm = std::min(c->wgroup, c->batch - m0);
     const long long mo = (long long)m0 * c->slab;
@@ -855,8 +870,10 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
     aa.dt = dt;
     aa.is3d = c->is3d;
     set_pressure_gradient(c, aa);
+    aa.sum_partial = fuse_sums ? c->d_apartial + (long long)m0 * c->ablocks * 5 : nullptr;
     launch_assemble(aa, nm, c->stream);
   }
+  c->sums_from_assemble = fuse_sums;
 }
 
 // RK4_3D / RK4_2D: state `from` -> `to` (timeIntegration.F90:382-496); the stage
@@ -1103,6 +1120,12 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
     HIPCHK(hipMalloc(&p, sizeof(double) * 5 * c->sum_blocks * batch));
     c->dev_allocs.push_back(p);
     c->d_partial = static_cast<double*>(p);
+    if (c->np % 256 == 0) {
+      c->ablocks = (int)(c->np / 256);
+      HIPCHK(hipMalloc(&p, sizeof(double) * 5 * (size_t)c->ablocks * batch));
+      c->dev_allocs.push_back(p);
+      c->d_apartial = static_cast<double*>(p);
+    }
     HIPCHK(hipMalloc(&p, sizeof(double) * 8 * batch));
     c->dev_allocs.push_back(p);
     c->d_sums = static_cast<double*>(p);

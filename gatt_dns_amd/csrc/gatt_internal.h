@@ -208,6 +208,9 @@ struct AssembleArgs {
   int pg_box[6];
   double pg_dpdx;
   int nx, ny;
+  // when set (np a multiple of 256): the block partials of the turbulence-forcing sums
+  // of the values written to Out, 5 per block, member-major (the next stage's sums)
+  double* sum_partial;
 };
 void launch_assemble(const AssembleArgs& a, int nmembers, hipStream_t st);
 
@@ -219,6 +222,8 @@ struct SumArgs {
   int nblocks;
 };
 void launch_forcing_sums(const SumArgs& a, int nmembers, hipStream_t st);
+// the second stage of the forcing sums alone, over `nblocks` partials per member
+void launch_forcing_final(const SumArgs& a, int nmembers, hipStream_t st);
 
 struct SfdArgs {
   long long np, member_stride;

@@ -153,27 +153,59 @@ __global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs a) {
   double* Dl = a.Delta + mo;
   const double dt = a.dt;
   const int nf = 5;
+  // the first stage updates the state it differentiates (Q == S): its values are in registers
+  const bool q_is_s = a.Q == a.S;
+  const double sv[5] = {U, V, a.is3d ? S[2 * np + n] : 0.0, R, E};
+  double ov[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int f = 0; f < nf; ++f) {
     if (!a.is3d && f == 2) continue;
     const long long o = f * np + n;
-    switch (a.stage) {
-      case 0: O[o] = d[f]; break;
-      case 1: Dl[o] = dt / 6.0 * d[f]; O[o] = Q[o] + dt / 2.0 * d[f]; break;
-      case 2: Dl[o] = Dl[o] + dt / 3.0 * d[f]; O[o] = Q[o] + dt / 2.0 * d[f]; break;
-      case 3: Dl[o] = Dl[o] + dt / 3.0 * d[f]; O[o] = Q[o] + dt * d[f]; break;
-      case 4: O[o] = Q[o] + Dl[o] + dt / 6.0 * d[f]; break;
-      // EULER (timeIntegration.F90:553-557)
-      case 5: O[o] = Q[o] + dt * d[f]; break;
-      // SSPRK3 (timeIntegration.F90:623-627, 647-651, 672-676): Delta keeps k1, then k1 + k2
-      case 6: Dl[o] = d[f]; O[o] = Q[o] + dt * d[f]; break;
-      case 7: {
-        const double k = Dl[o] + d[f];
-        Dl[o] = k;
-        O[o] = Q[o] + dt * k / 4.0;
-        break;
+    double v;
+    if (q_is_s && a.stage == 1) {
+      Dl[o] = dt / 6.0 * d[f];
+      v = sv[f] + dt / 2.0 * d[f];
+    } else {
+      switch (a.stage) {
+        case 0: v = d[f]; break;
+        case 1: Dl[o] = dt / 6.0 * d[f]; v = Q[o] + dt / 2.0 * d[f]; break;
+        case 2: Dl[o] = Dl[o] + dt / 3.0 * d[f]; v = Q[o] + dt / 2.0 * d[f]; break;
+        case 3: Dl[o] = Dl[o] + dt / 3.0 * d[f]; v = Q[o] + dt * d[f]; break;
+        case 4: v = Q[o] + Dl[o] + dt / 6.0 * d[f]; break;
+        // EULER (timeIntegration.F90:553-557)
+        case 5: v = Q[o] + dt * d[f]; break;
+        // SSPRK3 (timeIntegration.F90:623-627, 647-651, 672-676): Delta keeps k1, then k1 + k2
+        case 6: Dl[o] = d[f]; v = Q[o] + dt * d[f]; break;
+        case 7: {
+          const double k = Dl[o] + d[f];
+          Dl[o] = k;
+          v = Q[o] + dt * k / 4.0;
+          break;
+        }
+        default: v = Q[o] + dt * (Dl[o] + 4.0 * d[f]) / 6.0; break;
       }
-      default: O[o] = Q[o] + dt * (Dl[o] + 4.0 * d[f]) / 6.0; break;
+    }
+    O[o] = v;
+    ov[f] = v;
+  }
+  if (a.sum_partial) {
+    // turbulenceGenerator.F90:39-44 on the stage state just written: wavefront trees, then
+    // the four wavefronts of the block in order (no thread of the block returned early)
+    double acc[5] = {ov[0] * ov[0] + ov[1] * ov[1] + ov[2] * ov[2], ov[0], ov[1], ov[2], ov[4]};
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) acc[k] += __shfl_xor(acc[k], m);
+    __shared__ double wsum[4][5];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) wsum[wv][k] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < 5) {
+      const int k = threadIdx.x;
+      a.sum_partial[((long long)blockIdx.y * gridDim.x + blockIdx.x) * 5 + k] =
+          ((wsum[0][k] + wsum[1][k]) + wsum[2][k]) + wsum[3][k];
     }
   }
 }
@@ -227,6 +259,10 @@ __global__ __launch_bounds__(256) void forcing_final_kernel(SumArgs a) {
 
 void launch_forcing_sums(const SumArgs& a, int nmembers, hipStream_t st) {
   hipLaunchKernelGGL(forcing_partial_kernel, dim3(a.nblocks, nmembers), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(forcing_final_kernel, dim3(nmembers), dim3(256), 0, st, a);
+}
+
+void launch_forcing_final(const SumArgs& a, int nmembers, hipStream_t st) {
   hipLaunchKernelGGL(forcing_final_kernel, dim3(nmembers), dim3(256), 0, st, a);
 }
 
