@@ -117,6 +117,8 @@ struct gatt_ctx {
   int forcing = 0;
   double f_eps = 0, f_e0 = 0;
   double* d_partial = nullptr;
+  void* d_points = nullptr;       // gatt_get_points: point indices then gathered values
+  int points_cap = 0;
   double* d_apartial = nullptr;    // forcing-sum partials written by the assemble kernel (np % 256 == 0)
   int ablocks = 0;                 // assemble blocks per member
   bool sums_from_assemble = false; // the last assemble left the next stage's partials there
@@ -1176,6 +1178,7 @@ int gatt_destroy(gatt_ctx* c) {
   if (c->d_rmap) hipFree(c->d_rmap);
   if (c->d_base) hipFree(c->d_base);
   if (c->d_scratch) hipFree(c->d_scratch);
+  if (c->d_points) hipFree(c->d_points);
   if (c->lin) hipFree(c->lin);
   if (c->tape) hipFree(c->tape);
   if (c->mem) hipFree(c->mem);
@@ -1189,6 +1192,8 @@ int gatt_destroy(gatt_ctx* c) {
   delete c;
   return GATT_OK;
 }
+
+static constexpr int32_t kMaxPoints = 1 << 20;
 
 static int copy_fields(gatt_ctx* c, int32_t member, double* dev, const double* const* hq, bool to_dev) {
   if (member < 0 || member >= c->batch) return fail(GATT_E_INVALID, "member out of range");
@@ -1214,6 +1219,32 @@ int gatt_get_state(gatt_ctx* c, int32_t member, double* const* q) {
   int rc = copy_fields(c, member, c->state(c->cur), (const double* const*)q, false);
   if (rc) return rc;
   if (!c->is3d && q[GATT_W]) std::memset(q[GATT_W], 0, sizeof(double) * c->np);
+  return GATT_OK;
+}
+
+int gatt_get_points(gatt_ctx* c, int32_t member, int32_t npts, const int64_t* idx, double* out) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  if (member < 0 || member >= c->batch) return fail(GATT_E_INVALID, "member out of range");
+  if (npts < 0 || npts > kMaxPoints) return fail(GATT_E_INVALID, "point count out of range");
+  if (npts == 0) return GATT_OK;
+  if (!idx || !out) return fail(GATT_E_INVALID, "null argument");
+  for (int32_t i = 0; i < npts; ++i)
+    if (idx[i] < 0 || idx[i] >= c->np) return fail(GATT_E_INVALID, "point index outside the grid");
+  if (npts > c->points_cap) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->d_points) HIPCHK(hipFree(c->d_points));
+    c->d_points = nullptr;
+    c->points_cap = 0;
+    HIPCHK(hipMalloc(&c->d_points, (size_t)npts * (sizeof(long long) + 5 * sizeof(double))));
+    c->points_cap = npts;
+  }
+  long long* d_idx = static_cast<long long*>(c->d_points);
+  double* d_out = reinterpret_cast<double*>(d_idx + c->points_cap);
+  HIPCHK(hipMemcpyAsync(d_idx, idx, sizeof(long long) * npts, hipMemcpyHostToDevice, c->stream));
+  launch_gather_points(c->state(c->cur) + (long long)member * c->slab, c->np, c->is3d, npts, d_idx, d_out, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, d_out, sizeof(double) * 5 * npts, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return GATT_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <ctime>
 #include <limits>
@@ -112,18 +113,10 @@ bool get_var(hid_t file, const char* name, int32_t nx, int32_t ny, int32_t nz, d
   return ok;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char* gatt_flow_last_error(void) { return g_err.c_str(); }
-
-int gatt_flow_write(const char* path, int32_t nx, int32_t ny, int32_t nz, double t, const double* const* q,
-                    const uint8_t* inside_wall) {
-  if (!path || !q) return fail(GATT_E_INVALID, "null argument");
-  if (nx < 1 || ny < 1 || nz < 1) return fail(GATT_E_INVALID, "grid sizes must be positive");
-  for (int f = 0; f < 5; ++f)
-    if (!q[f]) return fail(GATT_E_INVALID, "null field pointer");
+// one MAT 7.3 file of named nx*ny*nz arrays (optionally preceded by the 1x1 t), each
+// field NaN where inside_wall is set (exportFlow.F90 / exportMeanFlow.F90)
+int write_vars(const char* path, int32_t nx, int32_t ny, int32_t nz, const double* t, int nvars,
+               const char* const* names, const double* const* q, const uint8_t* inside_wall) {
   H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
   const size_t np = (size_t)nx * ny * nz;
   hsize_t d[3];
@@ -134,23 +127,200 @@ int gatt_flow_write(const char* path, int32_t nx, int32_t ny, int32_t nz, double
   H5Pclose(fcpl);
   if (file < 0) return fail(GATT_E_INVALID, std::string("cannot create ") + path);
   const hsize_t d1[2] = {1, 1};
-  bool ok = put_var(file, "t", 2, d1, &t);
+  bool ok = !t || put_var(file, "t", 2, d1, t);
   std::vector<double> tmp;
-  for (int f = 0; ok && f < 5; ++f) {
+  for (int f = 0; ok && f < nvars; ++f) {
     const double* src = q[f];
     if (inside_wall) {
-      // exportFlow.F90: NaN in every field inside the walls
       tmp.assign(src, src + np);
       for (size_t n = 0; n < np; ++n)
         if (inside_wall[n]) tmp[n] = std::numeric_limits<double>::quiet_NaN();
       src = tmp.data();
     }
-    ok = put_var(file, kNames[f], rank, d, src);
+    ok = put_var(file, names[f], rank, d, src);
   }
   if (H5Fclose(file) < 0) ok = false;
   if (!ok) return fail(GATT_E_INVALID, std::string("writing ") + path + " failed");
   if (!put_header(path)) return fail(GATT_E_INVALID, std::string("cannot write the MAT header of ") + path);
   return GATT_OK;
+}
+
+bool check_sizes(int32_t nx, int32_t ny, int32_t nz) { return nx >= 1 && ny >= 1 && nz >= 1; }
+
+// ---- Fortran edit descriptors of the log line ------------------------------------------
+// The processor-dependent parts follow the Fortran compiler of this image (amdflang):
+// no '+' on positive numbers, "NaN" / "Inf" / "-Inf" right-justified, a field too narrow
+// for its value (or an exponent wider than Ee allows) is w asterisks.
+
+std::string right(const std::string& s, int w) {
+  if ((int)s.size() > w) return std::string(w, '*');
+  return std::string(w - s.size(), ' ') + s;
+}
+
+std::string special(double x, int w) {
+  if (std::isnan(x)) return right("NaN", w);
+  return right(x < 0 ? "-Inf" : "Inf", w);
+}
+
+// ESw.dEe
+std::string fmt_es(double x, int w, int d, int e) {
+  if (!std::isfinite(x)) return special(x, w);
+  char b[64];
+  std::snprintf(b, sizeof(b), "%.*e", d, x);
+  char* ep = std::strchr(b, 'e');
+  const int ex = std::atoi(ep + 1);
+  *ep = 0;
+  int lim = 1;
+  for (int i = 0; i < e; ++i) lim *= 10;
+  if (std::abs(ex) >= lim) return std::string(w, '*');
+  char eb[16];
+  std::snprintf(eb, sizeof(eb), "E%c%0*d", ex < 0 ? '-' : '+', e, std::abs(ex));
+  return right(std::string(b) + eb, w);
+}
+
+// Fw.d (the optional zero before the decimal point is dropped when the field needs the room)
+std::string fmt_f(double x, int w, int d) {
+  if (!std::isfinite(x)) return special(x, w);
+  char b[512];
+  std::snprintf(b, sizeof(b), "%.*f", d, x);
+  std::string s(b);
+  if ((int)s.size() == w + 1) {
+    if (s.compare(0, 2, "0.") == 0) s.erase(0, 1);
+    else if (s.compare(0, 3, "-0.") == 0) s.erase(1, 1);
+  }
+  return right(s, w);
+}
+
+std::string fmt_i(long long n, int w) { return right(std::to_string(n), w); }
+
+int put_text(const std::string& s, char* buf, int64_t size) {
+  if (!buf || size < (int64_t)s.size() + 1) return fail(GATT_E_INVALID, "log buffer too small");
+  std::memcpy(buf, s.data(), s.size());
+  buf[s.size()] = 0;
+  return (int)s.size();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gatt_flow_last_error(void) { return g_err.c_str(); }
+
+int gatt_flow_write(const char* path, int32_t nx, int32_t ny, int32_t nz, double t, const double* const* q,
+                    const uint8_t* inside_wall) {
+  if (!path || !q) return fail(GATT_E_INVALID, "null argument");
+  if (!check_sizes(nx, ny, nz)) return fail(GATT_E_INVALID, "grid sizes must be positive");
+  for (int f = 0; f < 5; ++f)
+    if (!q[f]) return fail(GATT_E_INVALID, "null field pointer");
+  return write_vars(path, nx, ny, nz, &t, 5, kNames, q, inside_wall);
+}
+
+int gatt_flow_write_deriv(const char* path, int32_t nx, int32_t ny, int32_t nz, double t, const double* const* q,
+                          const double* const* qdot, const uint8_t* inside_wall) {
+  if (!path || !q || !qdot) return fail(GATT_E_INVALID, "null argument");
+  if (!check_sizes(nx, ny, nz)) return fail(GATT_E_INVALID, "grid sizes must be positive");
+  static const char* const names[10] = {"U", "V", "W", "R", "E", "Udot", "Vdot", "Wdot", "Rdot", "Edot"};
+  const double* all[10];
+  for (int f = 0; f < 5; ++f) {
+    all[f] = q[f];
+    all[5 + f] = qdot[f];
+    if (!q[f] || !qdot[f]) return fail(GATT_E_INVALID, "null field pointer");
+  }
+  return write_vars(path, nx, ny, nz, &t, 10, names, all, inside_wall);
+}
+
+int gatt_meanflow_write(const char* path, int32_t nx, int32_t ny, int32_t nz, const double* const* q,
+                        const uint8_t* inside_wall) {
+  if (!path || !q) return fail(GATT_E_INVALID, "null argument");
+  if (!check_sizes(nx, ny, nz)) return fail(GATT_E_INVALID, "grid sizes must be positive");
+  for (int f = 0; f < 5; ++f)
+    if (!q[f]) return fail(GATT_E_INVALID, "null field pointer");
+  return write_vars(path, nx, ny, nz, nullptr, 5, kNames, q, inside_wall);
+}
+
+int gatt_sfd_write(const char* path, int32_t nx, int32_t ny, int32_t nz, const double* sfd_x) {
+  if (!path || !sfd_x) return fail(GATT_E_INVALID, "null argument");
+  if (!check_sizes(nx, ny, nz)) return fail(GATT_E_INVALID, "grid sizes must be positive");
+  static const char* const names[1] = {"SFD_X"};
+  return write_vars(path, nx, ny, nz, nullptr, 1, names, &sfd_x, nullptr);
+}
+
+int gatt_sfd_read(const char* path, int32_t nx, int32_t ny, int32_t nz, double* sfd_x) {
+  if (!path || !sfd_x) return fail(GATT_E_INVALID, "null argument");
+  if (!check_sizes(nx, ny, nz)) return fail(GATT_E_INVALID, "grid sizes must be positive");
+  H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+  hid_t file = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+  if (file < 0) return fail(GATT_E_INVALID, std::string("cannot open ") + path + " as a MAT 7.3 (HDF5) file");
+  bool present = false;
+  const bool ok = get_var(file, "SFD_X", nx, ny, nz, sfd_x, &present) && present;
+  H5Fclose(file);
+  if (!ok) return fail(GATT_E_INVALID, std::string(path) + ": no SFD_X of the grid's sizes");
+  return GATT_OK;
+}
+
+int gatt_meanflow_read(const char* path, int32_t nx, int32_t ny, int32_t nz, double* const* q,
+                       const uint8_t* inside_wall) {
+  if (!path || !q) return fail(GATT_E_INVALID, "null argument");
+  if (!check_sizes(nx, ny, nz)) return fail(GATT_E_INVALID, "grid sizes must be positive");
+  for (int f = 0; f < 5; ++f)
+    if (!q[f]) return fail(GATT_E_INVALID, "null field pointer");
+  H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+  hid_t file = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+  if (file < 0) return fail(GATT_E_INVALID, std::string("cannot open ") + path + " as a MAT 7.3 (HDF5) file");
+  const size_t np = (size_t)nx * ny * nz;
+  bool present = false;
+  bool ok = true;
+  for (int f = 0; ok && f < 5; ++f) {
+    ok = get_var(file, kNames[f], nx, ny, nz, q[f], &present);
+    if (ok && !present) {
+      if (f != 2) ok = false;
+      else std::memset(q[f], 0, np * sizeof(double));
+    }
+  }
+  H5Fclose(file);
+  if (!ok) return fail(GATT_E_INVALID, std::string(path) + ": missing variable or sizes that do not match the grid");
+  // importMeanFlow.F90: the flow's wall points (or, without the flow's mask, the NaN points)
+  for (size_t n = 0; n < np; ++n) {
+    const bool wall = inside_wall ? inside_wall[n] != 0 : std::isnan(q[0][n]);
+    if (wall) {
+      q[0][n] = 0;
+      q[1][n] = 0;
+      q[2][n] = 0;
+      q[3][n] = 1;
+      q[4][n] = 1;
+    }
+  }
+  return GATT_OK;
+}
+
+int gatt_log_header(int32_t ntracked, char* buf, int64_t size) {
+  if (ntracked < 0) return fail(GATT_E_INVALID, "negative tracked-point count");
+  std::string s = "Save number\tIteration\tSimulation time\tdt       \tCFL      \tU change\tV change\tW change\tR change\tE change";
+  for (int i = 1; i <= ntracked; ++i) {
+    const std::string k = std::to_string(i);
+    for (const char* v : {"U", "V", "W", "R", "E"}) s += std::string("\t") + v + k + "            ";
+  }
+  s += "\n";
+  return put_text(s, buf, size);
+}
+
+int gatt_log_line(int32_t nsave, int32_t tstep, double t, double dt, double cfl, const double* max_change,
+                  int32_t ntracked, const double* tracked, double tracked_norm, char* buf, int64_t size) {
+  if (!max_change || (ntracked > 0 && !tracked) || ntracked < 0) return fail(GATT_E_INVALID, "null argument");
+  std::string s = fmt_i(nsave, 10) + "\t" + fmt_i(tstep, 10) + "\t" + fmt_es(t, 13, 8, 1) + "\t" + fmt_es(dt, 10, 5, 1) +
+                  "\t" + fmt_f(cfl, 8, 5);
+  for (int k = 0; k < 5; ++k) s += "\t" + fmt_es(max_change[k], 10, 4, 2);
+  for (int i = 0; i < ntracked; ++i) {
+    double v[5];
+    for (int k = 0; k < 5; ++k) v[k] = tracked[5 * i + k];
+    if (tracked_norm != 0) {
+      v[3] = v[3] - 1;
+      v[4] = v[4] / tracked_norm - 1;
+    }
+    for (int k = 0; k < 5; ++k) s += v[k] == 0 ? std::string("\t0") : "\t" + fmt_es(v[k], 15, 8, 2);
+  }
+  s += " \n";
+  return put_text(s, buf, size);
 }
 
 int gatt_flow_read(const char* path, int32_t nx, int32_t ny, int32_t nz, double* t, double* const* q,

@@ -244,6 +244,9 @@ struct ReduceArgs {
   int is3d;
 };
 void launch_clear_u64(unsigned long long* p, int n, hipStream_t st);
+// out[5*i + f] = field f of `state` at point idx[i] (W = 0 in 2-D); idx checked by the caller
+void launch_gather_points(const double* state, long long np, int is3d, int npts, const long long* idx, double* out,
+                          hipStream_t st);
 void launch_step_reduce(const ReduceArgs& a, int nmembers, hipStream_t st);
 void launch_copy(double* dst, const double* src, long long n, hipStream_t st);
 void launch_record(const unsigned long long* red, double* rec, int* flag, int nmembers, hipStream_t st);

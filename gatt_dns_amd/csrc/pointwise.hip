@@ -266,6 +266,23 @@ void launch_forcing_final(const SumArgs& a, int nmembers, hipStream_t st) {
   hipLaunchKernelGGL(forcing_final_kernel, dim3(nmembers), dim3(256), 0, st, a);
 }
 
+// the tracked points of the log line (main.F90:478-486): one thread per (point, field)
+__global__ __launch_bounds__(256) void gather_points_kernel(const double* __restrict__ state, long long np, int is3d,
+                                                            int npts, const long long* __restrict__ idx,
+                                                            double* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= 5 * npts) return;
+  const int p = i / 5, f = i - 5 * p;
+  out[i] = (f == 2 && !is3d) ? 0.0 : state[(long long)f * np + idx[p]];
+}
+
+void launch_gather_points(const double* state, long long np, int is3d, int npts, const long long* idx, double* out,
+                          hipStream_t st) {
+  if (npts <= 0) return;
+  hipLaunchKernelGGL(gather_points_kernel, dim3((5 * npts + 255) / 256), dim3(256), 0, st, state, np, is3d, npts, idx,
+                     out);
+}
+
 // main.F90:382-401
 __global__ __launch_bounds__(256) void sfd_kernel(SfdArgs a) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;

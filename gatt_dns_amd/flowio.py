@@ -4,7 +4,9 @@ The reference saves and loads them with MATLAB's MAT-file API
 (``source/Fortran/readWriteMat.F90``, ``exportFlow.F90``, ``importFlow.F90``);
 this module calls the library's HDF5 implementation of the same layout
 (``include/gatt_flow_io.h``): variables ``t``, ``U``, ``V``, ``W``, ``R``, ``E``,
-NaN inside walls on export, walls recognised from NaN on import.
+NaN inside walls on export, walls recognised from NaN on import.  Also the SFD
+mean (``meanflowSFD.mat``), the damping field (``SFD.mat``), the flow with its
+rates (``saveDerivs``) and the lines of ``log.txt`` (``main.F90:467-506``).
 """
 import ctypes as C
 import os
@@ -15,7 +17,9 @@ from .abi import DP, field_ptrs
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libgatt_flowio.so")
-EXPORTS = ["gatt_flow_last_error", "gatt_flow_write", "gatt_flow_read"]
+EXPORTS = ["gatt_flow_last_error", "gatt_flow_write", "gatt_flow_read", "gatt_flow_write_deriv",
+           "gatt_meanflow_write", "gatt_meanflow_read", "gatt_sfd_write", "gatt_sfd_read", "gatt_log_header",
+           "gatt_log_line"]
 
 _lib = None
 
@@ -35,6 +39,15 @@ def load_library():
                                       C.c_void_p]
         L.gatt_flow_read.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double),
                                      C.POINTER(DP), C.c_void_p]
+        dims = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32]
+        L.gatt_flow_write_deriv.argtypes = dims + [C.c_double, C.POINTER(DP), C.POINTER(DP), C.c_void_p]
+        L.gatt_meanflow_write.argtypes = dims + [C.POINTER(DP), C.c_void_p]
+        L.gatt_meanflow_read.argtypes = dims + [C.POINTER(DP), C.c_void_p]
+        L.gatt_sfd_write.argtypes = dims + [C.c_void_p]
+        L.gatt_sfd_read.argtypes = dims + [C.c_void_p]
+        L.gatt_log_header.argtypes = [C.c_int32, C.c_char_p, C.c_int64]
+        L.gatt_log_line.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int32,
+                                    C.c_void_p, C.c_double, C.c_char_p, C.c_int64]
         _lib = L
     return _lib
 
@@ -73,3 +86,95 @@ def read_flow(path, shape):
     _check(L.gatt_flow_read(os.fsencode(path), nx, ny, nz, C.byref(t), field_ptrs(arrs),
                             wall.ctypes.data_as(C.c_void_p)))
     return dict(zip("UVWRE", arrs)), t.value, wall.astype(bool)
+
+
+def _fields(q, shape):
+    return [np.asfortranarray(np.asarray(q[k], dtype=np.float64).reshape(shape, order="F")) for k in "UVWRE"]
+
+
+def _wall_mask(flow_region, shape):
+    if flow_region is None:
+        return None
+    return np.asfortranarray(~np.asarray(flow_region, dtype=bool).reshape(shape, order="F")).astype(np.uint8)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def write_flow_deriv(path, q, qdot, t, shape, flow_region=None):
+    """writeFlowDeriv: the state, its rates (dict Udot..Edot or U..E keys) and t; NaN outside
+    ``flow_region`` in all ten fields."""
+    L = load_library()
+    nx, ny, nz = shape
+    rates = {k: qdot.get(k + "dot", qdot.get(k)) for k in "UVWRE"}
+    arrs, darrs, wall = _fields(q, shape), _fields(rates, shape), _wall_mask(flow_region, shape)
+    _check(L.gatt_flow_write_deriv(os.fsencode(path), nx, ny, nz, float(t), field_ptrs(arrs), field_ptrs(darrs),
+                                   _ptr(wall)))
+
+
+def write_mean_flow(path, q, shape, flow_region=None):
+    """writeMeanFlow + exportMeanFlow: the SFD mean U..E, NaN outside ``flow_region``."""
+    L = load_library()
+    nx, ny, nz = shape
+    arrs, wall = _fields(q, shape), _wall_mask(flow_region, shape)
+    _check(L.gatt_meanflow_write(os.fsencode(path), nx, ny, nz, field_ptrs(arrs), _ptr(wall)))
+
+
+def read_mean_flow(path, shape, inside_wall=None):
+    """readMeanFlow + importMeanFlow: the mean U..E with U = V = W = 0, R = E = 1 at the
+    flow's wall points (``inside_wall``; the NaN points when it is None)."""
+    L = load_library()
+    nx, ny, nz = shape
+    arrs = [np.zeros(shape, order="F") for _ in range(5)]
+    wall = None if inside_wall is None else np.asfortranarray(np.asarray(inside_wall, bool)).astype(np.uint8)
+    _check(L.gatt_meanflow_read(os.fsencode(path), nx, ny, nz, field_ptrs(arrs), _ptr(wall)))
+    return dict(zip("UVWRE", arrs))
+
+
+def write_sfd(path, sfd_x):
+    """SFD.mat with the damping field ``SFD_X`` (preprocessing.m:104)."""
+    L = load_library()
+    a = np.asfortranarray(np.asarray(sfd_x, dtype=np.float64))
+    shape = a.shape + (1,) * (3 - a.ndim)
+    _check(L.gatt_sfd_write(os.fsencode(path), shape[0], shape[1], shape[2], _ptr(a)))
+
+
+def read_sfd(path, shape):
+    """readSFD: ``SFD_X`` of ``shape`` from a MAT 7.3 file."""
+    L = load_library()
+    nx, ny, nz = shape
+    a = np.zeros(shape, order="F")
+    _check(L.gatt_sfd_read(os.fsencode(path), nx, ny, nz, _ptr(a)))
+    return a
+
+
+def log_header(n_tracked=0):
+    """The header line runDNS.m:58-66 writes when log.txt does not exist."""
+    L = load_library()
+    buf = C.create_string_buffer(256 + 128 * n_tracked)
+    rc = L.gatt_log_header(int(n_tracked), buf, len(buf))
+    if rc < 0:
+        _check(rc)
+    return buf.value.decode()
+
+
+def log_line(nsave, tstep, t, dt, cfl, max_change, tracked=(), tracked_norm=0.0):
+    """One log.txt line of main.F90:467-506.  ``tracked``: per tracked point its five
+    values U, V, W, R, E (rows)."""
+    L = load_library()
+    mc = np.ascontiguousarray(np.asarray(max_change, dtype=np.float64).reshape(5))
+    tr = np.ascontiguousarray(np.asarray(tracked, dtype=np.float64).reshape(-1, 5)) if len(tracked) else None
+    n = 0 if tr is None else tr.shape[0]
+    buf = C.create_string_buffer(256 + 128 * n)
+    rc = L.gatt_log_line(int(nsave), int(tstep), float(t), float(dt), float(cfl), _ptr(mc), n, _ptr(tr),
+                         float(tracked_norm), buf, len(buf))
+    if rc < 0:
+        _check(rc)
+    return buf.value.decode()
+
+
+def append_log(path, line):
+    """Append one line to log.txt, writing the header first if the file does not exist."""
+    with open(path, "a") as f:
+        f.write(line)

@@ -123,6 +123,57 @@ module gatt_hip
       type(c_ptr), dimension(5), intent(in) :: q
       type(c_ptr), value :: inside_wall
     end function
+    integer(c_int) function gatt_flow_write_deriv(path, nx, ny, nz, t, q, qdot, inside_wall) &
+        & bind(C, name="gatt_flow_write_deriv")
+      import :: c_ptr, c_int, c_double, c_int32_t, c_char
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int32_t), value :: nx, ny, nz
+      real(c_double), value :: t
+      type(c_ptr), dimension(5), intent(in) :: q, qdot
+      type(c_ptr), value :: inside_wall
+    end function
+    integer(c_int) function gatt_meanflow_write(path, nx, ny, nz, q, inside_wall) bind(C, name="gatt_meanflow_write")
+      import :: c_ptr, c_int, c_int32_t, c_char
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int32_t), value :: nx, ny, nz
+      type(c_ptr), dimension(5), intent(in) :: q
+      type(c_ptr), value :: inside_wall
+    end function
+    integer(c_int) function gatt_meanflow_read(path, nx, ny, nz, q, inside_wall) bind(C, name="gatt_meanflow_read")
+      import :: c_ptr, c_int, c_int32_t, c_char
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int32_t), value :: nx, ny, nz
+      type(c_ptr), dimension(5), intent(in) :: q
+      type(c_ptr), value :: inside_wall
+    end function
+    integer(c_int) function gatt_sfd_read(path, nx, ny, nz, sfd_x) bind(C, name="gatt_sfd_read")
+      import :: c_int, c_int32_t, c_char, c_double
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int32_t), value :: nx, ny, nz
+      real(c_double), dimension(*), intent(out) :: sfd_x
+    end function
+    integer(c_int) function gatt_sfd_write(path, nx, ny, nz, sfd_x) bind(C, name="gatt_sfd_write")
+      import :: c_int, c_int32_t, c_char, c_double
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int32_t), value :: nx, ny, nz
+      real(c_double), dimension(*), intent(in) :: sfd_x
+    end function
+    integer(c_int) function gatt_log_header(ntracked, buf, size) bind(C, name="gatt_log_header")
+      import :: c_int, c_int32_t, c_int64_t, c_char
+      integer(c_int32_t), value :: ntracked
+      character(kind=c_char), dimension(*), intent(out) :: buf
+      integer(c_int64_t), value :: size
+    end function
+    integer(c_int) function gatt_log_line(nsave, tstep, t, dt, cfl, max_change, ntracked, tracked, tracked_norm, &
+        & buf, size) bind(C, name="gatt_log_line")
+      import :: c_int, c_int32_t, c_int64_t, c_char, c_double
+      integer(c_int32_t), value :: nsave, tstep, ntracked
+      real(c_double), value :: t, dt, cfl, tracked_norm
+      real(c_double), dimension(5), intent(in) :: max_change
+      real(c_double), dimension(*), intent(in) :: tracked
+      character(kind=c_char), dimension(*), intent(out) :: buf
+      integer(c_int64_t), value :: size
+    end function
     integer(c_int) function gatt_sfd(ctx, dt) bind(C, name="gatt_sfd")
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx

@@ -27,7 +27,7 @@ EXPORTS = ["gatt_last_error", "gatt_device_info", "gatt_create", "gatt_destroy",
            "gatt_perturb", "gatt_collect", "gatt_block_dot", "gatt_block_update", "gatt_group_create",
            "gatt_group_destroy", "gatt_group_advance", "gatt_set_perturbation", "gatt_get_perturbation",
            "gatt_copy_fields", "gatt_tlm_step", "gatt_adj_step", "gatt_line_solve_t", "gatt_euler",
-           "gatt_ssprk3", "gatt_set_single_beta"]
+           "gatt_ssprk3", "gatt_set_single_beta", "gatt_get_points"]
 
 SLOT_STATE = 0
 SLOT_PERTURBATION = 1
@@ -68,6 +68,7 @@ def load_library(path=LIB_PATH):
     L.gatt_destroy.argtypes = [V]
     for name in ("gatt_set_state", "gatt_get_state", "gatt_set_mean", "gatt_get_mean"):
         getattr(L, name).argtypes = [V, I, C.POINTER(DP)]
+    L.gatt_get_points.argtypes = [V, I, I, V, V]
     L.gatt_rk4.argtypes = [V, C.c_double, C.c_double]
     L.gatt_euler.argtypes = [V, C.c_double, C.c_double]
     L.gatt_set_single_beta.argtypes = [V, C.c_int32]
@@ -154,6 +155,19 @@ class Solver:
         out = [np.zeros(self.shape, order="F") for _ in range(5)]
         _check(self.lib.gatt_get_state(self.ctx, member, field_ptrs(out)))
         return dict(zip("UVWRE", out))
+
+    def get_points(self, points, member=0):
+        """U, V, W, R, E of the current state at grid points (rows of 0-based (i, j, k), the
+        reference's indTracked minus one): an (n, 5) array, read on the device."""
+        ijk = np.asarray(points, dtype=np.int64).reshape(-1, 3)
+        nx, ny, nz = self.shape
+        if ijk.size and (np.any(ijk < 0) or np.any(ijk >= np.array([nx, ny, nz]))):
+            raise GattError("tracked point outside the grid")
+        idx = np.ascontiguousarray(ijk[:, 0] + nx * (ijk[:, 1] + ny * ijk[:, 2]))
+        out = np.zeros((len(idx), 5))
+        _check(self.lib.gatt_get_points(self.ctx, member, len(idx), idx.ctypes.data_as(C.c_void_p),
+                                        out.ctypes.data_as(C.c_void_p)))
+        return out
 
     def set_mean(self, q, member=0):
         _check(self.lib.gatt_set_mean(self.ctx, member, field_ptrs(self._arrays(q))))

@@ -168,6 +168,11 @@ int gatt_destroy(gatt_ctx* ctx);
 int gatt_set_state(gatt_ctx* ctx, int32_t member, const double* const* q);
 int gatt_get_state(gatt_ctx* ctx, int32_t member, double* const* q);
 /* SFD mean of member m (main.F90:148-164; defaults to the state at the first step). */
+/* The five values of the current state at npts points (tracked points of the
+ * log file, main.F90:478-486): out[5*i + f] = field f (U,V,W,R,E) of `member` at
+ * the 0-based linear point idx[i] = i + nx*(j + ny*k); W = 0 in 2-D.  At most
+ * 2^20 points per call; every index must lie in the grid. */
+int gatt_get_points(gatt_ctx* ctx, int32_t member, int32_t npts, const int64_t* idx, double* out);
 int gatt_set_mean(gatt_ctx* ctx, int32_t member, const double* const* q);
 int gatt_get_mean(gatt_ctx* ctx, int32_t member, double* const* q);
 

@@ -8,6 +8,14 @@
   "IM") in a 512-byte userblock, the HDF5 signature sits at offset 512, every
   variable is a dataset with the attribute MATLAB_class = "double", 3-D arrays
   are stored with dimensions (nz, ny, nx) and 2-D ones as (ny, nx).
+* The SFD mean (meanflowSFD.mat: U..E without t), the damping field (SFD.mat:
+  SFD_X) and the flow with its rates (t, U..E, Udot..Edot) have the same layout,
+  checked the same way, and read back bit for bit.
+* log.txt: the header is runDNS.m:58-66's text; every field of a line is
+  compared with what the image's Fortran compiler (amdflang) prints for the
+  reference's own edit descriptors (main.F90:474, 499) on the same values,
+  over random magnitudes, signs, zeros, NaN, Inf and fields too narrow for
+  their value.
 * Compatibility with files written by MATLAB itself is not checked (parity
   unpinned): the reference ships no flow files and MATLAB is not available.
 """
@@ -99,6 +107,10 @@ def _h5():
 def test_mat73_layout(lib, tmp_path, shape, dims):
     path = tmp_path / "flow_0000000001.mat"
     flowio.write_flow(path, _state(shape, 2), 0.5, shape)
+    _check_layout(path, [("t", [1, 1])] + [(k, dims) for k in "UVWRE"])
+
+
+def _check_layout(path, variables):
     raw = open(path, "rb").read()
     assert raw[:19] == b"MATLAB 7.3 MAT-file"
     assert raw[124:128] == b"\x00\x02IM"
@@ -111,7 +123,7 @@ def test_mat73_layout(lib, tmp_path, shape, dims):
     h.H5Pget_userblock(cp, C.byref(ub))
     h.H5Pclose(cp)
     assert ub.value == 512
-    for name, want in [("t", [1, 1])] + [(k, dims) for k in "UVWRE"]:
+    for name, want in variables:
         d = h.H5Dopen2(f, name.encode(), 0)
         assert d >= 0, name
         sp = h.H5Dget_space(d)
@@ -131,6 +143,129 @@ def test_mat73_layout(lib, tmp_path, shape, dims):
     h.H5Fclose(f)
 
 
+@pytest.mark.parametrize("shape,dims", [((7, 5, 3), [3, 5, 7]), ((9, 6, 1), [6, 9])])
+def test_mean_flow_sfd_and_rates(lib, tmp_path, shape, dims):
+    q, qd = _state(shape, 5), _state(shape, 6)
+    fr = np.ones(shape, dtype=bool)
+    fr[-1, :, :] = False
+    # meanflowSFD.mat: no t, NaN in walls; read back with the flow's mask or from the NaN
+    path = tmp_path / "meanflowSFD.mat"
+    flowio.write_mean_flow(path, q, shape, flow_region=fr)
+    _check_layout(path, [(k, dims) for k in "UVWRE"])
+    for mask in (~fr, None):
+        got = flowio.read_mean_flow(path, shape, inside_wall=mask)
+        for k, v in zip("UVWRE", (0.0, 0.0, 0.0, 1.0, 1.0)):
+            np.testing.assert_array_equal(got[k][fr], q[k][fr])
+            assert np.all(got[k][~fr] == v)
+    with pytest.raises(flowio.FlowFileError):        # a mean file is not a flow file (no t)
+        flowio.read_flow(path, shape)
+    # SFD.mat
+    x = np.asfortranarray(np.random.Generator(np.random.PCG64(7)).random(shape))
+    path = tmp_path / "SFD.mat"
+    flowio.write_sfd(path, x if shape[2] > 1 else x[:, :, 0])
+    _check_layout(path, [("SFD_X", dims)])
+    np.testing.assert_array_equal(flowio.read_sfd(path, shape), x)
+    with pytest.raises(flowio.FlowFileError):
+        flowio.read_sfd(path, (shape[0] + 1,) + shape[1:])
+    # the flow with its rates: the flow part reads back as a flow file
+    path = tmp_path / "flow_0000000004.mat"
+    flowio.write_flow_deriv(path, q, {k + "dot": qd[k] for k in "UVWRE"}, 2.5, shape, flow_region=fr)
+    _check_layout(path, [("t", [1, 1])] + [(k, dims) for k in "UVWRE"] + [(k + "dot", dims) for k in "UVWRE"])
+    got, t, wall = flowio.read_flow(path, shape)
+    assert t == 2.5
+    np.testing.assert_array_equal(wall, ~fr)
+    np.testing.assert_array_equal(got["E"][fr], q["E"][fr])
+
+
+LOG_HEADER = ("Save number\tIteration\tSimulation time\tdt       \tCFL      \tU change\tV change\tW change"
+              "\tR change\tE change")
+
+
+def test_log_header(lib):
+    assert flowio.log_header(0) == LOG_HEADER + "\n"
+    two = flowio.log_header(2)
+    cols = two.rstrip("\n").split("\t")
+    assert cols[:10] == LOG_HEADER.split("\t")
+    assert cols[10:] == [v + str(i) + " " * 12 for i in (1, 2) for v in "UVWRE"]
+
+
+def test_log_tracked_values(lib):
+    """main.F90:490-501: with a tracked norm R - 1 and E/norm - 1; exact zeros are "0"."""
+    line = flowio.log_line(0, 7, 0.25, 1e-4, 0.9, [1e-3] * 5, [[0.0, 0.5, 0.0, 1.0, 2.0]], tracked_norm=4.0)
+    f = line.rstrip(" \n").split("\t")
+    assert f[10:] == ["0", " 5.00000000E-01", "0", "0", "-5.00000000E-01"]
+    line = flowio.log_line(5, 7, 0.25, 1e-4, 0.9, [1e-3] * 5, [[0.0, 0.5, 0.0, 1.0, 2.0]])
+    f = line.rstrip(" \n").split("\t")
+    assert f[0] == "         5" and f[10:] == ["0", " 5.00000000E-01", "0", " 1.00000000E+00", " 2.00000000E+00"]
+    assert line.endswith(" \n")
+
+
+LOG_FORTRAN = """
+program logfmt
+  implicit none
+  integer :: n, i, nsave, tstep
+  real*8 :: v(10)
+  open(1, file='in.bin', access='stream', form='unformatted', status='old')
+  open(2, file='out.txt', status='replace')
+  read(1) n
+  do i = 1, n
+    read(1) nsave, tstep, v
+    write(2,'(I10,A1,I10,A1,ES13.8E1,A1,ES10.5E1,A1,F8.5,5(A1,ES10.4E2))',advance='no') nsave, char(9), tstep, &
+      & char(9), v(1), char(9), v(2), char(9), v(3), char(9), v(4), char(9), v(5), char(9), v(6), char(9), v(7), &
+      & char(9), v(8)
+    write(2,'(A1,ES15.8E2,A1,ES15.8E2)',advance='no') char(9), v(9), char(9), v(10)
+    write(2,'(A)') ''
+  enddo
+  close(2)
+end program
+"""
+
+
+def _log_values(rng, count):
+    """Rows of (nsave, tstep, t, dt, cfl, 5 changes, 2 tracked values) over many magnitudes."""
+    rows = []
+    specials = [0.0, -0.0, np.nan, np.inf, -np.inf, 9.99995, 9.999949999, 0.5, 123.5, 1e-10, 9.5e9, 1e100, -1e-100]
+    for i in range(count):
+        mag = 10.0 ** rng.uniform(-14, 14, 10)
+        sgn = np.where(rng.random(10) < 0.3, -1.0, 1.0)
+        v = mag * sgn
+        if i % 4 == 0:
+            v[rng.integers(0, 10, 3)] = rng.choice(specials, 3)
+        if i % 3 == 0:
+            v[2:5] = np.abs(v[2:5]) * np.array([1e-9, 1e-4, 1e-9])    # cfl in the F8.5 range
+        nsave = int(rng.integers(0, 2 ** 31)) * (1 if i % 5 else -1)          # I10 overflows at -10^9
+        rows.append((nsave, int(rng.integers(0, 2 ** 31)), v))
+    return rows
+
+
+def test_log_line_matches_fortran_edit_descriptors(lib, tmp_path):
+    import struct
+    import subprocess
+    from gatt_dns_amd.fortran.build_driver import FC
+    if not os.path.exists(FC):
+        pytest.skip("no Fortran compiler")
+    rows = _log_values(np.random.Generator(np.random.PCG64(11)), 400)
+    with open(tmp_path / "in.bin", "wb") as f:
+        f.write(struct.pack("<i", len(rows)))
+        for nsave, tstep, v in rows:
+            f.write(struct.pack("<ii", nsave, tstep) + np.asarray(v, "<f8").tobytes())
+    (tmp_path / "t.f90").write_text(LOG_FORTRAN)
+    r = subprocess.run([FC, "-O0", "t.f90", "-o", "t"], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr[-2000:]
+    r = subprocess.run(["./t"], capture_output=True, text=True, cwd=tmp_path, timeout=60)
+    assert r.returncode == 0, r.stderr
+    want = (tmp_path / "out.txt").read_text().split("\n")
+    for k, (nsave, tstep, v) in enumerate(rows):
+        tracked = [[v[8], v[9], 0.0, 0.0, 0.0]]
+        line = flowio.log_line(nsave, tstep, v[0], v[1], v[2], v[3:8], tracked)
+        got = line[:-2].split("\t")                                # drop " \n"
+        exp = want[k].split("\t")
+        # the tracked zeros are "0" by the reference's branch, not an edit descriptor
+        tr = [exp[10] if v[8] != 0 else "0", exp[11] if v[9] != 0 else "0"]
+        assert got[:10] == exp[:10], (k, v)
+        assert got[10:12] == tr, (k, v)
+
+
 FORTRAN_PROGRAM = """
 program t
   use iso_c_binding
@@ -141,8 +276,10 @@ program t
   real(c_double), target :: U2(nx,ny,nz), V2(nx,ny,nz), W2(nx,ny,nz), R2(nx,ny,nz), E2(nx,ny,nz)
   integer(c_int8_t), target :: wall(nx,ny,nz), wall2(nx,ny,nz)
   type(c_ptr) :: q(5), q2(5)
-  real(c_double) :: t2
-  integer(c_int) :: rc
+  real(c_double) :: t2, mc(5), tr(5)
+  integer(c_int) :: rc, i
+  character(kind=c_char, len=256) :: line
+  character(len=110) :: ref
   call random_number(U); call random_number(V); call random_number(W)
   call random_number(R); call random_number(E)
   wall = 0_c_int8_t
@@ -157,6 +294,24 @@ program t
   if (any(wall2 /= wall)) stop 4
   if (any(U2(2:,:,:) /= U(2:,:,:)) .or. any(E2(2:,:,:) /= E(2:,:,:))) stop 5
   if (any(U2(1,:,:) /= 0d0) .or. any(R2(1,:,:) /= 1d0)) stop 6
+  rc = gatt_meanflow_write('meanflowSFD.mat'//c_null_char, nx, ny, nz, q, c_loc(wall))
+  if (rc /= 0) stop 7
+  rc = gatt_meanflow_read('meanflowSFD.mat'//c_null_char, nx, ny, nz, q2, c_loc(wall))
+  if (rc /= 0) stop 8
+  if (any(V2(2:,:,:) /= V(2:,:,:)) .or. any(E2(1,:,:) /= 1d0)) stop 9
+  rc = gatt_sfd_write('SFD.mat'//c_null_char, nx, ny, nz, R)
+  if (rc /= 0) stop 10
+  rc = gatt_sfd_read('SFD.mat'//c_null_char, nx, ny, nz, R2)
+  if (rc /= 0 .or. any(R2 /= R)) stop 11
+  mc = (/ 1d-3, 2d-3, 0d0, 4d-3, 5d-3 /)
+  tr = (/ 1d0, 2d0, 0d0, 1d0, 1d0 /)
+  rc = gatt_log_line(0, 10, 0.5d0, 1d-4, 0.25d0, mc, 1, tr, 0d0, line, int(len(line), c_int64_t))
+  if (rc /= 178) stop 12
+  write(ref,'(I10,A1,I10,A1,ES13.8E1,A1,ES10.5E1,A1,F8.5,5(A1,ES10.4E2))') 0, char(9), 10, char(9), 0.5d0, &
+    & char(9), 1d-4, char(9), 0.25d0, char(9), mc(1), char(9), mc(2), char(9), mc(3), char(9), mc(4), char(9), mc(5)
+  do i = 1, 110
+    if (line(i:i) /= ref(i:i)) stop 13
+  end do
   print *, 'fortran flow io ok'
 end program
 """

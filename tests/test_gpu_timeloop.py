@@ -1,0 +1,149 @@
+"""The main loop around the step (main.F90:170-531) and the tracked-point gather.
+
+* ``gatt_get_points`` returns exactly the state values at the requested points
+  (W = 0 in 2-D) and rejects points outside the grid.
+* Fixed dt (timeControl 1): the loop's captured stretches plus the final
+  ``gatt_step`` give the state of the same number of single steps bit for bit;
+  flow files appear at the save steps and hold the state of that step; the log
+  has the header and one line per ``logAll`` step and per save, with the save
+  number only on save lines, the time accumulated step by step, the CFL of the
+  state before the step, and the tracked values of the state after it.
+* Maximum CFL (timeControl 2): dt divides the save interval, stays under
+  maxCFL / CFLdt of the interval's first state, and the run ends with the first
+  step that reaches tmax.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from gatt_dns_amd.prep import cases
+
+
+@pytest.fixture(scope="module")
+def rt():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from gatt_dns_amd import runtime
+    runtime.load_library()
+    return runtime
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mk", [lambda: cases.lid_driven(33), lambda: cases.turbulence_box(16)], ids=["2d", "3d"])
+def test_get_points(rt, mk):
+    case = mk()
+    s = rt.Solver(case)
+    try:
+        s.set_state(case.state0)
+        s.step(0.0, case.dt)
+        st = s.get_state()
+        rng = np.random.default_rng(3)
+        pts = np.stack([rng.integers(0, n, 40) for n in (case.nx, case.ny, case.nz)], axis=1)
+        got = s.get_points(pts)
+        for f, k in enumerate("UVWRE"):
+            want = st[k][pts[:, 0], pts[:, 1], pts[:, 2]]
+            np.testing.assert_array_equal(got[:, f], want if case.is3d or k != "W" else 0.0)
+        assert s.get_points(np.zeros((0, 3))).shape == (0, 5)
+        with pytest.raises(rt.GattError):
+            s.get_points([[case.nx, 0, 0]])
+    finally:
+        s.close()
+
+
+def _moving(case, amp=0.02):
+    """The initial state plus a smooth velocity perturbation, so that every step changes the flow."""
+    st = {k: np.array(v, order="F", copy=True) for k, v in case.state0.items()}
+    x = np.linspace(0, 1, case.nx)[:, None, None]
+    y = np.linspace(0, 1, case.ny)[None, :, None]
+    st["U"] = np.asfortranarray(st["U"] + amp * np.sin(np.pi * x) * np.sin(2 * np.pi * y))
+    st["V"] = np.asfortranarray(st["V"] + amp * np.sin(2 * np.pi * x) * np.sin(np.pi * y))
+    return st
+
+
+def _log_rows(path):
+    lines = open(path).read().split("\n")
+    return lines[0], [l for l in lines[1:] if l]
+
+
+@pytest.mark.gpu
+def test_fixed_dt_loop(rt, tmp_path):
+    from gatt_dns_amd import flowio
+    from gatt_dns_amd.timeloop import TimeLoop
+    case = cases.lid_driven(33)
+    tracked = [[3, 4, 0], [16, 30, 0]]
+    s = rt.Solver(case)
+    ref = rt.Solver(case)
+    try:
+        q0 = _moving(case)
+        s.set_state(q0)
+        ref.set_state(q0)
+        loop = TimeLoop(s, case, str(tmp_path), time_control=1, qtimes=5, tmax=12, log_all=2, tracked=tracked,
+                        tracked_norm=case.E0)
+        t, tstep, nsave, diverged = loop.run()
+        assert (tstep, nsave, diverged) == (12, 10, False)
+        # the same 12 steps one at a time
+        tr, rows, saved = 0.0, {}, {}
+        for k in range(1, 13):
+            um, mc, dv = ref.step(tr, case.dt)
+            tr += case.dt
+            rows[k] = (um, mc, ref.get_points(tracked))
+            if k % 5 == 0:
+                saved[k] = ref.get_state()
+        assert t == tr
+        got, want = s.get_state(), ref.get_state()
+        for k in "UVRE":
+            np.testing.assert_array_equal(got[k], want[k])
+        for k in (5, 10):
+            q, tf, _ = flowio.read_flow(os.path.join(tmp_path, flowio.flow_file_name(k)), loop.shape)
+            assert tf == pytest.approx(k * case.dt, rel=1e-15)
+            fr = case.flow_region.reshape(loop.shape, order="F")      # walls come back as 0 / 1 by design
+            for f in "UVRE":
+                np.testing.assert_array_equal(q[f][fr], saved[k][f][fr])
+        assert not os.path.exists(os.path.join(tmp_path, flowio.flow_file_name(12)))
+        header, lines = _log_rows(os.path.join(tmp_path, "log.txt"))
+        assert header == flowio.log_header(2).rstrip("\n")
+        steps = [int(l.split("\t")[1]) for l in lines]
+        assert steps == [2, 4, 5, 6, 8, 10, 12]
+        tt = 0.0
+        times = {}
+        for k in range(1, 13):
+            tt += case.dt
+            times[k] = tt
+        for l, k in zip(lines, steps):
+            um, mc, vals = rows[k]
+            cfl = case.dt * ((1 / case.Ma + um[0]) * case.dxmin[0] + (1 / case.Ma + um[1]) * case.dxmin[1])
+            want = flowio.log_line(k if k % 5 == 0 else 0, k, times[k], case.dt, cfl, mc, vals, case.E0)
+            assert l + "\n" == want, (k, l, want)
+    finally:
+        s.close()
+        ref.close()
+
+
+@pytest.mark.gpu
+def test_cfl_loop(rt, tmp_path):
+    from gatt_dns_amd.prep.cases import cfl_dt
+    from gatt_dns_amd.timeloop import TimeLoop
+    case = cases.lid_driven(33)
+    s = rt.Solver(case)
+    try:
+        q0 = _moving(case)
+        s.set_state(q0)
+        cdt = cfl_dt(q0, case.Ma, case.dxmin, case.nz)
+        qtimes = 3.3 * case.dt
+        loop = TimeLoop(s, case, str(tmp_path), time_control=2, dt=10 * case.dt, qtimes=qtimes, tmax=2 * qtimes,
+                        max_cfl=0.5, log_all=1000)
+        t, tstep, nsave, diverged = loop.run()
+        dt = min(0.5 / cdt, 10 * case.dt)
+        steps = int(np.floor(qtimes / dt)) + 1
+        assert qtimes / steps <= 0.5 / cdt
+        assert not diverged and nsave >= 2 and t >= 2 * qtimes - 1e-12
+        lines = open(os.path.join(tmp_path, "log.txt")).read().split("\n")[1:]
+        lines = [l for l in lines if l]
+        assert int(lines[0].split("\t")[0]) == 1 and int(lines[0].split("\t")[1]) == steps
+        # the second interval takes its dt from its own first state; t may land an ulp short of
+        # tmax after it, and then one more step ends the run
+        assert int(lines[1].split("\t")[0]) == 2 and int(lines[1].split("\t")[1]) in (tstep, tstep - 1)
+    finally:
+        s.close()
