@@ -147,3 +147,39 @@ def test_cfl_loop(rt, tmp_path):
         assert int(lines[1].split("\t")[0]) == 2 and int(lines[1].split("\t")[1]) in (tstep, tstep - 1)
     finally:
         s.close()
+
+
+@pytest.mark.gpu
+def test_automean(rt, tmp_path):
+    """runDNSautomean.m: the averaged flow file, the log2 lines, the continued runs and the stop
+    after n_means passes; the averaged file is the in-order sum of the last files over their count."""
+    from gatt_dns_amd import automean, flowio
+    from gatt_dns_amd.timeloop import TimeLoop
+    case = cases.lid_driven(33)
+    shape = (case.nx, case.ny, case.nz)
+    s = rt.Solver(case)
+    try:
+        s.set_state(_moving(case))
+        TimeLoop(s, case, str(tmp_path), time_control=1, qtimes=2, tmax=8, log_all=1).run()
+        assert automean.flow_files(str(tmp_path))[-1] == flowio.flow_file_name(8)
+        passes = automean.run_automean(s, case, str(tmp_path), files_for_mean=3, spacing_for_mean=2, n_means=2,
+                                       tol=0.0, time_control=1, qtimes=2, log_all=1)
+        assert passes == 2
+        names = automean.flow_files(str(tmp_path))
+        steps = [int(n[5:15]) for n in names]
+        assert steps == [2, 4, 6, 8, 9, 10, 12, 13, 14, 16]
+        log2 = open(os.path.join(tmp_path, "log2.txt")).read().split("\n")
+        assert log2[:2] == ["Mean flow calculated from flow_0000000004.mat to flow_0000000008.mat",
+                            "Mean flow calculated from flow_0000000009.mat to flow_0000000012.mat"]
+        fr = case.flow_region.reshape(shape, order="F")
+        parts = [flowio.read_flow(os.path.join(tmp_path, flowio.flow_file_name(k)), shape) for k in (4, 6, 8)]
+        mean, t9, _ = flowio.read_flow(os.path.join(tmp_path, flowio.flow_file_name(9)), shape)
+        assert t9 == parts[-1][1]
+        for k in "UVRE":
+            want = ((0.0 + parts[0][0][k]) + parts[1][0][k]) + parts[2][0][k]
+            np.testing.assert_array_equal(mean[k][fr], (want / 3)[fr])
+        # the last log line is the last step of the second continued run
+        assert int(open(os.path.join(tmp_path, "log.txt")).read().strip().split("\n")[-1].split("\t")[1]) == 16
+        assert automean.last_change(os.path.join(tmp_path, "log.txt")) > 0
+    finally:
+        s.close()
