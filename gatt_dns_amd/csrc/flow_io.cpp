@@ -145,6 +145,37 @@ int write_vars(const char* path, int32_t nx, int32_t ny, int32_t nz, const doubl
   return GATT_OK;
 }
 
+// a double or complex double array of MATLAB size dims[0] x dims[1] x ... (column-major data);
+// complex values are the compound {real, imag} MATLAB stores in a 7.3 file
+bool put_array(hid_t file, const char* name, int rank, const int64_t* dims, const double* re, const double* im) {
+  std::vector<hsize_t> d(rank);
+  size_t n = 1;
+  for (int i = 0; i < rank; ++i) {
+    d[rank - 1 - i] = (hsize_t)dims[i];
+    n *= (size_t)dims[i];
+  }
+  if (!im) return put_var(file, name, rank, d.data(), re);
+  hid_t ct = H5Tcreate(H5T_COMPOUND, 2 * sizeof(double));
+  H5Tinsert(ct, "real", 0, H5T_NATIVE_DOUBLE);
+  H5Tinsert(ct, "imag", sizeof(double), H5T_NATIVE_DOUBLE);
+  hid_t ft = H5Tcreate(H5T_COMPOUND, 16);
+  H5Tinsert(ft, "real", 0, H5T_IEEE_F64LE);
+  H5Tinsert(ft, "imag", 8, H5T_IEEE_F64LE);
+  std::vector<double> buf(2 * n);
+  for (size_t k = 0; k < n; ++k) {
+    buf[2 * k] = re[k];
+    buf[2 * k + 1] = im[k];
+  }
+  hid_t sp = H5Screate_simple(rank, d.data(), nullptr);
+  hid_t ds = H5Dcreate2(file, name, ft, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+  bool ok = ds >= 0 && H5Dwrite(ds, ct, H5S_ALL, H5S_ALL, H5P_DEFAULT, buf.data()) >= 0 && put_class(ds);
+  if (ds >= 0) H5Dclose(ds);
+  H5Sclose(sp);
+  H5Tclose(ft);
+  H5Tclose(ct);
+  return ok;
+}
+
 bool check_sizes(int32_t nx, int32_t ny, int32_t nz) { return nx >= 1 && ny >= 1 && nz >= 1; }
 
 // ---- Fortran edit descriptors of the log line ------------------------------------------
@@ -290,6 +321,33 @@ int gatt_meanflow_read(const char* path, int32_t nx, int32_t ny, int32_t nz, dou
       q[4][n] = 1;
     }
   }
+  return GATT_OK;
+}
+
+int gatt_mat_write(const char* path, int32_t nvars, const char* const* names, const int32_t* ranks,
+                   const int64_t* dims, const double* const* re, const double* const* im) {
+  if (!path || nvars < 1 || !names || !ranks || !dims || !re) return fail(GATT_E_INVALID, "null argument");
+  H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+  hid_t fcpl = H5Pcreate(H5P_FILE_CREATE);
+  H5Pset_userblock(fcpl, 512);
+  hid_t file = H5Fcreate(path, H5F_ACC_TRUNC, fcpl, H5P_DEFAULT);
+  H5Pclose(fcpl);
+  if (file < 0) return fail(GATT_E_INVALID, std::string("cannot create ") + path);
+  bool ok = true;
+  const int64_t* d = dims;
+  for (int32_t v = 0; ok && v < nvars; ++v) {
+    if (!names[v] || !re[v] || ranks[v] < 2 || ranks[v] > 8) {
+      ok = false;
+      break;
+    }
+    for (int i = 0; i < ranks[v]; ++i)
+      if (d[i] < 0) ok = false;
+    if (ok) ok = put_array(file, names[v], ranks[v], d, re[v], im ? im[v] : nullptr);
+    d += ranks[v];
+  }
+  if (H5Fclose(file) < 0) ok = false;
+  if (!ok) return fail(GATT_E_INVALID, std::string("writing ") + path + " failed (names, ranks 2..8, sizes)");
+  if (!put_header(path)) return fail(GATT_E_INVALID, std::string("cannot write the MAT header of ") + path);
   return GATT_OK;
 }
 

@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libgatt_flowio.so")
 EXPORTS = ["gatt_flow_last_error", "gatt_flow_write", "gatt_flow_read", "gatt_flow_write_deriv",
            "gatt_meanflow_write", "gatt_meanflow_read", "gatt_sfd_write", "gatt_sfd_read", "gatt_log_header",
-           "gatt_log_line"]
+           "gatt_log_line", "gatt_mat_write"]
 
 _lib = None
 
@@ -48,6 +48,8 @@ def load_library():
         L.gatt_log_header.argtypes = [C.c_int32, C.c_char_p, C.c_int64]
         L.gatt_log_line.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int32,
                                     C.c_void_p, C.c_double, C.c_char_p, C.c_int64]
+        L.gatt_mat_write.argtypes = [C.c_char_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(DP),
+                                     C.c_void_p]
         _lib = L
     return _lib
 
@@ -178,3 +180,31 @@ def append_log(path, line):
     """Append one line to log.txt, writing the header first if the file does not exist."""
     with open(path, "a") as f:
         f.write(line)
+
+
+def write_mat(path, variables):
+    """MATLAB ``save(path, ..., '-v7.3')`` of ``variables`` (name -> real or complex array; a 1-D
+    array is stored as a column, a scalar as 1x1)."""
+    L = load_library()
+    names, ranks, dims, re, im, keep = [], [], [], [], [], []
+    for name, a in variables.items():
+        a = np.asarray(a)
+        if a.ndim == 0:
+            a = a.reshape(1, 1)
+        elif a.ndim == 1:
+            a = a.reshape(-1, 1)
+        r = np.asfortranarray(a.real, dtype=np.float64)
+        i = np.asfortranarray(a.imag, dtype=np.float64) if np.iscomplexobj(a) else None
+        keep += [r, i]
+        names.append(os.fsencode(name))
+        ranks.append(a.ndim)
+        dims += list(a.shape)
+        re.append(r.ctypes.data_as(DP))
+        im.append(None if i is None else i.ctypes.data_as(DP))
+    n = len(names)
+    cn = (C.c_char_p * n)(*names)
+    cr = (C.c_int32 * n)(*ranks)
+    cd = (C.c_int64 * len(dims))(*dims)
+    cre = (DP * n)(*re)
+    cim = (DP * n)(*[p if p is not None else DP() for p in im])
+    _check(L.gatt_mat_write(os.fsencode(path), n, cn, cr, cd, cre, cim))

@@ -362,7 +362,7 @@ def calc_instability(case, base, T, krylov_dim=64, block=32, epsilon=1e-7, first
     start = "random": ``block`` seeded N(0,1) vectors (numpy PCG64 ``seed``);
     start = "bump": calcInitialDisturb's Gaussian in the first column, random
     in the others.  Returns a dict with lambda, mu, H, the Krylov dimension, the
-    propagator's step count and dt, and ``modes`` (list of field dicts of the
+    propagator's step count and dt, the flow region (point indices), and ``modes`` (list of field dicts of the
     real and imaginary parts of the first ``n_modes`` Ritz vectors,
     vec2modes of calcInstability.m:437-446).  ``single_beta`` selects the
     singleBeta packing (one spanwise wavenumber, the z mesh one period of it;
@@ -382,7 +382,7 @@ def calc_instability(case, base, T, krylov_dim=64, block=32, epsilon=1e-7, first
                                 save_every=save_every)
         lam, mu, y = ritz(H, krylov_dim, st.T)
         out = {"lambda": lam, "mu": mu, "H": H, "m": krylov_dim, "nsteps": st.nsteps, "dt": st.dt,
-               "members_advanced": st.advanced_members}
+               "members_advanced": st.advanced_members, "region": np.asarray(st.region)}
         if n_modes:
             k = min(n_modes, krylov_dim)
             basis = V[0:krylov_dim]
@@ -396,3 +396,43 @@ def calc_instability(case, base, T, krylov_dim=64, block=32, epsilon=1e-7, first
         return out
     finally:
         st.close()
+
+
+# ------------------------------------------------------------ output files
+def write_results(path, V, H, m):
+    """The results file of calcInstability.m:419 (MAT 7.3): zeta (the n x m Krylov basis, one
+    column per vector), H (the Hessenberg matrix) and M (the Krylov dimension)."""
+    from . import flowio
+    flowio.write_mat(path, {"zeta": np.asarray(V[:m]).T, "H": np.asarray(H), "M": float(m)})
+
+
+def modes_fields(case, region, out, single_beta=False):
+    """vec2modes (calcInstability.m:584-688) of the Ritz vectors in ``out["modes"]``: mode i is
+    divided by its largest modulus and placed on the flow region, NaN elsewhere (W stays NaN in
+    2-D); a mode whose eigenvalue has a negative imaginary part stays NaN (the conjugate of
+    another).  Returns U..E of size (nx, ny, nz or 1, k), complex."""
+    k = len(out["modes"])
+    nz = 1 if (single_beta or case.nz == 1) else case.nz
+    shape = (case.nx, case.ny, nz)
+    fields = {f: np.full(shape + (k,), np.nan + 0j, order="F") for f in "UVWRE"}
+    used = "UVRE" if case.nz == 1 else "UVWRE"
+    for i, (re, im) in enumerate(out["modes"]):
+        if out["lambda"][i].imag < 0:
+            continue
+        flat = {f: (re[f] + 1j * im[f]).reshape(-1, order="F")[region] for f in used}
+        scale = max(np.max(np.abs(flat[f])) for f in used)
+        for f in used:
+            a = np.full(shape[0] * shape[1] * shape[2], np.nan + 0j)
+            a[region] = flat[f] / scale
+            fields[f][..., i] = a.reshape(shape, order="F")
+    return fields
+
+
+def write_modes(path, case, region, out, single_beta=False):
+    """The modes file of calcInstability.m:437-452 (MAT 7.3): U, V, W, R, E of
+    :func:`modes_fields`, lambda (every Ritz value, sorted by real part), X, Y, Z and nModes."""
+    from . import flowio
+    f = modes_fields(case, region, out, single_beta)
+    f.update({"lambda": np.asarray(out["lambda"]), "X": np.asarray(case.X, float), "Y": np.asarray(case.Y, float),
+              "Z": np.asarray(case.Z, float), "nModes": float(len(out["modes"]))})
+    flowio.write_mat(path, f)

@@ -64,6 +64,15 @@ int gatt_sfd_read(const char* path, int32_t nx, int32_t ny, int32_t nz, double* 
  * variable SFD_X. */
 int gatt_sfd_write(const char* path, int32_t nx, int32_t ny, int32_t nz, const double* sfd_x);
 
+/* Any set of double or complex double arrays as one MAT 7.3 file (MATLAB's
+ * save(..., '-v7.3') of the variables): variable v is names[v], of MATLAB size
+ * dims[o_v .. o_v + ranks[v] - 1] (ranks 2..8, the ranks concatenated in
+ * dims), column-major data re[v]; when im and im[v] are not NULL the variable
+ * is complex, stored as MATLAB stores it (compound {real, imag}).  Used for the
+ * results and modes files of calcInstability.m:419, 450-452. */
+int gatt_mat_write(const char* path, int32_t nvars, const char* const* names, const int32_t* ranks,
+                   const int64_t* dims, const double* const* re, const double* const* im);
+
 /* log.txt.  The header line of runDNS.m:58-66 (column names, five per tracked
  * point) and one line of main.F90:467-506 with its Fortran edit descriptors:
  * I10 save number (0 between saves), I10 step, ES13.8E1 t, ES10.5E1 dt, F8.5 CFL,

@@ -333,3 +333,100 @@ def test_fortran_interfaces(lib, tmp_path):
     assert r.returncode == 0, r.stderr[-3000:]
     r = subprocess.run([str(exe)], capture_output=True, text=True, cwd=tmp_path, timeout=60)
     assert r.returncode == 0 and "fortran flow io ok" in r.stdout, r.stdout + r.stderr
+
+
+def _read_dataset(h, f, name, complex_=False):
+    d = h.H5Dopen2(f, name.encode(), 0)
+    assert d >= 0, name
+    sp = h.H5Dget_space(d)
+    nd = h.H5Sget_simple_extent_ndims(sp)
+    dims = (C.c_uint64 * nd)()
+    h.H5Sget_simple_extent_dims(sp, dims, None)
+    n = int(np.prod(list(dims)))
+    native = C.c_int64.in_dll(h, "H5T_NATIVE_DOUBLE_g").value
+    h.H5Dread.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_void_p]
+    if complex_:
+        h.H5Tcreate.restype = C.c_int64
+        h.H5Tcreate.argtypes = [C.c_int, C.c_size_t]
+        h.H5Tinsert.argtypes = [C.c_int64, C.c_char_p, C.c_size_t, C.c_int64]
+        t = h.H5Tcreate(6, 16)                                           # H5T_COMPOUND
+        h.H5Tinsert(t, b"real", 0, native)
+        h.H5Tinsert(t, b"imag", 8, native)
+        buf = np.zeros(2 * n)
+        assert h.H5Dread(d, t, 0, 0, 0, buf.ctypes.data) >= 0
+        h.H5Tclose(t)
+        data = buf[0::2] + 1j * buf[1::2]
+    else:
+        buf = np.zeros(n)
+        assert h.H5Dread(d, native, 0, 0, 0, buf.ctypes.data) >= 0
+        data = buf
+    h.H5Sclose(sp)
+    h.H5Dclose(d)
+    return list(dims), data
+
+
+def test_mat_write_real_and_complex(lib, tmp_path):
+    """gatt_mat_write: MATLAB sizes reversed into HDF5 dimensions, column-major data, complex
+    values as the compound {real, imag}, 1-D arrays as columns and scalars as 1x1."""
+    rng = np.random.Generator(np.random.PCG64(8))
+    a = np.asfortranarray(rng.standard_normal((4, 3, 1, 2)))
+    z = np.asfortranarray(rng.standard_normal((5, 2)) + 1j * rng.standard_normal((5, 2)))
+    v = rng.standard_normal(6)
+    path = tmp_path / "m.mat"
+    flowio.write_mat(path, {"A": a, "Z": z, "v": v, "s": 3.0})
+    raw = open(path, "rb").read()
+    assert raw[:19] == b"MATLAB 7.3 MAT-file" and raw[512:516] == b"\x89HDF"
+    h = _h5()
+    f = h.H5Fopen(os.fsencode(path), 0, 0)
+    dims, data = _read_dataset(h, f, "A")
+    assert dims == [2, 1, 3, 4]
+    np.testing.assert_array_equal(data, a.reshape(-1, order="F"))
+    dims, data = _read_dataset(h, f, "Z", complex_=True)
+    assert dims == [2, 5]
+    np.testing.assert_array_equal(data, z.reshape(-1, order="F"))
+    dims, data = _read_dataset(h, f, "v")
+    assert dims == [1, 6]
+    np.testing.assert_array_equal(data, v)
+    dims, data = _read_dataset(h, f, "s")
+    assert dims == [1, 1] and data[0] == 3.0
+    h.H5Fclose(f)
+
+
+def test_modes_file(lib, tmp_path):
+    """vec2modes: each mode divided by its largest modulus, NaN off the region, W NaN in 2-D,
+    modes of eigenvalues with negative imaginary part left NaN."""
+    from types import SimpleNamespace
+    from gatt_dns_amd import lst
+    case = SimpleNamespace(nx=6, ny=5, nz=1, X=np.linspace(0, 1, 6), Y=np.linspace(0, 2, 5), Z=np.zeros(1))
+    region = np.array([1, 2, 3, 7, 8, 9, 20, 21])
+    rng = np.random.Generator(np.random.PCG64(9))
+    modes = []
+    for _ in range(3):
+        re = {f: rng.standard_normal((6, 5, 1)) for f in "UVWRE"}
+        im = {f: rng.standard_normal((6, 5, 1)) for f in "UVWRE"}
+        modes.append((re, im))
+    out = {"modes": modes, "lambda": np.array([0.1 + 0.5j, 0.1 - 0.5j, -0.2 + 0.0j, -0.3 + 0.1j])}
+    fields = lst.modes_fields(case, region, out)
+    assert fields["U"].shape == (6, 5, 1, 3)
+    off = np.ones(30, bool)
+    off[region] = False
+    for i in (0, 2):
+        vals = np.concatenate([fields[f][..., i].reshape(-1, order="F")[region] for f in "UVRE"])
+        assert np.max(np.abs(vals)) == pytest.approx(1.0, rel=1e-15)
+        want = (modes[i][0]["V"] + 1j * modes[i][1]["V"]).reshape(-1, order="F")[region]
+        ratio = fields["V"][..., i].reshape(-1, order="F")[region] / want
+        np.testing.assert_allclose(ratio, ratio[0], rtol=1e-14)
+        assert np.all(np.isnan(fields["U"][..., i].reshape(-1, order="F")[off]))
+        assert np.all(np.isnan(fields["W"][..., i]))
+    assert np.all(np.isnan(fields["U"][..., 1]))
+    path = tmp_path / "modes-case-M4.mat"
+    lst.write_modes(path, case, region, out)
+    h = _h5()
+    f = h.H5Fopen(os.fsencode(path), 0, 0)
+    dims, data = _read_dataset(h, f, "U", complex_=True)
+    assert dims == [3, 1, 5, 6]
+    np.testing.assert_array_equal(np.isnan(data), np.isnan(fields["U"].reshape(-1, order="F")))
+    dims, data = _read_dataset(h, f, "lambda", complex_=True)
+    assert dims == [1, 4]
+    np.testing.assert_array_equal(data, out["lambda"])
+    h.H5Fclose(f)

@@ -159,7 +159,7 @@ def test_full_space_block_arnoldi_closed_form(torch_cuda):
     assert np.abs(out["lambda"][:4]).max() < 1e-6
 
 
-def test_turbulence_box_driver_runs(torch_cuda):
+def test_turbulence_box_driver_runs(torch_cuda, tmp_path):
     from gatt_dns_amd import lst
     case = cases.turbulence_box(16, seed=7)
     out = lst.calc_instability(case, case.state0, 4 * case.dt, krylov_dim=16, block=8, epsilon=1e-7,
@@ -167,6 +167,14 @@ def test_turbulence_box_driver_runs(torch_cuda):
     assert np.all(np.isfinite(out["lambda"]))
     assert out["members_advanced"] == 8 * 3       # base run + two blocks
     assert len(out["modes"]) == 2 and np.all(np.isfinite(out["modes"][0][0]["U"]))
+    # the modes file of calcInstability.m:450-452
+    fields = lst.modes_fields(case, out["region"], out)
+    assert fields["U"].shape == (16, 16, 16, 2)
+    if out["lambda"][0].imag >= 0:
+        m0 = np.concatenate([fields[f][..., 0].ravel() for f in "UVWRE"])
+        assert np.isfinite(m0).all() and np.max(np.abs(m0)) == pytest.approx(1.0, rel=1e-14)
+    lst.write_modes(str(tmp_path / "modes-box16-M16.mat"), case, out["region"], out)
+    assert (tmp_path / "modes-box16-M16.mat").stat().st_size > 2 * 5 * 16 ** 3 * 16
 
 
 def test_single_beta_propagator_matches_oracle(torch_cuda):
