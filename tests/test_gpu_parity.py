@@ -347,6 +347,24 @@ def test_cavity_1000_steps(rt, oracle_mod):
         assert rel_err(got[k], want[k]) <= 1e-9, (k, rel_err(got[k], want[k]))
 
 
+def test_box_1000_steps(rt, oracle_mod):
+    """1000 fixed-dt steps of the forced 3-D turbulence box at 16^3 (SL4, filter every step,
+    turbulenceGenerator forcing).  A relative change of 1e-15 in the initial U grows to about 2e-13
+    over these steps in the oracle itself, so 1e-9 leaves room only for round-off, not for a
+    difference in the algorithm."""
+    case = cases.turbulence_box(16, seed=7)
+    print("box16 x1000", flush=True)
+    o = oracle_mod.Oracle(case)
+    want, mc_w, dv_w = o.run(case.state0, 0.0, case.dt, 1000)
+    s = rt.Solver(case)
+    s.set_state(case.state0)
+    mc, dv = s.advance(1000, 0.0, case.dt)
+    got = s.get_state()
+    assert not dv and not dv_w
+    for k in FIELDS3:
+        assert rel_err(got[k], want[k]) <= 1e-9, (k, rel_err(got[k], want[k]))
+
+
 def test_advance_equals_steps(rt):
     """The graph-replayed advance gives bit-identical states to step-by-step calls."""
     case = CASES["box32"]()
