@@ -36,7 +36,7 @@ def build(verbose=True):
         write_includes(mk(cases), inc)
         exe = os.path.join(LIB_DIR, "gatt_driver_%s" % name)
         cmd = [FC, "-O2", "-cpp", "-I", inc, "-I", work, os.path.join(HERE, "gatt_driver.F90"), mod_obj,
-               "-L", LIB_DIR, "-lgatt_dns", "-Wl,-rpath,$ORIGIN", "-o", exe]
+               "-L", LIB_DIR, "-lgatt_flowio", "-lgatt_dns", "-Wl,-rpath,$ORIGIN", "-o", exe]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError("amdflang failed for %s:\n%s" % (name, r.stderr[-4000:]))

@@ -43,7 +43,9 @@ program gatt_driver
   real(c_double), dimension(:,:,:), allocatable, target :: U, V, W, R, E
   type(gatt_problem_desc), target :: desc
   type(c_ptr) :: ctx, q(5)
-  real(c_double) :: t, dt, umax(3), maxChange(5)
+  real(c_double) :: t, dt, umax(3), maxChange(5), cfl, noTracked(5)
+  character(kind=c_char, len=1024) :: logLine
+  integer :: u_log, nLog
   integer(c_int32_t) :: diverged
   integer :: nsteps, tstep, i, rc, u_in, u_out
   character(len=512) :: arg, fin, fout
@@ -124,12 +126,27 @@ program gatt_driver
   rc = gatt_apply_bound_cond(ctx, t)
   call check(rc, 'gatt_apply_bound_cond')
 
+  ! log.txt with the header of runDNS.m:58-66 and a line per step (logAll = 1, main.F90:467-506)
+  noTracked = 0
+  nLog = gatt_log_header(0_c_int32_t, logLine, int(len(logLine), c_int64_t))
+  call check(min(nLog, 0), 'gatt_log_header')
+  open(newunit=u_log, file='log.txt', access='stream', form='unformatted', status='replace')
+  write(u_log) logLine(1:nLog)
+
   ! main.F90:192-533 with fixed dt (timeControl = 1)
   dt = dtmax
   do tstep = 1, nsteps
     rc = gatt_step(ctx, t, dt, umax, maxChange, diverged)
     call check(rc, 'gatt_step')
     t = t + dt
+    ! CFL of the state the step started from (main.F90:207-214, 252)
+    cfl = (1/Ma + umax(1))*dxmin(1) + (1/Ma + umax(2))*dxmin(2)
+    if (nz > 1) cfl = cfl + (1/Ma + umax(3))*dxmin(3)
+    cfl = dt*cfl
+    nLog = gatt_log_line(0_c_int32_t, int(tstep, c_int32_t), t, dt, cfl, maxChange, 0_c_int32_t, noTracked, 0d0, &
+                       & logLine, int(len(logLine), c_int64_t))
+    call check(min(nLog, 0), 'gatt_log_line')
+    write(u_log) logLine(1:nLog)
     if (mod(tstep, 100) == 0 .or. tstep == nsteps) then
       write(*,'(A9,I10,A13,ES14.6,A7,ES10.3)') 'Step = ', tstep, 'Time = ', t, 'dt = ', dt
       write(*,'(3(A13,ES10.4))') 'U change = ', maxChange(1), 'V change = ', maxChange(2), 'W change = ', maxChange(3)
@@ -141,6 +158,7 @@ program gatt_driver
     end if
   end do
 
+  close(u_log)
   rc = gatt_get_state(ctx, 0_c_int32_t, q)
   call check(rc, 'gatt_get_state')
   open(newunit=u_out, file=trim(fout), access='stream', form='unformatted', status='replace')

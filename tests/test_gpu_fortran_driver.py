@@ -25,12 +25,22 @@ def test_fortran_driver(oracle_mod, tmp_path, name, nsteps, tol):
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     np.concatenate([np.asarray(case.state0[k]).ravel(order="F") for k in "UVWRE"]).astype("<f8").tofile(fin)
     print("fortran driver", name, nsteps, flush=True)
-    r = subprocess.run([exe, str(nsteps), str(fin), str(fout)], capture_output=True, text=True, timeout=150)
+    r = subprocess.run([exe, str(nsteps), str(fin), str(fout)], capture_output=True, text=True, timeout=150,
+                       cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     got = np.fromfile(fout, dtype="<f8").reshape((5,) + shape[::-1]).transpose(0, 3, 2, 1)
     o = oracle_mod.Oracle(case)
     q0 = o.apply_bound_cond(case.state0, 0.0)
-    want, _, _ = o.run(q0, 0.0, case.dt, nsteps)
+    want, mc_want, _ = o.run(q0, 0.0, case.dt, nsteps)
+    # log.txt: the header, one line per step, the last line's changes those of the oracle's last step
+    from gatt_dns_amd import flowio
+    lines = (tmp_path / "log.txt").read_text().split("\n")
+    assert lines[0] + "\n" == flowio.log_header(0)
+    rows = [l.split("\t") for l in lines[1:] if l]
+    assert [int(r[1]) for r in rows] == list(range(1, nsteps + 1))
+    for k in range(5):
+        if mc_want[k] > 0:
+            assert float(rows[-1][5 + k]) == pytest.approx(mc_want[k], rel=1e-3)
     for i, k in enumerate("UVWRE"):
         if not case.is3d and k == "W":
             continue
