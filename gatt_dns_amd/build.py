@@ -51,7 +51,7 @@ def build_flowio(verbose=True):
     # the C++ runtime is linked in statically: the HDF5 directory on the run path must not
     # supply the process's libstdc++ (an older one there would break the HIP runtime)
     cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-static-libstdc++", "-static-libgcc", "-I", inc,
-           os.path.join(SRC, "flow_io.cpp"), "-L", libdir, "-lhdf5", "-Wl,-rpath," + libdir, "-o", tmp]
+           os.path.join(SRC, "flow_io.cpp"), "-L", libdir, "-lhdf5", "-lz", "-Wl,-rpath," + libdir, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("flow-file library build failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))

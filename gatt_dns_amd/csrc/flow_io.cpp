@@ -3,6 +3,7 @@
 // U,V,W,R,E), with the NaN convention of exportFlow.F90 / importFlow.F90:24-40.
 // Host code only; built against libhdf5 into its own library (libgatt_flowio.so).
 #include <hdf5.h>
+#include <zlib.h>
 
 #include <algorithm>
 #include <cmath>
@@ -176,6 +177,139 @@ bool put_array(hid_t file, const char* name, int rank, const int64_t* dims, cons
   return ok;
 }
 
+// ---- MAT v5 / v7 (MATLAB's default save format; v7 compresses each variable with zlib) ----
+// Level 5 MAT-file layout: a 128-byte header, then data elements of an 8-byte tag
+// (type, byte count; a "small" element packs both into 4 bytes and its data into the
+// next 4) padded to 8 bytes.  A variable is a miMATRIX element holding array flags,
+// dimensions, name and the real part, whose numeric type may be narrower than double.
+
+bool mat5_tag(const uint8_t* p, size_t left, uint32_t* type, uint32_t* nbytes, const uint8_t** data, size_t* adv) {
+  if (left < 8) return false;
+  uint32_t t, n;
+  std::memcpy(&t, p, 4);
+  std::memcpy(&n, p + 4, 4);
+  if (t >> 16) {            // small data element
+    *type = t & 0xffff;
+    *nbytes = t >> 16;
+    *data = p + 4;
+    *adv = 8;
+    return *nbytes <= 4;
+  }
+  *type = t;
+  *nbytes = n;
+  *data = p + 8;
+  *adv = 8 + ((size_t)n + 7) / 8 * 8;
+  if (t == 15) *adv = 8 + (size_t)n;     // miCOMPRESSED is not padded
+  return (size_t)n <= left - 8;
+}
+
+bool mat5_number(uint32_t type, const uint8_t* p, size_t i, double* v) {
+  switch (type) {
+    case 1: { int8_t x; std::memcpy(&x, p + i, 1); *v = x; return true; }
+    case 2: { uint8_t x; std::memcpy(&x, p + i, 1); *v = x; return true; }
+    case 3: { int16_t x; std::memcpy(&x, p + 2 * i, 2); *v = x; return true; }
+    case 4: { uint16_t x; std::memcpy(&x, p + 2 * i, 2); *v = x; return true; }
+    case 5: { int32_t x; std::memcpy(&x, p + 4 * i, 4); *v = x; return true; }
+    case 6: { uint32_t x; std::memcpy(&x, p + 4 * i, 4); *v = x; return true; }
+    case 7: { float x; std::memcpy(&x, p + 4 * i, 4); *v = x; return true; }
+    case 9: { std::memcpy(v, p + 8 * i, 8); return true; }
+    case 12: { int64_t x; std::memcpy(&x, p + 8 * i, 8); *v = (double)x; return true; }
+    case 13: { uint64_t x; std::memcpy(&x, p + 8 * i, 8); *v = (double)x; return true; }
+    default: return false;
+  }
+}
+
+size_t mat5_width(uint32_t type) {
+  switch (type) {
+    case 1: case 2: return 1;
+    case 3: case 4: return 2;
+    case 5: case 6: case 7: return 4;
+    case 9: case 12: case 13: return 8;
+    default: return 0;
+  }
+}
+
+// one miMATRIX element: 1 = found the variable (sizes checked, values in out), 0 = another
+// variable, -1 = malformed or sizes that do not match
+int mat5_matrix(const uint8_t* p, size_t n, const char* name, size_t want, double* out) {
+  uint32_t type, nb;
+  const uint8_t* d;
+  size_t adv;
+  if (!mat5_tag(p, n, &type, &nb, &d, &adv) || type != 6 || nb < 8) return -1;   // array flags
+  uint32_t flags;
+  std::memcpy(&flags, d, 4);
+  const bool complex_ = (flags >> 11) & 1;
+  p += adv; n -= adv;
+  if (!mat5_tag(p, n, &type, &nb, &d, &adv) || type != 5) return -1;             // dimensions
+  size_t count = 1;
+  for (uint32_t i = 0; i < nb / 4; ++i) {
+    int32_t x;
+    std::memcpy(&x, d + 4 * i, 4);
+    count *= (size_t)(x < 0 ? 0 : x);
+  }
+  p += adv; n -= adv;
+  if (!mat5_tag(p, n, &type, &nb, &d, &adv) || (type != 1 && type != 2)) return -1;   // name
+  const std::string vname((const char*)d, nb);
+  p += adv; n -= adv;
+  if (vname != name) return 0;
+  if ((flags & 0xff) != 6 || complex_ || count != want) return -1;               // mxDOUBLE_CLASS, real
+  if (!mat5_tag(p, n, &type, &nb, &d, &adv)) return -1;
+  const size_t w = mat5_width(type);
+  if (!w || nb != w * count) return -1;
+  for (size_t i = 0; i < count; ++i)
+    if (!mat5_number(type, d, i, out + i)) return -1;
+  return 1;
+}
+
+// the double array `name` of a v5/v7 MAT file: 1 found, 0 absent, -1 malformed / wrong size
+int mat5_read(const char* path, const char* name, size_t want, double* out) {
+  std::FILE* f = std::fopen(path, "rb");
+  if (!f) return -1;
+  std::vector<uint8_t> buf;
+  uint8_t chunk[65536];
+  size_t got;
+  while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
+  std::fclose(f);
+  if (buf.size() < 128 || buf[126] != 'I' || buf[127] != 'M') return -1;         // little-endian files only
+  size_t pos = 128;
+  while (pos + 8 <= buf.size()) {
+    uint32_t type, nb;
+    const uint8_t* d;
+    size_t adv;
+    if (!mat5_tag(buf.data() + pos, buf.size() - pos, &type, &nb, &d, &adv)) return -1;
+    int r = 0;
+    if (type == 14) {
+      r = mat5_matrix(d, nb, name, want, out);
+    } else if (type == 15) {
+      // inflate the element; its first 8 bytes are the inner miMATRIX tag
+      std::vector<uint8_t> inner(1 << 16);
+      z_stream zs{};
+      if (inflateInit(&zs) != Z_OK) return -1;
+      zs.next_in = const_cast<Bytef*>(d);
+      zs.avail_in = nb;
+      size_t have = 0;
+      int zr = Z_OK;
+      while (zr == Z_OK) {
+        if (have == inner.size()) inner.resize(inner.size() * 2);
+        zs.next_out = inner.data() + have;
+        zs.avail_out = (uInt)(inner.size() - have);
+        zr = inflate(&zs, Z_NO_FLUSH);
+        have = inner.size() - zs.avail_out;
+      }
+      inflateEnd(&zs);
+      if (zr != Z_STREAM_END) return -1;
+      uint32_t it, inb;
+      const uint8_t* idata;
+      size_t iadv;
+      if (!mat5_tag(inner.data(), have, &it, &inb, &idata, &iadv)) return -1;
+      if (it == 14) r = mat5_matrix(idata, inb, name, want, out);
+    }
+    if (r != 0) return r;
+    pos += adv;
+  }
+  return 0;
+}
+
 bool check_sizes(int32_t nx, int32_t ny, int32_t nz) { return nx >= 1 && ny >= 1 && nz >= 1; }
 
 // ---- Fortran edit descriptors of the log line ------------------------------------------
@@ -280,6 +414,13 @@ int gatt_sfd_read(const char* path, int32_t nx, int32_t ny, int32_t nz, double* 
   if (!path || !sfd_x) return fail(GATT_E_INVALID, "null argument");
   if (!check_sizes(nx, ny, nz)) return fail(GATT_E_INVALID, "grid sizes must be positive");
   H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+  if (H5Fis_hdf5(path) <= 0) {
+    // MATLAB's default save format (preprocessing.m:104 saves without '-v7.3')
+    const int r = mat5_read(path, "SFD_X", (size_t)nx * ny * nz, sfd_x);
+    if (r == 1) return GATT_OK;
+    return fail(GATT_E_INVALID, std::string(path) + (r == 0 ? ": no SFD_X" : ": not a readable MAT file, or SFD_X "
+                                                               "is not a real double array of the grid's size"));
+  }
   hid_t file = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
   if (file < 0) return fail(GATT_E_INVALID, std::string("cannot open ") + path + " as a MAT 7.3 (HDF5) file");
   bool present = false;

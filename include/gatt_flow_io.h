@@ -57,7 +57,9 @@ int gatt_meanflow_read(const char* path, int32_t nx, int32_t ny, int32_t nz, dou
 
 /* readSFD (readWriteMat.F90:87-135, main.F90:119-146, SFD type 2): the damping
  * field SFD_X of `path` (the reference's bin/SFD.mat, written by
- * preprocessing.m:104 from calcSFDregion.m).  Reads the MAT 7.3 layout only. */
+ * preprocessing.m:104 from calcSFDregion.m).  Reads MAT 7.3 files and MATLAB's
+ * default format (level 5 MAT files, v7 with zlib-compressed variables, little
+ * endian); SFD_X must be a real double array of nx*ny*nz values. */
 int gatt_sfd_read(const char* path, int32_t nx, int32_t ny, int32_t nz, double* sfd_x);
 
 /* The SFD.mat of the preprocessing (preprocessing.m:104) in the MAT 7.3 layout,

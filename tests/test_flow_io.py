@@ -430,3 +430,24 @@ def test_modes_file(lib, tmp_path):
     assert dims == [1, 4]
     np.testing.assert_array_equal(data, out["lambda"])
     h.H5Fclose(f)
+
+
+@pytest.mark.parametrize("compress", [True, False])
+@pytest.mark.parametrize("shape", [(7, 5, 3), (9, 6, 1)])
+def test_sfd_read_mat_v5(lib, tmp_path, compress, shape):
+    """readSFD of the SFD.mat preprocessing.m:104 writes in MATLAB's default format (v7: level 5
+    MAT file, zlib-compressed variables), checked against scipy's independent writer of that
+    format; another variable stored first, a size mismatch and a missing SFD_X are errors or skipped
+    as they should be.  (MATLAB may store integer-valued doubles in a narrower type; scipy does not,
+    so the narrow-type path is restated from the format description only.)"""
+    import scipy.io
+    x = np.asfortranarray(np.random.Generator(np.random.PCG64(12)).random(shape))
+    path = tmp_path / "SFD.mat"
+    arr = x if shape[2] > 1 else x[:, :, 0]
+    scipy.io.savemat(str(path), {"other": np.arange(5.0), "SFD_X": arr}, do_compression=compress, format="5")
+    np.testing.assert_array_equal(flowio.read_sfd(path, shape), x)
+    with pytest.raises(flowio.FlowFileError):
+        flowio.read_sfd(path, (shape[0] + 1,) + shape[1:])
+    scipy.io.savemat(str(path), {"other": np.arange(5.0)}, do_compression=compress, format="5")
+    with pytest.raises(flowio.FlowFileError, match="no SFD_X"):
+        flowio.read_sfd(path, shape)
