@@ -44,11 +44,13 @@ class TimeLoop:
     from ``max_cfl`` capped at ``dt``, a save every ``qtimes`` time units, the run ends
     at time ``tmax``.  ``tracked``: rows of 0-based (i, j, k) grid points whose values
     go into every log line; ``tracked_norm`` > 0 logs R - 1 and E/tracked_norm - 1
-    (``mesh.trackedNorm``, writeFortranParameters.m:27-31).
+    (``mesh.trackedNorm``, writeFortranParameters.m:27-31).  ``resume_mean``: start the SFD mean
+    from ``meanflowSFD.mat`` in ``outdir`` (``resumeMeanFlow``); ``save_mean`` (default: SFD with
+    Delta > 0) writes it at every save.
     """
 
     def __init__(self, solver, case, outdir, time_control=1, dt=None, qtimes=1, tmax=1, max_cfl=0.5, log_all=1,
-                 tracked=(), tracked_norm=0.0, save_mean=None):
+                 tracked=(), tracked_norm=0.0, save_mean=None, resume_mean=False):
         self.s, self.case, self.outdir = solver, case, outdir
         self.time_control = int(time_control)
         self.dtmax = float(case.dt if dt is None else dt)
@@ -58,6 +60,7 @@ class TimeLoop:
         self.tracked = np.asarray(tracked, dtype=np.int64).reshape(-1, 3)
         self.tracked_norm = float(tracked_norm)
         self.save_mean = (case.sfd_type > 0 and case.sfd_delta > 0) if save_mean is None else save_mean
+        self.resume_mean = bool(resume_mean)
         self.shape = (case.nx, case.ny, case.nz)
         self.log_path = os.path.join(outdir, "log.txt")
         if self.time_control not in (1, 2):
@@ -94,6 +97,10 @@ class TimeLoop:
         from); returns (t, tstep, nsave, diverged)."""
         os.makedirs(self.outdir, exist_ok=True)
         c = self.case
+        if self.resume_mean:
+            # resumeMeanFlow = 1 (main.F90:150-158, importMeanFlow.F90): the mean of the earlier run
+            wall = None if c.flow_region is None else ~np.asarray(c.flow_region, bool).reshape(self.shape, order="F")
+            self.s.set_mean(flowio.read_mean_flow(os.path.join(self.outdir, "meanflowSFD.mat"), self.shape, wall))
         t, nsave = float(t0), int(nsave0)
         if self.time_control == 1:
             tstep = nsave                                                # main.F90:170-172

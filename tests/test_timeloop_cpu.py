@@ -65,6 +65,12 @@ class FakeSolver:
     def set_state(self, q):
         self.q = {k: np.array(q[k], order="F", copy=True) for k in "UVWRE"}
 
+    def set_mean(self, q):
+        self.mean = {k: np.array(q[k], copy=True) for k in "UVWRE"}
+
+    def get_mean(self):
+        return self.mean
+
     def get_points(self, pts):
         return np.array([[self.q[k][tuple(p)] for k in "UVWRE"] for p in pts])
 
@@ -169,3 +175,24 @@ def test_automean_files(lib, tmp_path):
     q, _, _ = flowio.read_flow(os.path.join(tmp_path, "flow_0000000012.mat"), (4, 3, 1))
     assert np.all(q["U"] == 9.0)                       # restarted from the mean: 6 + 3 steps
     assert automean.run_automean(s, case, str(tmp_path), 3, 2, n_means=5, tol=1.0) == 0   # change below tol
+
+
+def test_mean_flow_resume_and_save(lib, tmp_path):
+    """resumeMeanFlow: the mean starts from meanflowSFD.mat (walls as importMeanFlow sets them);
+    with SFD Delta > 0 the mean is written at every save."""
+    from gatt_dns_amd.timeloop import TimeLoop
+    case = Case()
+    case.sfd_type, case.sfd_delta = 1, 0.5
+    fr = np.ones((4, 3, 1), bool)
+    fr[0] = False
+    case.flow_region = fr
+    mean = {k: np.full((4, 3, 1), v) for k, v in zip("UVWRE", (0.5, 0.25, 0.0, 1.5, 2.5))}
+    flowio.write_mean_flow(os.path.join(tmp_path, "meanflowSFD.mat"), mean, (4, 3, 1), flow_region=fr)
+    s = FakeSolver(case)
+    TimeLoop(s, case, str(tmp_path), qtimes=2, tmax=2, resume_mean=True).run()
+    assert np.all(s.mean["U"][fr] == 0.5) and np.all(s.mean["U"][~fr] == 0.0) and np.all(s.mean["E"][~fr] == 1.0)
+    # a later run without resume writes the solver's current mean at its save
+    s.set_mean({k: np.full((4, 3, 1), 7.0) for k in "UVWRE"})
+    TimeLoop(s, case, str(tmp_path), qtimes=2, tmax=4).run(t0=0.02, nsave0=2)
+    got = flowio.read_mean_flow(os.path.join(tmp_path, "meanflowSFD.mat"), (4, 3, 1))
+    assert np.all(got["U"][fr] == 7.0) and np.all(got["U"][~fr] == 0.0)
