@@ -61,20 +61,38 @@ def build_flowio(verbose=True):
     return FLOWIO_LIB
 
 
-def build(verbose=True):
-    """Compile every HIP source and link the shared library. Always recompiles."""
+def _stale(src, obj):
+    """An object is out of date when any source or header it may include is newer."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(SRC, src)] + [os.path.join(SRC, h) for h in os.listdir(SRC) if h.endswith(".h")]
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    deps += [os.path.join(inc, h) for h in os.listdir(inc) if h.endswith(".h")]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose=True, incremental=False):
+    """Compile every HIP source and link the shared library.  Always recompiles, unless
+    `incremental` is set: then objects kept in _lib/obj are reused when no source or header
+    is newer (a developer convenience; the default build starts from scratch)."""
     os.makedirs(OUT_DIR, exist_ok=True)
-    objs = [os.path.join(OUT_DIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
-    with ThreadPoolExecutor(max_workers=min(int(os.environ.get("MAX_JOBS", "8")), len(SOURCES))) as ex:
-        list(ex.map(lambda p: _compile(*p), zip(SOURCES, objs)))
+    obj_dir = os.path.join(OUT_DIR, "obj") if incremental else OUT_DIR
+    os.makedirs(obj_dir, exist_ok=True)
+    objs = [os.path.join(obj_dir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    todo = [(s, o) for s, o in zip(SOURCES, objs) if not incremental or _stale(s, o)]
+    if todo:
+        with ThreadPoolExecutor(max_workers=min(int(os.environ.get("MAX_JOBS", "8")), len(todo))) as ex:
+            list(ex.map(lambda p: _compile(*p), todo))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
     os.replace(tmp, LIB)
-    for o in objs:
-        os.remove(o)
+    if not incremental:
+        for o in objs:
+            os.remove(o)
     if verbose:
         print("built", LIB)
     build_flowio(verbose)
@@ -82,5 +100,5 @@ def build(verbose=True):
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
     sys.exit(0)

@@ -129,6 +129,9 @@ struct gatt_ctx {
   double sfdDelta = 0, sfdXval = 0;
   double* d_chi = nullptr;
   bool mean_ready = false;
+  // gatt_checkpoint: the state and SFD mean of every member (allocated on first use)
+  double* d_ckpt = nullptr;
+  bool ckpt_mean_ready = false;
   // reductions
   unsigned long long* d_red = nullptr;  // batch * 8
   double* d_rec = nullptr;              // 5
@@ -1178,6 +1181,7 @@ int gatt_destroy(gatt_ctx* c) {
   if (c->d_rmap) hipFree(c->d_rmap);
   if (c->d_base) hipFree(c->d_base);
   if (c->d_scratch) hipFree(c->d_scratch);
+  if (c->d_ckpt) hipFree(c->d_ckpt);
   if (c->d_points) hipFree(c->d_points);
   if (c->lin) hipFree(c->lin);
   if (c->tape) hipFree(c->tape);
@@ -1219,6 +1223,46 @@ int gatt_get_state(gatt_ctx* c, int32_t member, double* const* q) {
   int rc = copy_fields(c, member, c->state(c->cur), (const double* const*)q, false);
   if (rc) return rc;
   if (!c->is3d && q[GATT_W]) std::memset(q[GATT_W], 0, sizeof(double) * c->np);
+  return GATT_OK;
+}
+
+int gatt_get_previous_state(gatt_ctx* c, int32_t member, double* const* q) {
+  if (check_ctx(c) || !q) return fail(GATT_E_INVALID, "null argument");
+  int rc = copy_fields(c, member, c->state(1 - c->cur), (const double* const*)q, false);
+  if (rc) return rc;
+  if (!c->is3d && q[GATT_W]) std::memset(q[GATT_W], 0, sizeof(double) * c->np);
+  return GATT_OK;
+}
+
+int gatt_checkpoint(gatt_ctx* c, int32_t op) {
+  if (check_ctx(c)) return GATT_E_INVALID;
+  if (op != 0 && op != 1) return fail(GATT_E_INVALID, "checkpoint op must be 0 (save) or 1 (restore)");
+  // the state and, with SFD, its mean: ten arrays per member
+  const long long per = 10 * c->np;
+  if (!c->d_ckpt) {
+    if (op == 1) return fail(GATT_E_INVALID, "no checkpoint saved");
+    void* p = nullptr;
+    if (hipMalloc(&p, sizeof(double) * (size_t)per * c->batch) != hipSuccess)
+      return fail(GATT_E_NOMEM, "checkpoint allocation failed");
+    c->d_ckpt = static_cast<double*>(p);
+  }
+  const bool mean = c->sfdType > 0;
+  for (int m = 0; m < c->batch; ++m) {
+    double* st = c->state(c->cur) + (long long)m * c->slab;
+    double* mn = c->slab_ptr(SL_MEAN) + (long long)m * c->slab;
+    double* ck = c->d_ckpt + (long long)m * per;
+    if (op == 0) {
+      launch_copy(ck, st, 5 * c->np, c->stream);
+      if (mean) launch_copy(ck + 5 * c->np, mn, 5 * c->np, c->stream);
+    } else {
+      launch_copy(st, ck, 5 * c->np, c->stream);
+      if (mean) launch_copy(mn, ck + 5 * c->np, 5 * c->np, c->stream);
+    }
+  }
+  if (op == 0) c->ckpt_mean_ready = c->mean_ready;
+  else c->mean_ready = c->ckpt_mean_ready;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
   return GATT_OK;
 }
 
@@ -1356,6 +1400,10 @@ int gatt_step(gatt_ctx* c, double t, double dt, double* umax, double* maxchange,
   if (check_ctx(c)) return GATT_E_INVALID;
   ensure_mean(c);
   launch_clock_set(c->d_clock, t, c->stream);
+  // the divergence flag describes this call's steps only
+  launch_clear_i32(c->d_flag, 1, c->stream);
+  // holdInlet captures the inflow at the first boundary application (main.F90:112)
+  if (c->d_hold && !c->hold_captured) run_bc(c, c->state(c->cur), 0.0);
   enqueue_step(c, c->cur, dt);
   c->cur = 1 - c->cur;
   HIPCHK(hipGetLastError());
@@ -1369,6 +1417,8 @@ int gatt_advance(gatt_ctx* c, int32_t nsteps, double t0, double dt, double* maxc
   // holdInlet captures the inflow at the first boundary application; a problem that holds its
   // inflow and has not had one gets it here (main.F90:112); otherwise the state is left as given
   launch_clock_set(c->d_clock, t0, c->stream);
+  // cleared outside the captured graph: the flag then accumulates over this call's steps
+  launch_clear_i32(c->d_flag, 1, c->stream);
   if (c->d_hold && !c->hold_captured) run_bc(c, c->state(c->cur), 0.0);
   int s = 0;
   if (c->graph && (c->graph_dt != dt || c->graph_cur != c->cur)) {
@@ -1803,6 +1853,7 @@ struct gatt_group {
   std::vector<gatt_ctx*> ctx;
   int mode = 0;                         // 0: one graph with a branch per member; 1: per-member graphs
   hipGraphExec_t graph = nullptr;
+  std::vector<hipGraphExec_t> mgraph;   // mode 1: one double-step graph per member
   std::vector<double> graph_dt;
   std::vector<int> graph_cur;
   std::vector<hipEvent_t> ev;           // fork/join events (members + 1)
@@ -1815,27 +1866,47 @@ void group_drop_graph(gatt_group* g) {
     (void)hipGraphExecDestroy(g->graph);
     g->graph = nullptr;
   }
+  for (hipGraphExec_t& x : g->mgraph)
+    if (x) (void)hipGraphExecDestroy(x);
+  g->mgraph.clear();
 }
 
 // One double step of every member, as parallel branches forked from the first
-// member's stream and joined back into it.
+// member's stream and joined back into it.  Inside the group graph every member
+// launches its direction solves in order on its own stream (the group already
+// overlaps whole members); the members' own setting is restored afterwards, so
+// their gatt_advance graphs keep the parallel direction branches.
 int group_capture(gatt_group* g, const double* dt) {
   gatt_ctx* c0 = g->ctx[0];
   hipGraph_t graph;
-  HIPCHK(hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
-  HIPCHK(hipEventRecord(g->ev[0], c0->stream));
-  for (size_t m = 1; m < g->ctx.size(); ++m) HIPCHK(hipStreamWaitEvent(g->ctx[m]->stream, g->ev[0], 0));
+  std::vector<char> fork(g->ctx.size());
+  for (size_t m = 0; m < g->ctx.size(); ++m) {
+    fork[m] = g->ctx[m]->fork_dirs;
+    g->ctx[m]->fork_dirs = false;
+  }
+  auto restore = [&]() {
+    for (size_t m = 0; m < g->ctx.size(); ++m) g->ctx[m]->fork_dirs = fork[m] != 0;
+  };
+  hipError_t e = hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    restore();
+    return fail(GATT_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+  }
+  (void)hipEventRecord(g->ev[0], c0->stream);
+  for (size_t m = 1; m < g->ctx.size(); ++m) (void)hipStreamWaitEvent(g->ctx[m]->stream, g->ev[0], 0);
   for (size_t m = 0; m < g->ctx.size(); ++m) {
     gatt_ctx* c = g->ctx[m];
     enqueue_step(c, c->cur, dt[m]);
     enqueue_step(c, 1 - c->cur, dt[m]);
   }
   for (size_t m = 1; m < g->ctx.size(); ++m) {
-    HIPCHK(hipEventRecord(g->ev[m], g->ctx[m]->stream));
-    HIPCHK(hipStreamWaitEvent(c0->stream, g->ev[m], 0));
+    (void)hipEventRecord(g->ev[m], g->ctx[m]->stream);
+    (void)hipStreamWaitEvent(c0->stream, g->ev[m], 0);
   }
-  HIPCHK(hipStreamEndCapture(c0->stream, &graph));
-  hipError_t e = hipGraphInstantiate(&g->graph, graph, nullptr, nullptr, 0);
+  e = hipStreamEndCapture(c0->stream, &graph);
+  restore();
+  if (e != hipSuccess) return fail(GATT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&g->graph, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return fail(GATT_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   g->graph_dt.assign(dt, dt + g->ctx.size());
@@ -1880,44 +1951,50 @@ int gatt_group_advance(gatt_group* g, int32_t nsteps, const double* t0, const do
   if (!g || !t0 || !dt) return fail(GATT_E_INVALID, "null argument");
   if (nsteps < 0) return fail(GATT_E_INVALID, "nsteps < 0");
   const size_t n = g->ctx.size();
-  for (size_t m = 0; m < n; ++m) launch_clock_set(g->ctx[m]->d_clock, t0[m], g->ctx[m]->stream);
-  // members of a group launch their direction solves on their own stream: the group
-  // already overlaps whole members as parallel branches
-  for (gatt_ctx* c : g->ctx) {
-    if (c->fork_dirs) {
-      c->fork_dirs = false;
-      if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-      }
-    }
+  for (size_t m = 0; m < n; ++m) {
+    launch_clock_set(g->ctx[m]->d_clock, t0[m], g->ctx[m]->stream);
+    launch_clear_i32(g->ctx[m]->d_flag, 1, g->ctx[m]->stream);
   }
   for (gatt_ctx* c : g->ctx) {
     ensure_mean(c);
     if (c->d_hold && !c->hold_captured) run_bc(c, c->state(c->cur), 0.0);
   }
   if (g->mode == 1) {
-    // per-member double-step graphs, launched round-robin on the members' streams
-    for (size_t m = 0; m < n; ++m) {
-      gatt_ctx* c = g->ctx[m];
-      if (c->graph && (c->graph_dt != dt[m] || c->graph_cur != c->cur)) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-      }
-      if (nsteps >= 2 && !c->graph) {
+    // per-member double-step graphs owned by the group (direction solves in order on
+    // the member's stream, as in the group graph), launched round-robin on the members' streams
+    bool stale = g->mgraph.size() != n;
+    for (size_t m = 0; m < n && !stale; ++m)
+      stale = !g->mgraph[m] || g->graph_dt[m] != dt[m] || g->graph_cur[m] != g->ctx[m]->cur;
+    if (stale) group_drop_graph(g);
+    if (nsteps >= 2 && g->mgraph.empty()) {
+      g->mgraph.assign(n, nullptr);
+      for (size_t m = 0; m < n; ++m) {
+        gatt_ctx* c = g->ctx[m];
+        const bool fork = c->fork_dirs;
+        c->fork_dirs = false;
         hipGraph_t gr;
-        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_step(c, c->cur, dt[m]);
-        enqueue_step(c, 1 - c->cur, dt[m]);
-        HIPCHK(hipStreamEndCapture(c->stream, &gr));
-        HIPCHK(hipGraphInstantiate(&c->graph, gr, nullptr, nullptr, 0));
-        HIPCHK(hipGraphDestroy(gr));
-        c->graph_dt = dt[m];
-        c->graph_cur = c->cur;
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+          enqueue_step(c, c->cur, dt[m]);
+          enqueue_step(c, 1 - c->cur, dt[m]);
+          e = hipStreamEndCapture(c->stream, &gr);
+        }
+        c->fork_dirs = fork;
+        if (e == hipSuccess) {
+          e = hipGraphInstantiate(&g->mgraph[m], gr, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(gr);
+        }
+        if (e != hipSuccess) {
+          group_drop_graph(g);
+          return fail(GATT_E_HIP, std::string("member graph capture: ") + hipGetErrorString(e));
+        }
       }
+      g->graph_dt.assign(dt, dt + n);
+      g->graph_cur.clear();
+      for (gatt_ctx* c : g->ctx) g->graph_cur.push_back(c->cur);
     }
     for (int s = 0; s + 2 <= nsteps; s += 2)
-      for (gatt_ctx* c : g->ctx) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+      for (size_t m = 0; m < n; ++m) HIPCHK(hipGraphLaunch(g->mgraph[m], g->ctx[m]->stream));
   } else {
     bool stale = !g->graph;
     for (size_t m = 0; m < n && !stale; ++m)
@@ -1945,11 +2022,8 @@ int gatt_group_advance(gatt_group* g, int32_t nsteps, const double* t0, const do
       gatt_ctx* c = g->ctx[m];
       enqueue_step(c, c->cur, dt[m]);
       c->cur = 1 - c->cur;
-      if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-      }
     }
+    // the state buffers swapped: the captured double steps start from the other one
     group_drop_graph(g);
   }
   HIPCHK(hipGetLastError());
@@ -2636,6 +2710,7 @@ int gatt_tlm_step(gatt_ctx* c, double t, double dt, double* maxchange, int32_t* 
     return fail(GATT_E_INVALID, "apply the boundary conditions once first (holdInlet captures the inflow)");
   if ((rc = ensure_lin(c))) return rc;
   ensure_mean(c);
+  launch_clear_i32(c->d_flag, 1, c->stream);
   enqueue_tlm_step(c, t, dt);
   HIPCHK(hipGetLastError());
   return read_scalars(c, nullptr, maxchange, diverged);

@@ -200,6 +200,8 @@ bool mat5_tag(const uint8_t* p, size_t left, uint32_t* type, uint32_t* nbytes, c
   *data = p + 8;
   *adv = 8 + ((size_t)n + 7) / 8 * 8;
   if (t == 15) *adv = 8 + (size_t)n;     // miCOMPRESSED is not padded
+  // padding past the end of a truncated buffer: the caller's next step lands on the end
+  if (*adv > left) *adv = left;
   return (size_t)n <= left - 8;
 }
 
@@ -261,8 +263,12 @@ int mat5_matrix(const uint8_t* p, size_t n, const char* name, size_t want, doubl
   return 1;
 }
 
-// the double array `name` of a v5/v7 MAT file: 1 found, 0 absent, -1 malformed / wrong size
-int mat5_read(const char* path, const char* name, size_t want, double* out) {
+// The double arrays names[0..nv) of a v5/v7 MAT file, the file read and each element
+// inflated once: found[v] = 1 for each variable present; returns 0, or -1 when the file
+// is malformed or a named variable is not a real double array of want[v] values.
+int mat5_read_vars(const char* path, int nv, const char* const* names, const size_t* want, double* const* out,
+                   int* found) {
+  for (int v = 0; v < nv; ++v) found[v] = 0;
   std::FILE* f = std::fopen(path, "rb");
   if (!f) return -1;
   std::vector<uint8_t> buf;
@@ -271,6 +277,17 @@ int mat5_read(const char* path, const char* name, size_t want, double* out) {
   while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
   std::fclose(f);
   if (buf.size() < 128 || buf[126] != 'I' || buf[127] != 'M') return -1;         // little-endian files only
+  auto match = [&](const uint8_t* d, uint32_t nb) {
+    for (int v = 0; v < nv; ++v) {
+      const int r = mat5_matrix(d, nb, names[v], want[v], out[v]);
+      if (r < 0) return -1;
+      if (r == 1) {
+        found[v] = 1;
+        return 0;
+      }
+    }
+    return 0;
+  };
   size_t pos = 128;
   while (pos + 8 <= buf.size()) {
     uint32_t type, nb;
@@ -279,7 +296,7 @@ int mat5_read(const char* path, const char* name, size_t want, double* out) {
     if (!mat5_tag(buf.data() + pos, buf.size() - pos, &type, &nb, &d, &adv)) return -1;
     int r = 0;
     if (type == 14) {
-      r = mat5_matrix(d, nb, name, want, out);
+      r = match(d, nb);
     } else if (type == 15) {
       // inflate the element; its first 8 bytes are the inner miMATRIX tag
       std::vector<uint8_t> inner(1 << 16);
@@ -302,12 +319,20 @@ int mat5_read(const char* path, const char* name, size_t want, double* out) {
       const uint8_t* idata;
       size_t iadv;
       if (!mat5_tag(inner.data(), have, &it, &inb, &idata, &iadv)) return -1;
-      if (it == 14) r = mat5_matrix(idata, inb, name, want, out);
+      if (it == 14) r = match(idata, inb);
     }
     if (r != 0) return r;
+    if (adv == 0) break;
     pos += adv;
   }
   return 0;
+}
+
+// the double array `name` of a v5/v7 MAT file: 1 found, 0 absent, -1 malformed / wrong size
+int mat5_read(const char* path, const char* name, size_t want, double* out) {
+  int found = 0;
+  const int r = mat5_read_vars(path, 1, &name, &want, &out, &found);
+  return r < 0 ? -1 : found;
 }
 
 bool check_sizes(int32_t nx, int32_t ny, int32_t nz) { return nx >= 1 && ny >= 1 && nz >= 1; }
@@ -529,19 +554,37 @@ int gatt_flow_read(const char* path, int32_t nx, int32_t ny, int32_t nz, double*
   for (int f = 0; f < 5; ++f)
     if (!q[f]) return fail(GATT_E_INVALID, "null field pointer");
   H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
-  hid_t file = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
-  if (file < 0) return fail(GATT_E_INVALID, std::string("cannot open ") + path + " as a MAT 7.3 (HDF5) file");
   const size_t np = (size_t)nx * ny * nz;
-  bool present = false;
-  bool ok = get_var(file, "t", 1, 1, 1, t, &present) && present;
-  for (int f = 0; ok && f < 5; ++f) {
-    ok = get_var(file, kNames[f], nx, ny, nz, q[f], &present);
-    if (ok && !present) {
-      if (f != 2) ok = false;
-      else std::memset(q[f], 0, np * sizeof(double));
+  bool ok = true;
+  if (H5Fis_hdf5(path) <= 0) {
+    // MATLAB's default save format: the averaged flow files of runDNSautomean.m:80 are
+    // saved without '-v7.3' (level 5 MAT file, variables possibly zlib-compressed)
+    const char* names[6] = {"t", kNames[0], kNames[1], kNames[2], kNames[3], kNames[4]};
+    const size_t want[6] = {1, np, np, np, np, np};
+    double* outs[6] = {t, q[0], q[1], q[2], q[3], q[4]};
+    int found[6];
+    if (mat5_read_vars(path, 6, names, want, outs, found) < 0)
+      return fail(GATT_E_INVALID, std::string(path) + ": not a readable MAT file, or a flow variable is not a real "
+                                                      "double array of the grid's size");
+    for (int v = 0; v < 6; ++v)
+      if (!found[v]) {
+        if (v == 3) std::memset(q[2], 0, np * sizeof(double));
+        else ok = false;
+      }
+  } else {
+    hid_t file = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (file < 0) return fail(GATT_E_INVALID, std::string("cannot open ") + path + " as a MAT 7.3 (HDF5) file");
+    bool present = false;
+    ok = get_var(file, "t", 1, 1, 1, t, &present) && present;
+    for (int f = 0; ok && f < 5; ++f) {
+      ok = get_var(file, kNames[f], nx, ny, nz, q[f], &present);
+      if (ok && !present) {
+        if (f != 2) ok = false;
+        else std::memset(q[f], 0, np * sizeof(double));
+      }
     }
+    H5Fclose(file);
   }
-  H5Fclose(file);
   if (!ok) return fail(GATT_E_INVALID, std::string(path) + ": missing variable or sizes that do not match the grid");
   // importFlow.F90:26-40
   for (size_t n = 0; n < np; ++n) {

@@ -244,6 +244,7 @@ struct ReduceArgs {
   int is3d;
 };
 void launch_clear_u64(unsigned long long* p, int n, hipStream_t st);
+void launch_clear_i32(int* p, int n, hipStream_t st);
 // out[5*i + f] = field f of `state` at point idx[i] (W = 0 in 2-D); idx checked by the caller
 void launch_gather_points(const double* state, long long np, int is3d, int npts, const long long* idx, double* out,
                           hipStream_t st);

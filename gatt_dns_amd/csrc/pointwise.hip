@@ -315,6 +315,15 @@ void launch_clear_u64(unsigned long long* p, int n, hipStream_t st) {
   hipLaunchKernelGGL(clear_u64_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
 }
 
+__global__ void clear_i32_kernel(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+void launch_clear_i32(int* p, int n, hipStream_t st) {
+  hipLaunchKernelGGL(clear_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+}
+
 // max |q_new - q| per field (main.F90:414-418) and max |U|,|V|,|W| of the new
 // state (main.F90:197-203 of the next step).  Non-negative doubles order like
 // their bit patterns, and a NaN pattern orders above +inf, so NaN propagates.

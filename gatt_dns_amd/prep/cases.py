@@ -397,8 +397,14 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
     rnd = lambda v: float(fortran_round(v))
     chi = calc_sfd_region(nx, ny, nz, ((nbuf, nbuf), (0, nbuf), (0, 0)), sfd_chi, apply=(2, 0, 2))
     name = "boundary_layer_%dx%d" % (nx, ny) + ("x%d" % nz if nz > 1 else "")
+    dist = []
+    if hold:
+        # a domain that starts past the leading edge holds its inflow plane at the initial
+        # flow (boundaryLayerIsothermal.m:53-67: holdInlet on x = X(1), density not held)
+        box = [1, 1, 1, ny, 1, nz]
+        dist.append(Disturbance("holdInlet", box, fortran_round(X[:1]), fortran_round(Y), fortran_round(Z), [0.0]))
     case = Case(name, nx, ny, nz, X, Y, Z, rnd(Re), rnd(Ma), rnd(Pr), rnd(gamma),
-                rnd(T0), deriv, filt, types, (1, 1, 1), -1.0, bi, [], [],
+                rnd(T0), deriv, filt, types, (1, 1, 1), -1.0, bi, dist, [],
                 sfd_type=2, sfd_delta=rnd(sfd_delta), sfd_x_val=rnd(sfd_chi),
                 sfd_x=np.asfortranarray(fortran_round(chi)), dxmin=_dxmin(X, Y, Z, nz))
     case.flow_region = fr

@@ -27,7 +27,8 @@ EXPORTS = ["gatt_last_error", "gatt_device_info", "gatt_create", "gatt_destroy",
            "gatt_perturb", "gatt_collect", "gatt_block_dot", "gatt_block_update", "gatt_group_create",
            "gatt_group_destroy", "gatt_group_advance", "gatt_set_perturbation", "gatt_get_perturbation",
            "gatt_copy_fields", "gatt_tlm_step", "gatt_adj_step", "gatt_line_solve_t", "gatt_euler",
-           "gatt_ssprk3", "gatt_set_single_beta", "gatt_get_points"]
+           "gatt_ssprk3", "gatt_set_single_beta", "gatt_get_points", "gatt_get_previous_state",
+           "gatt_checkpoint"]
 
 SLOT_STATE = 0
 SLOT_PERTURBATION = 1
@@ -66,7 +67,7 @@ def load_library(path=LIB_PATH):
     L.gatt_device_info.argtypes = [C.c_char_p, I]
     L.gatt_create.argtypes = [V, I, C.POINTER(V)]
     L.gatt_destroy.argtypes = [V]
-    for name in ("gatt_set_state", "gatt_get_state", "gatt_set_mean", "gatt_get_mean"):
+    for name in ("gatt_set_state", "gatt_get_state", "gatt_set_mean", "gatt_get_mean", "gatt_get_previous_state"):
         getattr(L, name).argtypes = [V, I, C.POINTER(DP)]
     L.gatt_get_points.argtypes = [V, I, I, V, V]
     L.gatt_rk4.argtypes = [V, C.c_double, C.c_double]
@@ -81,6 +82,7 @@ def load_library(path=LIB_PATH):
     L.gatt_advance.argtypes = [V, I, C.c_double, C.c_double, DP, C.POINTER(I)]
     L.gatt_line_solve.argtypes = [V, I, I, I, C.POINTER(DP), C.POINTER(DP)]
     L.gatt_sync.argtypes = [V]
+    L.gatt_checkpoint.argtypes = [V, I]
     L.gatt_time_kernel.argtypes = [V, I, I, DP, DP]
     L.gatt_set_line_shape.argtypes = [V, I, I, I, I, I]
     IPt = C.POINTER(I)
@@ -213,6 +215,16 @@ class Solver:
         dv = C.c_int32(0)
         _check(self.lib.gatt_advance(self.ctx, nsteps, t0, dt, mc.ctypes.data_as(DP), C.byref(dv)))
         return mc, bool(dv.value)
+
+    def get_previous_state(self, member=0):
+        """The state before the last step (the pre-step U of a diverged step)."""
+        out = [np.zeros(self.shape, order="F") for _ in range(5)]
+        _check(self.lib.gatt_get_previous_state(self.ctx, member, field_ptrs(out)))
+        return dict(zip("UVWRE", out))
+
+    def checkpoint(self, op):
+        """op 0: save the state (and SFD mean) of every member on the device; op 1: restore it."""
+        _check(self.lib.gatt_checkpoint(self.ctx, op))
 
     def sync(self):
         _check(self.lib.gatt_sync(self.ctx))

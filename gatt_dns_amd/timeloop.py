@@ -10,8 +10,10 @@ diverges (a NaN or an all-zero change; then ``flow_diverged.mat``).
 The steps between two log or save points run as one captured-graph replay
 (``gatt_advance``); the last step of each stretch runs through ``gatt_step``,
 which returns the max |U|, |V|, |W| the log's CFL needs.  The divergence flag of
-``gatt_advance`` accumulates over the stretch, so a divergence is reported at
-the end of the stretch it happened in, with the state of that point.
+``gatt_advance`` accumulates over the stretch; when it is raised, the stretch is
+run again one step at a time from a device checkpoint taken before it, so the
+loop stops at the step that diverged and writes the state before that step, as
+the reference does.
 """
 import os
 
@@ -82,14 +84,32 @@ class TimeLoop:
             flowio.write_mean_flow(os.path.join(self.outdir, "meanflowSFD.mat"), self.s.get_mean(), self.shape,
                                    flow_region=fr)
 
+    def _save_diverged(self, t):
+        """flow_diverged.mat: the state before the diverged step, with the time after it
+        (exportFlow runs before U = Unew, main.F90:423-441)."""
+        fr = self.case.flow_region
+        flowio.write_flow(os.path.join(self.outdir, flowio.flow_file_name(-1)), self.s.get_previous_state(), t,
+                          self.shape, flow_region=fr)
+
     def _stretch(self, n, t, dt):
-        """n steps: n - 1 replayed, the last through gatt_step; returns (umax, max_change, diverged)."""
-        diverged = False
+        """n steps: n - 1 replayed, the last through gatt_step.  Returns (steps run, umax,
+        max_change, diverged).  A divergence inside the replayed part is found again one
+        step at a time from a checkpoint taken before it, so the loop stops at the step
+        that diverged, as main.F90:433-448 does."""
         if n > 1:
+            self.s.checkpoint(0)
             _, dv = self.s.advance(n - 1, t, dt)
-            diverged = dv
+            if dv:
+                self.s.checkpoint(1)
+                tt = t
+                for k in range(1, n + 1):
+                    um, mc, dv = self.s.step(tt, dt)
+                    tt += dt
+                    if dv:
+                        return k, um, mc, True
+                return n, um, mc, False
         um, mc, dv = self.s.step(_advance_time(t, dt, n - 1), dt)
-        return um, mc, diverged or dv
+        return n, um, mc, dv
 
     # -- the loop --------------------------------------------------------------
     def run(self, t0=0.0, nsave0=0):
@@ -102,6 +122,9 @@ class TimeLoop:
             wall = None if c.flow_region is None else ~np.asarray(c.flow_region, bool).reshape(self.shape, order="F")
             self.s.set_mean(flowio.read_mean_flow(os.path.join(self.outdir, "meanflowSFD.mat"), self.shape, wall))
         t, nsave = float(t0), int(nsave0)
+        # the boundary conditions on the imported flow before the loop (main.F90:110-117); a
+        # held inflow (holdInlet) is captured here
+        self.s.apply_bound_cond(t)
         if self.time_control == 1:
             tstep = nsave                                                # main.F90:170-172
             qt, tmax = int(self.qtimes), int(self.tmax)
@@ -110,12 +133,12 @@ class TimeLoop:
                 # steps to the next log point, save point or the end (main.F90:218-229)
                 stop = min(tmax, (tstep // qt + 1) * qt, (tstep // self.log_all + 1) * self.log_all)
                 n = max(1, stop - tstep)
-                um, mc, diverged = self._stretch(n, t, dt)
+                n, um, mc, diverged = self._stretch(n, t, dt)
                 tstep += n
                 t = _advance_time(t, dt, n)
                 cfl = dt * _cfl_dt_from_umax(um, c.Ma, c.dxmin, c.nz)
                 if diverged:                                             # main.F90:434-444
-                    self._save(-1, t)
+                    self._save_diverged(t)
                     return t, tstep, -1, True
                 saving = tstep % qt == 0
                 if saving:
@@ -145,13 +168,13 @@ class TimeLoop:
                         n, last_in_run = j, True
                         break
                     tt += dt
-                um, mc, diverged = self._stretch(n, t, dt)
+                n, um, mc, diverged = self._stretch(n, t, dt)
                 tstep += n
                 t = _advance_time(t, dt, n)
                 left -= n
                 cfl = dt * _cfl_dt_from_umax(um, c.Ma, c.dxmin, c.nz)
                 if diverged:
-                    self._save(-1, t)
+                    self._save_diverged(t)
                     return t, tstep, -1, True
                 saving = left == 0
                 if tstep % self.log_all == 0 or saving:

@@ -222,6 +222,19 @@ int gatt_step(gatt_ctx* ctx, double t, double dt, double* umax, double* maxchang
  * which captures the inflow. */
 int gatt_advance(gatt_ctx* ctx, int32_t nsteps, double t0, double dt, double* maxchange, int32_t* diverged);
 
+/* The state before the last step (as gatt_get_state): after a gatt_step whose
+ * flow diverged it is the U that main.F90:433-441 writes to flow_diverged
+ * (exportFlow runs before U = Unew). */
+int gatt_get_previous_state(gatt_ctx* ctx, int32_t member, double* const* q);
+
+/* Device copy of the state (and, with SFD, the mean) of every member inside the
+ * context: op 0 saves, op 1 restores the saved copy (allocated on the first
+ * save).  No reference counterpart: the time loop saves before a replayed
+ * stretch, and when gatt_advance reports a divergence it restores and repeats
+ * the stretch one gatt_step at a time, so it stops at the diverged step as
+ * main.F90:433-448 does. */
+int gatt_checkpoint(gatt_ctx* ctx, int32_t op);
+
 /* Unit entry point: one operator along one direction on nfields device
  * arrays of nx*ny*nz doubles (calcDerivs{X,Y,Z} / one direction of
  * applyFilters: solveSystem{X,Y,Z}[per], source/Fortran/derivs.F90:556-948).
@@ -328,6 +341,7 @@ int gatt_collect(gatt_ctx* ctx, int32_t first, int32_t count, int32_t minus_firs
  * fixed order, so results are reproducible. */
 int gatt_block_dot(gatt_ctx* ctx, int64_t n, int32_t k, int32_t p, const double* X, int64_t ldx, const double* Y,
                    int64_t ldy, double* H);
+
 
 /* Y = alpha*Y + beta*X*M for X: n x k, M: k x p (host), Y: n x p -- the
  * update uPrime = uPrime - H(j,k)*zeta(:,j) (calcInstability.m:391-392), the

@@ -451,3 +451,43 @@ def test_sfd_read_mat_v5(lib, tmp_path, compress, shape):
     scipy.io.savemat(str(path), {"other": np.arange(5.0)}, do_compression=compress, format="5")
     with pytest.raises(flowio.FlowFileError, match="no SFD_X"):
         flowio.read_sfd(path, shape)
+
+
+@pytest.mark.parametrize("compress", [True, False])
+@pytest.mark.parametrize("shape", [(7, 5, 3), (9, 6, 1)])
+def test_flow_read_mat_v5(lib, tmp_path, compress, shape):
+    """The averaged flow files of runDNSautomean.m:80 are saved in MATLAB's default format (v7,
+    level 5); gatt_flow_read takes them as it takes MAT 7.3 files: t and U..E (W may be absent),
+    NaN inside walls replaced as importFlow.F90 does.  Written with scipy's independent writer."""
+    import scipy.io
+    rng = np.random.Generator(np.random.PCG64(5))
+    q = {k: np.asfortranarray(rng.random(shape)) for k in "UVWRE"}
+    q["U"][0, 0, 0] = np.nan
+    squeeze = (lambda a: a) if shape[2] > 1 else (lambda a: a[:, :, 0])
+    path = tmp_path / "flow_0000000009.mat"
+    data = {"t": np.array([[2.5]]), **{k: squeeze(v) for k, v in q.items()}}
+    if shape[2] == 1:
+        del data["W"]
+    scipy.io.savemat(str(path), data, do_compression=compress, format="5")
+    got, t, wall = flowio.read_flow(path, shape)
+    assert t == 2.5
+    assert wall[0, 0, 0] and wall.sum() == 1
+    for k in "UVWRE":
+        want = q[k].copy() if (k != "W" or shape[2] > 1) else np.zeros(shape)
+        want[0, 0, 0] = {"U": 0.0, "V": 0.0, "W": 0.0, "R": 1.0, "E": 1.0}[k]
+        np.testing.assert_array_equal(got[k], want)
+    scipy.io.savemat(str(path), {"t": np.array([[1.0]]), "U": squeeze(q["U"])}, format="5")
+    with pytest.raises(flowio.FlowFileError):
+        flowio.read_flow(path, shape)
+
+
+def test_sfd_read_truncated_v5(lib, tmp_path):
+    """A level 5 file cut inside an element's padding or data is an error, not a read past the buffer."""
+    import scipy.io
+    path = tmp_path / "SFD.mat"
+    scipy.io.savemat(str(path), {"SFD_X": np.arange(21.0).reshape(7, 3)}, format="5")
+    raw = path.read_bytes()
+    for cut in (len(raw) - 3, len(raw) - 20, 140, 131):
+        path.write_bytes(raw[:cut])
+        with pytest.raises(flowio.FlowFileError):
+            flowio.read_sfd(path, (7, 3, 1))

@@ -83,7 +83,8 @@ def test_fixed_dt_loop(rt, tmp_path):
                         tracked_norm=case.E0)
         t, tstep, nsave, diverged = loop.run()
         assert (tstep, nsave, diverged) == (12, 10, False)
-        # the same 12 steps one at a time
+        # the same 12 steps one at a time, after the loop's initial boundary application
+        ref.apply_bound_cond(0.0)
         tr, rows, saved = 0.0, {}, {}
         for k in range(1, 13):
             um, mc, dv = ref.step(tr, case.dt)
@@ -129,8 +130,12 @@ def test_cfl_loop(rt, tmp_path):
     s = rt.Solver(case)
     try:
         q0 = _moving(case)
+        # the first interval's dt comes from the state after the loop's initial boundary
+        # application (main.F90:110-117, then 232-246): the lid moves at U0
         s.set_state(q0)
-        cdt = cfl_dt(q0, case.Ma, case.dxmin, case.nz)
+        s.apply_bound_cond(0.0)
+        cdt = cfl_dt(s.get_state(), case.Ma, case.dxmin, case.nz)
+        s.set_state(q0)
         qtimes = 3.3 * case.dt
         loop = TimeLoop(s, case, str(tmp_path), time_control=2, dt=10 * case.dt, qtimes=qtimes, tmax=2 * qtimes,
                         max_cfl=0.5, log_all=1000)
@@ -181,5 +186,96 @@ def test_automean(rt, tmp_path):
         # the last log line is the last step of the second continued run
         assert int(open(os.path.join(tmp_path, "log.txt")).read().strip().split("\n")[-1].split("\t")[1]) == 16
         assert automean.last_change(os.path.join(tmp_path, "log.txt")) > 0
+    finally:
+        s.close()
+
+
+def _rel(a, b):
+    scale = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / scale) if scale else float(np.max(np.abs(a)))
+
+
+@pytest.mark.gpu
+def test_held_inflow_loop(rt, oracle_mod, tmp_path):
+    """A boundary layer that starts past the leading edge holds its inflow plane (holdInlet,
+    boundaryLayerIsothermal.m:53-67).  The loop's boundary application on the imported flow
+    (main.F90:110-117) captures it from the initial flow, also when every stretch is one step;
+    the result matches the oracle run with its own initial application, field by field."""
+    from gatt_dns_amd.timeloop import TimeLoop
+    case = cases.boundary_layer(nx=96, ny=48, nbuf=10, x_range=(30.0, 60.0))
+    assert [d.kind for d in case.disturb] == ["holdInlet"]
+    print("held-inflow boundary layer 96x48, 3 one-step stretches", flush=True)
+    o = oracle_mod.Oracle(case)
+    q0 = o.apply_bound_cond(case.state0, 0.0)
+    o.init_mean(q0)
+    want, _, _ = o.run(q0, 0.0, case.dt, 3)
+    s = rt.Solver(case)
+    try:
+        s.set_state(case.state0)
+        t, tstep, _, diverged = TimeLoop(s, case, str(tmp_path), time_control=1, qtimes=100, tmax=3,
+                                         log_all=1).run()
+        assert tstep == 3 and not diverged
+        got = s.get_state()
+        # each field on its own scale: V (about 1e-3) carries the round-off of the O(1)
+        # pressure terms, about 2e-14 absolute after three steps
+        for k in "UVRE":
+            assert _rel(got[k], want[k]) < 1e-10, (k, _rel(got[k], want[k]))
+    finally:
+        s.close()
+
+
+@pytest.mark.gpu
+def test_divergence_flag_per_call(rt):
+    """The divergence flag describes the steps of one call: a step whose change is NaN raises
+    it (main.F90:434), and later steps of a finite flow on the same context do not report it
+    again."""
+    case = cases.periodic_rest(16, 16)
+    s = rt.Solver(case)
+    try:
+        bad = {k: np.array(v, order="F", copy=True) for k, v in case.state0.items()}
+        bad["U"][3, 5, 0] = np.nan
+        s.set_state(bad)
+        _, mc, dv = s.step(0.0, case.dt)
+        assert dv and np.isnan(mc[0])
+        moving = {k: np.array(v, order="F", copy=True) for k, v in case.state0.items()}
+        x = np.arange(case.nx)[:, None, None] * (2 * np.pi / case.nx)
+        moving["U"] = np.asfortranarray(moving["U"] + 0.05 * np.sin(x) * np.ones(moving["U"].shape))
+        s.set_state(moving)
+        _, _, dv = s.step(0.0, case.dt)
+        assert not dv
+        _, dv = s.advance(4, 0.0, case.dt)
+        assert not dv
+    finally:
+        s.close()
+
+
+@pytest.mark.gpu
+def test_checkpoint_and_previous_state(rt):
+    """gatt_checkpoint restores the state (and the SFD mean) exactly, so a stretch repeated one
+    step at a time from it equals the replayed stretch bit for bit; gatt_get_previous_state is
+    the state before the last step."""
+    case = cases.boundary_layer(nx=96, ny=48, nbuf=10)
+    s = rt.Solver(case)
+    try:
+        s.set_state(case.state0)
+        s.apply_bound_cond(0.0)
+        before = s.get_state()
+        s.step(0.0, case.dt)
+        prev = s.get_previous_state()
+        for k in "UVWRE":
+            np.testing.assert_array_equal(prev[k], before[k])
+        t1 = case.dt
+        s.checkpoint(0)
+        s.advance(3, t1, case.dt)
+        replayed, mean_r = s.get_state(), s.get_mean()
+        s.checkpoint(1)
+        tt = t1
+        for _ in range(3):
+            s.step(tt, case.dt)
+            tt += case.dt
+        got, mean_g = s.get_state(), s.get_mean()
+        for k in "UVRE":
+            np.testing.assert_array_equal(got[k], replayed[k])
+            np.testing.assert_array_equal(mean_g[k], mean_r[k])
     finally:
         s.close()

@@ -38,13 +38,28 @@ class FakeSolver:
         self.shape = (case.nx, case.ny, case.nz)
         self.q = {k: np.full(self.shape, 1.0 if k in "RE" else 0.0, order="F") for k in "UVWRE"}
         self.calls = []
+        self.bc = []
         self.steps = 0
         self.diverge_at = diverge_at
 
     def _one(self):
+        self.prev = {k: v.copy() for k, v in self.q.items()}
         self.steps += 1
         self.q["U"] = self.q["U"] + 1.0
         return self.diverge_at is not None and self.steps >= self.diverge_at
+
+    def apply_bound_cond(self, t):
+        self.bc.append(t)
+
+    def checkpoint(self, op):
+        if op == 0:
+            self.saved = ({k: v.copy() for k, v in self.q.items()}, self.steps)
+        else:
+            q, self.steps = self.saved
+            self.q = {k: v.copy() for k, v in q.items()}
+
+    def get_previous_state(self):
+        return {k: v.copy() for k, v in self.prev.items()}
 
     def advance(self, n, t0, dt):
         self.calls.append(("advance", n, t0, dt))
@@ -117,9 +132,20 @@ def test_resume_and_divergence(lib, tmp_path):
     s = FakeSolver(case, diverge_at=4)
     loop = TimeLoop(s, case, str(tmp_path), time_control=1, qtimes=5, tmax=30, log_all=100)
     t, tstep, nsave, dv = loop.run(t0=1.0, nsave0=10)
-    # resumed at step 10: the first stretch runs to the save at 15 and carries the divergence
-    assert dv and nsave == -1 and tstep == 15
-    assert os.path.exists(os.path.join(tmp_path, "flow_diverged.mat"))
+    # resumed at step 10: the first stretch (to the save at 15) carries the divergence of its
+    # 4th step; it is run again one step at a time from the checkpoint and stops at step 14
+    assert dv and nsave == -1 and tstep == 14
+    kinds = [c[0] for c in s.calls]
+    assert kinds == ["advance", "step", "step", "step", "step"]
+    # the boundary conditions on the imported flow, once, at the start time (main.F90:110-117)
+    assert s.bc == [1.0]
+    tt = 1.0
+    for _ in range(4):
+        tt += case.dt
+    assert t == tt
+    # flow_diverged.mat holds the state before the diverged step with the time after it
+    q, tf, _ = flowio.read_flow(os.path.join(tmp_path, "flow_diverged.mat"), loop.shape)
+    assert tf == tt and np.all(q["U"] == 3.0)
     assert not os.path.exists(os.path.join(tmp_path, "flow_0000000015.mat"))
 
 
