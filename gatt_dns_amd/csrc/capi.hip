@@ -6,12 +6,20 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "gatt_internal.h"
 
 using namespace gatt;
+
+// Scratch of the allocation-free block product (gatt_block_dot_noalloc): the per-chunk
+// partial sums and the reduced k x p block.  Each device holds its own copy for the
+// life of the process; host code reaches it only through the address hipGetSymbolAddress
+// returns on that device (kept in g_scratch below), never through this name.
+constexpr size_t kStaticScratchDoubles = size_t(1) << 20;   // 8 MiB
+static __device__ __align__(256) double scratch[kStaticScratchDoubles];
 
 namespace {
 
@@ -28,6 +36,23 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess)                                                         \
       return fail(GATT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+// device address of `scratch` per device, looked up once with that device current
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+int init_static_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(GATT_E_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[dev]) return GATT_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(GATT_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch[dev] = static_cast<double*>(p);
+  return GATT_OK;
+}
 
 // Per-member slab: arrays of np doubles.
 enum : int {
@@ -81,6 +106,7 @@ bool overlap(const Box& a, const Box& b) {
 }  // namespace
 
 struct gatt_ctx {
+  int device = 0;              // the device current at gatt_create
   int nx = 0, ny = 0, nz = 0, is3d = 0, batch = 1;
   long long np = 0;
   long long slab = 0;          // doubles per member
@@ -1006,6 +1032,11 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
   if (batch < 1) return fail(GATT_E_INVALID, "batch must be >= 1");
   if (d->nx < 2 || d->ny < 2 || d->nz < 1) return fail(GATT_E_INVALID, "grid must be at least 2x2x1");
   std::unique_ptr<gatt_ctx> c(new gatt_ctx);
+  HIPCHK(hipGetDevice(&c->device));
+  {
+    const int rc = init_static_scratch(c->device);
+    if (rc) return rc;
+  }
   c->nx = d->nx;
   c->ny = d->ny;
   c->nz = d->nz;
@@ -1795,6 +1826,37 @@ int gatt_block_dot(gatt_ctx* c, int64_t n, int32_t k, int32_t p, const double* X
   launch_block_dot(a, c->d_scratch, c->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(H, c->d_scratch, sizeof(double) * kp, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GATT_OK;
+}
+
+int gatt_block_dot_noalloc(gatt_ctx* c, int64_t n, int32_t k, int32_t p, const double* X, int64_t ldx,
+                           const double* Y, int64_t ldy, double* H) {
+  if (check_ctx(c) || !X || !Y || !H) return fail(GATT_E_INVALID, "null argument");
+  if (n < 1 || k < 1 || p < 1 || ldx < n || ldy < n) return fail(GATT_E_INVALID, "bad block dimensions");
+  // the same chunking and kernels as gatt_block_dot, the partials in the static scratch
+  const int nch = block_dot_chunks(n);
+  const size_t kp = (size_t)k * p;
+  if (kp * ((size_t)nch + 1) > kStaticScratchDoubles)
+    return fail(GATT_E_INVALID, "block product needs " + std::to_string(kp * ((size_t)nch + 1)) +
+                                    " scratch doubles, more than the allocation-free limit of " +
+                                    std::to_string(kStaticScratchDoubles));
+  double* sp = g_scratch[c->device];
+  if (!sp) return fail(GATT_E_INVALID, "static scratch not set up on this device");
+  // every partial the reduction reads is written by the product kernel first, so the
+  // scratch needs no clearing between calls
+  BlockDotArgs a{};
+  a.n = n;
+  a.ldx = ldx;
+  a.ldy = ldy;
+  a.k = k;
+  a.p = p;
+  a.X = X;
+  a.Y = Y;
+  a.partial = sp + kp;
+  launch_block_dot(a, sp, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(H, sp, sizeof(double) * kp, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return GATT_OK;
 }

@@ -28,7 +28,7 @@ EXPORTS = ["gatt_last_error", "gatt_device_info", "gatt_create", "gatt_destroy",
            "gatt_group_destroy", "gatt_group_advance", "gatt_set_perturbation", "gatt_get_perturbation",
            "gatt_copy_fields", "gatt_tlm_step", "gatt_adj_step", "gatt_line_solve_t", "gatt_euler",
            "gatt_ssprk3", "gatt_set_single_beta", "gatt_get_points", "gatt_get_previous_state",
-           "gatt_checkpoint"]
+           "gatt_checkpoint", "gatt_block_dot_noalloc"]
 
 SLOT_STATE = 0
 SLOT_PERTURBATION = 1
@@ -94,6 +94,7 @@ def load_library(path=LIB_PATH):
     L.gatt_perturb.argtypes = [V, I, I, V, I64, C.c_double]
     L.gatt_collect.argtypes = [V, I, I, I, V, C.c_double, V, I64]
     L.gatt_block_dot.argtypes = [V, I64, I, I, V, I64, V, I64, DP]
+    L.gatt_block_dot_noalloc.argtypes = [V, I64, I, I, V, I64, V, I64, DP]
     L.gatt_block_update.argtypes = [V, I64, I, I, V, I64, DP, C.c_double, C.c_double, V, I64]
     L.gatt_group_create.argtypes = [C.POINTER(V), I, I, C.POINTER(V)]
     L.gatt_group_destroy.argtypes = [V]
@@ -296,15 +297,17 @@ class Solver:
         rp, _ = self._block(ref)
         _check(self.lib.gatt_collect(self.ctx, first, count, minus_first, rp, scale, wp, ld))
 
-    def block_dot(self, X, Y):
-        """H = X^T Y (k x p numpy array) for blocks X (k, n) and Y (p, n)."""
+    def block_dot(self, X, Y, noalloc=False):
+        """H = X^T Y (k x p numpy array) for blocks X (k, n) and Y (p, n).  ``noalloc`` calls
+        gatt_block_dot_noalloc (no device allocation; blocks up to the static scratch limit)."""
         xp, ldx = self._block(X)
         yp, ldy = self._block(Y)
         k, p, n = X.shape[0], Y.shape[0], X.shape[1]
         if X.dim() != 2 or Y.dim() != 2 or Y.shape[1] != n:
             raise GattError("X and Y must be (columns, n) blocks of the same n")
         H = np.zeros((k, p), order="F")
-        _check(self.lib.gatt_block_dot(self.ctx, n, k, p, xp, ldx, yp, ldy, H.ctypes.data_as(DP)))
+        fn = self.lib.gatt_block_dot_noalloc if noalloc else self.lib.gatt_block_dot
+        _check(fn(self.ctx, n, k, p, xp, ldx, yp, ldy, H.ctypes.data_as(DP)))
         return H
 
     def block_update(self, X, M, alpha, beta, Y):

@@ -342,6 +342,17 @@ int gatt_collect(gatt_ctx* ctx, int32_t first, int32_t count, int32_t minus_firs
 int gatt_block_dot(gatt_ctx* ctx, int64_t n, int32_t k, int32_t p, const double* X, int64_t ldx, const double* Y,
                    int64_t ldy, double* H);
 
+/* gatt_block_dot without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments, kernels
+ * and summation order as gatt_block_dot, so H is bitwise the same; the partial
+ * sums live in a fixed 8 MiB array of library device memory per device, set up
+ * at gatt_create, so k*p*(chunks+1) doubles must fit in 2^20, with
+ * chunks = min(512, ceil(n/8192)) (32 x 32 blocks fit at any n).  Larger blocks
+ * return GATT_E_INVALID.  Every call on one device shares that array: calls
+ * that overlap on one device (from several host threads or contexts) race. */
+int gatt_block_dot_noalloc(gatt_ctx* ctx, int64_t n, int32_t k, int32_t p, const double* X, int64_t ldx,
+                           const double* Y, int64_t ldy, double* H);
 
 /* Y = alpha*Y + beta*X*M for X: n x k, M: k x p (host), Y: n x p -- the
  * update uPrime = uPrime - H(j,k)*zeta(:,j) (calcInstability.m:391-392), the
