@@ -145,8 +145,10 @@ struct gatt_ctx {
   double* d_partial = nullptr;
   void* d_points = nullptr;       // gatt_get_points: point indices then gathered values
   int points_cap = 0;
-  double* d_apartial = nullptr;    // forcing-sum partials written by the assemble kernel (np % 256 == 0)
-  int ablocks = 0;                 // assemble blocks per member
+  double* d_apartial = nullptr;    // forcing-sum partials written by the kernel that wrote the stage state
+  int ablocks = 0;                 // assemble blocks per member (np % 256 == 0, else 0)
+  int apart_blocks = 0;            // partials per member the last producer wrote
+  bool fuse_rhs = false;           // y flux derivatives, assembly and stage update in one kernel
   bool sums_from_assemble = false; // the last assemble left the next stage's partials there
   double* d_sums = nullptr;
   int sum_blocks = 0;
@@ -738,6 +740,28 @@ void run_bc(gatt_ctx* c, double* q0, double toff, double* undo = nullptr) {
   c->hold_captured = true;
 }
 
+// Operator, line geometry and coefficient tables of the solve of `kind` along `dir`.
+void fill_solve_args(const gatt_ctx* c, int dir, int kind, SolveArgs& a) {
+  const DirOps& d = c->dir[dir];
+  a.op = d.op[kind];
+  a.types = d.types;
+  a.dir = dir;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  a.n1 = d.n1;
+  a.nlines = d.nlines;
+  a.stride = d.stride;
+  a.G = d.shape[kind].G;
+  a.coef = d.coef[kind];
+  a.coef_ends = d.coef_ends[kind];
+  a.fcoef = d.fcoef[kind];
+  a.fcoef_ends = d.fcoef_ends[kind];
+  a.fcoef_t = d.fcoef_t[kind];
+  a.fcoef_p = d.shape[kind].variant == 4 ? d.fcoef_p[kind][d.shape[kind].M == 4 ? 0 : (d.shape[kind].M == 8 ? 1 : 2)]
+                                          : nullptr;
+}
+
 // One line solve along `dir` for nf fields (member 0 pointers, member stride = slab).
 void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double* const* out, double theta,
                int mode, int nmembers, long long in_stride, long long out_stride, hipStream_t st = nullptr) {
@@ -753,16 +777,7 @@ void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double
     return;
   }
   SolveArgs a{};
-  a.op = d.op[kind];
-  a.types = d.types;
-  a.dir = dir;
-  a.nx = c->nx;
-  a.ny = c->ny;
-  a.nz = c->nz;
-  a.n1 = d.n1;
-  a.nlines = d.nlines;
-  a.stride = d.stride;
-  a.G = d.shape[kind].G;
+  fill_solve_args(c, dir, kind, a);
   a.nf = nf;
   a.in_stride = in_stride;
   a.out_stride = out_stride;
@@ -770,16 +785,42 @@ void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double
     a.in[f] = in[f];
     a.out[f] = out[f];
   }
-  a.coef = d.coef[kind];
-  a.coef_ends = d.coef_ends[kind];
-  a.fcoef = d.fcoef[kind];
-  a.fcoef_ends = d.fcoef_ends[kind];
-  a.fcoef_t = d.fcoef_t[kind];
-  a.fcoef_p = d.shape[kind].variant == 4 ? d.fcoef_p[kind][d.shape[kind].M == 4 ? 0 : (d.shape[kind].M == 8 ? 1 : 2)]
-                                          : nullptr;
   a.theta = theta;
   a.mode = mode;
   launch_line_solve(a, d.shape[kind], nmembers, st ? st : c->stream);
+}
+
+// The derivative solve along `dir` can take its input as viscous fluxes computed on the fly:
+// a fused wave-segment shape (variants 2 and 3) and a band the flux kernels are built for.
+bool flux_solve_ok(const gatt_ctx* c, int dir) {
+  const DirOps& d = c->dir[dir];
+  const int v = d.shape[GATT_DERIV].variant;
+  return !d.identity[GATT_DERIV] && (v == 2 || v == 3) && d.op[GATT_DERIV].nR <= kFluxMaxNR;
+}
+
+// Derivatives along `dir` of the viscous fluxes q[0..nf) of nmembers members: the inputs are
+// computed from E (member stride e_stride) and the first derivatives Dw (member stride ws) as
+// the kernel loads them (visc_flux); results in out (member stride ws).
+void run_solve_flux(gatt_ctx* c, int dir, int nf, const int* q, double* const* out, int nmembers, const double* E,
+                    long long e_stride, const double* Dw, long long ws, hipStream_t st) {
+  SolveArgs a{};
+  fill_solve_args(c, dir, GATT_DERIV, a);
+  a.nf = nf;
+  a.in_stride = e_stride;
+  a.out_stride = ws;
+  for (int f = 0; f < nf; ++f) {
+    a.in[f] = E;   // not read in flux mode
+    a.out[f] = out[f];
+    a.flux_q[f] = q[f];
+  }
+  a.has_flux = 1;
+  a.flux_E = E;
+  a.flux_D = Dw;
+  a.flux_d_stride = ws;
+  a.fc = c->fc;
+  a.is3d = c->is3d;
+  a.mode = 0;
+  launch_line_solve(a, c->dir[dir].shape[GATT_DERIV], nmembers, st);
 }
 
 // the pressureGradient forcing of the assemble kernel
@@ -827,7 +868,7 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
     if (c->sums_from_assemble && S == c->slab_ptr(SL_S)) {
       // the previous stage's assemble kernel summed the values it wrote here
       sa.partial = c->d_apartial;
-      sa.nblocks = c->ablocks;
+      sa.nblocks = c->apart_blocks;
       launch_forcing_final(sa, c->batch, c->stream);
     } else {
       sa.partial = c->d_partial;
@@ -836,10 +877,14 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
     }
   }
   c->sums_from_assemble = false;
+  const bool fused = c->fuse_rhs;
+  const int pblocks = fused ? rhs_line_blocks(c->dir[1].nlines) : c->ablocks;
   // the stage states of RK4 (stages 1-3) and SSPRK3 (6-7) feed the next stage's sums when no
   // boundary op touches them in between
-  const bool fuse_sums = c->forcing && c->is3d && c->d_apartial && c->waves.empty() && Out == c->slab_ptr(SL_S) &&
+  const bool fuse_sums = c->forcing && c->is3d && pblocks > 0 && c->waves.empty() && Out == c->slab_ptr(SL_S) &&
                          (stage == 1 || stage == 2 || stage == 3 || stage == 6 || stage == 7);
+  // second derivatives: x: txx,txy,txz,qx  y: txy,tyy,tyz,qy  z: txz,tyz,tzz,qz
+  const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
   for (int m0 = 0; m0 < c->batch; m0 += c->wgroup) {
     const int nm = std::min(c->wgroup, c->batch - m0);
     const long long mo = (long long)m0 * c->slab;
@@ -854,32 +899,6 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
         ++nf;
       }
       run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, c->slab, ws, dir_stream(d));
-    }
-    join();
-    ViscArgs va{};
-    va.c = c->fc;
-    va.np = np;
-    va.member_stride = c->slab;
-    va.work_stride = ws;
-    va.E = S + mo + 4 * np;
-    va.D = D;
-    va.T = T;
-    va.is3d = c->is3d;
-    launch_visc(va, nm, c->stream);
-    fork();
-    // second derivatives: x: txx,txy,txz,qx  y: txy,tyy,tyz,qy  z: txz,tyz,tzz,qz
-    const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
-    for (int d = 0; d < ndir; ++d) {
-      double* in[kMaxPtr];
-      double* out[kMaxPtr];
-      int nf = 0;
-      for (int q = 0; q < 4; ++q) {
-        if (!c->is3d && q == 2) continue;
-        in[nf] = T + (long long)tin3[d][q] * np;
-        out[nf] = G + (long long)(4 * d + q) * np;
-        ++nf;
-      }
-      run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, ws, ws, dir_stream(d));
     }
     join();
     AssembleArgs aa{};
@@ -901,10 +920,81 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
     aa.dt = dt;
     aa.is3d = c->is3d;
     set_pressure_gradient(c, aa);
-    aa.sum_partial = fuse_sums ? c->d_apartial + (long long)m0 * c->ablocks * 5 : nullptr;
+    aa.sum_partial = fuse_sums ? c->d_apartial + (long long)m0 * pblocks * 5 : nullptr;
+    if (fused) {
+      // x (and z) flux derivatives into G, the fluxes computed as the solves load them where
+      // the shape allows (otherwise from the stresses of visc_kernel), then the y flux
+      // derivatives, the assembly and the stage update in one kernel along y
+      const int other[2] = {0, 2};
+      const int nother = c->is3d ? 2 : 1;
+      bool need_t = false;
+      for (int k = 0; k < nother; ++k) need_t = need_t || !flux_solve_ok(c, other[k]);
+      if (need_t) {
+        ViscArgs va{};
+        va.c = c->fc;
+        va.np = np;
+        va.member_stride = c->slab;
+        va.work_stride = ws;
+        va.E = S + mo + 4 * np;
+        va.D = D;
+        va.T = T;
+        va.is3d = c->is3d;
+        launch_visc(va, nm, c->stream);
+      }
+      fork();
+      for (int k = 0; k < nother; ++k) {
+        const int d = other[k];
+        double* in[kMaxPtr];
+        double* out[kMaxPtr];
+        int q[kMaxPtr];
+        int nf = 0;
+        for (int j = 0; j < 4; ++j) {
+          if (!c->is3d && j == 2) continue;
+          q[nf] = tin3[d][j];
+          in[nf] = T + (long long)tin3[d][j] * np;
+          out[nf] = G + (long long)(4 * d + j) * np;
+          ++nf;
+        }
+        if (flux_solve_ok(c, d))
+          run_solve_flux(c, d, nf, q, out, nm, S + mo + 4 * np, c->slab, D, ws, dir_stream(d));
+        else
+          run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, ws, ws, dir_stream(d));
+      }
+      join();
+      RhsLineArgs r{};
+      fill_solve_args(c, 1, GATT_DERIV, r.s);
+      r.a = aa;
+      launch_rhs_line(r, nm, c->stream);
+      continue;
+    }
+    ViscArgs va{};
+    va.c = c->fc;
+    va.np = np;
+    va.member_stride = c->slab;
+    va.work_stride = ws;
+    va.E = S + mo + 4 * np;
+    va.D = D;
+    va.T = T;
+    va.is3d = c->is3d;
+    launch_visc(va, nm, c->stream);
+    fork();
+    for (int d = 0; d < ndir; ++d) {
+      double* in[kMaxPtr];
+      double* out[kMaxPtr];
+      int nf = 0;
+      for (int q = 0; q < 4; ++q) {
+        if (!c->is3d && q == 2) continue;
+        in[nf] = T + (long long)tin3[d][q] * np;
+        out[nf] = G + (long long)(4 * d + q) * np;
+        ++nf;
+      }
+      run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, ws, ws, dir_stream(d));
+    }
+    join();
     launch_assemble(aa, nm, c->stream);
   }
   c->sums_from_assemble = fuse_sums;
+  c->apart_blocks = pblocks;
 }
 
 // RK4_3D / RK4_2D: state `from` -> `to` (timeIntegration.F90:382-496); the stage
@@ -1122,6 +1212,16 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
         return fail(GATT_E_UNSUPPORTED, "no line-solve shape fits this line length");
     }
   }
+  // The y flux derivatives fused with the assembly (rhs_line.hip) where the y operator allows
+  // it (one line type per 64-line tile, lines of at most 256 points), only when
+  // GATT_FUSED_RHS=1: it moves fewer arrays but runs slower than the separate visc /
+  // second-derivative / assemble passes (two wavefronts per SIMD and a serial point loop
+  // in the fused kernel, the viscosity evaluated per flux), so those stay the default.
+  {
+    const DirOps& o = c->dir[1];
+    const char* e = std::getenv("GATT_FUSED_RHS");
+    c->fuse_rhs = e && e[0] == '1' && !o.identity[0] && o.uniform_tiles && rhs_line_fits(d->ny, o.op[0].nR);
+  }
   if ((rc = compile_bc(c.get(), d->bc))) return rc;
   for (int i = 0; i < d->nForcing; ++i) {
     if (d->forcing[i].kind == GATT_FORCING_PRESSURE_GRADIENT) {
@@ -1156,12 +1256,13 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
     HIPCHK(hipMalloc(&p, sizeof(double) * 5 * c->sum_blocks * batch));
     c->dev_allocs.push_back(p);
     c->d_partial = static_cast<double*>(p);
-    if (c->np % 256 == 0) {
-      c->ablocks = (int)(c->np / 256);
-      HIPCHK(hipMalloc(&p, sizeof(double) * 5 * (size_t)c->ablocks * batch));
-      c->dev_allocs.push_back(p);
-      c->d_apartial = static_cast<double*>(p);
-    }
+    // partial sums written by the producer of a stage state: assemble_kernel (256-point
+    // blocks) or the fused y-line kernel (one per 32 lines)
+    c->ablocks = c->np % 256 == 0 ? (int)(c->np / 256) : 0;
+    const int cap = std::max(c->ablocks, rhs_line_blocks(c->dir[1].nlines));
+    HIPCHK(hipMalloc(&p, sizeof(double) * 5 * (size_t)cap * batch));
+    c->dev_allocs.push_back(p);
+    c->d_apartial = static_cast<double*>(p);
     HIPCHK(hipMalloc(&p, sizeof(double) * 8 * batch));
     c->dev_allocs.push_back(p);
     c->d_sums = static_cast<double*>(p);
@@ -1575,17 +1676,38 @@ int gatt_time_kernel(gatt_ctx* c, int32_t kernel, int32_t reps, double* avg_ms, 
       int nf = 0;
       for (int f : flds) { in[nf] = S + (long long)f * np; out[nf] = D + (long long)(3 * f + d) * np; ++nf; }
       run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, c->slab, ws);
+      launches = 1;
+      arrays = 2.0 * nf;
+    } else if (kernel == GATT_K_FLUX_X || kernel == GATT_K_FLUX_Z) {
+      // flux derivatives, the fluxes computed from E and the first derivatives as they are loaded
+      const int d = kernel == GATT_K_FLUX_X ? 0 : 2;
       const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
-      int nf2 = 0;
-      for (int q = 0; q < 4; ++q) {
-        if (!c->is3d && q == 2) continue;
-        in[nf2] = T + (long long)tin3[d][q] * np;
-        out[nf2] = G + (long long)(4 * d + q) * np;
-        ++nf2;
+      double* out[kMaxPtr];
+      int q[kMaxPtr];
+      int nf = 0;
+      for (int j = 0; j < 4; ++j) {
+        if (!c->is3d && j == 2) continue;
+        q[nf] = tin3[d][j];
+        out[nf] = G + (long long)(4 * d + j) * np;
+        ++nf;
       }
-      run_solve(c, d, GATT_DERIV, nf2, in, out, 0.0, 0, nm, ws, ws);
-      launches = 2;
-      arrays = 2.0 * (nf + nf2);
+      run_solve_flux(c, d, nf, q, out, nm, S + 4 * np, c->slab, D, ws, c->stream);
+      launches = 1;
+      // distinct inputs (E and 8 first derivatives; 2-D: E and 5) and the outputs
+      arrays = c->is3d ? 9 + 4 : 6 + 3;
+    } else if (kernel == GATT_K_RHS_Y) {
+      AssembleArgs aa{};
+      aa.c = c->fc; aa.np = np; aa.member_stride = c->slab; aa.work_stride = ws; aa.S = c->slab_ptr(SL_S); aa.Q = S;
+      aa.Out = c->slab_ptr(SL_S); aa.Delta = c->slab_ptr(SL_DL); aa.D = D; aa.T = T; aa.G = G;
+      aa.sums = nullptr; aa.stage = 2; aa.dt = 0.0; aa.is3d = c->is3d;
+      set_pressure_gradient(c, aa);
+      RhsLineArgs r{};
+      fill_solve_args(c, 1, GATT_DERIV, r.s);
+      r.a = aa;
+      launch_rhs_line(r, nm, c->stream);
+      launches = 1;
+      // reads S(5) D(15) Gx(4) Gz(4) Q(5) Delta(5), writes Delta(5) Out(5); 2-D: 4, 8, 3, 0, 4, 4, 4, 4
+      arrays = c->is3d ? 48 : 31;
     } else if (kernel >= GATT_K_FILTER_X && kernel <= GATT_K_FILTER_Z) {
       const int d = kernel - GATT_K_FILTER_X;
       double* ptr[kMaxPtr];
@@ -1614,8 +1736,12 @@ int gatt_time_kernel(gatt_ctx* c, int32_t kernel, int32_t reps, double* avg_ms, 
       arrays = c->is3d ? 52 : 34;
     }
   };
-  if (kernel < 0 || kernel > GATT_K_ASSEMBLE || (!c->is3d && (kernel == GATT_K_DERIV_Z || kernel == GATT_K_FILTER_Z)))
+  if (kernel < 0 || kernel > GATT_K_RHS_Y ||
+      (!c->is3d && (kernel == GATT_K_DERIV_Z || kernel == GATT_K_FILTER_Z || kernel == GATT_K_FLUX_Z)))
     return fail(GATT_E_INVALID, "unknown kernel class");
+  if ((kernel == GATT_K_FLUX_X && !flux_solve_ok(c, 0)) || (kernel == GATT_K_FLUX_Z && !flux_solve_ok(c, 2)) ||
+      (kernel == GATT_K_RHS_Y && !c->fuse_rhs))
+    return fail(GATT_E_UNSUPPORTED, "this kernel class is not part of this context's step");
   (void)ndir;
   body();  // warm-up
   hipEvent_t e0, e1;

@@ -12,6 +12,7 @@
 namespace gatt {
 
 constexpr int kMaxPtr = 8;   // fields per line-solve launch
+constexpr int kFluxMaxNR = 6;   // widest derivative band (2nR-1 diagonals) of the flux-mode solves
 
 // Device copy of one pre-factored operator (see gatt_lineop).
 struct LineOpDev {
@@ -38,6 +39,42 @@ struct LineShape {
   int M = 4;
   int waves_per_block = 4;
 };
+
+// ---------------------------------------------------------------- pointwise constants
+struct FlowConst {
+  double gm1;        // gamma - 1
+  double invRe;      // 1/Re
+  double muA;        // 1 + 110/T0
+  double muB;        // 110/T0
+  double muC;        // gamma*(gamma-1)*Ma*Ma
+  double qk;         // gamma/Re/Pr
+};
+
+// Viscous stresses and heat fluxes of nseq.F90:218-239 (3-D) and 72-85 (2-D): flux q
+// (0 txx, 1 tyy, 2 tzz, 3 txy, 4 txz, 5 tyz, 6 qx, 7 qy, 8 qz) at point n from E and the
+// first derivatives D + (3f + d) np.  The expressions are those of visc_kernel, so the
+// value is bitwise the one that kernel stores.
+__device__ __forceinline__ double visc_flux(int q, const FlowConst& c, double E, const double* D, long long n,
+                                            long long np, int is3d) {
+  const double mu1 = c.muA / (c.muC * E + c.muB);
+  const double mu2 = c.muC * E;
+  const double mu = mu1 * mu2 * sqrt(mu2);
+  const double rmu = c.invRe * mu;
+  if (q >= 6) return -(c.qk * mu * D[(12 + (q - 6)) * np + n]);
+  if (q >= 3) {
+    // txy: DUy + DVx, txz: DUz + DWx, tyz: DVz + DWy
+    const int a = q == 3 ? 1 : (q == 4 ? 2 : 5), b = q == 3 ? 3 : (q == 4 ? 6 : 7);
+    return rmu * (D[a * np + n] + D[b * np + n]);
+  }
+  const double DUx = D[0 * np + n], DVy = D[4 * np + n];
+  const double own = q == 0 ? DUx : (q == 1 ? DVy : D[8 * np + n]);
+  if (is3d) {
+    const double tz = -(2.0 / 3.0 * (DUx + DVy + D[8 * np + n]));
+    return rmu * (2.0 * own + tz);
+  }
+  const double dv = 2.0 / 3.0 * (DUx + DVy);
+  return rmu * (2.0 * own - dv);
+}
 
 struct SolveArgs {
   LineOpDev op;
@@ -70,6 +107,16 @@ struct SolveArgs {
   const double* fcoef_p;      // the same lane-interleaved for M points per lane: [type][k][tt][lane] (variant 4)
   double theta;               // filter blend (mode 1)
   int mode;                   // 0 write result; 1 in-place filter with FilterCharTime blend
+  // viscous-flux input (fused wave-segment kernels, mode 0): when has_flux is set, field f's
+  // input at point n is visc_flux(flux_q[f], E[n], D, n) of member m (E + m in_stride,
+  // D + m flux_d_stride) instead of in[f][n], so no stress array is stored
+  int has_flux;
+  int flux_q[kMaxPtr];
+  const double* flux_E;
+  const double* flux_D;
+  long long flux_d_stride;
+  FlowConst fc;
+  int is3d;
 };
 
 template <int DIR>
@@ -166,15 +213,6 @@ inline int pow2_at_least(int v) {
 }
 
 // ---------------------------------------------------------------- pointwise
-struct FlowConst {
-  double gm1;        // gamma - 1
-  double invRe;      // 1/Re
-  double muA;        // 1 + 110/T0
-  double muB;        // 110/T0
-  double muC;        // gamma*(gamma-1)*Ma*Ma
-  double qk;         // gamma/Re/Pr
-};
-
 struct ViscArgs {
   FlowConst c;
   long long np, member_stride;  // member stride of E
@@ -213,6 +251,19 @@ struct AssembleArgs {
   double* sum_partial;
 };
 void launch_assemble(const AssembleArgs& a, int nmembers, hipStream_t st);
+
+// The last direction's flux derivatives fused with the assembly and the stage update
+// (rhs_line.hip): s describes the derivative operator and the lines of direction s.dir
+// (1 or 2, fused records, one line type per 32-line tile); a is what assemble_kernel takes,
+// its G holding the other directions' flux derivatives.  With a.sum_partial set, the
+// forcing sums are written per workgroup: rhs_line_blocks(s.nlines) partials per member.
+struct RhsLineArgs {
+  SolveArgs s;
+  AssembleArgs a;
+};
+bool rhs_line_fits(int N, int nR);
+int rhs_line_blocks(int nlines);
+void launch_rhs_line(const RhsLineArgs& r, int nmembers, hipStream_t st);
 
 struct SumArgs {
   long long np, member_stride;

@@ -44,7 +44,9 @@ __device__ __forceinline__ void wsp_phase() { asm volatile("" ::: "memory"); }
 // backward factor is zero at the last row, so no point needs a branch.
 // The caller is segment sg (of G) of line l (of the L lines of the workgroup);
 // the maps of segment k of line l sit at [k*L + l].  KMAX bounds G.
-template <int NR, int M, int KMAX = kMaxSegments>
+// ROLLED composes the maps of earlier segments in a loop over their count instead of a
+// fully unrolled one (the same operations in the same order; fewer registers in flight).
+template <int NR, int M, int KMAX = kMaxSegments, bool ROLLED = false>
 __device__ __forceinline__ double wsp_sweeps(double (&r)[M], const double* cf, const double* cends, double* shD,
                                              double* shP, double* shB, int cnt, int sg, int G, int l, int per,
                                              int L = 64, int GA = 0) {
@@ -64,9 +66,14 @@ __device__ __forceinline__ double wsp_sweeps(double (&r)[M], const double* cf, c
     shD[me] = part;
     __syncthreads();
     wsp_phase();
+    if constexpr (ROLLED) {
+#pragma unroll 4
+      for (int k = 0; k < G; ++k) x1 += shD[k * L + l];
+    } else {
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k)
-      if (k < G) x1 += shD[k * L + l];
+      for (int k = 0; k < KMAX; ++k)
+        if (k < G) x1 += shD[k * L + l];
+    }
     // row 2 and row N corrections (derivs.F90:792-793)
     if (sg == 0) r[1] = fma(-cends[0], x1, r[1]);
     if (sg == G - 1) {
@@ -89,9 +96,14 @@ __device__ __forceinline__ double wsp_sweeps(double (&r)[M], const double* cf, c
   __syncthreads();
   wsp_phase();
   double dprev = 0.0;
+  if constexpr (ROLLED) {
+#pragma unroll 4
+    for (int k = 0; k < sg; ++k) dprev = fma(shQ[k * L + l], dprev, shP[k * L + l]);
+  } else {
 #pragma unroll
-  for (int k = 0; k < KMAX; ++k)
-    if (k < sg) dprev = fma(shQ[k * L + l], dprev, shP[k * L + l]);
+    for (int k = 0; k < KMAX; ++k)
+      if (k < sg) dprev = fma(shQ[k * L + l], dprev, shP[k * L + l]);
+  }
 #pragma unroll
   for (int tt = 0; tt < M; ++tt) {
     const double d = fma(-cf[tt * CS + NW], dprev, r[tt]);
@@ -114,9 +126,14 @@ __device__ __forceinline__ double wsp_sweeps(double (&r)[M], const double* cf, c
   __syncthreads();
   wsp_phase();
   double xnext = 0.0;
+  if constexpr (ROLLED) {
+#pragma unroll 4
+    for (int k = G - 1; k > sg; --k) xnext = fma(shC[k * L + l], xnext, shB[k * L + l]);
+  } else {
 #pragma unroll
-  for (int k = KMAX - 1; k >= 0; --k)
-    if (k > sg && k < G) xnext = fma(shC[k * L + l], xnext, shB[k * L + l]);
+    for (int k = KMAX - 1; k >= 0; --k)
+      if (k > sg && k < G) xnext = fma(shC[k * L + l], xnext, shB[k * L + l]);
+  }
 #pragma unroll
   for (int tt = M - 1; tt >= 0; --tt) {
     const double x = fma(-cf[tt * CS + NW + 1], xnext, r[tt]);
@@ -141,6 +158,15 @@ __device__ __forceinline__ void wsp_band(double (&r)[M], double (&f0)[BLEND ? M 
   
     if (BLEND) f0[tt] = fw[tt + H];
   }
+}
+
+// Flux-mode input of member mem at point n (SolveArgs::has_flux): viscous flux q
+// computed from the stage state's E and the first derivatives.
+__device__ __forceinline__ double wsp_flux_at(const SolveArgs& a, int q, int mem, long long n) {
+  const long long np = (long long)a.nx * a.ny * a.nz;
+  const double* E = a.flux_E + (long long)mem * a.in_stride;
+  const double* D = a.flux_D + (long long)mem * a.flux_d_stride;
+  return visc_flux(q, a.fc, E[n], D, n, np, a.is3d);
 }
 
 template <int DIR>
@@ -183,12 +209,13 @@ __device__ __forceinline__ void wsp_load_x(const double* src, int N, int st, boo
 }
 
 // Two workgroups per compute unit where the registers allow it (64 lines x 256
-// points: 16 wavefronts of 16 points, or 8 of 32).
-template <int NR, int M, bool BLEND>
-constexpr int wsp_min_waves() { return M >= 32 ? 4 : ((NR <= 4 && !BLEND) ? 8 : 1); }
+// points: 16 wavefronts of 16 points, or 8 of 32).  Flux-mode loads evaluate the
+// viscous flux at every window point and get 128 registers (one workgroup).
+template <int NR, int M, bool BLEND, bool FLUX = false>
+constexpr int wsp_min_waves() { return M >= 32 ? 4 : (FLUX ? 4 : ((NR <= 4 && !BLEND) ? 8 : 1)); }
 
-template <int DIR, int NR, int M, bool BLEND>
-__global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>())) void line_solve_wsp_kernel(SolveArgs a) {
+template <int DIR, int NR, int M, bool BLEND, bool FLUX = false>
+__global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND, FLUX>())) void line_solve_wsp_kernel(SolveArgs a) {
   extern __shared__ double sh[];
   constexpr int H = NR - 1;
   constexpr int CS = wsp_coef_stride(NR);
@@ -236,7 +263,18 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>(
   }
 
   double fw[W];
-  if (XL) {
+  if (FLUX) {
+    const int q = a.flux_q[f];
+    const long long base = XL ? (long long)a.nx * line : wsp_base<DIR>(a, line);
+    const long long st = XL ? 1 : stride;
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      int e = s - H + u;
+      e = e < 0 ? e + N : (e >= N ? e - N : e);
+      e = e >= N ? N - 1 : e;
+      fw[u] = wsp_flux_at(a, q, mem, base + (long long)e * st);
+    }
+  } else if (XL) {
     wsp_load_x<W>(in + (long long)a.nx * line, N, s - H, a.xvec != 0, fw);
   } else {
     const long long base = wsp_base<DIR>(a, line);
@@ -298,7 +336,7 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND>(
 // (one per segment) reuse the tile's words between the window reads and the
 // result writes.  At N = 256 a 32-line tile takes 64 KiB (two workgroups per
 // compute unit), a 16-line tile 32 KiB (four).
-template <int NR, int M, bool BLEND, int LW>
+template <int NR, int M, bool BLEND, int LW, bool FLUX = false>
 __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
   extern __shared__ double sh[];
   constexpr int H = NR - 1;
@@ -350,7 +388,16 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
   const int total = nact * N;
   // pairs of doubles where the stretch is 16-byte aligned and a pair never straddles two lines
   const bool pairs = a.xvec && (N & 1) == 0;
-  if (pairs) {
+  if (FLUX) {
+    // the tile holds the viscous flux computed from E and the first derivatives
+    const int q = a.flux_q[f];
+    const long long p0 = (long long)a.nx * line0;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+      const int ep = idx - lp * N;
+      tile[tix(ep, lp)] = wsp_flux_at(a, q, mem, p0 + idx);
+    }
+  } else if (pairs) {
     const double2* in2 = reinterpret_cast<const double2*>(in);
     for (int i2 = threadIdx.x; i2 < total / 2; i2 += blockDim.x) {
       const double2 v = in2[i2];
@@ -431,19 +478,19 @@ inline size_t wspx_lds_bytes(int N, int M, int nR, int lw) {
   return sizeof(double) * (work > tile ? work : tile);
 }
 
-template <int DIR, int NR, int M, bool BLEND>
+template <int DIR, int NR, int M, bool BLEND, bool FLUX = false>
 static void launch_wsp_k(const SolveArgs& a, int G, hipStream_t st) {
   static_assert(M % 2 == 0, "pairs of points per lane");
-  auto k = line_solve_wsp_kernel<DIR, NR, M, BLEND>;
+  auto k = line_solve_wsp_kernel<DIR, NR, M, BLEND, FLUX>;
   const size_t lds = wsp_lds_bytes(G, M, NR);
   if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   dim3 grid((unsigned)((a.nlines + 63) / 64), (unsigned)(a.nf * a.nmem));
   hipLaunchKernelGGL(k, grid, dim3(64 * G), lds, st, a);
 }
 
-template <int NR, int M, bool BLEND, int LW>
+template <int NR, int M, bool BLEND, int LW, bool FLUX = false>
 static void launch_wspx_k(const SolveArgs& a, hipStream_t st) {
-  auto k = line_solve_wspx_kernel<NR, M, BLEND, LW>;
+  auto k = line_solve_wspx_kernel<NR, M, BLEND, LW, FLUX>;
   const size_t lds = wspx_lds_bytes(a.op.N, M, NR, LW);
   if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   constexpr int SPW = 64 / LW;
@@ -464,6 +511,21 @@ static void launch_wsp_m(const SolveArgs& a0, int nmembers, hipStream_t st) {
   a.xvec = vec ? 1 : 0;
   const int G = (a.op.N + M - 1) / M;
   const bool blend = a.mode == 1 && a.theta > 0.0;
+  if constexpr (NR <= kFluxMaxNR) if (a.has_flux) {
+    // derivatives of the viscous fluxes (mode 0; derivative bands only, the host checks nR)
+    if constexpr (DIR == 0) {
+      if (a.xtile == 16) {
+        launch_wspx_k<NR, M, false, 16, true>(a, st);
+        return;
+      }
+      if (a.xtile) {
+        launch_wspx_k<NR, M, false, 32, true>(a, st);
+        return;
+      }
+    }
+    launch_wsp_k<DIR, NR, M, false, true>(a, G, st);
+    return;
+  }
   if constexpr (DIR == 0) {
     if (a.xtile == 16) {
       if (blend) launch_wspx_k<NR, M, true, 16>(a, st);

@@ -3,6 +3,7 @@
 // blend and the per-step reductions.  One thread per grid point; every field
 // is an FP64 column-major array (x fastest), so all accesses are unit-stride.
 #include "gatt_internal.h"
+#include "rhs_point.h"
 
 namespace gatt {
 
@@ -57,137 +58,22 @@ void launch_visc(const ViscArgs& a, int nmembers, hipStream_t st) {
   hipLaunchKernelGGL(visc_kernel, dim3(grid_for(a.np, 256), nmembers), dim3(256), 0, st, a);
 }
 
-// nseq.F90:280-318 (3-D), 106-134 (2-D), the turbulenceGenerator forcing
-// (turbulenceGenerator.F90:57-61) and one stage update: stage 0 stores the right-hand
-// side, 1-4 are the RK4 stages (timeIntegration.F90:393-487), 5 the Euler update and
-// 6-8 the SSPRK3 stages.
+// nseq.F90:280-318 (3-D), 106-134 (2-D), the forcings and one stage update (rhs_point.h),
+// one thread per point; the flux derivatives come from the G arrays.
 __global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs a) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= a.np) return;
   const long long mo = (long long)blockIdx.y * a.member_stride;
   const long long np = a.np;
   const long long wo = (long long)blockIdx.y * a.work_stride;
-  const double* S = a.S + mo;
-  const double* D = a.D + wo;
   const double* G = a.G + wo;
-  const double U = S[0 * np + n], V = S[1 * np + n], R = S[3 * np + n], E = S[4 * np + n];
-  const FlowConst c = a.c;
-  const double DUx = D[0 * np + n], DUy = D[1 * np + n];
-  const double DVx = D[3 * np + n], DVy = D[4 * np + n];
-  const double DRx = D[9 * np + n], DRy = D[10 * np + n];
-  const double DEx = D[12 * np + n], DEy = D[13 * np + n];
-  const double P = c.gm1 * R * E;
-  const double DPx = c.gm1 * (R * DEx + DRx * E);
-  const double DPy = c.gm1 * (R * DEy + DRy * E);
-  // the stresses are recomputed from the derivatives and E already in registers, with the
-  // expressions of visc_kernel (so they equal the values it stored) instead of re-read
-  const double mu1 = c.muA / (c.muC * E + c.muB);
-  const double mu2 = c.muC * E;
-  const double mu = mu1 * mu2 * sqrt(mu2);
-  const double rmu = c.invRe * mu;
-  double d[5];
-  if (a.is3d) {
-    const double W = S[2 * np + n];
-    const double DUz = D[2 * np + n], DVz = D[5 * np + n];
-    const double DWx = D[6 * np + n], DWy = D[7 * np + n], DWz = D[8 * np + n];
-    const double DRz = D[11 * np + n], DEz = D[14 * np + n];
-    const double DPz = c.gm1 * (R * DEz + DRz * E);
-    const double tz = -(2.0 / 3.0 * (DUx + DVy + DWz));
-    const double txx = rmu * (2.0 * DUx + tz), tyy = rmu * (2.0 * DVy + tz), tzz = rmu * (2.0 * DWz + tz);
-    const double txy = rmu * (DUy + DVx), txz = rmu * (DUz + DWx), tyz = rmu * (DVz + DWy);
-    // second derivatives: x: txx,txy,txz,qx | y: txy,tyy,tyz,qy | z: txz,tyz,tzz,qz
-    const double dtxxdx = G[0 * np + n], dtxydx = G[1 * np + n], dtxzdx = G[2 * np + n], htx = G[3 * np + n];
-    const double dtxydy = G[4 * np + n], dtyydy = G[5 * np + n], dtyzdy = G[6 * np + n], hty = G[7 * np + n];
-    const double dtxzdz = G[8 * np + n], dtyzdz = G[9 * np + n], dtzzdz = G[10 * np + n], htz = G[11 * np + n];
-    const double vtx = dtxxdx + dtxydy + dtxzdz;
-    const double vty = dtyydy + dtxydx + dtyzdz;
-    const double vtz = dtzzdz + dtxzdx + dtyzdy;
-    const double div = DUx + DVy + DWz;
-    d[3] = -(R * div) - DRx * U - DRy * V - DRz * W;
-    d[0] = -(DUx * U + DUy * V + DUz * W) + (vtx - DPx) / R;
-    d[1] = -(DVx * U + DVy * V + DVz * W) + (vty - DPy) / R;
-    d[2] = -(DWx * U + DWy * V + DWz * W) + (vtz - DPz) / R;
-    d[4] = -(D[12 * np + n] * U + DEy * V + DEz * W) +
-           (-(P * div) + txx * DUx + tyy * DVy + tzz * DWz + txy * (DVx + DUy) + txz * (DWx + DUz) +
-            tyz * (DWy + DVz) - htx - hty - htz) / R;
-    if (a.sums) {
-      // turbulenceGenerator.F90:57-61 with the global sums of this member
-      const double* sm = a.sums + 8 * blockIdx.y;
-      const double ET = sm[0], Us = sm[1], Vs = sm[2], Ws = sm[3], Es = sm[4], Nd = sm[5];
-      const double gam = 10 * a.forcing_eps;
-      const double fk = 0.25 * a.forcing_eps * Nd / ET;
-      d[0] = d[0] + fk * U - gam * Us / Nd;
-      d[1] = d[1] + fk * V - gam * Vs / Nd;
-      d[2] = d[2] + fk * W - gam * Ws / Nd;
-      d[4] = d[4] - gam * (Es / Nd - a.forcing_e0);
-    }
-  } else {
-    const double dv = 2.0 / 3.0 * (DUx + DVy);
-    const double txx = rmu * (2.0 * DUx - dv), tyy = rmu * (2.0 * DVy - dv), txy = rmu * (DUy + DVx);
-    const double dtxxdx = G[0 * np + n], dtxydx = G[1 * np + n], htx = G[3 * np + n];
-    const double dtxydy = G[4 * np + n], dtyydy = G[5 * np + n], hty = G[7 * np + n];
-    const double vtx = dtxxdx + dtxydy;
-    const double vty = dtyydy + dtxydx;
-    const double div = DUx + DVy;
-    d[3] = -(R * div) - DRx * U - DRy * V;
-    d[0] = -(DUx * U + DUy * V) + (vtx - DPx) / R;
-    d[1] = -(DVx * U + DVy * V) + (vty - DPy) / R;
-    d[2] = 0.0;
-    d[4] = -(DEx * U + DEy * V) + (-(P * div) + txx * DUx + tyy * DVy + txy * (DVx + DUy) - htx - hty) / R;
-  }
-  if (a.pg) {
-    // pressureGradient.F90:13 (du = du + dpdx/r) on its box, after the other forcings
-    const int i = (int)(n % a.nx);
-    const int j = (int)((n / a.nx) % a.ny);
-    const int k = (int)(n / ((long long)a.nx * a.ny));
-    if (i >= a.pg_box[0] && i <= a.pg_box[1] && j >= a.pg_box[2] && j <= a.pg_box[3] && k >= a.pg_box[4] &&
-        k <= a.pg_box[5]) {
-      const double r = S[(long long)a.pg_r * np + n];
+  double g[12];
 #pragma unroll
-      for (int f = 0; f < 5; ++f)
-        if (f == a.pg_u) d[f] = d[f] + a.pg_dpdx / r;
-    }
-  }
-  const double* Q = a.Q + mo;
-  double* O = a.Out + mo;
-  double* Dl = a.Delta + mo;
-  const double dt = a.dt;
-  const int nf = 5;
-  // the first stage updates the state it differentiates (Q == S): its values are in registers
-  const bool q_is_s = a.Q == a.S;
-  const double sv[5] = {U, V, a.is3d ? S[2 * np + n] : 0.0, R, E};
-  double ov[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int f = 0; f < nf; ++f) {
-    if (!a.is3d && f == 2) continue;
-    const long long o = f * np + n;
-    double v;
-    if (q_is_s && a.stage == 1) {
-      Dl[o] = dt / 6.0 * d[f];
-      v = sv[f] + dt / 2.0 * d[f];
-    } else {
-      switch (a.stage) {
-        case 0: v = d[f]; break;
-        case 1: Dl[o] = dt / 6.0 * d[f]; v = Q[o] + dt / 2.0 * d[f]; break;
-        case 2: Dl[o] = Dl[o] + dt / 3.0 * d[f]; v = Q[o] + dt / 2.0 * d[f]; break;
-        case 3: Dl[o] = Dl[o] + dt / 3.0 * d[f]; v = Q[o] + dt * d[f]; break;
-        case 4: v = Q[o] + Dl[o] + dt / 6.0 * d[f]; break;
-        // EULER (timeIntegration.F90:553-557)
-        case 5: v = Q[o] + dt * d[f]; break;
-        // SSPRK3 (timeIntegration.F90:623-627, 647-651, 672-676): Delta keeps k1, then k1 + k2
-        case 6: Dl[o] = d[f]; v = Q[o] + dt * d[f]; break;
-        case 7: {
-          const double k = Dl[o] + d[f];
-          Dl[o] = k;
-          v = Q[o] + dt * k / 4.0;
-          break;
-        }
-        default: v = Q[o] + dt * (Dl[o] + 4.0 * d[f]) / 6.0; break;
-      }
-    }
-    O[o] = v;
-    ov[f] = v;
-  }
+  for (int k = 0; k < 12; ++k) g[k] = (a.is3d || k == 0 || k == 1 || k == 3 || k == 4 || k == 5 || k == 7)
+                                          ? G[k * np + n] : 0.0;
+  double d[5], sv[5], ov[5];
+  rhs_point(a, a.S + mo, a.D + wo, n, g, (int)blockIdx.y, d, sv);
+  stage_point(a, mo, n, d, sv, ov);
   if (a.sum_partial) {
     // turbulenceGenerator.F90:39-44 on the stage state just written: wavefront trees, then
     // the four wavefronts of the block in order (no thread of the block returned early)

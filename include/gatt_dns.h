@@ -271,20 +271,24 @@ int gatt_set_line_shape(gatt_ctx* ctx, int32_t dir, int32_t kind, int32_t varian
 int gatt_get_line_shape(gatt_ctx* ctx, int32_t dir, int32_t kind, int32_t* variant, int32_t* G, int32_t* M,
                         int32_t* waves_per_block);
 
-/* Kernel classes of one time step, for timing. */
+/* Kernel classes of one time step, for timing.  DERIV: first derivatives of the
+ * state fields along one direction; FILTER: the filter in place; VISC and
+ * ASSEMBLE: the stresses and the assembly of the unfused stage; FLUX_X / FLUX_Z:
+ * derivatives of the viscous fluxes along x / z with the fluxes computed as they
+ * are loaded; RHS_Y: the y flux derivatives fused with the assembly and the
+ * stage update (the last two kinds only where the context's step uses them). */
 enum {
   GATT_K_DERIV_X = 0, GATT_K_DERIV_Y = 1, GATT_K_DERIV_Z = 2,
   GATT_K_FILTER_X = 3, GATT_K_FILTER_Y = 4, GATT_K_FILTER_Z = 5,
-  GATT_K_VISC = 6, GATT_K_ASSEMBLE = 7
+  GATT_K_VISC = 6, GATT_K_ASSEMBLE = 7,
+  GATT_K_FLUX_X = 8, GATT_K_FLUX_Z = 9, GATT_K_RHS_Y = 10
 };
 
-/* Time one kernel class on the context's stream with HIP events: the launches
- * that class makes in one RK stage (derivatives: the state fields, then the
- * stress/heat-flux fields; filters: the state fields in place) are replayed
- * `reps` times.  Returns the mean duration of one launch in ms and the
- * algorithmic bytes of that launch (8 B per point for every array read or
- * written once).  Filter timing overwrites the state.  No reference counterpart
- * (measurement only). */
+/* Time one kernel class on the context's stream with HIP events: one launch of
+ * that class is replayed `reps` times.  Returns the mean duration of one launch
+ * in ms and the algorithmic bytes of that launch (8 B per point for every
+ * distinct array read or written once).  Filter timing overwrites the state.
+ * No reference counterpart (measurement only). */
 int gatt_time_kernel(gatt_ctx* ctx, int32_t kernel, int32_t reps, double* avg_ms, double* bytes);
 
 /* ------------------------------------------------------------------------
