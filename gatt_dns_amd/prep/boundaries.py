@@ -6,6 +6,9 @@ types the time-advance configurations use:
   * ``source/findWallsForBoundaries.m`` -> :func:`find_walls_for_boundaries`
   * ``source/boundaries/lidDrivenFlow.m`` -> :func:`lid_driven_flow`
   * ``source/boundaries/periodicBox.m`` -> :func:`periodic_box`
+  * ``source/boundaries/boundaryLayerIsothermal.m`` (and its Adiabatic, SymmZ and
+    PressureInlet variants) -> :func:`boundary_layer_isothermal`
+  * ``source/boundaries/boundaryLayerFreeSlip.m`` -> :func:`boundary_layer_free_slip`
   * ``source/getBoundaryConditions.m`` -> :func:`get_boundary_conditions`
   * ``source/initBoundaries.m`` (one rank) -> :func:`init_boundaries`
   * ``source/calcSFDregion.m`` -> :func:`calc_sfd_region`
@@ -331,6 +334,62 @@ def boundary_layer_isothermal(X, Y, Z, E0, P0, z_periodic=True, t_wall_relative=
     return fr, walls, corners, bc, bp, hold
 
 
+def boundary_layer_free_slip(X, Y, Z, E0, P0, x_periodic=False, y_periodic=False, z_periodic=True,
+                             t_wall_relative=None):
+    """BC list of ``source/boundaries/boundaryLayerFreeSlip.m`` (no cavities or roughness).
+
+    The in- and outflow of the isothermal boundary layer when x is not periodic (``:4-144``,
+    held inflow when the domain starts past x = 0), the zero-gradient top when y is not
+    periodic (``:146-209``), zero-gradient z ends for a non-periodic 3-D z (``:211-334``) and
+    the plate below y = 0 when y is not periodic (``:336-344``).  Its wall faces carry
+    zero-gradient u, w and p (``:371-433``: all u boxes, then w, then p, for each face
+    direction) and the points inside the walls u = v = w = 0, p = P0, e = eWall
+    (``:435-498``); with v = 0 from the inside boxes this is a slip wall.  There is no
+    x break point and no separate slip region ahead of the plate.
+
+    Returns ``(flow_region, walls, corners, bc, None, hold_inlet)`` like
+    :func:`boundary_layer_isothermal`.
+    """
+    nx, ny, nz = len(X), len(Y), len(Z)
+    bc = BCList()
+    full = lambda i1, i2, j1, j2, k1, k2: [i1, i2, j1, j2, k1, k2]
+    hold = False
+    if not x_periodic:
+        if X[0] <= 0:
+            for var, val in (("u", 1.0), ("v", 0.0), ("w", 0.0), ("e", E0)):
+                bc.add(var, "dir", "xi", val, full(1, 1, 1, ny, 1, nz))
+        else:
+            hold = True
+        bc.add("p", "neu", "xi", 0.0, full(1, 1, 1, ny, 1, nz))
+        for var in "uvwe":
+            bc.add(var, "sec", "xf", 0.0, full(nx, nx, 1, ny, 1, nz))
+        bc.add("p", "dir", "xf", P0, full(nx, nx, 1, ny, 1, nz))
+    if not y_periodic:
+        for var in "uvwep":
+            bc.add(var, "sec", "yf", 0.0, full(1, nx, ny, ny, 1, nz))
+    if nz > 1 and not z_periodic:
+        for d, k in (("zi", 1), ("zf", nz)):
+            for var in "uvwep":
+                bc.add(var, "neu", d, 0.0, full(1, nx, 1, ny, k, k))
+    fr = np.ones((nx, ny, nz), dtype=bool)
+    if not y_periodic:
+        wall_j = int(np.argmin(np.abs(np.asarray(Y)))) + 1
+        fr[:, :wall_j, :] = False
+    fr, walls, corners, inside = find_walls_for_boundaries(fr)
+    for key, d in (("front", "xi"), ("back", "xf"), ("up", "yi"), ("down", "yf"), ("right", "zi"), ("left", "zf")):
+        for var in "uwp":
+            for box in walls[key]:
+                bc.add(var, "neu", d, 0.0, box)
+    e_wall = E0 * t_wall_relative if t_wall_relative is not None else E0
+    for box in inside:
+        bc.add("p", "dir", "yi", P0, box)
+        bc.add("u", "dir", "yi", 0.0, box)
+        bc.add("v", "dir", "yi", 0.0, box)
+        bc.add("w", "dir", "yi", 0.0, box)
+        bc.add("e", "dir", "yi", e_wall, box)
+    return fr, walls, corners, bc, None, hold
+
+
 def periodic_box(nx, ny, nz):
     """``source/boundaries/periodicBox.m``: all flow, no walls."""
     fr, walls, corners, inside = find_walls_for_boundaries(np.ones((nx, ny, nz), dtype=bool))
@@ -397,12 +456,33 @@ def disturbance_index(X, Y, Z, x, y, z, extra=(0, 0, 0, 0, 0, 0)):
     return [out[0] - ex[0], out[1] + ex[1], out[2] - ex[2], out[3] + ex[3], out[4] - ex[4], out[5] + ex[5]]
 
 
-def calc_sfd_region(nx, ny, nz, buffers, chi, apply=(2, 2, 2)):
-    """Buffer-zone damping field ``chi(x,y,z)`` (``calcSFDregion.m:4-50``).
+def calc_sfd_region(nx, ny, nz, buffers, chi, apply=(2, 2, 2), X=None, Y=None, Z=None, extra_regions=()):
+    """Buffer-zone damping field ``chi(x,y,z)`` (``calcSFDregion.m:4-62``).
 
     ``buffers`` = ((nxi, nxf), (nyi, nyf), (nzi, nzf)); ``apply`` = applyX/Y/Z with
-    2 = both ends, -1 = start only, 1 = end only, 0 = none.
+    2 = both ends, -1 = start only, 1 = end only, 0 = none.  ``extra_regions``
+    (``numMethods.SFD.extraRegion``, ``:52-62``): dicts with ``location`` (x, y, z),
+    ``size`` (sx, sy, sz) and ``X``; each adds ``X (0.5 + 0.5 cos(pi R))`` with R the
+    ellipsoidal distance from the location, capped at 1, on the mesh X, Y, Z.
     """
+    base = _sfd_buffers(nx, ny, nz, buffers, chi, apply)
+    if not extra_regions:
+        return base
+    Xc = np.asarray(X, float)[:, None, None]
+    Yc = np.asarray(Y, float)[None, :, None]
+    Zc = np.asarray(Z, float)[None, None, :]
+    for er in extra_regions:
+        loc, size = er["location"], er["size"]
+        R = np.sqrt((Xc - loc[0]) ** 2 / size[0] ** 2 + (Yc - loc[1]) ** 2 / size[1] ** 2 +
+                    (Zc - loc[2]) ** 2 / size[2] ** 2)
+        R = np.minimum(R, 1.0)
+        base = base + er["X"] * (0.5 + 0.5 * np.cos(np.pi * R))
+    return base
+
+
+def _sfd_buffers(nx, ny, nz, buffers, chi, apply):
+    """The cosine ramps of ``calcSFDregion.m:4-50``: 0 in the physical region, chi at the
+    outer edge of each buffer, the factors of overlapping ramps multiplied."""
     f = np.ones((nx, ny, nz))
     (xi, xf), (yi, yf), (zi, zf) = buffers
     ax, ay, az = apply

@@ -15,9 +15,10 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .boundaries import (BoundaryInfo, boundary_layer_isothermal, calc_sfd_region, disturbance_index, init_boundaries,
+from .boundaries import (BoundaryInfo, boundary_layer_free_slip, boundary_layer_isothermal, calc_sfd_region, disturbance_index, init_boundaries,
                          lid_driven_flow, periodic_box, poiseuille_flow)
 from .matrices import BufferSide, DirMesh, LineCoeffs, NumMethods, fortran_round, make_matrices, prepare_thomas
+from .compressible_bl import compressible_bl_flow
 from .mesh import MeshSpec, generate_mesh
 
 FIELDS = ("U", "V", "W", "R", "E")
@@ -215,16 +216,22 @@ def turbulence_box(n=256, Re=1000.0, Ma=0.3, Pr=0.71, gamma=1.4, T0=300.0, alpha
 
 
 def periodic_rest(nx, ny, nz=1, Re=1000.0, Ma=0.3, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0.49,
-                  spatial="SL4", cfl=0.5):
+                  spatial="SL4", cfl=0.5, stretch=0.0):
     """Fluid at rest in a periodic box (periodicBox.m) on uniform meshes, no forcing.
 
     The linearised step about this state is diagonal in Fourier modes, so the
     spectrum of the time-T propagator is known in closed form; used to check
-    the stability analysis.
+    the stability analysis.  ``stretch`` = a in [0, 1) makes the meshes
+    non-uniform, x_i = (i + a n / (2 pi) sin(2 pi (i + 1/2) / n)) / n shifted to x_0 = 0,
+    still periodic on [0, 1) (dX/deta then differs between the first and the last node, where the
+    periodic solve's index quirks of derivs.F90:786-802 show).
     """
     meshes = []
     for n in (nx, ny, nz):
-        X = np.arange(n) / n if n > 1 else np.array([0.0])
+        i = np.arange(n)
+        X = (i + stretch * n / (2 * np.pi) * np.sin(2 * np.pi * (i + 0.5) / n)) / n if n > 1 else np.array([0.0])
+        if n > 1 and stretch:
+            X = X - X[0]
         meshes.append(DirMesh(X, periodic=True, fix_periodic_domain_size=True, xf=1.0))
     X, Y, Z = (m.X for m in meshes)
     E0 = 1.0 / ((gamma ** 2 - gamma) * Ma ** 2)
@@ -349,9 +356,11 @@ def blasius(eta_max=20.0, n=4001):
 def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wall_ratio=1.02, Re=1000.0,
                    Ma=0.5, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0.49, spatial="SL4", cfl=0.5, sfd_chi=0.02,
                    sfd_delta=10.0, noise=1e-5, seed=20261, buffer_growth=8.0, nz=1, z_length=4.0,
-                   adiabatic=False, z_symmetry=False, pressure_inlet=False):
+                   adiabatic=False, z_symmetry=False, pressure_inlet=False, free_slip=False,
+                   initial="compressible"):
     """Isothermal-wall boundary layer (boundaryLayerIsothermal.m; ``adiabatic=True``: the walls of
-    boundaryLayerAdiabatic.m) on a stretched mesh with buffers.
+    boundaryLayerAdiabatic.m; ``free_slip=True``: the slip plate of boundaryLayerFreeSlip.m, with
+    zero-gradient u and w on the wall and no x break point) on a stretched mesh with buffers.
 
     With nz > 1 the 2-D configuration is extruded (``z_symmetry``: between the symmetry planes of
     boundaryLayerIsothermalSymmZ.m on [0, z_length]; otherwise) over a spanwise-periodic uniform z of length
@@ -362,8 +371,11 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
     nbuf-point buffer above; buffer spacings grow along a sigmoid to ``buffer_growth`` times the edge
     spacing.  Numerics as parameters.m: SL4 with EX4 in the buffers, 'reducedOrder' filter borders with
     the x-start, x-end and y-end closures off, SFD type 2 (chi in the x buffers only, applyY = false).
-    Initial flow: Blasius velocity scaled so that the displacement thickness is 1 at x = 0 (uniform flow
-    upstream of the plate), uniform R and E, plus seeded N(0, noise^2) on u and v in the flow region.
+    Initial flow (``initial``): 'compressible' (the default) is calcCompressibleBL.m's similarity
+    solution for the isothermal (or, with ``adiabatic``, the adiabatic) wall, uniform upstream of
+    the plate; 'blasius' is the incompressible Blasius velocity scaled so that the displacement
+    thickness is 1 at x = 0 with uniform R and E.  Either gets seeded N(0, noise^2) on u and v in
+    the flow region.
     """
     npx, npy = nx - 2 * nbuf, ny - nbuf
     Xp = np.linspace(x_range[0], x_range[1], npx)
@@ -381,9 +393,14 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
         Z = np.arange(nz) * (z_length / nz) if nz > 1 else np.array([0.0])
     E0 = 1.0 / ((gamma ** 2 - gamma) * Ma ** 2)
     P0 = (gamma - 1) * E0
-    fr, walls, corners, bc, bp, hold = boundary_layer_isothermal(X, Y, Z, E0, P0, adiabatic=adiabatic,
-                                                                 z_periodic=not z_sym, z_symmetry=z_sym,
-                                                                 pressure_inlet=pressure_inlet)
+    if free_slip:
+        if adiabatic or z_sym or pressure_inlet:
+            raise ValueError("free_slip takes none of adiabatic, z_symmetry, pressure_inlet")
+        fr, walls, corners, bc, bp, hold = boundary_layer_free_slip(X, Y, Z, E0, P0)
+    else:
+        fr, walls, corners, bc, bp, hold = boundary_layer_isothermal(X, Y, Z, E0, P0, adiabatic=adiabatic,
+                                                                     z_periodic=not z_sym, z_symmetry=z_sym,
+                                                                     pressure_inlet=pressure_inlet)
     buf = BufferSide(n=nbuf, upwind=False, transition=0.2)
     meshes = [DirMesh(X, buffer_i=buf, buffer_f=buf, break_points=[bp] if bp else []),
               DirMesh(Y, buffer_f=buf),
@@ -396,7 +413,8 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
     bi = init_boundaries(bc, corners, nm.neumann_order, nm.neumann2_order)
     rnd = lambda v: float(fortran_round(v))
     chi = calc_sfd_region(nx, ny, nz, ((nbuf, nbuf), (0, nbuf), (0, 0)), sfd_chi, apply=(2, 0, 2))
-    name = "boundary_layer_%dx%d" % (nx, ny) + ("x%d" % nz if nz > 1 else "")
+    name = ("boundary_layer_free_slip_%dx%d" if free_slip else "boundary_layer_%dx%d") % (nx, ny) + \
+        ("x%d" % nz if nz > 1 else "")
     dist = []
     if hold:
         # a domain that starts past the leading edge holds its inflow plane at the initial
@@ -409,24 +427,32 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
                 sfd_x=np.asfortranarray(fortran_round(chi)), dxmin=_dxmin(X, Y, Z, nz))
     case.flow_region = fr
     case.buffers = ((nbuf, nbuf), (0, nbuf), (0, 0))
-    eta, sol = blasius()
-    x0 = Re / 1.7207876573 ** 2                 # displacement thickness 1 at x = 0
-    xe = np.maximum(X, 0.0)[:, None] + x0
-    et = Y[None, :] * np.sqrt(Re / xe)
-    fp = np.interp(et, eta, sol[:, 1], right=1.0)
-    f = np.where(et <= eta[-1], np.interp(et, eta, sol[:, 0]), sol[-1, 0] + (et - eta[-1]))
-    U = fp.copy()
-    V = 0.5 * np.sqrt(1.0 / (Re * xe)) * (et * fp - f)
-    upstream = X < 0
-    U[upstream, :] = 1.0
-    V[upstream, :] = 0.0
-    rng = np.random.Generator(np.random.PCG64(seed))
     shape = (nx, ny, nz)
+    if initial == "compressible":
+        # calcCompressibleBL.m ('compressibleBL_isothermal' / '_adiabatic' of generateInitialFlow.m:110-126)
+        fl = compressible_bl_flow(case.Re, case.Ma, case.Pr, case.T0, case.gamma, X, Y, Z, adiabatic=adiabatic)
+        U, V = fl["U"][:, :, 0], fl["V"][:, :, 0]
+        R0, Ei = fl["R"], fl["E"]
+    else:
+        # incompressible Blasius scaled so that the displacement thickness is 1 at x = 0
+        eta, sol = blasius()
+        x0 = Re / 1.7207876573 ** 2
+        xe = np.maximum(X, 0.0)[:, None] + x0
+        et = Y[None, :] * np.sqrt(Re / xe)
+        fp = np.interp(et, eta, sol[:, 1], right=1.0)
+        f = np.where(et <= eta[-1], np.interp(et, eta, sol[:, 0]), sol[-1, 0] + (et - eta[-1]))
+        U = fp.copy()
+        V = 0.5 * np.sqrt(1.0 / (Re * xe)) * (et * fp - f)
+        upstream = X < 0
+        U[upstream, :] = 1.0
+        V[upstream, :] = 0.0
+        R0, Ei = np.ones(shape), np.full(shape, E0)
+    rng = np.random.Generator(np.random.PCG64(seed))
     U = U[:, :, None] + noise * rng.standard_normal(shape)
     V = V[:, :, None] + noise * rng.standard_normal(shape)
     st = {"U": np.asfortranarray(U), "V": np.asfortranarray(V),
-          "W": np.zeros(shape, order="F"), "R": np.ones(shape, order="F"),
-          "E": np.full(shape, E0, order="F")}
+          "W": np.zeros(shape, order="F"), "R": np.asfortranarray(R0),
+          "E": np.asfortranarray(Ei)}
     wall = ~fr
     for k in ("U", "V"):
         st[k][wall] = 0.0

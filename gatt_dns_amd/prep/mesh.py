@@ -4,7 +4,9 @@ One direction at a time: the physical nodes (``uniform``, ``power``, ``tanh`` re
 the start ``'i'``, the end ``'f'`` or both ``'b'``, ``attractors``), the matching of
 fixed points (``matchFixed`` 1 = spline, 2 = pchip correction), the buffer zones
 (``exponential`` with or without a ramp, ``sigmoid``), the extra refinement nodes and
-the temporary end node of periodic directions with a fixed domain size.  MATLAB's
+the temporary end node of periodic directions with a fixed domain size.  The ``file``
+type reads the nodes from a MAT file (the variable named after the direction) or a text
+file of numbers.  MATLAB's
 ``interp1(..., 'spline')`` is the not-a-knot cubic spline and ``'pchip'`` the
 shape-preserving piecewise cubic; scipy's ``CubicSpline`` and ``PchipInterpolator``
 implement the same interpolants.
@@ -45,6 +47,25 @@ class MeshSpec:
     extra_refinement: int = 0
     buffer_i: BufferSpec = field(default_factory=BufferSpec)
     buffer_f: BufferSpec = field(default_factory=BufferSpec)
+    file: str = None                 # mesh.file of the 'file' type
+    file_calc_buffer: bool = False   # mesh.fileCalcBuffer: the file holds the physical nodes only
+
+
+def load_mesh_file(path, direction):
+    """The node coordinates of a 'file' mesh (generateMesh.m:162-169): a ``.mat`` file's
+    variable named ``direction`` ('x', 'y' or 'z'), otherwise a text file of numbers as
+    MATLAB's ``load`` reads it; a column becomes a row."""
+    if str(path).endswith(".mat"):
+        from scipy.io import loadmat
+        d = loadmat(path)
+        if direction not in d:
+            raise ValueError("%s has no variable %r" % (path, direction))
+        X = np.asarray(d[direction], dtype=float)
+    else:
+        X = np.atleast_2d(np.loadtxt(path, dtype=float))
+    if X.ndim == 2 and X.shape[1] == 1:
+        X = X.T
+    return np.asarray(X, dtype=float).ravel()
 
 
 def calc_buffer_zone(b):
@@ -112,9 +133,14 @@ def _physical(xi, xf, m):
     raise ValueError("mesh type %r is not restated" % m.type)
 
 
-def generate_mesh(xi, xf, m):
+def generate_mesh(xi, xf, m, direction="x"):
     """``generateMesh(xi, xf, mesh, direction)`` (generateMesh.m:1-249) for one direction;
-    returns the node coordinates X (buffers included)."""
+    returns the node coordinates X (buffers included).
+
+    A 'file' mesh (``:162-180``) without ``file_calc_buffer`` is returned as read, after
+    checking that it holds n plus the buffer nodes; with it, the file holds the n physical
+    nodes and the fixed-point matching, buffers and extra refinement follow as for the other
+    types.  A 'file' mesh takes no temporary periodic end node (``:84``)."""
     import copy
     m = copy.deepcopy(m)
     if m.n is None:                                               # :6-19
@@ -124,7 +150,7 @@ def generate_mesh(xi, xf, m):
         t.extra_refinement = 0
         t.fix_periodic_domain_size = False
         t.buffer_i, t.buffer_f = BufferSpec(), BufferSpec()
-        d_base = np.max(np.diff(generate_mesh(xi, xf, t)))
+        d_base = np.max(np.diff(generate_mesh(xi, xf, t, direction)))
         m.n = int(np.ceil(t.n * d_base / m.d0))
     for side in ("buffer_i", "buffer_f"):                          # :22-69
         b = getattr(m, side)
@@ -134,7 +160,7 @@ def generate_mesh(xi, xf, m):
             t.extra_refinement = 0
             t.fix_periodic_domain_size = False
             t.buffer_i, t.buffer_f = BufferSpec(), BufferSpec()
-            Xt = generate_mesh(xi, xf, t)
+            Xt = generate_mesh(xi, xf, t, direction)
             d_base = Xt[1] - Xt[0] if side == "buffer_i" else Xt[-1] - Xt[-2]
             b.n = int(np.floor(b.l / d_base))
             value = calc_buffer_zone(b)[-1] * d_base
@@ -148,12 +174,24 @@ def generate_mesh(xi, xf, m):
         m.type = "uniform"
     if m.periodic and (m.buffer_i.n > 0 or m.buffer_f.n > 0):    # :77-82
         m.buffer_i.n = m.buffer_f.n = 0
-    added = m.periodic and m.fix_periodic_domain_size and m.n > 1
+    added = m.periodic and m.fix_periodic_domain_size and m.n > 1 and m.type != "file"
     if added:
         m.n += 1
     ni, nf = m.buffer_i.n, m.buffer_f.n
     nx = m.n + ni + nf
-    Xp = _physical(xi, xf, m)
+    if m.type == "file":
+        Xf = load_mesh_file(m.file, direction)
+        if not m.file_calc_buffer:
+            if len(Xf) != nx:
+                raise ValueError("%s mesh from file %s contains %d nodes instead of %d as specified in parameters"
+                                 % (direction, m.file, len(Xf), nx))
+            return Xf
+        if len(Xf) != m.n:
+            raise ValueError("%s mesh from file %s contains %d nodes instead of %d as specified in parameters"
+                             % (direction, m.file, len(Xf), m.n))
+        Xp = Xf
+    else:
+        Xp = _physical(xi, xf, m)
     if m.match_fixed and m.n > 1:                                  # :182-204
         fix = np.unique(np.concatenate([Xp[[0, 1]], np.asarray(m.fix_points, float), Xp[[-2, -1]]]))
         closest = np.argmin(np.abs(Xp[:, None] - fix[None, :]), axis=0) + 1     # 1-based

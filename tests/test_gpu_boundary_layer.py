@@ -88,10 +88,11 @@ def test_steps_small(rt, oracle_mod, small, nsteps, tol):
     assert dv == dv_w
 
 
-@pytest.mark.parametrize("variant", ["adiabatic", "pressure_inlet"])
+@pytest.mark.parametrize("variant", ["adiabatic", "pressure_inlet", "free_slip"])
 def test_wall_variant_steps(rt, oracle_mod, variant):
-    """boundaryLayerAdiabatic.m walls (zero-gradient E, adiabatic corners) and the pressure
-    conditions of boundaryLayerIsothermalPressureInlet.m: 10 steps."""
+    """boundaryLayerAdiabatic.m walls (zero-gradient E, adiabatic corners), the pressure
+    conditions of boundaryLayerIsothermalPressureInlet.m and the slip plate of
+    boundaryLayerFreeSlip.m: 10 steps."""
     case = cases.boundary_layer(nx=160, ny=64, nbuf=12, **{variant: True})
     print("boundary layer 160x64", variant, flush=True)
     o = oracle_mod.Oracle(case)
@@ -122,10 +123,13 @@ def test_z_symmetry_steps(rt, oracle_mod):
     mc, dv = s.advance(4, 0.0, case.dt)
     got = s.get_state()
     assert dv == dv_w
-    errs = {k: rel_err(got[k], want[k]) for k in "UVRE"}
-    errs["W"] = float(np.max(np.abs(got["W"] - want["W"])) / np.max(np.abs(want["U"])))
+    # every field against its own scale; W grows from the seeded 1e-5 noise, so its bar allows
+    # for the round-off of the O(1) terms it is computed from
+    errs = {k: rel_err(got[k], want[k]) for k in "UVWRE"}
+    print("symmetric-z boundary layer", errs, flush=True)
+    tol = {"U": 1e-10, "V": 1e-10, "W": 1e-8, "R": 1e-10, "E": 1e-10}
     for k, e in errs.items():
-        assert e <= 1e-10, (k, e)
+        assert e <= tol[k], (k, e)
 
 
 # V is O(1e-3) here: after 3 steps its largest difference (2e-14 absolute, in the first row above
@@ -162,9 +166,9 @@ def test_spanwise_periodic_step(rt, oracle_mod):
         want, _, _ = o.run(q0 if nsteps == 1 else want, 0.0 if nsteps == 1 else case.dt, case.dt, nsteps)
         s.advance(nsteps, 0.0 if nsteps == 1 else case.dt, case.dt)
         got = s.get_state()
-        errs = {k: rel_err(got[k], want[k]) for k in "UVRE"}
-        # W grows from the seeded 1e-5 noise: its error is measured against the velocity scale
-        errs["W"] = float(np.max(np.abs(got["W"] - want["W"])) / np.max(np.abs(want["U"])))
+        # every field against its own scale; W grows from the seeded 1e-5 noise, so its bar
+        # allows for the round-off of the O(1) terms it is computed from
+        errs = {k: rel_err(got[k], want[k]) for k in "UVWRE"}
         print("3-D boundary layer", nsteps, errs, flush=True)
         for k, e in errs.items():
-            assert e <= tol, (nsteps, k, e)
+            assert e <= (tol * 100 if k == "W" else tol), (nsteps, k, e)

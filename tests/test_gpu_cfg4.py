@@ -1,0 +1,117 @@
+"""The batched stability analysis and the ragged sweep on their own workloads
+(calcInstability.m:347-400 with a block of 32 Krylov vectors; runBatch.m's parameter sweep).
+
+* The Jacobian-free propagator of 32 members advanced together on the forced turbulence
+  base (turbulenceGenerator on, second-order differences) equals the oracle's on the same
+  vectors within 1e-7 (the finite difference divides the round-off of the states by eps).
+* At 256^3, 32 members perturbed about the turbulence box and advanced two steps together
+  equal the same perturbations advanced alone, bit for bit.
+* The 16 lid-driven cavities of the sweep (256 x 128 ... 768 x 256, Re 500 ... 2000) in one
+  group: three of them against the oracle after four steps, each field on its own scale.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from gatt_dns_amd.prep import cases
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def rel_err(a, b):
+    scale = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / scale) if scale else float(np.max(np.abs(a)))
+
+
+def test_box_propagator_32_members_vs_oracle(torch_cuda):
+    from gatt_dns_amd import lst
+    from oracle import lst as OL
+    case = cases.turbulence_box(32, seed=31)
+    base = case.state0
+    T = 3 * case.dt
+    st = lst.Stability(case, base, T, block=32, epsilon=1e-5, first_order=False)
+    P = OL.OraclePropagator(case, base, T, epsilon=1e-5, first_order=False)
+    assert st.n == P.N and st.nsteps == P.nsteps == 3
+    rng = np.random.Generator(np.random.PCG64(32))
+    V = rng.standard_normal((32, st.n))
+    V /= np.linalg.norm(V, axis=1)[:, None]
+    W = st.be.empty(32, st.n)
+    st.apply(torch_cuda.from_numpy(V).cuda(), W)
+    Wg = st.be.download(W)
+    for b in (0, 5, 11, 18, 24, 31):
+        print("box 32^3 propagator, member", b, flush=True)
+        Wo = P.apply(V[b])
+        rel = np.linalg.norm(Wg[b] - Wo) / np.linalg.norm(Wo)
+        assert rel < 1e-7, (b, rel)
+    st.close()
+
+
+def test_32_members_256_equal_alone(torch_cuda):
+    from gatt_dns_amd.runtime import Solver
+    torch = torch_cuda
+    case = cases.turbulence_box(256)
+    sb = Solver(case, batch=32)
+    try:
+        sb.set_base(case.state0)
+        n = sb.vector_length()
+        g = torch.Generator(device="cuda").manual_seed(256)
+        V = torch.randn(32, n, generator=g, dtype=torch.float64, device="cuda")
+        V /= torch.linalg.norm(V, dim=1, keepdim=True)
+        scale = 1e-7 * math.sqrt(n)
+        torch.cuda.synchronize()
+        sb.perturb(0, 32, V, scale)
+        print("32 members of 256^3, two steps together", flush=True)
+        sb.advance(2, 0.0, case.dt)
+        for b in (0, 13, 31):
+            s1 = Solver(case)
+            try:
+                s1.set_base(case.state0)
+                s1.perturb(0, 1, V[b:b + 1], scale)
+                s1.advance(2, 0.0, case.dt)
+                q1, qb = s1.get_state(), sb.get_state(b)
+                for k in "UVWRE":
+                    np.testing.assert_array_equal(qb[k], q1[k])
+            finally:
+                s1.close()
+    finally:
+        sb.close()
+        del V
+        torch.cuda.empty_cache()
+
+
+def test_cavity_sweep_group_vs_oracle(torch_cuda, oracle_mod):
+    from gatt_dns_amd.runtime import Group, Solver
+    sweep = cases.cavity_sweep()
+    assert len(sweep) == 16 and (sweep[0].nx, sweep[0].ny) == (256, 128) and (sweep[-1].nx, sweep[-1].ny) == (768, 256)
+    solvers = []
+    for c in sweep:
+        s = Solver(c)
+        s.set_state(c.state0)
+        s.apply_bound_cond(0.0)
+        solvers.append(s)
+    g = Group(solvers, mode=0)
+    nsteps = 4
+    mc, dv = g.advance(nsteps, 0.0, [c.dt for c in sweep])
+    assert not any(dv)
+    for m in (0, 7, 15):
+        c = sweep[m]
+        print("cavity sweep member", m, c.nx, c.ny, c.Re, flush=True)
+        o = oracle_mod.Oracle(c)
+        q0 = o.apply_bound_cond(c.state0, 0.0)
+        want, mc_w, _ = o.run(q0, 0.0, c.dt, nsteps)
+        got = solvers[m].get_state()
+        for k in "UVRE":
+            assert rel_err(got[k], want[k]) <= 1e-11, (m, k, rel_err(got[k], want[k]))
+        np.testing.assert_allclose(mc[m], mc_w, rtol=1e-8)
+    g.close()
+    for s in solvers:
+        s.close()

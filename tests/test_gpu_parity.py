@@ -284,12 +284,13 @@ def test_time_dependent_disturbances(rt, oracle_mod, nz, stepping):
     mc, dv = s.advance(6, t0, case.dt)
     got = s.get_state()
     assert dv == dv_w
-    # V and W are three orders below U here (only the wall disturbances drive them), so the
-    # velocity components are measured against the velocity scale max |U|
-    vscale = np.max(np.abs(want["U"]))
-    for k in fields_of(case):
-        err = rel_err(got[k], want[k]) if k not in "VW" else np.max(np.abs(got[k] - want[k])) / vscale
-        assert err <= 1e-11, (nz, stepping, k, err)
+    # every field against its own scale; V and W are three orders below U here (only the wall
+    # disturbances drive them), so their bar allows for the round-off of the O(1) terms
+    tol = {"U": 1e-11, "V": 1e-9, "W": 1e-9, "R": 1e-11, "E": 1e-11}
+    errs = {k: rel_err(got[k], want[k]) for k in fields_of(case)}
+    print("disturbed channel", nz, stepping, errs, flush=True)
+    for k, err in errs.items():
+        assert err <= tol[k], (nz, stepping, k, err)
     # the disturbances changed the state: V is non-zero on both walls
     assert np.max(np.abs(got["V"][:, 0, :])) > 1e-6 and np.max(np.abs(got["V"][:, -1, :])) > 1e-6
 

@@ -85,3 +85,33 @@ def test_periodic_fixed_domain_size_and_refinement():
 def test_node_count_from_spacing():
     X = generate_mesh(0.0, 4.0, MeshSpec(d0=0.1))
     assert np.max(np.diff(X)) <= 0.1 + 1e-15
+
+
+def test_file_mesh(tmp_path):
+    """'file' meshes (generateMesh.m:162-180): read as they are when the file holds every node,
+    the buffers added to the file's physical nodes with fileCalcBuffer, the node count checked
+    either way; MAT files give the variable named after the direction, text a column or row."""
+    from scipy.io import savemat
+    nodes = np.cumsum(np.linspace(0.1, 0.3, 17)) - 0.1
+    txt = tmp_path / "y.txt"
+    np.savetxt(txt, nodes[:, None])
+    mat = tmp_path / "mesh.mat"
+    savemat(mat, {"x": nodes[None, :], "y": 2 * nodes[None, :]})
+    np.testing.assert_array_equal(generate_mesh(0.0, 1.0, MeshSpec(n=17, type="file", file=str(txt)), "y"), nodes)
+    np.testing.assert_array_equal(generate_mesh(0.0, 1.0, MeshSpec(n=17, type="file", file=str(mat)), "y"), 2 * nodes)
+    b = BufferSpec(n=6, type="sigmoid", stretching=4.0)
+    with pytest.raises(ValueError, match="contains 17 nodes instead of 29"):
+        generate_mesh(0.0, 1.0, MeshSpec(n=17, type="file", file=str(mat), buffer_i=b, buffer_f=b), "x")
+    X = generate_mesh(0.0, 1.0, MeshSpec(n=17, type="file", file=str(mat), buffer_i=b, buffer_f=b,
+                                         file_calc_buffer=True), "x")
+    np.testing.assert_array_equal(X[6:23], nodes)
+    # the buffers are those a built-in mesh with the same edge spacings gets
+    ref = generate_mesh(0.0, nodes[-1], MeshSpec(n=17, buffer_i=b, buffer_f=b))
+    dref, dfile = ref[6] - ref[5], X[6] - X[5]
+    np.testing.assert_allclose((X[6] - X[:6]) / dfile, (ref[6] - ref[:6]) / dref, rtol=1e-12)
+    with pytest.raises(ValueError, match="contains 17 nodes instead of 18"):
+        generate_mesh(0.0, 1.0, MeshSpec(n=18, type="file", file=str(txt), file_calc_buffer=True), "y")
+    # a periodic file mesh takes no temporary end node
+    Xp = generate_mesh(0.0, 1.0, MeshSpec(n=17, type="file", file=str(txt), periodic=True,
+                                          fix_periodic_domain_size=True), "y")
+    assert len(Xp) == 17

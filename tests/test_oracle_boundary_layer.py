@@ -133,3 +133,48 @@ def test_pressure_inlet(oracle_mod):
     c = np.asarray(bl.bi.neumann)
     ext = sum(c[k] * P[-2 - k, 1:-1, 0] for k in range(len(c)))
     np.testing.assert_allclose(gm1 * st["R"][-1, 1:-1, 0] * q["E"][-1, 1:-1, 0], ext, rtol=1e-12)
+
+
+def test_free_slip_wall(oracle_mod):
+    """boundaryLayerFreeSlip.m: the whole wall carries zero-gradient u, w and p (one-sided
+    extrapolation of the neumann order), v = 0 and E = E0 from the inside boxes; no x break
+    point; the inflow, outflow and top as in the isothermal case."""
+    bl = cases.boundary_layer(nx=160, ny=64, nbuf=12, free_slip=True)
+    assert bl.deriv[0].ntypes == 1
+    o = oracle_mod.Oracle(bl)
+    rng = np.random.default_rng(7)
+    q = {k: np.asfortranarray(v + 0.01 * rng.standard_normal(v.shape)) for k, v in bl.state0.items()}
+    q["W"][:] = 0.0
+    st = o.apply_bound_cond(q, 0.0)
+    c = np.asarray(bl.bi.neumann)
+    L = len(c)
+    ext = sum(c[k] * st["U"][:, 1 + k, 0] for k in range(L))
+    np.testing.assert_allclose(st["U"][1:-1, 0, 0], ext[1:-1], rtol=1e-13, atol=1e-15)
+    assert np.all(st["U"][1:-1, 0, 0] != 0.0)                    # slip: no zero velocity on the plate
+    assert np.all(st["V"][:, 0, 0] == 0.0)
+    np.testing.assert_allclose(st["E"][1:-1, 0, 0], bl.E0, rtol=1e-15)
+    P = (bl.gamma - 1) * st["R"] * st["E"]
+    extP = sum(c[k] * P[1:-1, 1 + k, 0] for k in range(L))
+    np.testing.assert_allclose(P[1:-1, 0, 0], extP, rtol=1e-12)
+    assert np.all(st["U"][0, 1:, 0] == 1.0)
+    q0 = o.apply_bound_cond(bl.state0, 0.0)
+    o.init_mean(q0)
+    st, mc, dv = o.run(q0, 0.0, bl.dt, 5)
+    assert not dv and np.all(np.isfinite(st["U"]))
+
+
+def test_free_slip_periodic_lists():
+    """The periodic switches of boundaryLayerFreeSlip.m: x periodic drops the in- and outflow,
+    y periodic drops the top and the plate (all flow, no wall boxes)."""
+    from gatt_dns_amd.prep.boundaries import boundary_layer_free_slip
+    X = np.linspace(-1.0, 5.0, 24)
+    Y = np.linspace(0.0, 3.0, 12)
+    Z = np.array([0.0])
+    fr, walls, corners, bc, bp, hold = boundary_layer_free_slip(X, Y, Z, 2.0, 0.8, x_periodic=True)
+    assert bp is None and not hold
+    assert not any(d in ("xi", "xf") and t != "neu" for d, t in zip(bc.dir, bc.type))
+    assert sorted(set(v for v, d in zip(bc.var, bc.dir) if d == "yi")) == ["e", "p", "u", "v", "w"]
+    fr, walls, corners, bc, bp, hold = boundary_layer_free_slip(X, Y, Z, 2.0, 0.8, y_periodic=True)
+    assert fr.all() and not any(d in ("yi", "yf") for d in bc.dir)
+    fr, walls, corners, bc, bp, hold = boundary_layer_free_slip(X + 2.0, Y, Z, 2.0, 0.8)
+    assert hold and not any(t == "dir" and d == "xi" for d, t in zip(bc.dir, bc.type))
