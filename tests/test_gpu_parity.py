@@ -284,13 +284,18 @@ def test_time_dependent_disturbances(rt, oracle_mod, nz, stepping):
     mc, dv = s.advance(6, t0, case.dt)
     got = s.get_state()
     assert dv == dv_w
-    # every field against its own scale; V and W are three orders below U here (only the wall
-    # disturbances drive them), so their bar allows for the round-off of the O(1) terms
-    tol = {"U": 1e-11, "V": 1e-9, "W": 1e-9, "R": 1e-11, "E": 1e-11}
-    errs = {k: rel_err(got[k], want[k]) for k in fields_of(case)}
+    # every field against its own scale; V is three orders below U here (only the wall
+    # disturbances drive it), so its bar allows for the round-off of the O(1) terms.  The state
+    # and both disturbances are uniform in z, so W is zero in exact arithmetic: the oracle's and
+    # the GPU's W are round-off (5e-15 here), bounded against the velocity scale instead
+    tol = {"U": 1e-11, "V": 1e-9, "R": 1e-11, "E": 1e-11}
+    errs = {k: rel_err(got[k], want[k]) for k in fields_of(case) if k != "W"}
     print("disturbed channel", nz, stepping, errs, flush=True)
     for k, err in errs.items():
         assert err <= tol[k], (nz, stepping, k, err)
+    if case.is3d:
+        vscale = np.max(np.abs(want["U"]))
+        assert np.max(np.abs(got["W"])) <= 1e-13 * vscale and np.max(np.abs(want["W"])) <= 1e-13 * vscale
     # the disturbances changed the state: V is non-zero on both walls
     assert np.max(np.abs(got["V"][:, 0, :])) > 1e-6 and np.max(np.abs(got["V"][:, -1, :])) > 1e-6
 
