@@ -7,12 +7,17 @@ boundary conditions after every stage, the filter in x, y and z, the step
 scalars and the state update.
 
 Prints one JSON line.  ``value`` is Mpoint-steps/s with the state resident in
-HBM; ``roofline`` covers the kernel class that takes the largest share of the
-step, with its algorithmic bytes per launch divided by its mean launch time
-measured with HIP events on the library's stream; ``step_roofline`` applies the
-minimum-pass traffic model of the whole step (8 B x 377 array touches per
-point-step without SFD).  ``cpu_baseline`` times the CPU oracle (a scalar C
-restatement of the reference path) on a bounded sample of the same physics.
+HBM.  ``roofline`` treats the step as the unit of work: the algorithmic bytes
+of the minimum-pass schedule (8 B x 377 array touches per point-step in 3-D
+without SFD, 8 B x 196 plus the SFD arrays in 2-D) over the duration of one
+step, measured with HIP events around replays of the step graph on the
+library's stream; ``traffic`` is the HBM bytes one step moves, from rocprofv3
+FETCH_SIZE / WRITE_SIZE runs of the same workload (profiles/).  ``kernels``
+breaks the step down by kernel class (isolated launches, HIP events).
+``other_configs`` gives the 2-D lid-driven cavity (129^2, 1000 steps) and the
+2-D boundary layer (1024 x 256) the same way.  ``cpu_baseline`` times the CPU
+oracle (a scalar C restatement of the reference path) on a bounded sample of
+the same physics.
 """
 import argparse
 import json
@@ -26,9 +31,9 @@ sys.path.insert(0, ROOT)
 METRIC = "FP64 Mpoint-steps/s + %HBM roofline; batched-LST & adjoint cost vs plain"
 HBM_PEAK_GBPS = 8000.0
 MODEL_TOUCHES_3D = 377   # minimum-pass pencil schedule, SFD off
+STEP_CLASS = 11          # gatt_time_kernel: one step of the step graph
 
-# kernel symbols of the classes as rocprofv3 names them (profiles/*_hbm_bytes.json)
-CLASS_SYMBOL = {"assemble": "gatt::assemble_kernel(gatt::AssembleArgs)", "visc": "gatt::visc_kernel(gatt::ViscArgs)"}
+# measured HBM bytes per step of the 256^3 box (rocprofv3 FETCH_SIZE / WRITE_SIZE runs)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "turbulence_box_256_hbm_bytes.json")
 
 KERNELS = {0: "deriv_x", 1: "deriv_y", 2: "deriv_z", 3: "filter_x", 4: "filter_y", 5: "filter_z",
@@ -53,6 +58,38 @@ def cpu_baseline(n=64, steps=6):
     return {"value": case.npoints * steps / el / 1e6, "unit": "Mpoint-steps/s", "cores": 1, "kind": "port",
             "sample": "CPU oracle (scalar C restatement, gcc -O2, 1 thread) on the same turbulence box at "
                       "%d^3, %d steps, %.1f s" % (n, steps, el)}
+
+
+def step_roofline(s, reps=5):
+    """HIP-event duration of one step of the step graph and the minimum-pass bytes of a step."""
+    ms, by = s.time_kernel(STEP_CLASS, reps)
+    gbps = by / (ms * 1e-3) / 1e9
+    return ms, by, gbps
+
+
+def plain_config(runtime, label, case, steps, workload):
+    """Mpoint-steps/s of one plain run of `steps` steps from the case's initial flow, and the step
+    roofline of the same case."""
+    s = runtime.Solver(case)
+    s.set_state(case.state0)
+    s.apply_bound_cond(0.0)
+    s.advance(2, 0.0, case.dt)
+    s.sync()
+    t0 = time.perf_counter()
+    mc, dv = s.advance(steps, 2 * case.dt, case.dt)
+    s.sync()
+    el = time.perf_counter() - t0
+    ms, by, gbps = step_roofline(s)
+    s.close()
+    out = {"workload": workload, "grid": [case.nx, case.ny] + ([case.nz] if case.is3d else []),
+           "steps": steps, "value": round(case.npoints * steps / el / 1e6, 3), "unit": "Mpoint-steps/s",
+           "ms_per_step": round(el / steps * 1e3, 4), "diverged": bool(dv),
+           "step_ms_hip_events": round(ms, 4),
+           "model_bytes_per_point_step": round(by / case.npoints, 1),
+           "roofline": {"bound": "hbm", "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                        "frac": round(gbps / HBM_PEAK_GBPS, 4)}}
+    log("  %s: %s" % (label, out))
+    return out
 
 
 def batched_throughput(runtime, case, members, steps, shapes=None):
@@ -217,6 +254,9 @@ def main():
                     help="only time --steps adjoint steps of the boundary layer (prints their JSON; for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference for HBM byte counters)")
+    ap.add_argument("--no-other", action="store_true", help="skip the cavity and boundary-layer lines")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="only run this many steps of the box with the default shapes (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the time-advance path runs on a single MI355X here; "
@@ -232,6 +272,26 @@ def main():
         sys.exit(1)
     if args.adjoint_only:
         print(json.dumps(adjoint_cost(runtime, steps=args.steps, only_adjoint=True)), flush=True)
+        return
+    if args.profile_steps:
+        # the profiled form: nothing but the steps (one graph of two steps replayed)
+        case = cases.turbulence_box(args.n)
+        s = runtime.Solver(case)
+        s.set_state(case.state0)
+        s.sync()
+        t0 = time.perf_counter()
+        s.advance(args.profile_steps, 0.0, case.dt)
+        s.sync()
+        el = time.perf_counter() - t0
+        s.close()
+        if args.calibrate_copy:
+            src = torch.rand(1 << 25, dtype=torch.float64, device="cuda")
+            dst = torch.empty_like(src)
+            for _ in range(3):
+                dst.copy_(src)
+            torch.cuda.synchronize()
+        print(json.dumps({"profile_steps": args.profile_steps, "grid": [args.n] * 3,
+                          "ms_per_step": round(el / args.profile_steps * 1e3, 4)}), flush=True)
         return
     log("preparing turbulence box %d^3" % args.n)
     case = cases.turbulence_box(args.n)
@@ -271,23 +331,40 @@ def main():
     for k in (3, 4, 5):   # filters overwrite the state: time them last
         ms, by = s.time_kernel(k, 10)
         per_class[KERNELS[k]] = {"ms": ms, "bytes": by, "share_ms_per_step": ms * LAUNCHES_PER_STEP[k]}
-    dom = max(per_class, key=lambda n: per_class[n]["share_ms_per_step"])
-    d = per_class[dom]
-    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
     for name, v in per_class.items():
         log("  %-9s %.4f ms/launch  %.0f GB/s algorithmic  %.3f ms/step"
             % (name, v["ms"], v["bytes"] / (v["ms"] * 1e-3) / 1e9, v["share_ms_per_step"]))
-    # measured HBM bytes per launch of the dominant kernel (rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
-    # the same workload, calibrated against a device copy; committed under profiles/)
-    traffic = None
-    if args.n == 256 and dom in CLASS_SYMBOL and os.path.exists(TRAFFIC_FILE):
-        rec = json.load(open(TRAFFIC_FILE))["kernels"].get(CLASS_SYMBOL[dom])
-        if rec:
-            traffic = rec["read_bytes_per_dispatch"] + rec["write_bytes_per_dispatch"]
     shapes_out = {"%s_%s" % ("xyz"[d], "deriv" if k == 0 else "filter"): list(s.get_line_shape(d, k))
                   for d in range(3) for k in (0, 1)}
-    step_bytes = 8.0 * MODEL_TOUCHES_3D * case.npoints
-    step_gbps = step_bytes / (ms_step * 1e-3) / 1e9
+    # the step as the unit: minimum-pass bytes over the HIP-event duration of one graph-replayed step
+    s.set_state(case.state0)
+    s.advance(2, 0.0, dt)
+    step_ms, step_bytes, step_gbps = step_roofline(s)
+    log("step graph: %.4f ms/step (HIP events), %.0f GB/s of minimum-pass bytes" % (step_ms, step_gbps))
+    # measured HBM bytes of one step (rocprofv3 FETCH_SIZE / WRITE_SIZE runs of the same workload,
+    # calibrated against a device copy; committed under profiles/)
+    traffic = None
+    if args.n == 256 and os.path.exists(TRAFFIC_FILE):
+        traffic = json.load(open(TRAFFIC_FILE)).get("bytes_per_step")
+
+    other = None
+    if not args.no_other:
+        s.close()
+        del s
+        torch.cuda.empty_cache()
+        other = {}
+        log("lid-driven cavity 129^2")
+        other["lid_driven_129"] = plain_config(
+            runtime, "lid_driven_129", cases.lid_driven(129), 1000,
+            "2-D lid-driven cavity 129x129, Re 1000, Ma 0.5, fluid at rest, lid u = 1, fixed-dt RK4 (CFL 0.5), "
+            "filter every step")
+        log("boundary layer 1024x256")
+        other["boundary_layer_1024x256"] = plain_config(
+            runtime, "boundary_layer_1024x256", cases.boundary_layer(), 200,
+            "2-D isothermal-wall boundary layer 1024x256 (x in [-1, 30], y in [0, 8], wall ratio 1.02, "
+            "40-point buffers), Re 1000, Ma 0.5, compressible similarity initial flow + 1e-5 noise, "
+            "SFD chi 0.02, Delta 10")
+        s = None
 
     if args.calibrate_copy:
         src = torch.rand(1 << 25, dtype=torch.float64, device="cuda")
@@ -302,8 +379,9 @@ def main():
         # stability analysis: a block of 32 Krylov members advanced together versus
         # 32 sequential plain runs (calcInstability.m launches one DNS per vector)
         lst = {}
-        s.close()
-        del s
+        if s is not None:
+            s.close()
+            s = None
         torch.cuda.empty_cache()
         for label, mk, nsteps, shapes in (
                 ("turbulence_box_%d" % args.n, lambda: case, 2, tuned),
@@ -336,15 +414,17 @@ def main():
         "config": {"workload": "3-D periodic turbulence box %d^3, Re 1000, Ma 0.3, SL4 + 10th-order filter "
                                "(alpha 0.49) every step, turbulenceGenerator forcing, fixed-dt RK4" % args.n,
                    "grid": [args.n] * 3, "dt": dt, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+        "roofline": {"bound": "hbm", "kernel": "rk4_step_graph", "achieved": round(step_gbps, 1),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(step_gbps / HBM_PEAK_GBPS, 4),
+                     "traffic": traffic,
                      "traffic_source": "profiles/turbulence_box_256_hbm_bytes.json" if traffic else None,
-                     "bytes_per_launch": d["bytes"], "ms_per_launch": round(d["ms"], 5)},
-        "step_roofline": {"model_bytes_per_point_step": 8 * MODEL_TOUCHES_3D, "achieved": round(step_gbps, 1),
-                          "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(step_gbps / HBM_PEAK_GBPS, 4)},
-        "kernels": {k: {"ms_per_launch": round(v["ms"], 5), "ms_per_step": round(v["share_ms_per_step"], 4)}
+                     "model_bytes_per_point_step": 8 * MODEL_TOUCHES_3D, "bytes_per_step": step_bytes,
+                     "ms_per_step_hip_events": round(step_ms, 5)},
+        "kernels": {k: {"ms_per_launch": round(v["ms"], 5), "ms_per_step": round(v["share_ms_per_step"], 4),
+                        "algorithmic_gbps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)}
                     for k, v in per_class.items()},
         "line_shapes": shapes_out,
+        "other_configs": other,
         "batched_lst": lst,
         "adjoint": adj,
         "cpu_baseline": cpu,

@@ -1736,6 +1736,30 @@ int gatt_time_kernel(gatt_ctx* c, int32_t kernel, int32_t reps, double* avg_ms, 
       arrays = c->is3d ? 52 : 34;
     }
   };
+  if (kernel == GATT_K_STEP) {
+    // the graph of two steps gatt_advance replays; each replay leaves cur where it was
+    if (!c->graph) return fail(GATT_E_INVALID, "advance at least two steps before timing the step graph");
+    HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    HIPCHK(hipGetLastError());
+    int sfd = 0;
+    if (c->sfdType == 2 && c->sfdDelta > 0) sfd = c->is3d ? 11 : 9;
+    else if (c->sfdType == 1 && c->sfdDelta > 0) sfd = c->is3d ? 10 : 8;
+    else if (c->sfdType > 0) sfd = c->is3d ? 6 : 5;
+    if (avg_ms) *avg_ms = (double)ms / (2.0 * reps);
+    if (bytes) *bytes = 8.0 * ((c->is3d ? 377 : 196) + sfd) * (double)np * c->batch;
+    return GATT_OK;
+  }
   if (kernel < 0 || kernel > GATT_K_RHS_Y ||
       (!c->is3d && (kernel == GATT_K_DERIV_Z || kernel == GATT_K_FILTER_Z || kernel == GATT_K_FLUX_Z)))
     return fail(GATT_E_INVALID, "unknown kernel class");

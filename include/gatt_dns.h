@@ -276,19 +276,28 @@ int gatt_get_line_shape(gatt_ctx* ctx, int32_t dir, int32_t kind, int32_t* varia
  * ASSEMBLE: the stresses and the assembly of the unfused stage; FLUX_X / FLUX_Z:
  * derivatives of the viscous fluxes along x / z with the fluxes computed as they
  * are loaded; RHS_Y: the y flux derivatives fused with the assembly and the
- * stage update (the last two kinds only where the context's step uses them). */
+ * stage update (the last two kinds only where the context's step uses them).
+ * STEP: one whole main-loop step, as gatt_advance replays it (a graph of two
+ * steps launched on the context's stream; gatt_advance must have run at least
+ * two steps first, and the timing advances the state). */
 enum {
   GATT_K_DERIV_X = 0, GATT_K_DERIV_Y = 1, GATT_K_DERIV_Z = 2,
   GATT_K_FILTER_X = 3, GATT_K_FILTER_Y = 4, GATT_K_FILTER_Z = 5,
   GATT_K_VISC = 6, GATT_K_ASSEMBLE = 7,
-  GATT_K_FLUX_X = 8, GATT_K_FLUX_Z = 9, GATT_K_RHS_Y = 10
+  GATT_K_FLUX_X = 8, GATT_K_FLUX_Z = 9, GATT_K_RHS_Y = 10,
+  GATT_K_STEP = 11
 };
 
 /* Time one kernel class on the context's stream with HIP events: one launch of
  * that class is replayed `reps` times.  Returns the mean duration of one launch
  * in ms and the algorithmic bytes of that launch (8 B per point for every
- * distinct array read or written once).  Filter timing overwrites the state.
- * No reference counterpart (measurement only). */
+ * distinct array read or written once).  For STEP the duration is that of one
+ * step and the bytes are those of the minimum-pass schedule of an RK4 step
+ * with the filter and the step scalars: 8 B x (377 + s) per point and member in
+ * 3-D, 8 B x (196 + s) in 2-D, s the SFD arrays (3-D / 2-D: 11 / 9 for type 2
+ * with an evolving mean, 10 / 8 for type 1, 6 / 5 for a frozen mean, 0 off).
+ * Filter timing overwrites the state.  No reference counterpart (measurement
+ * only). */
 int gatt_time_kernel(gatt_ctx* ctx, int32_t kernel, int32_t reps, double* avg_ms, double* bytes);
 
 /* ------------------------------------------------------------------------
