@@ -40,9 +40,9 @@ __device__ __forceinline__ double powi18(double x) {
   return x16 * x2;
 }
 
-__global__ __launch_bounds__(256) void bc_wave_kernel(BcArgs a) {
+__device__ __forceinline__ void bc_wave_body(const BcArgs& a, BlockPos bp) {
   // find this block's op
-  const long long blk = blockIdx.x;
+  const long long blk = bp.x;
   int o = a.op_begin;
   while (o + 1 < a.op_end && a.ops[o + 1].first_block <= blk) ++o;
   const BcOp& op = a.ops[o];
@@ -54,14 +54,14 @@ __global__ __launch_bounds__(256) void bc_wave_kernel(BcArgs a) {
   const int j = op.box[2] + (int)((p / bi) % bj);
   const int k = op.box[4] + (int)(p / ((long long)bi * bj));
   const int nx = a.nx, ny = a.ny;
-  const long long mo = (long long)blockIdx.y * a.member_stride;
+  const long long mo = (long long)bp.y * a.member_stride;
   double* U = a.q[0] + mo;
   double* R = a.q[3] + mo;
   double* E = a.q[4] + mo;
   const long long n = pidx(i, j, k, nx, ny);
   if (a.undo) {
     // a point is written by one op of a wave only, and read by no other op of it
-    double* u = a.undo + (long long)blockIdx.y * a.undo_member_stride + op.undo;
+    double* u = a.undo + (long long)bp.y * a.undo_member_stride + op.undo;
 #pragma unroll
     for (int f = 0; f < 5; ++f) u[(long long)f * op.npts + p] = (a.q[f] + mo)[n];
   }
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void bc_wave_kernel(BcArgs a) {
     }
     case BC_HOLD: {
       // holdInlet.F90:18-48: capture on the first call, restore afterwards
-      double* save = a.hold + (long long)blockIdx.y * a.hold_member_stride + op.aux;
+      double* save = a.hold + (long long)bp.y * a.hold_member_stride + op.aux;
       const int vars[5] = {0, 1, 2, 4, 3};
       const int nv = op.hold_r ? 5 : 4;
       for (int v = 0; v < nv; ++v) {
@@ -168,6 +168,13 @@ __global__ __launch_bounds__(256) void bc_wave_kernel(BcArgs a) {
   }
 }
 
+__global__ __launch_bounds__(256) void bc_wave_kernel(BcArgs a) { bc_wave_body(a, grid_pos()); }
+__global__ __launch_bounds__(256) void bc_wave_ragged(const BcArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const int m = ragged_pos(map, bp);
+  bc_wave_body(args[m], bp);
+}
+
 __global__ void clock_set_kernel(double* clock, double t) {
   if (threadIdx.x == 0) clock[0] = t;
 }
@@ -175,18 +182,26 @@ __global__ void clock_set_kernel(double* clock, double t) {
 __global__ void clock_add_kernel(double* clock, double dt) {
   if (threadIdx.x == 0) clock[0] = clock[0] + dt;
 }
+__global__ void clock_add_ragged(const ClockArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const ClockArgs& a = args[ragged_pos(map, bp)];
+  if (threadIdx.x == 0) a.clock[0] = a.clock[0] + a.dt;
+}
 
 void launch_clock_set(double* clock, double t, hipStream_t st) {
   hipLaunchKernelGGL(clock_set_kernel, dim3(1), dim3(64), 0, st, clock, t);
 }
 
 void launch_clock_add(double* clock, double dt, hipStream_t st) {
+  if (record_launch((const void*)clock_add_ragged, ClockArgs{clock, dt}, dim3(1), dim3(64), 0)) return;
   hipLaunchKernelGGL(clock_add_kernel, dim3(1), dim3(64), 0, st, clock, dt);
 }
 
 void launch_bc_wave(const BcArgs& a, long long nblocks, int nmembers, hipStream_t st) {
   if (nblocks <= 0) return;
-  hipLaunchKernelGGL(bc_wave_kernel, dim3((unsigned)nblocks, nmembers), dim3(256), 0, st, a);
+  const dim3 grid((unsigned)nblocks, nmembers);
+  if (record_launch((const void*)bc_wave_ragged, a, grid, dim3(256), 0)) return;
+  hipLaunchKernelGGL(bc_wave_kernel, grid, dim3(256), 0, st, a);
 }
 
 }  // namespace gatt

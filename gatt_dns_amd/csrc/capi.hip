@@ -1,6 +1,7 @@
 // C ABI of libgatt_dns (include/gatt_dns.h): context set-up, the boundary
 // program compiler, and the orchestration of the time step on one HIP stream.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -13,6 +14,10 @@
 #include "gatt_internal.h"
 
 using namespace gatt;
+
+namespace gatt {
+thread_local LaunchRecorder* g_recorder = nullptr;
+}
 
 // Scratch of the allocation-free block product (gatt_block_dot_noalloc): the per-chunk
 // partial sums and the reduced k x p block.  Each device holds its own copy for the
@@ -714,6 +719,14 @@ int compile_bc(gatt_ctx* c, const gatt_bc_desc& bc) {
 // applyBoundCond at time *clock + toff (the clock holds the t of the step in progress);
 // with `undo`, every op point first saves its five values there (member stride undo_total)
 void run_bc(gatt_ctx* c, double* q0, double toff, double* undo = nullptr) {
+  if (g_recorder) {
+    // a recorded step sequence marks where each boundary application starts, so that the
+    // members' applications line up when their numbers of waves differ
+    g_recorder->bc = g_recorder->nbc++;
+    LaunchRecord mark;
+    mark.bc = g_recorder->bc;
+    g_recorder->recs.push_back(std::move(mark));
+  }
   for (const BcWave& w : c->waves) {
     BcArgs a{};
     a.ops = c->d_ops;
@@ -737,6 +750,7 @@ void run_bc(gatt_ctx* c, double* q0, double toff, double* undo = nullptr) {
     a.undo_member_stride = c->undo_total;
     launch_bc_wave(a, w.nblocks, c->batch, c->stream);
   }
+  if (g_recorder) g_recorder->bc = -1;
   c->hold_captured = true;
 }
 
@@ -2063,8 +2077,12 @@ int gatt_block_update(gatt_ctx* c, int64_t n, int32_t k, int32_t p, const double
 
 struct gatt_group {
   std::vector<gatt_ctx*> ctx;
-  int mode = 0;                         // 0: one graph with a branch per member; 1: per-member graphs
+  int mode = 0;                         // 0: one graph with a branch per member; 1: per-member graphs;
+                                        // 2: one graph of ragged launches (every member in each launch)
   hipGraphExec_t graph = nullptr;
+  void* ragged_buf = nullptr;           // mode 2: argument blocks and block tables of the graph's launches
+  int ragged_launches = 0;              // mode 2: launches of one double step
+  std::vector<std::array<gatt::LineShape, 6>> shapes0;   // mode 2: the members' own launch shapes
   std::vector<hipGraphExec_t> mgraph;   // mode 1: one double-step graph per member
   std::vector<double> graph_dt;
   std::vector<int> graph_cur;
@@ -2077,6 +2095,12 @@ void group_drop_graph(gatt_group* g) {
   if (g->graph) {
     (void)hipGraphExecDestroy(g->graph);
     g->graph = nullptr;
+  }
+  if (g->ragged_buf) {
+    // the graph's launches read the argument blocks: none may be in flight
+    for (gatt_ctx* c : g->ctx) (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(g->ragged_buf);
+    g->ragged_buf = nullptr;
   }
   for (hipGraphExec_t& x : g->mgraph)
     if (x) (void)hipGraphExecDestroy(x);
@@ -2127,13 +2151,245 @@ int group_capture(gatt_group* g, const double* dt) {
   return GATT_OK;
 }
 
+// Mode 2: one launch shape per direction and operator that every member can run with the
+// same kernel instantiation and workgroup size.  The member with the longest lines picks
+// the shape; the one-wavefront x solve (variant 4) and the lane-group solve (variant 0)
+// take any shorter line with the same points per lane, the wave-segment variants (1-3)
+// size their workgroups by the line length and are replaced by the lane-group solve
+// when the members' lines differ in length.
+int group_harmonise_shapes(gatt_group* g) {
+  const gatt_ctx* c0 = g->ctx[0];
+  const int ndir = c0->is3d ? 3 : 2;
+  for (int d = 0; d < ndir; ++d)
+    for (int kind : {GATT_DERIV, GATT_FILTER}) {
+      // y and z lines: the wave-segment solve (lane = line) with one workgroup size for all,
+      // the segment count of the longest line, when every member's tiles carry one operator
+      // type, no line is periodic and the members bring enough lines between them to fill it
+      if (d > 0) {
+        bool ok = true;
+        long long lines = 0;
+        int nmax = 0;
+        for (gatt_ctx* c : g->ctx) {
+          const DirOps& o = c->dir[d];
+          ok = ok && !o.identity[kind] && o.uniform_tiles && !o.op[kind].periodic;
+          lines += o.nlines;
+          nmax = std::max(nmax, o.op[kind].N);
+        }
+        bool done = false;
+        for (int M : {16, 32}) {
+          if (!ok || lines < 1024 || done) break;
+          LineShape t;
+          t.variant = 2;
+          t.M = M;
+          t.G = (nmax + M - 1) / M;
+          t.waves_per_block = t.G;
+          bool fits = true;
+          for (gatt_ctx* c : g->ctx)
+            fits = fits && line_shape_ok(t, c->dir[d].op[kind].N, c->dir[d].op[kind].nR, d, c->max_lds, true);
+          if (!fits) continue;
+          for (gatt_ctx* c : g->ctx) {
+            c->dir[d].shape[kind] = t;
+            c->drop_graphs();
+          }
+          done = true;
+        }
+        if (done) continue;
+      }
+      int big = -1;
+      bool same_n = true;
+      for (size_t m = 0; m < g->ctx.size(); ++m) {
+        const DirOps& o = g->ctx[m]->dir[d];
+        if (o.identity[kind]) continue;
+        if (big >= 0 && o.op[kind].N != g->ctx[big]->dir[d].op[kind].N) same_n = false;
+        if (big < 0 || o.op[kind].N > g->ctx[big]->dir[d].op[kind].N) big = (int)m;
+      }
+      if (big < 0) continue;
+      const DirOps& ob = g->ctx[big]->dir[d];
+      LineShape s = ob.shape[kind];
+      if (s.variant >= 1 && s.variant <= 3 && !same_n) {
+        s.variant = -1;
+        const int N = ob.op[kind].N;
+        for (int M : {16, 8, 4, 32}) {
+          for (int wpb : {4, 2, 1}) {
+            LineShape t;
+            t.M = M;
+            t.G = std::max(2, pow2_at_least((N + M - 1) / M));
+            t.waves_per_block = wpb;
+            if (line_shape_ok(t, N, ob.op[kind].nR)) {
+              s = t;
+              break;
+            }
+          }
+          if (s.variant == 0) break;
+        }
+        if (s.variant != 0) return fail(GATT_E_UNSUPPORTED, "ragged group: no common line shape");
+      }
+      for (gatt_ctx* c : g->ctx) {
+        DirOps& o = c->dir[d];
+        if (o.identity[kind]) continue;
+        const int N = o.op[kind].N;
+        LineShape t = s;
+        if (s.variant == 0) {
+          t.G = std::min(64, std::max(2, pow2_at_least((N + s.M - 1) / s.M)));
+          if (std::max(4, pow2_at_least((N + t.G - 1) / t.G)) != s.M)
+            return fail(GATT_E_UNSUPPORTED, "ragged group: line lengths too far apart for one lane-group shape");
+        } else if (s.variant >= 1 && s.variant <= 3) {
+          t.G = (N + s.M - 1) / s.M;
+        }
+        if (!line_shape_ok(t, N, o.op[kind].nR, d, c->max_lds, o.uniform_tiles))
+          return fail(GATT_E_UNSUPPORTED, "ragged group: a member cannot take the common line shape");
+        o.shape[kind] = t;
+        c->drop_graphs();
+      }
+    }
+  return GATT_OK;
+}
+
+// Mode 2: record the double step of every member (the launches it would make, with their
+// argument blocks and grids) and merge the i-th launches of all members into one ragged
+// launch.  Boundary applications are merged wave by wave (a member with fewer waves takes
+// no blocks in the later ones).  The merged launches are captured into one graph on the
+// first member's stream; their argument blocks and block tables live in one device buffer
+// owned by the group.
+int group_capture_ragged(gatt_group* g, const double* dt) {
+  const size_t n = g->ctx.size();
+  std::vector<LaunchRecorder> rec(n);
+  for (size_t m = 0; m < n; ++m) {
+    gatt_ctx* c = g->ctx[m];
+    const bool fork = c->fork_dirs;
+    c->fork_dirs = false;
+    g_recorder = &rec[m];
+    enqueue_step(c, c->cur, dt[m]);
+    enqueue_step(c, 1 - c->cur, dt[m]);
+    g_recorder = nullptr;
+    c->fork_dirs = fork;
+  }
+  // each member's sequence as segments: a single launch, or the waves of one boundary application
+  struct Seg {
+    int bc;
+    std::vector<const LaunchRecord*> r;
+  };
+  std::vector<std::vector<Seg>> segs(n);
+  for (size_t m = 0; m < n; ++m)
+    for (const LaunchRecord& r : rec[m].recs) {
+      if (r.bc == -2) return fail(GATT_E_UNSUPPORTED, "ragged group: a member's step has a launch without a ragged form");
+      if (!r.fn) segs[m].push_back(Seg{r.bc, {}});
+      else if (r.bc >= 0) segs[m].back().r.push_back(&r);
+      else segs[m].push_back(Seg{-1, {&r}});
+    }
+  for (size_t m = 1; m < n; ++m) {
+    if (segs[m].size() != segs[0].size()) return fail(GATT_E_UNSUPPORTED, "ragged group: the members' steps differ");
+    for (size_t k = 0; k < segs[0].size(); ++k) {
+      const Seg& a = segs[0][k];
+      const Seg& b = segs[m][k];
+      if ((a.bc >= 0) != (b.bc >= 0)) return fail(GATT_E_UNSUPPORTED, "ragged group: the members' steps differ");
+      if (a.bc < 0 && (a.r[0]->fn != b.r[0]->fn || a.r[0]->args.size() != b.r[0]->args.size() ||
+                       a.r[0]->block.x != b.r[0]->block.x))
+        return fail(GATT_E_UNSUPPORTED, "ragged group: the members' launches differ (kernel or workgroup size) at launch " +
+                                            std::to_string(k) + " of member " + std::to_string(m));
+    }
+  }
+  // the merged launches: per launch one record (or none) per member
+  std::vector<std::vector<const LaunchRecord*>> launches;
+  for (size_t k = 0; k < segs[0].size(); ++k) {
+    size_t waves = 0;
+    for (size_t m = 0; m < n; ++m) waves = std::max(waves, segs[m][k].r.size());
+    for (size_t w = 0; w < waves; ++w) {
+      std::vector<const LaunchRecord*> l(n, nullptr);
+      for (size_t m = 0; m < n; ++m)
+        if (w < segs[m][k].r.size()) l[m] = segs[m][k].r[w];
+      launches.push_back(l);
+    }
+  }
+  // device layout: per launch the n argument blocks, then start[n + 1] and gx[n]
+  auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  struct Plan {
+    const void* fn;
+    dim3 block;
+    size_t shmem;
+    size_t args_off, start_off, gx_off;
+    int total;
+  };
+  std::vector<Plan> plan;
+  size_t bytes = 0;
+  for (const auto& l : launches) {
+    Plan p{};
+    size_t asz = 0;
+    for (const LaunchRecord* r : l)
+      if (r) {
+        if (p.fn && (r->fn != p.fn || r->block.x != p.block.x))
+          return fail(GATT_E_UNSUPPORTED, "ragged group: the members' launches differ (kernel or workgroup size)");
+        p.fn = r->fn;
+        p.block = r->block;
+        p.shmem = std::max(p.shmem, r->shmem);
+        asz = r->args.size();
+      }
+    p.args_off = bytes;
+    bytes = align(bytes + asz * n);
+    p.start_off = bytes;
+    bytes = align(bytes + sizeof(int) * (n + 1));
+    p.gx_off = bytes;
+    bytes = align(bytes + sizeof(int) * n);
+    plan.push_back(p);
+  }
+  std::vector<char> host(bytes, 0);
+  for (size_t i = 0; i < launches.size(); ++i) {
+    Plan& p = plan[i];
+    int* start = reinterpret_cast<int*>(host.data() + p.start_off);
+    int* gx = reinterpret_cast<int*>(host.data() + p.gx_off);
+    long long total = 0;
+    for (size_t m = 0; m < n; ++m) {
+      const LaunchRecord* r = launches[i][m];
+      start[m] = (int)total;
+      gx[m] = 1;
+      if (!r) continue;
+      std::memcpy(host.data() + p.args_off + m * r->args.size(), r->args.data(), r->args.size());
+      gx[m] = (int)r->gx;
+      total += (long long)r->gx * r->gy;
+    }
+    if (total <= 0 || total > 0x7fffffffLL) return fail(GATT_E_UNSUPPORTED, "ragged group: launch size out of range");
+    start[n] = (int)total;
+    p.total = (int)total;
+  }
+  group_drop_graph(g);
+  HIPCHK(hipMalloc(&g->ragged_buf, std::max<size_t>(bytes, 256)));
+  HIPCHK(hipMemcpy(g->ragged_buf, host.data(), bytes, hipMemcpyHostToDevice));
+  char* base = static_cast<char*>(g->ragged_buf);
+  for (const Plan& p : plan)
+    if (p.shmem > 65536)
+      HIPCHK(hipFuncSetAttribute(p.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.shmem));
+  gatt_ctx* c0 = g->ctx[0];
+  hipGraph_t graph;
+  HIPCHK(hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  for (const Plan& p : plan) {
+    const void* args = base + p.args_off;
+    RaggedMap map{reinterpret_cast<const int*>(base + p.start_off), reinterpret_cast<const int*>(base + p.gx_off),
+                  (int)n};
+    void* params[] = {&args, &map};
+    e = hipLaunchKernel(p.fn, dim3((unsigned)p.total), p.block, params, p.shmem, c0->stream);
+    if (e != hipSuccess) break;
+  }
+  const hipError_t e2 = hipStreamEndCapture(c0->stream, &graph);
+  if (e != hipSuccess || e2 != hipSuccess)
+    return fail(GATT_E_HIP, std::string("ragged group capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+  e = hipGraphInstantiate(&g->graph, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(GATT_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  g->ragged_launches = (int)plan.size();
+  g->graph_dt.assign(dt, dt + n);
+  g->graph_cur.clear();
+  for (gatt_ctx* c : g->ctx) g->graph_cur.push_back(c->cur);
+  return GATT_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 int gatt_group_create(gatt_ctx* const* ctxs, int32_t n, int32_t mode, gatt_group** out) {
   if (!ctxs || !out || n < 1) return fail(GATT_E_INVALID, "bad group arguments");
-  if (mode != 0 && mode != 1) return fail(GATT_E_INVALID, "group mode must be 0 or 1");
+  if (mode < 0 || mode > 2) return fail(GATT_E_INVALID, "group mode must be 0, 1 or 2");
   *out = nullptr;
   std::unique_ptr<gatt_group> g(new gatt_group);
   for (int32_t m = 0; m < n; ++m) {
@@ -2143,6 +2399,23 @@ int gatt_group_create(gatt_ctx* const* ctxs, int32_t n, int32_t mode, gatt_group
     g->ctx.push_back(ctxs[m]);
   }
   g->mode = mode;
+  if (mode == 2) {
+    for (gatt_ctx* c : g->ctx) {
+      if (c->is3d != g->ctx[0]->is3d) return fail(GATT_E_UNSUPPORTED, "ragged group: 2-D and 3-D members mixed");
+      if (c->fuse_rhs) return fail(GATT_E_UNSUPPORTED, "ragged group: the fused stage kernel has no ragged form");
+      std::array<LineShape, 6> sh;
+      for (int d = 0; d < 3; ++d)
+        for (int k = 0; k < 2; ++k) sh[2 * d + k] = c->dir[d].shape[k];
+      g->shapes0.push_back(sh);
+    }
+    const int rc = group_harmonise_shapes(g.get());
+    if (rc) {
+      for (size_t m = 0; m < g->ctx.size(); ++m)
+        for (int d = 0; d < 3; ++d)
+          for (int k = 0; k < 2; ++k) g->ctx[m]->dir[d].shape[k] = g->shapes0[m][2 * d + k];
+      return rc;
+    }
+  }
   g->ev.resize(n);
   for (auto& e : g->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   *out = g.release();
@@ -2153,6 +2426,13 @@ int gatt_group_destroy(gatt_group* g) {
   if (!g) return GATT_OK;
   for (gatt_ctx* c : g->ctx) (void)hipStreamSynchronize(c->stream);
   group_drop_graph(g);
+  // mode 2: the members get their own launch shapes back
+  for (size_t m = 0; m < g->shapes0.size(); ++m) {
+    gatt_ctx* c = g->ctx[m];
+    for (int d = 0; d < 3; ++d)
+      for (int k = 0; k < 2; ++k) c->dir[d].shape[k] = g->shapes0[m][2 * d + k];
+    c->drop_graphs();
+  }
   for (auto& e : g->ev) (void)hipEventDestroy(e);
   delete g;
   return GATT_OK;
@@ -2213,7 +2493,7 @@ int gatt_group_advance(gatt_group* g, int32_t nsteps, const double* t0, const do
       stale = g->graph_dt[m] != dt[m] || g->graph_cur[m] != g->ctx[m]->cur;
     if (stale) group_drop_graph(g);
     if (nsteps >= 2 && !g->graph) {
-      int rc = group_capture(g, dt);
+      int rc = g->mode == 2 ? group_capture_ragged(g, dt) : group_capture(g, dt);
       if (rc) return rc;
     }
     // the graph runs on the first member's stream: it follows work pending on every member's stream

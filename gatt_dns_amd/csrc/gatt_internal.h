@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/gatt_dns.h"
+#include "ragged.h"
 
 namespace gatt {
 
@@ -177,7 +178,8 @@ inline bool line_shape_ok(const LineShape& s, int N, int nR, int dir = 1, long l
     // (variant 2) or, along x only, through an LDS tile (variant 3); LDS holds
     // the scan maps and coefficient slots (in the tile's words for variant 3)
     if (!(s.M == 16 || s.M == 32) || !uniform_tiles || (s.variant == 3 && dir != 0)) return false;
-    const int G = (N + s.M - 1) / s.M;
+    // variant 2 may carry more segments than the line needs (empty ones past its end)
+    const int G = s.variant == 2 && s.G > (N + s.M - 1) / s.M ? s.G : (N + s.M - 1) / s.M;
     const int gmax = s.variant == 3 ? 16 : (s.M >= 32 ? 8 : 16);   // variant 3: two segments per wavefront
     if (G < 1 || G > gmax || s.M + nR - 1 > N + 1 || nR - 1 > N) return false;
     const long long cs = (2LL * nR - 1 + 3 + 1) & ~1LL;

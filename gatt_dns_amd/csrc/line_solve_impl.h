@@ -43,7 +43,7 @@ __device__ __forceinline__ long long line_base(const SolveArgs& a, int line) {
 }
 
 template <int DIR, int NR, int M>
-__global__ __launch_bounds__(256) void line_solve_kernel(SolveArgs a) {
+__device__ __forceinline__ void line_solve_lg_body(const SolveArgs& a, BlockPos bp) {
   extern __shared__ double lds[];
   constexpr int H = NR - 1;
   constexpr int W = M + 2 * H;
@@ -55,10 +55,10 @@ __global__ __launch_bounds__(256) void line_solve_kernel(SolveArgs a) {
   const int lane = tid & 63;
   const int g = lane & (G - 1);
   const int l = (tid >> 6) * (64 / G) + lane / G;
-  const int line0 = blockIdx.x * TL;
+  const int line0 = bp.x * TL;
   const int ntile = min(TL, a.nlines - line0);
-  const int f = (int)blockIdx.y % a.nf;
-  const int mem = (int)blockIdx.y / a.nf;
+  const int f = (int)bp.y % a.nf;
+  const int mem = (int)bp.y / a.nf;
   const double* in = a.in[f] + (long long)mem * a.in_stride;
   double* out = a.out[f] + (long long)mem * a.out_stride;
   const long long stride = DIR == 0 ? 1 : a.stride;
@@ -234,6 +234,17 @@ __global__ __launch_bounds__(256) void line_solve_kernel(SolveArgs a) {
 }
 
 template <int DIR, int NR, int M>
+__global__ __launch_bounds__(256) void line_solve_kernel(SolveArgs a) {
+  line_solve_lg_body<DIR, NR, M>(a, grid_pos());
+}
+template <int DIR, int NR, int M>
+__global__ __launch_bounds__(256) void line_solve_ragged(const SolveArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const int m = ragged_pos(map, bp);
+  line_solve_lg_body<DIR, NR, M>(args[m], bp);
+}
+
+template <int DIR, int NR, int M>
 static void launch_nr_m(const SolveArgs& a0, const LineShape& s, int nmembers, hipStream_t st) {
   SolveArgs a = a0;
   a.TL = (64 / s.G) * s.waves_per_block;
@@ -242,6 +253,8 @@ static void launch_nr_m(const SolveArgs& a0, const LineShape& s, int nmembers, h
   const long long tiles = (a.nlines + a.TL - 1) / a.TL;
   const size_t lds_bytes = (size_t)M * a.lds_row * sizeof(double);
   dim3 grid((unsigned)tiles, (unsigned)(a.nf * nmembers));
+  if (record_launch((const void*)line_solve_ragged<DIR, NR, M>, a, grid, dim3(64 * s.waves_per_block), lds_bytes))
+    return;
   hipLaunchKernelGGL((line_solve_kernel<DIR, NR, M>), grid, dim3(64 * s.waves_per_block), lds_bytes, st, a);
 }
 

@@ -144,7 +144,7 @@ __device__ __forceinline__ void ws_solve(const SolveArgs& a, double* sh, double*
 }
 
 template <int DIR, int NR, int M, bool XLDS, bool BLEND>
-__global__ __launch_bounds__(M >= 32 ? 512 : 1024) void line_solve_ws_kernel(SolveArgs a) {
+__device__ __forceinline__ void line_solve_ws_body(const SolveArgs& a, BlockPos bp) {
   extern __shared__ double sh[];
   constexpr int H = NR - 1;
   constexpr int CS = ws_coef_stride(NR);
@@ -153,12 +153,12 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024) void line_solve_ws_kernel(Sol
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int G = blockDim.x >> 6;
   const int N = a.op.N;
-  const int line0 = blockIdx.x * 64;
+  const int line0 = bp.x * 64;
   const int line = line0 + lane;
   const int nact = min(64, a.nlines - line0);
   const bool active = lane < nact;
-  const int f = (int)blockIdx.y % a.nf;
-  const int mem = (int)blockIdx.y / a.nf;
+  const int f = (int)bp.y % a.nf;
+  const int mem = (int)bp.y / a.nf;
   const double* in = a.in[f] + (long long)mem * a.in_stride;
   double* out = a.out[f] + (long long)mem * a.out_stride;
   const long long stride = a.stride;
@@ -225,6 +225,18 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024) void line_solve_ws_kernel(Sol
   }
 }
 
+template <int DIR, int NR, int M, bool XLDS, bool BLEND>
+__global__ __launch_bounds__(M >= 32 ? 512 : 1024) void line_solve_ws_kernel(SolveArgs a) {
+  line_solve_ws_body<DIR, NR, M, XLDS, BLEND>(a, grid_pos());
+}
+template <int DIR, int NR, int M, bool XLDS, bool BLEND>
+__global__ __launch_bounds__(M >= 32 ? 512 : 1024) void line_solve_ws_ragged(const SolveArgs* __restrict__ args,
+                                                                              RaggedMap map) {
+  BlockPos bp;
+  const int m = ragged_pos(map, bp);
+  line_solve_ws_body<DIR, NR, M, XLDS, BLEND>(args[m], bp);
+}
+
 // LDS words: scan maps (5*G*64), then per wavefront M*CS + 2 coefficient words.
 // Along x the tile (65*N words) is dead once the windows are in registers, so
 // maps and coefficients reuse its words.
@@ -250,6 +262,8 @@ static void launch_ws_m(const SolveArgs& a0, const LineShape& s, int nmembers, h
   dim3 grid((unsigned)tiles, (unsigned)(a.nf * nmembers));
   const bool blend = a.mode == 1 && a.theta > 0.0;
   constexpr bool X = DIR == 0;
+  auto kr = blend ? line_solve_ws_ragged<DIR, NR, M, X, true> : line_solve_ws_ragged<DIR, NR, M, X, false>;
+  if (record_launch((const void*)kr, a, grid, dim3(64 * G), lds)) return;
   auto k = blend ? line_solve_ws_kernel<DIR, NR, M, X, true> : line_solve_ws_kernel<DIR, NR, M, X, false>;
   if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(k, grid, dim3(64 * G), lds, st, a);

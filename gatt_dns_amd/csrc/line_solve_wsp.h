@@ -215,7 +215,7 @@ template <int NR, int M, bool BLEND, bool FLUX = false>
 constexpr int wsp_min_waves() { return M >= 32 ? 4 : (FLUX ? 4 : ((NR <= 4 && !BLEND) ? 8 : 1)); }
 
 template <int DIR, int NR, int M, bool BLEND, bool FLUX = false>
-__global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND, FLUX>())) void line_solve_wsp_kernel(SolveArgs a) {
+__device__ __forceinline__ void line_solve_wsp_body(const SolveArgs& a, BlockPos bp) {
   extern __shared__ double sh[];
   constexpr int H = NR - 1;
   constexpr int CS = wsp_coef_stride(NR);
@@ -236,10 +236,10 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND, 
   double* shB = sh + 3 * G * 64;                 // backward maps [2][G][64]
   double* cl = sh + 5 * G * 64 + w * SLOT;       // this wavefront's coefficients
   const long long stride = a.stride;
-  const int line0 = blockIdx.x * 64;
+  const int line0 = bp.x * 64;
   const int nact = min(64, a.nlines - line0);
-  const int f = (int)blockIdx.y % a.nf;
-  const int mem = (int)blockIdx.y / a.nf;
+  const int f = (int)bp.y % a.nf;
+  const int mem = (int)bp.y / a.nf;
   const double* in = a.in[f] + (long long)mem * a.in_stride;
   double* out = a.out[f] + (long long)mem * a.out_stride;
   const int line = lane < nact ? line0 + lane : line0;
@@ -329,6 +329,18 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND, 
   }
 }
 
+template <int DIR, int NR, int M, bool BLEND, bool FLUX = false>
+__global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND, FLUX>())) void line_solve_wsp_kernel(SolveArgs a) {
+  line_solve_wsp_body<DIR, NR, M, BLEND, FLUX>(a, grid_pos());
+}
+template <int DIR, int NR, int M, bool BLEND>
+__global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND, false>())) void line_solve_wsp_ragged(
+    const SolveArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const int m = ragged_pos(map, bp);
+  line_solve_wsp_body<DIR, NR, M, BLEND, false>(args[m], bp);
+}
+
 // Variant 3, x lines through an LDS tile: a workgroup owns LW (16 or 32) lines,
 // one contiguous stretch of LW*N doubles moved between HBM and the tile (N rows
 // of LW words) with unit-stride accesses.  Lane l of wavefront w works on line
@@ -337,7 +349,7 @@ __global__ __launch_bounds__(M >= 32 ? 512 : 1024, (wsp_min_waves<NR, M, BLEND, 
 // result writes.  At N = 256 a 32-line tile takes 64 KiB (two workgroups per
 // compute unit), a 16-line tile 32 KiB (four).
 template <int NR, int M, bool BLEND, int LW, bool FLUX = false>
-__global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
+__device__ __forceinline__ void line_solve_wspx_body(const SolveArgs& a, BlockPos bp) {
   extern __shared__ double sh[];
   constexpr int H = NR - 1;
   constexpr int CS = wsp_coef_stride(NR);
@@ -364,10 +376,10 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
   double* shP = sh + GA * LW;                  // forward maps  [2][GA][LW]
   double* shB = sh + 3 * GA * LW;              // backward maps [2][GA][LW]
   double* cl = sh + 5 * GA * LW + sg * SLOT;   // this segment's coefficients
-  const int line0 = blockIdx.x * LW;
+  const int line0 = bp.x * LW;
   const int nact = min(LW, a.nlines - line0);
-  const int f = (int)blockIdx.y % a.nf;
-  const int mem = (int)blockIdx.y / a.nf;
+  const int f = (int)bp.y % a.nf;
+  const int mem = (int)bp.y / a.nf;
   const double* in = a.in[f] + (long long)mem * a.in_stride + (long long)a.nx * line0;
   double* out = a.out[f] + (long long)mem * a.out_stride + (long long)a.nx * line0;
 
@@ -465,6 +477,17 @@ __global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
   }
 }
 
+template <int NR, int M, bool BLEND, int LW, bool FLUX = false>
+__global__ __launch_bounds__(512) void line_solve_wspx_kernel(SolveArgs a) {
+  line_solve_wspx_body<NR, M, BLEND, LW, FLUX>(a, grid_pos());
+}
+template <int NR, int M, bool BLEND, int LW>
+__global__ __launch_bounds__(512) void line_solve_wspx_ragged(const SolveArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const int m = ragged_pos(map, bp);
+  line_solve_wspx_body<NR, M, BLEND, LW, false>(args[m], bp);
+}
+
 // LDS words: scan maps (5*G*64), then one coefficient slot per wavefront.
 inline size_t wsp_lds_bytes(int G, int M, int nR) {
   return sizeof(double) * ((size_t)5 * G * 64 + (size_t)G * (M * wsp_coef_stride(nR) + 2));
@@ -483,8 +506,13 @@ static void launch_wsp_k(const SolveArgs& a, int G, hipStream_t st) {
   static_assert(M % 2 == 0, "pairs of points per lane");
   auto k = line_solve_wsp_kernel<DIR, NR, M, BLEND, FLUX>;
   const size_t lds = wsp_lds_bytes(G, M, NR);
-  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   dim3 grid((unsigned)((a.nlines + 63) / 64), (unsigned)(a.nf * a.nmem));
+  if constexpr (FLUX) {
+    if (record_unsupported()) return;
+  } else {
+    if (record_launch((const void*)line_solve_wsp_ragged<DIR, NR, M, BLEND>, a, grid, dim3(64 * G), lds)) return;
+  }
+  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(k, grid, dim3(64 * G), lds, st, a);
 }
 
@@ -492,10 +520,17 @@ template <int NR, int M, bool BLEND, int LW, bool FLUX = false>
 static void launch_wspx_k(const SolveArgs& a, hipStream_t st) {
   auto k = line_solve_wspx_kernel<NR, M, BLEND, LW, FLUX>;
   const size_t lds = wspx_lds_bytes(a.op.N, M, NR, LW);
-  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   constexpr int SPW = 64 / LW;
   const int G = (a.op.N + M - 1) / M;
   dim3 grid((unsigned)((a.nlines + LW - 1) / LW), (unsigned)(a.nf * a.nmem));
+  if constexpr (FLUX) {
+    if (record_unsupported()) return;
+  } else {
+    if (record_launch((const void*)line_solve_wspx_ragged<NR, M, BLEND, LW>, a, grid, dim3(64 * ((G + SPW - 1) / SPW)),
+                      lds))
+      return;
+  }
+  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(k, grid, dim3(64 * ((G + SPW - 1) / SPW)), lds, st, a);
 }
 
@@ -509,7 +544,9 @@ static void launch_wsp_m(const SolveArgs& a0, int nmembers, hipStream_t st) {
   for (int f = 0; f < a.nf && vec; ++f)
     vec = ((uintptr_t)a.in[f] % 16 == 0) && ((uintptr_t)a.out[f] % 16 == 0);
   a.xvec = vec ? 1 : 0;
-  const int G = (a.op.N + M - 1) / M;
+  // segments (wavefronts) per line: enough for the line, or the shape's count when that is larger
+  // (the members of a ragged group share one workgroup size; segments past the line end are empty)
+  const int G = std::max((a.op.N + M - 1) / M, a.xtile ? 0 : a.G);
   const bool blend = a.mode == 1 && a.theta > 0.0;
   if constexpr (NR <= kFluxMaxNR) if (a.has_flux) {
     // derivatives of the viscous fluxes (mode 0; derivative bands only, the host checks nR)

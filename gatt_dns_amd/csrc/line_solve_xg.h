@@ -28,16 +28,16 @@ __device__ __forceinline__ double xg_shfl_down(double v, int d) { return __shfl_
 __device__ __forceinline__ double xg_shfl_xor(double v, int m) { return __shfl_xor(v, m, 64); }
 
 template <int NR, int M, bool BLEND>
-__global__ __launch_bounds__(256) void line_solve_xg_kernel(SolveArgs a) {
+__device__ __forceinline__ void line_solve_xg_body(const SolveArgs& a, BlockPos bp) {
   extern __shared__ double shx[];
   constexpr int H = NR - 1;
   constexpr int NW = 2 * H + 1;
   constexpr int W = M + 2 * H;
   const int lane = threadIdx.x & 63;
-  const int line = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  const int line = (int)bp.x * 4 + (int)(threadIdx.x >> 6);
   if (line >= a.nlines) return;   // whole wavefronts
-  const int f = (int)blockIdx.y % a.nf;
-  const int mem = (int)blockIdx.y / a.nf;
+  const int f = (int)bp.y % a.nf;
+  const int mem = (int)bp.y / a.nf;
   const int N = a.op.N;
   const int per = a.op.periodic;
   const double* src = a.in[f] + (long long)mem * a.in_stride + (long long)a.nx * line;
@@ -160,11 +160,26 @@ __global__ __launch_bounds__(256) void line_solve_xg_kernel(SolveArgs a) {
   for (int e = lane; e < N; e += 64) dst[e] = buf[e];
 }
 
+template <int NR, int M, bool BLEND>
+__global__ __launch_bounds__(256) void line_solve_xg_kernel(SolveArgs a) {
+  line_solve_xg_body<NR, M, BLEND>(a, grid_pos());
+}
+template <int NR, int M, bool BLEND>
+__global__ __launch_bounds__(256) void line_solve_xg_ragged(const SolveArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const int m = ragged_pos(map, bp);
+  line_solve_xg_body<NR, M, BLEND>(args[m], bp);
+}
+
 template <int NR, int M>
 static void launch_xg_m(const SolveArgs& a, int nmembers, hipStream_t st) {
   dim3 grid((unsigned)((a.nlines + 3) / 4), (unsigned)(a.nf * nmembers));
   const size_t lds = sizeof(double) * 4 * (size_t)a.op.N;
-  if (a.mode == 1 && a.theta > 0.0)
+  const bool blend = a.mode == 1 && a.theta > 0.0;
+  if (record_launch(blend ? (const void*)line_solve_xg_ragged<NR, M, true> : (const void*)line_solve_xg_ragged<NR, M, false>,
+                    a, grid, dim3(256), lds))
+    return;
+  if (blend)
     hipLaunchKernelGGL((line_solve_xg_kernel<NR, M, true>), grid, dim3(256), lds, st, a);
   else
     hipLaunchKernelGGL((line_solve_xg_kernel<NR, M, false>), grid, dim3(256), lds, st, a);

@@ -13,11 +13,11 @@ static inline unsigned grid_for(long long n, int threads) {
 }
 
 // nseq.F90:218-239 (3-D) and 72-85 (2-D): Sutherland viscosity, stresses, heat flux.
-__global__ __launch_bounds__(256) void visc_kernel(ViscArgs a) {
-  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void visc_body(const ViscArgs& a, BlockPos bp) {
+  const long long n = bp.x * (long long)blockDim.x + threadIdx.x;
   if (n >= a.np) return;
-  const long long mo = (long long)blockIdx.y * a.member_stride;
-  const long long wo = (long long)blockIdx.y * a.work_stride;
+  const long long mo = (long long)bp.y * a.member_stride;
+  const long long wo = (long long)bp.y * a.work_stride;
   const long long np = a.np;
   const double* D = a.D + wo;
   double* T = a.T + wo;
@@ -54,25 +54,34 @@ __global__ __launch_bounds__(256) void visc_kernel(ViscArgs a) {
   }
 }
 
+__global__ __launch_bounds__(256) void visc_kernel(ViscArgs a) { visc_body(a, grid_pos()); }
+__global__ __launch_bounds__(256) void visc_ragged(const ViscArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const int m = ragged_pos(map, bp);
+  visc_body(args[m], bp);
+}
+
 void launch_visc(const ViscArgs& a, int nmembers, hipStream_t st) {
-  hipLaunchKernelGGL(visc_kernel, dim3(grid_for(a.np, 256), nmembers), dim3(256), 0, st, a);
+  const dim3 grid(grid_for(a.np, 256), nmembers);
+  if (record_launch((const void*)visc_ragged, a, grid, dim3(256), 0)) return;
+  hipLaunchKernelGGL(visc_kernel, grid, dim3(256), 0, st, a);
 }
 
 // nseq.F90:280-318 (3-D), 106-134 (2-D), the forcings and one stage update (rhs_point.h),
 // one thread per point; the flux derivatives come from the G arrays.
-__global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs a) {
-  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void assemble_body(const AssembleArgs& a, BlockPos bp) {
+  const long long n = bp.x * (long long)blockDim.x + threadIdx.x;
   if (n >= a.np) return;
-  const long long mo = (long long)blockIdx.y * a.member_stride;
+  const long long mo = (long long)bp.y * a.member_stride;
   const long long np = a.np;
-  const long long wo = (long long)blockIdx.y * a.work_stride;
+  const long long wo = (long long)bp.y * a.work_stride;
   const double* G = a.G + wo;
   double g[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) g[k] = (a.is3d || k == 0 || k == 1 || k == 3 || k == 4 || k == 5 || k == 7)
                                           ? G[k * np + n] : 0.0;
   double d[5], sv[5], ov[5];
-  rhs_point(a, a.S + mo, a.D + wo, n, g, (int)blockIdx.y, d, sv);
+  rhs_point(a, a.S + mo, a.D + wo, n, g, (int)bp.y, d, sv);
   stage_point(a, mo, n, d, sv, ov);
   if (a.sum_partial) {
     // turbulenceGenerator.F90:39-44 on the stage state just written: wavefront trees, then
@@ -90,24 +99,33 @@ __global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs a) {
     __syncthreads();
     if (threadIdx.x < 5) {
       const int k = threadIdx.x;
-      a.sum_partial[((long long)blockIdx.y * gridDim.x + blockIdx.x) * 5 + k] =
+      a.sum_partial[((long long)bp.y * bp.gx + bp.x) * 5 + k] =
           ((wsum[0][k] + wsum[1][k]) + wsum[2][k]) + wsum[3][k];
     }
   }
 }
 
+__global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs a) { assemble_body(a, grid_pos()); }
+__global__ __launch_bounds__(256) void assemble_ragged(const AssembleArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const int m = ragged_pos(map, bp);
+  assemble_body(args[m], bp);
+}
+
 void launch_assemble(const AssembleArgs& a, int nmembers, hipStream_t st) {
-  hipLaunchKernelGGL(assemble_kernel, dim3(grid_for(a.np, 256), nmembers), dim3(256), 0, st, a);
+  const dim3 grid(grid_for(a.np, 256), nmembers);
+  if (record_launch((const void*)assemble_ragged, a, grid, dim3(256), 0)) return;
+  hipLaunchKernelGGL(assemble_kernel, grid, dim3(256), 0, st, a);
 }
 
 // Global sums of turbulenceGenerator.F90:39-44, deterministic two-stage reduction.
-__global__ __launch_bounds__(256) void forcing_partial_kernel(SumArgs a) {
+__device__ __forceinline__ void forcing_partial_body(const SumArgs& a, BlockPos bp) {
   __shared__ double sh[6][256];
-  const long long mo = (long long)blockIdx.y * a.member_stride;
+  const long long mo = (long long)bp.y * a.member_stride;
   const long long np = a.np;
   const double* S = a.S + mo;
   double acc[5] = {0, 0, 0, 0, 0};
-  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < np; n += (long long)gridDim.x * blockDim.x) {
+  for (long long n = bp.x * (long long)blockDim.x + threadIdx.x; n < np; n += (long long)bp.gx * blockDim.x) {
     const double U = S[n], V = S[np + n], W = S[2 * np + n], E = S[4 * np + n];
     acc[0] += U * U + V * V + W * W;
     acc[1] += U;
@@ -122,13 +140,19 @@ __global__ __launch_bounds__(256) void forcing_partial_kernel(SumArgs a) {
       for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x < 5) a.partial[((long long)blockIdx.y * a.nblocks + blockIdx.x) * 5 + threadIdx.x] = sh[threadIdx.x][0];
+  if (threadIdx.x < 5) a.partial[((long long)bp.y * a.nblocks + bp.x) * 5 + threadIdx.x] = sh[threadIdx.x][0];
+}
+__global__ __launch_bounds__(256) void forcing_partial_kernel(SumArgs a) { forcing_partial_body(a, grid_pos()); }
+__global__ __launch_bounds__(256) void forcing_partial_ragged(const SumArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const int m = ragged_pos(map, bp);
+  forcing_partial_body(args[m], bp);
 }
 
 // Fixed-order tree over the block partials: thread k sums partials k, k+256, ...
-__global__ __launch_bounds__(256) void forcing_final_kernel(SumArgs a) {
+__device__ __forceinline__ void forcing_final_body(const SumArgs& a, BlockPos bp) {
   __shared__ double sh[5][256];
-  const int mem = blockIdx.x;
+  const int mem = bp.x;
   double acc[5] = {0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x)
     for (int k = 0; k < 5; ++k) acc[k] += a.partial[((long long)mem * a.nblocks + b) * 5 + k];
@@ -143,13 +167,23 @@ __global__ __launch_bounds__(256) void forcing_final_kernel(SumArgs a) {
   if (threadIdx.x == 5) a.sums[8 * mem + 5] = (double)a.np;
 }
 
-void launch_forcing_sums(const SumArgs& a, int nmembers, hipStream_t st) {
-  hipLaunchKernelGGL(forcing_partial_kernel, dim3(a.nblocks, nmembers), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(forcing_final_kernel, dim3(nmembers), dim3(256), 0, st, a);
+__global__ __launch_bounds__(256) void forcing_final_kernel(SumArgs a) { forcing_final_body(a, grid_pos()); }
+__global__ __launch_bounds__(256) void forcing_final_ragged(const SumArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const int m = ragged_pos(map, bp);
+  forcing_final_body(args[m], bp);
 }
 
 void launch_forcing_final(const SumArgs& a, int nmembers, hipStream_t st) {
+  if (record_launch((const void*)forcing_final_ragged, a, dim3(nmembers), dim3(256), 0)) return;
   hipLaunchKernelGGL(forcing_final_kernel, dim3(nmembers), dim3(256), 0, st, a);
+}
+
+void launch_forcing_sums(const SumArgs& a, int nmembers, hipStream_t st) {
+  const dim3 grid(a.nblocks, nmembers);
+  if (!record_launch((const void*)forcing_partial_ragged, a, grid, dim3(256), 0))
+    hipLaunchKernelGGL(forcing_partial_kernel, grid, dim3(256), 0, st, a);
+  launch_forcing_final(a, nmembers, st);
 }
 
 // the tracked points of the log line (main.F90:478-486): one thread per (point, field)
@@ -170,10 +204,10 @@ void launch_gather_points(const double* state, long long np, int is3d, int npts,
 }
 
 // main.F90:382-401
-__global__ __launch_bounds__(256) void sfd_kernel(SfdArgs a) {
-  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void sfd_body(const SfdArgs& a, BlockPos bp) {
+  const long long n = bp.x * (long long)blockDim.x + threadIdx.x;
   if (n >= a.np) return;
-  const long long mo = (long long)blockIdx.y * a.member_stride;
+  const long long mo = (long long)bp.y * a.member_stride;
   const double X = a.chi ? a.chi[n] : a.chi_val;
   const double dtX = a.dt * X;
   const double dtD = a.delta > 0 ? a.dt / a.delta : 0.0;
@@ -188,17 +222,34 @@ __global__ __launch_bounds__(256) void sfd_kernel(SfdArgs a) {
   }
 }
 
+__global__ __launch_bounds__(256) void sfd_kernel(SfdArgs a) { sfd_body(a, grid_pos()); }
+__global__ __launch_bounds__(256) void sfd_ragged(const SfdArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const int m = ragged_pos(map, bp);
+  sfd_body(args[m], bp);
+}
+
 void launch_sfd(const SfdArgs& a, int nmembers, hipStream_t st) {
-  hipLaunchKernelGGL(sfd_kernel, dim3(grid_for(a.np, 256), nmembers), dim3(256), 0, st, a);
+  const dim3 grid(grid_for(a.np, 256), nmembers);
+  if (record_launch((const void*)sfd_ragged, a, grid, dim3(256), 0)) return;
+  hipLaunchKernelGGL(sfd_kernel, grid, dim3(256), 0, st, a);
 }
 
 __global__ void clear_u64_kernel(unsigned long long* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
 }
+__global__ void clear_u64_ragged(const ClearArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const ClearArgs& a = args[ragged_pos(map, bp)];
+  const int i = bp.x * blockDim.x + threadIdx.x;
+  if (i < a.n) a.p[i] = 0ull;
+}
 
 void launch_clear_u64(unsigned long long* p, int n, hipStream_t st) {
-  hipLaunchKernelGGL(clear_u64_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  const dim3 grid((n + 255) / 256);
+  if (record_launch((const void*)clear_u64_ragged, ClearArgs{p, n}, grid, dim3(256), 0)) return;
+  hipLaunchKernelGGL(clear_u64_kernel, grid, dim3(256), 0, st, p, n);
 }
 
 __global__ void clear_i32_kernel(int* p, int n) {
@@ -213,12 +264,12 @@ void launch_clear_i32(int* p, int n, hipStream_t st) {
 // max |q_new - q| per field (main.F90:414-418) and max |U|,|V|,|W| of the new
 // state (main.F90:197-203 of the next step).  Non-negative doubles order like
 // their bit patterns, and a NaN pattern orders above +inf, so NaN propagates.
-__global__ __launch_bounds__(256) void step_reduce_kernel(ReduceArgs a) {
+__device__ __forceinline__ void step_reduce_body(const ReduceArgs& a, BlockPos bp) {
   __shared__ unsigned long long sh[8][256];
-  const long long mo = (long long)blockIdx.y * a.member_stride;
+  const long long mo = (long long)bp.y * a.member_stride;
   const long long np = a.np;
   unsigned long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < np; n += (long long)gridDim.x * blockDim.x) {
+  for (long long n = bp.x * (long long)blockDim.x + threadIdx.x; n < np; n += (long long)bp.gx * blockDim.x) {
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
       if (!a.is3d && f == 2) continue;
@@ -243,29 +294,47 @@ __global__ __launch_bounds__(256) void step_reduce_kernel(ReduceArgs a) {
       }
     __syncthreads();
   }
-  if (threadIdx.x < 8) atomicMax(a.out + 8 * blockIdx.y + threadIdx.x, sh[threadIdx.x][0]);
+  if (threadIdx.x < 8) atomicMax(a.out + 8 * bp.y + threadIdx.x, sh[threadIdx.x][0]);
+}
+__global__ __launch_bounds__(256) void step_reduce_kernel(ReduceArgs a) { step_reduce_body(a, grid_pos()); }
+__global__ __launch_bounds__(256) void step_reduce_ragged(const ReduceArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const int m = ragged_pos(map, bp);
+  step_reduce_body(args[m], bp);
 }
 
 void launch_step_reduce(const ReduceArgs& a, int nmembers, hipStream_t st) {
   long long blocks = (a.np + 255) / 256;
   if (blocks > 2048) blocks = 2048;
-  hipLaunchKernelGGL(step_reduce_kernel, dim3((unsigned)blocks, nmembers), dim3(256), 0, st, a);
+  const dim3 grid((unsigned)blocks, nmembers);
+  if (record_launch((const void*)step_reduce_ragged, a, grid, dim3(256), 0)) return;
+  hipLaunchKernelGGL(step_reduce_kernel, grid, dim3(256), 0, st, a);
 }
 
-__global__ __launch_bounds__(256) void copy_kernel(double* dst, const double* src, long long n) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+__device__ __forceinline__ void copy_body(double* dst, const double* src, long long n, BlockPos bp) {
+  for (long long i = bp.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)bp.gx * blockDim.x)
     dst[i] = src[i];
+}
+__global__ __launch_bounds__(256) void copy_kernel(double* dst, const double* src, long long n) {
+  copy_body(dst, src, n, grid_pos());
+}
+__global__ __launch_bounds__(256) void copy_ragged(const CopyArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const CopyArgs& a = args[ragged_pos(map, bp)];
+  copy_body(a.dst, a.src, a.n, bp);
 }
 
 void launch_copy(double* dst, const double* src, long long n, hipStream_t st) {
   long long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, st, dst, src, n);
+  const dim3 grid((unsigned)blocks);
+  if (record_launch((const void*)copy_ragged, CopyArgs{dst, src, n}, grid, dim3(256), 0)) return;
+  hipLaunchKernelGGL(copy_kernel, grid, dim3(256), 0, st, dst, src, n);
 }
 
 // Append member 0's max changes to the step record and raise the sticky
 // divergence flag when a change is NaN or all changes are zero (main.F90:434).
-__global__ void record_kernel(const unsigned long long* red, double* rec, int* flag, int nmembers) {
+__device__ __forceinline__ void record_body(const unsigned long long* red, double* rec, int* flag, int nmembers) {
   if (threadIdx.x != 0) return;
   int div = 0;
   for (int mbr = 0; mbr < nmembers; ++mbr) {
@@ -282,7 +351,17 @@ __global__ void record_kernel(const unsigned long long* red, double* rec, int* f
   if (div) *flag = 1;
 }
 
+__global__ void record_kernel(const unsigned long long* red, double* rec, int* flag, int nmembers) {
+  record_body(red, rec, flag, nmembers);
+}
+__global__ void record_ragged(const RecordArgs* __restrict__ args, RaggedMap map) {
+  BlockPos bp;
+  const RecordArgs& a = args[ragged_pos(map, bp)];
+  record_body(a.red, a.rec, a.flag, a.nmembers);
+}
+
 void launch_record(const unsigned long long* red, double* rec, int* flag, int nmembers, hipStream_t st) {
+  if (record_launch((const void*)record_ragged, RecordArgs{red, rec, flag, nmembers}, dim3(1), dim3(64), 0)) return;
   hipLaunchKernelGGL(record_kernel, dim3(1), dim3(64), 0, st, red, rec, flag, nmembers);
 }
 

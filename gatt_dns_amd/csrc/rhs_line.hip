@@ -222,6 +222,7 @@ bool rhs_line_fits(int N, int nR) {
 int rhs_line_blocks(int nlines) { return (nlines + kRhsLines - 1) / kRhsLines; }
 
 void launch_rhs_line(const RhsLineArgs& r, int nmembers, hipStream_t st) {
+  if (record_unsupported()) return;
   if (r.s.dir == 2) launch_dir<2>(r, nmembers, st);
   else launch_dir<1>(r, nmembers, st);
 }

@@ -374,7 +374,8 @@ class Group:
     """Ragged group of solvers (different cases) advanced together (gatt_group_*).
 
     mode 0 replays one HIP graph per double step with a branch per member;
-    mode 1 replays per-member graphs round-robin on the members' streams.
+    mode 1 replays per-member graphs round-robin on the members' streams;
+    mode 2 replays one graph of ragged launches, each serving every member.
     """
 
     def __init__(self, solvers, mode=0):

@@ -416,16 +416,22 @@ int gatt_adj_step(gatt_ctx* ctx, double t, double dt);
 /* ------------------------------------------------------------------------
  * Ragged groups: independent problems of different grid sizes and
  * parameters (a parameter sweep, runBatch.m / one DNS process per case in the
- * reference) advanced together.  Each member is its own context; a group
- * replays one captured HIP graph per double step in which every member's step
- * is a parallel branch, so the small members' launches overlap on the GPU.
+ * reference) advanced together.  Each member is its own context.
  * ------------------------------------------------------------------------ */
 typedef struct gatt_group gatt_group;
 
-/* Group n distinct contexts of one device.  mode 0: one graph with a branch
- * per member; mode 1: per-member graphs launched round-robin on the members'
- * own streams.  The contexts stay owned by the caller and must outlive the
- * group. */
+/* Group n distinct contexts of one device.  mode 0: one captured graph per
+ * double step in which every member's step is a parallel branch; mode 1:
+ * per-member graphs launched round-robin on the members' own streams; mode 2:
+ * one captured graph of ragged launches, each kernel launch of the step
+ * serving every member (a per-member table of argument blocks and the first
+ * block of each member, walked inside the kernels).  Mode 2 needs members whose
+ * steps make the same launches (same dimensionality, scheme bands, integrator,
+ * SFD and forcing settings; boundary programs may differ); it gives the
+ * members one common launch shape per direction and operator for the life of
+ * the group and restores their own on gatt_group_destroy, and
+ * gatt_group_advance returns GATT_E_UNSUPPORTED when the steps differ.  The
+ * contexts stay owned by the caller and must outlive the group. */
 int gatt_group_create(gatt_ctx* const* ctxs, int32_t n, int32_t mode, gatt_group** out);
 int gatt_group_destroy(gatt_group* group);
 

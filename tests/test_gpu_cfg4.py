@@ -7,8 +7,9 @@
 * At 256^3, 32 members perturbed about the turbulence box and advanced two steps together
   equal the same perturbations advanced alone, bit for bit.
 * The 16 lid-driven cavities of the sweep (256 x 128 ... 768 x 256, Re 500 ... 2000) in one
-  group: three of them against the oracle after four steps, each field on its own scale
-  (U, R, E within 1e-11, V within 1e-10).
+  group of ragged launches (every member in each launch): three of them against the oracle
+  after four steps, each field on its own scale
+  (U, R, E within 1e-11, V within 5e-10).
 """
 import math
 
@@ -99,7 +100,7 @@ def test_cavity_sweep_group_vs_oracle(torch_cuda, oracle_mod):
         s.set_state(c.state0)
         s.apply_bound_cond(0.0)
         solvers.append(s)
-    g = Group(solvers, mode=0)
+    g = Group(solvers, mode=2)
     nsteps = 4
     mc, dv = g.advance(nsteps, 0.0, [c.dt for c in sweep])
     assert not any(dv)
@@ -110,12 +111,14 @@ def test_cavity_sweep_group_vs_oracle(torch_cuda, oracle_mod):
         q0 = o.apply_bound_cond(c.state0, 0.0)
         want, mc_w, _ = o.run(q0, 0.0, c.dt, nsteps)
         got = solvers[m].get_state()
-        # V is four orders below the lid's U after four steps, so its own-scale bar allows
-        # for the round-off of the O(1) terms it is computed from
+        # V is four orders below the lid's U after four steps (max |V| 1.6e-4), so its own-scale
+        # bar allows for the round-off of the O(1) terms it is computed from: the y solves of the
+        # ragged launch run as segment scans, whose rounding differs from the sequential sweep by
+        # about 2e-14 absolute here, 1.5e-10 of max |V|
         errs = {k: rel_err(got[k], want[k]) for k in "UVRE"}
         print("cavity sweep member", m, errs, flush=True)
         for k, e in errs.items():
-            assert e <= (1e-10 if k == "V" else 1e-11), (m, k, e)
+            assert e <= (5e-10 if k == "V" else 1e-11), (m, k, e)
         np.testing.assert_allclose(mc[m], mc_w, rtol=1e-8)
     g.close()
     for s in solvers:
