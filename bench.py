@@ -46,7 +46,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n=64, steps=6):
+def cpu_baseline(n=64, steps=18):
     from gatt_dns_amd.prep import cases
     from oracle.oracle import Oracle
     case = cases.turbulence_box(n)
@@ -167,16 +167,22 @@ def ragged_sweep(runtime, steps=20):
                                            / res["sequential"], 3)}
 
 
-def adjoint_cost(runtime, nx=512, ny=192, nz=64, steps=6, only_adjoint=False):
+def adjoint_cost(runtime, nx=512, ny=192, nz=64, steps=6, only_adjoint=False, base_steps=10000, long_steps=2000,
+                 every=100):
     """Tangent and adjoint step cost against the plain step on the adjoint configuration: the
     spanwise-periodic boundary layer 512x192x64 (the 2-D boundary layer extruded in z; walls, buffers,
-    SFD with a frozen mean in the buffers), Blasius base with seeded noise."""
+    SFD with a frozen mean in the buffers) about a steady base (an SFD run of the 2-D boundary layer,
+    replicated in z); then the wall time of `long_steps` forward steps and of the gradient of
+    E = 1/2 ||q'(T)||^2 over `long_steps` linearised steps with checkpoints every `every` steps."""
     import torch
-    from gatt_dns_amd.adjoint import Propagator, state_tensor
+    from gatt_dns_amd.adjoint import Propagator, optimal_energy_gradient, state_tensor
+    from gatt_dns_amd.baseflow import extrude, sfd_base
     from gatt_dns_amd.prep import cases
-    case = cases.boundary_layer(nx=nx, ny=ny, nz=nz, sfd_delta=-1.0)
+    # (the profiled form times steps only: a short base run)
+    _, q2, base_info = sfd_base(nx, ny, steps=500 if only_adjoint else base_steps)
+    case = cases.boundary_layer(nx=nx, ny=ny, nz=nz, noise=0.0, sfd_delta=-1.0)
     prop = Propagator(case)
-    q0 = state_tensor(case, case.state0)
+    q0 = state_tensor(case, extrude(q2, nz))
     v = torch.randn(prop.shape, dtype=torch.float64, device="cuda")
     res = {}
     prop.load(q0)
@@ -224,13 +230,34 @@ def adjoint_cost(runtime, nx=512, ny=192, nz=64, steps=6, only_adjoint=False):
         return {"workload": "spanwise-periodic boundary layer %dx%dx%d adjoint steps" % (nx, ny, nz),
                 "steps": steps, "adjoint_ms_per_step": round(ms, 3)}
     measure("")
+    # the long run: forward steps, then the checkpointed gradient of the perturbation energy
+    prop.load(q0)
+    prop.solver.sync()
+    t0 = time.perf_counter()
+    prop.steps(long_steps)
+    prop.solver.sync()
+    long_fwd = time.perf_counter() - t0
+    dq0 = v / torch.linalg.norm(v)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    energy, grad = optimal_energy_gradient(prop, q0, dq0, long_steps, checkpoint_every=every)
+    torch.cuda.synchronize()
+    long_grad = time.perf_counter() - t0
+    del grad
+    torch.cuda.empty_cache()
     # the same with the line-solve shapes picked by the autotuner for this case
     from gatt_dns_amd import autotune
     autotune.tune(prop.solver, reps=3)
     measure("_tuned")
     prop.close()
     return {"workload": "spanwise-periodic boundary layer %dx%dx%d, Re 1000, Ma 0.5, walls, buffers, "
-                        "SFD (frozen mean) in the buffers" % (nx, ny, nz),
+                        "SFD (frozen mean) in the buffers, about a steady base (2-D SFD run of %d steps, "
+                        "chi 0.1, Delta 5, last max change %.2e, replicated in z)"
+                        % (nx, ny, nz, base_info["steps"], max(base_info["max_change"])),
+            "long_run": {"steps": long_steps, "checkpoint_every": every, "forward_s": round(long_fwd, 3),
+                         "gradient_s": round(long_grad, 3), "gradient_over_forward": round(long_grad / long_fwd, 3),
+                         "note": "gradient = tangent run keeping checkpoints + per segment recompute of the base "
+                                 "states and the adjoint steps in reverse", "energy": energy},
             "forward_ms_per_step": round(res["forward"] * 1e3, 3),
             "tangent_ms_per_step": round(res["tangent"] * 1e3, 3),
             "adjoint_ms_per_step": round(res["adjoint"] * 1e3, 3),

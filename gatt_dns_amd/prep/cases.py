@@ -357,7 +357,7 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
                    Ma=0.5, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0.49, spatial="SL4", cfl=0.5, sfd_chi=0.02,
                    sfd_delta=10.0, noise=1e-5, seed=20261, buffer_growth=8.0, nz=1, z_length=4.0,
                    adiabatic=False, z_symmetry=False, pressure_inlet=False, free_slip=False,
-                   initial="compressible"):
+                   initial="compressible", sfd_type=2):
     """Isothermal-wall boundary layer (boundaryLayerIsothermal.m; ``adiabatic=True``: the walls of
     boundaryLayerAdiabatic.m; ``free_slip=True``: the slip plate of boundaryLayerFreeSlip.m, with
     zero-gradient u and w on the wall and no x break point) on a stretched mesh with buffers.
@@ -370,7 +370,8 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
     buffers at both ends, y geometric from the wall with ratio ``wall_ratio`` up to ``y_top`` and an
     nbuf-point buffer above; buffer spacings grow along a sigmoid to ``buffer_growth`` times the edge
     spacing.  Numerics as parameters.m: SL4 with EX4 in the buffers, 'reducedOrder' filter borders with
-    the x-start, x-end and y-end closures off, SFD type 2 (chi in the x buffers only, applyY = false).
+    the x-start, x-end and y-end closures off, SFD type 2 (chi in the x buffers only, applyY = false;
+    ``sfd_type=1``: chi everywhere, main.F90:390, as a steady-base computation uses it).
     Initial flow (``initial``): 'compressible' (the default) is calcCompressibleBL.m's similarity
     solution for the isothermal (or, with ``adiabatic``, the adiabatic) wall, uniform upstream of
     the plate; 'blasius' is the incompressible Blasius velocity scaled so that the displacement
@@ -423,8 +424,8 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
         dist.append(Disturbance("holdInlet", box, fortran_round(X[:1]), fortran_round(Y), fortran_round(Z), [0.0]))
     case = Case(name, nx, ny, nz, X, Y, Z, rnd(Re), rnd(Ma), rnd(Pr), rnd(gamma),
                 rnd(T0), deriv, filt, types, (1, 1, 1), -1.0, bi, dist, [],
-                sfd_type=2, sfd_delta=rnd(sfd_delta), sfd_x_val=rnd(sfd_chi),
-                sfd_x=np.asfortranarray(fortran_round(chi)), dxmin=_dxmin(X, Y, Z, nz))
+                sfd_type=sfd_type, sfd_delta=rnd(sfd_delta), sfd_x_val=rnd(sfd_chi),
+                sfd_x=np.asfortranarray(fortran_round(chi)) if sfd_type == 2 else None, dxmin=_dxmin(X, Y, Z, nz))
     case.flow_region = fr
     case.buffers = ((nbuf, nbuf), (0, nbuf), (0, 0))
     shape = (nx, ny, nz)
