@@ -228,21 +228,26 @@ int dev_upload(gatt_ctx* c, const T* host, size_t count, T** out) {
 }
 
 // Default shape before tuning.  Many lines of at most 256 points: the fused
-// wave-segment kernel (along x through the 32-line tile); otherwise the
+// wave-segment kernel (along x through the 16-line tile); otherwise the
 // lane-group kernel with 16 points per lane where the line allows it.
 LineShape choose_shape(int dir, int N, int nR, long long nlines, long long max_lds, bool uniform_tiles) {
   if (nlines >= 1024 && uniform_tiles) {
-    // x: the 32-line LDS tile (unit-stride loads and stores) with 16 points per lane;
+    // x: the 16-line LDS tile (unit-stride loads and stores; more workgroups per compute
+    // unit than the 32-line tile) with 16 points per lane;
     // y and z: 32 points per lane for wide bands (fewer, fuller wavefronts), else 16
     const bool wide = dir == 0 || nR >= 5;
     for (int v : {3, 2, 1}) {
       if (v == 3 && dir != 0) continue;
       for (int M : {(wide && v != 3) ? 32 : 16, (wide && v != 3) ? 16 : 32}) {
-        LineShape s;
-        s.variant = v;
-        s.M = M;
-        s.G = (N + M - 1) / M;
-        if (line_shape_ok(s, N, nR, dir, max_lds, uniform_tiles)) return s;
+        for (int wpb : {2, 1}) {
+          if (v != 3 && wpb == 2) continue;
+          LineShape s;
+          s.variant = v;
+          s.M = M;
+          s.G = (N + M - 1) / M;
+          if (v == 3) s.waves_per_block = wpb;
+          if (line_shape_ok(s, N, nR, dir, max_lds, uniform_tiles)) return s;
+        }
       }
     }
   }

@@ -77,6 +77,28 @@ __device__ __forceinline__ double visc_flux(int q, const FlowConst& c, double E,
   return rmu * (2.0 * own - dv);
 }
 
+// Moves n values with a block-strided loop in batches of XB per thread: every load of a
+// batch is issued before the first store needs its value, so each thread keeps XB loads
+// in flight instead of one (the tile loads of the x-line kernels are latency-bound
+// otherwise).  ld(i) returns value i, st(i, v) stores it.
+template <int XB, typename Load, typename Store>
+__device__ __forceinline__ void batched_move(int n, Load ld, Store st) {
+  const int bs = blockDim.x;
+  for (int b0 = threadIdx.x; b0 < n; b0 += XB * bs) {
+    double v[XB];
+#pragma unroll
+    for (int k = 0; k < XB; ++k) {
+      const int i = b0 + k * bs;
+      if (i < n) v[k] = ld(i);
+    }
+#pragma unroll
+    for (int k = 0; k < XB; ++k) {
+      const int i = b0 + k * bs;
+      if (i < n) st(i, v[k]);
+    }
+  }
+}
+
 struct SolveArgs {
   LineOpDev op;
   const int* types;           // 0-based type per line (a + n1*b)

@@ -67,17 +67,25 @@ __device__ __forceinline__ void line_solve_lg_body(const SolveArgs& a, BlockPos 
   const int total = ntile * N;
   if (DIR == 0) {
     const double* src = in + line_base<0>(a, line0);   // the tile is contiguous
-    for (int idx = tid; idx < total; idx += blockDim.x) {
+    batched_move<8>(total, [&](int idx) { return src[idx]; }, [&](int idx, double v) {
       const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
       const int ep = idx - lp * N;
-      lds[(ep & (M - 1)) * ROW + lp * G + ((unsigned)ep / M)] = src[idx];
-    }
+      lds[(ep & (M - 1)) * ROW + lp * G + ((unsigned)ep / M)] = v;
+    });
   } else {
-    for (int idx = tid; idx < TL * N; idx += blockDim.x) {
-      const int ep = (unsigned)idx / (unsigned)TL;   // TL is a power of two
-      const int lp = idx - ep * TL;
-      if (lp < ntile) lds[(ep & (M - 1)) * ROW + lp * G + ((unsigned)ep / M)] = in[line_base<DIR>(a, line0 + lp) + ep * stride];
-    }
+    // TL is a power of two
+    batched_move<8>(
+        TL * N,
+        [&](int idx) {
+          const int ep = (unsigned)idx / (unsigned)TL;
+          const int lp = idx - ep * TL;
+          return lp < ntile ? in[line_base<DIR>(a, line0 + lp) + ep * stride] : 0.0;
+        },
+        [&](int idx, double v) {
+          const int ep = (unsigned)idx / (unsigned)TL;
+          const int lp = idx - ep * TL;
+          if (lp < ntile) lds[(ep & (M - 1)) * ROW + lp * G + ((unsigned)ep / M)] = v;
+        });
   }
   __syncthreads();
 
@@ -219,17 +227,27 @@ __device__ __forceinline__ void line_solve_lg_body(const SolveArgs& a, BlockPos 
   __syncthreads();
   if (DIR == 0) {
     double* dst = out + line_base<0>(a, line0);
-    for (int idx = tid; idx < total; idx += blockDim.x) {
-      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
-      const int ep = idx - lp * N;
-      dst[idx] = lds[(ep & (M - 1)) * ROW + lp * G + ((unsigned)ep / M)];
-    }
+    batched_move<8>(
+        total,
+        [&](int idx) {
+          const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+          const int ep = idx - lp * N;
+          return lds[(ep & (M - 1)) * ROW + lp * G + ((unsigned)ep / M)];
+        },
+        [&](int idx, double v) { dst[idx] = v; });
   } else {
-    for (int idx = tid; idx < TL * N; idx += blockDim.x) {
-      const int ep = (unsigned)idx / (unsigned)TL;
-      const int lp = idx - ep * TL;
-      if (lp < ntile) out[line_base<DIR>(a, line0 + lp) + ep * stride] = lds[(ep & (M - 1)) * ROW + lp * G + ((unsigned)ep / M)];
-    }
+    batched_move<8>(
+        TL * N,
+        [&](int idx) {
+          const int ep = (unsigned)idx / (unsigned)TL;
+          const int lp = idx - ep * TL;
+          return lp < ntile ? lds[(ep & (M - 1)) * ROW + lp * G + ((unsigned)ep / M)] : 0.0;
+        },
+        [&](int idx, double v) {
+          const int ep = (unsigned)idx / (unsigned)TL;
+          const int lp = idx - ep * TL;
+          if (lp < ntile) out[line_base<DIR>(a, line0 + lp) + ep * stride] = v;
+        });
   }
 }
 

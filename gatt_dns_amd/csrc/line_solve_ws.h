@@ -180,11 +180,11 @@ __device__ __forceinline__ void line_solve_ws_body(const SolveArgs& a, BlockPos 
   if (XLDS) {
     const double* src = in + (long long)a.nx * line0;
     const int total = nact * N;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    batched_move<8>(total, [&](int idx) { return src[idx]; }, [&](int idx, double v) {
       const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
       const int ep = idx - lp * N;
-      tile[ep * 65 + lp] = src[idx];
-    }
+      tile[ep * 65 + lp] = v;
+    });
     __syncthreads();
   }
 
@@ -217,11 +217,14 @@ __device__ __forceinline__ void line_solve_ws_body(const SolveArgs& a, BlockPos 
     __syncthreads();
     double* dst = out + (long long)a.nx * line0;
     const int total = nact * N;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
-      const int ep = idx - lp * N;
-      dst[idx] = tile[ep * 65 + lp];
-    }
+    batched_move<8>(
+        total,
+        [&](int idx) {
+          const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+          const int ep = idx - lp * N;
+          return tile[ep * 65 + lp];
+        },
+        [&](int idx, double v) { dst[idx] = v; });
   }
 }
 

@@ -410,21 +410,37 @@ __device__ __forceinline__ void line_solve_wspx_body(const SolveArgs& a, BlockPo
       tile[tix(ep, lp)] = wsp_flux_at(a, q, mem, p0 + idx);
     }
   } else if (pairs) {
+    // batches of XB pairs per thread: every load of a batch is issued before the first
+    // LDS store waits for its data
+    constexpr int XB = 8;
     const double2* in2 = reinterpret_cast<const double2*>(in);
-    for (int i2 = threadIdx.x; i2 < total / 2; i2 += blockDim.x) {
-      const double2 v = in2[i2];
-      const int idx = 2 * i2;
-      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
-      const int ep = idx - lp * N;
-      tile[tix(ep, lp)] = v.x;
-      tile[tix(ep + 1, lp)] = v.y;
+    const int npair = total / 2;
+    const int bs = blockDim.x;
+    for (int b0 = threadIdx.x; b0 < npair; b0 += XB * bs) {
+      double2 v[XB];
+#pragma unroll
+      for (int k = 0; k < XB; ++k) {
+        const int i2 = b0 + k * bs;
+        if (i2 < npair) v[k] = in2[i2];
+      }
+#pragma unroll
+      for (int k = 0; k < XB; ++k) {
+        const int i2 = b0 + k * bs;
+        if (i2 < npair) {
+          const int idx = 2 * i2;
+          const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+          const int ep = idx - lp * N;
+          tile[tix(ep, lp)] = v[k].x;
+          tile[tix(ep + 1, lp)] = v[k].y;
+        }
+      }
     }
   } else {
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    batched_move<8>(total, [&](int idx) { return in[idx]; }, [&](int idx, double v) {
       const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
       const int ep = idx - lp * N;
-      tile[tix(ep, lp)] = in[idx];
-    }
+      tile[tix(ep, lp)] = v;
+    });
   }
   __syncthreads();
   double fw[W];
@@ -461,19 +477,37 @@ __device__ __forceinline__ void line_solve_wspx_body(const SolveArgs& a, BlockPo
     if (tt < cnt) tile[tix(s + tt, l)] = r[tt];
   __syncthreads();
   if (pairs) {
+    constexpr int XB = 8;
     double2* out2 = reinterpret_cast<double2*>(out);
-    for (int i2 = threadIdx.x; i2 < total / 2; i2 += blockDim.x) {
-      const int idx = 2 * i2;
-      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
-      const int ep = idx - lp * N;
-      out2[i2] = make_double2(tile[tix(ep, lp)], tile[tix(ep + 1, lp)]);
+    const int npair = total / 2;
+    const int bs = blockDim.x;
+    for (int b0 = threadIdx.x; b0 < npair; b0 += XB * bs) {
+      double2 v[XB];
+#pragma unroll
+      for (int k = 0; k < XB; ++k) {
+        const int i2 = b0 + k * bs;
+        if (i2 < npair) {
+          const int idx = 2 * i2;
+          const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+          const int ep = idx - lp * N;
+          v[k] = make_double2(tile[tix(ep, lp)], tile[tix(ep + 1, lp)]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < XB; ++k) {
+        const int i2 = b0 + k * bs;
+        if (i2 < npair) out2[i2] = v[k];
+      }
     }
   } else {
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
-      const int ep = idx - lp * N;
-      out[idx] = tile[tix(ep, lp)];
-    }
+    batched_move<8>(
+        total,
+        [&](int idx) {
+          const int lp = (int)(((float)idx + 0.5f) * a.inv_n);
+          const int ep = idx - lp * N;
+          return tile[tix(ep, lp)];
+        },
+        [&](int idx, double v) { out[idx] = v; });
   }
 }
 

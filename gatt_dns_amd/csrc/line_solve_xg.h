@@ -52,7 +52,20 @@ __device__ __forceinline__ void line_solve_xg_body(const SolveArgs& a, BlockPos 
   // the line goes through this wavefront's LDS buffer: unit-stride loads and stores
   // in HBM, each lane's window of M + 2H points read from LDS
   double* buf = shx + (size_t)(threadIdx.x >> 6) * N;
-  for (int e = lane; e < N; e += 64) buf[e] = src[e];
+  {
+    // all M loads of the lane in flight before the LDS stores (N <= 64 M)
+    double v[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int e = lane + 64 * k;
+      if (e < N) v[k] = src[e];
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int e = lane + 64 * k;
+      if (e < N) buf[e] = v[k];
+    }
+  }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -251,7 +264,20 @@ __global__ __launch_bounds__(256) void line_solve_xgt_kernel(BandTArgs a) {
   // the line in through this wavefront's LDS buffer (unit-stride loads)
   double* buf = sh + (size_t)wv * N;
   double* obuf = sh + (size_t)(4 + wv) * N;
-  for (int e = lane; e < N; e += 64) buf[e] = in[e];
+  {
+    // all M loads of the lane in flight before the LDS stores (N <= 64 M)
+    double v[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int e = lane + 64 * k;
+      if (e < N) v[k] = in[e];
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int e = lane + 64 * k;
+      if (e < N) buf[e] = v[k];
+    }
+  }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -102,7 +102,9 @@ __device__ __forceinline__ void rhs_point(const AssembleArgs& a, const double* S
 // One stage update at point n of member offset mo (state arrays Q, Out, Delta of AssembleArgs):
 // stage 0 stores the right-hand side, 1-4 are the RK4 stages (timeIntegration.F90:393-487),
 // 5 the Euler update (553-557) and 6-8 the SSPRK3 stages (623-627, 647-651, 672-676).  The
-// first RK4 stage updates the state it differentiates (Q == S): its values come from sv.
+// first RK4 stage updates the state it differentiates (Q == S): its values come from sv.  The
+// third leaves q + Delta in the accumulator, which the fourth reads in place of q and Delta
+// (the sum (q + Delta) + dt/6 k4 of timeIntegration.F90:482-487 is unchanged).
 // ov receives the values written to Out.
 __device__ __forceinline__ void stage_point(const AssembleArgs& a, long long mo, long long n, const double* d,
                                             const double* sv, double* ov) {
@@ -126,8 +128,15 @@ __device__ __forceinline__ void stage_point(const AssembleArgs& a, long long mo,
         case 0: v = d[f]; break;
         case 1: Dl[o] = dt / 6.0 * d[f]; v = Q[o] + dt / 2.0 * d[f]; break;
         case 2: Dl[o] = Dl[o] + dt / 3.0 * d[f]; v = Q[o] + dt / 2.0 * d[f]; break;
-        case 3: Dl[o] = Dl[o] + dt / 3.0 * d[f]; v = Q[o] + dt * d[f]; break;
-        case 4: v = Q[o] + Dl[o] + dt / 6.0 * d[f]; break;
+        case 3: {
+          // the last stage's q + Delta, stored in the accumulator: stage 4 reads one array
+          // where it would read two, and adds the same two values it would add
+          const double q = Q[o];
+          Dl[o] = q + (Dl[o] + dt / 3.0 * d[f]);
+          v = q + dt * d[f];
+          break;
+        }
+        case 4: v = Dl[o] + dt / 6.0 * d[f]; break;
         case 5: v = Q[o] + dt * d[f]; break;
         case 6: Dl[o] = d[f]; v = Q[o] + dt * d[f]; break;
         case 7: {

@@ -64,16 +64,18 @@ def test_line_solves_256_bench_shapes(rt, oracle_mod):
         expect = {(0, 0): (3, 16), (0, 1): (3, 16), (1, 0): (2, 16), (1, 1): (2, 32), (2, 0): (2, 16), (2, 1): (2, 32)}
         rng = np.random.default_rng(256)
         for (d, kind), (v, m) in expect.items():
-            got_v, _, got_m, _ = s.get_line_shape(d, kind)
+            got_v, _, got_m, got_w = s.get_line_shape(d, kind)
             assert (got_v, got_m) == (v, m), (d, kind, got_v, got_m)
+            if d == 0:
+                assert got_w == 2, got_w   # the 16-line tile
             F = np.asfortranarray(rng.standard_normal((256, 256, 256)))
             print("256^3 line solve dir", d, "kind", kind, flush=True)
             solve_and_compare(rt, s, o, case, d, kind, F)
-            # the 16-line tile along x as well
+            # the 32-line tile along x as well
             if d == 0:
-                s.set_line_shape(0, kind, 3, 16, 2)
-                solve_and_compare(rt, s, o, case, d, kind, F)
                 s.set_line_shape(0, kind, 3, 16, 1)
+                solve_and_compare(rt, s, o, case, d, kind, F)
+                s.set_line_shape(0, kind, 3, 16, 2)
     finally:
         s.close()
 
