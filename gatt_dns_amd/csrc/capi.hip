@@ -228,8 +228,10 @@ int dev_upload(gatt_ctx* c, const T* host, size_t count, T** out) {
 }
 
 // Default shape before tuning.  Many lines of at most 256 points: the fused
-// wave-segment kernel (along x through the 16-line tile); otherwise the
-// lane-group kernel with 16 points per lane where the line allows it.
+// wave-segment kernel (along x through the 16-line tile); long x lines: one wavefront
+// per line; otherwise the lane-group kernel with the most lanes per line (the
+// shortest stretch per lane, one wavefront per workgroup: the most workgroups), whose
+// band sums follow the reference's order term by term.
 LineShape choose_shape(int dir, int N, int nR, long long nlines, long long max_lds, bool uniform_tiles) {
   if (nlines >= 1024 && uniform_tiles) {
     // x: the 16-line LDS tile (unit-stride loads and stores; more workgroups per compute
@@ -263,8 +265,10 @@ LineShape choose_shape(int dir, int N, int nR, long long nlines, long long max_l
       if (line_shape_ok(s, N, nR, dir, max_lds, uniform_tiles)) return s;
     }
   }
-  for (int M : {16, 8, 4, 32}) {
-    for (int wpb : {4, 2, 1}) {
+  // the lane-group kernel with as many lanes per line as the line can use (the shortest
+  // sequential stretch per lane) and one wavefront per workgroup (more workgroups)
+  for (int M : {4, 8, 16, 32}) {
+    for (int wpb : {1, 2, 4}) {
       LineShape s;
       s.M = M;
       s.G = pow2_at_least((N + M - 1) / M);
