@@ -680,32 +680,47 @@ int compile_bc(gatt_ctx* c, const gatt_bc_desc& bc) {
     }
     if ((rc = add(op, st))) return rc;
   }
-  // group into waves of mutually independent consecutive ops
-  std::vector<BcWave> waves;
-  std::vector<Box> wr_acc, rd_acc;
-  int begin = 0;
-  long long blocks = 0;
-  for (int i = 0; i < (int)ops.size(); ++i) {
-    bool conflict = false;
-    for (auto& w : sets[i].wr) {
-      for (auto& x : wr_acc) conflict |= overlap(w, x);
-      for (auto& x : rd_acc) conflict |= overlap(w, x);
+  // Waves: each op goes one wave after the last earlier op it conflicts with (it reads what
+  // that op writes, or writes what that op reads or writes), so the ops of one wave are
+  // mutually independent, every pair of conflicting ops keeps its order, and independent
+  // ops of the reference's list share a wave even when a conflicting op stands between
+  // them.  The ops are then stored wave after wave (stable within a wave).
+  auto conflicts = [&](const OpSets& a, const OpSets& b) {
+    for (auto& w : a.wr) {
+      for (auto& x : b.wr)
+        if (overlap(w, x)) return true;
+      for (auto& x : b.rd)
+        if (overlap(w, x)) return true;
     }
-    for (auto& r : sets[i].rd)
-      for (auto& x : wr_acc) conflict |= overlap(r, x);
-    if (conflict && i > begin) {
-      waves.push_back({begin, i, blocks});
-      begin = i;
-      blocks = 0;
-      wr_acc.clear();
-      rd_acc.clear();
-    }
-    ops[i].first_block = blocks;
-    blocks += (ops[i].npts + 255) / 256;
-    wr_acc.insert(wr_acc.end(), sets[i].wr.begin(), sets[i].wr.end());
-    rd_acc.insert(rd_acc.end(), sets[i].rd.begin(), sets[i].rd.end());
+    for (auto& r : a.rd)
+      for (auto& x : b.wr)
+        if (overlap(r, x)) return true;
+    return false;
+  };
+  const int nops = (int)ops.size();
+  std::vector<int> level(nops, 0);
+  int nlev = 0;
+  for (int i = 0; i < nops; ++i) {
+    for (int j = 0; j < i; ++j)
+      if (level[j] + 1 > level[i] && conflicts(sets[i], sets[j])) level[i] = level[j] + 1;
+    nlev = std::max(nlev, level[i] + 1);
   }
-  if ((int)ops.size() > begin) waves.push_back({begin, (int)ops.size(), blocks});
+  std::vector<BcWave> waves;
+  std::vector<BcOp> sorted;
+  sorted.reserve(nops);
+  for (int l = 0; l < nlev; ++l) {
+    const int begin = (int)sorted.size();
+    long long blocks = 0;
+    for (int i = 0; i < nops; ++i) {
+      if (level[i] != l) continue;
+      BcOp op = ops[i];
+      op.first_block = blocks;
+      blocks += (op.npts + 255) / 256;
+      sorted.push_back(op);
+    }
+    waves.push_back({begin, (int)sorted.size(), blocks});
+  }
+  ops.swap(sorted);
   c->ops = ops;
   c->waves = waves;
   if (!ops.empty() && (rc = dev_upload(c, ops.data(), ops.size(), &c->d_ops))) return rc;
