@@ -77,6 +77,16 @@ __device__ __forceinline__ double visc_flux(int q, const FlowConst& c, double E,
   return rmu * (2.0 * own - dv);
 }
 
+// Streaming accesses of arrays a kernel reads or writes once per point: the non-temporal
+// hint lets the caches evict those lines first (the point-wise kernels of the step stream
+// 20 to 50 full arrays per launch).
+__device__ __forceinline__ double ld_nt(const double* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_nt(double* p, double v) { __builtin_nontemporal_store(v, p); }
+typedef double nt_double2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_nt2(double* p, double a, double b) {
+  __builtin_nontemporal_store(nt_double2{a, b}, reinterpret_cast<nt_double2*>(p));
+}
+
 // Moves n values with a block-strided loop in batches of XB per thread: every load of a
 // batch is issued before the first store needs its value, so each thread keeps XB loads
 // in flight instead of one (the tile loads of the x-line kernels are latency-bound

@@ -320,11 +320,11 @@ __device__ __forceinline__ void line_solve_wsp_body(const SolveArgs& a, BlockPos
     const long long base = wsp_base<DIR>(a, line) + (long long)s * stride;
     if (full) {
 #pragma unroll
-      for (int tt = 0; tt < M; ++tt) out[base + (long long)tt * stride] = r[tt];
+      for (int tt = 0; tt < M; ++tt) st_nt(out + base + (long long)tt * stride, r[tt]);
     } else if (lane < nact) {
 #pragma unroll
       for (int tt = 0; tt < M; ++tt)
-        if (tt < cnt) out[base + (long long)tt * stride] = r[tt];
+        if (tt < cnt) st_nt(out + base + (long long)tt * stride, r[tt]);
     }
   }
 }
@@ -478,7 +478,6 @@ __device__ __forceinline__ void line_solve_wspx_body(const SolveArgs& a, BlockPo
   __syncthreads();
   if (pairs) {
     constexpr int XB = 8;
-    double2* out2 = reinterpret_cast<double2*>(out);
     const int npair = total / 2;
     const int bs = blockDim.x;
     for (int b0 = threadIdx.x; b0 < npair; b0 += XB * bs) {
@@ -496,7 +495,7 @@ __device__ __forceinline__ void line_solve_wspx_body(const SolveArgs& a, BlockPo
 #pragma unroll
       for (int k = 0; k < XB; ++k) {
         const int i2 = b0 + k * bs;
-        if (i2 < npair) out2[i2] = v[k];
+        if (i2 < npair) st_nt2(out + 2 * i2, v[k].x, v[k].y);
       }
     }
   } else {

@@ -21,36 +21,36 @@ __device__ __forceinline__ void visc_body(const ViscArgs& a, BlockPos bp) {
   const long long np = a.np;
   const double* D = a.D + wo;
   double* T = a.T + wo;
-  const double E = a.E[mo + n];
+  const double E = ld_nt(a.E + mo + n);
   const FlowConst c = a.c;
   const double mu1 = c.muA / (c.muC * E + c.muB);
   const double mu2 = c.muC * E;
   const double mu = mu1 * mu2 * sqrt(mu2);
   const double rmu = c.invRe * mu;
-  const double DUx = D[0 * np + n], DUy = D[1 * np + n];
-  const double DVx = D[3 * np + n], DVy = D[4 * np + n];
-  const double DEx = D[12 * np + n], DEy = D[13 * np + n];
+  const double DUx = ld_nt(D + 0 * np + n), DUy = ld_nt(D + 1 * np + n);
+  const double DVx = ld_nt(D + 3 * np + n), DVy = ld_nt(D + 4 * np + n);
+  const double DEx = ld_nt(D + 12 * np + n), DEy = ld_nt(D + 13 * np + n);
   if (a.is3d) {
-    const double DUz = D[2 * np + n], DVz = D[5 * np + n];
-    const double DWx = D[6 * np + n], DWy = D[7 * np + n], DWz = D[8 * np + n];
-    const double DEz = D[14 * np + n];
+    const double DUz = ld_nt(D + 2 * np + n), DVz = ld_nt(D + 5 * np + n);
+    const double DWx = ld_nt(D + 6 * np + n), DWy = ld_nt(D + 7 * np + n), DWz = ld_nt(D + 8 * np + n);
+    const double DEz = ld_nt(D + 14 * np + n);
     const double tz = -(2.0 / 3.0 * (DUx + DVy + DWz));
-    T[0 * np + n] = rmu * (2.0 * DUx + tz);
-    T[1 * np + n] = rmu * (2.0 * DVy + tz);
-    T[2 * np + n] = rmu * (2.0 * DWz + tz);
-    T[3 * np + n] = rmu * (DUy + DVx);
-    T[4 * np + n] = rmu * (DUz + DWx);
-    T[5 * np + n] = rmu * (DVz + DWy);
-    T[6 * np + n] = -(c.qk * mu * DEx);
-    T[7 * np + n] = -(c.qk * mu * DEy);
-    T[8 * np + n] = -(c.qk * mu * DEz);
+    st_nt(T + 0 * np + n, rmu * (2.0 * DUx + tz));
+    st_nt(T + 1 * np + n, rmu * (2.0 * DVy + tz));
+    st_nt(T + 2 * np + n, rmu * (2.0 * DWz + tz));
+    st_nt(T + 3 * np + n, rmu * (DUy + DVx));
+    st_nt(T + 4 * np + n, rmu * (DUz + DWx));
+    st_nt(T + 5 * np + n, rmu * (DVz + DWy));
+    st_nt(T + 6 * np + n, -(c.qk * mu * DEx));
+    st_nt(T + 7 * np + n, -(c.qk * mu * DEy));
+    st_nt(T + 8 * np + n, -(c.qk * mu * DEz));
   } else {
     const double dv = 2.0 / 3.0 * (DUx + DVy);
-    T[0 * np + n] = rmu * (2.0 * DUx - dv);
-    T[1 * np + n] = rmu * (2.0 * DVy - dv);
-    T[3 * np + n] = rmu * (DUy + DVx);
-    T[6 * np + n] = -(c.qk * mu * DEx);
-    T[7 * np + n] = -(c.qk * mu * DEy);
+    st_nt(T + 0 * np + n, rmu * (2.0 * DUx - dv));
+    st_nt(T + 1 * np + n, rmu * (2.0 * DVy - dv));
+    st_nt(T + 3 * np + n, rmu * (DUy + DVx));
+    st_nt(T + 6 * np + n, -(c.qk * mu * DEx));
+    st_nt(T + 7 * np + n, -(c.qk * mu * DEy));
   }
 }
 
@@ -79,7 +79,7 @@ __device__ __forceinline__ void assemble_body(const AssembleArgs& a, BlockPos bp
   double g[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) g[k] = (a.is3d || k == 0 || k == 1 || k == 3 || k == 4 || k == 5 || k == 7)
-                                          ? G[k * np + n] : 0.0;
+                                          ? ld_nt(G + k * np + n) : 0.0;
   double d[5], sv[5], ov[5];
   rhs_point(a, a.S + mo, a.D + wo, n, g, (int)bp.y, d, sv);
   stage_point(a, mo, n, d, sv, ov);
@@ -273,7 +273,7 @@ __device__ __forceinline__ void step_reduce_body(const ReduceArgs& a, BlockPos b
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
       if (!a.is3d && f == 2) continue;
-      const double d = fabs(a.Qn[mo + f * np + n] - a.Q[mo + f * np + n]);
+      const double d = fabs(ld_nt(a.Qn + mo + f * np + n) - ld_nt(a.Q + mo + f * np + n));
       const unsigned long long b = (unsigned long long)__double_as_longlong(d);
       acc[f] = b > acc[f] ? b : acc[f];
     }

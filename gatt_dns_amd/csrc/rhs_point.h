@@ -17,12 +17,12 @@ namespace gatt {
 __device__ __forceinline__ void rhs_point(const AssembleArgs& a, const double* S, const double* D, long long n,
                                           const double* g, int member, double* d, double* sv) {
   const long long np = a.np;
-  const double U = S[0 * np + n], V = S[1 * np + n], R = S[3 * np + n], E = S[4 * np + n];
+  const double U = ld_nt(S + 0 * np + n), V = ld_nt(S + 1 * np + n), R = ld_nt(S + 3 * np + n), E = ld_nt(S + 4 * np + n);
   const FlowConst c = a.c;
-  const double DUx = D[0 * np + n], DUy = D[1 * np + n];
-  const double DVx = D[3 * np + n], DVy = D[4 * np + n];
-  const double DRx = D[9 * np + n], DRy = D[10 * np + n];
-  const double DEx = D[12 * np + n], DEy = D[13 * np + n];
+  const double DUx = ld_nt(D + 0 * np + n), DUy = ld_nt(D + 1 * np + n);
+  const double DVx = ld_nt(D + 3 * np + n), DVy = ld_nt(D + 4 * np + n);
+  const double DRx = ld_nt(D + 9 * np + n), DRy = ld_nt(D + 10 * np + n);
+  const double DEx = ld_nt(D + 12 * np + n), DEy = ld_nt(D + 13 * np + n);
   const double P = c.gm1 * R * E;
   const double DPx = c.gm1 * (R * DEx + DRx * E);
   const double DPy = c.gm1 * (R * DEy + DRy * E);
@@ -32,10 +32,10 @@ __device__ __forceinline__ void rhs_point(const AssembleArgs& a, const double* S
   const double rmu = c.invRe * mu;
   double W = 0.0;
   if (a.is3d) {
-    W = S[2 * np + n];
-    const double DUz = D[2 * np + n], DVz = D[5 * np + n];
-    const double DWx = D[6 * np + n], DWy = D[7 * np + n], DWz = D[8 * np + n];
-    const double DRz = D[11 * np + n], DEz = D[14 * np + n];
+    W = ld_nt(S + 2 * np + n);
+    const double DUz = ld_nt(D + 2 * np + n), DVz = ld_nt(D + 5 * np + n);
+    const double DWx = ld_nt(D + 6 * np + n), DWy = ld_nt(D + 7 * np + n), DWz = ld_nt(D + 8 * np + n);
+    const double DRz = ld_nt(D + 11 * np + n), DEz = ld_nt(D + 14 * np + n);
     const double DPz = c.gm1 * (R * DEz + DRz * E);
     const double tz = -(2.0 / 3.0 * (DUx + DVy + DWz));
     const double txx = rmu * (2.0 * DUx + tz), tyy = rmu * (2.0 * DVy + tz), tzz = rmu * (2.0 * DWz + tz);
@@ -86,7 +86,7 @@ __device__ __forceinline__ void rhs_point(const AssembleArgs& a, const double* S
     const int k = (int)(n / ((long long)a.nx * a.ny));
     if (i >= a.pg_box[0] && i <= a.pg_box[1] && j >= a.pg_box[2] && j <= a.pg_box[3] && k >= a.pg_box[4] &&
         k <= a.pg_box[5]) {
-      const double r = S[(long long)a.pg_r * np + n];
+      const double r = ld_nt(S + (long long)a.pg_r * np + n);
 #pragma unroll
       for (int f = 0; f < 5; ++f)
         if (f == a.pg_u) d[f] = d[f] + a.pg_dpdx / r;
@@ -121,34 +121,34 @@ __device__ __forceinline__ void stage_point(const AssembleArgs& a, long long mo,
     const long long o = f * np + n;
     double v;
     if (q_is_s && a.stage == 1) {
-      Dl[o] = dt / 6.0 * d[f];
+      st_nt(Dl + o, dt / 6.0 * d[f]);
       v = sv[f] + dt / 2.0 * d[f];
     } else {
       switch (a.stage) {
         case 0: v = d[f]; break;
-        case 1: Dl[o] = dt / 6.0 * d[f]; v = Q[o] + dt / 2.0 * d[f]; break;
-        case 2: Dl[o] = Dl[o] + dt / 3.0 * d[f]; v = Q[o] + dt / 2.0 * d[f]; break;
+        case 1: st_nt(Dl + o, dt / 6.0 * d[f]); v = ld_nt(Q + o) + dt / 2.0 * d[f]; break;
+        case 2: st_nt(Dl + o, ld_nt(Dl + o) + dt / 3.0 * d[f]); v = ld_nt(Q + o) + dt / 2.0 * d[f]; break;
         case 3: {
           // the last stage's q + Delta, stored in the accumulator: stage 4 reads one array
           // where it would read two, and adds the same two values it would add
-          const double q = Q[o];
-          Dl[o] = q + (Dl[o] + dt / 3.0 * d[f]);
+          const double q = ld_nt(Q + o);
+          st_nt(Dl + o, q + (ld_nt(Dl + o) + dt / 3.0 * d[f]));
           v = q + dt * d[f];
           break;
         }
-        case 4: v = Dl[o] + dt / 6.0 * d[f]; break;
-        case 5: v = Q[o] + dt * d[f]; break;
-        case 6: Dl[o] = d[f]; v = Q[o] + dt * d[f]; break;
+        case 4: v = ld_nt(Dl + o) + dt / 6.0 * d[f]; break;
+        case 5: v = ld_nt(Q + o) + dt * d[f]; break;
+        case 6: st_nt(Dl + o, d[f]); v = ld_nt(Q + o) + dt * d[f]; break;
         case 7: {
-          const double k = Dl[o] + d[f];
-          Dl[o] = k;
-          v = Q[o] + dt * k / 4.0;
+          const double k = ld_nt(Dl + o) + d[f];
+          st_nt(Dl + o, k);
+          v = ld_nt(Q + o) + dt * k / 4.0;
           break;
         }
-        default: v = Q[o] + dt * (Dl[o] + 4.0 * d[f]) / 6.0; break;
+        default: v = ld_nt(Q + o) + dt * (ld_nt(Dl + o) + 4.0 * d[f]) / 6.0; break;
       }
     }
-    O[o] = v;
+    st_nt(O + o, v);
     ov[f] = v;
   }
 }
