@@ -47,9 +47,9 @@ __global__ __launch_bounds__(256) void band_t_kernel(BandTArgs a, int m0) {
   }
   double* out = a.out[f] + (long long)mem * a.out_stride;
   double v = a.alpha * acc;
-  if (a.src[f]) v = a.beta * a.src[f][(long long)mem * a.out_stride + p] + v;
-  if (a.accumulate) v = out[p] + v;
-  out[p] = v;
+  if (a.src[f]) v = a.beta * ld_nt(a.src[f] + (long long)mem * a.out_stride + p) + v;
+  if (a.accumulate) v = ld_nt(out + p) + v;
+  st_nt(out + p, v);
 }
 
 // y and z: lane = line (64 consecutive lines, every point a 512-byte row) and
@@ -111,9 +111,9 @@ __global__ __launch_bounds__(1024) void band_t_seg_kernel(BandTArgs a) {
       }
       const long long p = base + (long long)e * stride;
       double v = a.alpha * acc;
-      if (src) v = a.beta * src[p] + v;
-      if (a.accumulate) v = out[p] + v;
-      out[p] = v;
+      if (src) v = a.beta * ld_nt(src + p) + v;
+      if (a.accumulate) v = ld_nt(out + p) + v;
+      st_nt(out + p, v);
     }
   }
 }
@@ -166,10 +166,10 @@ __global__ __launch_bounds__(256) void axpby_kernel(AxpbyArgs a) {
   if (p >= a.np) return;
   const int f = (int)blockIdx.y % a.nf;
   const int mem = (int)blockIdx.y / a.nf;
-  const double x = a.x[f][(long long)mem * a.x_stride + p];
+  const double x = ld_nt(a.x[f] + (long long)mem * a.x_stride + p);
   double* y = a.y[f] + (long long)mem * a.y_stride + p;
   // b == 0 overwrites y (and a == b == 0 clears it, whatever y held)
-  *y = a.b == 0.0 ? (a.a == 0.0 ? 0.0 : a.a * x) : a.a * x + a.b * *y;
+  st_nt(y, a.b == 0.0 ? (a.a == 0.0 ? 0.0 : a.a * x) : a.a * x + a.b * ld_nt(y));
 }
 
 void launch_axpby(const AxpbyArgs& a, int nmembers, hipStream_t st) {
@@ -181,16 +181,16 @@ __global__ __launch_bounds__(256) void adj_combine_kernel(AdjCombineArgs a) {
   if (p >= a.np) return;
   const int f = (int)blockIdx.y % a.nf;
   const long long o = (long long)((int)blockIdx.y / a.nf) * a.stride + p;
-  const double sb = a.sb[f][o];
+  const double sb = ld_nt(a.sb[f] + o);
   if (a.mode == 0) {
     double* acc = a.acc[f] + o;
-    *acc = a.first ? 1.0 * sb : 1.0 * sb + 1.0 * *acc;
-    const double t = a.a1 * a.qb[f][o];
-    a.k[f][o] = a.a2 * sb + 1.0 * t;
+    st_nt(acc, a.first ? 1.0 * sb : 1.0 * sb + 1.0 * ld_nt(acc));
+    const double t = a.a1 * ld_nt(a.qb[f] + o);
+    st_nt(a.k[f] + o, a.a2 * sb + 1.0 * t);
   } else {
     double* qb = a.qb[f] + o;
-    const double t = 1.0 * a.acc[f][o] + 1.0 * *qb;
-    *qb = 1.0 * sb + 1.0 * t;
+    const double t = 1.0 * ld_nt(a.acc[f] + o) + 1.0 * ld_nt(qb);
+    st_nt(qb, 1.0 * sb + 1.0 * t);
   }
 }
 
@@ -224,39 +224,39 @@ __global__ __launch_bounds__(256) void visc_tl_kernel(ViscLinArgs a) {
   const double* dD = a.dD + lo;
   double* dT = a.dT + lo;
   const FlowConst c = a.c;
-  const Mu m = sutherland(c, a.E[mo + n]);
-  const double dE = a.dE[lo + n];
+  const Mu m = sutherland(c, ld_nt(a.E + mo + n));
+  const double dE = ld_nt(a.dE + lo + n);
   const double rmu = c.invRe * m.mu;
   const double drmu = c.invRe * (m.dmu * dE);
   const double dmu = m.dmu * dE;
-  const double DUx = D[0 * np + n], DUy = D[1 * np + n], DVx = D[3 * np + n], DVy = D[4 * np + n];
-  const double DEx = D[12 * np + n], DEy = D[13 * np + n];
-  const double tDUx = dD[0 * np + n], tDUy = dD[1 * np + n], tDVx = dD[3 * np + n], tDVy = dD[4 * np + n];
-  const double tDEx = dD[12 * np + n], tDEy = dD[13 * np + n];
+  const double DUx = ld_nt(D + 0 * np + n), DUy = ld_nt(D + 1 * np + n), DVx = ld_nt(D + 3 * np + n), DVy = ld_nt(D + 4 * np + n);
+  const double DEx = ld_nt(D + 12 * np + n), DEy = ld_nt(D + 13 * np + n);
+  const double tDUx = ld_nt(dD + 0 * np + n), tDUy = ld_nt(dD + 1 * np + n), tDVx = ld_nt(dD + 3 * np + n), tDVy = ld_nt(dD + 4 * np + n);
+  const double tDEx = ld_nt(dD + 12 * np + n), tDEy = ld_nt(dD + 13 * np + n);
   if (a.is3d) {
-    const double DUz = D[2 * np + n], DVz = D[5 * np + n], DWx = D[6 * np + n], DWy = D[7 * np + n];
-    const double DWz = D[8 * np + n], DEz = D[14 * np + n];
-    const double tDUz = dD[2 * np + n], tDVz = dD[5 * np + n], tDWx = dD[6 * np + n], tDWy = dD[7 * np + n];
-    const double tDWz = dD[8 * np + n], tDEz = dD[14 * np + n];
+    const double DUz = ld_nt(D + 2 * np + n), DVz = ld_nt(D + 5 * np + n), DWx = ld_nt(D + 6 * np + n), DWy = ld_nt(D + 7 * np + n);
+    const double DWz = ld_nt(D + 8 * np + n), DEz = ld_nt(D + 14 * np + n);
+    const double tDUz = ld_nt(dD + 2 * np + n), tDVz = ld_nt(dD + 5 * np + n), tDWx = ld_nt(dD + 6 * np + n), tDWy = ld_nt(dD + 7 * np + n);
+    const double tDWz = ld_nt(dD + 8 * np + n), tDEz = ld_nt(dD + 14 * np + n);
     const double tz = -(2.0 / 3.0 * (DUx + DVy + DWz));
     const double dtz = -(2.0 / 3.0 * (tDUx + tDVy + tDWz));
-    dT[0 * np + n] = drmu * (2.0 * DUx + tz) + rmu * (2.0 * tDUx + dtz);
-    dT[1 * np + n] = drmu * (2.0 * DVy + tz) + rmu * (2.0 * tDVy + dtz);
-    dT[2 * np + n] = drmu * (2.0 * DWz + tz) + rmu * (2.0 * tDWz + dtz);
-    dT[3 * np + n] = drmu * (DUy + DVx) + rmu * (tDUy + tDVx);
-    dT[4 * np + n] = drmu * (DUz + DWx) + rmu * (tDUz + tDWx);
-    dT[5 * np + n] = drmu * (DVz + DWy) + rmu * (tDVz + tDWy);
-    dT[6 * np + n] = -(c.qk * (dmu * DEx + m.mu * tDEx));
-    dT[7 * np + n] = -(c.qk * (dmu * DEy + m.mu * tDEy));
-    dT[8 * np + n] = -(c.qk * (dmu * DEz + m.mu * tDEz));
+    st_nt(dT + 0 * np + n, drmu * (2.0 * DUx + tz) + rmu * (2.0 * tDUx + dtz));
+    st_nt(dT + 1 * np + n, drmu * (2.0 * DVy + tz) + rmu * (2.0 * tDVy + dtz));
+    st_nt(dT + 2 * np + n, drmu * (2.0 * DWz + tz) + rmu * (2.0 * tDWz + dtz));
+    st_nt(dT + 3 * np + n, drmu * (DUy + DVx) + rmu * (tDUy + tDVx));
+    st_nt(dT + 4 * np + n, drmu * (DUz + DWx) + rmu * (tDUz + tDWx));
+    st_nt(dT + 5 * np + n, drmu * (DVz + DWy) + rmu * (tDVz + tDWy));
+    st_nt(dT + 6 * np + n, -(c.qk * (dmu * DEx + m.mu * tDEx)));
+    st_nt(dT + 7 * np + n, -(c.qk * (dmu * DEy + m.mu * tDEy)));
+    st_nt(dT + 8 * np + n, -(c.qk * (dmu * DEz + m.mu * tDEz)));
   } else {
     const double dv = 2.0 / 3.0 * (DUx + DVy);
     const double ddv = 2.0 / 3.0 * (tDUx + tDVy);
-    dT[0 * np + n] = drmu * (2.0 * DUx - dv) + rmu * (2.0 * tDUx - ddv);
-    dT[1 * np + n] = drmu * (2.0 * DVy - dv) + rmu * (2.0 * tDVy - ddv);
-    dT[3 * np + n] = drmu * (DUy + DVx) + rmu * (tDUy + tDVx);
-    dT[6 * np + n] = -(c.qk * (dmu * DEx + m.mu * tDEx));
-    dT[7 * np + n] = -(c.qk * (dmu * DEy + m.mu * tDEy));
+    st_nt(dT + 0 * np + n, drmu * (2.0 * DUx - dv) + rmu * (2.0 * tDUx - ddv));
+    st_nt(dT + 1 * np + n, drmu * (2.0 * DVy - dv) + rmu * (2.0 * tDVy - ddv));
+    st_nt(dT + 3 * np + n, drmu * (DUy + DVx) + rmu * (tDUy + tDVx));
+    st_nt(dT + 6 * np + n, -(c.qk * (dmu * DEx + m.mu * tDEx)));
+    st_nt(dT + 7 * np + n, -(c.qk * (dmu * DEy + m.mu * tDEy)));
   }
 }
 
@@ -271,48 +271,48 @@ __global__ __launch_bounds__(256) void visc_adj_kernel(ViscLinArgs a) {
   const double* Tb = a.dT + lo;
   double* Db = a.Dbar + lo;
   const FlowConst c = a.c;
-  const Mu m = sutherland(c, a.E[mo + n]);
+  const Mu m = sutherland(c, ld_nt(a.E + mo + n));
   const double rmu = c.invRe * m.mu;
-  const double DUx = D[0 * np + n], DUy = D[1 * np + n], DVx = D[3 * np + n], DVy = D[4 * np + n];
-  const double DEx = D[12 * np + n], DEy = D[13 * np + n];
-  const double b0 = Tb[0 * np + n], b1 = Tb[1 * np + n], b3 = Tb[3 * np + n];
-  const double b6 = Tb[6 * np + n], b7 = Tb[7 * np + n];
+  const double DUx = ld_nt(D + 0 * np + n), DUy = ld_nt(D + 1 * np + n), DVx = ld_nt(D + 3 * np + n), DVy = ld_nt(D + 4 * np + n);
+  const double DEx = ld_nt(D + 12 * np + n), DEy = ld_nt(D + 13 * np + n);
+  const double b0 = ld_nt(Tb + 0 * np + n), b1 = ld_nt(Tb + 1 * np + n), b3 = ld_nt(Tb + 3 * np + n);
+  const double b6 = ld_nt(Tb + 6 * np + n), b7 = ld_nt(Tb + 7 * np + n);
   double rmub, mub;
   if (a.is3d) {
-    const double DUz = D[2 * np + n], DVz = D[5 * np + n], DWx = D[6 * np + n], DWy = D[7 * np + n];
-    const double DWz = D[8 * np + n], DEz = D[14 * np + n];
-    const double b2 = Tb[2 * np + n], b4 = Tb[4 * np + n], b5 = Tb[5 * np + n], b8 = Tb[8 * np + n];
+    const double DUz = ld_nt(D + 2 * np + n), DVz = ld_nt(D + 5 * np + n), DWx = ld_nt(D + 6 * np + n), DWy = ld_nt(D + 7 * np + n);
+    const double DWz = ld_nt(D + 8 * np + n), DEz = ld_nt(D + 14 * np + n);
+    const double b2 = ld_nt(Tb + 2 * np + n), b4 = ld_nt(Tb + 4 * np + n), b5 = ld_nt(Tb + 5 * np + n), b8 = ld_nt(Tb + 8 * np + n);
     const double tz = -(2.0 / 3.0 * (DUx + DVy + DWz));
     rmub = b0 * (2.0 * DUx + tz) + b1 * (2.0 * DVy + tz) + b2 * (2.0 * DWz + tz) + b3 * (DUy + DVx) +
            b4 * (DUz + DWx) + b5 * (DVz + DWy);
     const double tzb = rmu * (b0 + b1 + b2);
     const double divb = -(2.0 / 3.0) * tzb;
-    Db[0 * np + n] += 2.0 * rmu * b0 + divb;
-    Db[4 * np + n] += 2.0 * rmu * b1 + divb;
-    Db[8 * np + n] += 2.0 * rmu * b2 + divb;
-    Db[1 * np + n] += rmu * b3;
-    Db[3 * np + n] += rmu * b3;
-    Db[2 * np + n] += rmu * b4;
-    Db[6 * np + n] += rmu * b4;
-    Db[5 * np + n] += rmu * b5;
-    Db[7 * np + n] += rmu * b5;
-    Db[12 * np + n] += -(c.qk * m.mu * b6);
-    Db[13 * np + n] += -(c.qk * m.mu * b7);
-    Db[14 * np + n] += -(c.qk * m.mu * b8);
+    st_nt(Db + 0 * np + n, ld_nt(Db + 0 * np + n) + (2.0 * rmu * b0 + divb));
+    st_nt(Db + 4 * np + n, ld_nt(Db + 4 * np + n) + (2.0 * rmu * b1 + divb));
+    st_nt(Db + 8 * np + n, ld_nt(Db + 8 * np + n) + (2.0 * rmu * b2 + divb));
+    st_nt(Db + 1 * np + n, ld_nt(Db + 1 * np + n) + (rmu * b3));
+    st_nt(Db + 3 * np + n, ld_nt(Db + 3 * np + n) + (rmu * b3));
+    st_nt(Db + 2 * np + n, ld_nt(Db + 2 * np + n) + (rmu * b4));
+    st_nt(Db + 6 * np + n, ld_nt(Db + 6 * np + n) + (rmu * b4));
+    st_nt(Db + 5 * np + n, ld_nt(Db + 5 * np + n) + (rmu * b5));
+    st_nt(Db + 7 * np + n, ld_nt(Db + 7 * np + n) + (rmu * b5));
+    st_nt(Db + 12 * np + n, ld_nt(Db + 12 * np + n) + (-(c.qk * m.mu * b6)));
+    st_nt(Db + 13 * np + n, ld_nt(Db + 13 * np + n) + (-(c.qk * m.mu * b7)));
+    st_nt(Db + 14 * np + n, ld_nt(Db + 14 * np + n) + (-(c.qk * m.mu * b8)));
     mub = c.invRe * rmub - c.qk * (DEx * b6 + DEy * b7 + DEz * b8);
   } else {
     const double dv = 2.0 / 3.0 * (DUx + DVy);
     rmub = b0 * (2.0 * DUx - dv) + b1 * (2.0 * DVy - dv) + b3 * (DUy + DVx);
     const double divb = -(2.0 / 3.0) * (rmu * (b0 + b1));
-    Db[0 * np + n] += 2.0 * rmu * b0 + divb;
-    Db[4 * np + n] += 2.0 * rmu * b1 + divb;
-    Db[1 * np + n] += rmu * b3;
-    Db[3 * np + n] += rmu * b3;
-    Db[12 * np + n] += -(c.qk * m.mu * b6);
-    Db[13 * np + n] += -(c.qk * m.mu * b7);
+    st_nt(Db + 0 * np + n, ld_nt(Db + 0 * np + n) + (2.0 * rmu * b0 + divb));
+    st_nt(Db + 4 * np + n, ld_nt(Db + 4 * np + n) + (2.0 * rmu * b1 + divb));
+    st_nt(Db + 1 * np + n, ld_nt(Db + 1 * np + n) + (rmu * b3));
+    st_nt(Db + 3 * np + n, ld_nt(Db + 3 * np + n) + (rmu * b3));
+    st_nt(Db + 12 * np + n, ld_nt(Db + 12 * np + n) + (-(c.qk * m.mu * b6)));
+    st_nt(Db + 13 * np + n, ld_nt(Db + 13 * np + n) + (-(c.qk * m.mu * b7)));
     mub = c.invRe * rmub - c.qk * (DEx * b6 + DEy * b7);
   }
-  a.dEbar[lo + n] += m.dmu * mub;
+  st_nt(a.dEbar + lo + n, ld_nt(a.dEbar + lo + n) + (m.dmu * mub));
 }
 
 void launch_visc_tl(const ViscLinArgs& a, int nmembers, hipStream_t st) {
@@ -341,33 +341,33 @@ __device__ __forceinline__ Base load_base(const AssembleLinArgs& a, long long n)
   const double* G = a.G + wo;
   const double g1 = a.c.gm1;
   Base b;
-  b.U = S[0 * np + n]; b.V = S[1 * np + n]; b.R = S[3 * np + n]; b.E = S[4 * np + n];
-  b.DUx = D[0 * np + n]; b.DUy = D[1 * np + n]; b.DVx = D[3 * np + n]; b.DVy = D[4 * np + n];
-  b.DRx = D[9 * np + n]; b.DRy = D[10 * np + n]; b.DEx = D[12 * np + n]; b.DEy = D[13 * np + n];
-  b.txx = T[0 * np + n]; b.tyy = T[1 * np + n]; b.txy = T[3 * np + n];
+  b.U = ld_nt(S + 0 * np + n); b.V = ld_nt(S + 1 * np + n); b.R = ld_nt(S + 3 * np + n); b.E = ld_nt(S + 4 * np + n);
+  b.DUx = ld_nt(D + 0 * np + n); b.DUy = ld_nt(D + 1 * np + n); b.DVx = ld_nt(D + 3 * np + n); b.DVy = ld_nt(D + 4 * np + n);
+  b.DRx = ld_nt(D + 9 * np + n); b.DRy = ld_nt(D + 10 * np + n); b.DEx = ld_nt(D + 12 * np + n); b.DEy = ld_nt(D + 13 * np + n);
+  b.txx = ld_nt(T + 0 * np + n); b.tyy = ld_nt(T + 1 * np + n); b.txy = ld_nt(T + 3 * np + n);
   b.P = g1 * b.R * b.E;
   b.DPx = g1 * (b.R * b.DEx + b.DRx * b.E);
   b.DPy = g1 * (b.R * b.DEy + b.DRy * b.E);
   if (a.is3d) {
-    b.W = S[2 * np + n];
-    b.DUz = D[2 * np + n]; b.DVz = D[5 * np + n]; b.DWx = D[6 * np + n]; b.DWy = D[7 * np + n];
-    b.DWz = D[8 * np + n]; b.DRz = D[11 * np + n]; b.DEz = D[14 * np + n];
-    b.tzz = T[2 * np + n]; b.txz = T[4 * np + n]; b.tyz = T[5 * np + n];
+    b.W = ld_nt(S + 2 * np + n);
+    b.DUz = ld_nt(D + 2 * np + n); b.DVz = ld_nt(D + 5 * np + n); b.DWx = ld_nt(D + 6 * np + n); b.DWy = ld_nt(D + 7 * np + n);
+    b.DWz = ld_nt(D + 8 * np + n); b.DRz = ld_nt(D + 11 * np + n); b.DEz = ld_nt(D + 14 * np + n);
+    b.tzz = ld_nt(T + 2 * np + n); b.txz = ld_nt(T + 4 * np + n); b.tyz = ld_nt(T + 5 * np + n);
     b.DPz = g1 * (b.R * b.DEz + b.DRz * b.E);
-    b.vtx = G[0 * np + n] + G[4 * np + n] + G[8 * np + n];
-    b.vty = G[5 * np + n] + G[1 * np + n] + G[9 * np + n];
-    b.vtz = G[10 * np + n] + G[2 * np + n] + G[6 * np + n];
+    b.vtx = ld_nt(G + 0 * np + n) + ld_nt(G + 4 * np + n) + ld_nt(G + 8 * np + n);
+    b.vty = ld_nt(G + 5 * np + n) + ld_nt(G + 1 * np + n) + ld_nt(G + 9 * np + n);
+    b.vtz = ld_nt(G + 10 * np + n) + ld_nt(G + 2 * np + n) + ld_nt(G + 6 * np + n);
     b.div = b.DUx + b.DVy + b.DWz;
     b.Phi = -(b.P * b.div) + b.txx * b.DUx + b.tyy * b.DVy + b.tzz * b.DWz + b.txy * (b.DVx + b.DUy) +
-            b.txz * (b.DWx + b.DUz) + b.tyz * (b.DWy + b.DVz) - G[3 * np + n] - G[7 * np + n] - G[11 * np + n];
+            b.txz * (b.DWx + b.DUz) + b.tyz * (b.DWy + b.DVz) - ld_nt(G + 3 * np + n) - ld_nt(G + 7 * np + n) - ld_nt(G + 11 * np + n);
   } else {
     b.W = 0; b.DUz = b.DVz = b.DWx = b.DWy = b.DWz = b.DRz = b.DEz = 0;
     b.tzz = b.txz = b.tyz = 0; b.DPz = 0; b.vtz = 0;
-    b.vtx = G[0 * np + n] + G[4 * np + n];
-    b.vty = G[5 * np + n] + G[1 * np + n];
+    b.vtx = ld_nt(G + 0 * np + n) + ld_nt(G + 4 * np + n);
+    b.vty = ld_nt(G + 5 * np + n) + ld_nt(G + 1 * np + n);
     b.div = b.DUx + b.DVy;
-    b.Phi = -(b.P * b.div) + b.txx * b.DUx + b.tyy * b.DVy + b.txy * (b.DVx + b.DUy) - G[3 * np + n] -
-            G[7 * np + n];
+    b.Phi = -(b.P * b.div) + b.txx * b.DUx + b.tyy * b.DVy + b.txy * (b.DVx + b.DUy) - ld_nt(G + 3 * np + n) -
+            ld_nt(G + 7 * np + n);
   }
   return b;
 }
@@ -383,10 +383,10 @@ __global__ __launch_bounds__(256) void assemble_tl_kernel(AssembleLinArgs a) {
   const double* dD = a.dD + lo;
   const double* dT = a.dT + lo;
   const double* dG = a.dG + lo;
-  const double tU = dS[0 * np + n], tV = dS[1 * np + n], tR = dS[3 * np + n], tE = dS[4 * np + n];
-  const double tDUx = dD[0 * np + n], tDUy = dD[1 * np + n], tDVx = dD[3 * np + n], tDVy = dD[4 * np + n];
-  const double tDRx = dD[9 * np + n], tDRy = dD[10 * np + n], tDEx = dD[12 * np + n], tDEy = dD[13 * np + n];
-  const double ttxx = dT[0 * np + n], ttyy = dT[1 * np + n], ttxy = dT[3 * np + n];
+  const double tU = ld_nt(dS + 0 * np + n), tV = ld_nt(dS + 1 * np + n), tR = ld_nt(dS + 3 * np + n), tE = ld_nt(dS + 4 * np + n);
+  const double tDUx = ld_nt(dD + 0 * np + n), tDUy = ld_nt(dD + 1 * np + n), tDVx = ld_nt(dD + 3 * np + n), tDVy = ld_nt(dD + 4 * np + n);
+  const double tDRx = ld_nt(dD + 9 * np + n), tDRy = ld_nt(dD + 10 * np + n), tDEx = ld_nt(dD + 12 * np + n), tDEy = ld_nt(dD + 13 * np + n);
+  const double ttxx = ld_nt(dT + 0 * np + n), ttyy = ld_nt(dT + 1 * np + n), ttxy = ld_nt(dT + 3 * np + n);
   const double tP = g1 * (tR * b.E + b.R * tE);
   const double tDPx = g1 * (tR * b.DEx + b.R * tDEx + tDRx * b.E + b.DRx * tE);
   const double tDPy = g1 * (tR * b.DEy + b.R * tDEy + tDRy * b.E + b.DRy * tE);
@@ -394,14 +394,14 @@ __global__ __launch_bounds__(256) void assemble_tl_kernel(AssembleLinArgs a) {
   const double invR2 = invR * invR;
   double d[5];
   if (a.is3d) {
-    const double tW = dS[2 * np + n];
-    const double tDUz = dD[2 * np + n], tDVz = dD[5 * np + n], tDWx = dD[6 * np + n], tDWy = dD[7 * np + n];
-    const double tDWz = dD[8 * np + n], tDRz = dD[11 * np + n], tDEz = dD[14 * np + n];
-    const double ttzz = dT[2 * np + n], ttxz = dT[4 * np + n], ttyz = dT[5 * np + n];
+    const double tW = ld_nt(dS + 2 * np + n);
+    const double tDUz = ld_nt(dD + 2 * np + n), tDVz = ld_nt(dD + 5 * np + n), tDWx = ld_nt(dD + 6 * np + n), tDWy = ld_nt(dD + 7 * np + n);
+    const double tDWz = ld_nt(dD + 8 * np + n), tDRz = ld_nt(dD + 11 * np + n), tDEz = ld_nt(dD + 14 * np + n);
+    const double ttzz = ld_nt(dT + 2 * np + n), ttxz = ld_nt(dT + 4 * np + n), ttyz = ld_nt(dT + 5 * np + n);
     const double tDPz = g1 * (tR * b.DEz + b.R * tDEz + tDRz * b.E + b.DRz * tE);
-    const double tvtx = dG[0 * np + n] + dG[4 * np + n] + dG[8 * np + n];
-    const double tvty = dG[5 * np + n] + dG[1 * np + n] + dG[9 * np + n];
-    const double tvtz = dG[10 * np + n] + dG[2 * np + n] + dG[6 * np + n];
+    const double tvtx = ld_nt(dG + 0 * np + n) + ld_nt(dG + 4 * np + n) + ld_nt(dG + 8 * np + n);
+    const double tvty = ld_nt(dG + 5 * np + n) + ld_nt(dG + 1 * np + n) + ld_nt(dG + 9 * np + n);
+    const double tvtz = ld_nt(dG + 10 * np + n) + ld_nt(dG + 2 * np + n) + ld_nt(dG + 6 * np + n);
     const double tdiv = tDUx + tDVy + tDWz;
     d[3] = -(tR * b.div + b.R * tdiv) - (tDRx * b.U + b.DRx * tU) - (tDRy * b.V + b.DRy * tV) -
            (tDRz * b.W + b.DRz * tW);
@@ -414,12 +414,12 @@ __global__ __launch_bounds__(256) void assemble_tl_kernel(AssembleLinArgs a) {
     const double tPhi = -(tP * b.div + b.P * tdiv) + ttxx * b.DUx + b.txx * tDUx + ttyy * b.DVy + b.tyy * tDVy +
                         ttzz * b.DWz + b.tzz * tDWz + ttxy * (b.DVx + b.DUy) + b.txy * (tDVx + tDUy) +
                         ttxz * (b.DWx + b.DUz) + b.txz * (tDWx + tDUz) + ttyz * (b.DWy + b.DVz) +
-                        b.tyz * (tDWy + tDVz) - dG[3 * np + n] - dG[7 * np + n] - dG[11 * np + n];
+                        b.tyz * (tDWy + tDVz) - ld_nt(dG + 3 * np + n) - ld_nt(dG + 7 * np + n) - ld_nt(dG + 11 * np + n);
     d[4] = -(tDEx * b.U + b.DEx * tU + tDEy * b.V + b.DEy * tV + tDEz * b.W + b.DEz * tW) + tPhi * invR -
            b.Phi * tR * invR2;
   } else {
-    const double tvtx = dG[0 * np + n] + dG[4 * np + n];
-    const double tvty = dG[5 * np + n] + dG[1 * np + n];
+    const double tvtx = ld_nt(dG + 0 * np + n) + ld_nt(dG + 4 * np + n);
+    const double tvty = ld_nt(dG + 5 * np + n) + ld_nt(dG + 1 * np + n);
     const double tdiv = tDUx + tDVy;
     d[3] = -(tR * b.div + b.R * tdiv) - (tDRx * b.U + b.DRx * tU) - (tDRy * b.V + b.DRy * tV);
     d[0] = -(tDUx * b.U + b.DUx * tU + tDUy * b.V + b.DUy * tV) + (tvtx - tDPx) * invR -
@@ -428,7 +428,7 @@ __global__ __launch_bounds__(256) void assemble_tl_kernel(AssembleLinArgs a) {
            (b.vty - b.DPy) * tR * invR2;
     d[2] = 0.0;
     const double tPhi = -(tP * b.div + b.P * tdiv) + ttxx * b.DUx + b.txx * tDUx + ttyy * b.DVy + b.tyy * tDVy +
-                        ttxy * (b.DVx + b.DUy) + b.txy * (tDVx + tDUy) - dG[3 * np + n] - dG[7 * np + n];
+                        ttxy * (b.DVx + b.DUy) + b.txy * (tDVx + tDUy) - ld_nt(dG + 3 * np + n) - ld_nt(dG + 7 * np + n);
     d[4] = -(tDEx * b.U + b.DEx * tU + tDEy * b.V + b.DEy * tV) + tPhi * invR - b.Phi * tR * invR2;
   }
   const double* Q = a.dQ + lo;
@@ -440,11 +440,11 @@ __global__ __launch_bounds__(256) void assemble_tl_kernel(AssembleLinArgs a) {
     if (!a.is3d && f == 2) continue;
     const long long o = f * np + n;
     switch (a.stage) {
-      case 0: O[o] = d[f]; break;
-      case 1: Dl[o] = dt / 6.0 * d[f]; O[o] = Q[o] + dt / 2.0 * d[f]; break;
-      case 2: Dl[o] = Dl[o] + dt / 3.0 * d[f]; O[o] = Q[o] + dt / 2.0 * d[f]; break;
-      case 3: Dl[o] = Dl[o] + dt / 3.0 * d[f]; O[o] = Q[o] + dt * d[f]; break;
-      default: O[o] = Q[o] + Dl[o] + dt / 6.0 * d[f]; break;
+      case 0: st_nt(O + o, d[f]); break;
+      case 1: st_nt(Dl + o, dt / 6.0 * d[f]); st_nt(O + o, ld_nt(Q + o) + dt / 2.0 * d[f]); break;
+      case 2: st_nt(Dl + o, ld_nt(Dl + o) + dt / 3.0 * d[f]); st_nt(O + o, ld_nt(Q + o) + dt / 2.0 * d[f]); break;
+      case 3: st_nt(Dl + o, ld_nt(Dl + o) + dt / 3.0 * d[f]); st_nt(O + o, ld_nt(Q + o) + dt * d[f]); break;
+      default: st_nt(O + o, ld_nt(Q + o) + ld_nt(Dl + o) + dt / 6.0 * d[f]); break;
     }
   }
 }
@@ -457,8 +457,8 @@ __global__ __launch_bounds__(256) void assemble_adj_kernel(AssembleLinArgs a) {
   const Base b = load_base(a, n);
   const double g1 = a.c.gm1;
   const double* db = a.dbar + lo;
-  const double aU = db[0 * np + n], aV = db[1 * np + n], aR = db[3 * np + n], aE = db[4 * np + n];
-  const double aW = a.is3d ? db[2 * np + n] : 0.0;
+  const double aU = ld_nt(db + 0 * np + n), aV = ld_nt(db + 1 * np + n), aR = ld_nt(db + 3 * np + n), aE = ld_nt(db + 4 * np + n);
+  const double aW = a.is3d ? ld_nt(db + 2 * np + n) : 0.0;
   const double invR = 1.0 / b.R;
   const double invR2 = invR * invR;
   // momentum
@@ -513,29 +513,29 @@ __global__ __launch_bounds__(256) void assemble_adj_kernel(AssembleLinArgs a) {
   DWzb += divb;
 
   double* Sb = a.Sbar + lo;   // the stage-state adjoint starts here (later kernels accumulate)
-  Sb[0 * np + n] = Ub;
-  Sb[1 * np + n] = Vb;
-  if (a.is3d) Sb[2 * np + n] = Wb;
-  Sb[3 * np + n] = Rb;
-  Sb[4 * np + n] = Eb;
+  st_nt(Sb + 0 * np + n, Ub);
+  st_nt(Sb + 1 * np + n, Vb);
+  if (a.is3d) st_nt(Sb + 2 * np + n, Wb);
+  st_nt(Sb + 3 * np + n, Rb);
+  st_nt(Sb + 4 * np + n, Eb);
   double* Db = a.Dbar + lo;
-  Db[0 * np + n] = DUxb; Db[1 * np + n] = DUyb; Db[2 * np + n] = DUzb;
-  Db[3 * np + n] = DVxb; Db[4 * np + n] = DVyb; Db[5 * np + n] = DVzb;
-  Db[6 * np + n] = DWxb; Db[7 * np + n] = DWyb; Db[8 * np + n] = DWzb;
-  Db[9 * np + n] = DRxb; Db[10 * np + n] = DRyb; Db[11 * np + n] = DRzb;
-  Db[12 * np + n] = DExb; Db[13 * np + n] = DEyb; Db[14 * np + n] = DEzb;
+  st_nt(Db + 0 * np + n, DUxb); st_nt(Db + 1 * np + n, DUyb); st_nt(Db + 2 * np + n, DUzb);
+  st_nt(Db + 3 * np + n, DVxb); st_nt(Db + 4 * np + n, DVyb); st_nt(Db + 5 * np + n, DVzb);
+  st_nt(Db + 6 * np + n, DWxb); st_nt(Db + 7 * np + n, DWyb); st_nt(Db + 8 * np + n, DWzb);
+  st_nt(Db + 9 * np + n, DRxb); st_nt(Db + 10 * np + n, DRyb); st_nt(Db + 11 * np + n, DRzb);
+  st_nt(Db + 12 * np + n, DExb); st_nt(Db + 13 * np + n, DEyb); st_nt(Db + 14 * np + n, DEzb);
   double* Tb = a.Tbar + lo;
-  Tb[0 * np + n] = txxb; Tb[1 * np + n] = tyyb; Tb[2 * np + n] = tzzb;
-  Tb[3 * np + n] = txyb; Tb[4 * np + n] = txzb; Tb[5 * np + n] = tyzb;
-  Tb[6 * np + n] = 0.0; Tb[7 * np + n] = 0.0; Tb[8 * np + n] = 0.0;
+  st_nt(Tb + 0 * np + n, txxb); st_nt(Tb + 1 * np + n, tyyb); st_nt(Tb + 2 * np + n, tzzb);
+  st_nt(Tb + 3 * np + n, txyb); st_nt(Tb + 4 * np + n, txzb); st_nt(Tb + 5 * np + n, tyzb);
+  st_nt(Tb + 6 * np + n, 0.0); st_nt(Tb + 7 * np + n, 0.0); st_nt(Tb + 8 * np + n, 0.0);
   double* Gb = a.Gbar + lo;
   // x: txx,txy,txz,qx | y: txy,tyy,tyz,qy | z: txz,tyz,tzz,qz
-  Gb[0 * np + n] = vtxb; Gb[1 * np + n] = vtyb; Gb[2 * np + n] = vtzb; Gb[3 * np + n] = -Phib;
-  Gb[4 * np + n] = vtxb; Gb[5 * np + n] = vtyb; Gb[6 * np + n] = vtzb; Gb[7 * np + n] = -Phib;
-  Gb[8 * np + n] = vtxb; Gb[9 * np + n] = vtyb; Gb[10 * np + n] = vtzb; Gb[11 * np + n] = -Phib;
+  st_nt(Gb + 0 * np + n, vtxb); st_nt(Gb + 1 * np + n, vtyb); st_nt(Gb + 2 * np + n, vtzb); st_nt(Gb + 3 * np + n, -Phib);
+  st_nt(Gb + 4 * np + n, vtxb); st_nt(Gb + 5 * np + n, vtyb); st_nt(Gb + 6 * np + n, vtzb); st_nt(Gb + 7 * np + n, -Phib);
+  st_nt(Gb + 8 * np + n, vtxb); st_nt(Gb + 9 * np + n, vtyb); st_nt(Gb + 10 * np + n, vtzb); st_nt(Gb + 11 * np + n, -Phib);
   if (!a.is3d) {
-    Gb[2 * np + n] = 0.0; Gb[6 * np + n] = 0.0;
-    Gb[8 * np + n] = 0.0; Gb[9 * np + n] = 0.0; Gb[10 * np + n] = 0.0; Gb[11 * np + n] = 0.0;
+    st_nt(Gb + 2 * np + n, 0.0); st_nt(Gb + 6 * np + n, 0.0);
+    st_nt(Gb + 8 * np + n, 0.0); st_nt(Gb + 9 * np + n, 0.0); st_nt(Gb + 10 * np + n, 0.0); st_nt(Gb + 11 * np + n, 0.0);
   }
 }
 

@@ -83,6 +83,9 @@ __device__ __forceinline__ double visc_flux(int q, const FlowConst& c, double E,
 __device__ __forceinline__ double ld_nt(const double* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void st_nt(double* p, double v) { __builtin_nontemporal_store(v, p); }
 typedef double nt_double2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ nt_double2 ld_nt2(const double* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const nt_double2*>(p));
+}
 __device__ __forceinline__ void st_nt2(double* p, double a, double b) {
   __builtin_nontemporal_store(nt_double2{a, b}, reinterpret_cast<nt_double2*>(p));
 }

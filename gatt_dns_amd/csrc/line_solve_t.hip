@@ -241,7 +241,7 @@ __global__ __launch_bounds__(1024) void line_solve_band_t_kernel(SolveArgs a, Ba
 #pragma unroll
   for (int tt = 0; tt < M; ++tt) {
     const int e = min(s + tt, N - 1);
-    r[tt] = in[base + (long long)e * stride];
+    r[tt] = ld_nt(in + base + (long long)e * stride);
   }
 
   const int type = UNI ? __builtin_amdgcn_readfirstlane(a.types[line0]) : a.types[lsafe];
@@ -375,9 +375,9 @@ __global__ __launch_bounds__(1024) void line_solve_band_t_kernel(SolveArgs a, Ba
       }
       const long long p = base + (long long)e * stride;
       double v = b.alpha * acc;
-      if (src) v = b.beta * src[p] + v;
-      if (b.accumulate) v = out[p] + v;
-      out[p] = v;
+      if (src) v = b.beta * ld_nt(src + p) + v;
+      if (b.accumulate) v = ld_nt(out + p) + v;
+      st_nt(out + p, v);
     }
   }
 }

@@ -283,7 +283,7 @@ __device__ __forceinline__ void line_solve_wsp_body(const SolveArgs& a, BlockPos
       int e = s - H + u;
       e = e < 0 ? e + N : (e >= N ? e - N : e);
       e = e >= N ? N - 1 : e;
-      fw[u] = in[base + (long long)e * stride];
+      fw[u] = in[base + (long long)e * stride];   // neighbouring segments re-read the halo rows
     }
   }
   // only this wavefront reads its slot, and its LDS accesses run in order
@@ -413,7 +413,6 @@ __device__ __forceinline__ void line_solve_wspx_body(const SolveArgs& a, BlockPo
     // batches of XB pairs per thread: every load of a batch is issued before the first
     // LDS store waits for its data
     constexpr int XB = 8;
-    const double2* in2 = reinterpret_cast<const double2*>(in);
     const int npair = total / 2;
     const int bs = blockDim.x;
     for (int b0 = threadIdx.x; b0 < npair; b0 += XB * bs) {
@@ -421,7 +420,10 @@ __device__ __forceinline__ void line_solve_wspx_body(const SolveArgs& a, BlockPo
 #pragma unroll
       for (int k = 0; k < XB; ++k) {
         const int i2 = b0 + k * bs;
-        if (i2 < npair) v[k] = in2[i2];
+        if (i2 < npair) {
+          const nt_double2 t = ld_nt2(in + 2 * i2);
+          v[k] = make_double2(t.x, t.y);
+        }
       }
 #pragma unroll
       for (int k = 0; k < XB; ++k) {

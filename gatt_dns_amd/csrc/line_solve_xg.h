@@ -58,7 +58,7 @@ __device__ __forceinline__ void line_solve_xg_body(const SolveArgs& a, BlockPos 
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       const int e = lane + 64 * k;
-      if (e < N) v[k] = src[e];
+      if (e < N) v[k] = ld_nt(src + e);
     }
 #pragma unroll
     for (int k = 0; k < M; ++k) {
@@ -170,7 +170,7 @@ __device__ __forceinline__ void line_solve_xg_body(const SolveArgs& a, BlockPos 
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (int e = lane; e < N; e += 64) dst[e] = buf[e];
+  for (int e = lane; e < N; e += 64) st_nt(dst + e, buf[e]);
 }
 
 template <int NR, int M, bool BLEND>
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(256) void line_solve_xgt_kernel(BandTArgs a) {
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       const int e = lane + 64 * k;
-      if (e < N) v[k] = in[e];
+      if (e < N) v[k] = ld_nt(in + e);
     }
 #pragma unroll
     for (int k = 0; k < M; ++k) {
@@ -380,12 +380,22 @@ __global__ __launch_bounds__(256) void line_solve_xgt_kernel(BandTArgs a) {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // unit-stride combination with src and out
-  for (int j = lane; j < N; j += 64) {
-    double v = obuf[j];
-    if (src) v = a.beta * src[j] + v;
-    if (a.accumulate) v = out[j] + v;
-    out[j] = v;
+  // unit-stride combination with src and out (N <= 64 M): every load of the lane in flight
+  // before the first store
+  double v[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const int j = lane + 64 * k;
+    if (j < N) {
+      v[k] = obuf[j];
+      if (src) v[k] = a.beta * ld_nt(src + j) + v[k];
+      if (a.accumulate) v[k] = ld_nt(out + j) + v[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const int j = lane + 64 * k;
+    if (j < N) st_nt(out + j, v[k]);
   }
 }
 
