@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void band_t_kernel(BandTArgs a, int m0) {
   double* out = a.out[f] + (long long)mem * a.out_stride;
   double v = a.alpha * acc;
   if (a.src[f]) v = a.beta * ld_nt(a.src[f] + (long long)mem * a.out_stride + p) + v;
-  if (a.accumulate) v = ld_nt(out + p) + v;
+  if ((a.accumulate >> f) & 1) v = ld_nt(out + p) + v;
   st_nt(out + p, v);
 }
 
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(1024) void band_t_seg_kernel(BandTArgs a) {
       const long long p = base + (long long)e * stride;
       double v = a.alpha * acc;
       if (src) v = a.beta * ld_nt(src + p) + v;
-      if (a.accumulate) v = ld_nt(out + p) + v;
+      if ((a.accumulate >> f) & 1) v = ld_nt(out + p) + v;
       st_nt(out + p, v);
     }
   }
@@ -524,18 +524,20 @@ __global__ __launch_bounds__(256) void assemble_adj_kernel(AssembleLinArgs a) {
   st_nt(Db + 6 * np + n, DWxb); st_nt(Db + 7 * np + n, DWyb); st_nt(Db + 8 * np + n, DWzb);
   st_nt(Db + 9 * np + n, DRxb); st_nt(Db + 10 * np + n, DRyb); st_nt(Db + 11 * np + n, DRzb);
   st_nt(Db + 12 * np + n, DExb); st_nt(Db + 13 * np + n, DEyb); st_nt(Db + 14 * np + n, DEzb);
+  // the stresses (the heat-flux adjoints get their one contribution from the transposed
+  // flux solves, which store rather than add)
   double* Tb = a.Tbar + lo;
   st_nt(Tb + 0 * np + n, txxb); st_nt(Tb + 1 * np + n, tyyb); st_nt(Tb + 2 * np + n, tzzb);
   st_nt(Tb + 3 * np + n, txyb); st_nt(Tb + 4 * np + n, txzb); st_nt(Tb + 5 * np + n, tyzb);
-  st_nt(Tb + 6 * np + n, 0.0); st_nt(Tb + 7 * np + n, 0.0); st_nt(Tb + 8 * np + n, 0.0);
+  // flux derivatives x: txx,txy,txz,qx | y: txy,tyy,tyz,qy | z: txz,tyz,tzz,qz: the same
+  // four values for every direction, stored once per direction or once for all of them
   double* Gb = a.Gbar + lo;
-  // x: txx,txy,txz,qx | y: txy,tyy,tyz,qy | z: txz,tyz,tzz,qz
-  st_nt(Gb + 0 * np + n, vtxb); st_nt(Gb + 1 * np + n, vtyb); st_nt(Gb + 2 * np + n, vtzb); st_nt(Gb + 3 * np + n, -Phib);
-  st_nt(Gb + 4 * np + n, vtxb); st_nt(Gb + 5 * np + n, vtyb); st_nt(Gb + 6 * np + n, vtzb); st_nt(Gb + 7 * np + n, -Phib);
-  st_nt(Gb + 8 * np + n, vtxb); st_nt(Gb + 9 * np + n, vtyb); st_nt(Gb + 10 * np + n, vtzb); st_nt(Gb + 11 * np + n, -Phib);
-  if (!a.is3d) {
-    st_nt(Gb + 2 * np + n, 0.0); st_nt(Gb + 6 * np + n, 0.0);
-    st_nt(Gb + 8 * np + n, 0.0); st_nt(Gb + 9 * np + n, 0.0); st_nt(Gb + 10 * np + n, 0.0); st_nt(Gb + 11 * np + n, 0.0);
+  for (int d = 0; d < a.gbar_copies; ++d) {
+    double* g = Gb + (long long)(4 * d) * np + n;
+    st_nt(g, vtxb);
+    st_nt(g + np, vtyb);
+    if (a.is3d) st_nt(g + 2 * np, vtzb);
+    st_nt(g + 3 * np, -Phib);
   }
 }
 

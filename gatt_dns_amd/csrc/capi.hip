@@ -2805,10 +2805,12 @@ void tl_stage(gatt_ctx* c, const double* S, double* dS, double* dQ, double* dOut
 }
 
 // Transposed operator of `kind` along `dir` applied to nf fields: out = beta*src
-// + (accumulate ? out : 0) + alpha * R^T (Thomas^T in); `in` is workspace (the
-// two-launch path runs the Thomas part in place on it).
+// + (bit f of acc_mask ? out : 0) + alpha * R^T (Thomas^T in) for field f; `in` is
+// workspace (the two-launch path runs the Thomas part in place on it, see
+// solve_t_one_launch).
+constexpr int kAccAll = 0xffff;
 void run_solve_t(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double* const* out,
-                 double* const* src, double alpha, double beta, int accumulate, int nm, long long in_stride,
+                 double* const* src, double alpha, double beta, int acc_mask, int nm, long long in_stride,
                  long long out_stride) {
   DirOps& d = c->dir[dir];
   BandTArgs b{};
@@ -2839,7 +2841,7 @@ void run_solve_t(gatt_ctx* c, int dir, int kind, int nf, double* const* in, doub
   }
   b.alpha = alpha;
   b.beta = beta;
-  b.accumulate = accumulate;
+  b.accumulate = acc_mask;
   // along x: one launch, one wavefront per line
   if (dir == 0 && launch_line_solve_xgt(b, nm, c->stream)) return;
   SolveArgs a{};
@@ -2866,6 +2868,14 @@ void run_solve_t(gatt_ctx* c, int dir, int kind, int nf, double* const* in, doub
   if (launch_line_solve_band_t(a, b, nm, c->max_lds, c->stream)) return;
   launch_line_solve_t(a, nm, c->stream);
   launch_band_t(b, nm, c->stream);
+}
+
+// The transposed solve of `kind` along `dir` for nf fields runs in one launch that reads
+// its whole input before writing (so it leaves `in` as it was, and may act in place).
+bool solve_t_one_launch(const gatt_ctx* c, int dir, int kind, int nf) {
+  const LineOpDev& o = c->dir[dir].op[kind];
+  if (dir == 0) return o.N <= 512 && o.nR >= 2 && o.nR <= 11 && o.nR - 1 <= o.N;
+  return line_solve_band_t_fits(dir, o.N, o.nR, nf, c->batch, c->max_lds);
 }
 
 // Sbar (lin slab, all members) = J(S)^T kbar for the base stage state S
@@ -2899,19 +2909,28 @@ void adj_rhs(gatt_ctx* c, const double* S, long long ss, double* kbar, double* s
     a.Tbar = Tb;
     a.Gbar = Gb;
     a.is3d = c->is3d;
+    // the flux-derivative adjoints of the three directions are the same four values
+    // (vtx, vty, vtz and -Phi each enter once per direction): one copy serves every
+    // direction when no transposed solve works in place on its input
+    const int nflux = c->is3d ? 4 : 3;
+    bool shared = true;
+    for (int d = 0; d < ndir; ++d) shared = shared && solve_t_one_launch(c, d, GATT_DERIV, nflux);
+    a.gbar_copies = shared ? 1 : ndir;
     launch_assemble_adj(a, nm, c->stream);
-    // Tbar += d^T Gbar, direction by direction
+    // Tbar += d^T Gbar, direction by direction; the heat fluxes get their one contribution
+    // here (assemble_adj leaves them unset)
     for (int d = 0; d < ndir; ++d) {
       double* in[kMaxPtr];
       double* out[kMaxPtr];
-      int nf = 0;
+      int nf = 0, acc = 0;
       for (int q = 0; q < 4; ++q) {
         if (!c->is3d && q == 2) continue;
-        in[nf] = Gb + (long long)(4 * d + q) * np;
+        in[nf] = Gb + (long long)((shared ? 0 : 4 * d) + q) * np;
         out[nf] = Tb + (long long)tin3[d][q] * np;
+        if (q != 3) acc |= 1 << nf;
         ++nf;
       }
-      run_solve_t(c, d, GATT_DERIV, nf, in, out, nullptr, 1.0, 0.0, 1, nm, ls, ls);
+      run_solve_t(c, d, GATT_DERIV, nf, in, out, nullptr, 1.0, 0.0, acc, nm, ls, ls);
     }
     ViscLinArgs v{};
     v.c = c->fc;
@@ -2936,7 +2955,7 @@ void adj_rhs(gatt_ctx* c, const double* S, long long ss, double* kbar, double* s
         out[nf] = sb + (long long)f * np;
         ++nf;
       }
-      run_solve_t(c, d, GATT_DERIV, nf, in, out, nullptr, 1.0, 0.0, 1, nm, ls, ls);
+      run_solve_t(c, d, GATT_DERIV, nf, in, out, nullptr, 1.0, 0.0, kAccAll, nm, ls, ls);
     }
   }
 }
@@ -3112,10 +3131,7 @@ void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
     if (!c->filterDir[d]) continue;
     // the one-launch transposed solves read their whole input before writing, so the
     // filter can act in place; the two-launch path gets a copy
-    const LineOpDev& fo = c->dir[d].op[GATT_FILTER];
-    const bool in_place = d == 0 ? (fo.N <= 512 && fo.nR >= 2 && fo.nR <= 11 && fo.nR - 1 <= fo.N)
-                                 : line_solve_band_t_fits(d, fo.N, fo.nR, (int)state_fields(c).size(), c->batch,
-                                                          c->max_lds);
+    const bool in_place = solve_t_one_launch(c, d, GATT_FILTER, (int)state_fields(c).size());
     if (!in_place) fields_axpby(c, KB, ls, QB, ls, 1.0, 0.0);
     double* in[kMaxPtr];
     double* out[kMaxPtr];

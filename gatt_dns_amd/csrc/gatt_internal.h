@@ -352,7 +352,7 @@ bool launch_line_solve_band_t(const SolveArgs& a, const BandTArgs& b, int nmembe
 bool line_solve_band_t_fits(int dir, int N, int nR, int nf, int nmembers, long long max_lds);
 
 // Transposed RHS band along one direction:
-//   out = beta*src + (accumulate ? out : 0) + alpha * R^T in
+//   out = beta*src + (accumulate bit f ? out : 0) + alpha * R^T in   (field f of the launch)
 struct BandTArgs {
   int dir, nx, ny, nz, N, nR, cs;
   long long np;
@@ -370,7 +370,7 @@ struct BandTArgs {
   double* out[kMaxPtr];
   const double* src[kMaxPtr];   // or nullptr
   double alpha, beta;
-  int accumulate;
+  int accumulate;         // bit f: field f adds to what out holds
 };
 void launch_band_t(const BandTArgs& a, int nmembers, hipStream_t st);
 // Transposed solve and band along x in one launch (line_solve_xg.h); false if the
@@ -408,7 +408,8 @@ struct AssembleLinArgs {
   // adjoint
   const double* dbar;   // 5 arrays (lin stride)
   double* Sbar;         // 5 arrays, set (lin stride)
-  double* Dbar; double* Tbar; double* Gbar;   // written (lin stride)
+  double* Dbar; double* Tbar; double* Gbar;   // written (lin stride): Tbar 0-5, Gbar 4 per copy
+  int gbar_copies;      // flux-derivative adjoints stored once for all directions (1) or per direction
   int is3d;
 };
 void launch_assemble_tl(const AssembleLinArgs& a, int nmembers, hipStream_t st);

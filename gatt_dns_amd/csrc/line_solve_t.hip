@@ -376,7 +376,7 @@ __global__ __launch_bounds__(1024) void line_solve_band_t_kernel(SolveArgs a, Ba
       const long long p = base + (long long)e * stride;
       double v = b.alpha * acc;
       if (src) v = b.beta * ld_nt(src + p) + v;
-      if (b.accumulate) v = ld_nt(out + p) + v;
+      if ((b.accumulate >> f) & 1) v = ld_nt(out + p) + v;
       st_nt(out + p, v);
     }
   }

@@ -389,7 +389,7 @@ __global__ __launch_bounds__(256) void line_solve_xgt_kernel(BandTArgs a) {
     if (j < N) {
       v[k] = obuf[j];
       if (src) v[k] = a.beta * ld_nt(src + j) + v[k];
-      if (a.accumulate) v[k] = ld_nt(out + j) + v[k];
+      if ((a.accumulate >> f) & 1) v[k] = ld_nt(out + j) + v[k];
     }
   }
 #pragma unroll
