@@ -1089,6 +1089,29 @@ void run_filters(gatt_ctx* c, double* q, double theta) {
   }
 }
 
+// The filters from src into dst (member stride slab for both): the first direction that
+// filters reads src, the others act in place on dst; src is copied when none filters.
+void run_filters_from(gatt_ctx* c, const double* src, double* dst, double theta) {
+  std::vector<int> flds = c->is3d ? std::vector<int>{0, 1, 2, 3, 4} : std::vector<int>{0, 1, 3, 4};
+  const int ndir = c->is3d ? 3 : 2;
+  bool first = true;
+  for (int d = 0; d < ndir; ++d) {
+    if (!c->filterDir[d] || c->dir[d].identity[GATT_FILTER]) continue;
+    double* in[kMaxPtr];
+    double* out[kMaxPtr];
+    int nf = 0;
+    for (int f : flds) {
+      in[nf] = const_cast<double*>(first ? src : dst) + (long long)f * c->np;
+      out[nf] = dst + (long long)f * c->np;
+      ++nf;
+    }
+    run_solve(c, d, GATT_FILTER, nf, in, out, theta, 1, c->batch, c->slab, c->slab);
+    first = false;
+  }
+  if (first)
+    for (int b = 0; b < c->batch; ++b) launch_copy(dst + b * c->slab, src + b * c->slab, 5 * c->np, c->stream);
+}
+
 void run_sfd(gatt_ctx* c, double* q, double dt) {
   if (c->sfdType <= 0) return;
   SfdArgs s{};
@@ -3089,23 +3112,42 @@ void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
   double* qn = c->state(1 - c->cur);     // scratch for the last stage; the base q is not modified
   double* B4 = lin_ptr(c, LN_B4);
   // every boundary application logs what it overwrites, for the transposed program
-  base_dtg(c, q, sl, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
-  forward_assemble(c, q, q, S, 1, dt, 0, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
-  run_bc(c, S, dt / 2.0, undo_log(c, 0));
-  fields_axpby(c, B1, ls, S, sl, 1.0, 0.0);
-  base_dtg(c, S, sl, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
-  forward_assemble(c, S, q, S, 2, dt, 0, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
-  run_bc(c, S, dt / 2.0, undo_log(c, 1));
-  fields_axpby(c, B2, ls, S, sl, 1.0, 0.0);
-  base_dtg(c, S, sl, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
-  forward_assemble(c, S, q, S, 3, dt, 0, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
-  run_bc(c, S, dt, undo_log(c, 2));
-  fields_axpby(c, B3, ls, S, sl, 1.0, 0.0);
-  base_dtg(c, S, sl, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
-  forward_assemble(c, S, q, qn, 4, dt, 0, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
-  run_bc(c, qn, dt, undo_log(c, 3));
-  fields_axpby(c, B4, ls, qn, sl, 1.0, 0.0);
-  run_filters(c, qn, dt / c->filterCharTime);
+  if (nb == 1) {
+    // one member: the member strides of the state slab and the linearised slab do not
+    // matter, so the stage states go straight into B1..B4 (no copies), and the filters
+    // read B4 and write the step's result into qn
+    base_dtg(c, q, sl, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
+    forward_assemble(c, q, q, B1, 1, dt, 0, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
+    run_bc(c, B1, dt / 2.0, undo_log(c, 0));
+    base_dtg(c, B1, ls, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
+    forward_assemble(c, B1, q, B2, 2, dt, 0, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
+    run_bc(c, B2, dt / 2.0, undo_log(c, 1));
+    base_dtg(c, B2, ls, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
+    forward_assemble(c, B2, q, B3, 3, dt, 0, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
+    run_bc(c, B3, dt, undo_log(c, 2));
+    base_dtg(c, B3, ls, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
+    forward_assemble(c, B3, q, B4, 4, dt, 0, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
+    run_bc(c, B4, dt, undo_log(c, 3));
+    run_filters_from(c, B4, qn, dt / c->filterCharTime);
+  } else {
+    base_dtg(c, q, sl, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
+    forward_assemble(c, q, q, S, 1, dt, 0, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
+    run_bc(c, S, dt / 2.0, undo_log(c, 0));
+    fields_axpby(c, B1, ls, S, sl, 1.0, 0.0);
+    base_dtg(c, S, sl, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
+    forward_assemble(c, S, q, S, 2, dt, 0, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
+    run_bc(c, S, dt / 2.0, undo_log(c, 1));
+    fields_axpby(c, B2, ls, S, sl, 1.0, 0.0);
+    base_dtg(c, S, sl, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
+    forward_assemble(c, S, q, S, 3, dt, 0, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
+    run_bc(c, S, dt, undo_log(c, 2));
+    fields_axpby(c, B3, ls, S, sl, 1.0, 0.0);
+    base_dtg(c, S, sl, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
+    forward_assemble(c, S, q, qn, 4, dt, 0, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
+    run_bc(c, qn, dt, undo_log(c, 3));
+    fields_axpby(c, B4, ls, qn, sl, 1.0, 0.0);
+    run_filters(c, qn, dt / c->filterCharTime);
+  }
   run_sfd(c, qn, dt);
   run_bc(c, qn, dt, undo_log(c, 4));
   // the final boundary conditions (main.F90:405), walked back on qn
