@@ -95,8 +95,19 @@ def hbm(fetch_csv, write_csv, steps, points, cmd):
     return res, "\n".join(lines) + "\n"
 
 
-def stats(trace_csv, steps, cmd):
-    rows = [r for r in _rows(trace_csv) if _lib_kernel(r["Kernel_Name"])]
+def _grid(r):
+    return int(r.get("Grid_Size_X", 1) or 1) * int(r.get("Grid_Size_Y", 1) or 1) * int(r.get("Grid_Size_Z", 1) or 1)
+
+
+def stats(trace_csv, steps, cmd, title=None, start_grid=0, note=""):
+    """Per-kernel table of a kernel trace.  With start_grid, only the dispatches from the first
+    one whose grid holds at least start_grid work-items onwards are counted (a run that sets up a
+    smaller case first)."""
+    rows = sorted(_rows(trace_csv), key=lambda r: int(r["Start_Timestamp"]))
+    if start_grid:
+        first = next((i for i, r in enumerate(rows) if _grid(r) >= start_grid), len(rows))
+        rows = rows[first:]
+    rows = [r for r in rows if _lib_kernel(r["Kernel_Name"])]
     per = collections.defaultdict(lambda: [0, 0.0])
     iv = []
     for r in rows:
@@ -115,12 +126,13 @@ def stats(trace_csv, steps, cmd):
             cur1 = max(cur1, b)
     if cur1 is not None:
         busy += cur1 - cur0
-    lines = ["# Kernel time of the step, turbulence box 256^3 (FP64 RK4 step, default shapes)", "",
+    lines = ["# " + (title or "Kernel time of the step, turbulence box 256^3 (FP64 RK4 step, default shapes)"), "",
              "Command (from the repository root, on one MI355X):", "", "```", cmd, "```", "",
              "Per step: kernel time sum %.4f ms; busy union %.4f ms (the time during which at least one kernel of "
              "the step runs; the direction solves of a pass run concurrently on forked streams, so a kernel's mean "
-             "here can exceed its isolated launch time)." % (sum(v[1] for v in per.values()) / steps / 1e3,
-                                                              busy / 1e6 / steps), "",
+             "here can exceed its isolated launch time).%s" % (sum(v[1] for v in per.values()) / steps / 1e3,
+                                                                busy / 1e6 / steps, (" " + note) if note else ""),
+             "",
              "| kernel | dispatches per step | mean us | us per step |", "|---|---:|---:|---:|"]
     for name, (n, us) in sorted(per.items(), key=lambda kv: -kv[1][1]):
         lines.append("| `%s` | %g | %.1f | %.1f |" % (name, n / steps, us / n, us / steps))
@@ -134,6 +146,9 @@ def main(argv=None):
     a.add_argument("--trace", required=True)
     a.add_argument("--steps", type=int, required=True)
     a.add_argument("--cmd", default="")
+    a.add_argument("--title", default=None)
+    a.add_argument("--note", default="")
+    a.add_argument("--start-grid", type=int, default=0)
     a.add_argument("--out", required=True)
     b = sub.add_parser("hbm")
     b.add_argument("--fetch", required=True)
@@ -144,7 +159,8 @@ def main(argv=None):
     b.add_argument("--out", required=True)
     args = ap.parse_args(argv)
     if args.kind == "stats":
-        open(args.out + ".md", "w").write(stats(args.trace, args.steps, args.cmd))
+        open(args.out + ".md", "w").write(stats(args.trace, args.steps, args.cmd, args.title, args.start_grid,
+                                                args.note))
     else:
         res, md = hbm(args.fetch, args.write, args.steps, args.points, args.cmd)
         json.dump(res, open(args.out + ".json", "w"), indent=1)
