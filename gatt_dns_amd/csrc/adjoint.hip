@@ -431,6 +431,21 @@ __global__ __launch_bounds__(256) void assemble_tl_kernel(AssembleLinArgs a) {
                         ttxy * (b.DVx + b.DUy) + b.txy * (tDVx + tDUy) - ld_nt(dG + 3 * np + n) - ld_nt(dG + 7 * np + n);
     d[4] = -(tDEx * b.U + b.DEx * tU + tDEy * b.V + b.DEy * tV) + tPhi * invR - b.Phi * tR * invR2;
   }
+  if (a.sums && a.is3d) {
+    // turbulenceGenerator.F90:57-61 linearised: du = fk u - g Us / N with fk = eps N / (4 ET)
+    // and ET = sum(u u + v v + w w), so d(du) = fk du' + u dfk - g dUs / N with
+    // dfk = -eps N (2 sum(u u' + v v' + w w')) / (4 ET^2); dE = -g (Es / N - e0) gives -g dEs / N
+    const double* sm = a.sums + 8 * blockIdx.y;
+    const double* ls = a.lsums + 8 * blockIdx.y;
+    const double ET = sm[0], Nd = sm[5];
+    const double gam = 10 * a.forcing_eps;
+    const double fk = 0.25 * a.forcing_eps * Nd / ET;
+    const double dfk = -(0.25 * a.forcing_eps * Nd * (2.0 * ls[0]) / (ET * ET));
+    d[0] = d[0] + (fk * tU + dfk * b.U) - gam * ls[1] / Nd;
+    d[1] = d[1] + (fk * tV + dfk * b.V) - gam * ls[2] / Nd;
+    d[2] = d[2] + (fk * ld_nt(dS + 2 * np + n) + dfk * b.W) - gam * ls[3] / Nd;
+    d[4] = d[4] - gam * ls[4] / Nd;
+  }
   const double* Q = a.dQ + lo;
   double* O = a.dOut + lo;
   double* Dl = a.dDelta + lo;
@@ -512,6 +527,21 @@ __global__ __launch_bounds__(256) void assemble_adj_kernel(AssembleLinArgs a) {
   DVyb += divb;
   DWzb += divb;
 
+  if (a.sums && a.is3d) {
+    // transpose of the linearised forcing (see assemble_tl_kernel): the point-wise fk terms,
+    // ET's adjoint (2 u per point times sum(a_u u + a_v v + a_w w) dfk/dET) and the means'
+    // adjoints (-g sum(a) / N on every point)
+    const double* sm = a.sums + 8 * blockIdx.y;
+    const double* ls = a.lsums + 8 * blockIdx.y;
+    const double ET = sm[0], Nd = sm[5];
+    const double gam = 10 * a.forcing_eps;
+    const double fk = 0.25 * a.forcing_eps * Nd / ET;
+    const double ETb = -(0.25 * a.forcing_eps * Nd / (ET * ET)) * ls[0];
+    Ub += fk * aU + 2.0 * b.U * ETb - gam * ls[1] / Nd;
+    Vb += fk * aV + 2.0 * b.V * ETb - gam * ls[2] / Nd;
+    Wb += fk * aW + 2.0 * b.W * ETb - gam * ls[3] / Nd;
+    Eb += -(gam * ls[4] / Nd);
+  }
   double* Sb = a.Sbar + lo;   // the stage-state adjoint starts here (later kernels accumulate)
   st_nt(Sb + 0 * np + n, Ub);
   st_nt(Sb + 1 * np + n, Vb);
@@ -539,6 +569,41 @@ __global__ __launch_bounds__(256) void assemble_adj_kernel(AssembleLinArgs a) {
     if (a.is3d) st_nt(g + 2 * np, vtzb);
     st_nt(g + 3 * np, -Phib);
   }
+}
+
+// Linear sums of the forcing: block partials, then the second stage of the forcing sums.
+__global__ __launch_bounds__(256) void lin_sums_partial_kernel(LinSumArgs a) {
+  __shared__ double sh[5][256];
+  const long long np = a.np;
+  const double* S = a.S + (long long)blockIdx.y * a.base_stride;
+  const double* X = a.X + (long long)blockIdx.y * a.lin_stride;
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < np; n += (long long)gridDim.x * blockDim.x) {
+    const double xu = ld_nt(X + n), xv = ld_nt(X + np + n), xw = ld_nt(X + 2 * np + n);
+    acc[0] += xu * ld_nt(S + n) + xv * ld_nt(S + np + n) + xw * ld_nt(S + 2 * np + n);
+    acc[1] += xu;
+    acc[2] += xv;
+    acc[3] += xw;
+    acc[4] += ld_nt(X + 4 * np + n);
+  }
+  for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) a.partial[((long long)blockIdx.y * a.nblocks + blockIdx.x) * 5 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+void launch_lin_sums(const LinSumArgs& a, int nmembers, hipStream_t st) {
+  hipLaunchKernelGGL(lin_sums_partial_kernel, dim3(a.nblocks, nmembers), dim3(256), 0, st, a);
+  SumArgs f{};
+  f.np = a.np;
+  f.partial = a.partial;
+  f.sums = a.sums;
+  f.nblocks = a.nblocks;
+  launch_forcing_final(f, nmembers, st);
 }
 
 void launch_assemble_tl(const AssembleLinArgs& a, int nmembers, hipStream_t st) {

@@ -156,6 +156,7 @@ struct gatt_ctx {
   bool fuse_rhs = false;           // y flux derivatives, assembly and stage update in one kernel
   bool sums_from_assemble = false; // the last assemble left the next stage's partials there
   double* d_sums = nullptr;
+  double* d_lsums = nullptr;       // linear sums of the linearised forcing (8 per member)
   int sum_blocks = 0;
   // SFD
   int sfdType = 0;
@@ -1327,6 +1328,9 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
     HIPCHK(hipMalloc(&p, sizeof(double) * 8 * batch));
     c->dev_allocs.push_back(p);
     c->d_sums = static_cast<double*>(p);
+    HIPCHK(hipMalloc(&p, sizeof(double) * 8 * batch));
+    c->dev_allocs.push_back(p);
+    c->d_lsums = static_cast<double*>(p);
   }
   c->sfdType = d->sfdType;
   c->sfdDelta = d->sfdDelta;
@@ -2634,7 +2638,6 @@ int ensure_tape(gatt_ctx* c) {
 double* tape_ptr(gatt_ctx* c, int s, int wk) { return c->tape + ((long long)s * WK_N + wk) * c->np; }
 
 int lin_supported(gatt_ctx* c) {
-  if (c->forcing) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps do not cover the turbulence forcing");
   if (c->stepping != 1) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps follow the RK4 integrator only");
   if (c->timed) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps do not cover time-dependent disturbances");
   if (c->pg) return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps do not cover the pressureGradient forcing");
@@ -2733,6 +2736,37 @@ void base_dtg(gatt_ctx* c, const double* S, long long ss, int nm, double* D, dou
   }
 }
 
+// The turbulence forcing of the linearised steps (turbulenceGenerator, 3-D).
+bool lin_forcing(const gatt_ctx* c) { return c->forcing && c->is3d; }
+
+// The forcing sums of the base stage state S (member stride ss) of every member into d_sums.
+void base_forcing_sums(gatt_ctx* c, const double* S, long long ss) {
+  if (!lin_forcing(c)) return;
+  SumArgs sa{};
+  sa.np = c->np;
+  sa.member_stride = ss;
+  sa.S = S;
+  sa.sums = c->d_sums;
+  sa.partial = c->d_partial;
+  sa.nblocks = c->sum_blocks;
+  launch_forcing_sums(sa, c->batch, c->stream);
+}
+
+// The linear sums of X (lin stride) against the base S (stride ss) of every member into d_lsums.
+void lin_forcing_sums(gatt_ctx* c, const double* S, long long ss, const double* X) {
+  if (!lin_forcing(c)) return;
+  LinSumArgs la{};
+  la.np = c->np;
+  la.base_stride = ss;
+  la.lin_stride = c->lin_stride;
+  la.S = S;
+  la.X = X;
+  la.partial = c->d_partial;
+  la.sums = c->d_lsums;
+  la.nblocks = c->sum_blocks;
+  launch_lin_sums(la, c->batch, c->stream);
+}
+
 // forward assembly of one stage for nm members from member m0 (the tail of
 // run_rhs_stage) from the derivative arrays D, T, G (member stride ws)
 void forward_assemble(gatt_ctx* c, double* S, double* Q, double* Out, int stage, double dt, int m0, int nm,
@@ -2750,7 +2784,10 @@ void forward_assemble(gatt_ctx* c, double* S, double* Q, double* Out, int stage,
   aa.D = D;
   aa.T = T;
   aa.G = G;
-  aa.sums = nullptr;
+  // the turbulence forcing with the sums base_forcing_sums left for this stage state
+  aa.sums = lin_forcing(c) ? c->d_sums + 8LL * m0 : nullptr;
+  aa.forcing_eps = c->f_eps;
+  aa.forcing_e0 = c->f_e0;
   aa.stage = stage;
   aa.dt = dt;
   aa.is3d = c->is3d;
@@ -2824,6 +2861,11 @@ void tl_stage(gatt_ctx* c, const double* S, double* dS, double* dQ, double* dOut
   a.stage = stage;
   a.dt = dt;
   a.is3d = c->is3d;
+  if (lin_forcing(c)) {
+    a.sums = c->d_sums + 8LL * m0;
+    a.lsums = c->d_lsums + 8LL * m0;
+    a.forcing_eps = c->f_eps;
+  }
   launch_assemble_tl(a, nm, c->stream);
 }
 
@@ -2939,6 +2981,14 @@ void adj_rhs(gatt_ctx* c, const double* S, long long ss, double* kbar, double* s
     bool shared = true;
     for (int d = 0; d < ndir; ++d) shared = shared && solve_t_one_launch(c, d, GATT_DERIV, nflux);
     a.gbar_copies = shared ? 1 : ndir;
+    if (lin_forcing(c)) {
+      // the forcing's sums of the base stage state and of the RHS adjoint against it
+      base_forcing_sums(c, S, ss);
+      lin_forcing_sums(c, S, ss, kbar + lo);
+      a.sums = c->d_sums;
+      a.lsums = c->d_lsums;
+      a.forcing_eps = c->f_eps;
+    }
     launch_assemble_adj(a, nm, c->stream);
     // Tbar += d^T Gbar, direction by direction; the heat fluxes get their one contribution
     // here (assemble_adj leaves them unset)
@@ -3046,6 +3096,9 @@ void enqueue_tlm_step(gatt_ctx* c, double t, double dt) {
     double* Out = stage == 4 ? qn : S;
     double* dSin = stage == 1 ? dq : dS;
     double* dOut = stage == 4 ? dq : dS;
+    // the forcing's sums of the base stage state and of the perturbation against it
+    base_forcing_sums(c, Sin, c->slab);
+    lin_forcing_sums(c, Sin, c->slab, dSin);
     for (int m0 = 0; m0 < c->batch; m0 += c->wgroup) {
       const int nm = std::min(c->wgroup, c->batch - m0);
       base_dtg(c, Sin + (long long)m0 * c->slab, c->slab, nm, c->work_ptr(WK_D), c->work_ptr(WK_T),
@@ -3117,32 +3170,40 @@ void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
     // matter, so the stage states go straight into B1..B4 (no copies), and the filters
     // read B4 and write the step's result into qn
     base_dtg(c, q, sl, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
+    base_forcing_sums(c, q, sl);
     forward_assemble(c, q, q, B1, 1, dt, 0, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
     run_bc(c, B1, dt / 2.0, undo_log(c, 0));
     base_dtg(c, B1, ls, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
+    base_forcing_sums(c, B1, ls);
     forward_assemble(c, B1, q, B2, 2, dt, 0, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
     run_bc(c, B2, dt / 2.0, undo_log(c, 1));
     base_dtg(c, B2, ls, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
+    base_forcing_sums(c, B2, ls);
     forward_assemble(c, B2, q, B3, 3, dt, 0, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
     run_bc(c, B3, dt, undo_log(c, 2));
     base_dtg(c, B3, ls, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
+    base_forcing_sums(c, B3, ls);
     forward_assemble(c, B3, q, B4, 4, dt, 0, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
     run_bc(c, B4, dt, undo_log(c, 3));
     run_filters_from(c, B4, qn, dt / c->filterCharTime);
   } else {
     base_dtg(c, q, sl, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
+    base_forcing_sums(c, q, sl);
     forward_assemble(c, q, q, S, 1, dt, 0, nb, tape_ptr(c, 0, WK_D), tape_ptr(c, 0, WK_T), tape_ptr(c, 0, WK_G), ts);
     run_bc(c, S, dt / 2.0, undo_log(c, 0));
     fields_axpby(c, B1, ls, S, sl, 1.0, 0.0);
     base_dtg(c, S, sl, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
+    base_forcing_sums(c, S, sl);
     forward_assemble(c, S, q, S, 2, dt, 0, nb, tape_ptr(c, 1, WK_D), tape_ptr(c, 1, WK_T), tape_ptr(c, 1, WK_G), ts);
     run_bc(c, S, dt / 2.0, undo_log(c, 1));
     fields_axpby(c, B2, ls, S, sl, 1.0, 0.0);
     base_dtg(c, S, sl, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
+    base_forcing_sums(c, S, sl);
     forward_assemble(c, S, q, S, 3, dt, 0, nb, tape_ptr(c, 2, WK_D), tape_ptr(c, 2, WK_T), tape_ptr(c, 2, WK_G), ts);
     run_bc(c, S, dt, undo_log(c, 2));
     fields_axpby(c, B3, ls, S, sl, 1.0, 0.0);
     base_dtg(c, S, sl, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
+    base_forcing_sums(c, S, sl);
     forward_assemble(c, S, q, qn, 4, dt, 0, nb, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
     run_bc(c, qn, dt, undo_log(c, 3));
     fields_axpby(c, B4, ls, qn, sl, 1.0, 0.0);

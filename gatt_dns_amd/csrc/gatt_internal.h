@@ -411,8 +411,25 @@ struct AssembleLinArgs {
   double* Dbar; double* Tbar; double* Gbar;   // written (lin stride): Tbar 0-5, Gbar 4 per copy
   int gbar_copies;      // flux-derivative adjoints stored once for all directions (1) or per direction
   int is3d;
+  // turbulenceGenerator (3-D): the base state's sums (ET, U, V, W, E, N; 8 per member of the
+  // launch) and the linear sums of lin_sums (8 per member), or null without the forcing
+  const double* sums;
+  const double* lsums;
+  double forcing_eps;
 };
 void launch_assemble_tl(const AssembleLinArgs& a, int nmembers, hipStream_t st);
+// Global sums of a linearised field X against the base state S for the turbulence forcing:
+// sums[8 m + 0..4] = sum(X_u S_u + X_v S_v + X_w S_w), sum X_u, sum X_v, sum X_w, sum X_e, and
+// sums[8 m + 5] = np; deterministic two-stage reduction through `partial` (nblocks x 5 per member).
+struct LinSumArgs {
+  long long np, base_stride, lin_stride;
+  const double* S;
+  const double* X;
+  double* partial;
+  double* sums;
+  int nblocks;
+};
+void launch_lin_sums(const LinSumArgs& a, int nmembers, hipStream_t st);
 void launch_assemble_adj(const AssembleLinArgs& a, int nmembers, hipStream_t st);
 
 // y[f] = a*x[f] + b*y[f] over nf arrays of np points per member (b == 0 writes a*x)

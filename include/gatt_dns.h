@@ -386,8 +386,11 @@ int gatt_block_update(gatt_ctx* ctx, int64_t n, int32_t k, int32_t p, const doub
  * sweeps with the transposed periodic correction and the transposed RHS band,
  * the filter blend, the SFD damping with a frozen mean and the boundary
  * program (Dirichlet, Neumann and null-second-derivative boxes, pressure
- * conditions through R = P/E, corners, lid and held inflow).  The turbulence
- * forcing and an evolving SFD mean (Delta > 0) return GATT_E_UNSUPPORTED.
+ * conditions through R = P/E, corners, lid and held inflow) and the
+ * turbulenceGenerator forcing (its global sums differentiated: the energy sum's
+ * and the means' tangents are global sums of the perturbation against the base
+ * state, their adjoints global sums of the RHS adjoint).  An evolving SFD mean
+ * (Delta > 0) returns GATT_E_UNSUPPORTED.
  * ------------------------------------------------------------------------ */
 enum { GATT_SLOT_STATE = 0, GATT_SLOT_PERTURBATION = 1 };
 

@@ -1,7 +1,8 @@
 """GPU checks of the tangent-linear and adjoint steps.
 
 * dot-product test <J v, w> = <v, J^T w> within 1e-11 (one step and a
-  checkpointed multi-step propagator): 2-D and 3-D periodic boxes, the lid-driven
+  checkpointed multi-step propagator): 2-D and 3-D periodic boxes (3-D also with the
+  turbulenceGenerator forcing, whose global sums the linearised steps differentiate), the lid-driven
   cavity (walls, corners, lid) and the boundary layer (inflow/outflow/top boxes,
   free-slip segment, buffers, SFD with a frozen mean);
 * the tangent against central differences of the oracle's step (the
@@ -64,7 +65,8 @@ def _case(kind):
         # segment, EX4 buffers, SFD with a frozen mean in the buffers
         c = cases.boundary_layer(nx=96, ny=48, nbuf=10, sfd_delta=-1.0)
         return c, {k: np.array(v, copy=True, order="F") for k, v in c.state0.items()}
-    c = cases.turbulence_box(16, seed=3, forcing=False)
+    # box3d: at rest of any forcing; box3d_forced: with turbulenceGenerator (global sums)
+    c = cases.turbulence_box(16, seed=3, forcing=kind == "box3d_forced")
     q = {k: np.array(v, copy=True) for k, v in c.state0.items()}
     q["R"] = q["R"] + 0.03 * np.sin(2 * np.pi * np.arange(16) / 16)[None, :, None]
     return c, q
@@ -79,7 +81,7 @@ def _rand(prop, seed):
     return torch.from_numpy(v).cuda()
 
 
-@pytest.mark.parametrize("kind", ["box2d", "box3d", "cavity", "bl", "bl3d"])
+@pytest.mark.parametrize("kind", ["box2d", "box3d", "box3d_forced", "cavity", "bl", "bl3d"])
 def test_dot_product_one_step(torch_cuda, kind):
     from gatt_dns_amd.adjoint import Propagator, state_tensor
     case, q = _case(kind)
@@ -95,7 +97,7 @@ def test_dot_product_one_step(torch_cuda, kind):
     prop.close()
 
 
-@pytest.mark.parametrize("kind", ["box2d", "box3d", "cavity", "bl", "bl3d"])
+@pytest.mark.parametrize("kind", ["box2d", "box3d", "box3d_forced", "cavity", "bl", "bl3d"])
 def test_tangent_matches_oracle_differences(torch_cuda, kind, oracle_mod):
     from gatt_dns_amd.adjoint import Propagator, state_tensor
     case, q = _case(kind)
@@ -147,7 +149,7 @@ def test_transposed_solves_are_transposes(torch_cuda, name):
     s.close()
 
 
-@pytest.mark.parametrize("kind", ["box2d", "box3d", "cavity", "bl"])
+@pytest.mark.parametrize("kind", ["box2d", "box3d", "box3d_forced", "cavity", "bl"])
 def test_checkpointed_propagator_dot_product_and_gradient(torch_cuda, kind):
     import torch
     from gatt_dns_amd.adjoint import Propagator, advance, optimal_energy_gradient, state_tensor
