@@ -199,3 +199,32 @@ def test_linearised_steps_need_rk4(torch_cuda, stepping):
         s.tlm_step(0.0, c.dt)
     with pytest.raises(runtime.GattError, match="RK4"):
         s.adj_step(0.0, c.dt)
+
+
+@pytest.mark.parametrize("kind", ["box3d_forced", "bl3d"])
+def test_members_linearised_alone(torch_cuda, kind):
+    """Two members of one context (different base states and perturbations; the forcing's sums
+    per member) advance their tangent and adjoint steps as each does alone, bit for bit."""
+    from gatt_dns_amd import runtime
+    case, q = _case(kind)
+    rng = np.random.Generator(np.random.PCG64(17))
+    q2 = {k: np.asfortranarray(v * (1.0 + 1e-3 * rng.standard_normal(v.shape))) for k, v in q.items()}
+    bases = [q, q2]
+    perts = [{k: np.asfortranarray(rng.standard_normal(q[k].shape)) for k in "UVWRE"} for _ in range(2)]
+    for step in ("tlm", "adj"):
+        pair = runtime.Solver(case, batch=2)
+        for m in range(2):
+            pair.set_state(bases[m], member=m)
+            pair.set_perturbation(perts[m], member=m)
+        getattr(pair, step + "_step")(0.0, case.dt)
+        together = [pair.get_perturbation(member=m) for m in range(2)]
+        pair.close()
+        for m in range(2):
+            one = runtime.Solver(case)
+            one.set_state(bases[m])
+            one.set_perturbation(perts[m])
+            getattr(one, step + "_step")(0.0, case.dt)
+            alone = one.get_perturbation()
+            one.close()
+            for k in "UVWRE":
+                assert np.array_equal(together[m][k], alone[k]), (kind, step, m, k)
