@@ -1214,7 +1214,11 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
     HIPCHK(hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming));
   }
   HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-  // GATT_SERIAL_DIRECTIONS=1 keeps every launch on one stream (for comparisons)
+  // The direction solves of a pass run as parallel branches where the pass is large enough
+  // for them to overlap (at least 2^20 points over the members); on smaller grids the fork and
+  // join cost more than the overlap saves.  GATT_SERIAL_DIRECTIONS=1 (or 0) forces one stream
+  // (or the branches) for comparisons.
+  c->fork_dirs = c->np * batch >= (1LL << 20);
   if (const char* e = std::getenv("GATT_SERIAL_DIRECTIONS")) c->fork_dirs = e[0] != '1';
   {
     int dev = 0, v = 0;

@@ -400,6 +400,24 @@ def test_advance_equals_steps(rt):
         assert np.array_equal(ga[k], gb[k]), k
 
 
+@pytest.mark.parametrize("name", ["box32", "bl160"])
+def test_direction_branches_equal_serial(rt, name, monkeypatch):
+    """The direction solves of a pass as parallel graph branches (the default at 2^20 points
+    and more, forced here on small grids) give bit-identical states to one stream."""
+    case = CASES["box32"]() if name == "box32" else cases.boundary_layer(nx=160, ny=64, nbuf=12)
+    states = []
+    for serial in ("0", "1"):
+        monkeypatch.setenv("GATT_SERIAL_DIRECTIONS", serial)
+        s = rt.Solver(case)
+        s.set_state(case.state0)
+        s.apply_bound_cond(0.0)
+        s.advance(4, 0.0, case.dt)
+        states.append(s.get_state())
+        s.close()
+    for k in FIELDS3:
+        assert np.array_equal(states[0][k], states[1][k]), (name, k)
+
+
 def test_box_steps(rt, oracle_mod):
     case = CASES["box32"]()
     o = oracle_mod.Oracle(case)
