@@ -9,24 +9,38 @@ checkpoint, recomputes the states of its segment and applies the adjoint
 steps in reverse.  The tangent (``gatt_tlm_step``) is exposed for
 dot-product and finite-difference checks.
 
+Where the SFD mean evolves (SFD on with Delta > 0, main.F90:382-388) the mean is
+part of the dynamical state: the tensors then hold ten fields, the state's
+U,V,W,R,E followed by the mean's, and every map (the step, its tangent, its
+adjoint) acts on the pair.
+
 Torch provides device memory and the autograd graph only; every arithmetic
 operation of the step and of its adjoint runs in the library's HIP kernels.
 """
 import numpy as np
 import torch
 
-from .runtime import SLOT_PERTURBATION, SLOT_STATE, Solver
+from .runtime import SLOT_MEAN, SLOT_MEAN_PERTURBATION, SLOT_PERTURBATION, SLOT_STATE, Solver
+
+
+def evolving_mean(case):
+    """True where the SFD mean is part of the state (SFD on, Delta > 0)."""
+    return case.sfd_type > 0 and case.sfd_delta > 0
 
 
 class Propagator:
-    """One-member context of a case with fixed dt; moves states between torch and the library."""
+    """One-member context of a case with fixed dt; moves states between torch and the library.
+
+    A state tensor has shape (5, nz, ny, nx), or (10, nz, ny, nx) where the SFD mean evolves
+    (the mean's five fields after the state's)."""
 
     def __init__(self, case, dt=None, t0=0.0):
         self.case = case
         self.solver = Solver(case)
         self.dt = float(case.dt if dt is None else dt)
         self.t0 = float(t0)
-        self.shape = (5, case.nz, case.ny, case.nx)
+        self.with_mean = evolving_mean(case)
+        self.shape = (10 if self.with_mean else 5, case.nz, case.ny, case.nx)
 
     def empty(self):
         return torch.empty(self.shape, dtype=torch.float64, device="cuda")
@@ -34,25 +48,35 @@ class Propagator:
     def _sync_torch(self):
         torch.cuda.current_stream().synchronize()
 
-    def load(self, q):
+    def _copy(self, slots, to_ctx, t):
         self._sync_torch()
-        self.solver.copy_fields(SLOT_STATE, True, q.contiguous())
+        self.solver.copy_fields(slots[0], to_ctx, t[:5])
+        if self.with_mean:
+            self.solver.copy_fields(slots[1], to_ctx, t[5:])
+
+    def load(self, q):
+        self._copy((SLOT_STATE, SLOT_MEAN), True, q.contiguous())
 
     def save(self):
         out = self.empty()
-        self._sync_torch()
-        self.solver.copy_fields(SLOT_STATE, False, out)
+        self._copy((SLOT_STATE, SLOT_MEAN), False, out)
         return out
 
     def load_perturbation(self, v):
-        self._sync_torch()
-        self.solver.copy_fields(SLOT_PERTURBATION, True, v.contiguous())
+        self._copy((SLOT_PERTURBATION, SLOT_MEAN_PERTURBATION), True, v.contiguous())
 
     def save_perturbation(self):
         out = self.empty()
-        self._sync_torch()
-        self.solver.copy_fields(SLOT_PERTURBATION, False, out)
+        self._copy((SLOT_PERTURBATION, SLOT_MEAN_PERTURBATION), False, out)
         return out
+
+    def state_tensor(self, q, mean=None):
+        """Host fields -> this propagator's state tensor; where the mean evolves, `mean` (default:
+        the state itself, the mean the solver starts from) follows the state's fields."""
+        t = state_tensor(self.case, q)
+        if not self.with_mean:
+            return t
+        return torch.cat([t, state_tensor(self.case, q if mean is None else mean)])
 
     def steps(self, n, t=None):
         if n > 0:
@@ -67,7 +91,7 @@ class Propagator:
         return self.save(), self.save_perturbation()
 
     def adjoint(self, q, w):
-        """J(q)^T w for one step at q."""
+        """J(q)^T w for one step at q (q and w with the mean's fields where the mean evolves)."""
         self.load(q)
         self.load_perturbation(w)
         self.solver.adj_step(self.t0, self.dt)

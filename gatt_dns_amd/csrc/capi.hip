@@ -1113,7 +1113,8 @@ void run_filters_from(gatt_ctx* c, const double* src, double* dst, double theta)
     for (int b = 0; b < c->batch; ++b) launch_copy(dst + b * c->slab, src + b * c->slab, 5 * c->np, c->stream);
 }
 
-void run_sfd(gatt_ctx* c, double* q, double dt) {
+// mean_ro: the mean is read, not updated (the adjoint's recompute of a base step about a given mean)
+void run_sfd(gatt_ctx* c, double* q, double dt, bool mean_ro = false) {
   if (c->sfdType <= 0) return;
   SfdArgs s{};
   s.np = c->np;
@@ -1125,6 +1126,7 @@ void run_sfd(gatt_ctx* c, double* q, double dt) {
   s.dt = dt;
   s.delta = c->sfdDelta;
   s.nf_mask = c->is3d ? 0x1f : 0x1b;
+  s.mean_ro = mean_ro ? 1 : 0;
   launch_sfd(s, c->batch, c->stream);
 }
 
@@ -1496,15 +1498,17 @@ int gatt_get_points(gatt_ctx* c, int32_t member, int32_t npts, const int64_t* id
 int gatt_set_mean(gatt_ctx* c, int32_t member, const double* const* q) {
   if (check_ctx(c) || !q) return fail(GATT_E_INVALID, "null argument");
   if (c->sfdType <= 0) return fail(GATT_E_INVALID, "SFD is off: no mean to set");
-  int rc = copy_fields(c, member, c->slab_ptr(SL_MEAN), q, true);
-  if (!rc) c->mean_ready = true;
-  return rc;
+  // the other members keep the state as their mean, as at the first step
+  ensure_mean(c);
+  return copy_fields(c, member, c->slab_ptr(SL_MEAN), q, true);
 }
 
 int gatt_get_mean(gatt_ctx* c, int32_t member, double* const* q) {
   if (check_ctx(c) || !q) return fail(GATT_E_INVALID, "null argument");
   if (c->sfdType <= 0) return fail(GATT_E_INVALID, "SFD is off: no mean to read");
-  return copy_fields(c, member, c->slab_ptr(SL_MEAN), (const double* const*)q, false);
+  // before the mean is set or first used it is the state
+  return copy_fields(c, member, c->mean_ready ? c->slab_ptr(SL_MEAN) : c->state(c->cur), (const double* const*)q,
+                     false);
 }
 
 int gatt_rk4(gatt_ctx* c, double t, double dt) {
@@ -2600,7 +2604,8 @@ enum : int {
   LN_T = 45,    // 9 stress and heat-flux tangents / adjoints
   LN_G = 54,    // 12 second-derivative tangents / adjoints
   LN_B4 = 66,   // adjoint: the base step's state after the stage-4 boundary conditions
-  LN_N = 71
+  LN_M = 71,    // tangent / adjoint of the SFD mean (evolving mean, Delta > 0)
+  LN_N = 76
 };
 
 // the boundary programs of one base step write five undo logs (stages 1..4 and the
@@ -2648,8 +2653,6 @@ int lin_supported(gatt_ctx* c) {
   for (int d = 0; d < (c->is3d ? 3 : 2); ++d)
     if (c->dir[d].identity[0] || c->dir[d].identity[1])
       return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps need a line operator in every direction");
-  if (c->sfdType > 0 && c->sfdDelta > 0)
-    return fail(GATT_E_UNSUPPORTED, "tangent and adjoint steps need a frozen SFD mean (Delta = inf)");
   const int ndir = c->is3d ? 3 : 2;
   for (int d = 0; d < ndir; ++d)
     if (line_solve_t_points_per_lane(c->dir[d].op[0].N) == 0)
@@ -3086,6 +3089,23 @@ void run_bc_adj(gatt_ctx* c, double* post, long long ps, double* lam, const doub
   }
 }
 
+// The SFD blend of main.F90:382-401 on the tangents (transpose = false) or the adjoints
+// (transpose = true) of the state and, where the mean evolves (Delta > 0), of the mean.
+SfdArgs sfd_lin_args(gatt_ctx* c, double dt, bool transpose) {
+  SfdArgs sa{};
+  sa.np = c->np;
+  sa.member_stride = c->lin_stride;
+  sa.Q = lin_ptr(c, LN_Q);
+  sa.Mean = c->sfdDelta > 0 ? lin_ptr(c, LN_M) : nullptr;
+  sa.chi = c->sfdType == 2 ? c->d_chi : nullptr;
+  sa.chi_val = c->sfdXval;
+  sa.dt = dt;
+  sa.delta = c->sfdDelta > 0 ? c->sfdDelta : 0.0;
+  sa.nf_mask = c->is3d ? 0x1f : 0x1b;
+  sa.transpose = transpose ? 1 : 0;
+  return sa;
+}
+
 // One step of the base state with its tangent (main.F90:255-420 and its derivative).
 void enqueue_tlm_step(gatt_ctx* c, double t, double dt) {
   (void)t;
@@ -3125,17 +3145,9 @@ void enqueue_tlm_step(gatt_ctx* c, double t, double dt) {
   }
   if (c->sfdType > 0) {
     run_sfd(c, qn, dt);
-    SfdArgs sa{};
-    sa.np = c->np;
-    sa.member_stride = c->lin_stride;
-    sa.Q = dq;
-    sa.Mean = nullptr;
-    sa.chi = c->sfdType == 2 ? c->d_chi : nullptr;
-    sa.chi_val = c->sfdXval;
-    sa.dt = dt;
-    sa.delta = 0.0;
-    sa.nf_mask = c->is3d ? 0x1f : 0x1b;
-    launch_sfd(sa, c->batch, c->stream);
+    // the blend is linear in (q, mean): the same kernel on the tangents, with the mean's
+    // tangent where the mean evolves (Delta > 0), a zero mean tangent where it is frozen
+    launch_sfd(sfd_lin_args(c, dt, false), c->batch, c->stream);
   }
   run_bc_tl(c, qn, c->slab, dq, c->lin_stride);
   ReduceArgs r{};
@@ -3213,24 +3225,13 @@ void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
     fields_axpby(c, B4, ls, qn, sl, 1.0, 0.0);
     run_filters(c, qn, dt / c->filterCharTime);
   }
-  run_sfd(c, qn, dt);
+  run_sfd(c, qn, dt, true);   // the base mean stays as given
   run_bc(c, qn, dt, undo_log(c, 4));
   // the final boundary conditions (main.F90:405), walked back on qn
   run_bc_adj(c, qn, sl, QB, undo_log(c, 4));
-  // SFD with a frozen mean is diagonal
-  if (c->sfdType > 0) {
-    SfdArgs sa{};
-    sa.np = c->np;
-    sa.member_stride = ls;
-    sa.Q = QB;
-    sa.Mean = nullptr;
-    sa.chi = c->sfdType == 2 ? c->d_chi : nullptr;
-    sa.chi_val = c->sfdXval;
-    sa.dt = dt;
-    sa.delta = 0.0;
-    sa.nf_mask = c->is3d ? 0x1f : 0x1b;
-    launch_sfd(sa, c->batch, c->stream);
-  }
+  // the transposed blend: diagonal with a frozen mean, coupled to the mean's adjoint where
+  // the mean evolves
+  if (c->sfdType > 0) launch_sfd(sfd_lin_args(c, dt, true), c->batch, c->stream);
   // filters in reverse order: qbar <- (1-theta) qbar + theta F^T qbar (theta <= 0: F^T qbar)
   const double theta = dt / c->filterCharTime;
   const int ndir = c->is3d ? 3 : 2;
@@ -3311,10 +3312,15 @@ int gatt_copy_fields(gatt_ctx* c, int32_t member, int32_t slot, int32_t to_ctx, 
   double* base;
   if (slot == GATT_SLOT_STATE) {
     base = c->state(c->cur) + (long long)member * c->slab;
-  } else if (slot == GATT_SLOT_PERTURBATION) {
+  } else if (slot == GATT_SLOT_PERTURBATION || slot == GATT_SLOT_MEAN_PERTURBATION) {
     int rc = ensure_lin(c);
     if (rc) return rc;
-    base = lin_ptr(c, LN_Q) + (long long)member * c->lin_stride;
+    base = lin_ptr(c, slot == GATT_SLOT_PERTURBATION ? LN_Q : LN_M) + (long long)member * c->lin_stride;
+  } else if (slot == GATT_SLOT_MEAN) {
+    if (c->sfdType <= 0) return fail(GATT_E_INVALID, "the problem has no SFD mean");
+    // before the mean is set or first used it is the state (every member's, as at the first step)
+    if (to_ctx) ensure_mean(c);
+    base = (c->mean_ready ? c->slab_ptr(SL_MEAN) : c->state(c->cur)) + (long long)member * c->slab;
   } else {
     return fail(GATT_E_INVALID, "unknown field slot");
   }

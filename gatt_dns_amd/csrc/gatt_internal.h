@@ -321,6 +321,8 @@ struct SfdArgs {
   double chi_val;
   double dt, delta;
   int nf_mask;          // bit f set => field f is damped
+  int mean_ro;          // 1: the mean is read and updated in registers only (Mean is not written)
+  int transpose;        // 1: the transpose of the blend on adjoints (Q: state adjoint, Mean: mean adjoint)
 };
 void launch_sfd(const SfdArgs& a, int nmembers, hipStream_t st);
 

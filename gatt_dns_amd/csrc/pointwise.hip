@@ -214,10 +214,26 @@ __device__ __forceinline__ void sfd_body(const SfdArgs& a, BlockPos bp) {
   for (int f = 0; f < 5; ++f) {
     if (!(a.nf_mask & (1 << f))) continue;
     const long long o = mo + f * a.np + n;
+    if (a.transpose) {
+      // transpose of  M' = M + dtD (q - M),  q' = q - dtX (q - M')  on the adjoints (qb', Mb'):
+      // Mb'' = Mb' + dtX qb',  qb = qb' - dtX qb' + dtD Mb'',  Mb = Mb'' - dtD Mb''
+      const double qb = a.Q[o];
+      if (a.delta > 0 && a.Mean) {
+        const double mb = a.Mean[o] + dtX * qb;
+        a.Q[o] = (qb - dtX * qb) + dtD * mb;
+        a.Mean[o] = mb - dtD * mb;
+      } else {
+        a.Q[o] = qb - dtX * qb;
+      }
+      continue;
+    }
     // without a mean array the blend acts on a perturbation of a frozen mean
     double M = a.Mean ? a.Mean[o] : 0.0;
     const double q = a.Q[o];
-    if (a.delta > 0 && a.Mean) { M = M + dtD * (q - M); a.Mean[o] = M; }
+    if (a.delta > 0 && a.Mean) {
+      M = M + dtD * (q - M);
+      if (!a.mean_ro) a.Mean[o] = M;
+    }
     a.Q[o] = q - dtX * (q - M);
   }
 }

@@ -32,6 +32,8 @@ EXPORTS = ["gatt_last_error", "gatt_device_info", "gatt_create", "gatt_destroy",
 
 SLOT_STATE = 0
 SLOT_PERTURBATION = 1
+SLOT_MEAN = 2
+SLOT_MEAN_PERTURBATION = 3
 
 
 class GattError(RuntimeError):
@@ -333,8 +335,9 @@ class Solver:
         return dict(zip("UVWRE", out))
 
     def copy_fields(self, slot, to_ctx, tensor, member=0):
-        """Device copy of the state (slot 0) or perturbation (slot 1) to/from a float64
-        CUDA tensor of 5*nx*ny*nz contiguous values (U,V,W,R,E, each x fastest)."""
+        """Device copy of the state (slot 0), perturbation (1), SFD mean (2) or the mean's
+        perturbation (3) to/from a float64 CUDA tensor of 5*nx*ny*nz contiguous values
+        (U,V,W,R,E, each x fastest)."""
         npts = self.shape[0] * self.shape[1] * self.shape[2]
         if (str(tensor.dtype) != "torch.float64" or not tensor.is_cuda or not tensor.is_contiguous()
                 or tensor.numel() != 5 * npts):

@@ -381,6 +381,10 @@ int gatt_block_update(gatt_ctx* ctx, int64_t n, int32_t k, int32_t p, const doub
  * Each member carries a perturbation q' (5 fields, same layout as the state).
  *   gatt_tlm_step : q <- Phi(q) and q' <- J(q) q'   (J = dPhi/dq at the old q)
  *   gatt_adj_step : q' <- J(q)^T q'                 (q is not advanced)
+ * Where the SFD mean evolves (SFD on, Delta > 0) the mean m is part of the
+ * state: Phi maps (q, m) to (q, m) after the step, the member also carries the
+ * mean's perturbation m' (slot GATT_SLOT_MEAN_PERTURBATION), both steps act on
+ * the pair (q', m'), and gatt_adj_step leaves the base mean as given.
  * The adjoint is the exact transpose of the tangent: RK4 stages, the
  * Navier-Stokes RHS (nseq.F90), the compact solves as transposed Thomas
  * sweeps with the transposed periodic correction and the transposed RHS band,
@@ -389,18 +393,19 @@ int gatt_block_update(gatt_ctx* ctx, int64_t n, int32_t k, int32_t p, const doub
  * conditions through R = P/E, corners, lid and held inflow) and the
  * turbulenceGenerator forcing (its global sums differentiated: the energy sum's
  * and the means' tangents are global sums of the perturbation against the base
- * state, their adjoints global sums of the RHS adjoint).  An evolving SFD mean
- * (Delta > 0) returns GATT_E_UNSUPPORTED.
+ * state, their adjoints global sums of the RHS adjoint).
  * ------------------------------------------------------------------------ */
-enum { GATT_SLOT_STATE = 0, GATT_SLOT_PERTURBATION = 1 };
+enum { GATT_SLOT_STATE = 0, GATT_SLOT_PERTURBATION = 1, GATT_SLOT_MEAN = 2, GATT_SLOT_MEAN_PERTURBATION = 3 };
 
 /* Host copies of member m's perturbation, as gatt_set_state / gatt_get_state. */
 int gatt_set_perturbation(gatt_ctx* ctx, int32_t member, const double* const* q);
 int gatt_get_perturbation(gatt_ctx* ctx, int32_t member, double* const* q);
 
-/* Device-to-device copy of member m's state or perturbation (slot) to or from
- * a caller buffer of 5*nx*ny*nz doubles (U,V,W,R,E one after the other):
- * to_ctx != 0 copies dev into the context.  For checkpoints kept on the GPU. */
+/* Device-to-device copy of member m's state, perturbation, SFD mean or the
+ * mean's perturbation (slot) to or from a caller buffer of 5*nx*ny*nz doubles
+ * (U,V,W,R,E one after the other): to_ctx != 0 copies dev into the context.
+ * For checkpoints kept on the GPU.  The mean slot needs SFD on; before it is
+ * set or first used, the mean reads as the state. */
 int gatt_copy_fields(gatt_ctx* ctx, int32_t member, int32_t slot, int32_t to_ctx, double* dev);
 
 /* Unit entry point: the transpose of gatt_line_solve along `dir` (the

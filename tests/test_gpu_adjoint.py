@@ -4,7 +4,8 @@
   checkpointed multi-step propagator): 2-D and 3-D periodic boxes (3-D also with the
   turbulenceGenerator forcing, whose global sums the linearised steps differentiate), the lid-driven
   cavity (walls, corners, lid) and the boundary layer (inflow/outflow/top boxes,
-  free-slip segment, buffers, SFD with a frozen mean);
+  free-slip segment, buffers, SFD with a frozen mean, and with an evolving mean, whose
+  five fields then belong to the state, the perturbation and the adjoint);
 * the tangent against central differences of the oracle's step (the
   reference's non-linear step restated on the CPU);
 * the transposed line solves against the transposes of the forward solves
@@ -77,8 +78,120 @@ def _rand(prop, seed):
     rng = np.random.Generator(np.random.PCG64(seed))
     v = rng.standard_normal(prop.shape)
     if prop.case.nz == 1:
-        v[2] = 0.0
+        v[2::5] = 0.0     # W (and the mean's W)
     return torch.from_numpy(v).cuda()
+
+
+SFD_KINDS = ["bl_sfd", "bl_sfd1", "bl3d_sfd"]
+
+
+def _sfd_case(kind):
+    """Boundary layers whose SFD mean evolves (Delta > 0): chi in the buffers (type 2) or
+    everywhere (type 1), a strong blend (chi = 0.5, Delta = 0.05, so dt/Delta is about 0.5 and
+    dt chi about 0.01) so that the coupling of state and mean weighs in every check, and a
+    mean that starts away from the state."""
+    nz = 12 if kind == "bl3d_sfd" else 1
+    kw = dict(nx=64, ny=40, nbuf=8, nz=nz) if nz > 1 else dict(nx=96, ny=48, nbuf=10)
+    c = cases.boundary_layer(sfd_chi=0.5, sfd_delta=0.05, sfd_type=1 if kind == "bl_sfd1" else 2, **kw)
+    q = {k: np.array(v, copy=True, order="F") for k, v in c.state0.items()}
+    rng = np.random.Generator(np.random.PCG64(23))
+    if nz > 1:
+        z = np.arange(nz) / nz
+        q["W"] = np.asfortranarray(q["W"] + 1e-3 * np.sin(2 * np.pi * z)[None, None, :] * c.flow_region)
+    mean = {k: np.asfortranarray(v * (1.0 + 1e-3 * rng.standard_normal(v.shape))) for k, v in q.items()}
+    if nz > 1:
+        mean["W"] = np.asfortranarray(q["W"] + 1e-4 * rng.standard_normal(q["W"].shape))
+    return c, q, mean
+
+
+@pytest.mark.parametrize("kind", SFD_KINDS)
+def test_evolving_mean_dot_product_one_step(torch_cuda, kind):
+    """<J (v, mu), (w, nu)> = <(v, mu), J^T (w, nu)> with the SFD mean in the state; the adjoint
+    step leaves the base state and the base mean as given."""
+    import torch
+    from gatt_dns_amd.adjoint import Propagator
+    case, q, mean = _sfd_case(kind)
+    prop = Propagator(case)
+    assert prop.shape[0] == 10
+    q0 = prop.state_tensor(q, mean)
+    for seed in (1, 2):
+        v, w = _rand(prop, seed), _rand(prop, seed + 10)
+        _, Jv = prop.tangent(q0, v)
+        JTw = prop.adjoint(q0, w)
+        assert torch.equal(prop.save(), q0)
+        a = float((Jv * w).sum())
+        b = float((v * JTw).sum())
+        assert abs(a - b) <= 1e-11 * max(abs(a), abs(b)), (a, b, abs(a - b) / abs(a))
+        # the mean's rows of the adjoint carry the coupling (not the identity)
+        assert float((JTw[5:] - w[5:]).abs().max()) > 1e-3
+    prop.close()
+
+
+@pytest.mark.parametrize("kind", SFD_KINDS)
+def test_evolving_mean_tangent_matches_oracle_differences(torch_cuda, kind, oracle_mod):
+    """The tangent of (state, mean) against central differences of the oracle's step, whose
+    SFD block updates the mean before the blend (main.F90:382-401)."""
+    from gatt_dns_amd.adjoint import Propagator
+    case, q, mean = _sfd_case(kind)
+    prop = Propagator(case)
+    q0 = prop.state_tensor(q, mean)
+    v = _rand(prop, 5)
+    _, Jv = prop.tangent(q0, v)
+    Jv = Jv.cpu().numpy()
+    vh = v.cpu().numpy()
+    eps = 1e-5
+    outs = []
+    for sgn in (1.0, -1.0):
+        o = oracle_mod.Oracle(case)
+        qq = {k: np.asfortranarray(q[k] + sgn * eps * np.transpose(vh[i], (2, 1, 0))) for i, k in enumerate("UVWRE")}
+        mm = {k: np.asfortranarray(mean[k] + sgn * eps * np.transpose(vh[5 + i], (2, 1, 0)))
+              for i, k in enumerate("UVWRE")}
+        o.init_mean(mm)
+        st, _, _, _ = o.step(qq, 0.0, case.dt)
+        mo = o.mean()
+        outs.append(np.stack([np.transpose(st[k], (2, 1, 0)) for k in "UVWRE"]
+                             + [np.transpose(mo[k], (2, 1, 0)) for k in "UVWRE"]))
+    fd = (outs[0] - outs[1]) / (2 * eps)
+    if case.nz == 1:
+        fd[2::5] = 0.0
+    for part in (slice(0, 5), slice(5, 10)):
+        rel = np.linalg.norm(Jv[part] - fd[part]) / np.linalg.norm(fd[part])
+        assert rel < 1e-7, (part, rel)
+    prop.close()
+
+
+@pytest.mark.parametrize("kind", ["bl_sfd", "bl3d_sfd"])
+def test_evolving_mean_checkpointed_gradient(torch_cuda, kind):
+    """torch.autograd through checkpointed steps of (state, mean) against finite differences, and
+    <lam, v> = ||J^n v||^2 through the tangent and the adjoint."""
+    import torch
+    from gatt_dns_amd.adjoint import Propagator, advance, optimal_energy_gradient
+    case, q, mean = _sfd_case(kind)
+    prop = Propagator(case)
+    q0 = prop.state_tensor(q, mean)
+    nsteps, every = 5, 2
+    qref = q0.clone()
+    x = q0.clone().requires_grad_(True)
+    J = 0.5 * ((advance(prop, x, nsteps, checkpoint_every=every) - qref) ** 2).sum()
+    J.backward()
+    g = x.grad.detach()
+    v = _rand(prop, 9) * 1e-3
+    eps = 1e-3
+
+    def Jof(z):
+        with torch.no_grad():
+            return float(0.5 * ((advance(prop, z, nsteps, checkpoint_every=every) - qref) ** 2).sum())
+
+    fd = (Jof(q0 + eps * v) - Jof(q0 - eps * v)) / (2 * eps)
+    an = float((g * v).sum())
+    assert abs(fd - an) <= 1e-7 * abs(an), (fd, an)
+    _, Jv = prop.tangent(q0, v, n=nsteps)
+    E, lam = optimal_energy_gradient(prop, q0, v, nsteps, checkpoint_every=every)
+    assert abs(E - 0.5 * float((Jv * Jv).sum())) <= 1e-12 * E
+    a = float((lam * v).sum())
+    b = float((Jv * Jv).sum())
+    assert abs(a - b) <= 1e-11 * abs(b), (a, b)
+    prop.close()
 
 
 @pytest.mark.parametrize("kind", ["box2d", "box3d", "box3d_forced", "cavity", "bl", "bl3d"])
