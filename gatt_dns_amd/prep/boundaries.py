@@ -9,7 +9,8 @@ types the time-advance configurations use:
   * ``source/boundaries/boundaryLayerIsothermal.m`` (and its Adiabatic, SymmZ and
     PressureInlet variants) -> :func:`boundary_layer_isothermal`
   * ``source/boundaries/boundaryLayerFreeSlip.m`` -> :func:`boundary_layer_free_slip`
-  * ``source/getBoundaryConditions.m`` -> :func:`get_boundary_conditions`
+  * the cavities and roughness elements every flow type cuts into its flow region
+    (``flowType.cav`` / ``flowType.rug``) -> :func:`cut_cavities_and_roughness`
   * ``source/initBoundaries.m`` (one rank) -> :func:`init_boundaries`
   * ``source/calcSFDregion.m`` -> :func:`calc_sfd_region`
 
@@ -34,6 +35,34 @@ def _runs(mask):
     starts = list(np.nonzero(np.concatenate([[m[0]], d == 1]))[0] + 1)
     ends = list(np.nonzero(np.concatenate([d == -1, [m[-1]]]))[0] + 1)
     return starts, ends
+
+
+def _xyz(el):
+    """(x, y, z) pairs of one cavity or roughness element (a dict or an object with x, y, z)."""
+    get = (lambda k: el[k]) if isinstance(el, dict) else (lambda k: getattr(el, k))
+    return [tuple(float(v) for v in get(k)) for k in "xyz"]
+
+
+def cut_cavities_and_roughness(fr, X, Y, Z, cavities=(), roughness=()):
+    """Open the cavities into the walls of a flow region and close the roughness elements, in
+    the order and with the bounds every flow type uses (``boundaryLayerIsothermal.m:338-358``,
+    ``lidDrivenFlow.m:5-25``, ``poiseuilleFlow.m:5-25``, ``boundaryLayerFreeSlip.m:346-366``):
+    a cavity sets the nodes strictly inside its x, y and z intervals to flow, a roughness
+    element sets the nodes inside or on its intervals to wall.  Each element has x, y and z
+    pairs (infinite ends allowed).  Modifies and returns ``fr`` (nx x ny x nz)."""
+    X, Y, Z = (np.asarray(a, dtype=float) for a in (X, Y, Z))
+    for el in cavities:
+        x, y, z = _xyz(el)
+        fr[np.ix_((X > x[0]) & (X < x[1]), (Y > y[0]) & (Y < y[1]), (Z > z[0]) & (Z < z[1]))] = True
+    for el in roughness:
+        x, y, z = _xyz(el)
+        fr[np.ix_((X >= x[0]) & (X <= x[1]), (Y >= y[0]) & (Y <= y[1]), (Z >= z[0]) & (Z <= z[1]))] = False
+    return fr
+
+
+def _need_xyz(X, Y, Z, cavities, roughness):
+    if (cavities or roughness) and (X is None or Y is None or Z is None):
+        raise ValueError("cavities and roughness elements need the mesh coordinates X, Y, Z")
 
 
 def find_walls_for_boundaries(flow_region):
@@ -213,9 +242,14 @@ class BCList:
         self.val.append(float(val)); self.box.append([int(v) for v in box])
 
 
-def lid_driven_flow(nx, ny, nz, E0, P0, t_wall_relative=None):
-    """BC list of ``source/boundaries/lidDrivenFlow.m`` (no cavities or roughness)."""
+def lid_driven_flow(nx, ny, nz, E0, P0, t_wall_relative=None, X=None, Y=None, Z=None, cavities=(),
+                    roughness=()):
+    """BC list of ``source/boundaries/lidDrivenFlow.m``; cavities and roughness elements (obstacles
+    in the cavity) as :func:`cut_cavities_and_roughness`, before the outer walls (``:5-29``)."""
     fr = np.ones((nx, ny, nz), dtype=bool)
+    _need_xyz(X, Y, Z, cavities, roughness)
+    if cavities or roughness:
+        cut_cavities_and_roughness(fr, X, Y, Z, cavities, roughness)
     fr[[0, -1], :, :] = False
     fr[:, [0, -1], :] = False
     fr, walls, corners, inside = find_walls_for_boundaries(fr)
@@ -233,10 +267,15 @@ def lid_driven_flow(nx, ny, nz, E0, P0, t_wall_relative=None):
     return fr, walls, corners, bc
 
 
-def poiseuille_flow(nx, ny, nz, E0, P0, lower_wall_velocity=0.0, upper_wall_velocity=0.0, t_wall_relative=None):
-    """BC list of ``source/boundaries/poiseuilleFlow.m`` (no cavities or roughness):
-    the first and last y rows are walls, with moving-wall U values (``:148-168``)."""
+def poiseuille_flow(nx, ny, nz, E0, P0, lower_wall_velocity=0.0, upper_wall_velocity=0.0, t_wall_relative=None,
+                    X=None, Y=None, Z=None, cavities=(), roughness=()):
+    """BC list of ``source/boundaries/poiseuilleFlow.m``: the first and last y rows are walls, with
+    moving-wall U values (``:148-168``); cavities and roughness elements as
+    :func:`cut_cavities_and_roughness`, before the outer walls (``:5-28``)."""
     fr = np.ones((nx, ny, nz), dtype=bool)
+    _need_xyz(X, Y, Z, cavities, roughness)
+    if cavities or roughness:
+        cut_cavities_and_roughness(fr, X, Y, Z, cavities, roughness)
     fr[:, [0, -1], :] = False
     fr, walls, corners, inside = find_walls_for_boundaries(fr)
     bc = BCList()
@@ -256,8 +295,9 @@ def poiseuille_flow(nx, ny, nz, E0, P0, lower_wall_velocity=0.0, upper_wall_velo
 
 
 def boundary_layer_isothermal(X, Y, Z, E0, P0, z_periodic=True, t_wall_relative=None, adiabatic=False,
-                              z_symmetry=False, pressure_inlet=False):
-    """BC list of ``source/boundaries/boundaryLayerIsothermal.m`` (no cavities or roughness);
+                              z_symmetry=False, pressure_inlet=False, cavities=(), roughness=()):
+    """BC list of ``source/boundaries/boundaryLayerIsothermal.m``, with the cavities and roughness
+    elements of ``:338-358`` cut into the plate (:func:`cut_cavities_and_roughness`);
     ``adiabatic=True`` gives ``boundaryLayerAdiabatic.m``: zero-gradient E on every wall
     box beside the pressure condition, adiabatic corners (``:363``), E0 inside the walls,
     and that file's z wall limits (``:387-392``: the left limits carry 'zi', the right 'zf').
@@ -303,6 +343,7 @@ def boundary_layer_isothermal(X, Y, Z, E0, P0, z_periodic=True, t_wall_relative=
     fr = np.ones((nx, ny, nz), dtype=bool)               # :328-332
     wall_j = int(np.argmin(np.abs(np.asarray(Y)))) + 1
     fr[:, :wall_j, :] = False
+    cut_cavities_and_roughness(fr, X, Y, Z, cavities, roughness)
     fr, walls, corners, inside = find_walls_for_boundaries(fr)
     bp = None
     if X[0] < 0:                                          # :363-369
@@ -335,8 +376,9 @@ def boundary_layer_isothermal(X, Y, Z, E0, P0, z_periodic=True, t_wall_relative=
 
 
 def boundary_layer_free_slip(X, Y, Z, E0, P0, x_periodic=False, y_periodic=False, z_periodic=True,
-                             t_wall_relative=None):
-    """BC list of ``source/boundaries/boundaryLayerFreeSlip.m`` (no cavities or roughness).
+                             t_wall_relative=None, cavities=(), roughness=()):
+    """BC list of ``source/boundaries/boundaryLayerFreeSlip.m``, with its cavities and roughness
+    elements (``:346-366``, :func:`cut_cavities_and_roughness`).
 
     The in- and outflow of the isothermal boundary layer when x is not periodic (``:4-144``,
     held inflow when the domain starts past x = 0), the zero-gradient top when y is not
@@ -375,6 +417,7 @@ def boundary_layer_free_slip(X, Y, Z, E0, P0, x_periodic=False, y_periodic=False
     if not y_periodic:
         wall_j = int(np.argmin(np.abs(np.asarray(Y)))) + 1
         fr[:, :wall_j, :] = False
+    cut_cavities_and_roughness(fr, X, Y, Z, cavities, roughness)
     fr, walls, corners, inside = find_walls_for_boundaries(fr)
     for key, d in (("front", "xi"), ("back", "xf"), ("up", "yi"), ("down", "yf"), ("right", "zi"), ("left", "zf")):
         for var in "uwp":

@@ -124,8 +124,10 @@ def cfl_dt(state, Ma, dxmin, nz):
 
 
 def lid_driven(n=129, Re=1000.0, Ma=0.5, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0.49,
-               spatial="SL4", cfl=0.5, nx=None, ny=None, lid_u0=1.0):
+               spatial="SL4", cfl=0.5, nx=None, ny=None, lid_u0=1.0, roughness=()):
     """2-D lid-driven cavity on [0,1]^2 with a uniform mesh (examples/lidDriven.m, uniform variant).
+    ``roughness``: blocks (dicts with x, y, z intervals) closed off as walls inside the cavity
+    (lidDrivenFlow.m:16-25).
 
     Filter borders 'reducedOrder' with the x start/end and y end closures switched
     off (``examples/lidDriven.m:119-122``).  dt = cfl / CFLdt on the initial state
@@ -139,7 +141,7 @@ def lid_driven(n=129, Re=1000.0, Ma=0.5, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0
     meshes = [DirMesh(X), DirMesh(Y), DirMesh(Z, periodic=True, fix_periodic_domain_size=True, xf=1.0)]
     E0 = 1.0 / ((gamma ** 2 - gamma) * Ma ** 2)
     P0 = (gamma - 1) * E0
-    fr, walls, corners, bc = lid_driven_flow(nx, ny, 1, E0, P0)
+    fr, walls, corners, bc = lid_driven_flow(nx, ny, 1, E0, P0, X=X, Y=Y, Z=Z, roughness=roughness)
     nm = NumMethods(spatial_derivs=spatial, spatial_derivs_buffer="EX4", filter_strength=alpha_f,
                     filter_borders="reducedOrder", filter_borders_start=(False, True, True),
                     filter_borders_end=(False, False, True))
@@ -150,7 +152,7 @@ def lid_driven(n=129, Re=1000.0, Ma=0.5, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0
                        fortran_round(Y[box[2] - 1:box[3]]), fortran_round(Z[box[4] - 1:box[5]]),
                        [float(fortran_round(lid_u0))])
     rnd = lambda v: float(fortran_round(v))
-    case = Case("lid_driven_%dx%d" % (nx, ny), nx, ny, 1, X, Y, Z, rnd(Re), rnd(Ma), rnd(Pr), rnd(gamma),
+    case = Case("lid_driven_%dx%d" % (nx, ny) + ("_rug%d" % len(roughness) if roughness else ""), nx, ny, 1, X, Y, Z, rnd(Re), rnd(Ma), rnd(Pr), rnd(gamma),
                 rnd(T0), deriv, filt, types, (1, 1, 1), -1.0, bi, [dist], [],
                 dxmin=_dxmin(X, Y, Z, 1))
     case.flow_region = fr
@@ -357,7 +359,7 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
                    Ma=0.5, Pr=0.71, gamma=1.4, T0=300.0, alpha_f=0.49, spatial="SL4", cfl=0.5, sfd_chi=0.02,
                    sfd_delta=10.0, noise=1e-5, seed=20261, buffer_growth=8.0, nz=1, z_length=4.0,
                    adiabatic=False, z_symmetry=False, pressure_inlet=False, free_slip=False,
-                   initial="compressible", sfd_type=2):
+                   initial="compressible", sfd_type=2, cavities=(), roughness=(), wall_depth=0.0, n_below=0):
     """Isothermal-wall boundary layer (boundaryLayerIsothermal.m; ``adiabatic=True``: the walls of
     boundaryLayerAdiabatic.m; ``free_slip=True``: the slip plate of boundaryLayerFreeSlip.m, with
     zero-gradient u and w on the wall and no x break point) on a stretched mesh with buffers.
@@ -377,6 +379,12 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
     the plate; 'blasius' is the incompressible Blasius velocity scaled so that the displacement
     thickness is 1 at x = 0 with uniform R and E.  Either gets seeded N(0, noise^2) on u and v in
     the flow region.
+
+    Cavities and roughness elements (``flowType.cav`` / ``flowType.rug``: dicts with x, y, z
+    intervals) are cut into the plate as the flow-type scripts do; a cavity below the plate needs
+    mesh rows below y = 0: ``n_below`` uniform rows from y = -``wall_depth`` (``domain.yi < 0`` of
+    the reference's cavity configuration, parameters.m:28-44), which come on top of the ``ny``
+    rows from the wall up.
     """
     npx, npy = nx - 2 * nbuf, ny - nbuf
     Xp = np.linspace(x_range[0], x_range[1], npx)
@@ -387,6 +395,9 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
     Yp = np.concatenate([[0.0], np.cumsum(dy0 * wall_ratio ** np.arange(npy - 1))])
     Yp[-1] = y_top
     Y = np.concatenate([Yp, _buffer_points(Yp[-1], Yp[-1] - Yp[-2], nbuf, buffer_growth, 1.0)])
+    if n_below:
+        Y = np.concatenate([-wall_depth + (wall_depth / n_below) * np.arange(n_below), Y])
+        ny = len(Y)
     z_sym = z_symmetry and nz > 1
     if z_sym:      # symmetry planes at both ends of a uniform z on [0, z_length]
         Z = np.linspace(0.0, z_length, nz)
@@ -397,11 +408,13 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
     if free_slip:
         if adiabatic or z_sym or pressure_inlet:
             raise ValueError("free_slip takes none of adiabatic, z_symmetry, pressure_inlet")
-        fr, walls, corners, bc, bp, hold = boundary_layer_free_slip(X, Y, Z, E0, P0)
+        fr, walls, corners, bc, bp, hold = boundary_layer_free_slip(X, Y, Z, E0, P0, cavities=cavities,
+                                                                    roughness=roughness)
     else:
         fr, walls, corners, bc, bp, hold = boundary_layer_isothermal(X, Y, Z, E0, P0, adiabatic=adiabatic,
                                                                      z_periodic=not z_sym, z_symmetry=z_sym,
-                                                                     pressure_inlet=pressure_inlet)
+                                                                     pressure_inlet=pressure_inlet,
+                                                                     cavities=cavities, roughness=roughness)
     buf = BufferSide(n=nbuf, upwind=False, transition=0.2)
     meshes = [DirMesh(X, buffer_i=buf, buffer_f=buf, break_points=[bp] if bp else []),
               DirMesh(Y, buffer_f=buf),
@@ -415,7 +428,8 @@ def boundary_layer(nx=1024, ny=256, nbuf=40, x_range=(-1.0, 30.0), y_top=8.0, wa
     rnd = lambda v: float(fortran_round(v))
     chi = calc_sfd_region(nx, ny, nz, ((nbuf, nbuf), (0, nbuf), (0, 0)), sfd_chi, apply=(2, 0, 2))
     name = ("boundary_layer_free_slip_%dx%d" if free_slip else "boundary_layer_%dx%d") % (nx, ny) + \
-        ("x%d" % nz if nz > 1 else "")
+        ("x%d" % nz if nz > 1 else "") + ("_cav%d" % len(cavities) if cavities else "") + \
+        ("_rug%d" % len(roughness) if roughness else "")
     dist = []
     if hold:
         # a domain that starts past the leading edge holds its inflow plane at the initial

@@ -223,3 +223,44 @@ def generate_mesh(xi, xf, m, direction="x"):
     if added:                                                      # :238-243
         X = X[:-1]
     return X
+
+
+def mesh_fixed_points(domain, cavities=(), roughness=(), disturbances=(), tracked_points=None,
+                      fit_tracked_points=False):
+    """The fixed points of each direction (``source/meshAddFixedPoints.m:1-52``), which
+    :func:`generate_mesh` matches when ``match_fixed`` is set.
+
+    x and y start from 0, z from nothing; every cavity's and roughness element's x, y and z
+    intervals are added, then those of the disturbances with ``fitPoints`` (all three
+    directions) or ``fitPointsX/Y/Z`` (that direction), then the tracked points when
+    ``fit_tracked_points``.  Each list is sorted without repeats, and infinite points and
+    points outside ``domain`` = (xi, xf, yi, yf, zi, zf) are dropped.  Elements and
+    disturbances are dicts or objects with x, y, z (and the fit flags); returns (fx, fy, fz).
+    """
+    def get(el, k, default=None):
+        if isinstance(el, dict):
+            return el.get(k, default)
+        return getattr(el, k, default)
+
+    pts = [[0.0], [0.0], []]
+    for el in list(cavities) + list(roughness):
+        for d, k in enumerate("xyz"):
+            pts[d] += [float(v) for v in get(el, k)]
+    for el in disturbances:
+        if get(el, "fitPoints", False):
+            for d, k in enumerate("xyz"):
+                pts[d] += [float(v) for v in get(el, k)]
+        else:
+            for d, k in enumerate("xyz"):
+                if get(el, "fitPoints" + k.upper(), False):
+                    pts[d] += [float(v) for v in get(el, k)]
+    if tracked_points is not None and fit_tracked_points:
+        for row in np.atleast_2d(np.asarray(tracked_points, dtype=float)):
+            for d in range(3):
+                pts[d].append(float(row[d]))
+    out = []
+    for d in range(3):
+        u = np.unique(np.asarray(pts[d], dtype=float))
+        lo, hi = domain[2 * d], domain[2 * d + 1]
+        out.append(u[~(np.isinf(u) | (u < lo) | (u > hi))])
+    return tuple(out)

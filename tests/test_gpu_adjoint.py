@@ -5,7 +5,8 @@
   turbulenceGenerator forcing, whose global sums the linearised steps differentiate), the lid-driven
   cavity (walls, corners, lid) and the boundary layer (inflow/outflow/top boxes,
   free-slip segment, buffers, SFD with a frozen mean, and with an evolving mean, whose
-  five fields then belong to the state, the perturbation and the adjoint);
+  five fields then belong to the state, the perturbation and the adjoint), and the boundary
+  layer over a cavity with a roughness element;
 * the tangent against central differences of the oracle's step (the
   reference's non-linear step restated on the CPU);
 * the transposed line solves against the transposes of the forward solves
@@ -61,6 +62,15 @@ def _case(kind):
         z = np.arange(12) / 12
         q["W"] = np.asfortranarray(q["W"] + 1e-3 * np.sin(2 * np.pi * z)[None, None, :] * c.flow_region)
         return c, q
+    if kind == "bl_cavity":
+        # a cavity cut into the plate and a roughness element on it: interior wall faces in
+        # every orientation, convex and concave corners, line types split by walls
+        kw = dict(nx=96, ny=48, nbuf=10, n_below=8, wall_depth=2.0, sfd_delta=-1.0)
+        c0 = cases.boundary_layer(**kw)
+        X, Y = np.asarray(c0.X), np.asarray(c0.Y)
+        c = cases.boundary_layer(cavities=[dict(x=(X[30], X[40]), y=(-2.0, 0.0), z=(-np.inf, np.inf))],
+                                 roughness=[dict(x=(X[50], X[53]), y=(0.0, Y[12]), z=(-np.inf, np.inf))], **kw)
+        return c, {k: np.array(v, copy=True, order="F") for k, v in c.state0.items()}
     if kind == "bl":
         # inflow, outflow and top boxes (null second derivatives), the wall with its free-slip
         # segment, EX4 buffers, SFD with a frozen mean in the buffers
@@ -194,7 +204,7 @@ def test_evolving_mean_checkpointed_gradient(torch_cuda, kind):
     prop.close()
 
 
-@pytest.mark.parametrize("kind", ["box2d", "box3d", "box3d_forced", "cavity", "bl", "bl3d"])
+@pytest.mark.parametrize("kind", ["box2d", "box3d", "box3d_forced", "cavity", "bl", "bl3d", "bl_cavity"])
 def test_dot_product_one_step(torch_cuda, kind):
     from gatt_dns_amd.adjoint import Propagator, state_tensor
     case, q = _case(kind)
@@ -210,7 +220,7 @@ def test_dot_product_one_step(torch_cuda, kind):
     prop.close()
 
 
-@pytest.mark.parametrize("kind", ["box2d", "box3d", "box3d_forced", "cavity", "bl", "bl3d"])
+@pytest.mark.parametrize("kind", ["box2d", "box3d", "box3d_forced", "cavity", "bl", "bl3d", "bl_cavity"])
 def test_tangent_matches_oracle_differences(torch_cuda, kind, oracle_mod):
     from gatt_dns_amd.adjoint import Propagator, state_tensor
     case, q = _case(kind)

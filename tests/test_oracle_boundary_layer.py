@@ -178,3 +178,124 @@ def test_free_slip_periodic_lists():
     assert fr.all() and not any(d in ("yi", "yf") for d in bc.dir)
     fr, walls, corners, bc, bp, hold = boundary_layer_free_slip(X + 2.0, Y, Z, 2.0, 0.8)
     assert hold and not any(t == "dir" and d == "xi" for d, t in zip(bc.dir, bc.type))
+
+
+INF = np.inf
+
+
+def cavity_case():
+    """The reference's cavity configuration in small (parameters.m:28-44: domain.yi = -D, a cavity
+    [x1 x2] x [-D 0] cut into the plate) with a roughness element on the plate downstream.  Every
+    interval ends on a mesh node, as the matched fixed points of the reference's mesh put them, so
+    the strict (cavity) and inclusive (element) bounds decide which nodes are walls."""
+    kw = dict(nx=96, ny=48, nbuf=10, n_below=8, wall_depth=2.0)
+    c0 = cases.boundary_layer(**kw)
+    X, Y = np.asarray(c0.X), np.asarray(c0.Y)
+    return cases.boundary_layer(cavities=[dict(x=(X[30], X[40]), y=(-2.0, 0.0), z=(-INF, INF))],
+                                roughness=[dict(x=(X[50], X[53]), y=(0.0, Y[12]), z=(-INF, INF))], **kw)
+
+
+def test_cavity_and_roughness(oracle_mod):
+    """Cavity and roughness element of boundaryLayerIsothermal.m:338-358: the nodes strictly inside
+    the cavity's intervals are opened (and the one-node plate row above them, a wall between flow
+    on both sides, goes with them), the nodes on or inside the element's are closed.  After the
+    boundary conditions every wall node with flow beside it is no-slip with E = E0, and the
+    pressure has zero normal gradient (one-sided extrapolation into the flow) on the cavity's
+    side walls and floor and on the element's faces."""
+    c = cavity_case()
+    X, Y = np.asarray(c.X), np.asarray(c.Y)
+    fr = c.flow_region[:, :, 0]
+    j0 = int(np.argmin(np.abs(Y)))                      # the plate row, Y = 0
+    assert Y[j0] == 0.0 and j0 == 8
+    ic = np.arange(31, 40)                               # strictly inside X[30] .. X[40]
+    jc = np.arange(1, j0)                                # strictly inside -2 .. 0
+    ir = np.arange(50, 54)                               # X[50] .. X[53] inclusive
+    jr = np.arange(j0, 13)                               # 0 .. Y[12] inclusive
+    want = np.zeros_like(fr)
+    want[:, j0 + 1:] = True
+    want[np.ix_(ic, jc)] = True
+    want[ic, j0] = True                                  # the lip row between the cavity and the flow
+    want[np.ix_(ir, jr)] = False
+    assert np.array_equal(fr, want)
+    o = oracle_mod.Oracle(c)
+    rng = np.random.default_rng(4)
+    q = {k: np.asfortranarray(v + 0.01 * rng.standard_normal(v.shape)) for k, v in c.state0.items()}
+    q["W"][:] = 0.0
+    st = o.apply_bound_cond(q, 0.0)
+    U, V, E = (st[k][:, :, 0] for k in "UVE")
+    P = (c.gamma - 1) * st["R"][:, :, 0] * E
+    wall = ~fr
+    near = np.zeros_like(wall)                            # wall nodes with a flow node beside them
+    near[:-1] |= fr[1:]
+    near[1:] |= fr[:-1]
+    near[:, :-1] |= fr[:, 1:]
+    near[:, 1:] |= fr[:, :-1]
+    faces = wall & near & (X[:, None] >= 0)
+    assert np.all(U[faces] == 0.0) and np.all(V[faces] == 0.0)
+    np.testing.assert_allclose(E[faces], c.E0, rtol=1e-15)
+    cn = np.asarray(c.bi.neumann)
+    L = len(cn)
+
+    def extrap(i, j, di, dj):
+        return sum(cn[k] * P[i + (k + 1) * di, j + (k + 1) * dj] for k in range(L))
+
+    checks = [(ic[0] - 1, j, 1, 0) for j in jc[1:]]               # cavity's upstream wall
+    checks += [(ic[-1] + 1, j, -1, 0) for j in jc[1:]]            # downstream wall
+    checks += [(i, jc[0] - 1, 0, 1) for i in ic[1:-1]]            # floor
+    checks += [(i, jr[-1], 0, 1) for i in ir[1:-1]]               # element's top
+    checks += [(ir[0], j, -1, 0) for j in jr[1:-1]]               # element's upstream face
+    checks += [(ir[-1], j, 1, 0) for j in jr[1:-1]]               # downstream face
+    assert len(checks) > 20
+    for i, j, di, dj in checks:
+        assert P[i, j] == pytest.approx(extrap(i, j, di, dj), rel=1e-12), (i, j, di, dj)
+    st, mc, dv = o.run(o.apply_bound_cond(c.state0, 0.0), 0.0, c.dt, 5)
+    assert not dv and np.all(np.isfinite(st["U"]))
+
+
+def test_mesh_fixed_points():
+    """meshAddFixedPoints.m: 0 in x and y, the cavities', elements' and fitted disturbances'
+    intervals and the fitted tracked points; sorted, unique, finite and inside the domain."""
+    from gatt_dns_amd.prep.mesh import mesh_fixed_points
+    fx, fy, fz = mesh_fixed_points(
+        (0.0, 300.0, -5.0, 30.0, 0.0, 1.0),
+        cavities=[dict(x=(200.0, 210.0), y=(-5.0, 0.0), z=(-INF, INF))],
+        roughness=[dict(x=(1.0, 1.2), y=(-0.1, 0.0), z=(0.3, 0.8))],
+        disturbances=[dict(x=(25.0, 50.0), y=(0.0, 0.0), z=(-INF, INF), fitPoints=False, fitPointsX=True),
+                      dict(x=(400.0, 500.0), y=(1.0, 2.0), z=(0.1, 0.2), fitPoints=True)],
+        tracked_points=[[7.0, 3.0, 0.5]], fit_tracked_points=True)
+    np.testing.assert_array_equal(fx, [0.0, 1.0, 1.2, 7.0, 25.0, 50.0, 200.0, 210.0])
+    np.testing.assert_array_equal(fy, [-5.0, -0.1, 0.0, 1.0, 2.0, 3.0])
+    np.testing.assert_array_equal(fz, [0.1, 0.2, 0.3, 0.5, 0.8])
+    fx, fy, fz = mesh_fixed_points((0.0, 1.0, 0.0, 1.0, 0.0, 1.0), tracked_points=[[0.5, 0.5, 0.5]])
+    np.testing.assert_array_equal(fx, [0.0])
+    np.testing.assert_array_equal(fz, [])
+
+
+def test_lid_driven_obstacle(oracle_mod):
+    """A roughness block inside the lid-driven cavity (lidDrivenFlow.m:16-25, before the outer
+    walls): its nodes are walls, its faces no-slip with zero normal pressure gradient."""
+    c = cases.lid_driven(nx=48, ny=40, roughness=[dict(x=(0.35, 0.6), y=(0.3, 0.55), z=(-INF, INF))])
+    X, Y = np.asarray(c.X), np.asarray(c.Y)
+    fr = c.flow_region[:, :, 0]
+    ib = np.flatnonzero((X >= 0.35) & (X <= 0.6))
+    jb = np.flatnonzero((Y >= 0.3) & (Y <= 0.55))
+    want = np.ones_like(fr)
+    want[[0, -1], :] = False
+    want[:, [0, -1]] = False
+    want[np.ix_(ib, jb)] = False
+    assert np.array_equal(fr, want)
+    o = oracle_mod.Oracle(c)
+    rng = np.random.default_rng(6)
+    q = {k: np.asfortranarray(v + 0.01 * rng.standard_normal(v.shape)) for k, v in c.state0.items()}
+    q["W"][:] = 0.0
+    st = o.apply_bound_cond(q, 0.0)
+    U, V = st["U"][:, :, 0], st["V"][:, :, 0]
+    assert np.all(U[np.ix_(ib, jb)] == 0.0) and np.all(V[np.ix_(ib, jb)] == 0.0)
+    P = (c.gamma - 1) * st["R"][:, :, 0] * st["E"][:, :, 0]
+    cn = np.asarray(c.bi.neumann)
+    for j in jb[1:-1]:
+        assert P[ib[0], j] == pytest.approx(sum(cn[k] * P[ib[0] - 1 - k, j] for k in range(len(cn))), rel=1e-12)
+        assert P[ib[-1], j] == pytest.approx(sum(cn[k] * P[ib[-1] + 1 + k, j] for k in range(len(cn))), rel=1e-12)
+    for i in ib[1:-1]:
+        assert P[i, jb[0]] == pytest.approx(sum(cn[k] * P[i, jb[0] - 1 - k] for k in range(len(cn))), rel=1e-12)
+        assert P[i, jb[-1]] == pytest.approx(sum(cn[k] * P[i, jb[-1] + 1 + k] for k in range(len(cn))), rel=1e-12)
