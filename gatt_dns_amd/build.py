@@ -11,7 +11,7 @@ OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libgatt_dns.so")
 SOURCES = ["line_solve_x_xg.hip", "line_solve_x_wsp.hip", "line_solve_y_wsp.hip", "line_solve_z_wsp.hip",
            "line_solve_x_ws.hip", "line_solve_y_ws.hip", "line_solve_z_ws.hip", "line_solve_x_lg.hip",
-           "line_solve_y_lg.hip", "line_solve_z_lg.hip", "pointwise.hip", "boundary.hip", "lst.hip", "line_solve_t.hip", "adjoint.hip", "rhs_line.hip", "capi.hip"]
+           "line_solve_y_lg.hip", "line_solve_z_lg.hip", "line_solve_multi.hip", "pointwise.hip", "boundary.hip", "lst.hip", "line_solve_t.hip", "adjoint.hip", "rhs_line.hip", "capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GATT_OFFLOAD_ARCH", "gfx950")

@@ -178,6 +178,7 @@ struct gatt_ctx {
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
   bool fork_dirs = true;
+  bool multi_dirs = true;   // serial direction solves of one lane-group shape as one launch
   // tangent / adjoint arrays, per member (allocated on first use)
   double* lin = nullptr;
   long long lin_stride = 0;
@@ -829,6 +830,38 @@ void run_solve(gatt_ctx* c, int dir, int kind, int nf, double* const* in, double
   launch_line_solve(a, d.shape[kind], nmembers, st ? st : c->stream);
 }
 
+// The derivative solves of one RHS pass along every direction (in[d], out[d]: the nf fields of
+// direction d) as one launch, where the directions run on one stream with lane-group shapes of
+// one point count per lane and workgroup size (the lanes per line may differ) and one band
+// width: each workgroup solves what it would in its direction's own launch.
+// Returns false, with nothing launched, otherwise (also while a ragged group records launches,
+// the combined kernel having no ragged twin).
+bool run_solve_dirs_once(gatt_ctx* c, int ndir, int nf, double* const (*in)[kMaxPtr],
+                         double* const (*out)[kMaxPtr], int nmembers, long long in_stride, long long out_stride) {
+  if (c->fork_dirs || !c->multi_dirs || g_recorder || ndir < 2) return false;
+  const LineShape& s0 = c->dir[0].shape[GATT_DERIV];
+  SolveArgs parts[3];
+  for (int d = 0; d < ndir; ++d) {
+    const DirOps& dd = c->dir[d];
+    const LineShape& s = dd.shape[GATT_DERIV];
+    if (dd.identity[GATT_DERIV] || s.variant != 0 || s.M != s0.M ||
+        s.waves_per_block != s0.waves_per_block || dd.op[GATT_DERIV].nR != c->dir[0].op[GATT_DERIV].nR)
+      return false;
+    SolveArgs& a = parts[d];
+    a = SolveArgs{};
+    fill_solve_args(c, d, GATT_DERIV, a);
+    a.nf = nf;
+    a.in_stride = in_stride;
+    a.out_stride = out_stride;
+    for (int f = 0; f < nf; ++f) {
+      a.in[f] = in[d][f];
+      a.out[f] = out[d][f];
+    }
+    a.mode = 0;
+  }
+  return launch_line_solve_multi(parts, ndir, s0, nmembers, c->stream);
+}
+
 // The derivative solve along `dir` can take its input as viscous fluxes computed on the fly:
 // a fused wave-segment shape (variants 2 and 3) and a band the flux kernels are built for.
 bool flux_solve_ok(const gatt_ctx* c, int dir) {
@@ -928,16 +961,20 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
     const int nm = std::min(c->wgroup, c->batch - m0);
     const long long mo = (long long)m0 * c->slab;
     if (m0 > 0) fork();
-    for (int d = 0; d < ndir; ++d) {
-      double* in[kMaxPtr];
-      double* out[kMaxPtr];
+    {
+      double* in[3][kMaxPtr];
+      double* out[3][kMaxPtr];
       int nf = 0;
-      for (int f : flds) {
-        in[nf] = S + mo + (long long)f * np;
-        out[nf] = D + (long long)(3 * f + d) * np;
-        ++nf;
+      for (int d = 0; d < ndir; ++d) {
+        nf = 0;
+        for (int f : flds) {
+          in[d][nf] = S + mo + (long long)f * np;
+          out[d][nf] = D + (long long)(3 * f + d) * np;
+          ++nf;
+        }
       }
-      run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, c->slab, ws, dir_stream(d));
+      if (!run_solve_dirs_once(c, ndir, nf, in, out, nm, c->slab, ws))
+        for (int d = 0; d < ndir; ++d) run_solve(c, d, GATT_DERIV, nf, in[d], out[d], 0.0, 0, nm, c->slab, ws, dir_stream(d));
     }
     join();
     AssembleArgs aa{};
@@ -1017,17 +1054,21 @@ void run_rhs_stage(gatt_ctx* c, double* S, double* Q, double* Out, int stage, do
     va.is3d = c->is3d;
     launch_visc(va, nm, c->stream);
     fork();
-    for (int d = 0; d < ndir; ++d) {
-      double* in[kMaxPtr];
-      double* out[kMaxPtr];
+    {
+      double* in[3][kMaxPtr];
+      double* out[3][kMaxPtr];
       int nf = 0;
-      for (int q = 0; q < 4; ++q) {
-        if (!c->is3d && q == 2) continue;
-        in[nf] = T + (long long)tin3[d][q] * np;
-        out[nf] = G + (long long)(4 * d + q) * np;
-        ++nf;
+      for (int d = 0; d < ndir; ++d) {
+        nf = 0;
+        for (int q = 0; q < 4; ++q) {
+          if (!c->is3d && q == 2) continue;
+          in[d][nf] = T + (long long)tin3[d][q] * np;
+          out[d][nf] = G + (long long)(4 * d + q) * np;
+          ++nf;
+        }
       }
-      run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, ws, ws, dir_stream(d));
+      if (!run_solve_dirs_once(c, ndir, nf, in, out, nm, ws, ws))
+        for (int d = 0; d < ndir; ++d) run_solve(c, d, GATT_DERIV, nf, in[d], out[d], 0.0, 0, nm, ws, ws, dir_stream(d));
     }
     join();
     launch_assemble(aa, nm, c->stream);
@@ -1222,6 +1263,10 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
   // (or the branches) for comparisons.
   c->fork_dirs = c->np * batch >= (1LL << 20);
   if (const char* e = std::getenv("GATT_SERIAL_DIRECTIONS")) c->fork_dirs = e[0] != '1';
+  // on one stream, the lane-group solves of all directions go out as one launch where they
+  // share a shape; GATT_MULTI_DIRECTIONS=0 launches each direction on its own
+  c->multi_dirs = true;
+  if (const char* e = std::getenv("GATT_MULTI_DIRECTIONS")) c->multi_dirs = e[0] != '0';
   {
     int dev = 0, v = 0;
     HIPCHK(hipGetDevice(&dev));

@@ -173,6 +173,17 @@ extern template void launch_line_solve_wsp_dir<0>(const SolveArgs&, const LineSh
 extern template void launch_line_solve_wsp_dir<1>(const SolveArgs&, const LineShape&, int, hipStream_t);
 extern template void launch_line_solve_wsp_dir<2>(const SolveArgs&, const LineShape&, int, hipStream_t);
 void launch_line_solve_xg(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st);
+// Lane-group solves of up to three directions (one SolveArgs each, with its own lanes per
+// line a.G; one band width, points per lane and workgroup size, given by s) in one launch
+// (line_solve_multi.hip); false, with nothing launched, where these are not compiled for it.
+struct SolveMulti {
+  SolveArgs a[3];
+  unsigned first[4];   // first flat workgroup of each part; first[n] is the total
+  unsigned gx[3];      // x extent of each part's own grid
+  int n;
+};
+bool multi_solve_supported(int nR, int M);
+bool launch_line_solve_multi(const SolveArgs* parts, int n, const LineShape& s, int nmembers, hipStream_t st);
 inline void launch_line_solve(const SolveArgs& a, const LineShape& s, int nmembers, hipStream_t st) {
   if (s.variant == 4) {
     launch_line_solve_xg(a, s, nmembers, st);

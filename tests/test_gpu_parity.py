@@ -418,6 +418,44 @@ def test_direction_branches_equal_serial(rt, name, monkeypatch):
         assert np.array_equal(states[0][k], states[1][k]), (name, k)
 
 
+@pytest.mark.parametrize("name", ["cavity129", "bl160", "box16_forced"])
+def test_directions_in_one_launch_equal_separate(rt, oracle_mod, name, monkeypatch):
+    """On one stream the lane-group derivative solves of every direction of a pass go out as one
+    launch (the default where the directions share the points per lane and the workgroup size,
+    as on these grids; the boundary layer's x and y lines take 64 and 16 lanes); the states equal
+    those of a launch per direction bit for bit, and the oracle's at the one-step bar."""
+    if name == "bl160":
+        case = cases.boundary_layer(nx=160, ny=64, nbuf=12)
+    elif name == "box16_forced":
+        case = cases.turbulence_box(16, seed=3, forcing=True)
+    else:
+        case = CASES[name]()
+    states = []
+    for multi in ("1", "0"):
+        monkeypatch.setenv("GATT_MULTI_DIRECTIONS", multi)
+        s = rt.Solver(case)
+        # lane-group shapes of one point count per lane and workgroup size
+        shapes = {tuple(s.get_line_shape(d, 0)[i] for i in (0, 2, 3)) for d in range(3 if case.nz > 1 else 2)}
+        assert len(shapes) == 1 and next(iter(shapes))[0] == 0, shapes
+        s.set_state(case.state0)
+        s.apply_bound_cond(0.0)
+        s.advance(1, 0.0, case.dt)
+        one = s.get_state()
+        s.advance(3, case.dt, case.dt)
+        states.append(s.get_state())
+        s.close()
+        if multi == "1":
+            o = oracle_mod.Oracle(case)
+            q0 = o.apply_bound_cond(case.state0, 0.0)
+            if case.sfd_type > 0:
+                o.init_mean(q0)
+            want, _, _, _ = o.step(q0, 0.0, case.dt)
+            for k in ("UVWRE" if case.nz > 1 else "UVRE"):
+                assert_field(one, want, k, 1e-12, (name,))
+    for k in FIELDS3:
+        assert np.array_equal(states[0][k], states[1][k]), (name, k)
+
+
 def test_box_steps(rt, oracle_mod):
     case = CASES["box32"]()
     o = oracle_mod.Oracle(case)
