@@ -475,6 +475,19 @@ int compile_bc(gatt_ctx* c, const gatt_bc_desc& bc) {
       case 6: out[4] = b[4] - L; out[5] = b[4]; break;
     }
   };
+  // the points a one-sided extrapolation from a plane box reads: 1..L steps along direction d
+  // (extrap in boundary.hip; the plane itself is not read)
+  auto stencil_ex = [](const int b[6], int d, int L, int out[6]) {
+    for (int q = 0; q < 6; ++q) out[q] = b[q];
+    switch (d) {
+      case 1: out[0] = b[0] + 1; out[1] = b[0] + L; break;
+      case 2: out[0] = b[0] - L; out[1] = b[0] - 1; break;
+      case 3: out[2] = b[2] + 1; out[3] = b[2] + L; break;
+      case 4: out[2] = b[2] - L; out[3] = b[2] - 1; break;
+      case 5: out[4] = b[4] + 1; out[5] = b[4] + L; break;
+      case 6: out[4] = b[4] - L; out[5] = b[4] - 1; break;
+    }
+  };
   auto plane = [](const int b[6], int d, int out[6]) {
     for (int q = 0; q < 6; ++q) out[q] = b[q];
     const int ax = (d - 1) / 2;
@@ -540,7 +553,7 @@ int compile_bc(gatt_ctx* c, const gatt_bc_desc& bc) {
         plane(b, op.dir, op.box);
         OpSets st;
         Box r{f, {}}, w{f, {}};
-        stencil(op.box, op.dir, L, r.b);
+        stencil_ex(op.box, op.dir, L, r.b);
         std::memcpy(w.b, op.box, sizeof(w.b));
         st.rd.push_back(r);
         st.wr.push_back(w);
@@ -561,9 +574,10 @@ int compile_bc(gatt_ctx* c, const gatt_bc_desc& bc) {
       op.coef = coef;
       plane(b, op.dir, op.box);
       OpSets st;
+      // P = R E along the stencil, then R = P / E at the plane
       Box r1{GATT_R, {}}, r2{GATT_E, {}}, w{GATT_R, {}};
-      stencil(op.box, op.dir, L, r1.b);
-      std::memcpy(r2.b, r1.b, sizeof(r2.b));
+      stencil_ex(op.box, op.dir, L, r1.b);
+      stencil(op.box, op.dir, L, r2.b);
       std::memcpy(w.b, op.box, sizeof(w.b));
       st.rd.push_back(r1);
       st.rd.push_back(r2);
@@ -682,11 +696,50 @@ int compile_bc(gatt_ctx* c, const gatt_bc_desc& bc) {
     }
     if ((rc = add(op, st))) return rc;
   }
-  // Waves: each op goes one wave after the last earlier op it conflicts with (it reads what
-  // that op writes, or writes what that op reads or writes), so the ops of one wave are
-  // mutually independent, every pair of conflicting ops keeps its order, and independent
-  // ops of the reference's list share a wave even when a conflicting op stands between
-  // them.  The ops are then stored wave after wave (stable within a wave).
+  // Dead writes: where a later op B writes field f over points an earlier single-field op A
+  // (a Dirichlet value, R from P, a Neumann or null-second-derivative extrapolation) wrote,
+  // and neither B nor any op between them reads f there, A's values there are overwritten
+  // unread (a corner point of an inflow plane and a wall row, say).  A's box loses those
+  // points where what remains is a box (the overlap covers A's box in two axes from one
+  // end), or A goes entirely; the program's result is unchanged, and the write-write
+  // conflicts between boundary planes that meet at a corner no longer order the waves.
+  auto reads_of = [&](const BcOp& op, const OpSets& st, std::vector<Box>& out) {
+    out.clear();
+    if (op.kind == BC_SET) return;
+    if (op.kind == BC_SET_R_FROM_P) {
+      Box r{GATT_E, {}};
+      std::memcpy(r.b, op.box, sizeof(r.b));
+      out.push_back(r);
+      return;
+    }
+    if (op.kind == BC_NEU) {
+      Box r{op.field, {}};
+      stencil_ex(op.box, op.dir, op.L, r.b);
+      out.push_back(r);
+      return;
+    }
+    if (op.kind == BC_NEU_P) {
+      Box r1{GATT_R, {}}, r2{GATT_E, {}};
+      stencil_ex(op.box, op.dir, op.L, r1.b);
+      stencil(op.box, op.dir, op.L, r2.b);
+      out.push_back(r1);
+      out.push_back(r2);
+      return;
+    }
+    out = st.rd;
+  };
+  auto trimmable = [](const BcOp& op) {
+    return op.kind == BC_SET || op.kind == BC_SET_R_FROM_P || op.kind == BC_NEU || op.kind == BC_NEU_P;
+  };
+  auto intersect = [](const int a[6], const int b[6], int out[6]) {
+    for (int d = 0; d < 3; ++d) {
+      out[2 * d] = std::max(a[2 * d], b[2 * d]);
+      out[2 * d + 1] = std::min(a[2 * d + 1], b[2 * d + 1]);
+      if (out[2 * d] > out[2 * d + 1]) return false;
+    }
+    return true;
+  };
+  // wave of each op (see below); returns the number of waves
   auto conflicts = [&](const OpSets& a, const OpSets& b) {
     for (auto& w : a.wr) {
       for (auto& x : b.wr)
@@ -699,14 +752,99 @@ int compile_bc(gatt_ctx* c, const gatt_bc_desc& bc) {
         if (overlap(r, x)) return true;
     return false;
   };
-  const int nops = (int)ops.size();
-  std::vector<int> level(nops, 0);
-  int nlev = 0;
-  for (int i = 0; i < nops; ++i) {
-    for (int j = 0; j < i; ++j)
-      if (level[j] + 1 > level[i] && conflicts(sets[i], sets[j])) level[i] = level[j] + 1;
-    nlev = std::max(nlev, level[i] + 1);
+  auto levels_of = [&](const std::vector<OpSets>& st, std::vector<int>& level) {
+    const int n = (int)st.size();
+    level.assign(n, 0);
+    int nl = 0;
+    for (int i = 0; i < n; ++i) {
+      for (int j = 0; j < i; ++j)
+        if (level[j] + 1 > level[i] && conflicts(st[i], st[j])) level[i] = level[j] + 1;
+      nl = std::max(nl, level[i] + 1);
+    }
+    return nl;
+  };
+  {
+    const std::vector<BcOp> ops0 = ops;
+    const std::vector<OpSets> sets0 = sets;
+    std::vector<char> alive(ops.size(), 1);
+    std::vector<Box> rd;
+    // GATT_BC_TRIM=0 keeps every op as listed, for comparisons
+    const char* trim_env = std::getenv("GATT_BC_TRIM");
+    bool changed = !(trim_env && trim_env[0] == '0');
+    while (changed) {
+      changed = false;
+      for (size_t bi = 0; bi < ops.size(); ++bi) {
+        if (!alive[bi]) continue;
+        for (size_t ai = 0; ai < bi; ++ai) {
+          if (!alive[ai] || !trimmable(ops[ai])) continue;
+          BcOp& A = ops[ai];
+          for (const Box& w : sets[bi].wr) {
+            int I[6];
+            if (w.f != A.field || !intersect(A.box, w.b, I)) continue;
+            bool read = false;
+            for (size_t ci = ai + 1; ci <= bi && !read; ++ci) {
+              if (!alive[ci]) continue;
+              reads_of(ops[ci], sets[ci], rd);
+              int J[6];
+              for (const Box& r : rd)
+                if (r.f == A.field && intersect(r.b, I, J)) read = true;
+            }
+            if (read) continue;
+            int full = 0, part = -1;
+            for (int d = 0; d < 3; ++d) {
+              if (I[2 * d] == A.box[2 * d] && I[2 * d + 1] == A.box[2 * d + 1]) ++full;
+              else part = d;
+            }
+            if (full == 3) {
+              alive[ai] = 0;
+            } else if (full == 2 && I[2 * part] == A.box[2 * part]) {
+              A.box[2 * part] = I[2 * part + 1] + 1;
+            } else if (full == 2 && I[2 * part + 1] == A.box[2 * part + 1]) {
+              A.box[2 * part + 1] = I[2 * part] - 1;
+            } else {
+              continue;
+            }
+            // the trimmed op's own sets follow its box
+            std::memcpy(sets[ai].wr[0].b, A.box, sizeof(A.box));
+            reads_of(A, sets[ai], sets[ai].rd);
+            changed = true;
+            break;
+          }
+        }
+      }
+    }
+    std::vector<BcOp> kept;
+    std::vector<OpSets> kept_sets;
+    for (size_t i = 0; i < ops.size(); ++i) {
+      if (!alive[i]) continue;
+      kept.push_back(ops[i]);
+      kept_sets.push_back(sets[i]);
+    }
+    // the trimmed program is kept only where it takes fewer waves (with as many waves, the
+    // program as listed measured faster on the lid-driven cavity)
+    std::vector<int> lv;
+    if (levels_of(kept_sets, lv) < levels_of(sets0, lv)) {
+      ops.swap(kept);
+      sets.swap(kept_sets);
+    } else {
+      ops = ops0;
+      sets = sets0;
+    }
+    c->undo_total = 0;
+    for (BcOp& op : ops) {
+      op.npts = (long long)(op.box[1] - op.box[0] + 1) * (op.box[3] - op.box[2] + 1) * (op.box[5] - op.box[4] + 1);
+      op.undo = c->undo_total;
+      c->undo_total += 5 * op.npts;
+    }
   }
+  // Waves: each op goes one wave after the last earlier op it conflicts with (it reads what
+  // that op writes, or writes what that op reads or writes), so the ops of one wave are
+  // mutually independent, every pair of conflicting ops keeps its order, and independent
+  // ops of the reference's list share a wave even when a conflicting op stands between
+  // them.  The ops are then stored wave after wave (stable within a wave).
+  const int nops = (int)ops.size();
+  std::vector<int> level;
+  const int nlev = levels_of(sets, level);
   std::vector<BcWave> waves;
   std::vector<BcOp> sorted;
   sorted.reserve(nops);
