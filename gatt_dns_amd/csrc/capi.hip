@@ -168,7 +168,7 @@ struct gatt_ctx {
   bool ckpt_mean_ready = false;
   // reductions
   unsigned long long* d_red = nullptr;  // batch * 8
-  double* d_rec = nullptr;              // 5
+  double* d_rec = nullptr;              // 8: member 0's maxima of the last recorded step
   int* d_flag = nullptr;
   int cur = 0;                          // 0: state in QA, 1: in QB
   hipStream_t stream = nullptr;
@@ -1322,12 +1322,10 @@ void ensure_mean(gatt_ctx* c) {
 void enqueue_step(gatt_ctx* c, int cur, double dt) {
   double* from = c->state(cur);
   double* to = c->state(1 - cur);
-  launch_clear_u64(c->d_red, 8 * c->batch, c->stream);
   run_time_step(c, from, to, dt);
   run_filters(c, to, dt / c->filterCharTime);
   run_sfd(c, to, dt);
   run_bc(c, to, dt);
-  launch_clock_add(c->d_clock, dt, c->stream);
   ReduceArgs r{};
   r.np = c->np;
   r.member_stride = c->slab;
@@ -1336,7 +1334,8 @@ void enqueue_step(gatt_ctx* c, int cur, double dt) {
   r.out = c->d_red;
   r.is3d = c->is3d;
   launch_step_reduce(r, c->batch, c->stream);
-  launch_record(c->d_red, c->d_rec, c->d_flag, c->batch, c->stream);
+  // the record also clears the maxima for the next step and advances the clock (main.F90:424)
+  launch_record(c->d_red, c->d_rec, c->d_flag, c->batch, c->stream, c->d_clock, dt);
 }
 
 int check_ctx(gatt_ctx* c) {
@@ -1532,6 +1531,8 @@ int gatt_create(const gatt_problem_desc* d, int32_t batch, gatt_ctx** out) {
   HIPCHK(hipMalloc(&p, sizeof(unsigned long long) * 8 * batch));
   c->dev_allocs.push_back(p);
   c->d_red = static_cast<unsigned long long*>(p);
+  // zero before the first step; every step record clears it for the next
+  HIPCHK(hipMemsetAsync(c->d_red, 0, sizeof(unsigned long long) * 8 * batch, c->stream));
   HIPCHK(hipMalloc(&p, sizeof(double) * 8));
   c->dev_allocs.push_back(p);
   c->d_rec = static_cast<double*>(p);
@@ -1769,13 +1770,11 @@ int gatt_rhs(gatt_ctx* c, double* const* dq) {
 }
 
 static int read_scalars(gatt_ctx* c, double* umax, double* maxchange, int32_t* diverged) {
-  unsigned long long red[8];
+  double v[8];
   int flag = 0;
-  HIPCHK(hipMemcpyAsync(red, c->d_red, sizeof(red), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(v, c->d_rec, sizeof(v), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  double v[8];
-  for (int i = 0; i < 8; ++i) std::memcpy(&v[i], &red[i], sizeof(double));
   if (maxchange)
     for (int i = 0; i < 5; ++i) maxchange[i] = v[i];
   if (umax)
@@ -3297,7 +3296,6 @@ void enqueue_tlm_step(gatt_ctx* c, double t, double dt) {
   double* S = c->slab_ptr(SL_S);
   double* dq = lin_ptr(c, LN_Q);
   double* dS = lin_ptr(c, LN_S);
-  launch_clear_u64(c->d_red, 8 * c->batch, c->stream);
   for (int stage = 1; stage <= 4; ++stage) {
     double* Sin = stage == 1 ? q : S;
     double* Out = stage == 4 ? qn : S;

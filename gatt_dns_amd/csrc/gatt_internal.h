@@ -351,7 +351,11 @@ void launch_gather_points(const double* state, long long np, int is3d, int npts,
                           hipStream_t st);
 void launch_step_reduce(const ReduceArgs& a, int nmembers, hipStream_t st);
 void launch_copy(double* dst, const double* src, long long n, hipStream_t st);
-void launch_record(const unsigned long long* red, double* rec, int* flag, int nmembers, hipStream_t st);
+// The step record: member 0's eight maxima (max changes, max |u|) as doubles into rec, the
+// divergence flag raised over every member's, red cleared for the next step's reduction,
+// and the clock (when given) advanced by dt (main.F90:424), in one single-thread launch.
+void launch_record(unsigned long long* red, double* rec, int* flag, int nmembers, hipStream_t st,
+                   double* clock = nullptr, double dt = 0.0);
 
 // ------------------------------------------------ tangent and adjoint step
 // Transposed Thomas sweeps of one direction (line_solve_t.hip); in may equal out.

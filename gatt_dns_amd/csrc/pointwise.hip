@@ -348,9 +348,12 @@ void launch_copy(double* dst, const double* src, long long n, hipStream_t st) {
   hipLaunchKernelGGL(copy_kernel, grid, dim3(256), 0, st, dst, src, n);
 }
 
-// Append member 0's max changes to the step record and raise the sticky
-// divergence flag when a change is NaN or all changes are zero (main.F90:434).
-__device__ __forceinline__ void record_body(const unsigned long long* red, double* rec, int* flag, int nmembers) {
+// The step record of member 0's eight maxima (max changes, max |u|) and the sticky divergence
+// flag, raised when a member's change is NaN or all its changes are zero (main.F90:434); then
+// the maxima are cleared for the next step's reduction and the clock is advanced by dt
+// (main.F90:424) when given.
+__device__ __forceinline__ void record_body(unsigned long long* red, double* rec, int* flag, int nmembers,
+                                            double* clock, double dt) {
   if (threadIdx.x != 0) return;
   int div = 0;
   for (int mbr = 0; mbr < nmembers; ++mbr) {
@@ -360,25 +363,27 @@ __device__ __forceinline__ void record_body(const unsigned long long* red, doubl
       const double v = __longlong_as_double((long long)red[8 * mbr + f]);
       if (v != v) nan = 1;
       s += v;
-      if (mbr == 0) rec[f] = v;
     }
     if (nan || s == 0.0) div = 1;
   }
+  for (int k = 0; k < 8; ++k) rec[k] = __longlong_as_double((long long)red[k]);
+  for (int k = 0; k < 8 * nmembers; ++k) red[k] = 0ull;
   if (div) *flag = 1;
+  if (clock) *clock = *clock + dt;
 }
 
-__global__ void record_kernel(const unsigned long long* red, double* rec, int* flag, int nmembers) {
-  record_body(red, rec, flag, nmembers);
-}
+__global__ void record_kernel(RecordArgs a) { record_body(a.red, a.rec, a.flag, a.nmembers, a.clock, a.dt); }
 __global__ void record_ragged(const RecordArgs* __restrict__ args, RaggedMap map) {
   BlockPos bp;
   const RecordArgs& a = args[ragged_pos(map, bp)];
-  record_body(a.red, a.rec, a.flag, a.nmembers);
+  record_body(a.red, a.rec, a.flag, a.nmembers, a.clock, a.dt);
 }
 
-void launch_record(const unsigned long long* red, double* rec, int* flag, int nmembers, hipStream_t st) {
-  if (record_launch((const void*)record_ragged, RecordArgs{red, rec, flag, nmembers}, dim3(1), dim3(64), 0)) return;
-  hipLaunchKernelGGL(record_kernel, dim3(1), dim3(64), 0, st, red, rec, flag, nmembers);
+void launch_record(unsigned long long* red, double* rec, int* flag, int nmembers, hipStream_t st, double* clock,
+                   double dt) {
+  const RecordArgs a{red, rec, flag, nmembers, clock, dt};
+  if (record_launch((const void*)record_ragged, a, dim3(1), dim3(64), 0)) return;
+  hipLaunchKernelGGL(record_kernel, dim3(1), dim3(64), 0, st, a);
 }
 
 }  // namespace gatt

@@ -110,10 +110,12 @@ struct CopyArgs {
   long long n;
 };
 struct RecordArgs {
-  const unsigned long long* red;
+  unsigned long long* red;
   double* rec;
   int* flag;
   int nmembers;
+  double* clock;
+  double dt;
 };
 
 }  // namespace gatt
