@@ -2470,7 +2470,14 @@ int group_harmonise_shapes(gatt_group* g) {
         if (o.identity[kind]) continue;
         const int N = o.op[kind].N;
         LineShape t = s;
-        if (s.variant == 0) {
+        if (s.variant == 4) {
+          // one wavefront per line: the ragged form takes each member's points per lane, so
+          // every member gets the fewest that cover its lines (all 64 lanes busy)
+          for (int M : {4, 8, 16}) {
+            t.M = M;
+            if (64 * M >= N && line_shape_ok(t, N, o.op[kind].nR, d, c->max_lds, o.uniform_tiles)) break;
+          }
+        } else if (s.variant == 0) {
           t.G = std::min(64, std::max(2, pow2_at_least((N + s.M - 1) / s.M)));
           if (std::max(4, pow2_at_least((N + t.G - 1) / t.G)) != s.M)
             return fail(GATT_E_UNSUPPORTED, "ragged group: line lengths too far apart for one lane-group shape");

@@ -131,6 +131,7 @@ struct SolveArgs {
   int nmem;                   // members (fused wave-segment kernel)
   int xvec;                   // x lines 16-byte aligned (fused wave-segment kernel)
   int xtile;                  // x lines staged through an LDS tile: lines per workgroup (16 or 32), 0 = no tile
+  int xg_m;                   // points per lane of the one-wavefront x solve (variant 4; its ragged form)
   long long in_stride;        // doubles between members in the input arrays
   long long out_stride;       // doubles between members in the output arrays
   const double* in[kMaxPtr];

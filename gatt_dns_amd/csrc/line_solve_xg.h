@@ -177,19 +177,28 @@ template <int NR, int M, bool BLEND>
 __global__ __launch_bounds__(256) void line_solve_xg_kernel(SolveArgs a) {
   line_solve_xg_body<NR, M, BLEND>(a, grid_pos());
 }
-template <int NR, int M, bool BLEND>
+// The ragged form takes each member's points per lane from its argument block, so that the
+// members of a ragged group share the launch with lines of different lengths, each with all
+// the lanes its lines can use.
+template <int NR, bool BLEND>
 __global__ __launch_bounds__(256) void line_solve_xg_ragged(const SolveArgs* __restrict__ args, RaggedMap map) {
   BlockPos bp;
   const int m = ragged_pos(map, bp);
-  line_solve_xg_body<NR, M, BLEND>(args[m], bp);
+  const SolveArgs& a = args[m];
+  if (a.xg_m == 4) line_solve_xg_body<NR, 4, BLEND>(a, bp);
+  else if (a.xg_m == 8) line_solve_xg_body<NR, 8, BLEND>(a, bp);
+  else line_solve_xg_body<NR, 16, BLEND>(a, bp);
 }
 
 template <int NR, int M>
-static void launch_xg_m(const SolveArgs& a, int nmembers, hipStream_t st) {
+static void launch_xg_m(const SolveArgs& a0, int nmembers, hipStream_t st) {
+  static_assert(M == 4 || M == 8 || M == 16, "points per lane of the ragged form");
+  SolveArgs a = a0;
+  a.xg_m = M;
   dim3 grid((unsigned)((a.nlines + 3) / 4), (unsigned)(a.nf * nmembers));
   const size_t lds = sizeof(double) * 4 * (size_t)a.op.N;
   const bool blend = a.mode == 1 && a.theta > 0.0;
-  if (record_launch(blend ? (const void*)line_solve_xg_ragged<NR, M, true> : (const void*)line_solve_xg_ragged<NR, M, false>,
+  if (record_launch(blend ? (const void*)line_solve_xg_ragged<NR, true> : (const void*)line_solve_xg_ragged<NR, false>,
                     a, grid, dim3(256), lds))
     return;
   if (blend)

@@ -79,8 +79,12 @@ def test_ragged_launch_equals_members_alone(nsteps):
     own = [_shapes(s, 2) for s in grouped]
     g = Group(grouped, mode=2)
     shapes = [_shapes(s, 2) for s in grouped]
-    # the cavities share one x shape (the longest line's points per lane), and the first had its own
-    assert shapes[0][(0, 0)][0] == 4 and shapes[0][(0, 0)][2] == shapes[2][(0, 0)][2] != own[0][(0, 0)][2]
+    # one wavefront per x line in every member, each with the fewest points per lane that cover its
+    # lines (the ragged form of that kernel takes them per member)
+    for m, c in enumerate(members):
+        v, G, M, w = shapes[m][(0, 0)]
+        assert v == 4 and 64 * M >= c.nx and (M == 4 or 32 * M < c.nx), (m, shapes[m][(0, 0)])
+    assert shapes[0][(0, 0)][2] < shapes[2][(0, 0)][2]
     print("ragged group shapes", shapes, flush=True)
     mc, dv = g.advance(nsteps, 0.0, [c.dt for c in members])
     assert not any(dv)
