@@ -2450,13 +2450,27 @@ int group_harmonise_shapes(gatt_group* g) {
       if (s.variant >= 1 && s.variant <= 3 && !same_n) {
         s.variant = -1;
         const int N = ob.op[kind].N;
+        // a lane-group shape whose points per lane every member's lines map back to (each
+        // member then takes its own lanes per line, below)
+        auto common = [&](const LineShape& t) {
+          for (gatt_ctx* c : g->ctx) {
+            const DirOps& o = c->dir[d];
+            if (o.identity[kind]) continue;
+            const int Nm = o.op[kind].N;
+            LineShape u = t;
+            u.G = std::min(64, std::max(2, pow2_at_least((Nm + t.M - 1) / t.M)));
+            if (std::max(4, pow2_at_least((Nm + u.G - 1) / u.G)) != t.M) return false;
+            if (!line_shape_ok(u, Nm, o.op[kind].nR, d, c->max_lds, o.uniform_tiles)) return false;
+          }
+          return true;
+        };
         for (int M : {16, 8, 4, 32}) {
           for (int wpb : {4, 2, 1}) {
             LineShape t;
             t.M = M;
             t.G = std::max(2, pow2_at_least((N + M - 1) / M));
             t.waves_per_block = wpb;
-            if (line_shape_ok(t, N, ob.op[kind].nR)) {
+            if (line_shape_ok(t, N, ob.op[kind].nR) && common(t)) {
               s = t;
               break;
             }

@@ -121,3 +121,46 @@ def test_ragged_launch_refuses_different_steps(other):
     with pytest.raises(RuntimeError, match="differ"):
         g.advance(2, 0.0, [c.dt for c in members])
     g.close()
+
+
+@pytest.mark.parametrize("kind", ["boxes", "boundary_layers"])
+def test_ragged_launch_3d_members(kind):
+    """Mode 2 with 3-D members: turbulence boxes of 16^3, 24^3 and 32^3 (their lines take one
+    lane-group shape of one point count per lane, each member its own lanes per line) and
+    spanwise-periodic boundary layers of two lengths, each equal bit for bit to the member
+    advanced alone with the same launch shapes."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from gatt_dns_amd.runtime import Group, Solver
+    if kind == "boxes":
+        members = [cases.turbulence_box(n, seed=3 + n, forcing=False) for n in (16, 24, 32)]
+    else:
+        members = [cases.boundary_layer(nx=nx, ny=40, nbuf=8, nz=12, noise=0.0) for nx in (64, 96)]
+    grouped = []
+    for c in members:
+        s = Solver(c)
+        s.set_state(c.state0)
+        s.apply_bound_cond(0.0)
+        grouped.append(s)
+    own = [_shapes(s, 3) for s in grouped]
+    g = Group(grouped, mode=2)
+    shapes = [_shapes(s, 3) for s in grouped]
+    print("3-D ragged group", kind, shapes, flush=True)
+    nsteps = 4
+    mc, dv = g.advance(nsteps, 0.0, [c.dt for c in members])
+    assert not any(dv)
+    for m, c in enumerate(members):
+        s1 = Solver(c)
+        for (d, k), (v, G, M, w) in shapes[m].items():
+            s1.set_line_shape(d, k, v, G if v == 0 else M, w)
+        s1.set_state(c.state0)
+        s1.apply_bound_cond(0.0)
+        mc1, _ = s1.advance(nsteps, 0.0, c.dt)
+        qa, qb = s1.get_state(), grouped[m].get_state()
+        for k in "UVWRE":
+            np.testing.assert_array_equal(qb[k], qa[k])
+        np.testing.assert_array_equal(mc[m], mc1)
+        s1.close()
+    g.close()
+    assert [_shapes(s, 3) for s in grouped] == own
