@@ -324,12 +324,13 @@ def test_linearised_steps_need_rk4(torch_cuda, stepping):
         s.adj_step(0.0, c.dt)
 
 
-@pytest.mark.parametrize("kind", ["box3d_forced", "bl3d"])
+@pytest.mark.parametrize("kind", ["box3d_forced", "bl3d", "bl_sfd"])
 def test_members_linearised_alone(torch_cuda, kind):
     """Two members of one context (different base states and perturbations; the forcing's sums
-    per member) advance their tangent and adjoint steps as each does alone, bit for bit."""
+    per member; an evolving SFD mean per member) advance their tangent and adjoint steps as each
+    does alone, bit for bit."""
     from gatt_dns_amd import runtime
-    case, q = _case(kind)
+    case, q = _sfd_case(kind)[:2] if kind in SFD_KINDS else _case(kind)
     rng = np.random.Generator(np.random.PCG64(17))
     q2 = {k: np.asfortranarray(v * (1.0 + 1e-3 * rng.standard_normal(v.shape))) for k, v in q.items()}
     bases = [q, q2]
