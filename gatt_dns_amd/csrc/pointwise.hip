@@ -285,19 +285,20 @@ __device__ __forceinline__ void step_reduce_body(const ReduceArgs& a, BlockPos b
   const long long mo = (long long)bp.y * a.member_stride;
   const long long np = a.np;
   unsigned long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // several points per thread: the loads of four iterations go out together
+#pragma unroll 4
   for (long long n = bp.x * (long long)blockDim.x + threadIdx.x; n < np; n += (long long)bp.gx * blockDim.x) {
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
       if (!a.is3d && f == 2) continue;
-      const double d = fabs(ld_nt(a.Qn + mo + f * np + n) - ld_nt(a.Q + mo + f * np + n));
+      const double qn = ld_nt(a.Qn + mo + f * np + n);
+      const double d = fabs(qn - ld_nt(a.Q + mo + f * np + n));
       const unsigned long long b = (unsigned long long)__double_as_longlong(d);
       acc[f] = b > acc[f] ? b : acc[f];
-    }
-#pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      if (!a.is3d && f == 2) continue;
-      const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(a.Qn[mo + f * np + n]));
-      acc[5 + f] = b > acc[5 + f] ? b : acc[5 + f];
+      if (f < 3) {   // max |u|, |v|, |w| from the same loads
+        const unsigned long long u = (unsigned long long)__double_as_longlong(fabs(qn));
+        acc[5 + f] = u > acc[5 + f] ? u : acc[5 + f];
+      }
     }
   }
   for (int k = 0; k < 8; ++k) sh[k][threadIdx.x] = acc[k];
@@ -320,7 +321,9 @@ __global__ __launch_bounds__(256) void step_reduce_ragged(const ReduceArgs* __re
 }
 
 void launch_step_reduce(const ReduceArgs& a, int nmembers, hipStream_t st) {
-  long long blocks = (a.np + 255) / 256;
+  // four points per thread where that still leaves 64 workgroups (fewer contend for the eight
+  // atomic maxima); on smaller grids one point per thread, up to 64 workgroups
+  long long blocks = std::max((a.np + 1023) / 1024, std::min((a.np + 255) / 256, 64LL));
   if (blocks > 2048) blocks = 2048;
   const dim3 grid((unsigned)blocks, nmembers);
   if (record_launch((const void*)step_reduce_ragged, a, grid, dim3(256), 0)) return;
