@@ -352,3 +352,31 @@ def test_members_linearised_alone(torch_cuda, kind):
             one.close()
             for k in "UVWRE":
                 assert np.array_equal(together[m][k], alone[k]), (kind, step, m, k)
+
+
+def test_mean_slots(torch_cuda):
+    """The SFD mean's copy slot: refused without SFD; with SFD the mean reads as the state until
+    it is set or first used, and a mean set through the slot is the one the step starts from."""
+    import torch
+    from gatt_dns_amd import runtime
+    box, q = _case("box2d")
+    s = runtime.Solver(box)
+    t = torch.zeros((5, box.nz, box.ny, box.nx), dtype=torch.float64, device="cuda")
+    with pytest.raises(runtime.GattError, match="no SFD mean"):
+        s.copy_fields(runtime.SLOT_MEAN, False, t)
+    s.close()
+    case, q, mean = _sfd_case("bl_sfd")
+    from gatt_dns_amd.adjoint import state_tensor
+    s = runtime.Solver(case)
+    s.set_state(q)
+    got = torch.empty_like(state_tensor(case, q))
+    s.copy_fields(runtime.SLOT_MEAN, False, got)
+    assert torch.equal(got[[0, 1, 3, 4]], state_tensor(case, q)[[0, 1, 3, 4]])
+    m = state_tensor(case, mean)
+    s.copy_fields(runtime.SLOT_MEAN, True, m)
+    back = s.get_mean()
+    for i, k in enumerate("UVWRE"):
+        if k == "W":
+            continue
+        assert np.array_equal(back[k], mean[k]), k
+    s.close()
