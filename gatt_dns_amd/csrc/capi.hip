@@ -768,9 +768,10 @@ int compile_bc(gatt_ctx* c, const gatt_bc_desc& bc) {
     const std::vector<OpSets> sets0 = sets;
     std::vector<char> alive(ops.size(), 1);
     std::vector<Box> rd;
-    // GATT_BC_TRIM=0 keeps every op as listed, for comparisons
+    // GATT_BC_TRIM=0 keeps every op as listed, for comparisons.  The search is cubic in the
+    // op count, so programs of more than 400 ops (many corner points) are kept as listed.
     const char* trim_env = std::getenv("GATT_BC_TRIM");
-    bool changed = !(trim_env && trim_env[0] == '0');
+    bool changed = !(trim_env && trim_env[0] == '0') && ops.size() <= 400;
     while (changed) {
       changed = false;
       for (size_t bi = 0; bi < ops.size(); ++bi) {
