@@ -2908,19 +2908,32 @@ void adj_combine(gatt_ctx* c, double* QB, const double* SB, double* ACC, double*
 
 // Base derivatives, stresses and their derivatives at S (member m0 pointer,
 // member stride ss) into D, T, G (member stride ws), for nm members.
+// The derivative solves of every direction (in[d], out[d]: nf fields of direction d) of the
+// linearised steps' base passes, on the context's stream: one launch where the directions share
+// a lane-group shape, otherwise a launch per direction.
+void run_deriv_dirs(gatt_ctx* c, int ndir, int nf, double* const (*in)[kMaxPtr], double* const (*out)[kMaxPtr],
+                    int nm, long long in_stride, long long out_stride) {
+  if (run_solve_dirs_once(c, ndir, nf, in, out, nm, in_stride, out_stride)) return;
+  for (int d = 0; d < ndir; ++d)
+    run_solve(c, d, GATT_DERIV, nf, in[d], out[d], 0.0, 0, nm, in_stride, out_stride);
+}
+
 void base_dtg(gatt_ctx* c, const double* S, long long ss, int nm, double* D, double* T, double* G, long long ws) {
   const long long np = c->np;
   const int ndir = c->is3d ? 3 : 2;
-  for (int d = 0; d < ndir; ++d) {
-    double* in[kMaxPtr];
-    double* out[kMaxPtr];
+  {
+    double* in[3][kMaxPtr];
+    double* out[3][kMaxPtr];
     int nf = 0;
-    for (int f : state_fields(c)) {
-      in[nf] = const_cast<double*>(S) + (long long)f * np;
-      out[nf] = D + (long long)(3 * f + d) * np;
-      ++nf;
+    for (int d = 0; d < ndir; ++d) {
+      nf = 0;
+      for (int f : state_fields(c)) {
+        in[d][nf] = const_cast<double*>(S) + (long long)f * np;
+        out[d][nf] = D + (long long)(3 * f + d) * np;
+        ++nf;
+      }
     }
-    run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, ss, ws);
+    run_deriv_dirs(c, ndir, nf, in, out, nm, ss, ws);
   }
   ViscArgs va{};
   va.c = c->fc;
@@ -2933,17 +2946,20 @@ void base_dtg(gatt_ctx* c, const double* S, long long ss, int nm, double* D, dou
   va.is3d = c->is3d;
   launch_visc(va, nm, c->stream);
   const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
-  for (int d = 0; d < ndir; ++d) {
-    double* in[kMaxPtr];
-    double* out[kMaxPtr];
+  {
+    double* in[3][kMaxPtr];
+    double* out[3][kMaxPtr];
     int nf = 0;
-    for (int q = 0; q < 4; ++q) {
-      if (!c->is3d && q == 2) continue;
-      in[nf] = T + (long long)tin3[d][q] * np;
-      out[nf] = G + (long long)(4 * d + q) * np;
-      ++nf;
+    for (int d = 0; d < ndir; ++d) {
+      nf = 0;
+      for (int q = 0; q < 4; ++q) {
+        if (!c->is3d && q == 2) continue;
+        in[d][nf] = T + (long long)tin3[d][q] * np;
+        out[d][nf] = G + (long long)(4 * d + q) * np;
+        ++nf;
+      }
     }
-    run_solve(c, d, GATT_DERIV, nf, in, out, 0.0, 0, nm, ws, ws);
+    run_deriv_dirs(c, ndir, nf, in, out, nm, ws, ws);
   }
 }
 
