@@ -472,8 +472,10 @@ __global__ __launch_bounds__(256) void assemble_adj_kernel(AssembleLinArgs a) {
   const Base b = load_base(a, n);
   const double g1 = a.c.gm1;
   const double* db = a.dbar + lo;
-  const double aU = ld_nt(db + 0 * np + n), aV = ld_nt(db + 1 * np + n), aR = ld_nt(db + 3 * np + n), aE = ld_nt(db + 4 * np + n);
-  const double aW = a.is3d ? ld_nt(db + 2 * np + n) : 0.0;
+  const double s = a.dbar_scale;
+  const double aU = s * ld_nt(db + 0 * np + n), aV = s * ld_nt(db + 1 * np + n), aR = s * ld_nt(db + 3 * np + n),
+               aE = s * ld_nt(db + 4 * np + n);
+  const double aW = a.is3d ? s * ld_nt(db + 2 * np + n) : 0.0;
   const double invR = 1.0 / b.R;
   const double invR2 = invR * invR;
   // momentum

@@ -3173,8 +3173,10 @@ bool solve_t_one_launch(const gatt_ctx* c, int dir, int kind, int nf) {
 // Sbar (lin slab, all members) = J(S)^T kbar for the base stage state S
 // (member 0 pointer, member stride ss) whose derivatives, stresses and their
 // derivatives are in D, T, G (member stride ws).
+// kscale: the RHS adjoint is kscale * kbar (kscale = 1 unless the turbulence forcing, whose
+// sums read kbar as it is, is off)
 void adj_rhs(gatt_ctx* c, const double* S, long long ss, double* kbar, double* sbar, const double* Dw,
-             const double* Tw, const double* Gw, long long ws) {
+             const double* Tw, const double* Gw, long long ws, double kscale = 1.0) {
   const long long np = c->np, ls = c->lin_stride;
   const int ndir = c->is3d ? 3 : 2;
   const int tin3[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
@@ -3196,6 +3198,7 @@ void adj_rhs(gatt_ctx* c, const double* S, long long ss, double* kbar, double* s
     a.T = Tw;
     a.G = Gw;
     a.dbar = kbar + lo;
+    a.dbar_scale = kscale;
     a.Sbar = sb;
     a.Dbar = Db;
     a.Tbar = Tb;
@@ -3476,8 +3479,14 @@ void enqueue_adj_step(gatt_ctx* c, double t, double dt) {
   run_bc_adj(c, B4, ls, QB, undo_log(c, 3));
   // RK4 (the accumulator form of timeIntegration.F90:382-496), backwards, with the
   // transposed boundary conditions after every stage
-  fields_axpby(c, KB, ls, QB, ls, dt / 6.0, 0.0);
-  adj_rhs(c, B3, ls, KB, SB, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
+  // the last stage's RHS adjoint is dt/6 QB: scaled as the point-wise adjoint reads it, unless
+  // the forcing's sums need it stored
+  if (lin_forcing(c)) {
+    fields_axpby(c, KB, ls, QB, ls, dt / 6.0, 0.0);
+    adj_rhs(c, B3, ls, KB, SB, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts);
+  } else {
+    adj_rhs(c, B3, ls, QB, SB, tape_ptr(c, 3, WK_D), tape_ptr(c, 3, WK_T), tape_ptr(c, 3, WK_G), ts, dt / 6.0);
+  }
   run_bc_adj(c, B3, ls, SB, undo_log(c, 2));   // B3 is not read again as a state
   // B3 now accumulates the stage adjoints; KB = dt/3 QB + dt SB
   adj_combine(c, QB, SB, B3, KB, dt / 3.0, dt, 1, 0);

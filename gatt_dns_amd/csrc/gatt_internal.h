@@ -425,6 +425,7 @@ struct AssembleLinArgs {
   int stage; double dt;
   // adjoint
   const double* dbar;   // 5 arrays (lin stride)
+  double dbar_scale;    // the adjoint point-wise RHS reads dbar_scale * dbar (set by the caller)
   double* Sbar;         // 5 arrays, set (lin stride)
   double* Dbar; double* Tbar; double* Gbar;   // written (lin stride): Tbar 0-5, Gbar 4 per copy
   int gbar_copies;      // flux-derivative adjoints stored once for all directions (1) or per direction
